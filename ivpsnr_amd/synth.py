@@ -1,0 +1,161 @@
+"""Deterministic synthetic YUV frame pairs for tests and benchmarks.
+
+A reference picture is a sum of eight low-frequency 2-D sinusoids plus uniform
+noise around mid-grey; the tested picture is the reference with a per-block
+integer shift (clamp-to-edge), a luma offset and Gaussian noise. Chroma planes
+use the same generator at 4:2:0 resolution with independent seeds, blocks of
+half the size and half the shift.
+
+Determinism: the 1-D sinusoid vectors come from ``math.sin``/``math.cos``
+(the C library), the 2-D field from separate numpy multiplies and adds (no
+fused operations) and the noise from numpy's PCG64 generator, so the same
+seed gives the same bytes on every host running this image. Callers that pin
+results to stored fixtures also record the SHA-256 of the generated bytes.
+
+Frames are returned in the raw planar layout of a ``yuv420p``/``yuv420p10le``
+file (Y plane, then Cb, then Cr; one byte per sample at 8 bits, little-endian
+uint16 above), which is the layout the device engine consumes directly.
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+__all__ = ["FramePair", "make_pair", "config_pair", "sha256", "CONFIGS"]
+
+
+@dataclass
+class FramePair:
+    width: int
+    height: int
+    bit_depth: int
+    chroma_format: int
+    ref: np.ndarray  # packed frame, uint8 or uint16, 1-D
+    tst: np.ndarray
+
+    def planes(self, which: str):
+        """Return (Y, Cb, Cr) 2-D views of the packed frame ``which`` ('ref' or 'tst')."""
+        return split_planes(getattr(self, which), self.width, self.height, self.chroma_format)
+
+
+def split_planes(packed: np.ndarray, w: int, h: int, chroma_format: int):
+    cw, ch = (w // 2, h // 2) if chroma_format == 420 else (w, h)
+    y = packed[: w * h].reshape(h, w)
+    cb = packed[w * h : w * h + cw * ch].reshape(ch, cw)
+    cr = packed[w * h + cw * ch : w * h + 2 * cw * ch].reshape(ch, cw)
+    return y, cb, cr
+
+
+def sha256(arr: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest()
+
+
+def _field(w: int, h: int, rng: np.random.Generator, scale: float, max_value: int) -> np.ndarray:
+    """Sum of 8 sinusoids (amplitude 20-60, 0.5-4 cycles per dimension) + U(-8, 8), centred at mid-grey."""
+    acc = np.zeros((h, w), dtype=np.float64)
+    xs = range(w)
+    ys = range(h)
+    for _ in range(8):
+        amp = rng.uniform(20.0, 60.0)
+        fx = rng.uniform(0.5, 4.0) * (1.0 if rng.uniform() < 0.5 else -1.0)
+        fy = rng.uniform(0.5, 4.0)
+        phi = rng.uniform(0.0, 2.0 * math.pi)
+        # sin(a + b) = sin(a)cos(b) + cos(a)sin(b), a over x (with phase), b over y
+        sa = np.array([math.sin(2.0 * math.pi * fx * x / w + phi) for x in xs], dtype=np.float64)
+        ca = np.array([math.cos(2.0 * math.pi * fx * x / w + phi) for x in xs], dtype=np.float64)
+        sb = np.array([math.sin(2.0 * math.pi * fy * y / h) for y in ys], dtype=np.float64)
+        cb = np.array([math.cos(2.0 * math.pi * fy * y / h) for y in ys], dtype=np.float64)
+        t1 = np.multiply(cb[:, None], sa[None, :])
+        t2 = np.multiply(sb[:, None], ca[None, :])
+        acc = np.add(acc, np.multiply(amp, np.add(t1, t2)))
+    noise = rng.uniform(-8.0, 8.0, size=(h, w))
+    acc = np.add(acc, noise)
+    mid = 128.0
+    val = np.add(np.multiply(acc, scale), mid * scale)
+    return np.clip(np.rint(val), 0, max_value)
+
+
+def _shift_blocks(plane: np.ndarray, block: int, shifts: np.ndarray, halve: bool) -> np.ndarray:
+    """Per-block integer shift with clamp-to-edge addressing: out(y, x) = in(clamp(y+dy), clamp(x+dx))."""
+    h, w = plane.shape
+    out = np.empty_like(plane)
+    by_n, bx_n = shifts.shape[:2]
+    for by in range(by_n):
+        y0, y1 = by * block, min(h, (by + 1) * block)
+        if y0 >= h:
+            break
+        for bx in range(bx_n):
+            x0, x1 = bx * block, min(w, (bx + 1) * block)
+            if x0 >= w:
+                break
+            dy, dx = int(shifts[by, bx, 0]), int(shifts[by, bx, 1])
+            if halve:
+                dy >>= 1
+                dx >>= 1
+            ys = np.clip(np.arange(y0, y1) + dy, 0, h - 1)
+            xs = np.clip(np.arange(x0, x1) + dx, 0, w - 1)
+            out[y0:y1, x0:x1] = plane[np.ix_(ys, xs)]
+    return out
+
+
+def make_pair(width: int, height: int, bit_depth: int, seed: int, *, chroma_format: int = 420,
+              luma_offset: float = 3.0, noise_sigma: float = 2.0, max_shift: int = 2,
+              block: int = 32) -> FramePair:
+    """Generate one (reference, tested) frame pair.
+
+    Amplitudes, offsets and noise are given in 8-bit units and scaled by
+    2**(bit_depth-8) before rounding, so a 10-bit pair is the 8-bit field x4.
+    """
+    if chroma_format not in (420, 444):
+        raise ValueError("chroma_format must be 420 or 444")
+    if chroma_format == 420 and (width % 2 or height % 2):
+        raise ValueError("4:2:0 needs even width and height")
+    scale = float(1 << (bit_depth - 8))
+    max_value = (1 << bit_depth) - 1
+    rng = np.random.default_rng(np.random.SeedSequence([int(seed), width, height, bit_depth]))
+    cw, ch = (width // 2, height // 2) if chroma_format == 420 else (width, height)
+
+    ref_y = _field(width, height, rng, scale, max_value)
+    ref_cb = _field(cw, ch, rng, scale, max_value)
+    ref_cr = _field(cw, ch, rng, scale, max_value)
+
+    nby = (height + block - 1) // block
+    nbx = (width + block - 1) // block
+    shifts = rng.integers(-max_shift, max_shift + 1, size=(nby, nbx, 2))
+    cblock = block // 2 if chroma_format == 420 else block
+    halve = chroma_format == 420
+
+    def distort(plane, blk, hv, offset):
+        shifted = _shift_blocks(plane, blk, shifts, hv)
+        noise = rng.normal(0.0, noise_sigma * scale, size=plane.shape)
+        return np.clip(np.rint(np.add(np.add(shifted, offset * scale), noise)), 0, max_value)
+
+    tst_y = distort(ref_y, block, False, luma_offset)
+    tst_cb = distort(ref_cb, cblock, halve, 0.0)
+    tst_cr = distort(ref_cr, cblock, halve, 0.0)
+
+    dt = np.uint8 if bit_depth <= 8 else np.uint16
+    ref = np.concatenate([p.astype(dt).ravel() for p in (ref_y, ref_cb, ref_cr)])
+    tst = np.concatenate([p.astype(dt).ravel() for p in (tst_y, tst_cb, tst_cr)])
+    return FramePair(width, height, bit_depth, chroma_format, ref, tst)
+
+
+# Workload presets (frame size, bit depth, frames, generator knobs).
+CONFIGS = {
+    # plain IV-PSNR + PSNR, 8-bit 1080p
+    "ivpsnr_1080p8": dict(width=1920, height=1080, bit_depth=8, frames=3, luma_offset=3.0),
+    # IV-PSNR + WS-PSNR on ERP, 10-bit 4096x2048 (luma offset x4 = +12, beyond the 10-level clip)
+    "ivpsnr_erp_4k10": dict(width=4096, height=2048, bit_depth=10, frames=8, luma_offset=3.0),
+    # SSIM / MS-SSIM / IV-SSIM, 10-bit 1080p
+    "ssim_1080p10": dict(width=1920, height=1080, bit_depth=10, frames=16, luma_offset=3.0),
+}
+
+
+def config_pair(name: str, frame: int, **overrides) -> FramePair:
+    c = dict(CONFIGS[name])
+    c.update(overrides)
+    return make_pair(c["width"], c["height"], c["bit_depth"], seed=1 + frame,
+                     luma_offset=c.get("luma_offset", 3.0))
