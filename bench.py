@@ -1,0 +1,153 @@
+"""Throughput of the IV-PSNR engine on MI355X.
+
+Workload (one step): IV-PSNR + WS-PSNR + PSNR of 8 frame pairs, 4096x2048, yuv420p10le,
+equirectangular (WS weights on), search range 2, GCD on - the configuration the headline
+metric is quoted on. Frames are synthetic (ivpsnr_amd.synth) and resident in HBM before the
+timed region starts.
+
+Prints one JSON line:
+  value      whole-job Mpix/s over the timed steps
+  roofline   the CPS kernel's algorithmic bytes per launch (compulsory 4:2:0 input of one frame
+             pair) over its average launch duration, measured with HIP events on the engine stream
+  cpu_baseline  the reference CPU path (oracle/_ref/qmiv_ref_v3, the reference's own sources
+             built for x86-64-v3) on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "IV-PSNR Mpix/s @4096x2048 10b 420 + %HBM roofline; batched vs plain"
+HBM_PEAK_GBPS = 8000.0
+W, H, BD, FRAMES = 4096, 2048, 10, 8
+
+
+def frame_pair_bytes(w, h, bd):
+    return 2 * w * h * 3 // 2 * (1 if bd <= 8 else 2)
+
+
+def cpu_baseline(pairs, sample_frames=4):
+    """Times the reference's own CPU path on `sample_frames` frames of the workload."""
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "qmiv_ref_v3")
+    cores = min(16, os.cpu_count() or 1)
+    if os.path.exists(ref_bin):
+        with tempfile.TemporaryDirectory() as td:
+            f0, f1 = os.path.join(td, "p0.yuv"), os.path.join(td, "p1.yuv")
+            with open(f0, "wb") as a, open(f1, "wb") as b:
+                for i in range(sample_frames):
+                    a.write(pairs[i % len(pairs)].ref.tobytes())
+                    b.write(pairs[i % len(pairs)].tst.tobytes())
+            cmd = [ref_bin, "-w", str(W), "-h", str(H), "-bd", str(BD), "-cf", "420", "-i0", f0, "-i1", f1,
+                   "-erp", "360", "180", "-ml", "PSNR,WSPSNR,IVPSNR", "-nth", str(cores), "-time"]
+            try:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+                if r.returncode == 0:
+                    t = json.loads(r.stdout)["time_ms_per_frame"]
+                    ms = sum(t[k] for k in ("validate", "preproc", "gcd", "psnr", "wspsnr", "ivpsnr"))
+                    return {"value": round(W * H / (ms * 1e3), 3), "unit": "Mpix/s", "cores": cores,
+                            "kind": "reference",
+                            "sample": f"{sample_frames} frames 4096x2048 10-bit ERP, IV-PSNR+WS-PSNR+PSNR incl. "
+                                      f"validation, preprocessing and GCD, {cores} threads; {ms:.1f} ms/frame"}
+            except (subprocess.TimeoutExpired, OSError, ValueError, KeyError):
+                pass
+    # The reference build is missing or does not run on this host: time the C restatement.
+    from oracle import pyoracle
+    p = pyoracle.params(W, H, BD, 420, metrics=pyoracle.PSNR | pyoracle.WSPSNR | pyoracle.IVPSNR, erp=True)
+    t0 = time.perf_counter()
+    pyoracle.frame(pairs[0].ref, pairs[0].tst, p)
+    ms = (time.perf_counter() - t0) * 1e3
+    return {"value": round(W * H / (ms * 1e3), 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": f"1 frame 4096x2048 10-bit ERP, IV-PSNR+WS-PSNR+PSNR, single thread; {ms:.1f} ms/frame"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this engine's workload runs on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    from ivpsnr_amd import IVPSNR, PSNR, WSPSNR, Engine, default_params
+    from ivpsnr_amd.synth import make_pair
+
+    print(f"[bench] generating {FRAMES} synthetic frame pairs {W}x{H} {BD}-bit", file=sys.stderr, flush=True)
+    pairs = [make_pair(W, H, BD, seed=1 + f) for f in range(FRAMES)]
+    metrics = PSNR | WSPSNR | IVPSNR
+    prm = default_params(erp=True)
+
+    eng = Engine(0)
+    pics = [(eng.picture(W, H, BD, 420, data=p.ref), eng.picture(W, H, BD, 420, data=p.tst)) for p in pairs]
+    eng.sync()
+
+    def step():
+        out = []
+        for p0, p1 in pics:
+            out.append(eng.frame(p0, p1, metrics, prm))
+        return out
+
+    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step()
+    eng.profile(True)
+    eng.profile_read(reset=True)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+    prof = eng.profile_read(reset=True)
+    eng.profile(False)
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    ms_per_step = elapsed / args.steps * 1e3
+    mpix = FRAMES * W * H / 1e6
+    value = mpix / (ms_per_step / 1e3)
+    cps_ms, cps_n = prof.get("cps", (0.0, 0))
+    cps_avg_ms = cps_ms / max(cps_n, 1)
+    bytes_per_launch = frame_pair_bytes(W, H, BD)
+    achieved = bytes_per_launch / (cps_avg_ms / 1e3) / 1e9 if cps_avg_ms > 0 else 0.0
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(pairs)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "IV-PSNR+WS-PSNR+PSNR, 8 frame pairs 4096x2048 yuv420p10le ERP, B=2, GCD on",
+                   "frames_per_step": FRAMES, "mode": "plain (one qmiv_frame call per frame)",
+                   "ivpsnr_last_frame": round(res[-1].ivpsnr, 6)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "k_cps_fast", "kernel_avg_us": round(cps_avg_ms * 1e3, 2),
+                     "bytes_per_launch": bytes_per_launch},
+        "kernels_us_per_frame": {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in prof.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,203 @@
+/*
+  qmiv.h - C ABI of the MI355X (gfx950) per-frame metric engine.
+
+  The engine computes IV-PSNR (corresponding-pixel-shift search with global
+  colour difference, both directions), PSNR, WS-PSNR, SSIM, MS-SSIM and IV-SSIM
+  for pairs of pictures resident in device memory, with results identical to
+  the reference CPU implementation (integer sums and the PSNR family bit-exact;
+  SSIM family within 1e-12 relative).
+
+  The reference exposes this functionality as C++ processor classes with no
+  FFI of its own; each entry point below names the reference interface it
+  replaces (file:line, relative to the reference repository root).
+
+  Conventions
+    - Plain C types only. Every call returns a qmiv_status; no exception or
+      C++ type crosses this boundary. qmiv_last_error() describes the last
+      failure on the calling thread.
+    - A context owns one HIP stream on one device and the device scratch its
+      calls need. Calls on one context are not thread-safe; use one context
+      per host thread.
+    - Pictures hold one frame in the raw planar file layout (Y plane, then Cb,
+      then Cr; 4:2:0 chroma planes are W/2 x H/2; one byte per sample at 8 bits,
+      little-endian uint16 above). "P0" and "P1" are the reference
+      application's InputFile0 and InputFile1.
+    - Metric calls are synchronous: they enqueue the kernels on the context's
+      stream, wait for the small result block and finish the double-precision
+      arithmetic on the host.
+*/
+#ifndef QMIV_H
+#define QMIV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QMIV_ABI_VERSION 1
+
+typedef enum qmiv_status {
+  QMIV_OK = 0,
+  QMIV_ERR_INVALID_ARG = 1,   /* null pointer, bad size, incompatible pictures */
+  QMIV_ERR_UNSUPPORTED = 2,   /* parameter combination this engine does not implement */
+  QMIV_ERR_HIP = 3,           /* HIP runtime error (message in qmiv_last_error) */
+  QMIV_ERR_OUT_OF_MEMORY = 4,
+  QMIV_ERR_PEL_RANGE = 5,     /* a pel exceeds 2^bit_depth - 1 (reference's InvalidPelActn=STOP) */
+} qmiv_status;
+
+/* Metric selection bits (order of the reference's metric list, apps/QMIV/src/xAppQMIV.h:119-131). */
+enum {
+  QMIV_PSNR = 1,
+  QMIV_WSPSNR = 2,
+  QMIV_IVPSNR = 4,
+  QMIV_SSIM = 8,
+  QMIV_MSSSIM = 16,
+  QMIV_IVSSIM = 32,
+};
+
+typedef struct qmiv_ctx qmiv_ctx;
+typedef struct qmiv_pic qmiv_pic;
+
+/* Processor parameters. Defaults (qmiv_params_default) are the reference's:
+   libs/pmbb-ivqm/src/xCorrespPixelShift.h:30-32 (B=2, weights 4:1:1:0),
+   libs/pmbb-ivqm/src/xGlobClrDiff.h:18 (unnoticeable coefficient 0.01),
+   libs/pmbb-ivqm/src/xWeightedSpherically.h:20 (WS off; 360 x 180 degrees). */
+typedef struct qmiv_params {
+  int32_t search_range;            /* B: the search window is (2B+1) x (2B+1) */
+  int32_t cmp_weights_search[4];   /* component weights inside the search */
+  int32_t cmp_weights_average[4];  /* component weights of the final average */
+  float unnoticeable_coef[4];      /* GCD clip: +-rint(coef * max) */
+  int32_t erp;                     /* 1: equirectangular, WS weights on (xWeightedSpherically::initWS) */
+  int32_t lon_range_deg, lat_range_deg;
+  /* 1 (default): per-row CPS sums wrap in int32 and are sign-extended, as the reference's
+     SSE row kernel does (xCorrespPixelShiftSSE.cpp:27-38, the application default);
+     0: exact uint64 row sums (xCorrespPixelShiftSTD.cpp:26-36). Identical unless a
+     row's sum exceeds 2^31 - 1. */
+  int32_t cps_int32_rows;
+  /* 1 (default): qmiv_frame replaces the PSNR / WS-PSNR of identical components by the
+     PSNR of SSD=1, as the application does (apps/QMIV/src/xAppQMIV.cpp:837-844,864-871). */
+  int32_t fake_psnr_for_exact;
+} qmiv_params;
+
+typedef struct qmiv_frame_result {
+  int32_t in_range[2];             /* per input: all pels <= 2^bd - 1 (xPicP::check) */
+  int32_t exact[3];                /* per component: pictures identical (xPicP::equalCmp) */
+  int32_t gcd[4];                  /* GCD R2T (xAppQMIV.cpp:693-698) */
+  double psnr[3], psnr_pic;
+  double wspsnr[3], wspsnr_pic;
+  double ivpsnr, ivpsnr_r2t, ivpsnr_t2r;
+  uint64_t cps_sum[2][3];          /* [R2T, T2R][component]: sum over rows of the per-row values */
+  double ssim[3], ssim_pic;
+  double msssim[3], msssim_pic;
+  double msssim_scales[5][3];
+  double ivssim, ivssim_r2t, ivssim_t2r;
+} qmiv_frame_result;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Context and pictures                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Creates a context on HIP device `device` with its own stream. Replaces the reference's
+   thread pool set-up (apps/QMIV/src/xAppQMIV.cpp:458-471, libs/pmbb-core/src/xThreadPool.h:20-134). */
+qmiv_status qmiv_create(int32_t device, qmiv_ctx** out);
+void qmiv_destroy(qmiv_ctx* ctx);
+/* Message of the last failed call on this thread ("" if none). */
+const char* qmiv_last_error(void);
+int32_t qmiv_abi_version(void);
+/* The context's HIP stream (hipStream_t), for callers that order their own work with it. */
+void* qmiv_stream(qmiv_ctx* ctx);
+/* Waits until all work enqueued on the context's stream has finished (uploads included). Replaces
+   xThreadPoolInterface::waitUntilTasksFinished (libs/pmbb-core/src/xThreadPool.cpp:147-170). */
+qmiv_status qmiv_sync(qmiv_ctx* ctx);
+
+void qmiv_params_default(qmiv_params* p);
+
+/* Bytes of one packed frame: xSeq::create (libs/pmbb-core/src/xSeq.cpp:408-428). */
+size_t qmiv_frame_bytes(int32_t width, int32_t height, int32_t bit_depth, int32_t chroma_format);
+
+/* Allocates a device picture for one packed frame (bit_depth 8..16; chroma_format 420 with even
+   width and height, or 444). Replaces xPicP::create (libs/pmbb-core/src/xPic.h:31) plus the
+   margin (the engine addresses edges by clamping instead, identical to ExtendMargin,
+   libs/pmbb-core/src/xPixelOpsSTD.cpp:210-236). */
+qmiv_status qmiv_pic_create(qmiv_ctx* ctx, int32_t width, int32_t height, int32_t bit_depth,
+                            int32_t chroma_format, qmiv_pic** out);
+/* Wraps caller-owned device memory holding one packed frame (not freed by the engine). */
+qmiv_status qmiv_pic_wrap(qmiv_ctx* ctx, int32_t width, int32_t height, int32_t bit_depth,
+                          int32_t chroma_format, void* device_ptr, qmiv_pic** out);
+/* Frees an engine-owned picture (waits for outstanding device work); valid after qmiv_destroy. */
+void qmiv_pic_destroy(qmiv_pic* pic);
+void* qmiv_pic_device_ptr(qmiv_pic* pic);
+/* Copies one packed frame from host memory (qmiv_frame_bytes bytes) onto the device, ordered on
+   the context's stream. Pinned host memory is copied asynchronously; pageable memory is staged
+   through the context's pinned buffer first. Replaces xSeqBase::readFrame's unpack step
+   (libs/pmbb-core/src/xSeq.cpp:179-249): the engine keeps the packed layout. */
+qmiv_status qmiv_pic_upload(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes);
+/* Copies a device picture back to host memory (packed layout). */
+qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, size_t bytes);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Timing (replaces the reference's per-stage RDTSC timing, apps/QMIV/src/xAppQMIV.cpp:670-744) */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Kernel kinds reported by qmiv_profile_read. */
+enum {
+  QMIV_K_STATS = 0,     /* per-row SD / SSD / range statistics */
+  QMIV_K_SUMMARY = 1,   /* plane totals and GCD */
+  QMIV_K_CPS = 2,       /* corresponding-pixel-shift search (IV-PSNR, SCP) */
+  QMIV_K_CPS_ROWS = 3,  /* per-row reduction of the CPS partial sums */
+  QMIV_K_SSIM = 4,      /* SSIM windowed moments */
+  QMIV_K_PYRAMID = 5,   /* MS-SSIM 2x2 pyramid */
+  QMIV_K_COUNT = 8
+};
+
+typedef struct qmiv_profile {
+  double kernel_ms[QMIV_K_COUNT];   /* accumulated device time, measured with HIP events */
+  int64_t launches[QMIV_K_COUNT];
+} qmiv_profile;
+
+/* When on, every kernel launch of the context is bracketed by HIP events on its stream. */
+qmiv_status qmiv_profile_enable(qmiv_ctx* ctx, int32_t on);
+/* Accumulated kernel times since the last reset; reset != 0 clears them. */
+qmiv_status qmiv_profile_read(qmiv_ctx* ctx, qmiv_profile* out, int32_t reset);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Frame flow                                                                                  */
+/* ------------------------------------------------------------------------------------------ */
+
+/* One frame through the reference application's per-frame flow: validation, exact-component
+   detection, GCD, SCP, then the selected metrics (apps/QMIV/src/xAppQMIV.cpp:661-935).
+   `metrics` is a mask of QMIV_* bits. Returns QMIV_ERR_PEL_RANGE (with `out` filled up to the
+   flags) when a pel exceeds the bit depth, as InvalidPelActn=STOP does (xAppQMIV.cpp:753-770). */
+qmiv_status qmiv_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm,
+                       uint32_t metrics, qmiv_frame_result* out);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Processor-level calls (same argument meaning as the reference's processor methods)          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* xGlobClrDiffProc::CalcGlobalColorDiff(Tst, Ref) (libs/pmbb-ivqm/src/xGlobClrDiff.h:41):
+   gcd = clip(rint(sum(Ref - Tst) / area), +-rint(coef * max)). */
+qmiv_status qmiv_calc_gcd(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm,
+                          int32_t gcd[4]);
+
+/* xPSNR::calcPicPSNR (libs/pmbb-ivqm/src/xPSNR.h:33): per component; SSD=0 gives DBL_MAX. */
+qmiv_status qmiv_calc_psnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, double psnr[3]);
+
+/* xWSPSNR::calcPicWSPSNR (libs/pmbb-ivqm/src/xWSPSNR.h:22); plain PSNR unless prm->erp. */
+qmiv_status qmiv_calc_wspsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm,
+                             double wspsnr[3]);
+
+/* xIVPSNR::calcPicIVPSNR(Tst, Ref, GlobalColorDiffRef2Tst) (libs/pmbb-ivqm/src/xIVPSNR.h:34):
+   R2T = Q(Tst + gcd searched in Ref), T2R = Q(Ref - gcd searched in Tst), result min(R2T, T2R).
+   r2t, t2r and rows may be NULL. rows (if given) receives 2*3*height uint64 per-row values
+   [direction][component][row] with the prm->cps_int32_rows semantics. */
+qmiv_status qmiv_calc_ivpsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm,
+                             const int32_t gcd_ref2tst[4], double* ivpsnr, double* r2t, double* t2r,
+                             uint64_t* rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,141 @@
+"""ctypes declarations of the C ABI in include/qmiv.h (libqmiv.so, built in-tree).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqmiv.so")
+
+QMIV_OK = 0
+STATUS_NAMES = {0: "QMIV_OK", 1: "QMIV_ERR_INVALID_ARG", 2: "QMIV_ERR_UNSUPPORTED", 3: "QMIV_ERR_HIP",
+                4: "QMIV_ERR_OUT_OF_MEMORY", 5: "QMIV_ERR_PEL_RANGE"}
+PSNR, WSPSNR, IVPSNR, SSIM, MSSSIM, IVSSIM = 1, 2, 4, 8, 16, 32
+
+# Every entry point declared in include/qmiv.h; the CPU tests check the library exports them all.
+EXPORTS = []
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("search_range", C.c_int32),
+        ("cmp_weights_search", C.c_int32 * 4),
+        ("cmp_weights_average", C.c_int32 * 4),
+        ("unnoticeable_coef", C.c_float * 4),
+        ("erp", C.c_int32), ("lon_range_deg", C.c_int32), ("lat_range_deg", C.c_int32),
+        ("cps_int32_rows", C.c_int32),
+        ("fake_psnr_for_exact", C.c_int32),
+    ]
+
+
+class FrameResult(C.Structure):
+    _fields_ = [
+        ("in_range", C.c_int32 * 2), ("exact", C.c_int32 * 3), ("gcd", C.c_int32 * 4),
+        ("psnr", C.c_double * 3), ("psnr_pic", C.c_double),
+        ("wspsnr", C.c_double * 3), ("wspsnr_pic", C.c_double),
+        ("ivpsnr", C.c_double), ("ivpsnr_r2t", C.c_double), ("ivpsnr_t2r", C.c_double),
+        ("cps_sum", (C.c_uint64 * 3) * 2),
+        ("ssim", C.c_double * 3), ("ssim_pic", C.c_double),
+        ("msssim", C.c_double * 3), ("msssim_pic", C.c_double),
+        ("msssim_scales", (C.c_double * 3) * 5),
+        ("ivssim", C.c_double), ("ivssim_r2t", C.c_double), ("ivssim_t2r", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            if hasattr(v, "__len__"):
+                v = [list(x) if hasattr(x, "__len__") else x for x in v]
+            out[name] = v
+        return out
+
+
+K_STATS, K_SUMMARY, K_CPS, K_CPS_ROWS, K_SSIM, K_PYRAMID, K_COUNT = 0, 1, 2, 3, 4, 5, 8
+KERNEL_NAMES = {K_STATS: "stats", K_SUMMARY: "summary", K_CPS: "cps", K_CPS_ROWS: "cps_rows", K_SSIM: "ssim",
+                K_PYRAMID: "pyramid"}
+
+
+class Profile(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double * 8), ("launches", C.c_int64 * 8)]
+
+
+_ctx_p = C.c_void_p
+_pic_p = C.c_void_p
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+
+_SIGNATURES = {
+    "qmiv_create": (C.c_int, [C.c_int32, C.POINTER(_ctx_p)]),
+    "qmiv_destroy": (None, [_ctx_p]),
+    "qmiv_last_error": (C.c_char_p, []),
+    "qmiv_abi_version": (C.c_int32, []),
+    "qmiv_stream": (C.c_void_p, [_ctx_p]),
+    "qmiv_sync": (C.c_int, [_ctx_p]),
+    "qmiv_profile_enable": (C.c_int, [_ctx_p, C.c_int32]),
+    "qmiv_profile_read": (C.c_int, [_ctx_p, C.POINTER(Profile), C.c_int32]),
+    "qmiv_params_default": (None, [C.POINTER(Params)]),
+    "qmiv_frame_bytes": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "qmiv_pic_create": (C.c_int, [_ctx_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_pic_p)]),
+    "qmiv_pic_wrap": (C.c_int, [_ctx_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                C.POINTER(_pic_p)]),
+    "qmiv_pic_destroy": (None, [_pic_p]),
+    "qmiv_pic_device_ptr": (C.c_void_p, [_pic_p]),
+    "qmiv_pic_upload": (C.c_int, [_ctx_p, _pic_p, C.c_void_p, C.c_size_t]),
+    "qmiv_pic_download": (C.c_int, [_ctx_p, _pic_p, C.c_void_p, C.c_size_t]),
+    "qmiv_frame": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), C.c_uint32, C.POINTER(FrameResult)]),
+    "qmiv_calc_gcd": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _i32p]),
+    "qmiv_calc_psnr": (C.c_int, [_ctx_p, _pic_p, _pic_p, _f64p]),
+    "qmiv_calc_wspsnr": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _f64p]),
+    "qmiv_calc_ivpsnr": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _i32p, _f64p, _f64p, _f64p,
+                                   _u64p]),
+}
+EXPORTS.extend(_SIGNATURES)
+
+_lib = None
+
+
+def lib():
+    """The loaded engine library. Raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class QmivError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        msg = lib().qmiv_last_error().decode(errors="replace")
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def check(status: int, where: str):
+    if status != QMIV_OK:
+        raise QmivError(status, where)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().qmiv_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k in ("cmp_weights_search", "cmp_weights_average"):
+            for i in range(4):
+                getattr(p, k)[i] = int(v[i])
+        elif k == "unnoticeable_coef":
+            for i in range(4):
+                p.unnoticeable_coef[i] = float(v[i])
+        else:
+            setattr(p, k, int(v))
+    return p

@@ -1,0 +1,90 @@
+// Host-side double-precision finalisation. Compiled with -ffp-contract=off (no fused
+// multiply-add), so each expression below rounds exactly like the reference's ISO C++ build;
+// the integer inputs come bit-exact from the device, hence the doubles are bit-exact too.
+#include "finalize.h"
+
+#include <cfloat>
+#include <cmath>
+
+namespace qmiv {
+namespace fin {
+
+double psnr_from_ssd(double ssd, int32_t area, int32_t bit_depth) {
+  const uint64_t points = (uint64_t)(int64_t)area;
+  const uint64_t maxv = (uint64_t)((1 << bit_depth) - 1);
+  const uint64_t peak = points * (maxv * maxv);
+  return ssd > 0 ? 10.0 * std::log10((double)peak / ssd) : DBL_MAX;
+}
+
+namespace {
+inline void neumaier(double& s, double& c, double v) {
+  const double t = s + v;
+  if (std::fabs(s) >= std::fabs(v)) c += ((s - t) + v);
+  else c += ((v - t) + s);
+  s = t;
+}
+}  // namespace
+
+double kbns(const double* x, size_t n) {
+  if (n == 0) return 0.0;
+  double s = x[0], c = 0.0;
+  for (size_t i = 1; i < n; i++) neumaier(s, c, x[i]);
+  return s + c;
+}
+
+void ws_weights(int32_t height, int32_t lat_range_deg, std::vector<double>& weights, double& correction) {
+  const double pi = 3.14159265358979323846;
+  const double eh = 180.0 * (double)height / (double)lat_range_deg;
+  const double off = (eh - height) / 2.0;
+  weights.resize((size_t)height);
+  for (int32_t i = 0; i < height; i++) weights[(size_t)i] = std::cos((i + off - (eh / 2 - 0.5)) * pi / eh);
+  correction = height / kbns(weights.data(), weights.size());
+}
+
+double qual_from_rows(const uint64_t* rows, int32_t height, const double* ws, int32_t area, int32_t bit_depth,
+                      const int32_t wavg[4]) {
+  double err[3];
+  for (int c = 0; c < 3; c++) {
+    const uint64_t* r = rows + (size_t)c * (size_t)height;
+    if (ws) {
+      double s = 0.0, k = 0.0;  // lane-wise xKBNS4, starting from zero
+      for (int32_t y = 0; y < height; y++) neumaier(s, k, (double)r[y] * ws[y]);
+      err[c] = s + k;
+    } else {
+      uint64_t s = 0;
+      for (int32_t y = 0; y < height; y++) s += r[y];
+      err[c] = (double)s;
+    }
+  }
+  double q[3];
+  for (int c = 0; c < 3; c++) q[c] = psnr_from_ssd(err[c] > 0 ? err[c] : 1.0, area, bit_depth);
+  return weighted_average4(q, wavg);
+}
+
+double wspsnr_from_rows(const uint64_t* rows444, int32_t height, const double* ws, double correction,
+                        int32_t area, int32_t bit_depth) {
+  double s = 0.0, k = 0.0;  // xKBNS::acc from zero
+  for (int32_t y = 0; y < height; y++) neumaier(s, k, (double)rows444[y] * ws[y]);
+  return psnr_from_ssd((s + k) * correction, area, bit_depth);
+}
+
+double pic_value(const double v[3], const int32_t wavg[4]) {
+  const double inv = 1.0 / (double)(wavg[0] + wavg[1] + wavg[2] + wavg[3]);
+  return (v[0] * wavg[0] + v[1] * wavg[1] + v[2] * wavg[2]) * inv;
+}
+
+double weighted_average4(const double v[3], const int32_t wavg[4]) {
+  const double inv = 1.0 / (double)(wavg[0] + wavg[1] + wavg[2] + wavg[3]);
+  const double sum = ((v[0] * (double)wavg[0] + v[1] * (double)wavg[1]) + v[2] * (double)wavg[2]) + 0.0 * (double)wavg[3];
+  return sum * inv;
+}
+
+int32_t gcd_max_diff(float coef, int32_t bit_depth) {
+  const float prod = coef * (float)((1 << bit_depth) - 1);
+  return (int32_t)std::nearbyint(prod);
+}
+
+double min2(double a, double b) { return (b < a) ? b : a; }
+
+}  // namespace fin
+}  // namespace qmiv

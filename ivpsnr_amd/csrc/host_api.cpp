@@ -1,0 +1,665 @@
+// Host runtime of the metric engine: contexts, device pictures, the per-frame flow and the
+// C ABI declared in include/qmiv.h.
+#include "../../include/qmiv.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "finalize.h"
+#include "qmiv_internal.h"
+
+using namespace qmiv;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+qmiv_status fail(qmiv_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+#define QMIV_HIP(expr)                                                                          \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return fail(QMIV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));              \
+  } while (0)
+
+#define QMIV_TRY(expr)                 \
+  do {                                 \
+    qmiv_status s_ = (expr);           \
+    if (s_ != QMIV_OK) return s_;      \
+  } while (0)
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+struct qmiv_pic {
+  qmiv_ctx* ctx;
+  Geom g;
+  void* dev;
+  bool owned;
+};
+
+struct qmiv_ctx {
+  int32_t device = 0;
+  hipStream_t stream = nullptr;
+  // Device scratch (grown on demand; one frame evaluation at a time per context).
+  void* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // Pinned host block receiving the per-frame results.
+  void* h_results = nullptr;
+  size_t results_bytes = 0;
+  // Shift-compensated pictures (IV-SSIM), 2 x [3][h][w] uint16.
+  void* d_scp = nullptr;
+  size_t scp_bytes = 0;
+  // Kernel timing: event pairs recorded around launches since the last harvest.
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  std::vector<int> ev_kind;
+  size_t ev_used = 0;
+  qmiv_profile prof{};
+};
+
+namespace {
+
+qmiv_status make_geom(int32_t w, int32_t h, int32_t bd, int32_t cf, Geom& g) {
+  if (w <= 0 || h <= 0) return fail(QMIV_ERR_INVALID_ARG, "width and height must be positive");
+  if (bd < 8 || bd > 16) return fail(QMIV_ERR_INVALID_ARG, "bit depth must be within 8..16");
+  if (cf != 420 && cf != 444) return fail(QMIV_ERR_UNSUPPORTED, "chroma format must be 420 or 444");
+  if (cf == 420 && ((w & 1) || (h & 1)))
+    return fail(QMIV_ERR_UNSUPPORTED, "4:2:0 pictures need even width and height");
+  g = Geom{};
+  g.w = w;
+  g.h = h;
+  g.cs = cf == 420 ? 1 : 0;
+  g.cw = w >> g.cs;
+  g.ch = h >> g.cs;
+  g.bd = bd;
+  g.bytes_per_sample = bd <= 8 ? 1 : 2;
+  const size_t samples = (size_t)w * h + 2 * (size_t)g.cw * g.ch;
+  const size_t bytes = samples * (size_t)g.bytes_per_sample;
+  if (bytes >= (size_t)1 << 31) return fail(QMIV_ERR_UNSUPPORTED, "frame larger than 2 GiB");
+  g.frame_bytes = (uint32_t)bytes;
+  g.cb_off = (int64_t)w * h;
+  g.cr_off = g.cb_off + (int64_t)g.cw * g.ch;
+  return QMIV_OK;
+}
+
+bool same_geom(const Geom& a, const Geom& b) {
+  return a.w == b.w && a.h == b.h && a.bd == b.bd && a.cs == b.cs;
+}
+
+qmiv_status ensure_scratch(qmiv_ctx* ctx, size_t bytes) {
+  if (ctx->scratch_bytes >= bytes) return QMIV_OK;
+  if (ctx->d_scratch) {
+    QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    QMIV_HIP(hipFree(ctx->d_scratch));
+    ctx->d_scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  const size_t sz = align_up(bytes, 1 << 20);
+  if (hipMalloc(&ctx->d_scratch, sz) != hipSuccess) return fail(QMIV_ERR_OUT_OF_MEMORY, "device scratch allocation failed");
+  ctx->scratch_bytes = sz;
+  return QMIV_OK;
+}
+
+qmiv_status ensure_results(qmiv_ctx* ctx, size_t bytes) {
+  if (ctx->results_bytes >= bytes) return QMIV_OK;
+  if (ctx->h_results) {
+    QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    QMIV_HIP(hipHostFree(ctx->h_results));
+    ctx->h_results = nullptr;
+    ctx->results_bytes = 0;
+  }
+  const size_t sz = align_up(bytes, 1 << 16);
+  if (hipHostMalloc(&ctx->h_results, sz, hipHostMallocDefault) != hipSuccess)
+    return fail(QMIV_ERR_OUT_OF_MEMORY, "pinned result buffer allocation failed");
+  ctx->results_bytes = sz;
+  return QMIV_OK;
+}
+
+qmiv_status ensure_scp(qmiv_ctx* ctx, size_t bytes) {
+  if (ctx->scp_bytes >= bytes) return QMIV_OK;
+  if (ctx->d_scp) {
+    QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    QMIV_HIP(hipFree(ctx->d_scp));
+    ctx->d_scp = nullptr;
+    ctx->scp_bytes = 0;
+  }
+  const size_t sz = align_up(bytes, 1 << 20);
+  if (hipMalloc(&ctx->d_scp, sz) != hipSuccess) return fail(QMIV_ERR_OUT_OF_MEMORY, "SCP allocation failed");
+  ctx->scp_bytes = sz;
+  return QMIV_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Scratch layout of one frame evaluation.
+//   device: [results block | row_sd | row_max | partials]
+//   results block (copied back in one transfer): FrameSummary | cps rows [2][3][h] | row_ssd [nrows]
+// ---------------------------------------------------------------------------------------------
+struct Layout {
+  size_t off_summary, off_cps_rows, off_row_ssd, results_bytes;
+  size_t off_row_sd, off_row_max, off_partials, total;
+  int32_t nseg;
+};
+
+// Fast CPS path eligibility: every operand of the packed key must stay inside the ranges the
+// kernel's 24-bit multiplies and 32-bit keys assume, and the reference's int32 error must not
+// wrap (the generic kernel reproduces wrapping).
+bool fast_cps_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) {
+  if (p.search_range != 2) return false;
+  const int64_t maxv = (1 << g.bd) - 1;
+  int64_t wsum = 0;
+  for (int c = 0; c < 3; c++) {
+    if (p.cmp_weights_search[c] < 0) return false;
+    wsum += p.cmp_weights_search[c];
+  }
+  if (p.cmp_weights_search[3] != 0 && gcd_bound[3] != 0) return false;
+  int64_t m = 0;
+  for (int c = 0; c < 3; c++) m = std::max<int64_t>(m, maxv + std::abs((int64_t)gcd_bound[c]));
+  for (int c = 0; c < 3; c++)
+    if (64 * (int64_t)p.cmp_weights_search[c] * m >= ((int64_t)1 << 23)) return false;
+  const int64_t key_bound = 32 * (2 * wsum * m * maxv + wsum * maxv * maxv) + 512;
+  if (key_bound >= ((int64_t)1 << 31)) return false;
+  int64_t ref_err = 0;
+  for (int c = 0; c < 3; c++) ref_err += (int64_t)p.cmp_weights_search[c] * (m + maxv) * (m + maxv);
+  if (ref_err >= ((int64_t)1 << 31)) return false;
+  return true;
+}
+
+int32_t pick_sweep(const Geom& g) {
+  const int32_t bands = (g.h + kCpsTileH - 1) / kCpsTileH;
+  const int32_t tiles_x = (g.w + kCpsTileW - 1) / kCpsTileW;
+  const int64_t tiles = (int64_t)bands * tiles_x;
+  // About four resident workgroups per CU on a 256-CU device; longer sweeps amortise the
+  // per-workgroup row reduction.
+  return (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles_x, tiles / 1024));
+}
+
+Layout make_layout(const Geom& g, bool fast, int32_t sweep) {
+  Layout L{};
+  const int32_t nrows = g.h + 2 * g.ch;
+  const int32_t tiles_x = (g.w + kCpsTileW - 1) / kCpsTileW;
+  L.nseg = fast ? (tiles_x + sweep - 1) / sweep : (g.w + kGenericCols - 1) / kGenericCols;
+  size_t o = 0;
+  L.off_summary = o;
+  o = align_up(o + sizeof(FrameSummary), 256);
+  L.off_cps_rows = o;
+  o = align_up(o + sizeof(uint64_t) * 6 * (size_t)g.h, 256);
+  L.off_row_ssd = o;
+  o = align_up(o + sizeof(uint64_t) * (size_t)nrows, 256);
+  L.results_bytes = o;
+  L.off_row_sd = o;
+  o = align_up(o + sizeof(int64_t) * (size_t)nrows, 256);
+  L.off_row_max = o;
+  o = align_up(o + sizeof(uint32_t) * 2 * (size_t)nrows, 256);
+  L.off_partials = o;
+  o = align_up(o + sizeof(uint64_t) * 6 * (size_t)g.h * (size_t)L.nseg, 256);
+  L.total = o;
+  return L;
+}
+
+// Brackets one kernel launch with events when profiling is on.
+struct KernelTimer {
+  qmiv_ctx* ctx;
+  int kind;
+  hipEvent_t end = nullptr;
+  KernelTimer(qmiv_ctx* c, int k) : ctx(c), kind(k) {
+    if (!ctx->profiling) return;
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess) return;
+      if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+      ctx->ev_pool.emplace_back(a, b);
+      ctx->ev_kind.push_back(0);
+    }
+    auto& pr = ctx->ev_pool[ctx->ev_used];
+    ctx->ev_kind[ctx->ev_used] = kind;
+    ctx->ev_used++;
+    (void)hipEventRecord(pr.first, ctx->stream);
+    end = pr.second;
+  }
+  ~KernelTimer() {
+    if (end) (void)hipEventRecord(end, ctx->stream);
+  }
+};
+
+// Adds the elapsed times of the recorded event pairs (after the stream has been synchronised).
+void harvest_events(qmiv_ctx* ctx) {
+  for (size_t i = 0; i < ctx->ev_used; i++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev_pool[i].first, ctx->ev_pool[i].second) == hipSuccess) {
+      ctx->prof.kernel_ms[ctx->ev_kind[i]] += ms;
+      ctx->prof.launches[ctx->ev_kind[i]] += 1;
+    }
+  }
+  ctx->ev_used = 0;
+}
+
+struct FrameWork {
+  bool need_gcd = false;      // compute the GCD on the device
+  const int32_t* gcd = nullptr;  // or use this one
+  bool need_cps = false;
+  bool need_scp = false;
+};
+
+// Enqueues statistics (+ GCD) and, when asked, the CPS search on the context stream, copies the
+// results block into pinned memory and waits for it.
+qmiv_status run_frame_kernels(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params& prm,
+                              const FrameWork& fw, Layout& L, bool& used_fast) {
+  const Geom& g = p0->g;
+  int32_t md[4];
+  for (int c = 0; c < 4; c++) md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
+  const int32_t* gbound = fw.gcd ? fw.gcd : md;
+  used_fast = fast_cps_ok(g, prm, gbound);
+  const int32_t sweep = pick_sweep(g);
+  L = make_layout(g, used_fast, sweep);
+  QMIV_TRY(ensure_scratch(ctx, L.total));
+  QMIV_TRY(ensure_results(ctx, L.results_bytes));
+  char* base = (char*)ctx->d_scratch;
+  FrameSummary* sum = (FrameSummary*)(base + L.off_summary);
+  uint64_t* cps_rows = (uint64_t*)(base + L.off_cps_rows);
+  uint64_t* row_ssd = (uint64_t*)(base + L.off_row_ssd);
+  int64_t* row_sd = (int64_t*)(base + L.off_row_sd);
+  uint32_t* row_max = (uint32_t*)(base + L.off_row_max);
+  uint64_t* partials = (uint64_t*)(base + L.off_partials);
+  hipStream_t st = ctx->stream;
+
+  if (fw.gcd) QMIV_HIP(hipMemcpyAsync(sum->gcd, fw.gcd, sizeof(int32_t) * 4, hipMemcpyHostToDevice, st));
+  {
+    KernelTimer t(ctx, QMIV_K_STATS);
+    QMIV_HIP(launch_stats(p0->dev, p1->dev, g, row_sd, row_ssd, row_max, st));
+  }
+  {
+    KernelTimer t(ctx, QMIV_K_SUMMARY);
+    QMIV_HIP(launch_summary(g, row_sd, row_ssd, row_max, md, fw.need_gcd ? 1 : 0, sum, st));
+  }
+  if (fw.need_cps || fw.need_scp) {
+    uint16_t* scp0 = nullptr;
+    uint16_t* scp1 = nullptr;
+    if (fw.need_scp) {
+      const size_t plane3 = 3 * (size_t)g.w * g.h;
+      QMIV_TRY(ensure_scp(ctx, 2 * plane3 * sizeof(uint16_t)));
+      scp0 = (uint16_t*)ctx->d_scp;
+      scp1 = scp0 + plane3;
+    }
+    CpsArgs a{};
+    a.p0 = p0->dev;
+    a.p1 = p1->dev;
+    a.g = g;
+    a.sum = sum;
+    for (int c = 0; c < 4; c++) a.ws[c] = prm.cmp_weights_search[c];
+    a.B = prm.search_range;
+    a.tiles_x = (g.w + kCpsTileW - 1) / kCpsTileW;
+    a.sweep = sweep;
+    a.nseg = L.nseg;
+    a.partials = partials;
+    a.scp0 = scp0;
+    a.scp1 = scp1;
+    a.maxv = (1 << g.bd) - 1;
+    {
+      KernelTimer t(ctx, QMIV_K_CPS);
+      if (used_fast) QMIV_HIP(launch_cps_fast(a, st));
+      else QMIV_HIP(launch_cps_generic(a, st));
+    }
+    {
+      KernelTimer t(ctx, QMIV_K_CPS_ROWS);
+      QMIV_HIP(launch_cps_rows(partials, L.nseg, g.h, cps_rows, st));
+    }
+  }
+  QMIV_HIP(hipMemcpyAsync(ctx->h_results, base, L.results_bytes, hipMemcpyDeviceToHost, st));
+  QMIV_HIP(hipStreamSynchronize(st));
+  harvest_events(ctx);
+  return QMIV_OK;
+}
+
+// Per-row 4:4:4 SSD of component c from the per-plane rows (4:2:0 chroma rows count twice
+// horizontally: each chroma row of the 4:4:4 picture is two copies of every sample).
+void rows444(const Geom& g, const uint64_t* row_ssd, int c, std::vector<uint64_t>& out) {
+  out.resize((size_t)g.h);
+  if (c == 0) {
+    for (int32_t y = 0; y < g.h; y++) out[(size_t)y] = row_ssd[y];
+    return;
+  }
+  const uint64_t* r = row_ssd + g.h + (size_t)(c - 1) * g.ch;
+  for (int32_t y = 0; y < g.h; y++) out[(size_t)y] = r[y >> g.cs] << g.cs;
+}
+
+void apply_row_semantics(uint64_t* rows, size_t n, bool int32_rows) {
+  if (!int32_rows) return;
+  for (size_t i = 0; i < n; i++) rows[i] = (uint64_t)(int64_t)(int32_t)(uint32_t)rows[i];
+}
+
+qmiv_status check_pair(const qmiv_pic* a, const qmiv_pic* b) {
+  if (!a || !b) return fail(QMIV_ERR_INVALID_ARG, "null picture");
+  if (!same_geom(a->g, b->g)) return fail(QMIV_ERR_INVALID_ARG, "pictures differ in size, bit depth or chroma format");
+  if (a->ctx != b->ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to different contexts");
+  return QMIV_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+const char* qmiv_last_error(void) { return g_last_error.c_str(); }
+int32_t qmiv_abi_version(void) { return QMIV_ABI_VERSION; }
+
+void qmiv_params_default(qmiv_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->search_range = 2;
+  p->cmp_weights_search[0] = 4;
+  p->cmp_weights_search[1] = 1;
+  p->cmp_weights_search[2] = 1;
+  p->cmp_weights_average[0] = 4;
+  p->cmp_weights_average[1] = 1;
+  p->cmp_weights_average[2] = 1;
+  p->unnoticeable_coef[0] = p->unnoticeable_coef[1] = p->unnoticeable_coef[2] = 0.01f;
+  p->lon_range_deg = 360;
+  p->lat_range_deg = 180;
+  p->cps_int32_rows = 1;
+  p->fake_psnr_for_exact = 1;
+}
+
+size_t qmiv_frame_bytes(int32_t width, int32_t height, int32_t bit_depth, int32_t chroma_format) {
+  Geom g;
+  if (make_geom(width, height, bit_depth, chroma_format, g) != QMIV_OK) return 0;
+  return g.frame_bytes;
+}
+
+qmiv_status qmiv_create(int32_t device, qmiv_ctx** out) {
+  if (!out) return fail(QMIV_ERR_INVALID_ARG, "out is null");
+  *out = nullptr;
+  int n = 0;
+  QMIV_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(QMIV_ERR_INVALID_ARG, "no such HIP device");
+  QMIV_HIP(hipSetDevice(device));
+  qmiv_ctx* ctx = new qmiv_ctx;
+  ctx->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(QMIV_ERR_HIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+  }
+  *out = ctx;
+  return QMIV_OK;
+}
+
+void qmiv_destroy(qmiv_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
+  if (ctx->d_scp) (void)hipFree(ctx->d_scp);
+  if (ctx->h_results) (void)hipHostFree(ctx->h_results);
+  for (auto& pr : ctx->ev_pool) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+void* qmiv_stream(qmiv_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+qmiv_status qmiv_profile_enable(qmiv_ctx* ctx, int32_t on) {
+  if (!ctx) return fail(QMIV_ERR_INVALID_ARG, "null context");
+  ctx->profiling = on != 0;
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_profile_read(qmiv_ctx* ctx, qmiv_profile* out, int32_t reset) {
+  if (!ctx || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  harvest_events(ctx);
+  *out = ctx->prof;
+  if (reset) ctx->prof = qmiv_profile{};
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_sync(qmiv_ctx* ctx) {
+  if (!ctx) return fail(QMIV_ERR_INVALID_ARG, "null context");
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_create(qmiv_ctx* ctx, int32_t width, int32_t height, int32_t bit_depth, int32_t chroma_format,
+                            qmiv_pic** out) {
+  if (!ctx || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  Geom g;
+  QMIV_TRY(make_geom(width, height, bit_depth, chroma_format, g));
+  QMIV_HIP(hipSetDevice(ctx->device));
+  void* dev = nullptr;
+  if (hipMalloc(&dev, align_up(g.frame_bytes, 256)) != hipSuccess)
+    return fail(QMIV_ERR_OUT_OF_MEMORY, "picture allocation failed");
+  *out = new qmiv_pic{ctx, g, dev, true};
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_wrap(qmiv_ctx* ctx, int32_t width, int32_t height, int32_t bit_depth, int32_t chroma_format,
+                          void* device_ptr, qmiv_pic** out) {
+  if (!ctx || !out || !device_ptr) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  Geom g;
+  QMIV_TRY(make_geom(width, height, bit_depth, chroma_format, g));
+  if (((uintptr_t)device_ptr) % (uintptr_t)g.bytes_per_sample)
+    return fail(QMIV_ERR_INVALID_ARG, "device pointer not aligned to the sample size");
+  *out = new qmiv_pic{ctx, g, device_ptr, false};
+  return QMIV_OK;
+}
+
+void qmiv_pic_destroy(qmiv_pic* pic) {
+  if (!pic) return;
+  // hipFree waits for outstanding device work itself; the picture never touches its context
+  // here, so pictures may outlive the context that created them.
+  if (pic->owned && pic->dev) (void)hipFree(pic->dev);
+  delete pic;
+}
+
+void* qmiv_pic_device_ptr(qmiv_pic* pic) { return pic ? pic->dev : nullptr; }
+
+qmiv_status qmiv_pic_upload(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes) {
+  if (!ctx || !pic || !host) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (bytes != pic->g.frame_bytes) return fail(QMIV_ERR_INVALID_ARG, "byte count differs from the frame size");
+  QMIV_HIP(hipMemcpyAsync(pic->dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, size_t bytes) {
+  if (!ctx || !pic || !host) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (bytes != pic->g.frame_bytes) return fail(QMIV_ERR_INVALID_ARG, "byte count differs from the frame size");
+  QMIV_HIP(hipMemcpyAsync(host, pic->dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm_in,
+                       uint32_t metrics, qmiv_frame_result* out) {
+  if (!ctx || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(p0, p1));
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  if (metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM))
+    return fail(QMIV_ERR_UNSUPPORTED, "SSIM-family metrics are not available in this build");
+  std::memset(out, 0, sizeof(*out));
+  const Geom& g = p0->g;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  FrameWork fw;
+  fw.need_gcd = (metrics & (QMIV_IVPSNR | QMIV_IVSSIM)) != 0;
+  fw.need_cps = (metrics & QMIV_IVPSNR) != 0;
+  fw.need_scp = (metrics & QMIV_IVSSIM) != 0;
+  Layout L;
+  bool fast = false;
+  QMIV_TRY(run_frame_kernels(ctx, p0, p1, prm, fw, L, fast));
+  const char* hb = (const char*)ctx->h_results;
+  const FrameSummary* sum = (const FrameSummary*)(hb + L.off_summary);
+  const uint64_t* h_cps = (const uint64_t*)(hb + L.off_cps_rows);
+  const uint64_t* h_ssd = (const uint64_t*)(hb + L.off_row_ssd);
+  const int32_t maxv = (1 << g.bd) - 1;
+  out->in_range[0] = g.bd >= 16 || sum->max_pel[0] <= maxv;
+  out->in_range[1] = g.bd >= 16 || sum->max_pel[1] <= maxv;
+  for (int c = 0; c < 3; c++) out->exact[c] = sum->ssd[c] == 0;
+  if (fw.need_gcd)
+    for (int c = 0; c < 4; c++) out->gcd[c] = sum->gcd[c];
+  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
+
+  const int32_t area = g.w * g.h;
+  const double fake = fin::psnr_from_ssd(1.0, area, g.bd);
+  std::vector<double> ws;
+  double corr = 1.0;
+  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  if (metrics & QMIV_PSNR) {
+    for (int c = 0; c < 3; c++) {
+      const double v = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
+      out->psnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : v;
+    }
+    out->psnr_pic = fin::pic_value(out->psnr, prm.cmp_weights_average);
+  }
+  if (metrics & QMIV_WSPSNR) {
+    std::vector<uint64_t> r;
+    for (int c = 0; c < 3; c++) {
+      double v;
+      if (prm.erp) {
+        rows444(g, h_ssd, c, r);
+        v = fin::wspsnr_from_rows(r.data(), g.h, ws.data(), corr, area, g.bd);
+      } else {
+        v = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
+      }
+      out->wspsnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : v;
+    }
+    out->wspsnr_pic = fin::pic_value(out->wspsnr, prm.cmp_weights_average);
+  }
+  if (metrics & QMIV_IVPSNR) {
+    std::vector<uint64_t> rows(h_cps, h_cps + 6 * (size_t)g.h);
+    apply_row_semantics(rows.data(), rows.size(), prm.cps_int32_rows != 0);
+    for (int d = 0; d < 2; d++)
+      for (int c = 0; c < 3; c++) {
+        uint64_t s = 0;
+        for (int32_t y = 0; y < g.h; y++) s += rows[((size_t)d * 3 + c) * g.h + y];
+        out->cps_sum[d][c] = s;
+      }
+    const double* wsp = prm.erp ? ws.data() : nullptr;
+    out->ivpsnr_r2t = fin::qual_from_rows(rows.data(), g.h, wsp, area, g.bd, prm.cmp_weights_average);
+    out->ivpsnr_t2r = fin::qual_from_rows(rows.data() + 3 * (size_t)g.h, g.h, wsp, area, g.bd, prm.cmp_weights_average);
+    out->ivpsnr = fin::min2(out->ivpsnr_r2t, out->ivpsnr_t2r);
+  }
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_gcd(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm_in,
+                          int32_t gcd[4]) {
+  if (!ctx || !gcd) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(tst, ref));
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  FrameWork fw;
+  fw.need_gcd = true;
+  Layout L;
+  bool fast;
+  // sum(Ref - Tst): the statistics pass computes sum(P1 - P0) with P0 = Tst, P1 = Ref.
+  QMIV_TRY(run_frame_kernels(ctx, tst, ref, prm, fw, L, fast));
+  const FrameSummary* sum = (const FrameSummary*)((const char*)ctx->h_results + L.off_summary);
+  for (int c = 0; c < 4; c++) gcd[c] = sum->gcd[c];
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_psnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, double psnr[3]) {
+  if (!ctx || !psnr) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(tst, ref));
+  qmiv_params prm;
+  qmiv_params_default(&prm);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  FrameWork fw;
+  Layout L;
+  bool fast;
+  QMIV_TRY(run_frame_kernels(ctx, tst, ref, prm, fw, L, fast));
+  const FrameSummary* sum = (const FrameSummary*)((const char*)ctx->h_results + L.off_summary);
+  const int32_t area = tst->g.w * tst->g.h;
+  for (int c = 0; c < 3; c++) psnr[c] = fin::psnr_from_ssd((double)sum->ssd[c], area, tst->g.bd);
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_wspsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm_in,
+                             double wspsnr[3]) {
+  if (!ctx || !wspsnr) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(tst, ref));
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  FrameWork fw;
+  Layout L;
+  bool fast;
+  QMIV_TRY(run_frame_kernels(ctx, tst, ref, prm, fw, L, fast));
+  const char* hb = (const char*)ctx->h_results;
+  const FrameSummary* sum = (const FrameSummary*)(hb + L.off_summary);
+  const uint64_t* h_ssd = (const uint64_t*)(hb + L.off_row_ssd);
+  const Geom& g = tst->g;
+  const int32_t area = g.w * g.h;
+  if (!prm.erp) {
+    for (int c = 0; c < 3; c++) wspsnr[c] = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
+    return QMIV_OK;
+  }
+  std::vector<double> ws;
+  double corr;
+  fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  std::vector<uint64_t> r;
+  for (int c = 0; c < 3; c++) {
+    rows444(g, h_ssd, c, r);
+    wspsnr[c] = fin::wspsnr_from_rows(r.data(), g.h, ws.data(), corr, area, g.bd);
+  }
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_ivpsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm_in,
+                             const int32_t gcd_ref2tst[4], double* ivpsnr, double* r2t, double* t2r,
+                             uint64_t* rows_out) {
+  if (!ctx || !gcd_ref2tst) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(tst, ref));
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  FrameWork fw;
+  fw.gcd = gcd_ref2tst;
+  fw.need_cps = true;
+  Layout L;
+  bool fast;
+  // R2T searches Tst + gcd in Ref: the kernel's direction 0 with P0 = Tst, P1 = Ref.
+  QMIV_TRY(run_frame_kernels(ctx, tst, ref, prm, fw, L, fast));
+  const char* hb = (const char*)ctx->h_results;
+  const FrameSummary* sum = (const FrameSummary*)(hb + L.off_summary);
+  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
+  const Geom& g = tst->g;
+  std::vector<uint64_t> rows((const uint64_t*)(hb + L.off_cps_rows), (const uint64_t*)(hb + L.off_cps_rows) + 6 * (size_t)g.h);
+  apply_row_semantics(rows.data(), rows.size(), prm.cps_int32_rows != 0);
+  std::vector<double> ws;
+  double corr;
+  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  const double* wsp = prm.erp ? ws.data() : nullptr;
+  const int32_t area = g.w * g.h;
+  const double q0 = fin::qual_from_rows(rows.data(), g.h, wsp, area, g.bd, prm.cmp_weights_average);
+  const double q1 = fin::qual_from_rows(rows.data() + 3 * (size_t)g.h, g.h, wsp, area, g.bd, prm.cmp_weights_average);
+  if (r2t) *r2t = q0;
+  if (t2r) *t2r = q1;
+  if (ivpsnr) *ivpsnr = fin::min2(q0, q1);
+  if (rows_out) std::memcpy(rows_out, rows.data(), rows.size() * sizeof(uint64_t));
+  return QMIV_OK;
+}
+
+}  // extern "C"

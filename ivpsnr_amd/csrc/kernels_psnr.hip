@@ -1,0 +1,545 @@
+// gfx950 kernels of the PSNR family: per-row statistics, the on-device GCD summary and the
+// corresponding-pixel-shift (CPS) search behind IV-PSNR and the shift-compensated pictures.
+//
+// Frames are read in their packed planar file layout; 4:2:0 chroma is addressed at
+// (y >> 1, x >> 1), which is exactly the nearest-neighbour 4:2:0 -> 4:4:4 upsampling of
+// libs/pmbb-core/src/xPixelOpsSTD.cpp:30-49, and picture edges are addressed by clamping,
+// which reads the same values as the reference's margin replication
+// (libs/pmbb-core/src/xPixelOpsSTD.cpp:210-236).
+#include "qmiv_internal.h"
+
+#include <climits>
+
+namespace qmiv {
+namespace {
+
+typedef short short2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <typename T>
+__device__ __forceinline__ uint32_t ldpel(const T* __restrict__ p, int64_t i) {
+  return (uint32_t)p[i];
+}
+// Buffer-resource view of one packed frame: 32-bit sample offsets, one VGPR per address, and
+// hardware range checking (reads beyond `bytes` return 0, writes are dropped).
+constexpr int32_t kBufDword3 = 0x00020000;  // raw buffer, gfx9-family data format
+template <typename T>
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Buf(const void* p, uint32_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, kBufDword3)) {}
+  __device__ __forceinline__ uint32_t ld(uint32_t i) const {
+    if constexpr (sizeof(T) == 2) return __builtin_amdgcn_raw_buffer_load_b16(r, i * 2u, 0, 0);
+    else return __builtin_amdgcn_raw_buffer_load_b8(r, i, 0, 0);
+  }
+  __device__ __forceinline__ void st(uint32_t i, uint32_t v) const {
+    if constexpr (sizeof(T) == 2) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, i * 2u, 0, 0);
+    else __builtin_amdgcn_raw_buffer_store_b8((unsigned char)v, r, i, 0, 0);
+  }
+};
+
+__device__ __forceinline__ int64_t wave_sum(int64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-row statistics: sum(P1 - P0), sum((P1 - P0)^2) and the largest pel of each picture.
+// One wave per row of any plane; 16-byte loads when the row is aligned.
+// Restates xDistortion::CalcSD / CalcSSD (libs/pmbb-core/src/xDistortionSTD.cpp:18-62) on the
+// 4:2:0 planes: the 4:4:4 sums are 4x the chroma-plane sums (each chroma sample appears 4 times).
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_stats(const T* __restrict__ p0, const T* __restrict__ p1, Geom g,
+                                               int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd,
+                                               uint32_t* __restrict__ row_max) {
+  const int lane = threadIdx.x & 63;
+  const int nrows = g.h + 2 * g.ch;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= nrows) return;
+  int64_t off;
+  int32_t width;
+  if (r < g.h) {
+    off = (int64_t)r * g.w;
+    width = g.w;
+  } else if (r < g.h + g.ch) {
+    off = g.cb_off + (int64_t)(r - g.h) * g.cw;
+    width = g.cw;
+  } else {
+    off = g.cr_off + (int64_t)(r - g.h - g.ch) * g.cw;
+    width = g.cw;
+  }
+  const T* a = p0 + off;
+  const T* b = p1 + off;
+  int64_t sd = 0;
+  uint64_t ssd = 0;
+  uint32_t m0 = 0, m1 = 0;
+  constexpr int kVec = 16 / sizeof(T);  // samples per 16-byte load
+  int32_t vec_end = 0;
+  if ((((uintptr_t)a | (uintptr_t)b) & 15) == 0) {
+    vec_end = (width / kVec) * kVec;
+    for (int32_t x = lane * kVec; x < vec_end; x += 64 * kVec) {
+      const uint4 va = *reinterpret_cast<const uint4*>(a + x);
+      const uint4 vb = *reinterpret_cast<const uint4*>(b + x);
+      const uint32_t wa[4] = {va.x, va.y, va.z, va.w};
+      const uint32_t wb[4] = {vb.x, vb.y, vb.z, vb.w};
+      int32_t sd_l = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+#pragma unroll
+        for (int s = 0; s < (int)(4 / sizeof(T)); s++) {
+          const uint32_t sh = s * 8 * sizeof(T);
+          const uint32_t mask = sizeof(T) == 1 ? 0xffu : 0xffffu;
+          const uint32_t ea = (wa[k] >> sh) & mask;
+          const uint32_t eb = (wb[k] >> sh) & mask;
+          const int32_t d = (int32_t)eb - (int32_t)ea;
+          sd_l += d;
+          const uint32_t sq = (uint32_t)d * (uint32_t)d;
+          ssd += sq;
+          m0 = max(m0, ea);
+          m1 = max(m1, eb);
+        }
+      }
+      sd += sd_l;
+    }
+  }
+  for (int32_t x = vec_end + lane; x < width; x += 64) {
+    const uint32_t ea = ldpel(a, x), eb = ldpel(b, x);
+    const int32_t d = (int32_t)eb - (int32_t)ea;
+    sd += d;
+    ssd += (uint32_t)d * (uint32_t)d;
+    m0 = max(m0, ea);
+    m1 = max(m1, eb);
+  }
+  sd = wave_sum(sd);
+  ssd = wave_sum(ssd);
+  m0 = wave_max(m0);
+  m1 = wave_max(m1);
+  if (lane == 0) {
+    row_sd[r] = sd;
+    row_ssd[r] = ssd;
+    row_max[r] = m0;
+    row_max[nrows + r] = m1;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Frame summary: plane totals, range flags and the global colour difference.
+// GCD restates xGlobClrDiff::CalcGlobalColorDiff (libs/pmbb-ivqm/src/xGlobClrDiff.cpp:15-48) as
+// the processor calls it (xGlobClrDiff.h:41): the 4:4:4 sum(P1 - P0) wraps in int32
+// (xDistortionSTD.cpp:18-28), is divided by the area in double precision and rounded half-even
+// (the SSE conversion of xCommonDefCORE.h:129-131), then clipped to +-maxdiff.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_summary(Geom g, const int64_t* __restrict__ row_sd,
+                                                 const uint64_t* __restrict__ row_ssd,
+                                                 const uint32_t* __restrict__ row_max, int32_t md0, int32_t md1,
+                                                 int32_t md2, int32_t md3, int32_t compute_gcd,
+                                                 FrameSummary* __restrict__ sum) {
+  __shared__ unsigned long long s_sd[3], s_ssd[3];
+  __shared__ unsigned int s_max[2];
+  if (threadIdx.x < 3) {
+    s_sd[threadIdx.x] = 0;
+    s_ssd[threadIdx.x] = 0;
+  }
+  if (threadIdx.x < 2) s_max[threadIdx.x] = 0;
+  __syncthreads();
+  const int nrows = g.h + 2 * g.ch;
+  int64_t sd[3] = {0, 0, 0};
+  uint64_t ssd[3] = {0, 0, 0};
+  uint32_t m0 = 0, m1 = 0;
+  for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+    const int c = r < g.h ? 0 : (r < g.h + g.ch ? 1 : 2);
+    sd[c] += row_sd[r];
+    ssd[c] += row_ssd[r];
+    m0 = max(m0, row_max[r]);
+    m1 = max(m1, row_max[nrows + r]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    sd[c] = wave_sum(sd[c]);
+    ssd[c] = wave_sum(ssd[c]);
+  }
+  m0 = wave_max(m0);
+  m1 = wave_max(m1);
+  if ((threadIdx.x & 63) == 0) {
+    for (int c = 0; c < 3; c++) {
+      atomicAdd(&s_sd[c], (unsigned long long)sd[c]);
+      atomicAdd(&s_ssd[c], (unsigned long long)ssd[c]);
+    }
+    atomicMax(&s_max[0], m0);
+    atomicMax(&s_max[1], m1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int64_t mult = g.cs ? 4 : 1;
+    const int32_t maxv = (1 << g.bd) - 1;
+    const int32_t md[4] = {md0, md1, md2, md3};
+    int64_t sd444[3];
+    for (int c = 0; c < 3; c++) {
+      sd444[c] = (int64_t)s_sd[c] * (c ? mult : 1);
+      sum->sd[c] = sd444[c];
+      sum->ssd[c] = (uint64_t)s_ssd[c] * (uint64_t)(c ? mult : 1);
+    }
+    sum->max_pel[0] = (int32_t)s_max[0];
+    sum->max_pel[1] = (int32_t)s_max[1];
+    sum->in_range = g.bd >= 16 ? 1 : ((int32_t)s_max[0] <= maxv && (int32_t)s_max[1] <= maxv);
+    if (compute_gcd) {
+      const double area = (double)(g.w * g.h);
+      for (int c = 0; c < 4; c++) {
+        const int32_t sd32 = c < 3 ? (int32_t)(uint32_t)(uint64_t)sd444[c] : 0;
+        const double avg = (double)sd32 / area;
+        const int32_t rnd = (int32_t)__builtin_rint(avg);
+        sum->gcd[c] = max(-md[c], min(rnd, md[c]));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fast CPS search, B = 2 (the reference default), for parameter sets whose terms fit the 24-bit
+// multiplier operands (checked on the host).
+//
+// For a pel p with value X (= tested pel + GCD) and candidate q of the searched picture Y:
+//   E(p, q) = sum_c w_c (X_c - Y_c(q))^2 = S(q) - 2 <w X, Y(q)> + sum_c w_c X_c^2.
+// The last term does not depend on q, so ordering the window by E equals ordering it by
+//   t(p, q) = S(q) - 2 <w X, Y(q)>,   S(q) = sum_c w_c Y_c(q)^2.
+// Each LDS record holds C(q) = 32 S(q) + h(q), with h(q) = 5 ly + lx its local raster position,
+// and each pel holds a_c = -64 w_c X_c, so three v_mad_i32_i24 give
+//   key(p, q) = 32 t(p, q) + h(q).
+// Within one 5x5 window h(q) - h(p) = 5 dy + dx spans [-12, 12], so the signed minimum of the
+// keys is the candidate with the smallest error and, among equal errors, the first in raster
+// order: the strict '<' scan of xCorrespPixelShiftSSE::xCalcDistWithinBlock
+// (libs/pmbb-ivqm/src/xCorrespPixelShiftSSE.cpp:40-71). The winner's offset is recovered as
+// (key - h(p) + 12) mod 32.
+//
+// Workgroup: 4 waves, tile 64 x 32 pels, each lane owns one column x 8 rows. The searched
+// picture's (tile + 2) haloed records sit in LDS (68 x 36 x 16 bytes). A workgroup sweeps
+// `sweep` tiles along x; per-row per-component sums are reduced over the wave after every tile
+// and accumulated in LDS, then written as one partial per (segment, direction, component, row).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int32_t mad24(int32_t a, int32_t b, int32_t c) {
+  // Kept as one instruction: the compiler would otherwise reassociate a chain of these into
+  // mul + mul + add3. ALU only.
+  int32_t d;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
+  return v;
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, int x0, int y0, int lane, int wave,
+                                             const int32_t (&gcd)[3], uint4* s_rec, uint64_t* s_acc) {
+  // Opaque copies: keeps the compiler from hoisting per-row address arithmetic out of the tile
+  // sweep and holding it in registers across the search.
+  asm volatile("" : "+v"(lane));
+  asm volatile("" : "+v"(wave));
+  constexpr int B = 2, TW = kCpsTileW, TH = kCpsTileH;
+  constexpr int LW = TW + 2 * B, LH = TH + 2 * B;
+  // R2T (DIR 0): P0 + gcd searched in P1; T2R (DIR 1): P1 - gcd searched in P0.
+  const Buf<T> Xp(DIR == 0 ? a.p0 : a.p1, g.frame_bytes);
+  const Buf<T> Yp(DIR == 0 ? a.p1 : a.p0, g.frame_bytes);
+  uint16_t* scp_ptr = DIR == 0 ? a.scp1 : a.scp0;
+  const Buf<uint16_t> scp(scp_ptr, scp_ptr ? 6u * (uint32_t)g.w * (uint32_t)g.h : 0u);
+  constexpr int32_t sgn = DIR == 0 ? 1 : -1;
+  const int32_t w0 = a.ws[0], w1 = a.ws[1], w2 = a.ws[2];
+  const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
+
+  __syncthreads();
+  for (int i = threadIdx.x; i < LH * LW; i += 256) {
+    const int ly = i / LW;
+    const int lx = i - ly * LW;
+    const int gy = clampi(y0 - B + ly, 0, g.h - 1);
+    const int gx = clampi(x0 - B + lx, 0, g.w - 1);
+    const uint32_t yv = Yp.ld((uint32_t)(gy * g.w + gx));
+    const uint32_t co = (uint32_t)((gy >> g.cs) * g.cw + (gx >> g.cs));
+    const uint32_t uv = Yp.ld(cb_off + co);
+    const uint32_t vv = Yp.ld(cr_off + co);
+    const int32_t S = w0 * (int32_t)(yv * yv) + w1 * (int32_t)(uv * uv) + w2 * (int32_t)(vv * vv);
+    s_rec[i] = make_uint4(yv, uv, vv, (uint32_t)(S * 32 + 5 * ly + lx));
+  }
+  __syncthreads();
+
+  // The lane's pels: X = pel + sgn * gcd (not clipped: xc_CLIP_CURR_TST_RANGE = false,
+  // libs/pmbb-ivqm/src/xCommonDefIVQM.h:36). Only the scaled coefficients stay live during the
+  // search; X is re-read for the winner's error.
+  const int x = x0 + lane;
+  const int xc = min(x, g.w - 1);
+  const int32_t m0 = -64 * w0, m1 = -64 * w1, m2 = -64 * w2;
+  int32_t a0[8], a1[8], a2[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int y = min(y0 + 8 * wave + k, g.h - 1);
+    const uint32_t co = (uint32_t)((y >> g.cs) * g.cw + (xc >> g.cs));
+    const int32_t v0 = (int32_t)Xp.ld((uint32_t)(y * g.w + xc)) + sgn * gcd[0];
+    const int32_t v1 = (int32_t)Xp.ld(cb_off + co) + sgn * gcd[1];
+    const int32_t v2 = (int32_t)Xp.ld(cr_off + co) + sgn * gcd[2];
+    a0[k] = __mul24(m0, v0);
+    a1[k] = __mul24(m1, v1);
+    a2[k] = __mul24(m2, v2);
+  }
+
+  int32_t best[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) best[k] = INT_MAX;
+#pragma unroll
+  for (int rr = 0; rr < 12; rr++) {
+    const int base = (8 * wave + rr) * LW + lane;
+    uint4 q[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) q[m] = s_rec[base + m];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int dy = rr - 2 - k;
+      if (dy < -2 || dy > 2) continue;
+      int32_t key[5];
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        int32_t t = mad24(a0[k], (int32_t)q[m].x, (int32_t)q[m].w);
+        t = mad24(a1[k], (int32_t)q[m].y, t);
+        key[m] = mad24(a2[k], (int32_t)q[m].z, t);
+      }
+      best[k] = min(min(best[k], key[0]), key[1]);
+      best[k] = min(min(best[k], key[2]), key[3]);
+      best[k] = min(best[k], key[4]);
+      // Materialise the running minimum now: otherwise the min chain is re-associated across
+      // window rows and the keys wait in scratch.
+      asm volatile("" : "+v"(best[k]));
+    }
+    // One window row at a time: bounds the live keys and records.
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  // Winning candidate: per-component squared error of X against it (and the SCP pel).
+  // The clobber makes the X re-read below a real load instead of keeping the prologue's
+  // values (and addresses) live through the search.
+  asm volatile("" ::: "memory");
+  asm volatile("" : "+v"(lane));
+  asm volatile("" : "+v"(wave));
+  const int xe = x0 + lane;
+  const int xce = min(xe, g.w - 1);
+  uint32_t sq[8][3];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int ly = 8 * wave + k + 2;
+    const int hp = 5 * ly + lane + 2;
+    const int idx = (best[k] - hp + 12) & 31;
+    const int dy = idx / 5;
+    const int dx = idx - dy * 5;
+    const uint4 r = s_rec[(ly - 2 + dy) * LW + lane + dx];
+    const int y = y0 + 8 * wave + k;
+    const int yc = min(y, g.h - 1);
+    const uint32_t co = (uint32_t)((yc >> g.cs) * g.cw + (xce >> g.cs));
+    const int32_t xv0 = (int32_t)Xp.ld((uint32_t)(yc * g.w + xce)) + sgn * gcd[0];
+    const int32_t xv1 = (int32_t)Xp.ld(cb_off + co) + sgn * gcd[1];
+    const int32_t xv2 = (int32_t)Xp.ld(cr_off + co) + sgn * gcd[2];
+    const int32_t d0 = xv0 - (int32_t)r.x, d1 = xv1 - (int32_t)r.y, d2 = xv2 - (int32_t)r.z;
+    const bool inside = xe < g.w && y < g.h;
+    sq[k][0] = inside ? (uint32_t)(d0 * d0) : 0u;
+    sq[k][1] = inside ? (uint32_t)(d1 * d1) : 0u;
+    sq[k][2] = inside ? (uint32_t)(d2 * d2) : 0u;
+    if (scp_ptr && inside) {
+      const uint32_t o = (uint32_t)(y * g.w + xe);
+      const uint32_t plane = (uint32_t)(g.w * g.h);
+      scp.st(o, (uint32_t)clampi((int32_t)r.x - sgn * gcd[0], 0, a.maxv));
+      scp.st(plane + o, (uint32_t)clampi((int32_t)r.y - sgn * gcd[1], 0, a.maxv));
+      scp.st(2 * plane + o, (uint32_t)clampi((int32_t)r.z - sgn * gcd[2], 0, a.maxv));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const uint32_t v = wave_sum_u32(sq[k][c]);
+      if (lane == 0) s_acc[(DIR * kCpsTileH + 8 * wave + k) * 3 + c] += v;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_cps_fast(CpsArgs a) {
+  constexpr int TW = kCpsTileW, TH = kCpsTileH;
+  constexpr int LW = TW + 4, LH = TH + 4;
+  __shared__ uint4 s_rec[LH * LW];
+  __shared__ uint64_t s_acc[2 * TH * 3];
+
+  const FrameSummary* __restrict__ sum = a.sum;
+  if (!sum->in_range) return;
+  const Geom g = a.g;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int band = blockIdx.x / a.nseg;
+  const int seg = blockIdx.x - band * a.nseg;
+  const int y0 = band * TH;
+  const int tx_begin = seg * a.sweep;
+  const int tx_end = min(a.tiles_x, tx_begin + a.sweep);
+  const int32_t gcd[3] = {sum->gcd[0], sum->gcd[1], sum->gcd[2]};
+  for (int i = threadIdx.x; i < 2 * TH * 3; i += 256) s_acc[i] = 0;
+
+  for (int tx = tx_begin; tx < tx_end; tx++) {
+    const int x0 = tx * TW;
+    cps_tile_dir<T, 0>(a, g, x0, y0, lane, wave, gcd, s_rec, s_acc);
+    cps_tile_dir<T, 1>(a, g, x0, y0, lane, wave, gcd, s_rec, s_acc);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 2 * TH * 3; i += 256) {
+    const int d = i / (TH * 3);
+    const int row = (i / 3) % TH;
+    const int c = i % 3;
+    const int y = y0 + row;
+    if (y < g.h) a.partials[(((int64_t)seg * 2 + d) * 3 + c) * g.h + y] = s_acc[i];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Generic CPS search: any search range, weights and bit depth. One thread per pel, the window
+// read straight from memory, errors in wrapping 32-bit arithmetic over the four interleaved
+// lanes exactly as xCorrespPixelShiftSSE::xCalcDistWithinBlock computes them (the fourth lane
+// holds 0 in the picture and the GCD's fourth entry in the tested value).
+// grid = (ceil(w / 256), h, 2 directions).
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_cps_generic(CpsArgs a) {
+  __shared__ uint64_t red[3][256];
+  const FrameSummary* __restrict__ sum = a.sum;
+  if (!sum->in_range) return;
+  const Geom g = a.g;
+  const int y = blockIdx.y;
+  const int dir = blockIdx.z;
+  const int x = blockIdx.x * kGenericCols + threadIdx.x;
+  const int32_t sgn = dir == 0 ? 1 : -1;
+  const T* __restrict__ Xp = (const T*)(dir == 0 ? a.p0 : a.p1);
+  const T* __restrict__ Yp = (const T*)(dir == 0 ? a.p1 : a.p0);
+  uint16_t* __restrict__ scp = dir == 0 ? a.scp1 : a.scp0;
+  uint32_t dist_best[3] = {0, 0, 0};
+  if (x < g.w) {
+    int32_t t[4];
+    const int64_t co = (int64_t)(y >> g.cs) * g.cw + (x >> g.cs);
+    t[0] = (int32_t)ldpel(Xp, (int64_t)y * g.w + x) + sgn * sum->gcd[0];
+    t[1] = (int32_t)ldpel(Xp, g.cb_off + co) + sgn * sum->gcd[1];
+    t[2] = (int32_t)ldpel(Xp, g.cr_off + co) + sgn * sum->gcd[2];
+    t[3] = sgn * sum->gcd[3];
+    int32_t best_err = INT_MAX;
+    int64_t best_q = -1, best_co = -1;
+    for (int dy = -a.B; dy <= a.B; dy++) {
+      const int qy = clampi(y + dy, 0, g.h - 1);
+      for (int dx = -a.B; dx <= a.B; dx++) {
+        const int qx = clampi(x + dx, 0, g.w - 1);
+        const int64_t q = (int64_t)qy * g.w + qx;
+        const int64_t qc = (int64_t)(qy >> g.cs) * g.cw + (qx >> g.cs);
+        const int32_t r[4] = {(int32_t)ldpel(Yp, q), (int32_t)ldpel(Yp, g.cb_off + qc),
+                              (int32_t)ldpel(Yp, g.cr_off + qc), 0};
+        uint32_t dist[4], err = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const uint32_t d = (uint32_t)(t[c] - r[c]);
+          dist[c] = d * d;
+          err += dist[c] * (uint32_t)a.ws[c];
+        }
+        if ((int32_t)err < best_err) {
+          best_err = (int32_t)err;
+          dist_best[0] = dist[0];
+          dist_best[1] = dist[1];
+          dist_best[2] = dist[2];
+          best_q = q;
+          best_co = qc;
+        }
+      }
+    }
+    if (scp && best_q >= 0) {
+      const int64_t o = (int64_t)y * g.w + x;
+      const int64_t plane = (int64_t)g.w * g.h;
+      const int32_t sel[3] = {(int32_t)ldpel(Yp, best_q), (int32_t)ldpel(Yp, g.cb_off + best_co),
+                              (int32_t)ldpel(Yp, g.cr_off + best_co)};
+      for (int c = 0; c < 3; c++)
+        scp[c * plane + o] = (uint16_t)clampi(sel[c] - sgn * sum->gcd[c], 0, a.maxv);
+    }
+  }
+  for (int c = 0; c < 3; c++) red[c][threadIdx.x] = dist_best[c];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) {
+    uint64_t* part = a.partials;
+    part[(((int64_t)blockIdx.x * 2 + dir) * 3 + threadIdx.x) * g.h + y] = red[threadIdx.x][0];
+  }
+}
+
+// rows[d][c][y] = sum over segments of the partials.
+__global__ __launch_bounds__(256) void k_cps_rows(const uint64_t* __restrict__ part, int32_t nseg, int32_t h,
+                                                  uint64_t* __restrict__ rows) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = 6 * (int64_t)h;
+  if (i >= n) return;
+  uint64_t s = 0;
+  for (int k = 0; k < nseg; k++) s += part[(int64_t)k * n + i];
+  rows[i] = s;
+}
+
+}  // namespace
+
+hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* row_sd, uint64_t* row_ssd,
+                        uint32_t* row_max, hipStream_t stream) {
+  const int nrows = g.h + 2 * g.ch;
+  dim3 grid((nrows + 3) / 4), block(256);
+  if (g.bytes_per_sample == 1)
+    hipLaunchKernelGGL(k_stats<uint8_t>, grid, block, 0, stream, (const uint8_t*)p0, (const uint8_t*)p1, g, row_sd,
+                       row_ssd, row_max);
+  else
+    hipLaunchKernelGGL(k_stats<uint16_t>, grid, block, 0, stream, (const uint16_t*)p0, (const uint16_t*)p1, g,
+                       row_sd, row_ssd, row_max);
+  return hipGetLastError();
+}
+
+hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* row_ssd, const uint32_t* row_max,
+                          const int32_t maxdiff[4], int32_t compute_gcd, FrameSummary* sum, hipStream_t stream) {
+  hipLaunchKernelGGL(k_summary, dim3(1), dim3(256), 0, stream, g, row_sd, row_ssd, row_max, maxdiff[0], maxdiff[1],
+                     maxdiff[2], maxdiff[3], compute_gcd, sum);
+  return hipGetLastError();
+}
+
+hipError_t launch_cps_fast(const CpsArgs& a, hipStream_t stream) {
+  const int bands = (a.g.h + kCpsTileH - 1) / kCpsTileH;
+  dim3 grid(bands * a.nseg), block(256);
+  if (a.g.bytes_per_sample == 1)
+    hipLaunchKernelGGL(k_cps_fast<uint8_t>, grid, block, 0, stream, a);
+  else
+    hipLaunchKernelGGL(k_cps_fast<uint16_t>, grid, block, 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_cps_generic(const CpsArgs& a, hipStream_t stream) {
+  dim3 grid(a.nseg, a.g.h, 2), block(kGenericCols);
+  if (a.g.bytes_per_sample == 1)
+    hipLaunchKernelGGL(k_cps_generic<uint8_t>, grid, block, 0, stream, a);
+  else
+    hipLaunchKernelGGL(k_cps_generic<uint16_t>, grid, block, 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, uint64_t* rows, hipStream_t stream) {
+  const int64_t n = 6 * (int64_t)h;
+  hipLaunchKernelGGL(k_cps_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, partials, nseg, h, rows);
+  return hipGetLastError();
+}
+
+}  // namespace qmiv

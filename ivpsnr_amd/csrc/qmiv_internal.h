@@ -1,0 +1,67 @@
+// Internal declarations shared by the gfx950 kernels (*.hip) and the host runtime (*.cpp).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace qmiv {
+
+// Geometry of one packed planar frame as it sits in device memory.
+struct Geom {
+  int32_t w, h;      // luma size
+  int32_t cw, ch;    // chroma plane size
+  int32_t cs;        // chroma subsampling shift: 1 for 4:2:0, 0 for 4:4:4
+  int32_t bd;        // bit depth
+  int32_t bytes_per_sample;
+  uint32_t frame_bytes;  // bytes of one packed frame (< 4 GiB)
+  int64_t cb_off;    // sample offset of the Cb plane
+  int64_t cr_off;    // sample offset of the Cr plane
+};
+
+// Per-frame summary written by the statistics pass (device memory, read back by the host).
+struct FrameSummary {
+  int32_t gcd[4];          // GCD R2T (also the GCD consumed by the CPS kernels)
+  int32_t max_pel[2];      // largest pel of P0 / P1 over all planes
+  int32_t in_range;        // both pictures within [0, 2^bd - 1]
+  int32_t pad_;
+  int64_t sd[3];           // 4:4:4-domain sum(P1 - P0), exact
+  uint64_t ssd[3];         // 4:4:4-domain sum((P1 - P0)^2), exact
+};
+
+// Row layout of the per-row statistics: luma rows [0, h), Cb rows [h, h + ch), Cr rows [h + ch, h + 2ch).
+inline int32_t stat_rows(const Geom& g) { return g.h + 2 * g.ch; }
+
+// Arguments of the corresponding-pixel-shift kernels.
+struct CpsArgs {
+  const void* p0;
+  const void* p1;
+  Geom g;
+  const FrameSummary* sum;   // gcd and in_range
+  int32_t ws[4];             // search weights
+  int32_t B;                 // search range (generic kernel)
+  int32_t tiles_x;           // tiles per band
+  int32_t sweep;             // tiles per workgroup
+  int32_t nseg;              // segments per band (= ceil(tiles_x / sweep)); generic kernel: column blocks
+  uint64_t* partials;        // [nseg][2][3][h] per-segment row sums
+  uint16_t* scp0;            // optional [3][h][w]: clipU(P0(best of P1 - g) + g)   (SCP of InputFile0)
+  uint16_t* scp1;            // optional [3][h][w]: clipU(P1(best of P0 + g) - g)   (SCP of InputFile1)
+  int32_t maxv;
+};
+
+// Kernel launchers (kernels.hip). All launch on `stream` and return hipSuccess or the launch error.
+hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* row_sd, uint64_t* row_ssd,
+                        uint32_t* row_max, hipStream_t stream);
+hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* row_ssd, const uint32_t* row_max,
+                          const int32_t maxdiff[4], int32_t compute_gcd, FrameSummary* sum, hipStream_t stream);
+hipError_t launch_cps_fast(const CpsArgs& a, hipStream_t stream);
+hipError_t launch_cps_generic(const CpsArgs& a, hipStream_t stream);
+hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, uint64_t* rows, hipStream_t stream);
+
+// Tile shape of the fast CPS kernel.
+constexpr int kCpsTileW = 64;   // 64 lanes x 1 column
+constexpr int kCpsTileH = 32;   // 4 waves x 8 rows
+constexpr int kGenericCols = 256;
+
+}  // namespace qmiv

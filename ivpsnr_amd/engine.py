@@ -1,0 +1,157 @@
+"""Python mirror of the engine's C ABI (include/qmiv.h), used by the tests, the smoke check and the
+benchmark. Every call goes through libqmiv.so; nothing here computes a metric.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import weakref
+
+import numpy as np
+
+from . import capi
+from .capi import FrameResult, Params, check, default_params, lib
+
+__all__ = ["Engine", "Picture", "default_params", "FrameResult", "Params"]
+
+
+class Picture:
+    """One packed frame in device memory (owned by the engine, or wrapping caller memory)."""
+
+    def __init__(self, engine: "Engine", handle: int, width: int, height: int, bit_depth: int,
+                 chroma_format: int, keepalive=None):
+        self.engine = engine
+        self.handle = C.c_void_p(handle)
+        self.width, self.height = width, height
+        self.bit_depth, self.chroma_format = bit_depth, chroma_format
+        self.nbytes = lib().qmiv_frame_bytes(width, height, bit_depth, chroma_format)
+        self._keepalive = keepalive
+
+    def upload(self, packed: np.ndarray) -> "Picture":
+        a = np.ascontiguousarray(packed)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"frame has {a.nbytes} bytes, expected {self.nbytes}")
+        check(lib().qmiv_pic_upload(self.engine.ctx, self.handle, a.ctypes.data, a.nbytes), "qmiv_pic_upload")
+        # The copy is ordered on the engine stream; keep the source alive until it completes.
+        self.engine.sync()
+        return self
+
+    def download(self) -> np.ndarray:
+        dt = np.uint8 if self.bit_depth <= 8 else np.uint16
+        out = np.empty(self.nbytes // np.dtype(dt).itemsize, dt)
+        check(lib().qmiv_pic_download(self.engine.ctx, self.handle, out.ctypes.data, out.nbytes), "qmiv_pic_download")
+        return out
+
+    @property
+    def device_ptr(self) -> int:
+        return lib().qmiv_pic_device_ptr(self.handle)
+
+    def close(self):
+        if self.handle:
+            lib().qmiv_pic_destroy(self.handle)
+            self.handle = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """A context on one HIP device with its own stream."""
+
+    def __init__(self, device: int = 0):
+        ctx = C.c_void_p()
+        check(lib().qmiv_create(device, C.byref(ctx)), "qmiv_create")
+        self.ctx = ctx
+        self._pics = []
+
+    @property
+    def stream(self) -> int:
+        return lib().qmiv_stream(self.ctx) or 0
+
+    def sync(self):
+        check(lib().qmiv_sync(self.ctx), "qmiv_sync")
+
+    def profile(self, on: bool = True):
+        """Bracket every kernel launch with HIP events on the engine stream."""
+        check(lib().qmiv_profile_enable(self.ctx, int(on)), "qmiv_profile_enable")
+
+    def profile_read(self, reset: bool = True) -> dict:
+        """{kernel name: (total ms, launches)} accumulated since the last reset."""
+        p = capi.Profile()
+        check(lib().qmiv_profile_read(self.ctx, C.byref(p), int(reset)), "qmiv_profile_read")
+        return {name: (p.kernel_ms[k], p.launches[k]) for k, name in capi.KERNEL_NAMES.items() if p.launches[k]}
+
+    def picture(self, width: int, height: int, bit_depth: int, chroma_format: int = 420,
+                data: np.ndarray | None = None) -> Picture:
+        h = C.c_void_p()
+        check(lib().qmiv_pic_create(self.ctx, width, height, bit_depth, chroma_format, C.byref(h)), "qmiv_pic_create")
+        p = Picture(self, h.value, width, height, bit_depth, chroma_format)
+        self._pics.append(weakref.ref(p))
+        if data is not None:
+            p.upload(data)
+        return p
+
+    def wrap(self, device_ptr: int, width: int, height: int, bit_depth: int, chroma_format: int = 420,
+             keepalive=None) -> Picture:
+        h = C.c_void_p()
+        check(lib().qmiv_pic_wrap(self.ctx, width, height, bit_depth, chroma_format, C.c_void_p(device_ptr),
+                                  C.byref(h)), "qmiv_pic_wrap")
+        return Picture(self, h.value, width, height, bit_depth, chroma_format, keepalive)
+
+    def frame(self, p0: Picture, p1: Picture, metrics: int, params: Params | None = None,
+              raise_on_range: bool = True) -> FrameResult:
+        r = FrameResult()
+        st = lib().qmiv_frame(self.ctx, p0.handle, p1.handle, C.byref(params) if params else None, metrics,
+                              C.byref(r))
+        if st != capi.QMIV_OK and (raise_on_range or st != 5):
+            check(st, "qmiv_frame")
+        r.status = st
+        return r
+
+    def calc_gcd(self, tst: Picture, ref: Picture, params: Params | None = None) -> np.ndarray:
+        g = (C.c_int32 * 4)()
+        check(lib().qmiv_calc_gcd(self.ctx, tst.handle, ref.handle, C.byref(params) if params else None, g),
+              "qmiv_calc_gcd")
+        return np.array(list(g), np.int32)
+
+    def calc_psnr(self, tst: Picture, ref: Picture) -> np.ndarray:
+        v = (C.c_double * 3)()
+        check(lib().qmiv_calc_psnr(self.ctx, tst.handle, ref.handle, v), "qmiv_calc_psnr")
+        return np.array(list(v))
+
+    def calc_wspsnr(self, tst: Picture, ref: Picture, params: Params | None = None) -> np.ndarray:
+        v = (C.c_double * 3)()
+        check(lib().qmiv_calc_wspsnr(self.ctx, tst.handle, ref.handle, C.byref(params) if params else None, v),
+              "qmiv_calc_wspsnr")
+        return np.array(list(v))
+
+    def calc_ivpsnr(self, tst: Picture, ref: Picture, gcd, params: Params | None = None, want_rows: bool = False):
+        g = (C.c_int32 * 4)(*[int(x) for x in gcd])
+        iv, r2t, t2r = C.c_double(), C.c_double(), C.c_double()
+        rows = np.zeros(6 * tst.height, np.uint64) if want_rows else None
+        check(lib().qmiv_calc_ivpsnr(self.ctx, tst.handle, ref.handle, C.byref(params) if params else None, g,
+                                     C.byref(iv), C.byref(r2t), C.byref(t2r),
+                                     rows.ctypes.data_as(C.POINTER(C.c_uint64)) if want_rows else None),
+              "qmiv_calc_ivpsnr")
+        out = {"ivpsnr": iv.value, "r2t": r2t.value, "t2r": t2r.value}
+        if want_rows:
+            out["rows"] = rows.reshape(2, 3, tst.height)
+        return out
+
+    def close(self):
+        for ref in self._pics:
+            p = ref()
+            if p is not None:
+                p.close()
+        self._pics.clear()
+        if self.ctx:
+            lib().qmiv_destroy(self.ctx)
+            self.ctx = C.c_void_p(0)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

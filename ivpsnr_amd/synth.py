@@ -81,24 +81,14 @@ def _field(w: int, h: int, rng: np.random.Generator, scale: float, max_value: in
 def _shift_blocks(plane: np.ndarray, block: int, shifts: np.ndarray, halve: bool) -> np.ndarray:
     """Per-block integer shift with clamp-to-edge addressing: out(y, x) = in(clamp(y+dy), clamp(x+dx))."""
     h, w = plane.shape
-    out = np.empty_like(plane)
-    by_n, bx_n = shifts.shape[:2]
-    for by in range(by_n):
-        y0, y1 = by * block, min(h, (by + 1) * block)
-        if y0 >= h:
-            break
-        for bx in range(bx_n):
-            x0, x1 = bx * block, min(w, (bx + 1) * block)
-            if x0 >= w:
-                break
-            dy, dx = int(shifts[by, bx, 0]), int(shifts[by, bx, 1])
-            if halve:
-                dy >>= 1
-                dx >>= 1
-            ys = np.clip(np.arange(y0, y1) + dy, 0, h - 1)
-            xs = np.clip(np.arange(x0, x1) + dx, 0, w - 1)
-            out[y0:y1, x0:x1] = plane[np.ix_(ys, xs)]
-    return out
+    sh = shifts >> 1 if halve else shifts
+    by = np.arange(h) // block
+    bx = np.arange(w) // block
+    dy = sh[by[:, None], bx[None, :], 0]
+    dx = sh[by[:, None], bx[None, :], 1]
+    ys = np.clip(np.arange(h)[:, None] + dy, 0, h - 1)
+    xs = np.clip(np.arange(w)[None, :] + dx, 0, w - 1)
+    return plane[ys, xs]
 
 
 def make_pair(width: int, height: int, bit_depth: int, seed: int, *, chroma_format: int = 420,

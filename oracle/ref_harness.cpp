@@ -183,6 +183,19 @@ double ms(Clock::time_point a, Clock::time_point b) {
 }  // namespace
 
 int main(int argc, char** argv) {
+  if (argc == 2 && std::string(argv[1]) == "-filter") {
+    // The SSIM window table as the reference stores it (xStructSimConsts.h:30-43), fp32 bit patterns.
+    fputc('[', stdout);
+    for (int i = 0; i < 11; i++)
+      for (int j = 0; j < 11; j++) {
+        uint32_t bits;
+        const float f = xStructSimConsts::c_FilterF[i][j];
+        memcpy(&bits, &f, 4);
+        fprintf(stdout, "%s%u", (i || j) ? "," : "", bits);
+      }
+    fputs("]\n", stdout);
+    return 0;
+  }
   Args A;
   if (!parse(argc, argv, A)) { usage(); return 2; }
   const eCrF CF = A.CF == 420 ? eCrF::CF420 : A.CF == 444 ? eCrF::CF444 : A.CF == 422 ? eCrF::CF422 : eCrF::CF400;

@@ -1,0 +1,159 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own code.
+
+Runs only where the reference tree is present: it drives oracle/_ref/qmiv_ref (the reference's
+pmbb-core + pmbb-ivqm sources built for x86-64-v2 without FMA, linked with oracle/ref_harness.cpp)
+on seeded synthetic inputs and on the reference's example frames, and stores inputs' SHA-256
+plus every output as JSON. The example YUV files themselves are not copied; only the numbers
+the reference computes on them are stored.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from ivpsnr_amd.synth import make_pair, sha256  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = os.path.join(ROOT, "oracle", "_ref", "qmiv_ref")
+EXAMPLES = "/root/reference/example"
+
+
+def rows_hash(rows) -> str:
+    return hashlib.sha256(np.asarray(rows, dtype="<u8").tobytes()).hexdigest()
+
+
+def run_ref(w, h, bd, cf, f0, f1, metrics, erp=False, extra=()):
+    cmd = [REF, "-w", str(w), "-h", str(h), "-bd", str(bd), "-cf", str(cf), "-i0", f0, "-i1", f1,
+           "-ml", metrics, "-nth", "8"] + (["-erp", "360", "180"] if erp else []) + list(extra)
+    r = subprocess.run(cmd, capture_output=True, text=True, check=True)
+    return json.loads(r.stdout)
+
+
+def write_pairs(td, pairs):
+    f0, f1 = os.path.join(td, "p0.yuv"), os.path.join(td, "p1.yuv")
+    with open(f0, "wb") as a, open(f1, "wb") as b:
+        for p in pairs:
+            a.write(p.ref.tobytes())
+            b.write(p.tst.tobytes())
+    return f0, f1
+
+
+def compact_frame(fr, keep_rows: bool):
+    out = {}
+    for k, v in fr.items():
+        if k in ("cps_rows", "ws_rows"):
+            arr = np.asarray(v, dtype=np.uint64)
+            out[k + "_sha256"] = [[rows_hash(c) for c in d] for d in arr] if k == "cps_rows" else \
+                [rows_hash(c) for c in arr]
+            out[k + "_sum"] = arr.sum(axis=-1).tolist()
+            if keep_rows:
+                out[k] = arr.tolist()
+        elif k in ("scp0", "scp1"):
+            out[k + "_sha256"] = [hashlib.sha256(bytes.fromhex(x)).hexdigest() for x in v]
+        elif k == "ws_weights":
+            if keep_rows:
+                out[k] = v
+        else:
+            out[k] = v
+    return out
+
+
+SMALL_CASES = [
+    # w, h, bd, chroma, seed, generator knobs
+    (64, 48, 8, 420, 1, {}),
+    (130, 66, 10, 420, 2, {}),
+    (67, 45, 8, 444, 3, {}),
+    (256, 128, 10, 420, 4, {"luma_offset": 12.0}),
+    (96, 70, 10, 420, 11, {}),
+    (200, 40, 12, 420, 5, {}),
+]
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("oracle/_ref/qmiv_ref is not built (make -C oracle ref)")
+    fixtures = {}
+    with tempfile.TemporaryDirectory() as td:
+        # 1. Small synthetic cases: every metric, full per-row arrays, both WS settings.
+        cases = []
+        for (w, h, bd, cf, seed, kw) in SMALL_CASES:
+            pair = make_pair(w, h, bd, seed, chroma_format=cf, **kw)
+            f0, f1 = write_pairs(td, [pair])
+            entry = {"width": w, "height": h, "bit_depth": bd, "chroma_format": cf, "seed": seed, "knobs": kw,
+                     "sha256_ref": sha256(pair.ref), "sha256_tst": sha256(pair.tst)}
+            for erp in (False, True):
+                out = run_ref(w, h, bd, cf, f0, f1, "All", erp, ["-rows", "-pics"])
+                entry["erp" if erp else "plain"] = compact_frame(out["frames"][0], keep_rows=True)
+            cases.append(entry)
+            print("small", w, h, bd, cf, file=sys.stderr)
+        fixtures["small_cases"] = cases
+
+        # 2. Plain IV-PSNR + PSNR: 3 frames 1920x1080 8-bit (seeds 1..3).
+        pairs = [make_pair(1920, 1080, 8, seed=1 + f) for f in range(3)]
+        f0, f1 = write_pairs(td, pairs)
+        out = run_ref(1920, 1080, 8, 420, f0, f1, "PSNR,IVPSNR", False, ["-rows"])
+        fixtures["plain_1080p8"] = {
+            "width": 1920, "height": 1080, "bit_depth": 8, "chroma_format": 420, "seeds": [1, 2, 3],
+            "sha256_ref": [sha256(p.ref) for p in pairs], "sha256_tst": [sha256(p.tst) for p in pairs],
+            "frames": [compact_frame(fr, keep_rows=False) for fr in out["frames"]], "avg": out["avg"]}
+        print("plain_1080p8", file=sys.stderr)
+
+        # 3. IV-PSNR + WS-PSNR on ERP: one 4096x2048 10-bit frame (seed 1).
+        pair = make_pair(4096, 2048, 10, seed=1)
+        f0, f1 = write_pairs(td, [pair])
+        out = run_ref(4096, 2048, 10, 420, f0, f1, "PSNR,WSPSNR,IVPSNR", True, ["-rows"])
+        fixtures["erp_4k10"] = {
+            "width": 4096, "height": 2048, "bit_depth": 10, "chroma_format": 420, "seed": 1,
+            "sha256_ref": sha256(pair.ref), "sha256_tst": sha256(pair.tst),
+            "frame": compact_frame(out["frames"][0], keep_rows=False)}
+        print("erp_4k10", file=sys.stderr)
+
+        # 4. SSIM family: one 1920x1080 10-bit frame (seed 1).
+        pair = make_pair(1920, 1080, 10, seed=1)
+        f0, f1 = write_pairs(td, [pair])
+        out = run_ref(1920, 1080, 10, 420, f0, f1, "SSIM,MSSSIM,IVSSIM", False, ["-rows"])
+        fixtures["ssim_1080p10"] = {
+            "width": 1920, "height": 1080, "bit_depth": 10, "chroma_format": 420, "seed": 1,
+            "sha256_ref": sha256(pair.ref), "sha256_tst": sha256(pair.tst),
+            "frame": compact_frame(out["frames"][0], keep_rows=False)}
+        print("ssim_1080p10", file=sys.stderr)
+
+    # 5. The reference's example frames (1 frame 1920x1080 8-bit each); numbers only.
+    if os.path.isdir(EXAMPLES):
+        ex = {}
+        orig = os.path.join(EXAMPLES, "Fencing_v8_original_1920x1080_yuv420p.yuv")
+        for name in ("origJpgCompressed", "synthesized", "original"):
+            other = os.path.join(EXAMPLES, f"Fencing_v8_{name}_1920x1080_yuv420p.yuv")
+            out = run_ref(1920, 1080, 8, 420, orig, other, "All", False, ["-rows"])
+            with open(orig, "rb") as a, open(other, "rb") as b:
+                sh = [hashlib.sha256(a.read()).hexdigest(), hashlib.sha256(b.read()).hexdigest()]
+            ex[name] = {"sha256_input0": sh[0], "sha256_input1": sh[1],
+                        "frame": compact_frame(out["frames"][0], keep_rows=False)}
+        # Published values (example/README.md:46-47), four decimals.
+        ex["published"] = {"origJpgCompressed": {"psnr_y": 34.3362, "ivpsnr": 39.1335},
+                           "synthesized": {"psnr_y": 34.0128, "ivpsnr": 44.3555}}
+        fixtures["examples"] = ex
+        print("examples", file=sys.stderr)
+
+    # 6. The SSIM window as stored by the reference (fp32 bit patterns, 11 x 11).
+    r = subprocess.run([REF, "-filter"], capture_output=True, text=True, check=True)
+    fixtures["ssim_filter_bits"] = json.loads(r.stdout)
+
+    for name, data in fixtures.items():
+        with open(os.path.join(HERE, f"{name}.json"), "w") as f:
+            json.dump(data, f, separators=(",", ":"))
+    print("wrote", sorted(fixtures), file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()
