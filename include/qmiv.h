@@ -173,6 +173,13 @@ qmiv_status qmiv_profile_read(qmiv_ctx* ctx, qmiv_profile* out, int32_t reset);
 qmiv_status qmiv_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm,
                        uint32_t metrics, qmiv_frame_result* out);
 
+/* The same flow for n frame pairs (sizes may differ between pairs): every frame's kernels are
+   enqueued back to back on the context's stream, then one wait, then the host finalisation of
+   each. Replaces the reference application's frame loop (apps/QMIV/src/xAppQMIV.cpp:666-748).
+   out[i] receives frame i; the first failing frame's status is returned. */
+qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                        const qmiv_params* prm, uint32_t metrics, qmiv_frame_result* out);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Processor-level calls (same argument meaning as the reference's processor methods)          */
 /* ------------------------------------------------------------------------------------------ */

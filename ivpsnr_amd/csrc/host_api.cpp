@@ -60,6 +60,8 @@ struct qmiv_ctx {
   // Shift-compensated pictures (IV-SSIM), 2 x [3][h][w] uint16.
   void* d_scp = nullptr;
   size_t scp_bytes = 0;
+  int32_t num_cus = 256;
+  int32_t cps_blocks_per_cu[2] = {0, 0};  // [u8, u16], queried on first use
   // Kernel timing: event pairs recorded around launches since the last harvest.
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -175,13 +177,24 @@ bool fast_cps_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) 
   return true;
 }
 
-int32_t pick_sweep(const Geom& g) {
+// Tiles per workgroup along x: just enough that every workgroup is resident at once
+// (CUs x resident workgroups per CU), which gives one round with no tail, and within the bound
+// that keeps the per-lane row sums (one pel per tile) below 2^32.
+int32_t pick_sweep(qmiv_ctx* ctx, const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) {
   const int32_t bands = (g.h + kCpsTileH - 1) / kCpsTileH;
   const int32_t tiles_x = (g.w + kCpsTileW - 1) / kCpsTileW;
   const int64_t tiles = (int64_t)bands * tiles_x;
-  // About four resident workgroups per CU on a 256-CU device; longer sweeps amortise the
-  // per-workgroup row reduction.
-  return (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles_x, tiles / 1024));
+  int32_t& bpc = ctx->cps_blocks_per_cu[g.bytes_per_sample == 1 ? 0 : 1];
+  if (bpc == 0) bpc = cps_fast_blocks_per_cu(g.bytes_per_sample);
+  const int64_t resident = std::max<int64_t>(1, (int64_t)ctx->num_cus * bpc);
+  int64_t sweep = (tiles + resident - 1) / resident;
+  const int64_t maxv = (1 << g.bd) - 1;
+  int64_t m = 0;
+  for (int c = 0; c < 3; c++) m = std::max<int64_t>(m, maxv + std::abs((int64_t)gcd_bound[c]));
+  const int64_t sq = (m + maxv) * (m + maxv);
+  sweep = std::min<int64_t>(sweep, std::max<int64_t>(1, (int64_t)0xffffffffLL / std::max<int64_t>(sq, 1)));
+  (void)p;
+  return (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles_x, sweep));
 }
 
 Layout make_layout(const Geom& g, bool fast, int32_t sweep) {
@@ -245,32 +258,42 @@ void harvest_events(qmiv_ctx* ctx) {
 }
 
 struct FrameWork {
-  bool need_gcd = false;      // compute the GCD on the device
+  bool need_gcd = false;         // compute the GCD on the device
   const int32_t* gcd = nullptr;  // or use this one
   bool need_cps = false;
   bool need_scp = false;
 };
 
-// Enqueues statistics (+ GCD) and, when asked, the CPS search on the context stream, copies the
-// results block into pinned memory and waits for it.
-qmiv_status run_frame_kernels(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params& prm,
-                              const FrameWork& fw, Layout& L, bool& used_fast) {
-  const Geom& g = p0->g;
+// Everything decided on the host before a frame's kernels are enqueued.
+struct FramePlan {
+  Layout L;
+  bool fast = false;
+  int32_t sweep = 1;
   int32_t md[4];
-  for (int c = 0; c < 4; c++) md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
-  const int32_t* gbound = fw.gcd ? fw.gcd : md;
-  used_fast = fast_cps_ok(g, prm, gbound);
-  const int32_t sweep = pick_sweep(g);
-  L = make_layout(g, used_fast, sweep);
-  QMIV_TRY(ensure_scratch(ctx, L.total));
-  QMIV_TRY(ensure_results(ctx, L.results_bytes));
-  char* base = (char*)ctx->d_scratch;
-  FrameSummary* sum = (FrameSummary*)(base + L.off_summary);
-  uint64_t* cps_rows = (uint64_t*)(base + L.off_cps_rows);
-  uint64_t* row_ssd = (uint64_t*)(base + L.off_row_ssd);
-  int64_t* row_sd = (int64_t*)(base + L.off_row_sd);
-  uint32_t* row_max = (uint32_t*)(base + L.off_row_max);
-  uint64_t* partials = (uint64_t*)(base + L.off_partials);
+};
+
+FramePlan plan_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, const FrameWork& fw) {
+  FramePlan P;
+  for (int c = 0; c < 4; c++) P.md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
+  const int32_t* gbound = fw.gcd ? fw.gcd : P.md;
+  P.fast = fast_cps_ok(g, prm, gbound);
+  P.sweep = pick_sweep(ctx, g, prm, gbound);
+  P.L = make_layout(g, P.fast, P.sweep);
+  return P;
+}
+
+// Enqueues one frame's statistics (+ GCD) and, when asked, its CPS search on the context stream,
+// with device scratch at `dbase` and the results block copied into pinned `hbase`.
+qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params& prm,
+                          const FrameWork& fw, const FramePlan& P, char* dbase, char* hbase) {
+  const Geom& g = p0->g;
+  const Layout& L = P.L;
+  FrameSummary* sum = (FrameSummary*)(dbase + L.off_summary);
+  uint64_t* cps_rows = (uint64_t*)(dbase + L.off_cps_rows);
+  uint64_t* row_ssd = (uint64_t*)(dbase + L.off_row_ssd);
+  int64_t* row_sd = (int64_t*)(dbase + L.off_row_sd);
+  uint32_t* row_max = (uint32_t*)(dbase + L.off_row_max);
+  uint64_t* partials = (uint64_t*)(dbase + L.off_partials);
   hipStream_t st = ctx->stream;
 
   if (fw.gcd) QMIV_HIP(hipMemcpyAsync(sum->gcd, fw.gcd, sizeof(int32_t) * 4, hipMemcpyHostToDevice, st));
@@ -280,7 +303,7 @@ qmiv_status run_frame_kernels(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic*
   }
   {
     KernelTimer t(ctx, QMIV_K_SUMMARY);
-    QMIV_HIP(launch_summary(g, row_sd, row_ssd, row_max, md, fw.need_gcd ? 1 : 0, sum, st));
+    QMIV_HIP(launch_summary(g, row_sd, row_ssd, row_max, P.md, fw.need_gcd ? 1 : 0, sum, st));
   }
   if (fw.need_cps || fw.need_scp) {
     uint16_t* scp0 = nullptr;
@@ -299,7 +322,7 @@ qmiv_status run_frame_kernels(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic*
     for (int c = 0; c < 4; c++) a.ws[c] = prm.cmp_weights_search[c];
     a.B = prm.search_range;
     a.tiles_x = (g.w + kCpsTileW - 1) / kCpsTileW;
-    a.sweep = sweep;
+    a.sweep = P.sweep;
     a.nseg = L.nseg;
     a.partials = partials;
     a.scp0 = scp0;
@@ -307,7 +330,7 @@ qmiv_status run_frame_kernels(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic*
     a.maxv = (1 << g.bd) - 1;
     {
       KernelTimer t(ctx, QMIV_K_CPS);
-      if (used_fast) QMIV_HIP(launch_cps_fast(a, st));
+      if (P.fast) QMIV_HIP(launch_cps_fast(a, st));
       else QMIV_HIP(launch_cps_generic(a, st));
     }
     {
@@ -315,10 +338,26 @@ qmiv_status run_frame_kernels(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic*
       QMIV_HIP(launch_cps_rows(partials, L.nseg, g.h, cps_rows, st));
     }
   }
-  QMIV_HIP(hipMemcpyAsync(ctx->h_results, base, L.results_bytes, hipMemcpyDeviceToHost, st));
-  QMIV_HIP(hipStreamSynchronize(st));
+  QMIV_HIP(hipMemcpyAsync(hbase, dbase, L.results_bytes, hipMemcpyDeviceToHost, st));
+  return QMIV_OK;
+}
+
+qmiv_status wait_stream(qmiv_ctx* ctx) {
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
   harvest_events(ctx);
   return QMIV_OK;
+}
+
+// Single frame through enqueue + wait, results at the start of ctx->h_results.
+qmiv_status run_frame_kernels(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params& prm,
+                              const FrameWork& fw, Layout& L, bool& used_fast) {
+  const FramePlan P = plan_frame(ctx, p0->g, prm, fw);
+  L = P.L;
+  used_fast = P.fast;
+  QMIV_TRY(ensure_scratch(ctx, P.L.total));
+  QMIV_TRY(ensure_results(ctx, P.L.results_bytes));
+  QMIV_TRY(enqueue_frame(ctx, p0, p1, prm, fw, P, (char*)ctx->d_scratch, (char*)ctx->h_results));
+  return wait_stream(ctx);
 }
 
 // Per-row 4:4:4 SSD of component c from the per-plane rows (4:2:0 chroma rows count twice
@@ -342,6 +381,64 @@ qmiv_status check_pair(const qmiv_pic* a, const qmiv_pic* b) {
   if (!a || !b) return fail(QMIV_ERR_INVALID_ARG, "null picture");
   if (!same_geom(a->g, b->g)) return fail(QMIV_ERR_INVALID_ARG, "pictures differ in size, bit depth or chroma format");
   if (a->ctx != b->ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to different contexts");
+  return QMIV_OK;
+}
+
+// Host finalisation of one frame from its results block (apps/QMIV/src/xAppQMIV.cpp:831-904).
+qmiv_status finalize_frame(const Geom& g, const qmiv_params& prm, uint32_t metrics, const FrameWork& fw,
+                           const Layout& L, const char* hb, qmiv_frame_result* out) {
+  std::memset(out, 0, sizeof(*out));
+  const FrameSummary* sum = (const FrameSummary*)(hb + L.off_summary);
+  const uint64_t* h_cps = (const uint64_t*)(hb + L.off_cps_rows);
+  const uint64_t* h_ssd = (const uint64_t*)(hb + L.off_row_ssd);
+  const int32_t maxv = (1 << g.bd) - 1;
+  out->in_range[0] = g.bd >= 16 || sum->max_pel[0] <= maxv;
+  out->in_range[1] = g.bd >= 16 || sum->max_pel[1] <= maxv;
+  for (int c = 0; c < 3; c++) out->exact[c] = sum->ssd[c] == 0;
+  if (fw.need_gcd)
+    for (int c = 0; c < 4; c++) out->gcd[c] = sum->gcd[c];
+  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
+
+  const int32_t area = g.w * g.h;
+  const double fake = fin::psnr_from_ssd(1.0, area, g.bd);
+  std::vector<double> ws;
+  double corr = 1.0;
+  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  if (metrics & QMIV_PSNR) {
+    for (int c = 0; c < 3; c++) {
+      const double v = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
+      out->psnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : v;
+    }
+    out->psnr_pic = fin::pic_value(out->psnr, prm.cmp_weights_average);
+  }
+  if (metrics & QMIV_WSPSNR) {
+    std::vector<uint64_t> r;
+    for (int c = 0; c < 3; c++) {
+      double v;
+      if (prm.erp) {
+        rows444(g, h_ssd, c, r);
+        v = fin::wspsnr_from_rows(r.data(), g.h, ws.data(), corr, area, g.bd);
+      } else {
+        v = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
+      }
+      out->wspsnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : v;
+    }
+    out->wspsnr_pic = fin::pic_value(out->wspsnr, prm.cmp_weights_average);
+  }
+  if (metrics & QMIV_IVPSNR) {
+    std::vector<uint64_t> rows(h_cps, h_cps + 6 * (size_t)g.h);
+    apply_row_semantics(rows.data(), rows.size(), prm.cps_int32_rows != 0);
+    for (int d = 0; d < 2; d++)
+      for (int c = 0; c < 3; c++) {
+        uint64_t s = 0;
+        for (int32_t y = 0; y < g.h; y++) s += rows[((size_t)d * 3 + c) * g.h + y];
+        out->cps_sum[d][c] = s;
+      }
+    const double* wsp = prm.erp ? ws.data() : nullptr;
+    out->ivpsnr_r2t = fin::qual_from_rows(rows.data(), g.h, wsp, area, g.bd, prm.cmp_weights_average);
+    out->ivpsnr_t2r = fin::qual_from_rows(rows.data() + 3 * (size_t)g.h, g.h, wsp, area, g.bd, prm.cmp_weights_average);
+    out->ivpsnr = fin::min2(out->ivpsnr_r2t, out->ivpsnr_t2r);
+  }
   return QMIV_OK;
 }
 
@@ -387,6 +484,9 @@ qmiv_status qmiv_create(int32_t device, qmiv_ctx** out) {
   QMIV_HIP(hipSetDevice(device));
   qmiv_ctx* ctx = new qmiv_ctx;
   ctx->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    ctx->num_cus = cus;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete ctx;
@@ -487,76 +587,50 @@ qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, si
 
 qmiv_status qmiv_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm_in,
                        uint32_t metrics, qmiv_frame_result* out) {
-  if (!ctx || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
-  QMIV_TRY(check_pair(p0, p1));
+  const qmiv_pic* a[1] = {p0};
+  const qmiv_pic* b[1] = {p1};
+  return qmiv_frames(ctx, 1, a, b, prm_in, metrics, out);
+}
+
+qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                        const qmiv_params* prm_in, uint32_t metrics, qmiv_frame_result* out) {
+  if (!ctx || !out || !p0 || !p1 || n < 0) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return QMIV_OK;
+  for (int32_t i = 0; i < n; i++) QMIV_TRY(check_pair(p0[i], p1[i]));
   qmiv_params prm;
   if (prm_in) prm = *prm_in;
   else qmiv_params_default(&prm);
   if (metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM))
     return fail(QMIV_ERR_UNSUPPORTED, "SSIM-family metrics are not available in this build");
-  std::memset(out, 0, sizeof(*out));
-  const Geom& g = p0->g;
   QMIV_HIP(hipSetDevice(ctx->device));
   FrameWork fw;
   fw.need_gcd = (metrics & (QMIV_IVPSNR | QMIV_IVSSIM)) != 0;
   fw.need_cps = (metrics & QMIV_IVPSNR) != 0;
   fw.need_scp = (metrics & QMIV_IVSSIM) != 0;
-  Layout L;
-  bool fast = false;
-  QMIV_TRY(run_frame_kernels(ctx, p0, p1, prm, fw, L, fast));
-  const char* hb = (const char*)ctx->h_results;
-  const FrameSummary* sum = (const FrameSummary*)(hb + L.off_summary);
-  const uint64_t* h_cps = (const uint64_t*)(hb + L.off_cps_rows);
-  const uint64_t* h_ssd = (const uint64_t*)(hb + L.off_row_ssd);
-  const int32_t maxv = (1 << g.bd) - 1;
-  out->in_range[0] = g.bd >= 16 || sum->max_pel[0] <= maxv;
-  out->in_range[1] = g.bd >= 16 || sum->max_pel[1] <= maxv;
-  for (int c = 0; c < 3; c++) out->exact[c] = sum->ssd[c] == 0;
-  if (fw.need_gcd)
-    for (int c = 0; c < 4; c++) out->gcd[c] = sum->gcd[c];
-  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
-
-  const int32_t area = g.w * g.h;
-  const double fake = fin::psnr_from_ssd(1.0, area, g.bd);
-  std::vector<double> ws;
-  double corr = 1.0;
-  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
-  if (metrics & QMIV_PSNR) {
-    for (int c = 0; c < 3; c++) {
-      const double v = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
-      out->psnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : v;
-    }
-    out->psnr_pic = fin::pic_value(out->psnr, prm.cmp_weights_average);
+  // Plan every frame, then lay their scratch and result blocks end to end.
+  std::vector<FramePlan> plans((size_t)n);
+  std::vector<size_t> doff((size_t)n), hoff((size_t)n);
+  size_t dtot = 0, htot = 0;
+  for (int32_t i = 0; i < n; i++) {
+    plans[(size_t)i] = plan_frame(ctx, p0[i]->g, prm, fw);
+    doff[(size_t)i] = dtot;
+    hoff[(size_t)i] = htot;
+    dtot += align_up(plans[(size_t)i].L.total, 256);
+    htot += align_up(plans[(size_t)i].L.results_bytes, 256);
   }
-  if (metrics & QMIV_WSPSNR) {
-    std::vector<uint64_t> r;
-    for (int c = 0; c < 3; c++) {
-      double v;
-      if (prm.erp) {
-        rows444(g, h_ssd, c, r);
-        v = fin::wspsnr_from_rows(r.data(), g.h, ws.data(), corr, area, g.bd);
-      } else {
-        v = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
-      }
-      out->wspsnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : v;
-    }
-    out->wspsnr_pic = fin::pic_value(out->wspsnr, prm.cmp_weights_average);
+  QMIV_TRY(ensure_scratch(ctx, dtot));
+  QMIV_TRY(ensure_results(ctx, htot));
+  for (int32_t i = 0; i < n; i++)
+    QMIV_TRY(enqueue_frame(ctx, p0[i], p1[i], prm, fw, plans[(size_t)i], (char*)ctx->d_scratch + doff[(size_t)i],
+                           (char*)ctx->h_results + hoff[(size_t)i]));
+  QMIV_TRY(wait_stream(ctx));
+  qmiv_status first_err = QMIV_OK;
+  for (int32_t i = 0; i < n; i++) {
+    qmiv_status st = finalize_frame(p0[i]->g, prm, metrics, fw, plans[(size_t)i].L,
+                                    (const char*)ctx->h_results + hoff[(size_t)i], &out[i]);
+    if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
   }
-  if (metrics & QMIV_IVPSNR) {
-    std::vector<uint64_t> rows(h_cps, h_cps + 6 * (size_t)g.h);
-    apply_row_semantics(rows.data(), rows.size(), prm.cps_int32_rows != 0);
-    for (int d = 0; d < 2; d++)
-      for (int c = 0; c < 3; c++) {
-        uint64_t s = 0;
-        for (int32_t y = 0; y < g.h; y++) s += rows[((size_t)d * 3 + c) * g.h + y];
-        out->cps_sum[d][c] = s;
-      }
-    const double* wsp = prm.erp ? ws.data() : nullptr;
-    out->ivpsnr_r2t = fin::qual_from_rows(rows.data(), g.h, wsp, area, g.bd, prm.cmp_weights_average);
-    out->ivpsnr_t2r = fin::qual_from_rows(rows.data() + 3 * (size_t)g.h, g.h, wsp, area, g.bd, prm.cmp_weights_average);
-    out->ivpsnr = fin::min2(out->ivpsnr_r2t, out->ivpsnr_t2r);
-  }
-  return QMIV_OK;
+  return first_err;
 }
 
 qmiv_status qmiv_calc_gcd(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm_in,

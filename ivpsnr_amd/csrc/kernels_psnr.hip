@@ -237,52 +237,112 @@ __device__ __forceinline__ int32_t mad24(int32_t a, int32_t b, int32_t c) {
   return d;
 }
 
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+// LDS tile of the fast kernel: rows [y0 - 2, y0 + 34), columns [x0 - 4, x0 + 68). The two extra
+// columns on each side keep every row start 8-byte aligned for the vectorised fill; the search
+// reads columns [x0 - 2, x0 + 66).
+constexpr int kCpsLW = kCpsTileW + 8;
+constexpr int kCpsLH = kCpsTileH + 4;
+
+// Fills the LDS records of picture Yp for the tile at (x0, y0):
+//   rec = {Y, U, V, C},  C = 32 * (w0 Y^2 + w1 U^2 + w2 V^2) + 5 ly + lx.
+template <typename T>
+__device__ __forceinline__ void cps_fill(const Buf<T>& Yp, const Geom& g, int x0, int y0, int32_t w0, int32_t w1,
+                                         int32_t w2, uint4* s_rec) {
+  constexpr int LW = kCpsLW, LH = kCpsLH;
+  const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
+  const bool interior = x0 >= 4 && x0 + kCpsTileW + 4 <= g.w && y0 >= 2 && y0 + kCpsTileH + 2 <= g.h && (g.w & 3) == 0;
+  if (interior) {
+    // Four records per thread and step: one aligned 8-byte (u16) or 4-byte (u8) luma load; chroma
+    // either two samples (4:2:0, each shared by two records) or four (4:4:4).
+    constexpr int QW = LW / 4;
+    for (int i = threadIdx.x; i < LH * QW; i += 256) {
+      const int ly = i / QW;
+      const int lx = (i - ly * QW) * 4;
+      const int gy = y0 - 2 + ly;
+      const int gx = x0 - 4 + lx;
+      uint32_t yv[4], uv[4], vv[4];
+      const uint32_t lo = (uint32_t)(gy * g.w + gx);
+      if constexpr (sizeof(T) == 2) {
+        const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(Yp.r, lo * 2u, 0, 0));
+        yv[0] = w.x & 0xffffu; yv[1] = w.x >> 16; yv[2] = w.y & 0xffffu; yv[3] = w.y >> 16;
+      } else {
+        const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, lo, 0, 0);
+        yv[0] = w & 0xffu; yv[1] = (w >> 8) & 0xffu; yv[2] = (w >> 16) & 0xffu; yv[3] = w >> 24;
+      }
+      if (g.cs) {
+        const uint32_t co = (uint32_t)((gy >> 1) * g.cw + (gx >> 1));
+        uint32_t u0, u1, v0, v1;
+        if constexpr (sizeof(T) == 2) {
+          const uint32_t wu = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, (cb_off + co) * 2u, 0, 0);
+          const uint32_t wv = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, (cr_off + co) * 2u, 0, 0);
+          u0 = wu & 0xffffu; u1 = wu >> 16; v0 = wv & 0xffffu; v1 = wv >> 16;
+        } else {
+          const uint32_t wu = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cb_off + co, 0, 0);
+          const uint32_t wv = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cr_off + co, 0, 0);
+          u0 = wu & 0xffu; u1 = wu >> 8; v0 = wv & 0xffu; v1 = wv >> 8;
+        }
+        uv[0] = uv[1] = u0; uv[2] = uv[3] = u1;
+        vv[0] = vv[1] = v0; vv[2] = vv[3] = v1;
+      } else {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
-  return v;
+        for (int m = 0; m < 4; m++) {
+          uv[m] = Yp.ld(cb_off + lo + m);
+          vv[m] = Yp.ld(cr_off + lo + m);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const int32_t S = w0 * (int32_t)(yv[m] * yv[m]) + w1 * (int32_t)(uv[m] * uv[m]) + w2 * (int32_t)(vv[m] * vv[m]);
+        s_rec[ly * LW + lx + m] = make_uint4(yv[m], uv[m], vv[m], (uint32_t)(S * 32 + 5 * ly + lx + m));
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < LH * LW; i += 256) {
+      const int ly = i / LW;
+      const int lx = i - ly * LW;
+      const int gy = clampi(y0 - 2 + ly, 0, g.h - 1);
+      const int gx = clampi(x0 - 4 + lx, 0, g.w - 1);
+      const uint32_t yv = Yp.ld((uint32_t)(gy * g.w + gx));
+      const uint32_t co = (uint32_t)((gy >> g.cs) * g.cw + (gx >> g.cs));
+      const uint32_t uv = Yp.ld(cb_off + co);
+      const uint32_t vv = Yp.ld(cr_off + co);
+      const int32_t S = w0 * (int32_t)(yv * yv) + w1 * (int32_t)(uv * uv) + w2 * (int32_t)(vv * vv);
+      s_rec[i] = make_uint4(yv, uv, vv, (uint32_t)(S * 32 + 5 * ly + lx));
+    }
+  }
 }
 
+// One direction over one tile: each of the lane's 8 pels finds its best candidate; the winner's
+// per-component squared error is added to acc (per-lane row sums kept across the tile sweep).
 template <typename T, int DIR>
 __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, int x0, int y0, int lane, int wave,
-                                             const int32_t (&gcd)[3], uint4* s_rec, uint64_t* s_acc) {
-  // Opaque copies: keeps the compiler from hoisting per-row address arithmetic out of the tile
-  // sweep and holding it in registers across the search.
+                                             const int32_t (&gcd)[3], uint4* s_rec, uint32_t (&acc)[8][3]) {
+  // Opaque copies: keeps per-row address arithmetic from being hoisted out of the tile sweep and
+  // held in registers across it.
   asm volatile("" : "+v"(lane));
   asm volatile("" : "+v"(wave));
-  constexpr int B = 2, TW = kCpsTileW, TH = kCpsTileH;
-  constexpr int LW = TW + 2 * B, LH = TH + 2 * B;
+  constexpr int LW = kCpsLW;
   // R2T (DIR 0): P0 + gcd searched in P1; T2R (DIR 1): P1 - gcd searched in P0.
   const Buf<T> Xp(DIR == 0 ? a.p0 : a.p1, g.frame_bytes);
   const Buf<T> Yp(DIR == 0 ? a.p1 : a.p0, g.frame_bytes);
   uint16_t* scp_ptr = DIR == 0 ? a.scp1 : a.scp0;
-  const Buf<uint16_t> scp(scp_ptr, scp_ptr ? 6u * (uint32_t)g.w * (uint32_t)g.h : 0u);
   constexpr int32_t sgn = DIR == 0 ? 1 : -1;
   const int32_t w0 = a.ws[0], w1 = a.ws[1], w2 = a.ws[2];
   const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
 
   __syncthreads();
-  for (int i = threadIdx.x; i < LH * LW; i += 256) {
-    const int ly = i / LW;
-    const int lx = i - ly * LW;
-    const int gy = clampi(y0 - B + ly, 0, g.h - 1);
-    const int gx = clampi(x0 - B + lx, 0, g.w - 1);
-    const uint32_t yv = Yp.ld((uint32_t)(gy * g.w + gx));
-    const uint32_t co = (uint32_t)((gy >> g.cs) * g.cw + (gx >> g.cs));
-    const uint32_t uv = Yp.ld(cb_off + co);
-    const uint32_t vv = Yp.ld(cr_off + co);
-    const int32_t S = w0 * (int32_t)(yv * yv) + w1 * (int32_t)(uv * uv) + w2 * (int32_t)(vv * vv);
-    s_rec[i] = make_uint4(yv, uv, vv, (uint32_t)(S * 32 + 5 * ly + lx));
-  }
+  cps_fill<T>(Yp, g, x0, y0, w0, w1, w2, s_rec);
   __syncthreads();
 
   // The lane's pels: X = pel + sgn * gcd (not clipped: xc_CLIP_CURR_TST_RANGE = false,
-  // libs/pmbb-ivqm/src/xCommonDefIVQM.h:36). Only the scaled coefficients stay live during the
-  // search; X is re-read for the winner's error.
+  // libs/pmbb-ivqm/src/xCommonDefIVQM.h:36), kept packed for the winner's error, and the scaled
+  // coefficients a_c = -64 w_c X_c of the key.
   const int x = x0 + lane;
   const int xc = min(x, g.w - 1);
   const int32_t m0 = -64 * w0, m1 = -64 * w1, m2 = -64 * w2;
   int32_t a0[8], a1[8], a2[8];
+  uint32_t x01[8];
+  int32_t x2[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const int y = min(y0 + 8 * wave + k, g.h - 1);
@@ -293,6 +353,8 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
     a0[k] = __mul24(m0, v0);
     a1[k] = __mul24(m1, v1);
     a2[k] = __mul24(m2, v2);
+    x01[k] = ((uint32_t)v0 & 0xffffu) | ((uint32_t)v1 << 16);
+    x2[k] = v2;
   }
 
   int32_t best[8];
@@ -300,7 +362,7 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
   for (int k = 0; k < 8; k++) best[k] = INT_MAX;
 #pragma unroll
   for (int rr = 0; rr < 12; rr++) {
-    const int base = (8 * wave + rr) * LW + lane;
+    const int base = (8 * wave + rr) * LW + lane + 2;
     uint4 q[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) q[m] = s_rec[base + m];
@@ -327,57 +389,37 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
   }
 
   // Winning candidate: per-component squared error of X against it (and the SCP pel).
-  // The clobber makes the X re-read below a real load instead of keeping the prologue's
-  // values (and addresses) live through the search.
-  asm volatile("" ::: "memory");
-  asm volatile("" : "+v"(lane));
-  asm volatile("" : "+v"(wave));
-  const int xe = x0 + lane;
-  const int xce = min(xe, g.w - 1);
-  uint32_t sq[8][3];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const int ly = 8 * wave + k + 2;
-    const int hp = 5 * ly + lane + 2;
+    const int hp = 5 * ly + lane + 4;
     const int idx = (best[k] - hp + 12) & 31;
     const int dy = idx / 5;
     const int dx = idx - dy * 5;
-    const uint4 r = s_rec[(ly - 2 + dy) * LW + lane + dx];
+    const uint4 r = s_rec[(ly - 2 + dy) * LW + lane + 2 + dx];
+    const int32_t xv0 = (int32_t)(int16_t)(x01[k] & 0xffffu);
+    const int32_t xv1 = (int32_t)(int16_t)(x01[k] >> 16);
+    const int32_t d0 = xv0 - (int32_t)r.x, d1 = xv1 - (int32_t)r.y, d2 = x2[k] - (int32_t)r.z;
     const int y = y0 + 8 * wave + k;
-    const int yc = min(y, g.h - 1);
-    const uint32_t co = (uint32_t)((yc >> g.cs) * g.cw + (xce >> g.cs));
-    const int32_t xv0 = (int32_t)Xp.ld((uint32_t)(yc * g.w + xce)) + sgn * gcd[0];
-    const int32_t xv1 = (int32_t)Xp.ld(cb_off + co) + sgn * gcd[1];
-    const int32_t xv2 = (int32_t)Xp.ld(cr_off + co) + sgn * gcd[2];
-    const int32_t d0 = xv0 - (int32_t)r.x, d1 = xv1 - (int32_t)r.y, d2 = xv2 - (int32_t)r.z;
-    const bool inside = xe < g.w && y < g.h;
-    sq[k][0] = inside ? (uint32_t)(d0 * d0) : 0u;
-    sq[k][1] = inside ? (uint32_t)(d1 * d1) : 0u;
-    sq[k][2] = inside ? (uint32_t)(d2 * d2) : 0u;
-    if (scp_ptr && inside) {
-      const uint32_t o = (uint32_t)(y * g.w + xe);
-      const uint32_t plane = (uint32_t)(g.w * g.h);
-      scp.st(o, (uint32_t)clampi((int32_t)r.x - sgn * gcd[0], 0, a.maxv));
-      scp.st(plane + o, (uint32_t)clampi((int32_t)r.y - sgn * gcd[1], 0, a.maxv));
-      scp.st(2 * plane + o, (uint32_t)clampi((int32_t)r.z - sgn * gcd[2], 0, a.maxv));
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const uint32_t v = wave_sum_u32(sq[k][c]);
-      if (lane == 0) s_acc[(DIR * kCpsTileH + 8 * wave + k) * 3 + c] += v;
+    if (x < g.w && y < g.h) {
+      acc[k][0] += (uint32_t)(d0 * d0);
+      acc[k][1] += (uint32_t)(d1 * d1);
+      acc[k][2] += (uint32_t)(d2 * d2);
+      if (scp_ptr) {
+        const Buf<uint16_t> scp(scp_ptr, 6u * (uint32_t)g.w * (uint32_t)g.h);
+        const uint32_t o = (uint32_t)(y * g.w + x);
+        const uint32_t plane = (uint32_t)(g.w * g.h);
+        scp.st(o, (uint32_t)clampi((int32_t)r.x - sgn * gcd[0], 0, a.maxv));
+        scp.st(plane + o, (uint32_t)clampi((int32_t)r.y - sgn * gcd[1], 0, a.maxv));
+        scp.st(2 * plane + o, (uint32_t)clampi((int32_t)r.z - sgn * gcd[2], 0, a.maxv));
+      }
     }
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_cps_fast(CpsArgs a) {
-  constexpr int TW = kCpsTileW, TH = kCpsTileH;
-  constexpr int LW = TW + 4, LH = TH + 4;
-  __shared__ uint4 s_rec[LH * LW];
-  __shared__ uint64_t s_acc[2 * TH * 3];
+__global__ __launch_bounds__(256) void k_cps_fast(CpsArgs a) {
+  __shared__ uint4 s_rec[kCpsLH * kCpsLW];
 
   const FrameSummary* __restrict__ sum = a.sum;
   if (!sum->in_range) return;
@@ -386,24 +428,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const int wave = threadIdx.x >> 6;
   const int band = blockIdx.x / a.nseg;
   const int seg = blockIdx.x - band * a.nseg;
-  const int y0 = band * TH;
+  const int y0 = band * kCpsTileH;
   const int tx_begin = seg * a.sweep;
   const int tx_end = min(a.tiles_x, tx_begin + a.sweep);
   const int32_t gcd[3] = {sum->gcd[0], sum->gcd[1], sum->gcd[2]};
-  for (int i = threadIdx.x; i < 2 * TH * 3; i += 256) s_acc[i] = 0;
+
+  // Per-lane row sums over the sweep (the host bounds sweep * max squared error below 2^32).
+  uint32_t acc0[8][3], acc1[8][3];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc0[k][c] = acc1[k][c] = 0;
 
   for (int tx = tx_begin; tx < tx_end; tx++) {
-    const int x0 = tx * TW;
-    cps_tile_dir<T, 0>(a, g, x0, y0, lane, wave, gcd, s_rec, s_acc);
-    cps_tile_dir<T, 1>(a, g, x0, y0, lane, wave, gcd, s_rec, s_acc);
+    const int x0 = tx * kCpsTileW;
+    cps_tile_dir<T, 0>(a, g, x0, y0, lane, wave, gcd, s_rec, acc0);
+    cps_tile_dir<T, 1>(a, g, x0, y0, lane, wave, gcd, s_rec, acc1);
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < 2 * TH * 3; i += 256) {
-    const int d = i / (TH * 3);
-    const int row = (i / 3) % TH;
-    const int c = i % 3;
-    const int y = y0 + row;
-    if (y < g.h) a.partials[(((int64_t)seg * 2 + d) * 3 + c) * g.h + y] = s_acc[i];
+
+  // One partial per (segment, direction, component, row): reduce over the wave's 64 lanes.
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int y = y0 + 8 * wave + k;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const uint64_t v0 = wave_sum((uint64_t)acc0[k][c]);
+      const uint64_t v1 = wave_sum((uint64_t)acc1[k][c]);
+      if (lane == 0 && y < g.h) {
+        a.partials[(((int64_t)seg * 2 + 0) * 3 + c) * g.h + y] = v0;
+        a.partials[(((int64_t)seg * 2 + 1) * 3 + c) * g.h + y] = v1;
+      }
+    }
   }
 }
 
@@ -525,6 +580,14 @@ hipError_t launch_cps_fast(const CpsArgs& a, hipStream_t stream) {
   else
     hipLaunchKernelGGL(k_cps_fast<uint16_t>, grid, block, 0, stream, a);
   return hipGetLastError();
+}
+
+int cps_fast_blocks_per_cu(int bytes_per_sample) {
+  int n = 0;
+  hipError_t e = bytes_per_sample == 1
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast<uint8_t>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast<uint16_t>, 256, 0);
+  return e == hipSuccess && n > 0 ? n : 1;
 }
 
 hipError_t launch_cps_generic(const CpsArgs& a, hipStream_t stream) {

@@ -58,6 +58,8 @@ hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* 
 hipError_t launch_cps_fast(const CpsArgs& a, hipStream_t stream);
 hipError_t launch_cps_generic(const CpsArgs& a, hipStream_t stream);
 hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, uint64_t* rows, hipStream_t stream);
+// Resident workgroups of the fast CPS kernel per CU (from the HIP occupancy calculator).
+int cps_fast_blocks_per_cu(int bytes_per_sample);
 
 // Tile shape of the fast CPS kernel.
 constexpr int kCpsTileW = 64;   // 64 lanes x 1 column

@@ -110,6 +110,17 @@ class Engine:
         r.status = st
         return r
 
+    def frames(self, pairs, metrics: int, params: Params | None = None, raise_on_error: bool = True):
+        """All (p0, p1) pairs enqueued back to back on the stream, one wait (qmiv_frames)."""
+        n = len(pairs)
+        a = (C.c_void_p * n)(*[p[0].handle.value for p in pairs])
+        b = (C.c_void_p * n)(*[p[1].handle.value for p in pairs])
+        out = (FrameResult * n)()
+        st = lib().qmiv_frames(self.ctx, n, a, b, C.byref(params) if params else None, metrics, out)
+        if raise_on_error:
+            check(st, "qmiv_frames")
+        return list(out)
+
     def calc_gcd(self, tst: Picture, ref: Picture, params: Params | None = None) -> np.ndarray:
         g = (C.c_int32 * 4)()
         check(lib().qmiv_calc_gcd(self.ctx, tst.handle, ref.handle, C.byref(params) if params else None, g),

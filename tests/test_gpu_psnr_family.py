@@ -163,3 +163,20 @@ def test_erp_full_size_frame(engine, oracle):
     want = oracle.frame(pair.ref, pair.tst, oracle_params(oracle, w, h, bd, 420, ALL_PSNR, prm))
     assert_frame_equal(got, want, ALL_PSNR, "4096x2048 erp")
     assert list(got.gcd)[:3] == [10, 0, 0]  # +12 luma offset clipped to rint(0.01 * 1023)
+
+
+def test_frames_batch_matches_single_calls(engine, oracle):
+    """qmiv_frames on mixed sizes and bit depths equals per-frame qmiv_frame and the oracle."""
+    specs = [(256, 128, 10, 420, 1), (130, 66, 8, 420, 2), (67, 45, 10, 444, 3), (512, 256, 10, 420, 4)]
+    pairs, pics = [], []
+    for (w, h, bd, cf, seed) in specs:
+        pr = make_pair(w, h, bd, seed, chroma_format=cf)
+        pairs.append(pr)
+        pics.append(_upload_pair(engine, pr))
+    prm = default_params(erp=True)
+    batch = engine.frames(pics, ALL_PSNR, prm)
+    for (w, h, bd, cf, seed), pr, (p0, p1), got in zip(specs, pairs, pics, batch):
+        single = engine.frame(p0, p1, ALL_PSNR, prm)
+        assert got.as_dict() == single.as_dict()
+        want = oracle.frame(pr.ref, pr.tst, oracle_params(oracle, w, h, bd, cf, ALL_PSNR, prm))
+        assert_frame_equal(got, want, ALL_PSNR, f"batch {w}x{h}")
