@@ -74,7 +74,20 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--frames", type=int, default=FRAMES, help="frame pairs per step (profiling runs only)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of 512 MiB (calibrates rocprof byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = 512 << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's workload runs on a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
@@ -83,8 +96,9 @@ def main():
     from ivpsnr_amd import IVPSNR, PSNR, WSPSNR, Engine, default_params
     from ivpsnr_amd.synth import make_pair
 
-    print(f"[bench] generating {FRAMES} synthetic frame pairs {W}x{H} {BD}-bit", file=sys.stderr, flush=True)
-    pairs = [make_pair(W, H, BD, seed=1 + f) for f in range(FRAMES)]
+    print(f"[bench] generating {args.frames} synthetic frame pairs {W}x{H} {BD}-bit", file=sys.stderr, flush=True)
+    nframes = args.frames
+    pairs = [make_pair(W, H, BD, seed=1 + f) for f in range(nframes)]
     metrics = PSNR | WSPSNR | IVPSNR
     prm = default_params(erp=True)
 
@@ -127,7 +141,7 @@ def main():
     elapsed = min(elapsed, el_noprof)
 
     ms_per_step = elapsed / args.steps * 1e3
-    mpix = FRAMES * W * H / 1e6
+    mpix = nframes * W * H / 1e6
     value = mpix / (ms_per_step / 1e3)
     cps_ms, cps_n = prof.get("cps", (0.0, 0))
     cps_avg_ms = cps_ms / max(cps_n, 1)
@@ -149,7 +163,7 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": "IV-PSNR+WS-PSNR+PSNR, 8 frame pairs 4096x2048 yuv420p10le ERP, B=2, GCD on",
-                   "frames_per_step": FRAMES, "mode": "stream (qmiv_frames: frame set enqueued, one wait)",
+                   "frames_per_step": nframes, "mode": "stream (qmiv_frames: frame set enqueued, one wait)",
                    "plain_loop_mpix_s": round(mpix / (el_plain / args.steps), 2),
                    "stream_vs_plain": round(el_plain / elapsed, 3),
                    "ivpsnr_last_frame": round(res[-1].ivpsnr, 6)},

@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libqmiv.so")
+# QMIV_LIB selects another build of the library (e.g. an experimental variant).
+LIB_PATH = os.environ.get("QMIV_LIB", os.path.join(HERE, "libqmiv.so"))
 
 QMIV_OK = 0
 STATUS_NAMES = {0: "QMIV_OK", 1: "QMIV_ERR_INVALID_ARG", 2: "QMIV_ERR_UNSUPPORTED", 3: "QMIV_ERR_HIP",

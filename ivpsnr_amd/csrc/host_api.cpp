@@ -181,8 +181,10 @@ bool fast_cps_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) 
 // (CUs x resident workgroups per CU), which gives one round with no tail, and within the bound
 // that keeps the per-lane row sums (one pel per tile) below 2^32.
 int32_t pick_sweep(qmiv_ctx* ctx, const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) {
-  const int32_t bands = (g.h + kCpsTileH - 1) / kCpsTileH;
-  const int32_t tiles_x = (g.w + kCpsTileW - 1) / kCpsTileW;
+  int32_t tw, th;
+  cps_fast_tile(&tw, &th);
+  const int32_t bands = (g.h + th - 1) / th;
+  const int32_t tiles_x = (g.w + tw - 1) / tw;
   const int64_t tiles = (int64_t)bands * tiles_x;
   int32_t& bpc = ctx->cps_blocks_per_cu[g.bytes_per_sample == 1 ? 0 : 1];
   if (bpc == 0) bpc = cps_fast_blocks_per_cu(g.bytes_per_sample);
@@ -200,7 +202,9 @@ int32_t pick_sweep(qmiv_ctx* ctx, const Geom& g, const qmiv_params& p, const int
 Layout make_layout(const Geom& g, bool fast, int32_t sweep) {
   Layout L{};
   const int32_t nrows = g.h + 2 * g.ch;
-  const int32_t tiles_x = (g.w + kCpsTileW - 1) / kCpsTileW;
+  int32_t tw, th;
+  cps_fast_tile(&tw, &th);
+  const int32_t tiles_x = (g.w + tw - 1) / tw;
   L.nseg = fast ? (tiles_x + sweep - 1) / sweep : (g.w + kGenericCols - 1) / kGenericCols;
   size_t o = 0;
   L.off_summary = o;
@@ -321,7 +325,9 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     a.sum = sum;
     for (int c = 0; c < 4; c++) a.ws[c] = prm.cmp_weights_search[c];
     a.B = prm.search_range;
-    a.tiles_x = (g.w + kCpsTileW - 1) / kCpsTileW;
+    int32_t tw, th;
+    cps_fast_tile(&tw, &th);
+    a.tiles_x = (g.w + tw - 1) / tw;
     a.sweep = P.sweep;
     a.nseg = L.nseg;
     a.partials = partials;

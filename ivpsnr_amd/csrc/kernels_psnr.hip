@@ -224,8 +224,8 @@ __global__ __launch_bounds__(256) void k_summary(Geom g, const int64_t* __restri
 // (libs/pmbb-ivqm/src/xCorrespPixelShiftSSE.cpp:40-71). The winner's offset is recovered as
 // (key - h(p) + 12) mod 32.
 //
-// Workgroup: 4 waves, tile 64 x 32 pels, each lane owns one column x 8 rows. The searched
-// picture's (tile + 2) haloed records sit in LDS (68 x 36 x 16 bytes). A workgroup sweeps
+// Workgroup: 4 waves, tile 64 x 16 pels, each lane owns one column x 4 rows. The searched
+// picture's (tile + 2) haloed records sit in LDS (72 x 20 x 16 bytes). A workgroup sweeps
 // `sweep` tiles along x; per-row per-component sums are reduced over the wave after every tile
 // and accumulated in LDS, then written as one partial per (segment, direction, component, row).
 // ---------------------------------------------------------------------------------------------
@@ -316,7 +316,8 @@ __device__ __forceinline__ void cps_fill(const Buf<T>& Yp, const Geom& g, int x0
 // per-component squared error is added to acc (per-lane row sums kept across the tile sweep).
 template <typename T, int DIR>
 __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, int x0, int y0, int lane, int wave,
-                                             const int32_t (&gcd)[3], uint4* s_rec, uint32_t (&acc)[8][3]) {
+                                             const int32_t (&gcd)[3], uint4* s_rec,
+                                             uint32_t (&acc)[kCpsTileH / 4][3]) {
   // Opaque copies: keeps per-row address arithmetic from being hoisted out of the tile sweep and
   // held in registers across it.
   asm volatile("" : "+v"(lane));
@@ -340,12 +341,13 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
   const int x = x0 + lane;
   const int xc = min(x, g.w - 1);
   const int32_t m0 = -64 * w0, m1 = -64 * w1, m2 = -64 * w2;
-  int32_t a0[8], a1[8], a2[8];
-  uint32_t x01[8];
-  int32_t x2[8];
+  constexpr int K = kCpsTileH / 4;  // rows per lane
+  int32_t a0[K], a1[K], a2[K];
+  uint32_t x01[K];
+  int32_t x2[K];
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const int y = min(y0 + 8 * wave + k, g.h - 1);
+  for (int k = 0; k < K; k++) {
+    const int y = min(y0 + K * wave + k, g.h - 1);
     const uint32_t co = (uint32_t)((y >> g.cs) * g.cw + (xc >> g.cs));
     const int32_t v0 = (int32_t)Xp.ld((uint32_t)(y * g.w + xc)) + sgn * gcd[0];
     const int32_t v1 = (int32_t)Xp.ld(cb_off + co) + sgn * gcd[1];
@@ -357,17 +359,25 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
     x2[k] = v2;
   }
 
-  int32_t best[8];
+  int32_t best[K];
 #pragma unroll
-  for (int k = 0; k < 8; k++) best[k] = INT_MAX;
+  for (int k = 0; k < K; k++) best[k] = INT_MAX;
+  // Window rows are read one ahead of use: the next row's records load while this row's keys are
+  // computed.
+  uint4 qn[5];
 #pragma unroll
-  for (int rr = 0; rr < 12; rr++) {
-    const int base = (8 * wave + rr) * LW + lane + 2;
+  for (int m = 0; m < 5; m++) qn[m] = s_rec[(K * wave) * LW + lane + 2 + m];
+#pragma unroll
+  for (int rr = 0; rr < K + 4; rr++) {
     uint4 q[5];
 #pragma unroll
-    for (int m = 0; m < 5; m++) q[m] = s_rec[base + m];
+    for (int m = 0; m < 5; m++) q[m] = qn[m];
+    if (rr + 1 < K + 4) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
+      for (int m = 0; m < 5; m++) qn[m] = s_rec[(K * wave + rr + 1) * LW + lane + 2 + m];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
       const int dy = rr - 2 - k;
       if (dy < -2 || dy > 2) continue;
       int32_t key[5];
@@ -390,8 +400,8 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
 
   // Winning candidate: per-component squared error of X against it (and the SCP pel).
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const int ly = 8 * wave + k + 2;
+  for (int k = 0; k < K; k++) {
+    const int ly = K * wave + k + 2;
     const int hp = 5 * ly + lane + 4;
     const int idx = (best[k] - hp + 12) & 31;
     const int dy = idx / 5;
@@ -400,7 +410,7 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
     const int32_t xv0 = (int32_t)(int16_t)(x01[k] & 0xffffu);
     const int32_t xv1 = (int32_t)(int16_t)(x01[k] >> 16);
     const int32_t d0 = xv0 - (int32_t)r.x, d1 = xv1 - (int32_t)r.y, d2 = x2[k] - (int32_t)r.z;
-    const int y = y0 + 8 * wave + k;
+    const int y = y0 + K * wave + k;
     if (x < g.w && y < g.h) {
       acc[k][0] += (uint32_t)(d0 * d0);
       acc[k][1] += (uint32_t)(d1 * d1);
@@ -434,9 +444,10 @@ __global__ __launch_bounds__(256) void k_cps_fast(CpsArgs a) {
   const int32_t gcd[3] = {sum->gcd[0], sum->gcd[1], sum->gcd[2]};
 
   // Per-lane row sums over the sweep (the host bounds sweep * max squared error below 2^32).
-  uint32_t acc0[8][3], acc1[8][3];
+  constexpr int K = kCpsTileH / 4;
+  uint32_t acc0[K][3], acc1[K][3];
 #pragma unroll
-  for (int k = 0; k < 8; k++)
+  for (int k = 0; k < K; k++)
 #pragma unroll
     for (int c = 0; c < 3; c++) acc0[k][c] = acc1[k][c] = 0;
 
@@ -448,8 +459,8 @@ __global__ __launch_bounds__(256) void k_cps_fast(CpsArgs a) {
 
   // One partial per (segment, direction, component, row): reduce over the wave's 64 lanes.
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const int y = y0 + 8 * wave + k;
+  for (int k = 0; k < K; k++) {
+    const int y = y0 + K * wave + k;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const uint64_t v0 = wave_sum((uint64_t)acc0[k][c]);
@@ -574,12 +585,18 @@ hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* 
 
 hipError_t launch_cps_fast(const CpsArgs& a, hipStream_t stream) {
   const int bands = (a.g.h + kCpsTileH - 1) / kCpsTileH;
+  if (a.tiles_x != (a.g.w + kCpsTileW - 1) / kCpsTileW) return hipErrorInvalidValue;
   dim3 grid(bands * a.nseg), block(256);
   if (a.g.bytes_per_sample == 1)
     hipLaunchKernelGGL(k_cps_fast<uint8_t>, grid, block, 0, stream, a);
   else
     hipLaunchKernelGGL(k_cps_fast<uint16_t>, grid, block, 0, stream, a);
   return hipGetLastError();
+}
+
+void cps_fast_tile(int32_t* tile_w, int32_t* tile_h) {
+  *tile_w = kCpsTileW;
+  *tile_h = kCpsTileH;
 }
 
 int cps_fast_blocks_per_cu(int bytes_per_sample) {

@@ -58,12 +58,17 @@ hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* 
 hipError_t launch_cps_fast(const CpsArgs& a, hipStream_t stream);
 hipError_t launch_cps_generic(const CpsArgs& a, hipStream_t stream);
 hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, uint64_t* rows, hipStream_t stream);
+// Tile shape of the fast CPS kernel (pels per workgroup tile).
+void cps_fast_tile(int32_t* tile_w, int32_t* tile_h);
 // Resident workgroups of the fast CPS kernel per CU (from the HIP occupancy calculator).
 int cps_fast_blocks_per_cu(int bytes_per_sample);
 
-// Tile shape of the fast CPS kernel.
+// Tile shape of the fast CPS kernel (device code only; the host asks cps_fast_tile()).
 constexpr int kCpsTileW = 64;   // 64 lanes x 1 column
-constexpr int kCpsTileH = 32;   // 4 waves x 8 rows
+#ifndef QMIV_CPS_TILE_H
+#define QMIV_CPS_TILE_H 16
+#endif
+constexpr int kCpsTileH = QMIV_CPS_TILE_H;  // 4 waves x (kCpsTileH / 4) rows
 constexpr int kGenericCols = 256;
 
 }  // namespace qmiv
