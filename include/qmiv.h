@@ -204,6 +204,27 @@ qmiv_status qmiv_calc_ivpsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
                              const int32_t gcd_ref2tst[4], double* ivpsnr, double* r2t, double* t2r,
                              uint64_t* rows);
 
+/* xSSIM::calcPicSSIM(Tst, Ref, CalcL) (libs/pmbb-ivqm/src/xSSIM.h:37): per component; calc_l = 0
+   gives the contrast-structure term only. */
+qmiv_status qmiv_calc_ssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, int32_t calc_l, double ssim[3]);
+
+/* xSSIM::calcPicMSSSIM(Tst, Ref) (libs/pmbb-ivqm/src/xSSIM.h:38): per component; scales (nullable)
+   receives the five per-scale scores [scale][component]. */
+qmiv_status qmiv_calc_msssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, double msssim[3],
+                             double* scales);
+
+/* xIVSSIM::calcPicIVSSIM(Tst, Ref) (libs/pmbb-ivqm/src/xSSIM.h:74): GCD, shift-compensated pictures,
+   then min(SSIM(Tst, RefSCP), SSIM(Ref, TstSCP)) of the component-weighted averages. */
+qmiv_status qmiv_calc_ivssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm,
+                             double* ivssim, double* r2t, double* t2r);
+
+/* xShftCompPicProc::GenShftCompPics(DstRef, DstTst, SrcRef, SrcTst, GlobalColorDiffRef2Tst)
+   (libs/pmbb-ivqm/src/xShftCompPic.h:36): DstRef(p) = clip(SrcRef(p + best) - gcd), best searched
+   for SrcTst(p) + gcd; DstTst(p) = clip(SrcTst(p + best) + gcd) for SrcRef(p) - gcd. dst_ref and
+   dst_tst are host buffers of 3 * width * height uint16 (4:4:4 planes Y, Cb, Cr). */
+qmiv_status qmiv_gen_scp(qmiv_ctx* ctx, const qmiv_pic* src_ref, const qmiv_pic* src_tst, const qmiv_params* prm,
+                         const int32_t gcd_ref2tst[4], uint16_t* dst_ref, uint16_t* dst_tst);
+
 #ifdef __cplusplus
 }
 #endif

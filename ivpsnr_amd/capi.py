@@ -96,6 +96,10 @@ _SIGNATURES = {
     "qmiv_calc_wspsnr": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _f64p]),
     "qmiv_calc_ivpsnr": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _i32p, _f64p, _f64p, _f64p,
                                    _u64p]),
+    "qmiv_calc_ssim": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.c_int32, _f64p]),
+    "qmiv_calc_msssim": (C.c_int, [_ctx_p, _pic_p, _pic_p, _f64p, _f64p]),
+    "qmiv_calc_ivssim": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _f64p, _f64p, _f64p]),
+    "qmiv_gen_scp": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _i32p, C.c_void_p, C.c_void_p]),
 }
 EXPORTS.extend(_SIGNATURES)
 

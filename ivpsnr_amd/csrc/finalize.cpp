@@ -5,6 +5,8 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 namespace qmiv {
 namespace fin {
@@ -85,6 +87,32 @@ int32_t gcd_max_diff(float coef, int32_t bit_depth) {
 }
 
 double min2(double a, double b) { return (b < a) ? b : a; }
+
+void ssim_window(double out[100]) {
+  double g[11];
+  for (int i = 0; i < 11; i++) g[i] = std::exp(-((double)((i - 5) * (i - 5))) / (2.0 * 1.5 * 1.5));
+  double m[121], tot = 0.0;
+  for (int i = 0; i < 11; i++)
+    for (int j = 0; j < 11; j++) m[i * 11 + j] = g[i] * g[j];
+  for (int i = 0; i < 121; i++) tot += m[i];
+  for (int dy = 0; dy < 10; dy++)
+    for (int dx = 0; dx < 10; dx++) {
+      char buf[64];
+      std::snprintf(buf, sizeof buf, "%.10f", m[dy * 11 + dx] / tot);
+      out[dy * 10 + dx] = (double)std::strtof(buf, nullptr);
+    }
+}
+
+double ssim_from_rows(const double* rows, int32_t h, int32_t w) {
+  return kbns(rows, (size_t)(h > 0 ? h : 0)) / (double)((int64_t)w * (int64_t)h);
+}
+
+double msssim_combine(const double scores[5]) {
+  static const double wts[5] = {0.0448, 0.2856, 0.3001, 0.2363, 0.1333};  // xStructSimConsts.h:27
+  double prod = 1.0;
+  for (int i = 0; i < 5; i++) prod = prod * std::pow(scores[i], wts[i]);
+  return prod;
+}
 
 }  // namespace fin
 }  // namespace qmiv

@@ -37,6 +37,19 @@ double weighted_average4(const double v[3], const int32_t wavg[4]);
 // (libs/pmbb-ivqm/src/xGlobClrDiff.cpp:21).
 int32_t gcd_max_diff(float coef, int32_t bit_depth);
 
+// The SSIM window: the normalised 11 x 11 Gaussian (sigma 1.5) with each tap rounded to ten
+// decimals and stored in single precision, as the reference's table holds it
+// (libs/pmbb-ivqm/src/xStructSimConsts.h:30-43); out[dy * 10 + dx] = F[dy][dx] for the
+// 10 x 10 part the window uses (libs/pmbb-ivqm/src/xStructSim.cpp:19-21).
+void ssim_window(double out[100]);
+
+// SSIM of one picture from its per-row sums: KBNS over the rows / (w * h)
+// (libs/pmbb-ivqm/src/xSSIM.cpp:104-111).
+double ssim_from_rows(const double* rows, int32_t h, int32_t w);
+
+// MS-SSIM: product of score_i ^ weight_i in scale order with std::pow (xSSIM.cpp:83-84).
+double msssim_combine(const double scores[5]);
+
 // std::min argument order (xCommonDefCORE.h:93).
 double min2(double a, double b);
 

@@ -147,10 +147,27 @@ qmiv_status ensure_scp(qmiv_ctx* ctx, size_t bytes) {
 //   device: [results block | row_sd | row_max | partials]
 //   results block (copied back in one transfer): FrameSummary | cps rows [2][3][h] | row_ssd [nrows]
 // ---------------------------------------------------------------------------------------------
+// Internal flag beside the QMIV_* bits: the SSIM evaluation without the luminance term.
+constexpr uint32_t kSsimNoLuminance = 1u << 30;
+
+// One SSIM evaluation (one component of one picture pair at one pyramid level).
+struct SsimEval {
+  int kind;       // 0 SSIM (calc_l as asked), 1 MS-SSIM level, 2 IV-SSIM T2R, 3 IV-SSIM R2T, 4 SSIM without L
+  int cmp;        // component
+  int level;      // pyramid level (MS-SSIM)
+  int32_t w, h;   // size at this level
+  size_t off_rows;  // per-row sums in the results block
+};
+
 struct Layout {
   size_t off_summary, off_cps_rows, off_row_ssd, results_bytes;
   size_t off_row_sd, off_row_max, off_partials, total;
   int32_t nseg;
+  std::vector<SsimEval> ssim;
+  size_t off_ssim_part = 0;   // device: [tiles_x][h] partials (reused by every evaluation)
+  size_t off_pyr = 0;         // device: MS-SSIM pyramid levels 1..4, [picture][component][level]
+  size_t pyr_level_off[5] = {0, 0, 0, 0, 0};  // element offsets of the levels within one plane chain
+  size_t pyr_chain = 0;       // elements of one (picture, component) chain
 };
 
 // Fast CPS path eligibility: every operand of the packed key must stay inside the ranges the
@@ -199,7 +216,7 @@ int32_t pick_sweep(qmiv_ctx* ctx, const Geom& g, const qmiv_params& p, const int
   return (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles_x, sweep));
 }
 
-Layout make_layout(const Geom& g, bool fast, int32_t sweep) {
+Layout make_layout(const Geom& g, bool fast, int32_t sweep, uint32_t ssim_metrics) {
   Layout L{};
   const int32_t nrows = g.h + 2 * g.ch;
   int32_t tw, th;
@@ -213,6 +230,23 @@ Layout make_layout(const Geom& g, bool fast, int32_t sweep) {
   o = align_up(o + sizeof(uint64_t) * 6 * (size_t)g.h, 256);
   L.off_row_ssd = o;
   o = align_up(o + sizeof(uint64_t) * (size_t)nrows, 256);
+  // SSIM family: per-row sums of every evaluation, in the results block.
+  auto add_eval = [&](int kind, int cmp, int level) {
+    SsimEval e{kind, cmp, level, g.w >> level, g.h >> level, 0};
+    e.off_rows = o;
+    o = align_up(o + sizeof(double) * (size_t)std::max(e.h, 1), 256);
+    L.ssim.push_back(e);
+  };
+  if (ssim_metrics & QMIV_SSIM)
+    for (int c = 0; c < 3; c++) add_eval((ssim_metrics & kSsimNoLuminance) ? 4 : 0, c, 0);
+  if (ssim_metrics & QMIV_MSSSIM)
+    for (int c = 0; c < 3; c++)
+      for (int l = 0; l < 5; l++) add_eval(1, c, l);
+  if (ssim_metrics & QMIV_IVSSIM)
+    for (int c = 0; c < 3; c++) {
+      add_eval(2, c, 0);
+      add_eval(3, c, 0);
+    }
   L.results_bytes = o;
   L.off_row_sd = o;
   o = align_up(o + sizeof(int64_t) * (size_t)nrows, 256);
@@ -220,6 +254,22 @@ Layout make_layout(const Geom& g, bool fast, int32_t sweep) {
   o = align_up(o + sizeof(uint32_t) * 2 * (size_t)nrows, 256);
   L.off_partials = o;
   o = align_up(o + sizeof(uint64_t) * 6 * (size_t)g.h * (size_t)L.nseg, 256);
+  if (!L.ssim.empty()) {
+    size_t part = 0;
+    for (const SsimEval& e : L.ssim) part = std::max(part, (size_t)ssim_tiles_x(e.w) * (size_t)std::max(e.h, 1));
+    L.off_ssim_part = o;
+    o = align_up(o + sizeof(double) * std::max<size_t>(part, 1), 256);
+  }
+  if (ssim_metrics & QMIV_MSSSIM) {
+    size_t chain = 0;
+    for (int l = 1; l < 5; l++) {
+      L.pyr_level_off[l] = chain;
+      chain += (size_t)(g.w >> l) * (size_t)(g.h >> l);
+    }
+    L.pyr_chain = align_up(chain, 128);
+    L.off_pyr = o;
+    o = align_up(o + sizeof(uint16_t) * L.pyr_chain * 6, 256);
+  }
   L.total = o;
   return L;
 }
@@ -266,6 +316,7 @@ struct FrameWork {
   const int32_t* gcd = nullptr;  // or use this one
   bool need_cps = false;
   bool need_scp = false;
+  uint32_t ssim_metrics = 0;     // QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM
 };
 
 // Everything decided on the host before a frame's kernels are enqueued.
@@ -282,8 +333,95 @@ FramePlan plan_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, const
   const int32_t* gbound = fw.gcd ? fw.gcd : P.md;
   P.fast = fast_cps_ok(g, prm, gbound);
   P.sweep = pick_sweep(ctx, g, prm, gbound);
-  P.L = make_layout(g, P.fast, P.sweep);
+  P.L = make_layout(g, P.fast, P.sweep, fw.ssim_metrics);
   return P;
+}
+
+PlaneView pic_view(const qmiv_pic* p, int c) {
+  const Geom& g = p->g;
+  PlaneView v;
+  const int64_t off = c == 0 ? 0 : (c == 1 ? g.cb_off : g.cr_off);
+  v.base = (const char*)p->dev + off * g.bytes_per_sample;
+  v.stride = c == 0 ? g.w : g.cw;
+  v.shift = c == 0 ? 0 : g.cs;
+  v.bytes = g.bytes_per_sample;
+  v.nbytes = (uint32_t)((c == 0 ? (size_t)g.w * g.h : (size_t)g.cw * g.ch) * g.bytes_per_sample);
+  return v;
+}
+
+PlaneView u16_view(const uint16_t* base, int32_t w, int32_t h) {
+  PlaneView v;
+  v.base = base;
+  v.stride = w;
+  v.shift = 0;
+  v.bytes = 2;
+  v.nbytes = (uint32_t)((size_t)w * h * 2);
+  return v;
+}
+
+// Enqueues the SSIM-family evaluations of one frame (after the CPS kernel has written the SCP
+// pictures when IV-SSIM is asked): pyramid levels, then one windowed-moment pass per evaluation.
+qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const FramePlan& P, char* dbase) {
+  const Geom& g = p0->g;
+  const Layout& L = P.L;
+  hipStream_t st = ctx->stream;
+  uint16_t* pyr = L.pyr_chain ? (uint16_t*)(dbase + L.off_pyr) : nullptr;
+  auto level_ptr = [&](int pic, int c, int l) { return pyr + ((size_t)pic * 3 + c) * L.pyr_chain + L.pyr_level_off[l]; };
+  bool need_pyr = false;
+  for (const SsimEval& e : L.ssim) need_pyr |= e.kind == 1;
+  if (need_pyr) {
+    KernelTimer t(ctx, QMIV_K_PYRAMID);
+    for (int pic = 0; pic < 2; pic++)
+      for (int c = 0; c < 3; c++)
+        for (int l = 1; l < 5; l++) {
+          const PlaneView src = l == 1 ? pic_view(pic ? p1 : p0, c)
+                                       : u16_view(level_ptr(pic, c, l - 1), g.w >> (l - 1), g.h >> (l - 1));
+          QMIV_HIP(launch_downsample(src, g.w >> l, g.h >> l, level_ptr(pic, c, l), st));
+        }
+  }
+  SsimArgs a{};
+  fin::ssim_window(a.f);
+  const double maxv = (double)((1 << g.bd) - 1);
+  const double k1 = 0.01 * maxv, k2 = 0.03 * maxv;
+  a.c1 = k1 * k1;
+  a.c2 = k2 * k2;
+  a.partials = (double*)(dbase + L.off_ssim_part);
+  const uint16_t* scp0 = (const uint16_t*)ctx->d_scp;
+  const uint16_t* scp1 = scp0 ? scp0 + 3 * (size_t)g.w * g.h : nullptr;
+  for (const SsimEval& e : L.ssim) {
+    a.w = e.w;
+    a.h = e.h;
+    a.tiles_x = ssim_tiles_x(e.w);
+    a.calc_l = ((e.kind == 1 && (e.level == 0 || e.level == 4)) || e.kind == 4) ? 0 : 1;  // xSSIM.cpp:74,80
+    const size_t plane = (size_t)g.w * g.h;
+    switch (e.kind) {
+      case 0:  // calcPicSSIM(P0, P1)
+      case 4:
+        a.tst = pic_view(p0, e.cmp);
+        a.ref = pic_view(p1, e.cmp);
+        break;
+      case 1:
+        if (e.level == 0) {
+          a.tst = pic_view(p0, e.cmp);
+          a.ref = pic_view(p1, e.cmp);
+        } else {
+          a.tst = u16_view(level_ptr(0, e.cmp, e.level), e.w, e.h);
+          a.ref = u16_view(level_ptr(1, e.cmp, e.level), e.w, e.h);
+        }
+        break;
+      case 2:  // T2R = calcPicSSIM(P0, SCP1)
+        a.tst = pic_view(p0, e.cmp);
+        a.ref = u16_view(scp1 + e.cmp * plane, g.w, g.h);
+        break;
+      default:  // R2T = calcPicSSIM(P1, SCP0)
+        a.tst = pic_view(p1, e.cmp);
+        a.ref = u16_view(scp0 + e.cmp * plane, g.w, g.h);
+        break;
+    }
+    KernelTimer t(ctx, QMIV_K_SSIM);
+    QMIV_HIP(launch_ssim(a, (double*)(dbase + e.off_rows), st));
+  }
+  return QMIV_OK;
 }
 
 // Enqueues one frame's statistics (+ GCD) and, when asked, its CPS search on the context stream,
@@ -344,6 +482,7 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
       QMIV_HIP(launch_cps_rows(partials, L.nseg, g.h, cps_rows, st));
     }
   }
+  if (!L.ssim.empty()) QMIV_TRY(enqueue_ssim(ctx, p0, p1, P, dbase));
   QMIV_HIP(hipMemcpyAsync(hbase, dbase, L.results_bytes, hipMemcpyDeviceToHost, st));
   return QMIV_OK;
 }
@@ -444,6 +583,30 @@ qmiv_status finalize_frame(const Geom& g, const qmiv_params& prm, uint32_t metri
     out->ivpsnr_r2t = fin::qual_from_rows(rows.data(), g.h, wsp, area, g.bd, prm.cmp_weights_average);
     out->ivpsnr_t2r = fin::qual_from_rows(rows.data() + 3 * (size_t)g.h, g.h, wsp, area, g.bd, prm.cmp_weights_average);
     out->ivpsnr = fin::min2(out->ivpsnr_r2t, out->ivpsnr_t2r);
+  }
+  if (!L.ssim.empty()) {
+    double ms[3][5] = {{0}}, t2r[3] = {0, 0, 0}, r2t[3] = {0, 0, 0};
+    for (const SsimEval& e : L.ssim) {
+      const double v = fin::ssim_from_rows((const double*)(hb + e.off_rows), e.h, e.w);
+      if (e.kind == 0 || e.kind == 4) out->ssim[e.cmp] = v;
+      else if (e.kind == 1) ms[e.cmp][e.level] = v;
+      else if (e.kind == 2) t2r[e.cmp] = v;
+      else r2t[e.cmp] = v;
+    }
+    if (metrics & QMIV_SSIM) out->ssim_pic = fin::pic_value(out->ssim, prm.cmp_weights_average);
+    if (metrics & QMIV_MSSSIM) {
+      for (int c = 0; c < 3; c++) {
+        out->msssim[c] = fin::msssim_combine(ms[c]);
+        for (int l = 0; l < 5; l++) out->msssim_scales[l][c] = ms[c][l];
+      }
+      out->msssim_pic = fin::pic_value(out->msssim, prm.cmp_weights_average);
+    }
+    if (metrics & QMIV_IVSSIM) {
+      // xIVSSIM::calcPicIVSSIM (libs/pmbb-ivqm/src/xSSIM.cpp:192-212)
+      out->ivssim_t2r = fin::weighted_average4(t2r, prm.cmp_weights_average);
+      out->ivssim_r2t = fin::weighted_average4(r2t, prm.cmp_weights_average);
+      out->ivssim = fin::min2(out->ivssim_t2r, out->ivssim_r2t);
+    }
   }
   return QMIV_OK;
 }
@@ -606,13 +769,12 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
   qmiv_params prm;
   if (prm_in) prm = *prm_in;
   else qmiv_params_default(&prm);
-  if (metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM))
-    return fail(QMIV_ERR_UNSUPPORTED, "SSIM-family metrics are not available in this build");
   QMIV_HIP(hipSetDevice(ctx->device));
   FrameWork fw;
   fw.need_gcd = (metrics & (QMIV_IVPSNR | QMIV_IVSSIM)) != 0;
   fw.need_cps = (metrics & QMIV_IVPSNR) != 0;
   fw.need_scp = (metrics & QMIV_IVSSIM) != 0;
+  fw.ssim_metrics = metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM | kSsimNoLuminance);
   // Plan every frame, then lay their scratch and result blocks end to end.
   std::vector<FramePlan> plans((size_t)n);
   std::vector<size_t> doff((size_t)n), hoff((size_t)n);
@@ -739,6 +901,61 @@ qmiv_status qmiv_calc_ivpsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
   if (t2r) *t2r = q1;
   if (ivpsnr) *ivpsnr = fin::min2(q0, q1);
   if (rows_out) std::memcpy(rows_out, rows.data(), rows.size() * sizeof(uint64_t));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_ssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, int32_t calc_l, double ssim[3]) {
+  if (!ssim) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  qmiv_frame_result r;
+  const uint32_t m = QMIV_SSIM | (calc_l ? 0u : kSsimNoLuminance);
+  QMIV_TRY(qmiv_frame(ctx, tst, ref, nullptr, m, &r));
+  for (int c = 0; c < 3; c++) ssim[c] = r.ssim[c];
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_msssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, double msssim[3],
+                             double* scales) {
+  if (!msssim) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  qmiv_frame_result r;
+  QMIV_TRY(qmiv_frame(ctx, tst, ref, nullptr, QMIV_MSSSIM, &r));
+  for (int c = 0; c < 3; c++) msssim[c] = r.msssim[c];
+  if (scales) std::memcpy(scales, r.msssim_scales, sizeof(r.msssim_scales));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_ivssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm,
+                             double* ivssim, double* r2t, double* t2r) {
+  qmiv_frame_result r;
+  QMIV_TRY(qmiv_frame(ctx, tst, ref, prm, QMIV_IVSSIM, &r));
+  if (ivssim) *ivssim = r.ivssim;
+  if (r2t) *r2t = r.ivssim_r2t;
+  if (t2r) *t2r = r.ivssim_t2r;
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_gen_scp(qmiv_ctx* ctx, const qmiv_pic* src_ref, const qmiv_pic* src_tst, const qmiv_params* prm_in,
+                         const int32_t gcd_ref2tst[4], uint16_t* dst_ref, uint16_t* dst_tst) {
+  if (!ctx || !gcd_ref2tst || !dst_ref || !dst_tst) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(src_tst, src_ref));
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  FrameWork fw;
+  fw.gcd = gcd_ref2tst;
+  fw.need_scp = true;
+  Layout L;
+  bool fast;
+  // DstRef is rebuilt from SrcTst + gcd searched in SrcRef: the kernel's direction 0 with
+  // P0 = SrcTst, P1 = SrcRef, i.e. its SCP of P1; DstTst is its SCP of P0.
+  QMIV_TRY(run_frame_kernels(ctx, src_tst, src_ref, prm, fw, L, fast));
+  const FrameSummary* sum = (const FrameSummary*)((const char*)ctx->h_results + L.off_summary);
+  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
+  const size_t plane3 = 3 * (size_t)src_tst->g.w * src_tst->g.h;
+  const uint16_t* scp0 = (const uint16_t*)ctx->d_scp;
+  QMIV_HIP(hipMemcpyAsync(dst_tst, scp0, plane3 * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream));
+  QMIV_HIP(hipMemcpyAsync(dst_ref, scp0 + plane3, plane3 * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
   return QMIV_OK;
 }
 

@@ -50,6 +50,26 @@ struct CpsArgs {
   int32_t maxv;
 };
 
+// One plane read through its 4:4:4 view: value(x, y) = base[(y >> shift) * stride + (x >> shift)].
+struct PlaneView {
+  const void* base;
+  int32_t stride;   // samples per row
+  int32_t shift;    // 1 for 4:2:0 chroma of a packed frame, else 0
+  int32_t bytes;    // bytes per sample (1 or 2)
+  uint32_t nbytes;  // readable bytes from base
+};
+
+// Arguments of the SSIM kernel (the 10 x 10 window travels with the launch).
+struct SsimArgs {
+  PlaneView tst, ref;
+  int32_t w, h;          // 4:4:4 size of this (pyramid level's) picture
+  int32_t calc_l;        // 1: luminance x contrast-structure; 0: contrast-structure only
+  int32_t tiles_x;       // ssim_tiles_x(w)
+  double c1, c2;
+  double* partials;      // [tiles_x][h]
+  double f[100];         // window weights F[dy + 5][dx + 5], dy, dx in [-5, 4]
+};
+
 // Kernel launchers (kernels.hip). All launch on `stream` and return hipSuccess or the launch error.
 hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* row_sd, uint64_t* row_ssd,
                         uint32_t* row_max, hipStream_t stream);
@@ -62,6 +82,10 @@ hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, ui
 void cps_fast_tile(int32_t* tile_w, int32_t* tile_h);
 // Resident workgroups of the fast CPS kernel per CU (from the HIP occupancy calculator).
 int cps_fast_blocks_per_cu(int bytes_per_sample);
+
+hipError_t launch_ssim(const SsimArgs& a, double* rows, hipStream_t stream);
+int32_t ssim_tiles_x(int32_t w);
+hipError_t launch_downsample(const PlaneView& src, int32_t dw, int32_t dh, uint16_t* dst, hipStream_t stream);
 
 // Tile shape of the fast CPS kernel (device code only; the host asks cps_fast_tile()).
 constexpr int kCpsTileW = 64;   // 64 lanes x 1 column

@@ -151,6 +151,34 @@ class Engine:
             out["rows"] = rows.reshape(2, 3, tst.height)
         return out
 
+    def calc_ssim(self, tst: Picture, ref: Picture, calc_l: bool = True) -> np.ndarray:
+        v = (C.c_double * 3)()
+        check(lib().qmiv_calc_ssim(self.ctx, tst.handle, ref.handle, int(calc_l), v), "qmiv_calc_ssim")
+        return np.array(list(v))
+
+    def calc_msssim(self, tst: Picture, ref: Picture):
+        v = (C.c_double * 3)()
+        sc = (C.c_double * 15)()
+        check(lib().qmiv_calc_msssim(self.ctx, tst.handle, ref.handle, v, sc), "qmiv_calc_msssim")
+        return np.array(list(v)), np.array(list(sc)).reshape(5, 3)
+
+    def calc_ivssim(self, tst: Picture, ref: Picture, params: Params | None = None) -> dict:
+        iv, r2t, t2r = C.c_double(), C.c_double(), C.c_double()
+        check(lib().qmiv_calc_ivssim(self.ctx, tst.handle, ref.handle, C.byref(params) if params else None,
+                                     C.byref(iv), C.byref(r2t), C.byref(t2r)), "qmiv_calc_ivssim")
+        return {"ivssim": iv.value, "r2t": r2t.value, "t2r": t2r.value}
+
+    def gen_scp(self, src_ref: Picture, src_tst: Picture, gcd, params: Params | None = None):
+        """(DstRef, DstTst) as [3, h, w] uint16 arrays (xShftCompPicProc::GenShftCompPics)."""
+        n = 3 * src_ref.width * src_ref.height
+        dref = np.zeros(n, np.uint16)
+        dtst = np.zeros(n, np.uint16)
+        g = (C.c_int32 * 4)(*[int(x) for x in gcd])
+        check(lib().qmiv_gen_scp(self.ctx, src_ref.handle, src_tst.handle, C.byref(params) if params else None, g,
+                                 dref.ctypes.data, dtst.ctypes.data), "qmiv_gen_scp")
+        shape = (3, src_ref.height, src_ref.width)
+        return dref.reshape(shape), dtst.reshape(shape)
+
     def close(self):
         for ref in self._pics:
             p = ref()

@@ -1,0 +1,133 @@
+"""SSIM, MS-SSIM, IV-SSIM and the shift-compensated pictures on the engine.
+
+SSIM-family values are fp64 moments with fused multiply-adds and a tree-ordered row reduction,
+so they match the reference's (sequential, unfused) doubles to rounding: the tolerance here is
+1e-9 relative (the required bar is 1e-6). The shift-compensated pictures are integer: bit-exact.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ivpsnr_amd import IVPSNR, IVSSIM, MSSSIM, PSNR, SSIM, WSPSNR, default_params
+from ivpsnr_amd.synth import make_pair, sha256
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REL = 1e-9
+SSIM_ALL = SSIM | MSSSIM | IVSSIM
+ALL = PSNR | WSPSNR | IVPSNR | SSIM_ALL
+
+
+def load(name):
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        return json.load(f)
+
+
+def close(a, b, rel=REL):
+    """Relative closeness; NaN matches NaN (MS-SSIM of pictures too small for five scales)."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    nan = np.isnan(b)
+    return bool(np.array_equal(np.isnan(a), nan) and
+                np.all(np.abs(a[~nan] - b[~nan]) <= rel * np.maximum(np.abs(b[~nan]), 1e-300)))
+
+
+def _pics(engine, pair):
+    return (engine.picture(pair.width, pair.height, pair.bit_depth, pair.chroma_format, data=pair.ref),
+            engine.picture(pair.width, pair.height, pair.bit_depth, pair.chroma_format, data=pair.tst))
+
+
+@pytest.mark.parametrize("idx", range(len(load("small_cases"))))
+def test_small_cases_all_metrics(engine, idx):
+    case = load("small_cases")[idx]
+    w, h, bd, cf = case["width"], case["height"], case["bit_depth"], case["chroma_format"]
+    pair = make_pair(w, h, bd, case["seed"], chroma_format=cf, **case["knobs"])
+    assert sha256(pair.ref) == case["sha256_ref"]
+    p0, p1 = _pics(engine, pair)
+    r = engine.frame(p0, p1, ALL, default_params())
+    want = case["plain"]
+    # PSNR family stays bit-exact when computed together with the SSIM family
+    assert list(r.gcd) == want["gcd"] and r.ivpsnr == want["ivpsnr"] and list(r.psnr) == want["psnr"]
+    assert close(list(r.ssim), want["ssim"]), (list(r.ssim), want["ssim"])
+    assert close(list(r.msssim), want["msssim"]), (list(r.msssim), want["msssim"])
+    assert close(np.array(r.msssim_scales), want["msssim_scales"])
+    assert close([r.ivssim, r.ivssim_r2t, r.ivssim_t2r], [want["ivssim"], want["ivssim_r2t"], want["ivssim_t2r"]])
+    assert close([r.ssim_pic, r.msssim_pic], [want["ssim_pic"], want["msssim_pic"]])
+    # shift-compensated pictures, bit-exact
+    dst_ref, dst_tst = engine.gen_scp(p1, p0, want["gcd"])
+    for c in range(3):
+        assert hashlib.sha256(dst_ref[c].astype("<u2").tobytes()).hexdigest() == want["scp1_sha256"][c]
+        assert hashlib.sha256(dst_tst[c].astype("<u2").tobytes()).hexdigest() == want["scp0_sha256"][c]
+
+
+def test_ssim_family_1080p10(engine):
+    fx = load("ssim_1080p10")
+    pair = make_pair(1920, 1080, 10, seed=fx["seed"])
+    assert sha256(pair.ref) == fx["sha256_ref"]
+    p0, p1 = _pics(engine, pair)
+    r = engine.frame(p0, p1, SSIM_ALL, default_params())
+    want = fx["frame"]
+    assert close(list(r.ssim), want["ssim"]) and close(list(r.msssim), want["msssim"])
+    assert close(np.array(r.msssim_scales), want["msssim_scales"])
+    assert close([r.ivssim, r.ivssim_r2t, r.ivssim_t2r], [want["ivssim"], want["ivssim_r2t"], want["ivssim_t2r"]])
+
+
+@pytest.mark.parametrize("w,h,bd,cf,seed", [(67, 45, 8, 444, 3), (96, 80, 10, 420, 8), (12, 12, 8, 420, 2),
+                                            (10, 30, 8, 444, 4), (300, 170, 12, 420, 5)])
+def test_against_oracle(engine, oracle, w, h, bd, cf, seed):
+    pair = make_pair(w, h, bd, seed, chroma_format=cf)
+    p0, p1 = _pics(engine, pair)
+    r = engine.frame(p0, p1, SSIM_ALL, default_params())
+    o = oracle.frame(pair.ref, pair.tst, oracle.params(w, h, bd, cf, metrics=oracle.SSIM | oracle.MSSSIM | oracle.IVSSIM))
+    assert close(list(r.ssim), list(o.ssim)) and close(list(r.msssim), list(o.msssim))
+    assert close([r.ivssim, r.ivssim_r2t, r.ivssim_t2r], [o.ivssim, o.ivssim_r2t, o.ivssim_t2r])
+
+
+def test_processor_calls(engine, oracle):
+    w, h, bd = 160, 96, 10
+    pair = make_pair(w, h, bd, 12)
+    p0, p1 = _pics(engine, pair)
+    P0 = oracle.unpack(pair.ref, w, h, bd)
+    P1 = oracle.unpack(pair.tst, w, h, bd)
+    for calc_l in (True, False):
+        got = engine.calc_ssim(p0, p1, calc_l)
+        want = [oracle.ssim_plane(P0[c], P1[c], bd, calc_l) for c in range(3)]
+        assert close(got, want), calc_l
+    ms, scales = engine.calc_msssim(p0, p1)
+    for c in range(3):
+        v, sc = oracle.msssim_plane(P0[c], P1[c], bd)
+        assert close(ms[c], v) and close(scales[:, c], sc)
+    iv = engine.calc_ivssim(p0, p1)
+    o = oracle.frame(pair.ref, pair.tst, oracle.params(w, h, bd, metrics=oracle.IVSSIM))
+    assert close([iv["ivssim"], iv["r2t"], iv["t2r"]], [o.ivssim, o.ivssim_r2t, o.ivssim_t2r])
+
+
+def test_identical_pictures_closed_form(engine):
+    w, h = 1920, 1080
+    pair = make_pair(w, h, 10, 5)
+    p0 = engine.picture(w, h, 10, data=pair.ref)
+    p1 = engine.picture(w, h, 10, data=pair.ref)
+    r = engine.frame(p0, p1, SSIM | MSSSIM, default_params())
+    frac = (w - 10) * (h - 10) / (w * h)
+    assert np.allclose(list(r.ssim), [frac] * 3, rtol=1e-13, atol=0)
+    wts = [0.0448, 0.2856, 0.3001, 0.2363, 0.1333]
+    ms = 1.0
+    for i in range(5):
+        wi, hi = w >> i, h >> i
+        ms *= ((wi - 10) * (hi - 10) / (wi * hi)) ** wts[i]
+    assert np.allclose(list(r.msssim), [ms] * 3, rtol=1e-13, atol=0)
+
+
+def test_scp_against_oracle_with_generic_path(engine, oracle):
+    """SCP from the generic CPS kernel (search range 3) against the oracle's xGenShftCompRow."""
+    w, h, bd = 90, 60, 10
+    pair = make_pair(w, h, bd, 17)
+    p0, p1 = _pics(engine, pair)
+    prm = default_params(search_range=3)
+    g = engine.calc_gcd(p0, p1, prm)
+    dst_ref, dst_tst = engine.gen_scp(p1, p0, g, prm)
+    P0, P1 = oracle.unpack(pair.ref, w, h, bd), oracle.unpack(pair.tst, w, h, bd)
+    np.testing.assert_array_equal(dst_ref, oracle.scp(P1, P0, bd, g, B=3))
+    np.testing.assert_array_equal(dst_tst, oracle.scp(P0, P1, bd, -g, B=3))
