@@ -1,14 +1,17 @@
-"""Throughput of the IV-PSNR engine on MI355X.
+"""Throughput of the QMIV metric engine on MI355X.
 
-Workload (one step): IV-PSNR + WS-PSNR + PSNR of 8 frame pairs, 4096x2048, yuv420p10le,
-equirectangular (WS weights on), search range 2, GCD on - the configuration the headline
-metric is quoted on. Frames are synthetic (ivpsnr_amd.synth) and resident in HBM before the
-timed region starts.
+Default workload (one step): IV-PSNR + WS-PSNR + PSNR of 8 frame pairs, 4096x2048, yuv420p10le,
+equirectangular (WS weights on), search range 2, GCD on. Frames are synthetic (ivpsnr_amd.synth)
+and resident in HBM before the timed region starts.
+
+`--workload ssim-1080p10` times SSIM + MS-SSIM + IV-SSIM of 16 frame pairs 1920x1080 yuv420p10le
+instead (its dominant kernel is the fp64 windowed-moment kernel).
 
 Prints one JSON line:
   value      whole-job Mpix/s over the timed steps
-  roofline   the CPS kernel's algorithmic bytes per launch (compulsory 4:2:0 input of one frame
-             pair) over its average launch duration, measured with HIP events on the engine stream
+  roofline   the dominant kernel: algorithmic bytes (or flops) per launch over its average launch
+             duration, measured with HIP events on the engine stream; `traffic` is the HBM bytes per
+             launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that kernel
   cpu_baseline  the reference CPU path (oracle/_ref/qmiv_ref_v3, the reference's own sources
              built for x86-64-v3) on a bounded sample of the same workload
 """
@@ -25,17 +28,56 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "IV-PSNR Mpix/s @4096x2048 10b 420 + %HBM roofline; batched vs plain"
 HBM_PEAK_GBPS = 8000.0
-W, H, BD, FRAMES = 4096, 2048, 10, 8
+FP64_PEAK_TFLOPS = 78.6
+
+WORKLOADS = {
+    "ivpsnr-erp4k10": dict(
+        metric="IV-PSNR Mpix/s @4096x2048 10b 420 + %HBM roofline; batched vs plain",
+        w=4096, h=2048, bd=10, frames=8, erp=True, ml="PSNR,WSPSNR,IVPSNR", kernel="cps",
+        desc="IV-PSNR+WS-PSNR+PSNR, 8 frame pairs 4096x2048 yuv420p10le ERP, B=2, GCD on",
+        cpu_stages=("validate", "preproc", "gcd", "psnr", "wspsnr", "ivpsnr"), dtype="int32",
+        traffic_file="k_cps_fast_4096x2048_10b_erp.txt"),
+    "ssim-1080p10": dict(
+        metric="SSIM+MS-SSIM+IV-SSIM Mpix/s @1920x1080 10b 420",
+        w=1920, h=1080, bd=10, frames=16, erp=False, ml="SSIM,MSSSIM,IVSSIM", kernel="ssim",
+        desc="SSIM+MS-SSIM+IV-SSIM, 16 frame pairs 1920x1080 yuv420p10le, B=2, GCD on",
+        cpu_stages=("validate", "preproc", "gcd", "scp", "ssim", "msssim", "ivssim"), dtype="f64",
+        traffic_file="k_ssim_1920x1080_10b.txt"),
+}
 
 
 def frame_pair_bytes(w, h, bd):
     return 2 * w * h * 3 // 2 * (1 if bd <= 8 else 2)
 
 
-def cpu_baseline(pairs, sample_frames=4):
+def ssim_flops_per_frame(w, h):
+    """Algorithmic fp64 flops of the windowed moments of one frame pair (SSIM + MS-SSIM + IV-SSIM):
+    100 taps x 5 moments x 2 flops per interior pel, per component (chroma evaluated on the 4:4:4
+    grid), over SSIM (1 evaluation), the 5 MS-SSIM scales and IV-SSIM (2 evaluations)."""
+    def interior(a, b):
+        return max(a - 10, 0) * max(b - 10, 0)
+    per_cmp = 3 * interior(w, h) + sum(interior(w >> i, h >> i) for i in range(5))
+    return 3 * per_cmp * 1000, 3 * (1 + 5 + 2)
+
+
+def read_traffic(name):
+    """HBM bytes per launch of the dominant kernel from its committed rocprofv3 PMC summary
+    (line `traffic_bytes_per_launch: N`), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(wl, pairs, sample_frames):
     """Times the reference's own CPU path on `sample_frames` frames of the workload."""
+    W, H, BD = wl["w"], wl["h"], wl["bd"]
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "qmiv_ref_v3")
     cores = min(16, os.cpu_count() or 1)
     if os.path.exists(ref_bin):
@@ -46,26 +88,43 @@ def cpu_baseline(pairs, sample_frames=4):
                     a.write(pairs[i % len(pairs)].ref.tobytes())
                     b.write(pairs[i % len(pairs)].tst.tobytes())
             cmd = [ref_bin, "-w", str(W), "-h", str(H), "-bd", str(BD), "-cf", "420", "-i0", f0, "-i1", f1,
-                   "-erp", "360", "180", "-ml", "PSNR,WSPSNR,IVPSNR", "-nth", str(cores), "-time"]
+                   "-ml", wl["ml"], "-nth", str(cores), "-time"]
+            if wl["erp"]:
+                cmd[-5:-5] = ["-erp", "360", "180"]
             try:
                 r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
                 if r.returncode == 0:
                     t = json.loads(r.stdout)["time_ms_per_frame"]
-                    ms = sum(t[k] for k in ("validate", "preproc", "gcd", "psnr", "wspsnr", "ivpsnr"))
+                    ms = sum(t[k] for k in wl["cpu_stages"])
                     return {"value": round(W * H / (ms * 1e3), 3), "unit": "Mpix/s", "cores": cores,
                             "kind": "reference",
-                            "sample": f"{sample_frames} frames 4096x2048 10-bit ERP, IV-PSNR+WS-PSNR+PSNR incl. "
-                                      f"validation, preprocessing and GCD, {cores} threads; {ms:.1f} ms/frame"}
+                            "sample": f"{sample_frames} frames {W}x{H} {BD}-bit of the same workload ({wl['ml']}) "
+                                      f"incl. validation, preprocessing and GCD, {cores} threads; {ms:.1f} ms/frame"}
             except (subprocess.TimeoutExpired, OSError, ValueError, KeyError):
                 pass
     # The reference build is missing or does not run on this host: time the C restatement.
     from oracle import pyoracle
-    p = pyoracle.params(W, H, BD, 420, metrics=pyoracle.PSNR | pyoracle.WSPSNR | pyoracle.IVPSNR, erp=True)
+    m = 0
+    for name in wl["ml"].split(","):
+        m |= getattr(pyoracle, name)
+    p = pyoracle.params(W, H, BD, 420, metrics=m, erp=wl["erp"])
     t0 = time.perf_counter()
     pyoracle.frame(pairs[0].ref, pairs[0].tst, p)
     ms = (time.perf_counter() - t0) * 1e3
     return {"value": round(W * H / (ms * 1e3), 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": f"1 frame 4096x2048 10-bit ERP, IV-PSNR+WS-PSNR+PSNR, single thread; {ms:.1f} ms/frame"}
+            "sample": f"1 frame {W}x{H} {BD}-bit ({wl['ml']}), single thread; {ms:.1f} ms/frame"}
+
+
+def copy_probe():
+    """A plain device copy of 512 MiB, three times: calibrates the rocprof byte counters."""
+    import torch
+    n = 512 << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n, "launches": 3}))
 
 
 def main():
@@ -73,34 +132,34 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ivpsnr-erp4k10")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--frames", type=int, default=FRAMES, help="frame pairs per step (profiling runs only)")
+    ap.add_argument("--frames", type=int, default=0, help="frame pairs per step (default: the workload's)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 512 MiB (calibrates rocprof byte counters)")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
-        n = 512 << 20
-        src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": n, "launches": 3}))
+        copy_probe()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's workload runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+              "(multi-GPU sharding is a design note, not run here).", file=sys.stderr)
         sys.exit(2)
 
-    from ivpsnr_amd import IVPSNR, PSNR, WSPSNR, Engine, default_params
+    from ivpsnr_amd import Engine, default_params
+    import ivpsnr_amd
     from ivpsnr_amd.synth import make_pair
 
-    print(f"[bench] generating {args.frames} synthetic frame pairs {W}x{H} {BD}-bit", file=sys.stderr, flush=True)
-    nframes = args.frames
+    wl = WORKLOADS[args.workload]
+    W, H, BD = wl["w"], wl["h"], wl["bd"]
+    nframes = args.frames or wl["frames"]
+    print(f"[bench] {args.workload}: generating {nframes} synthetic frame pairs {W}x{H} {BD}-bit", file=sys.stderr,
+          flush=True)
     pairs = [make_pair(W, H, BD, seed=1 + f) for f in range(nframes)]
-    metrics = PSNR | WSPSNR | IVPSNR
-    prm = default_params(erp=True)
+    metrics = 0
+    for name in wl["ml"].split(","):
+        metrics |= getattr(ivpsnr_amd, name)
+    prm = default_params(erp=wl["erp"])
 
     eng = Engine(0)
     pics = [(eng.picture(W, H, BD, 420, data=p.ref), eng.picture(W, H, BD, 420, data=p.tst)) for p in pairs]
@@ -133,7 +192,8 @@ def main():
     el_plain, _, res_plain = timed(step_plain, args.steps, args.warmup, False)
     print(f"[bench] stream: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
     elapsed, prof, res = timed(step_stream, args.steps, args.warmup, True)
-    assert [r.ivpsnr for r in res] == [r.ivpsnr for r in res_plain]
+    key = "ivpsnr" if "IVPSNR" in wl["ml"] else "ivssim"
+    assert [getattr(r, key) for r in res] == [getattr(r, key) for r in res_plain]
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s (plain loop {el_plain:.3f} s)", file=sys.stderr,
           flush=True)
     # Kernel events serialise nothing but add a few microseconds per launch: time once more without them.
@@ -143,14 +203,25 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     mpix = nframes * W * H / 1e6
     value = mpix / (ms_per_step / 1e3)
-    cps_ms, cps_n = prof.get("cps", (0.0, 0))
-    cps_avg_ms = cps_ms / max(cps_n, 1)
-    bytes_per_launch = frame_pair_bytes(W, H, BD)
-    achieved = bytes_per_launch / (cps_avg_ms / 1e3) / 1e9 if cps_avg_ms > 0 else 0.0
+    k_ms, k_n = prof.get(wl["kernel"], (0.0, 0))
+    k_avg_ms = k_ms / max(k_n, 1)
+    if wl["kernel"] == "cps":
+        per_launch = frame_pair_bytes(W, H, BD)
+        achieved = per_launch / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic(wl["traffic_file"]),
+                "kernel": "k_cps_fast", "kernel_avg_us": round(k_avg_ms * 1e3, 2), "bytes_per_launch": per_launch}
+    else:
+        flops, launches = ssim_flops_per_frame(W, H)
+        per_launch = flops / launches
+        achieved = per_launch / (k_avg_ms / 1e3) / 1e12 if k_avg_ms > 0 else 0.0
+        roof = {"bound": "fp64-valu", "achieved": round(achieved, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": read_traffic(wl["traffic_file"]),
+                "kernel": "k_ssim", "kernel_avg_us": round(k_avg_ms * 1e3, 2), "flops_per_launch": per_launch}
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(pairs)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(wl, pairs, 4 if wl["kernel"] == "cps" else 2)
     out = {
-        "metric": METRIC,
+        "metric": wl["metric"],
         "value": round(value, 2),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -160,18 +231,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": wl["dtype"],
         "data": "synthetic",
-        "config": {"workload": "IV-PSNR+WS-PSNR+PSNR, 8 frame pairs 4096x2048 yuv420p10le ERP, B=2, GCD on",
-                   "frames_per_step": nframes, "mode": "stream (qmiv_frames: frame set enqueued, one wait)",
+        "config": {"workload": wl["desc"], "frames_per_step": nframes,
+                   "mode": "stream (qmiv_frames: frame set enqueued, one wait)",
                    "plain_loop_mpix_s": round(mpix / (el_plain / args.steps), 2),
                    "stream_vs_plain": round(el_plain / elapsed, 3),
-                   "ivpsnr_last_frame": round(res[-1].ivpsnr, 6)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "kernel": "k_cps_fast", "kernel_avg_us": round(cps_avg_ms * 1e3, 2),
-                     "bytes_per_launch": bytes_per_launch},
-        "kernels_us_per_frame": {k: round(v[0] / max(v[1], 1) * 1e3, 2) for k, v in prof.items()},
+                   key + "_last_frame": round(getattr(res[-1], key), 6)},
+        "roofline": roof,
+        "kernels_us_per_frame": {k: round(v[0] / nframes / args.steps * 1e3, 2) for k, v in prof.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

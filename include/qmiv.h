@@ -5,7 +5,7 @@
   colour difference, both directions), PSNR, WS-PSNR, SSIM, MS-SSIM and IV-SSIM
   for pairs of pictures resident in device memory, with results identical to
   the reference CPU implementation (integer sums and the PSNR family bit-exact;
-  SSIM family within 1e-12 relative).
+  SSIM family within 1e-9 relative).
 
   The reference exposes this functionality as C++ processor classes with no
   FFI of its own; each entry point below names the reference interface it
@@ -149,6 +149,7 @@ enum {
   QMIV_K_CPS_ROWS = 3,  /* per-row reduction of the CPS partial sums */
   QMIV_K_SSIM = 4,      /* SSIM windowed moments */
   QMIV_K_PYRAMID = 5,   /* MS-SSIM 2x2 pyramid */
+  QMIV_K_SSIM_ROWS = 6, /* per-row reduction of the SSIM tile partial sums */
   QMIV_K_COUNT = 8
 };
 

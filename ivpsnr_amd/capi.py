@@ -55,9 +55,9 @@ class FrameResult(C.Structure):
         return out
 
 
-K_STATS, K_SUMMARY, K_CPS, K_CPS_ROWS, K_SSIM, K_PYRAMID, K_COUNT = 0, 1, 2, 3, 4, 5, 8
+K_STATS, K_SUMMARY, K_CPS, K_CPS_ROWS, K_SSIM, K_PYRAMID, K_SSIM_ROWS, K_COUNT = 0, 1, 2, 3, 4, 5, 6, 8
 KERNEL_NAMES = {K_STATS: "stats", K_SUMMARY: "summary", K_CPS: "cps", K_CPS_ROWS: "cps_rows", K_SSIM: "ssim",
-                K_PYRAMID: "pyramid"}
+                K_PYRAMID: "pyramid", K_SSIM_ROWS: "ssim_rows"}
 
 
 class Profile(C.Structure):

@@ -418,8 +418,12 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
         a.ref = u16_view(scp0 + e.cmp * plane, g.w, g.h);
         break;
     }
-    KernelTimer t(ctx, QMIV_K_SSIM);
-    QMIV_HIP(launch_ssim(a, (double*)(dbase + e.off_rows), st));
+    {
+      KernelTimer t(ctx, QMIV_K_SSIM);
+      QMIV_HIP(launch_ssim(a, st));
+    }
+    KernelTimer t(ctx, QMIV_K_SSIM_ROWS);
+    QMIV_HIP(launch_ssim_rows(a, (double*)(dbase + e.off_rows), st));
   }
   return QMIV_OK;
 }

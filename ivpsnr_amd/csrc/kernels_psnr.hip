@@ -224,8 +224,9 @@ __global__ __launch_bounds__(256) void k_summary(Geom g, const int64_t* __restri
 // (libs/pmbb-ivqm/src/xCorrespPixelShiftSSE.cpp:40-71). The winner's offset is recovered as
 // (key - h(p) + 12) mod 32.
 //
-// Workgroup: 4 waves, tile 64 x 16 pels, each lane owns one column x 4 rows. The searched
-// picture's (tile + 2) haloed records sit in LDS (72 x 20 x 16 bytes). A workgroup sweeps
+// Workgroup: 4 waves, tile 64 x kCpsTileH pels (16 by default), each lane owns one column x
+// kCpsTileH / 4 rows. The searched picture's haloed records sit in LDS (72 x (kCpsTileH + 4)
+// records of 16 bytes), filled one tile row ahead of the search. A workgroup sweeps
 // `sweep` tiles along x; per-row per-component sums are reduced over the wave after every tile
 // and accumulated in LDS, then written as one partial per (segment, direction, component, row).
 // ---------------------------------------------------------------------------------------------
@@ -237,7 +238,7 @@ __device__ __forceinline__ int32_t mad24(int32_t a, int32_t b, int32_t c) {
   return d;
 }
 
-// LDS tile of the fast kernel: rows [y0 - 2, y0 + 34), columns [x0 - 4, x0 + 68). The two extra
+// LDS tile of the fast kernel: rows [y0 - 2, y0 + kCpsTileH + 2), columns [x0 - 4, x0 + 68). The two extra
 // columns on each side keep every row start 8-byte aligned for the vectorised fill; the search
 // reads columns [x0 - 2, x0 + 66).
 constexpr int kCpsLW = kCpsTileW + 8;
@@ -312,7 +313,7 @@ __device__ __forceinline__ void cps_fill(const Buf<T>& Yp, const Geom& g, int x0
   }
 }
 
-// One direction over one tile: each of the lane's 8 pels finds its best candidate; the winner's
+// One direction over one tile: each of the lane's kCpsTileH / 4 pels finds its best candidate; the winner's
 // per-component squared error is added to acc (per-lane row sums kept across the tile sweep).
 template <typename T, int DIR>
 __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, int x0, int y0, int lane, int wave,

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_downsample(PlaneView src, int32_t dw, i
 
 }  // namespace
 
-hipError_t launch_ssim(const SsimArgs& a, double* rows, hipStream_t stream) {
+hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream) {
   const int tiles_x = a.w > 10 ? (a.w - 10 + kSsimTW - 1) / kSsimTW : 0;
   const int tiles_y = a.h > 10 ? (a.h - 10 + kSsimTH - 1) / kSsimTH : 0;
   if (tiles_x != a.tiles_x) return hipErrorInvalidValue;
@@ -165,10 +165,14 @@ hipError_t launch_ssim(const SsimArgs& a, double* rows, hipStream_t stream) {
       hipLaunchKernelGGL(k_ssim<1>, dim3(tiles_x, tiles_y), dim3(256), 0, stream, a);
     else
       hipLaunchKernelGGL(k_ssim<0>, dim3(tiles_x, tiles_y), dim3(256), 0, stream, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_ssim_rows, dim3((a.h + 255) / 256), dim3(256), 0, stream, a.partials, tiles_x, a.h, rows);
+  return hipSuccess;
+}
+
+hipError_t launch_ssim_rows(const SsimArgs& a, double* rows, hipStream_t stream) {
+  if (a.h <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ssim_rows, dim3((a.h + 255) / 256), dim3(256), 0, stream, a.partials, a.tiles_x, a.h, rows);
   return hipGetLastError();
 }
 

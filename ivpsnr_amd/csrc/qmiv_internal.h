@@ -83,7 +83,10 @@ void cps_fast_tile(int32_t* tile_w, int32_t* tile_h);
 // Resident workgroups of the fast CPS kernel per CU (from the HIP occupancy calculator).
 int cps_fast_blocks_per_cu(int bytes_per_sample);
 
-hipError_t launch_ssim(const SsimArgs& a, double* rows, hipStream_t stream);
+// Windowed moments of one evaluation into a.partials ([tiles_x][h] row sums per tile column).
+hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream);
+// rows[y] = sum over tile columns of a.partials, in tile order (zero outside the interior).
+hipError_t launch_ssim_rows(const SsimArgs& a, double* rows, hipStream_t stream);
 int32_t ssim_tiles_x(int32_t w);
 hipError_t launch_downsample(const PlaneView& src, int32_t dw, int32_t dh, uint16_t* dst, hipStream_t stream);
 
