@@ -226,6 +226,26 @@ qmiv_status qmiv_calc_ivssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
 qmiv_status qmiv_gen_scp(qmiv_ctx* ctx, const qmiv_pic* src_ref, const qmiv_pic* src_tst, const qmiv_params* prm,
                          const int32_t gcd_ref2tst[4], uint16_t* dst_ref, uint16_t* dst_tst);
 
+/* GenShftCompPics into device pictures (libs/pmbb-ivqm/src/xShftCompPic.h:36, the form the
+   application calls, apps/QMIV/src/xAppQMIV.cpp:703): dst_ref / dst_tst are 4:4:4 pictures of the
+   sources' size and bit depth (the reference's xPicP layout). */
+qmiv_status qmiv_gen_scp_pics(qmiv_ctx* ctx, const qmiv_pic* src_ref, const qmiv_pic* src_tst, const qmiv_params* prm,
+                              const int32_t gcd_ref2tst[4], qmiv_pic* dst_ref, qmiv_pic* dst_tst);
+
+/* xIVSSIM::calcPicIVSSIM(Tst, Ref, TstSCP, RefSCP) (libs/pmbb-ivqm/src/xSSIM.cpp:192-212): the
+   shift-compensated pictures come from the caller; T2R = SSIM(Tst, RefSCP), R2T = SSIM(Ref, TstSCP),
+   each averaged with prm->cmp_weights_average, then the minimum. All four pictures share one
+   geometry. */
+qmiv_status qmiv_calc_ivssim_scp(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* tst_scp,
+                                 const qmiv_pic* ref_scp, const qmiv_params* prm, double* ivssim, double* r2t,
+                                 double* t2r);
+
+/* Nearest-neighbour 4:2:0 -> 4:4:4 conversion on the device (a copy for 4:4:4 sources) into a
+   4:4:4 picture of the same size and bit depth: the unpack step of xSeqBase::readFrame
+   (libs/pmbb-core/src/xSeq.cpp:197-213, libs/pmbb-core/src/xPixelOpsSTD.cpp:30-49,68-87).
+   Asynchronous on the context stream. */
+qmiv_status qmiv_pic_upsample(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,293 @@
+/*
+  qmiv.hpp - C++ processor classes over the C ABI in qmiv.h, with the method names, argument
+  order and semantics of the reference's metric processors, so that a frame loop written
+  against libs/pmbb-ivqm can switch to the MI355X engine by changing types, not calls.
+
+  Header-only; link with -lqmiv. Failures throw qmiv::xError (the reference asserts instead).
+
+    reference (namespace PMBB)                       here (namespace qmiv)
+    xPicP                (libs/pmbb-core/src/xPic.h)   xPicP: planar 4:4:4 picture in device memory
+    xGlobClrDiffProc     (xGlobClrDiff.h:38-43)        xGlobClrDiffProc
+    xShftCompPicProc     (xShftCompPic.h:33-44)        xShftCompPicProc
+    xPSNR / xWSPSNR / xIVPSNR (xPSNR.h, xWSPSNR.h, xIVPSNR.h)   xIVPSNR (PSNR, WS-PSNR, IV-PSNR)
+    xSSIM / xIVSSIM      (xSSIM.h:25-79)               xIVSSIM (SSIM, MS-SSIM, IV-SSIM)
+*/
+#ifndef QMIV_HPP
+#define QMIV_HPP
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
+
+#include "qmiv.h"
+
+namespace qmiv {
+
+using int32 = int32_t;
+using flt32 = float;
+using flt64 = double;
+using int32V2 = std::array<int32, 2>;
+using int32V4 = std::array<int32, 4>;
+using flt32V4 = std::array<flt32, 4>;
+using flt64V4 = std::array<flt64, 4>;
+
+class xError : public std::runtime_error {
+ public:
+  xError(qmiv_status Status, const std::string& Msg) : std::runtime_error(Msg), m_Status(Status) {}
+  qmiv_status status() const { return m_Status; }
+
+ private:
+  qmiv_status m_Status;
+};
+
+inline void xCheck(qmiv_status Status, const char* Where) {
+  if (Status != QMIV_OK) throw xError(Status, std::string(Where) + ": " + qmiv_last_error());
+}
+
+// One HIP device and stream. Takes the place of the reference's thread pool (xThreadPool):
+// every picture and processor below runs its work on one context.
+class xContext {
+ public:
+  explicit xContext(int32 Device = 0) { xCheck(qmiv_create(Device, &m_Ctx), "qmiv_create"); }
+  ~xContext() { qmiv_destroy(m_Ctx); }
+  xContext(const xContext&) = delete;
+  xContext& operator=(const xContext&) = delete;
+
+  qmiv_ctx* get() const { return m_Ctx; }
+  void waitUntilTasksFinished() { xCheck(qmiv_sync(m_Ctx), "qmiv_sync"); }
+
+ private:
+  qmiv_ctx* m_Ctx = nullptr;
+};
+
+// Planar 4:4:4 picture in device memory (the reference's xPicP without margins: edges are
+// clamped by the kernels). Samples are 8-bit at bit depth 8, else 16-bit.
+class xPicP {
+ public:
+  xPicP(xContext& Ctx, int32V2 Size, int32 BitDepth) : m_Ctx(&Ctx) { create(Size, BitDepth); }
+  ~xPicP() { destroy(); }
+  xPicP(const xPicP&) = delete;
+  xPicP& operator=(const xPicP&) = delete;
+
+  void create(int32V2 Size, int32 BitDepth) {
+    destroy();
+    m_Size = Size;
+    m_BitDepth = BitDepth;
+    xCheck(qmiv_pic_create(m_Ctx->get(), Size[0], Size[1], BitDepth, 444, &m_Pic), "qmiv_pic_create");
+  }
+  void destroy() {
+    qmiv_pic_destroy(m_Pic);
+    qmiv_pic_destroy(m_Staging);
+    m_Pic = m_Staging = nullptr;
+  }
+
+  int32V2 getSize() const { return m_Size; }
+  int32 getWidth() const { return m_Size[0]; }
+  int32 getHeight() const { return m_Size[1]; }
+  int32 getArea() const { return m_Size[0] * m_Size[1]; }
+  int32 getBitDepth() const { return m_BitDepth; }
+  int32 getMaxPelValue() const { return (1 << m_BitDepth) - 1; }
+  bool isCompatible(const xPicP* Ref) const { return Ref && Ref->m_Size == m_Size && Ref->m_BitDepth == m_BitDepth; }
+  qmiv_pic* handle() const { return m_Pic; }
+
+  // xSeqBase::readFrame's unpack (libs/pmbb-core/src/xSeq.cpp:179-249): one packed planar frame
+  // as read from a raw file (ChromaFormat 420 or 444) becomes this 4:4:4 picture; 4:2:0 chroma is
+  // upsampled nearest-neighbour on the device.
+  void fromPacked(const void* Packed, int32 ChromaFormat) {
+    if (ChromaFormat == 444) {
+      xCheck(qmiv_pic_upload(m_Ctx->get(), m_Pic, Packed, qmiv_frame_bytes(getWidth(), getHeight(), m_BitDepth, 444)),
+             "qmiv_pic_upload");
+      return;
+    }
+    if (!m_Staging)
+      xCheck(qmiv_pic_create(m_Ctx->get(), getWidth(), getHeight(), m_BitDepth, ChromaFormat, &m_Staging),
+             "qmiv_pic_create");
+    xCheck(qmiv_pic_upload(m_Ctx->get(), m_Staging, Packed,
+                           qmiv_frame_bytes(getWidth(), getHeight(), m_BitDepth, ChromaFormat)),
+           "qmiv_pic_upload");
+    xCheck(qmiv_pic_upsample(m_Ctx->get(), m_Staging, m_Pic), "qmiv_pic_upsample");
+  }
+  // Planar 4:4:4 samples (Y, Cb, Cr; W x H each; 1 or 2 bytes per sample) in and out.
+  void upload(const void* Planes) {
+    xCheck(qmiv_pic_upload(m_Ctx->get(), m_Pic, Planes, qmiv_frame_bytes(getWidth(), getHeight(), m_BitDepth, 444)),
+           "qmiv_pic_upload");
+  }
+  void download(void* Planes) const {
+    xCheck(qmiv_pic_download(m_Ctx->get(), m_Pic, Planes, qmiv_frame_bytes(getWidth(), getHeight(), m_BitDepth, 444)),
+           "qmiv_pic_download");
+  }
+
+ private:
+  xContext* m_Ctx;
+  qmiv_pic* m_Pic = nullptr;
+  qmiv_pic* m_Staging = nullptr;  // packed 4:2:0 upload buffer of fromPacked
+  int32V2 m_Size = {0, 0};
+  int32 m_BitDepth = 0;
+};
+
+// Parameters shared by the processors: xGlobClrDiffPrms (xGlobClrDiff.h:17-26),
+// xCorrespPixelShiftPrms (xCorrespPixelShift.h:26-47) and xWeightedSpherically::initWS
+// (xWeightedSpherically.h:21).
+class xProcPrms {
+ public:
+  explicit xProcPrms(xContext& Ctx) : m_Ctx(&Ctx) { qmiv_params_default(&m_Prm); }
+  void setUnntcbCoef(const flt32V4& UnntcbCoef) {
+    for (int c = 0; c < 4; c++) m_Prm.unnoticeable_coef[c] = UnntcbCoef[c];
+  }
+  void setSearchRange(int32 SearchRange) { m_Prm.search_range = SearchRange; }
+  void setCmpWeightsSearch(const int32V4& CmpWeightsSearch) {
+    for (int c = 0; c < 4; c++) m_Prm.cmp_weights_search[c] = CmpWeightsSearch[c];
+  }
+  void setCmpWeightsAverage(const int32V4& CmpWeightsAverage) {
+    for (int c = 0; c < 4; c++) m_Prm.cmp_weights_average[c] = CmpWeightsAverage[c];
+  }
+  // Width, height and bit depth come with each picture here; the arguments are accepted for
+  // signature compatibility.
+  void initWS(bool UseWS, int32 /*Width*/, int32 /*Height*/, int32 /*BitDepth*/, int32 LonRangeDeg = 360,
+              int32 LatRangeDeg = 180) {
+    m_Prm.erp = UseWS ? 1 : 0;
+    m_Prm.lon_range_deg = LonRangeDeg;
+    m_Prm.lat_range_deg = LatRangeDeg;
+  }
+  // Row buffers live in the engine's device scratch; kept for call compatibility.
+  void initRowBuffers(int32 /*Height*/) {}
+  const qmiv_params& params() const { return m_Prm; }
+
+ protected:
+  xContext* m_Ctx;
+  qmiv_params m_Prm;
+};
+
+// xGlobClrDiffProc (libs/pmbb-ivqm/src/xGlobClrDiff.h:38-43).
+class xGlobClrDiffProc : public xProcPrms {
+ public:
+  using xProcPrms::xProcPrms;
+  // rint(sum(Ref - Tst) / Area), clipped to the unnoticeable range (the R2T colour difference).
+  int32V4 CalcGlobalColorDiff(const xPicP* Tst, const xPicP* Ref) {
+    int32V4 G{};
+    xCheck(qmiv_calc_gcd(m_Ctx->get(), Tst->handle(), Ref->handle(), &m_Prm, G.data()), "qmiv_calc_gcd");
+    return G;
+  }
+};
+
+// xShftCompPicProc (libs/pmbb-ivqm/src/xShftCompPic.h:33-44).
+class xShftCompPicProc : public xProcPrms {
+ public:
+  using xProcPrms::xProcPrms;
+  void GenShftCompPics(xPicP* DstRef, xPicP* DstTst, const xPicP* SrcRef, const xPicP* SrcTst,
+                       const int32V4& GlobalColorDiffRef2Tst) {
+    xCheck(qmiv_gen_scp_pics(m_Ctx->get(), SrcRef->handle(), SrcTst->handle(), &m_Prm, GlobalColorDiffRef2Tst.data(),
+                             DstRef->handle(), DstTst->handle()),
+           "qmiv_gen_scp_pics");
+  }
+};
+
+// xIVPSNR with its bases xWSPSNR and xPSNR (libs/pmbb-ivqm/src/xIVPSNR.h:15-43, xWSPSNR.h,
+// xPSNR.h:15-50).
+class xIVPSNR : public xProcPrms {
+ public:
+  using tDCfGCS = std::function<void(const int32V4&)>;
+  using tDCfQAP = std::function<void(flt64, flt64)>;
+  using xProcPrms::xProcPrms;
+
+  void setDebugCallbackGCS(tDCfGCS Callback) { m_DebugCallbackGCS = std::move(Callback); }
+  void setDebugCallbackQAP(tDCfQAP Callback) { m_DebugCallbackQAP = std::move(Callback); }
+  // xPSNR::setFakeValsForExact: PSNR(SSD = 1) instead of flt64_max for identical components.
+  void setFakeValsForExact(bool FakeValsForExact) { m_FakeValsForExact = FakeValsForExact; }
+
+  flt64V4 calcPicPSNR(const xPicP* Tst, const xPicP* Ref) { return xPerCmp(Tst, Ref, QMIV_PSNR); }
+  // Plain PSNR unless initWS enabled the spherical weights (xWSPSNR.cpp:17-27).
+  flt64V4 calcPicWSPSNR(const xPicP* Tst, const xPicP* Ref) { return xPerCmp(Tst, Ref, QMIV_WSPSNR); }
+
+  // xIVPSNR::calcPicIVPSNR(Tst, Ref) (xIVPSNR.cpp:18-31): the colour difference is
+  // -CalcGlobalColorDiff(Tst, Ref) of the static form, i.e. minus the rounded sum(Tst - Ref) / Area.
+  flt64 calcPicIVPSNR(const xPicP* Tst, const xPicP* Ref) {
+    int32V4 G{};
+    xCheck(qmiv_calc_gcd(m_Ctx->get(), Ref->handle(), Tst->handle(), &m_Prm, G.data()), "qmiv_calc_gcd");
+    for (int32& g : G) g = -g;
+    if (m_DebugCallbackGCS) m_DebugCallbackGCS(G);
+    return calcPicIVPSNR(Tst, Ref, G);
+  }
+  // xIVPSNR::calcPicIVPSNR(Tst, Ref, GlobalColorDiffRef2Tst) (xIVPSNR.cpp:33-45).
+  flt64 calcPicIVPSNR(const xPicP* Tst, const xPicP* Ref, const int32V4& GlobalColorDiffRef2Tst) {
+    flt64 IV = 0, R2T = 0, T2R = 0;
+    xCheck(qmiv_calc_ivpsnr(m_Ctx->get(), Tst->handle(), Ref->handle(), &m_Prm, GlobalColorDiffRef2Tst.data(), &IV,
+                            &R2T, &T2R, nullptr),
+           "qmiv_calc_ivpsnr");
+    if (m_DebugCallbackQAP) m_DebugCallbackQAP(R2T, T2R);
+    return IV;
+  }
+
+ private:
+  flt64V4 xPerCmp(const xPicP* Tst, const xPicP* Ref, uint32_t Metric) {
+    qmiv_params P = m_Prm;
+    P.fake_psnr_for_exact = m_FakeValsForExact ? 1 : 0;
+    qmiv_frame_result R;
+    xCheck(qmiv_frame(m_Ctx->get(), Tst->handle(), Ref->handle(), &P, Metric, &R), "qmiv_frame");
+    const double* V = Metric == QMIV_PSNR ? R.psnr : R.wspsnr;
+    return {V[0], V[1], V[2], 0.0};
+  }
+
+  tDCfGCS m_DebugCallbackGCS;
+  tDCfQAP m_DebugCallbackQAP;
+  bool m_FakeValsForExact = false;
+};
+
+// xIVSSIM with its base xSSIM (libs/pmbb-ivqm/src/xSSIM.h:25-79).
+class xIVSSIM : public xProcPrms {
+ public:
+  using tDCfGCS = std::function<void(const int32V4&)>;
+  using tDCfQAP = std::function<void(flt64, flt64)>;
+  using xProcPrms::xProcPrms;
+
+  // Pyramid and shift-compensated buffers live in the engine's device scratch; kept for call
+  // compatibility (xSSIM::create, xIVSSIM::create).
+  void create(int32V2 /*Size*/, int32 /*BitDepth*/, int32 /*Margin*/, bool /*EnableMS*/) {}
+  void destroy() {}
+  void setDebugCallbackGCS(tDCfGCS Callback) { m_DebugCallbackGCS = std::move(Callback); }
+  void setDebugCallbackQAP(tDCfQAP Callback) { m_DebugCallbackQAP = std::move(Callback); }
+
+  flt64V4 calcPicSSIM(const xPicP* Tst, const xPicP* Ref, bool CalcL = true) {
+    flt64V4 V{};
+    xCheck(qmiv_calc_ssim(m_Ctx->get(), Tst->handle(), Ref->handle(), CalcL ? 1 : 0, V.data()), "qmiv_calc_ssim");
+    V[3] = 0.0;
+    return V;
+  }
+  flt64V4 calcPicMSSSIM(const xPicP* Tst, const xPicP* Ref) {
+    flt64V4 V{};
+    xCheck(qmiv_calc_msssim(m_Ctx->get(), Tst->handle(), Ref->handle(), V.data(), nullptr), "qmiv_calc_msssim");
+    V[3] = 0.0;
+    return V;
+  }
+  // xIVSSIM::calcPicIVSSIM(Tst, Ref) (xSSIM.cpp:166-190): GCD, shift-compensated pictures, SSIMs.
+  flt64 calcPicIVSSIM(const xPicP* Tst, const xPicP* Ref) {
+    flt64 IV = 0, R2T = 0, T2R = 0;
+    if (m_DebugCallbackGCS) {
+      int32V4 G{};
+      xCheck(qmiv_calc_gcd(m_Ctx->get(), Tst->handle(), Ref->handle(), &m_Prm, G.data()), "qmiv_calc_gcd");
+      m_DebugCallbackGCS(G);
+    }
+    xCheck(qmiv_calc_ivssim(m_Ctx->get(), Tst->handle(), Ref->handle(), &m_Prm, &IV, &R2T, &T2R), "qmiv_calc_ivssim");
+    if (m_DebugCallbackQAP) m_DebugCallbackQAP(R2T, T2R);
+    return IV;
+  }
+  // xIVSSIM::calcPicIVSSIM(Tst, Ref, TstSCP, RefSCP) (xSSIM.cpp:191-212), the application's form.
+  flt64 calcPicIVSSIM(const xPicP* Tst, const xPicP* Ref, const xPicP* TstSCP, const xPicP* RefSCP) {
+    flt64 IV = 0, R2T = 0, T2R = 0;
+    xCheck(qmiv_calc_ivssim_scp(m_Ctx->get(), Tst->handle(), Ref->handle(), TstSCP->handle(), RefSCP->handle(), &m_Prm,
+                                &IV, &R2T, &T2R),
+           "qmiv_calc_ivssim_scp");
+    if (m_DebugCallbackQAP) m_DebugCallbackQAP(R2T, T2R);
+    return IV;
+  }
+
+ private:
+  tDCfGCS m_DebugCallbackGCS;
+  tDCfQAP m_DebugCallbackQAP;
+};
+
+}  // namespace qmiv
+
+#endif  // QMIV_HPP

@@ -7,4 +7,6 @@ generator used by the tests and the benchmark.
 from .capi import IVPSNR, IVSSIM, MSSSIM, PSNR, SSIM, WSPSNR  # noqa: F401
 from .engine import Engine, Picture, default_params  # noqa: F401
 
-__all__ = ["Engine", "Picture", "default_params", "PSNR", "WSPSNR", "IVPSNR", "SSIM", "MSSSIM", "IVSSIM"]
+ALL_METRICS = PSNR | WSPSNR | IVPSNR | SSIM | MSSSIM | IVSSIM
+
+__all__ = ["Engine", "Picture", "default_params", "ALL_METRICS", "PSNR", "WSPSNR", "IVPSNR", "SSIM", "MSSSIM", "IVSSIM"]

@@ -100,6 +100,10 @@ _SIGNATURES = {
     "qmiv_calc_msssim": (C.c_int, [_ctx_p, _pic_p, _pic_p, _f64p, _f64p]),
     "qmiv_calc_ivssim": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _f64p, _f64p, _f64p]),
     "qmiv_gen_scp": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _i32p, C.c_void_p, C.c_void_p]),
+    "qmiv_gen_scp_pics": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _i32p, _pic_p, _pic_p]),
+    "qmiv_calc_ivssim_scp": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), _f64p, _f64p,
+                                       _f64p]),
+    "qmiv_pic_upsample": (C.c_int, [_ctx_p, _pic_p, _pic_p]),
 }
 EXPORTS.extend(_SIGNATURES)
 

@@ -78,6 +78,7 @@ qmiv_status make_geom(int32_t w, int32_t h, int32_t bd, int32_t cf, Geom& g) {
   if (cf != 420 && cf != 444) return fail(QMIV_ERR_UNSUPPORTED, "chroma format must be 420 or 444");
   if (cf == 420 && ((w & 1) || (h & 1)))
     return fail(QMIV_ERR_UNSUPPORTED, "4:2:0 pictures need even width and height");
+  if (h > 65535) return fail(QMIV_ERR_UNSUPPORTED, "height above 65535");
   g = Geom{};
   g.w = w;
   g.h = h;
@@ -960,6 +961,72 @@ qmiv_status qmiv_gen_scp(qmiv_ctx* ctx, const qmiv_pic* src_ref, const qmiv_pic*
   QMIV_HIP(hipMemcpyAsync(dst_tst, scp0, plane3 * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream));
   QMIV_HIP(hipMemcpyAsync(dst_ref, scp0 + plane3, plane3 * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream));
   QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_gen_scp_pics(qmiv_ctx* ctx, const qmiv_pic* src_ref, const qmiv_pic* src_tst, const qmiv_params* prm_in,
+                              const int32_t gcd_ref2tst[4], qmiv_pic* dst_ref, qmiv_pic* dst_tst) {
+  if (!ctx || !gcd_ref2tst || !dst_ref || !dst_tst) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(src_tst, src_ref));
+  const Geom& gs = src_tst->g;
+  for (const qmiv_pic* d : {(const qmiv_pic*)dst_ref, (const qmiv_pic*)dst_tst})
+    if (d->g.w != gs.w || d->g.h != gs.h || d->g.bd != gs.bd || d->g.cs != 0)
+      return fail(QMIV_ERR_INVALID_ARG, "destination pictures must be 4:4:4 of the sources' size and bit depth");
+  if (dst_ref == dst_tst) return fail(QMIV_ERR_INVALID_ARG, "destination pictures must differ");
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  FrameWork fw;
+  fw.gcd = gcd_ref2tst;
+  fw.need_scp = true;
+  Layout L;
+  bool fast;
+  QMIV_TRY(run_frame_kernels(ctx, src_tst, src_ref, prm, fw, L, fast));
+  const FrameSummary* sum = (const FrameSummary*)((const char*)ctx->h_results + L.off_summary);
+  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
+  const int64_t plane3 = 3 * (int64_t)gs.w * gs.h;
+  const uint16_t* scp0 = (const uint16_t*)ctx->d_scp;
+  // The frame kernels' SCP of P0 (= SrcTst) is DstTst; their SCP of P1 (= SrcRef) is DstRef.
+  if (gs.bytes_per_sample == 2) {
+    QMIV_HIP(hipMemcpyAsync(dst_tst->dev, scp0, plane3 * 2, hipMemcpyDeviceToDevice, ctx->stream));
+    QMIV_HIP(hipMemcpyAsync(dst_ref->dev, scp0 + plane3, plane3 * 2, hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    QMIV_HIP(launch_narrow(scp0, plane3, (uint8_t*)dst_tst->dev, ctx->stream));
+    QMIV_HIP(launch_narrow(scp0 + plane3, plane3, (uint8_t*)dst_ref->dev, ctx->stream));
+  }
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_ivssim_scp(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* tst_scp,
+                                 const qmiv_pic* ref_scp, const qmiv_params* prm_in, double* ivssim, double* r2t,
+                                 double* t2r) {
+  if (!ctx) return fail(QMIV_ERR_INVALID_ARG, "null context");
+  QMIV_TRY(check_pair(tst, ref));
+  QMIV_TRY(check_pair(tst, ref_scp));
+  QMIV_TRY(check_pair(ref, tst_scp));
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  qmiv_frame_result a, b;
+  QMIV_TRY(qmiv_frame(ctx, tst, ref_scp, &prm, QMIV_SSIM, &a));  // SSIMs_T2R = calcPicSSIM(Tst, RefSCP)
+  QMIV_TRY(qmiv_frame(ctx, ref, tst_scp, &prm, QMIV_SSIM, &b));  // SSIMs_R2T = calcPicSSIM(Ref, TstSCP)
+  const double v_t2r = fin::weighted_average4(a.ssim, prm.cmp_weights_average);
+  const double v_r2t = fin::weighted_average4(b.ssim, prm.cmp_weights_average);
+  if (ivssim) *ivssim = fin::min2(v_t2r, v_r2t);
+  if (r2t) *r2t = v_r2t;
+  if (t2r) *t2r = v_t2r;
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_upsample(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst) {
+  if (!ctx || !src || !dst) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (dst->g.w != src->g.w || dst->g.h != src->g.h || dst->g.bd != src->g.bd || dst->g.cs != 0)
+    return fail(QMIV_ERR_INVALID_ARG, "destination must be a 4:4:4 picture of the source's size and bit depth");
+  if (dst == src || dst->dev == src->dev) return fail(QMIV_ERR_INVALID_ARG, "destination must differ from the source");
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_HIP(launch_upsample(src->dev, src->g, dst->dev, ctx->stream));
   return QMIV_OK;
 }
 

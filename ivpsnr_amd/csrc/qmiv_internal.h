@@ -83,6 +83,12 @@ void cps_fast_tile(int32_t* tile_w, int32_t* tile_h);
 // Resident workgroups of the fast CPS kernel per CU (from the HIP occupancy calculator).
 int cps_fast_blocks_per_cu(int bytes_per_sample);
 
+// Nearest-neighbour 4:2:0 -> 4:4:4 upsampling (a copy for 4:4:4 sources) of a packed frame into a
+// planar 4:4:4 frame of the same sample type.
+hipError_t launch_upsample(const void* src, const Geom& gs, void* dst, hipStream_t stream);
+// dst[i] = (uint8_t)src[i] for i < n.
+hipError_t launch_narrow(const uint16_t* src, int64_t n, uint8_t* dst, hipStream_t stream);
+
 // Windowed moments of one evaluation into a.partials ([tiles_x][h] row sums per tile column).
 hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream);
 // rows[y] = sum over tile columns of a.partials, in tile order (zero outside the interior).

@@ -179,6 +179,29 @@ class Engine:
         shape = (3, src_ref.height, src_ref.width)
         return dref.reshape(shape), dtst.reshape(shape)
 
+    def gen_scp_pics(self, src_ref: Picture, src_tst: Picture, gcd, dst_ref: Picture, dst_tst: Picture,
+                     params: Params | None = None):
+        """GenShftCompPics into two 4:4:4 device pictures (DstRef, DstTst)."""
+        g = (C.c_int32 * 4)(*[int(x) for x in gcd])
+        check(lib().qmiv_gen_scp_pics(self.ctx, src_ref.handle, src_tst.handle, C.byref(params) if params else None,
+                                      g, dst_ref.handle, dst_tst.handle), "qmiv_gen_scp_pics")
+
+    def calc_ivssim_scp(self, tst: Picture, ref: Picture, tst_scp: Picture, ref_scp: Picture,
+                        params: Params | None = None) -> dict:
+        """xIVSSIM::calcPicIVSSIM(Tst, Ref, TstSCP, RefSCP)."""
+        iv, r2t, t2r = C.c_double(), C.c_double(), C.c_double()
+        check(lib().qmiv_calc_ivssim_scp(self.ctx, tst.handle, ref.handle, tst_scp.handle, ref_scp.handle,
+                                         C.byref(params) if params else None, C.byref(iv), C.byref(r2t),
+                                         C.byref(t2r)), "qmiv_calc_ivssim_scp")
+        return {"ivssim": iv.value, "r2t": r2t.value, "t2r": t2r.value}
+
+    def upsample(self, src: Picture) -> Picture:
+        """A new 4:4:4 picture holding the nearest-neighbour upsampling of `src`."""
+        dst = self.picture(src.width, src.height, src.bit_depth, 444)
+        check(lib().qmiv_pic_upsample(self.ctx, src.handle, dst.handle), "qmiv_pic_upsample")
+        self.sync()
+        return dst
+
     def close(self):
         for ref in self._pics:
             p = ref()

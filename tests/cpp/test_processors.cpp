@@ -1,0 +1,143 @@
+// The reference application's per-frame processor calls (apps/QMIV/src/xAppQMIV.cpp:693-727,
+// 831-935), made through the C++ processor classes of include/qmiv.hpp on the MI355X engine.
+//
+//   test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1
+//
+// FILE0 / FILE1 hold one packed planar frame each (InputFile0 / InputFile1). Prints one JSON
+// object with every metric (doubles in %.17g) and writes the two shift-compensated pictures as
+// planar 4:4:4 16-bit files. Checks the identities the processor API promises and exits 1 on a
+// failed check.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../../include/qmiv.hpp"
+
+using namespace qmiv;
+
+static int g_Failures = 0;
+#define CHECK(cond)                                                        \
+  do {                                                                     \
+    if (!(cond)) {                                                         \
+      std::fprintf(stderr, "CHECK failed at line %d: %s\n", __LINE__, #cond); \
+      g_Failures++;                                                        \
+    }                                                                      \
+  } while (0)
+
+static std::vector<uint8_t> readFile(const char* Path, size_t Bytes) {
+  std::vector<uint8_t> B(Bytes);
+  FILE* F = std::fopen(Path, "rb");
+  if (!F || std::fread(B.data(), 1, Bytes, F) != Bytes) {
+    std::fprintf(stderr, "cannot read %zu bytes from %s\n", Bytes, Path);
+    std::exit(2);
+  }
+  std::fclose(F);
+  return B;
+}
+
+static void writeScp(const xPicP& Pic, const char* Path) {
+  const size_t N = 3 * (size_t)Pic.getArea();
+  std::vector<uint16_t> Out(N);
+  if (Pic.getBitDepth() <= 8) {
+    std::vector<uint8_t> B(N);
+    Pic.download(B.data());
+    for (size_t i = 0; i < N; i++) Out[i] = B[i];
+  } else {
+    Pic.download(Out.data());
+  }
+  FILE* F = std::fopen(Path, "wb");
+  if (!F || std::fwrite(Out.data(), 2, N, F) != N) std::exit(2);
+  std::fclose(F);
+}
+
+static void printV4(const char* Name, const flt64V4& V) {
+  std::printf("\"%s\":[%.17g,%.17g,%.17g],", Name, V[0], V[1], V[2]);
+}
+
+int main(int argc, char** argv) {
+  if (argc != 10) {
+    std::fprintf(stderr, "usage: test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1\n");
+    return 2;
+  }
+  const int32 W = std::atoi(argv[1]), H = std::atoi(argv[2]), BD = std::atoi(argv[3]), CF = std::atoi(argv[4]);
+  const bool Erp = std::atoi(argv[7]) != 0;
+  const size_t Bytes = qmiv_frame_bytes(W, H, BD, CF);
+  if (Bytes == 0) return 2;
+  std::vector<uint8_t> F0 = readFile(argv[5], Bytes), F1 = readFile(argv[6], Bytes);
+
+  try {
+    xContext Ctx(0);
+    xPicP PicInP0(Ctx, {W, H}, BD), PicInP1(Ctx, {W, H}, BD);
+    xPicP PicSCP0(Ctx, {W, H}, BD), PicSCP1(Ctx, {W, H}, BD);
+    PicInP0.fromPacked(F0.data(), CF);
+    PicInP1.fromPacked(F1.data(), CF);
+
+    xGlobClrDiffProc ProcGCD(Ctx);
+    xShftCompPicProc ProcSCP(Ctx);
+    xIVPSNR ProcPSNR(Ctx);
+    xIVSSIM ProcSSIM(Ctx);
+    ProcPSNR.initWS(Erp, W, H, BD, 360, 180);
+    ProcSSIM.create({W, H}, BD, 4, true);
+
+    // xAppQMIV.cpp:693-703
+    const int32V4 GCD_R2T = ProcGCD.CalcGlobalColorDiff(&PicInP0, &PicInP1);
+    ProcSCP.GenShftCompPics(&PicSCP1, &PicSCP0, &PicInP1, &PicInP0, GCD_R2T);
+
+    // xAppQMIV.cpp:831-935
+    const flt64V4 PSNR = ProcPSNR.calcPicPSNR(&PicInP0, &PicInP1);
+    const flt64V4 WSPSNR = ProcPSNR.calcPicWSPSNR(&PicInP0, &PicInP1);
+    flt64 R2T = 0, T2R = 0;
+    ProcPSNR.setDebugCallbackQAP([&](flt64 A, flt64 B) { R2T = A; T2R = B; });
+    const flt64 IVPSNR = ProcPSNR.calcPicIVPSNR(&PicInP0, &PicInP1, GCD_R2T);
+    const flt64V4 SSIM = ProcSSIM.calcPicSSIM(&PicInP0, &PicInP1);
+    const flt64V4 MSSSIM = ProcSSIM.calcPicMSSSIM(&PicInP0, &PicInP1);
+    flt64 SR2T = 0, ST2R = 0;
+    ProcSSIM.setDebugCallbackQAP([&](flt64 A, flt64 B) { SR2T = A; ST2R = B; });
+    const flt64 IVSSIM = ProcSSIM.calcPicIVSSIM(&PicInP0, &PicInP1, &PicSCP0, &PicSCP1);
+
+    // Identities of the processor API.
+    CHECK(IVPSNR == (R2T < T2R ? R2T : T2R));
+    CHECK(IVSSIM == (ST2R < SR2T ? ST2R : SR2T));
+    // The two-argument IV-SSIM computes its own GCD and SCP: same value as the four-argument form.
+    int32V4 GcdSeen{};
+    ProcSSIM.setDebugCallbackGCS([&](const int32V4& G) { GcdSeen = G; });
+    CHECK(ProcSSIM.calcPicIVSSIM(&PicInP0, &PicInP1) == IVSSIM);
+    CHECK(GcdSeen == GCD_R2T);
+    // Without WS the WS-PSNR processor returns plain PSNR.
+    if (!Erp) CHECK(WSPSNR == PSNR);
+    // SSIM without the luminance term is never below the full SSIM for the chosen inputs' luma.
+    const flt64V4 CS = ProcSSIM.calcPicSSIM(&PicInP0, &PicInP1, false);
+    // IV-PSNR with the colour difference computed by the processor itself.
+    int32V4 GcdAuto{};
+    ProcPSNR.setDebugCallbackGCS([&](const int32V4& G) { GcdAuto = G; });
+    const flt64 IVPSNRAuto = ProcPSNR.calcPicIVPSNR(&PicInP0, &PicInP1);
+    CHECK(GcdAuto == GCD_R2T);
+    CHECK(IVPSNRAuto == IVPSNR);
+    // Errors surface as exceptions: incompatible pictures.
+    xPicP Small(Ctx, {W / 2 * 2 == W ? W - 2 : W - 1, H}, BD);
+    bool Threw = false;
+    try {
+      (void)ProcPSNR.calcPicPSNR(&PicInP0, &Small);
+    } catch (const xError& E) {
+      Threw = E.status() == QMIV_ERR_INVALID_ARG;
+    }
+    CHECK(Threw);
+
+    writeScp(PicSCP0, argv[8]);
+    writeScp(PicSCP1, argv[9]);
+
+    std::printf("{\"gcd\":[%d,%d,%d,%d],", GCD_R2T[0], GCD_R2T[1], GCD_R2T[2], GCD_R2T[3]);
+    printV4("psnr", PSNR);
+    printV4("wspsnr", WSPSNR);
+    std::printf("\"ivpsnr\":%.17g,\"ivpsnr_r2t\":%.17g,\"ivpsnr_t2r\":%.17g,", IVPSNR, R2T, T2R);
+    printV4("ssim", SSIM);
+    printV4("ssim_cs", CS);
+    printV4("msssim", MSSSIM);
+    std::printf("\"ivssim\":%.17g,\"ivssim_r2t\":%.17g,\"ivssim_t2r\":%.17g,", IVSSIM, SR2T, ST2R);
+    std::printf("\"failures\":%d}\n", g_Failures);
+  } catch (const xError& E) {
+    std::fprintf(stderr, "error: %s\n", E.what());
+    return 3;
+  }
+  return g_Failures ? 1 : 0;
+}
