@@ -5,7 +5,10 @@ equirectangular (WS weights on), search range 2, GCD on. Frames are synthetic (i
 and resident in HBM before the timed region starts.
 
 `--workload ssim-1080p10` times SSIM + MS-SSIM + IV-SSIM of 16 frame pairs 1920x1080 yuv420p10le
-instead (its dominant kernel is the fp64 windowed-moment kernel).
+instead (its dominant kernel is the fp64 windowed-moment kernel). `--workload multiview24` times
+24 views of uneven size (8 x 4096x2048 ERP, 8 x 1920x1080, 8 x 2048x1088, yuv420p10le) x 4 frames,
+IV-PSNR + WS-PSNR + IV-SSIM, as one captured graph replay per frame set against the plain
+per-view call loop; `multiview24-ivpsnr` is the same without IV-SSIM.
 
 Prints one JSON line:
   value      whole-job Mpix/s over the timed steps
@@ -45,6 +48,9 @@ WORKLOADS = {
         cpu_stages=("validate", "preproc", "gcd", "scp", "ssim", "msssim", "ivssim"), dtype="f64",
         traffic_file="k_ssim_1920x1080_10b.txt"),
 }
+
+
+MULTIVIEW = [(4096, 2048, True)] * 8 + [(1920, 1080, False)] * 8 + [(2048, 1088, False)] * 8
 
 
 def frame_pair_bytes(w, h, bd):
@@ -127,12 +133,94 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": n, "launches": 3}))
 
 
+def run_multiview(args, with_ssim):
+    """24 uneven views x 4 frames: one captured graph replay per frame set vs the plain loop."""
+    from ivpsnr_amd import IVPSNR, IVSSIM, PSNR, WSPSNR, Engine, default_params
+    from ivpsnr_amd.synth import make_pair
+    import numpy as np
+
+    bd, nframes = 10, 4
+    metrics = PSNR | WSPSNR | IVPSNR | (IVSSIM if with_ssim else 0)
+    print(f"[bench] multiview: generating {len(MULTIVIEW)} views", file=sys.stderr, flush=True)
+    base = [make_pair(w, h, bd, seed=11 + i) for i, (w, h, _) in enumerate(MULTIVIEW)]
+    eng = Engine(0)
+    # frame f of a view: its synthetic pair circularly shifted by 64 f samples (new content, same statistics)
+    sets = []
+    for f in range(nframes):
+        sets.append([(eng.picture(w, h, bd, 420, data=np.roll(p.ref, 64 * f)),
+                      eng.picture(w, h, bd, 420, data=np.roll(p.tst, 64 * f)))
+                     for (w, h, _), p in zip(MULTIVIEW, base)])
+    prms = [default_params(erp=erp) for (_, _, erp) in MULTIVIEW]
+    eng.sync()
+    print("[bench] multiview: capturing and autotuning one graph per frame set", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batches = [eng.batch(sets[0], [metrics] * len(sets[0]), prms)]
+    info = batches[0].info()
+    # the other frame sets have the same views: they take the tuned shape without tuning again
+    batches += [eng.batch(s, [metrics] * len(s), prms, branches=info["branches"], cps_sweep=info["cps_sweep"])
+                for s in sets[1:]]
+    tune_s = time.perf_counter() - t0
+
+    def step_plain():
+        return [[eng.frame(p0, p1, metrics, prm) for (p0, p1), prm in zip(s, prms)] for s in sets]
+
+    def step_batched():
+        return [bt.run() for bt in batches]
+
+    def timed(step, steps, warmup):
+        for _ in range(warmup):
+            step()
+        eng.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            r = step()
+        eng.sync()
+        return time.perf_counter() - t, r
+
+    print(f"[bench] multiview plain loop: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
+    el_plain, r_plain = timed(step_plain, args.steps, args.warmup)
+    print(f"[bench] multiview batched: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
+    el_b, r_b = timed(step_batched, args.steps, args.warmup)
+    for fp, fb in zip(r_plain, r_b):
+        for a, b in zip(fp, fb):
+            assert a.ivpsnr == b.ivpsnr and list(a.gcd) == list(b.gcd) and list(a.wspsnr) == list(b.wspsnr)
+            if with_ssim:
+                assert a.ivssim == b.ivssim
+    mpix = nframes * sum(w * h for (w, h, _) in MULTIVIEW) / 1e6
+    ms_per_step = el_b / args.steps * 1e3
+    value = mpix / (ms_per_step / 1e3)
+    alg = nframes * sum(frame_pair_bytes(w, h, bd) for (w, h, _) in MULTIVIEW)
+    achieved = alg / (ms_per_step / 1e3) / 1e9
+    out = {
+        "metric": "Batched mixed-view Mpix/s (24 views x 4 frames, 10b 420) + %HBM roofline; batched vs plain",
+        "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32+f64" if with_ssim else "int32", "data": "synthetic",
+        "config": {"workload": "24 views (8x4096x2048 ERP, 8x1920x1080, 8x2048x1088) x 4 frames yuv420p10le, "
+                               + ("IV-PSNR+WS-PSNR+PSNR+IV-SSIM" if with_ssim else "IV-PSNR+WS-PSNR+PSNR")
+                               + ", B=2, GCD on",
+                   "mode": "batched: one captured HIP graph replay per frame set (qmiv_batch_run)",
+                   "plain_loop_mpix_s": round(mpix / (el_plain / args.steps), 2),
+                   "batched_vs_plain": round(el_plain / el_b, 3),
+                   "autotuned": info, "graph_capture_and_tuning_s": round(tune_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "whole frame set (graph replay)", "bytes_per_step": alg},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    for bt in batches:
+        bt.close()
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ivpsnr-erp4k10")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["multiview24", "multiview24-ivpsnr"],
+                    default="ivpsnr-erp4k10")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--frames", type=int, default=0, help="frame pairs per step (default: the workload's)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -145,6 +233,10 @@ def main():
         print("bench.py measures one GPU: this engine's workload runs on a single MI355X "
               "(multi-GPU sharding is a design note, not run here).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload.startswith("multiview24"):
+        run_multiview(args, with_ssim=args.workload == "multiview24")
+        return
 
     from ivpsnr_amd import Engine, default_params
     import ivpsnr_amd

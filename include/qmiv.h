@@ -246,6 +246,41 @@ qmiv_status qmiv_calc_ivssim_scp(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_
    Asynchronous on the context stream. */
 qmiv_status qmiv_pic_upsample(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst);
 
+/* ---------------------------------------------------------------------------------------------
+   Batched frame sets (no counterpart in the reference, which runs its frame loop,
+   apps/QMIV/src/xAppQMIV.cpp:661-935, once per view and process). A batch binds n picture pairs
+   of any sizes and bit depths, each with its own parameters and metrics, and captures all their
+   work into one HIP graph whose branches run views concurrently. qmiv_batch_run replays it once
+   per frame set: upload the next frames into the same pictures, then run. An autotuner picks
+   the branch count and the cap on CPS tiles per workgroup by timing replays at creation (the
+   pictures' contents at that time do not matter). Destroy batches before their context.
+   --------------------------------------------------------------------------------------------- */
+typedef struct qmiv_batch qmiv_batch;
+
+typedef struct qmiv_batch_options {
+  int32_t branches;   /* concurrent graph branches, 1..16; 0: autotuned over {1, 2, 4, 8} */
+  int32_t cps_sweep;  /* cap on CPS tiles per workgroup; 0: occupancy rule; -1: autotuned over {rule, 1, 2, 4} */
+  int32_t tune_reps;  /* timed replays per autotuner candidate (default 3) */
+} qmiv_batch_options;
+
+typedef struct qmiv_batch_info {
+  int32_t views;
+  int32_t branches;          /* chosen branch count */
+  int32_t cps_sweep;         /* chosen sweep cap (0 = occupancy rule) */
+  int32_t candidates_tried;  /* autotuner candidates timed */
+  double tuned_ms;           /* replay time of the chosen graph while tuning (0 when not tuned) */
+} qmiv_batch_info;
+
+void qmiv_batch_options_default(qmiv_batch_options* opt);
+/* prm: n parameter sets (NULL: defaults); metrics: n metric masks. */
+qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                              const qmiv_params* prm, const uint32_t* metrics, const qmiv_batch_options* opt,
+                              qmiv_batch** out);
+/* One replay of the captured frame set; out receives n results (as qmiv_frame's). */
+qmiv_status qmiv_batch_run(qmiv_batch* batch, qmiv_frame_result* out);
+qmiv_status qmiv_batch_get_info(const qmiv_batch* batch, qmiv_batch_info* info);
+void qmiv_batch_destroy(qmiv_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,15 @@ class Profile(C.Structure):
     _fields_ = [("kernel_ms", C.c_double * 8), ("launches", C.c_int64 * 8)]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [("branches", C.c_int32), ("cps_sweep", C.c_int32), ("tune_reps", C.c_int32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("views", C.c_int32), ("branches", C.c_int32), ("cps_sweep", C.c_int32),
+                ("candidates_tried", C.c_int32), ("tuned_ms", C.c_double)]
+
+
 _ctx_p = C.c_void_p
 _pic_p = C.c_void_p
 _i32p = C.POINTER(C.c_int32)
@@ -104,6 +113,13 @@ _SIGNATURES = {
     "qmiv_calc_ivssim_scp": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), _f64p, _f64p,
                                        _f64p]),
     "qmiv_pic_upsample": (C.c_int, [_ctx_p, _pic_p, _pic_p]),
+    "qmiv_batch_options_default": (None, [C.POINTER(BatchOptions)]),
+    "qmiv_batch_create": (C.c_int, [_ctx_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                    C.POINTER(Params), C.POINTER(C.c_uint32), C.POINTER(BatchOptions),
+                                    C.POINTER(C.c_void_p)]),
+    "qmiv_batch_run": (C.c_int, [C.c_void_p, C.POINTER(FrameResult)]),
+    "qmiv_batch_get_info": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
+    "qmiv_batch_destroy": (None, [C.c_void_p]),
 }
 EXPORTS.extend(_SIGNATURES)
 

@@ -279,8 +279,9 @@ Layout make_layout(const Geom& g, bool fast, int32_t sweep, uint32_t ssim_metric
 struct KernelTimer {
   qmiv_ctx* ctx;
   int kind;
+  hipStream_t st;
   hipEvent_t end = nullptr;
-  KernelTimer(qmiv_ctx* c, int k) : ctx(c), kind(k) {
+  KernelTimer(qmiv_ctx* c, int k, hipStream_t s) : ctx(c), kind(k), st(s) {
     if (!ctx->profiling) return;
     if (ctx->ev_used == ctx->ev_pool.size()) {
       hipEvent_t a, b;
@@ -292,11 +293,11 @@ struct KernelTimer {
     auto& pr = ctx->ev_pool[ctx->ev_used];
     ctx->ev_kind[ctx->ev_used] = kind;
     ctx->ev_used++;
-    (void)hipEventRecord(pr.first, ctx->stream);
+    (void)hipEventRecord(pr.first, st);
     end = pr.second;
   }
   ~KernelTimer() {
-    if (end) (void)hipEventRecord(end, ctx->stream);
+    if (end) (void)hipEventRecord(end, st);
   }
 };
 
@@ -328,12 +329,16 @@ struct FramePlan {
   int32_t md[4];
 };
 
-FramePlan plan_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, const FrameWork& fw) {
+// sweep_override > 0 caps the CPS tiles per workgroup (the batch autotuner's knob); 0 keeps
+// the occupancy rule of pick_sweep.
+FramePlan plan_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, const FrameWork& fw,
+                     int32_t sweep_override = 0) {
   FramePlan P;
   for (int c = 0; c < 4; c++) P.md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
   const int32_t* gbound = fw.gcd ? fw.gcd : P.md;
   P.fast = fast_cps_ok(g, prm, gbound);
   P.sweep = pick_sweep(ctx, g, prm, gbound);
+  if (sweep_override > 0) P.sweep = std::min(P.sweep, sweep_override);
   P.L = make_layout(g, P.fast, P.sweep, fw.ssim_metrics);
   return P;
 }
@@ -362,16 +367,16 @@ PlaneView u16_view(const uint16_t* base, int32_t w, int32_t h) {
 
 // Enqueues the SSIM-family evaluations of one frame (after the CPS kernel has written the SCP
 // pictures when IV-SSIM is asked): pyramid levels, then one windowed-moment pass per evaluation.
-qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const FramePlan& P, char* dbase) {
+qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const FramePlan& P, char* dbase,
+                         const uint16_t* scp, hipStream_t st) {
   const Geom& g = p0->g;
   const Layout& L = P.L;
-  hipStream_t st = ctx->stream;
   uint16_t* pyr = L.pyr_chain ? (uint16_t*)(dbase + L.off_pyr) : nullptr;
   auto level_ptr = [&](int pic, int c, int l) { return pyr + ((size_t)pic * 3 + c) * L.pyr_chain + L.pyr_level_off[l]; };
   bool need_pyr = false;
   for (const SsimEval& e : L.ssim) need_pyr |= e.kind == 1;
   if (need_pyr) {
-    KernelTimer t(ctx, QMIV_K_PYRAMID);
+    KernelTimer t(ctx, QMIV_K_PYRAMID, st);
     for (int pic = 0; pic < 2; pic++)
       for (int c = 0; c < 3; c++)
         for (int l = 1; l < 5; l++) {
@@ -387,7 +392,7 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
   a.c1 = k1 * k1;
   a.c2 = k2 * k2;
   a.partials = (double*)(dbase + L.off_ssim_part);
-  const uint16_t* scp0 = (const uint16_t*)ctx->d_scp;
+  const uint16_t* scp0 = scp;
   const uint16_t* scp1 = scp0 ? scp0 + 3 * (size_t)g.w * g.h : nullptr;
   for (const SsimEval& e : L.ssim) {
     a.w = e.w;
@@ -420,19 +425,21 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
         break;
     }
     {
-      KernelTimer t(ctx, QMIV_K_SSIM);
+      KernelTimer t(ctx, QMIV_K_SSIM, st);
       QMIV_HIP(launch_ssim(a, st));
     }
-    KernelTimer t(ctx, QMIV_K_SSIM_ROWS);
+    KernelTimer t(ctx, QMIV_K_SSIM_ROWS, st);
     QMIV_HIP(launch_ssim_rows(a, (double*)(dbase + e.off_rows), st));
   }
   return QMIV_OK;
 }
 
-// Enqueues one frame's statistics (+ GCD) and, when asked, its CPS search on the context stream,
-// with device scratch at `dbase` and the results block copied into pinned `hbase`.
+// Enqueues one frame's statistics (+ GCD) and, when asked, its CPS search and SSIM family on
+// stream `st`, with device scratch at `dbase`, the shift-compensated pictures (when needed) at
+// `scp` (2 x [3][h][w] uint16) and the results block copied into pinned `hbase`.
 qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params& prm,
-                          const FrameWork& fw, const FramePlan& P, char* dbase, char* hbase) {
+                          const FrameWork& fw, const FramePlan& P, char* dbase, char* hbase, uint16_t* scp,
+                          hipStream_t st) {
   const Geom& g = p0->g;
   const Layout& L = P.L;
   FrameSummary* sum = (FrameSummary*)(dbase + L.off_summary);
@@ -441,25 +448,23 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
   int64_t* row_sd = (int64_t*)(dbase + L.off_row_sd);
   uint32_t* row_max = (uint32_t*)(dbase + L.off_row_max);
   uint64_t* partials = (uint64_t*)(dbase + L.off_partials);
-  hipStream_t st = ctx->stream;
 
   if (fw.gcd) QMIV_HIP(hipMemcpyAsync(sum->gcd, fw.gcd, sizeof(int32_t) * 4, hipMemcpyHostToDevice, st));
   {
-    KernelTimer t(ctx, QMIV_K_STATS);
+    KernelTimer t(ctx, QMIV_K_STATS, st);
     QMIV_HIP(launch_stats(p0->dev, p1->dev, g, row_sd, row_ssd, row_max, st));
   }
   {
-    KernelTimer t(ctx, QMIV_K_SUMMARY);
+    KernelTimer t(ctx, QMIV_K_SUMMARY, st);
     QMIV_HIP(launch_summary(g, row_sd, row_ssd, row_max, P.md, fw.need_gcd ? 1 : 0, sum, st));
   }
   if (fw.need_cps || fw.need_scp) {
     uint16_t* scp0 = nullptr;
     uint16_t* scp1 = nullptr;
     if (fw.need_scp) {
-      const size_t plane3 = 3 * (size_t)g.w * g.h;
-      QMIV_TRY(ensure_scp(ctx, 2 * plane3 * sizeof(uint16_t)));
-      scp0 = (uint16_t*)ctx->d_scp;
-      scp1 = scp0 + plane3;
+      if (!scp) return fail(QMIV_ERR_INVALID_ARG, "no buffer for the shift-compensated pictures");
+      scp0 = scp;
+      scp1 = scp0 + 3 * (size_t)g.w * g.h;
     }
     CpsArgs a{};
     a.p0 = p0->dev;
@@ -478,16 +483,16 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     a.scp1 = scp1;
     a.maxv = (1 << g.bd) - 1;
     {
-      KernelTimer t(ctx, QMIV_K_CPS);
+      KernelTimer t(ctx, QMIV_K_CPS, st);
       if (P.fast) QMIV_HIP(launch_cps_fast(a, st));
       else QMIV_HIP(launch_cps_generic(a, st));
     }
     {
-      KernelTimer t(ctx, QMIV_K_CPS_ROWS);
+      KernelTimer t(ctx, QMIV_K_CPS_ROWS, st);
       QMIV_HIP(launch_cps_rows(partials, L.nseg, g.h, cps_rows, st));
     }
   }
-  if (!L.ssim.empty()) QMIV_TRY(enqueue_ssim(ctx, p0, p1, P, dbase));
+  if (!L.ssim.empty()) QMIV_TRY(enqueue_ssim(ctx, p0, p1, P, dbase, scp, st));
   QMIV_HIP(hipMemcpyAsync(hbase, dbase, L.results_bytes, hipMemcpyDeviceToHost, st));
   return QMIV_OK;
 }
@@ -506,7 +511,12 @@ qmiv_status run_frame_kernels(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic*
   used_fast = P.fast;
   QMIV_TRY(ensure_scratch(ctx, P.L.total));
   QMIV_TRY(ensure_results(ctx, P.L.results_bytes));
-  QMIV_TRY(enqueue_frame(ctx, p0, p1, prm, fw, P, (char*)ctx->d_scratch, (char*)ctx->h_results));
+  uint16_t* scp = nullptr;
+  if (fw.need_scp) {
+    QMIV_TRY(ensure_scp(ctx, 6 * (size_t)p0->g.w * p0->g.h * sizeof(uint16_t)));
+    scp = (uint16_t*)ctx->d_scp;
+  }
+  QMIV_TRY(enqueue_frame(ctx, p0, p1, prm, fw, P, (char*)ctx->d_scratch, (char*)ctx->h_results, scp, ctx->stream));
   return wait_stream(ctx);
 }
 
@@ -621,6 +631,176 @@ qmiv_status finalize_frame(const Geom& g, const qmiv_params& prm, uint32_t metri
 // =============================================================================================
 // C ABI
 // =============================================================================================
+// =============================================================================================
+// Batched frame sets: every view's work captured once into one HIP graph with concurrent
+// branches, replayed per frame set.
+// =============================================================================================
+struct qmiv_batch {
+  qmiv_ctx* ctx = nullptr;
+  int32_t n = 0;
+  std::vector<const qmiv_pic*> p0, p1;
+  std::vector<qmiv_params> prm;
+  std::vector<uint32_t> metrics;
+  std::vector<FrameWork> fw;
+  std::vector<FramePlan> plans;
+  std::vector<size_t> doff, hoff;
+  std::vector<int32_t> branch_of;   // view -> branch
+  std::vector<size_t> scp_off;      // branch -> element offset of its SCP buffer in d_scp
+  void* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void* h_results = nullptr;
+  size_t results_bytes = 0;
+  void* d_scp = nullptr;
+  size_t scp_bytes = 0;
+  std::vector<hipStream_t> streams;  // branch streams, default priority
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  qmiv_batch_info info{};
+};
+
+namespace {
+
+constexpr int32_t kMaxBranches = 16;
+
+void batch_drop_graph(qmiv_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Grows a device (or pinned host) buffer; the old contents are not kept.
+qmiv_status batch_grow(void*& ptr, size_t& have, size_t need, bool host) {
+  if (have >= need) return QMIV_OK;
+  if (ptr) {
+    if (host) QMIV_HIP(hipHostFree(ptr));
+    else QMIV_HIP(hipFree(ptr));
+  }
+  ptr = nullptr;
+  have = 0;
+  const size_t sz = align_up(std::max<size_t>(need, 1), 1 << 16);
+  const hipError_t e = host ? hipHostMalloc(&ptr, sz, hipHostMallocDefault) : hipMalloc(&ptr, sz);
+  if (e != hipSuccess) return fail(QMIV_ERR_OUT_OF_MEMORY, "batch buffer allocation failed");
+  have = sz;
+  return QMIV_OK;
+}
+
+// Plans every view for (branches, sweep cap), assigns views to branches (longest work first onto
+// the least loaded branch), lays out the buffers and captures the graph.
+qmiv_status batch_build(qmiv_batch* b, int32_t branches, int32_t sweep) {
+  qmiv_ctx* ctx = b->ctx;
+  batch_drop_graph(b);
+  const size_t n = (size_t)b->n;
+  b->plans.assign(n, FramePlan{});
+  b->doff.assign(n, 0);
+  b->hoff.assign(n, 0);
+  size_t dtot = 0, htot = 0;
+  std::vector<double> cost(n);
+  for (size_t i = 0; i < n; i++) {
+    b->plans[i] = plan_frame(ctx, b->p0[i]->g, b->prm[i], b->fw[i], sweep);
+    b->doff[i] = dtot;
+    b->hoff[i] = htot;
+    dtot += align_up(b->plans[i].L.total, 256);
+    htot += align_up(b->plans[i].L.results_bytes, 256);
+    const Geom& g = b->p0[i]->g;
+    cost[i] = (double)g.w * g.h * (1.0 + (b->fw[i].need_cps || b->fw[i].need_scp ? 4.0 : 0.0) +
+                                   4.0 * (double)b->plans[i].L.ssim.size());
+  }
+  std::vector<size_t> order(n);
+  for (size_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return cost[a] > cost[c]; });
+  std::vector<double> load((size_t)branches, 0.0);
+  std::vector<size_t> scp_need((size_t)branches, 0);
+  b->branch_of.assign(n, 0);
+  for (size_t i : order) {
+    const size_t br = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    b->branch_of[i] = (int32_t)br;
+    load[br] += cost[i];
+    if (b->fw[i].need_scp)
+      scp_need[br] = std::max(scp_need[br], 6 * (size_t)b->p0[i]->g.w * b->p0[i]->g.h);
+  }
+  b->scp_off.assign((size_t)branches, 0);
+  size_t stot = 0;
+  for (int32_t br = 0; br < branches; br++) {
+    b->scp_off[(size_t)br] = stot;
+    stot += align_up(scp_need[(size_t)br], 128);
+  }
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  QMIV_TRY(batch_grow(b->d_scratch, b->scratch_bytes, dtot, false));
+  QMIV_TRY(batch_grow(b->h_results, b->results_bytes, htot, true));
+  if (stot) QMIV_TRY(batch_grow(b->d_scp, b->scp_bytes, stot * sizeof(uint16_t), false));
+
+  while ((int32_t)b->streams.size() < branches) {
+    hipStream_t s;
+    QMIV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    b->streams.push_back(s);
+    hipEvent_t e;
+    QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->join.push_back(e);
+  }
+  if (!b->fork) QMIV_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+
+  // Capture. Kernel timing is off inside the graph; enqueue errors end the capture first.
+  const bool prof = ctx->profiling;
+  ctx->profiling = false;
+  QMIV_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  qmiv_status st = QMIV_OK;
+  if (hipEventRecord(b->fork, ctx->stream) != hipSuccess) st = fail(QMIV_ERR_HIP, "hipEventRecord (fork)");
+  for (int32_t br = 0; br < branches && st == QMIV_OK; br++)
+    if (hipStreamWaitEvent(b->streams[(size_t)br], b->fork, 0) != hipSuccess)
+      st = fail(QMIV_ERR_HIP, "hipStreamWaitEvent (fork)");
+  for (size_t i = 0; i < n && st == QMIV_OK; i++) {
+    const int32_t br = b->branch_of[i];
+    uint16_t* scp = b->fw[i].need_scp ? (uint16_t*)b->d_scp + b->scp_off[(size_t)br] : nullptr;
+    st = enqueue_frame(ctx, b->p0[i], b->p1[i], b->prm[i], b->fw[i], b->plans[i], (char*)b->d_scratch + b->doff[i],
+                       (char*)b->h_results + b->hoff[i], scp, b->streams[(size_t)br]);
+  }
+  for (int32_t br = 0; br < branches && st == QMIV_OK; br++) {
+    if (hipEventRecord(b->join[(size_t)br], b->streams[(size_t)br]) != hipSuccess ||
+        hipStreamWaitEvent(ctx->stream, b->join[(size_t)br], 0) != hipSuccess)
+      st = fail(QMIV_ERR_HIP, "graph join");
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
+  ctx->profiling = prof;
+  if (st != QMIV_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (ee != hipSuccess) return fail(QMIV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+  b->graph = graph;
+  QMIV_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->info.branches = branches;
+  b->info.cps_sweep = sweep;
+  return QMIV_OK;
+}
+
+// Average replay time of the current graph over `reps` launches (after one untimed launch).
+qmiv_status batch_time(qmiv_batch* b, int32_t reps, double& ms) {
+  qmiv_ctx* ctx = b->ctx;
+  hipEvent_t t0, t1;
+  QMIV_HIP(hipEventCreate(&t0));
+  QMIV_HIP(hipEventCreate(&t1));
+  qmiv_status st = QMIV_OK;
+  if (hipGraphLaunch(b->exec, ctx->stream) != hipSuccess || hipEventRecord(t0, ctx->stream) != hipSuccess)
+    st = fail(QMIV_ERR_HIP, "graph launch while tuning");
+  for (int32_t r = 0; r < reps && st == QMIV_OK; r++)
+    if (hipGraphLaunch(b->exec, ctx->stream) != hipSuccess) st = fail(QMIV_ERR_HIP, "graph launch while tuning");
+  float el = 0.f;
+  if (st == QMIV_OK && (hipEventRecord(t1, ctx->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                        hipEventElapsedTime(&el, t0, t1) != hipSuccess))
+    st = fail(QMIV_ERR_HIP, "timing events while tuning");
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  ms = (double)el / std::max(reps, 1);
+  return st;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* qmiv_last_error(void) { return g_last_error.c_str(); }
@@ -793,9 +973,17 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
   }
   QMIV_TRY(ensure_scratch(ctx, dtot));
   QMIV_TRY(ensure_results(ctx, htot));
+  // Frames run in stream order, so one set of shift-compensated pictures serves them all.
+  uint16_t* scp = nullptr;
+  if (fw.need_scp) {
+    size_t most = 0;
+    for (int32_t i = 0; i < n; i++) most = std::max(most, (size_t)p0[i]->g.w * p0[i]->g.h);
+    QMIV_TRY(ensure_scp(ctx, 6 * most * sizeof(uint16_t)));
+    scp = (uint16_t*)ctx->d_scp;
+  }
   for (int32_t i = 0; i < n; i++)
     QMIV_TRY(enqueue_frame(ctx, p0[i], p1[i], prm, fw, plans[(size_t)i], (char*)ctx->d_scratch + doff[(size_t)i],
-                           (char*)ctx->h_results + hoff[(size_t)i]));
+                           (char*)ctx->h_results + hoff[(size_t)i], scp, ctx->stream));
   QMIV_TRY(wait_stream(ctx));
   qmiv_status first_err = QMIV_OK;
   for (int32_t i = 0; i < n; i++) {
@@ -1028,6 +1216,126 @@ qmiv_status qmiv_pic_upsample(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst)
   QMIV_HIP(hipSetDevice(ctx->device));
   QMIV_HIP(launch_upsample(src->dev, src->g, dst->dev, ctx->stream));
   return QMIV_OK;
+}
+
+
+void qmiv_batch_options_default(qmiv_batch_options* o) {
+  if (!o) return;
+  o->branches = 0;
+  o->cps_sweep = -1;
+  o->tune_reps = 3;
+}
+
+qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                              const qmiv_params* prm, const uint32_t* metrics, const qmiv_batch_options* opt_in,
+                              qmiv_batch** out) {
+  if (!ctx || !p0 || !p1 || !metrics || !out || n <= 0) return fail(QMIV_ERR_INVALID_ARG, "null argument or empty batch");
+  *out = nullptr;
+  qmiv_batch_options opt;
+  qmiv_batch_options_default(&opt);
+  if (opt_in) opt = *opt_in;
+  if (opt.branches < 0 || opt.branches > kMaxBranches) return fail(QMIV_ERR_INVALID_ARG, "branches must be 0..16");
+  if (opt.cps_sweep < -1) return fail(QMIV_ERR_INVALID_ARG, "cps_sweep must be -1, 0 or positive");
+  if (opt.tune_reps <= 0) opt.tune_reps = 3;
+  for (int32_t i = 0; i < n; i++) {
+    QMIV_TRY(check_pair(p0[i], p1[i]));
+    if (p0[i]->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
+    if (metrics[i] & ~(uint32_t)(QMIV_PSNR | QMIV_WSPSNR | QMIV_IVPSNR | QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM))
+      return fail(QMIV_ERR_INVALID_ARG, "unknown metric bits");
+  }
+  QMIV_HIP(hipSetDevice(ctx->device));
+  qmiv_batch* b = new qmiv_batch;
+  b->ctx = ctx;
+  b->n = n;
+  b->p0.assign(p0, p0 + n);
+  b->p1.assign(p1, p1 + n);
+  b->prm.resize((size_t)n);
+  b->metrics.assign(metrics, metrics + n);
+  b->fw.resize((size_t)n);
+  for (int32_t i = 0; i < n; i++) {
+    if (prm) b->prm[(size_t)i] = prm[i];
+    else qmiv_params_default(&b->prm[(size_t)i]);
+    FrameWork& f = b->fw[(size_t)i];
+    f.need_gcd = (metrics[i] & (QMIV_IVPSNR | QMIV_IVSSIM)) != 0;
+    f.need_cps = (metrics[i] & QMIV_IVPSNR) != 0;
+    f.need_scp = (metrics[i] & QMIV_IVSSIM) != 0;
+    f.ssim_metrics = metrics[i] & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM);
+  }
+  b->info.views = n;
+  // Candidates: branch counts first (with the occupancy sweep rule), then sweep caps.
+  std::vector<int32_t> br_cands, sw_cands;
+  if (opt.branches > 0) br_cands = {std::min(opt.branches, n)};
+  else
+    for (int32_t c : {1, 2, 4, 8})
+      if (c <= n) br_cands.push_back(c);
+  if (opt.cps_sweep >= 0) sw_cands = {opt.cps_sweep};
+  else sw_cands = {0, 1, 2, 4};
+  int32_t best_br = br_cands[0], best_sw = sw_cands[0];
+  const bool tune = br_cands.size() > 1 || sw_cands.size() > 1;
+  double best_ms = 0.0;
+  qmiv_status st = QMIV_OK;
+  if (tune) {
+    best_ms = 1e300;
+    auto try_one = [&](int32_t br, int32_t sw) -> qmiv_status {
+      QMIV_TRY(batch_build(b, br, sw));
+      double ms = 0.0;
+      QMIV_TRY(batch_time(b, opt.tune_reps, ms));
+      b->info.candidates_tried++;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_br = br;
+        best_sw = sw;
+      }
+      return QMIV_OK;
+    };
+    for (int32_t br : br_cands)
+      if ((st = try_one(br, sw_cands[0])) != QMIV_OK) break;
+    for (size_t k = 1; k < sw_cands.size() && st == QMIV_OK; k++) st = try_one(best_br, sw_cands[k]);
+  }
+  if (st == QMIV_OK) st = batch_build(b, best_br, best_sw);
+  if (st != QMIV_OK) {
+    qmiv_batch_destroy(b);
+    return st;
+  }
+  b->info.tuned_ms = tune ? best_ms : 0.0;
+  *out = b;
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_batch_run(qmiv_batch* b, qmiv_frame_result* out) {
+  if (!b || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  qmiv_ctx* ctx = b->ctx;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_HIP(hipGraphLaunch(b->exec, ctx->stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  qmiv_status first_err = QMIV_OK;
+  for (int32_t i = 0; i < b->n; i++) {
+    const size_t k = (size_t)i;
+    const qmiv_status st = finalize_frame(b->p0[k]->g, b->prm[k], b->metrics[k], b->fw[k], b->plans[k].L,
+                                          (const char*)b->h_results + b->hoff[k], &out[i]);
+    if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
+  }
+  return first_err;
+}
+
+qmiv_status qmiv_batch_get_info(const qmiv_batch* b, qmiv_batch_info* info) {
+  if (!b || !info) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  *info = b->info;
+  return QMIV_OK;
+}
+
+void qmiv_batch_destroy(qmiv_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_drop_graph(b);
+  for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->d_scratch) (void)hipFree(b->d_scratch);
+  if (b->d_scp) (void)hipFree(b->d_scp);
+  if (b->h_results) (void)hipHostFree(b->h_results);
+  delete b;
 }
 
 }  // extern "C"

@@ -65,6 +65,7 @@ class Engine:
         check(lib().qmiv_create(device, C.byref(ctx)), "qmiv_create")
         self.ctx = ctx
         self._pics = []
+        self._batches = []
 
     @property
     def stream(self) -> int:
@@ -120,6 +121,14 @@ class Engine:
         if raise_on_error:
             check(st, "qmiv_frames")
         return list(out)
+
+    def batch(self, pairs, metrics, params=None, branches: int = 0, cps_sweep: int = -1,
+              tune_reps: int = 3) -> "Batch":
+        """A captured frame set over the (p0, p1) pairs (qmiv_batch_create). `metrics` and
+        `params` are one value for every view or one per view."""
+        bt = Batch(self, pairs, metrics, params, branches, cps_sweep, tune_reps)
+        self._batches.append(weakref.ref(bt))
+        return bt
 
     def calc_gcd(self, tst: Picture, ref: Picture, params: Params | None = None) -> np.ndarray:
         g = (C.c_int32 * 4)()
@@ -203,6 +212,10 @@ class Engine:
         return dst
 
     def close(self):
+        for ref in getattr(self, "_batches", []):
+            bt = ref()
+            if bt is not None:
+                bt.close()
         for ref in self._pics:
             p = ref()
             if p is not None:
@@ -217,3 +230,49 @@ class Engine:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Batch:
+    """Views bound to one captured HIP graph; run() replays it for the frames the pictures hold."""
+
+    def __init__(self, engine: Engine, pairs, metrics, params, branches, cps_sweep, tune_reps):
+        n = len(pairs)
+        self.engine = engine
+        self.n = n
+        self._pairs = list(pairs)  # keep the pictures alive as long as the batch
+        a = (C.c_void_p * n)(*[p[0].handle.value for p in pairs])
+        b = (C.c_void_p * n)(*[p[1].handle.value for p in pairs])
+        ms = list(metrics) if hasattr(metrics, "__len__") else [metrics] * n
+        m = (C.c_uint32 * n)(*ms)
+        if params is None:
+            prm = None
+        else:
+            ps = list(params) if isinstance(params, (list, tuple)) else [params] * n
+            prm = (Params * n)(*ps)
+        opt = capi.BatchOptions(branches, cps_sweep, tune_reps)
+        h = C.c_void_p()
+        check(lib().qmiv_batch_create(engine.ctx, n, a, b, prm, m, C.byref(opt), C.byref(h)), "qmiv_batch_create")
+        self.handle = h
+
+    def run(self, raise_on_error: bool = True):
+        out = (FrameResult * self.n)()
+        st = lib().qmiv_batch_run(self.handle, out)
+        if raise_on_error:
+            check(st, "qmiv_batch_run")
+        return list(out)
+
+    def info(self) -> dict:
+        i = capi.BatchInfo()
+        check(lib().qmiv_batch_get_info(self.handle, C.byref(i)), "qmiv_batch_get_info")
+        return {k: getattr(i, k) for k, _ in capi.BatchInfo._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().qmiv_batch_destroy(self.handle)
+            self.handle = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
