@@ -61,7 +61,7 @@ struct qmiv_ctx {
   void* d_scp = nullptr;
   size_t scp_bytes = 0;
   int32_t num_cus = 256;
-  int32_t cps_blocks_per_cu[2] = {0, 0};  // [u8, u16], queried on first use
+  int32_t cps_blocks_per_cu[2][2] = {{0, 0}, {0, 0}};  // [4:4:4 kernel, 4:2:0 kernel][u8, u16], queried on first use
   // Kernel timing: event pairs recorded around launches since the last harvest.
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -158,6 +158,7 @@ struct SsimEval {
   int level;      // pyramid level (MS-SSIM)
   int32_t w, h;   // size at this level
   size_t off_rows;  // per-row sums in the results block
+  size_t off_part;  // device: its [tiles_x][h] tile partials
 };
 
 struct Layout {
@@ -165,7 +166,6 @@ struct Layout {
   size_t off_row_sd, off_row_max, off_partials, total;
   int32_t nseg;
   std::vector<SsimEval> ssim;
-  size_t off_ssim_part = 0;   // device: [tiles_x][h] partials (reused by every evaluation)
   size_t off_pyr = 0;         // device: MS-SSIM pyramid levels 1..4, [picture][component][level]
   size_t pyr_level_off[5] = {0, 0, 0, 0, 0};  // element offsets of the levels within one plane chain
   size_t pyr_chain = 0;       // elements of one (picture, component) chain
@@ -195,33 +195,39 @@ bool fast_cps_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) 
   return true;
 }
 
+// Tile of the fast CPS kernel that serves this geometry: the 4:2:0 kernel (two columns per lane)
+// or the general one (one column per lane).
+void cps_tile(bool fast420, int32_t* tw, int32_t* th) {
+  if (fast420) cps_fast420_tile(tw, th);
+  else cps_fast_tile(tw, th);
+}
+
 // Tiles per workgroup along x: just enough that every workgroup is resident at once
 // (CUs x resident workgroups per CU), which gives one round with no tail, and within the bound
-// that keeps the per-lane row sums (one pel per tile) below 2^32.
-int32_t pick_sweep(qmiv_ctx* ctx, const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) {
+// that keeps the per-lane row sums (one pel per tile and lane column) below 2^32.
+int32_t pick_sweep(qmiv_ctx* ctx, const Geom& g, bool fast420, const int32_t* gcd_bound) {
   int32_t tw, th;
-  cps_fast_tile(&tw, &th);
+  cps_tile(fast420, &tw, &th);
   const int32_t bands = (g.h + th - 1) / th;
   const int32_t tiles_x = (g.w + tw - 1) / tw;
   const int64_t tiles = (int64_t)bands * tiles_x;
-  int32_t& bpc = ctx->cps_blocks_per_cu[g.bytes_per_sample == 1 ? 0 : 1];
-  if (bpc == 0) bpc = cps_fast_blocks_per_cu(g.bytes_per_sample);
+  int32_t& bpc = ctx->cps_blocks_per_cu[fast420 ? 1 : 0][g.bytes_per_sample == 1 ? 0 : 1];
+  if (bpc == 0) bpc = fast420 ? cps_fast420_blocks_per_cu(g.bytes_per_sample) : cps_fast_blocks_per_cu(g.bytes_per_sample);
   const int64_t resident = std::max<int64_t>(1, (int64_t)ctx->num_cus * bpc);
   int64_t sweep = (tiles + resident - 1) / resident;
   const int64_t maxv = (1 << g.bd) - 1;
   int64_t m = 0;
   for (int c = 0; c < 3; c++) m = std::max<int64_t>(m, maxv + std::abs((int64_t)gcd_bound[c]));
-  const int64_t sq = (m + maxv) * (m + maxv);
+  const int64_t sq = (m + maxv) * (m + maxv) * (fast420 ? 2 : 1);
   sweep = std::min<int64_t>(sweep, std::max<int64_t>(1, (int64_t)0xffffffffLL / std::max<int64_t>(sq, 1)));
-  (void)p;
   return (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles_x, sweep));
 }
 
-Layout make_layout(const Geom& g, bool fast, int32_t sweep, uint32_t ssim_metrics) {
+Layout make_layout(const Geom& g, bool fast, bool fast420, int32_t sweep, uint32_t ssim_metrics) {
   Layout L{};
   const int32_t nrows = g.h + 2 * g.ch;
   int32_t tw, th;
-  cps_fast_tile(&tw, &th);
+  cps_tile(fast420, &tw, &th);
   const int32_t tiles_x = (g.w + tw - 1) / tw;
   L.nseg = fast ? (tiles_x + sweep - 1) / sweep : (g.w + kGenericCols - 1) / kGenericCols;
   size_t o = 0;
@@ -233,7 +239,7 @@ Layout make_layout(const Geom& g, bool fast, int32_t sweep, uint32_t ssim_metric
   o = align_up(o + sizeof(uint64_t) * (size_t)nrows, 256);
   // SSIM family: per-row sums of every evaluation, in the results block.
   auto add_eval = [&](int kind, int cmp, int level) {
-    SsimEval e{kind, cmp, level, g.w >> level, g.h >> level, 0};
+    SsimEval e{kind, cmp, level, g.w >> level, g.h >> level, 0, 0};
     e.off_rows = o;
     o = align_up(o + sizeof(double) * (size_t)std::max(e.h, 1), 256);
     L.ssim.push_back(e);
@@ -255,11 +261,9 @@ Layout make_layout(const Geom& g, bool fast, int32_t sweep, uint32_t ssim_metric
   o = align_up(o + sizeof(uint32_t) * 2 * (size_t)nrows, 256);
   L.off_partials = o;
   o = align_up(o + sizeof(uint64_t) * 6 * (size_t)g.h * (size_t)L.nseg, 256);
-  if (!L.ssim.empty()) {
-    size_t part = 0;
-    for (const SsimEval& e : L.ssim) part = std::max(part, (size_t)ssim_tiles_x(e.w) * (size_t)std::max(e.h, 1));
-    L.off_ssim_part = o;
-    o = align_up(o + sizeof(double) * std::max<size_t>(part, 1), 256);
+  for (SsimEval& e : L.ssim) {
+    e.off_part = o;
+    o = align_up(o + sizeof(double) * std::max<size_t>((size_t)ssim_tiles_x(e.w) * (size_t)std::max(e.h, 1), 1), 256);
   }
   if (ssim_metrics & QMIV_MSSSIM) {
     size_t chain = 0;
@@ -324,7 +328,8 @@ struct FrameWork {
 // Everything decided on the host before a frame's kernels are enqueued.
 struct FramePlan {
   Layout L;
-  bool fast = false;
+  bool fast = false;     // fast CPS kernel (else the generic one)
+  bool fast420 = false;  // ... its 4:2:0 form
   int32_t sweep = 1;
   int32_t md[4];
 };
@@ -337,9 +342,10 @@ FramePlan plan_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, const
   for (int c = 0; c < 4; c++) P.md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
   const int32_t* gbound = fw.gcd ? fw.gcd : P.md;
   P.fast = fast_cps_ok(g, prm, gbound);
-  P.sweep = pick_sweep(ctx, g, prm, gbound);
+  P.fast420 = P.fast && g.cs == 1;
+  P.sweep = pick_sweep(ctx, g, P.fast420, gbound);
   if (sweep_override > 0) P.sweep = std::min(P.sweep, sweep_override);
-  P.L = make_layout(g, P.fast, P.sweep, fw.ssim_metrics);
+  P.L = make_layout(g, P.fast, P.fast420, P.sweep, fw.ssim_metrics);
   return P;
 }
 
@@ -377,13 +383,19 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
   for (const SsimEval& e : L.ssim) need_pyr |= e.kind == 1;
   if (need_pyr) {
     KernelTimer t(ctx, QMIV_K_PYRAMID, st);
-    for (int pic = 0; pic < 2; pic++)
-      for (int c = 0; c < 3; c++)
-        for (int l = 1; l < 5; l++) {
-          const PlaneView src = l == 1 ? pic_view(pic ? p1 : p0, c)
-                                       : u16_view(level_ptr(pic, c, l - 1), g.w >> (l - 1), g.h >> (l - 1));
-          QMIV_HIP(launch_downsample(src, g.w >> l, g.h >> l, level_ptr(pic, c, l), st));
+    for (int l = 1; l < 5; l++) {
+      DownsampleJobs j{};
+      j.n = 6;
+      j.dw = g.w >> l;
+      j.dh = g.h >> l;
+      for (int pic = 0; pic < 2; pic++)
+        for (int c = 0; c < 3; c++) {
+          j.src[pic * 3 + c] = l == 1 ? pic_view(pic ? p1 : p0, c)
+                                      : u16_view(level_ptr(pic, c, l - 1), g.w >> (l - 1), g.h >> (l - 1));
+          j.dst[pic * 3 + c] = level_ptr(pic, c, l);
         }
+      QMIV_HIP(launch_downsample(j, st));
+    }
   }
   SsimArgs a{};
   fin::ssim_window(a.f);
@@ -391,7 +403,6 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
   const double k1 = 0.01 * maxv, k2 = 0.03 * maxv;
   a.c1 = k1 * k1;
   a.c2 = k2 * k2;
-  a.partials = (double*)(dbase + L.off_ssim_part);
   const uint16_t* scp0 = scp;
   const uint16_t* scp1 = scp0 ? scp0 + 3 * (size_t)g.w * g.h : nullptr;
   for (const SsimEval& e : L.ssim) {
@@ -424,13 +435,22 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
         a.ref = u16_view(scp0 + e.cmp * plane, g.w, g.h);
         break;
     }
-    {
-      KernelTimer t(ctx, QMIV_K_SSIM, st);
-      QMIV_HIP(launch_ssim(a, st));
-    }
-    KernelTimer t(ctx, QMIV_K_SSIM_ROWS, st);
-    QMIV_HIP(launch_ssim_rows(a, (double*)(dbase + e.off_rows), st));
+    a.partials = (double*)(dbase + e.off_part);
+    KernelTimer t(ctx, QMIV_K_SSIM, st);
+    QMIV_HIP(launch_ssim(a, st));
   }
+  SsimRowsJobs j{};
+  for (const SsimEval& e : L.ssim) {
+    if (j.n == kMaxSsimEvals) return fail(QMIV_ERR_UNSUPPORTED, "too many SSIM evaluations in one frame");
+    j.tiles_x[j.n] = ssim_tiles_x(e.w);
+    j.h[j.n] = e.h;
+    j.part[j.n] = (const double*)(dbase + e.off_part);
+    j.rows[j.n] = (double*)(dbase + e.off_rows);
+    j.max_h = std::max(j.max_h, e.h);
+    j.n++;
+  }
+  KernelTimer t(ctx, QMIV_K_SSIM_ROWS, st);
+  QMIV_HIP(launch_ssim_rows(j, st));
   return QMIV_OK;
 }
 
@@ -474,7 +494,7 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     for (int c = 0; c < 4; c++) a.ws[c] = prm.cmp_weights_search[c];
     a.B = prm.search_range;
     int32_t tw, th;
-    cps_fast_tile(&tw, &th);
+    cps_tile(P.fast420, &tw, &th);
     a.tiles_x = (g.w + tw - 1) / tw;
     a.sweep = P.sweep;
     a.nseg = L.nseg;
@@ -484,7 +504,8 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     a.maxv = (1 << g.bd) - 1;
     {
       KernelTimer t(ctx, QMIV_K_CPS, st);
-      if (P.fast) QMIV_HIP(launch_cps_fast(a, st));
+      if (P.fast420) QMIV_HIP(launch_cps_fast420(a, st));
+      else if (P.fast) QMIV_HIP(launch_cps_fast(a, st));
       else QMIV_HIP(launch_cps_generic(a, st));
     }
     {

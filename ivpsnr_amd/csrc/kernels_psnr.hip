@@ -7,52 +7,16 @@
 // which reads the same values as the reference's margin replication
 // (libs/pmbb-core/src/xPixelOpsSTD.cpp:210-236).
 #include "qmiv_internal.h"
+#include "device_common.h"
 
 #include <climits>
 
 namespace qmiv {
 namespace {
 
-typedef short short2v __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
 template <typename T>
 __device__ __forceinline__ uint32_t ldpel(const T* __restrict__ p, int64_t i) {
   return (uint32_t)p[i];
-}
-// Buffer-resource view of one packed frame: 32-bit sample offsets, one VGPR per address, and
-// hardware range checking (reads beyond `bytes` return 0, writes are dropped).
-constexpr int32_t kBufDword3 = 0x00020000;  // raw buffer, gfx9-family data format
-template <typename T>
-struct Buf {
-  __amdgpu_buffer_rsrc_t r;
-  __device__ __forceinline__ Buf(const void* p, uint32_t bytes)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, kBufDword3)) {}
-  __device__ __forceinline__ uint32_t ld(uint32_t i) const {
-    if constexpr (sizeof(T) == 2) return __builtin_amdgcn_raw_buffer_load_b16(r, i * 2u, 0, 0);
-    else return __builtin_amdgcn_raw_buffer_load_b8(r, i, 0, 0);
-  }
-  __device__ __forceinline__ void st(uint32_t i, uint32_t v) const {
-    if constexpr (sizeof(T) == 2) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, i * 2u, 0, 0);
-    else __builtin_amdgcn_raw_buffer_store_b8((unsigned char)v, r, i, 0, 0);
-  }
-};
-
-__device__ __forceinline__ int64_t wave_sum(int64_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
-  return v;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -230,13 +194,6 @@ __global__ __launch_bounds__(256) void k_summary(Geom g, const int64_t* __restri
 // `sweep` tiles along x; per-row per-component sums are reduced over the wave after every tile
 // and accumulated in LDS, then written as one partial per (segment, direction, component, row).
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ int32_t mad24(int32_t a, int32_t b, int32_t c) {
-  // Kept as one instruction: the compiler would otherwise reassociate a chain of these into
-  // mul + mul + add3. ALU only.
-  int32_t d;
-  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
-  return d;
-}
 
 // LDS tile of the fast kernel: rows [y0 - 2, y0 + kCpsTileH + 2), columns [x0 - 4, x0 + 68). The two extra
 // columns on each side keep every row start 8-byte aligned for the vectorised fill; the search

@@ -1,15 +1,28 @@
 // gfx950 kernels of the SSIM family: windowed moments (SSIM, MS-SSIM scales, IV-SSIM), the
 // integer 2x2 pyramid of MS-SSIM and the per-row reduction of tile partial sums.
+#include <utility>
+
 #include "qmiv_internal.h"
 
 namespace qmiv {
 namespace {
 
-constexpr int kSsimTW = 64;            // output columns per workgroup (one per lane)
-constexpr int kSsimK = 8;              // output rows per lane
-constexpr int kSsimTH = 4 * kSsimK;    // output rows per workgroup
-constexpr int kSsimLW = kSsimTW + 9;   // window columns dx in [-5, 4]
+// Tile shape: each lane owns kSsimC adjacent output columns x kSsimK output rows; 4 waves stack
+// their rows. A loaded window sample feeds up to kSsimC x kSsimK pels.
+#ifndef QMIV_SSIM_COLS
+#define QMIV_SSIM_COLS 1
+#endif
+#ifndef QMIV_SSIM_ROWS
+#define QMIV_SSIM_ROWS 4
+#endif
+constexpr int kSsimC = QMIV_SSIM_COLS;
+constexpr int kSsimK = QMIV_SSIM_ROWS;
+constexpr int kSsimTW = 64 * kSsimC;                 // output columns per workgroup
+constexpr int kSsimTH = 4 * kSsimK;                  // output rows per workgroup
+constexpr int kSsimLW = (kSsimTW + 9 + 1) & ~1;      // window columns dx in [-5, 4], even row stride
 constexpr int kSsimLH = kSsimTH + 9;
+constexpr int kSsimWRow = kSsimK + 16;               // stride of one transposed weight column in LDS
+static_assert(kSsimC == 1 || kSsimC == 2, "one or two columns per lane");
 
 constexpr int32_t kBufDword3 = 0x00020000;
 
@@ -33,6 +46,57 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
   return v;
 }
 
+// One loaded window row R of the lane's K x C output pels. Sample column c (relative to the lane's
+// first window column) feeds output column j with window column dxi = c - j, and output row k
+// with window row dy = R - k, weight F[dy][dxi]; k runs over max(0, R - 9) .. min(K - 1, R).
+template <int R>
+__device__ __forceinline__ void ssim_row(const float* s_t, const float* s_r, const double* s_w, int base,
+                                         double (&acc)[kSsimC][kSsimK][5]) {
+  constexpr int k_lo = R > 9 ? R - 9 : 0;
+  constexpr int k_hi = R < kSsimK - 1 ? R : kSsimK - 1;
+  const int row = base + R * kSsimLW;
+#pragma unroll
+  for (int c = 0; c < 9 + kSsimC; c++) {
+    const double tv = (double)s_t[row + c];
+    const double rv = (double)s_r[row + c];
+    const double rr = rv * rv, tt = tv * tv, rt = rv * tv;
+#pragma unroll
+    for (int j = 0; j < kSsimC; j++) {
+      const int dxi = c - j;
+      if (dxi < 0 || dxi > 9) continue;
+#pragma unroll
+      for (int k = k_lo; k <= k_hi; k++) {
+        const double w = s_w[dxi * kSsimWRow + R - k + 7];  // F[R - k][dxi]
+        acc[j][k][0] = __builtin_fma(rv, w, acc[j][k][0]);
+        acc[j][k][1] = __builtin_fma(tv, w, acc[j][k][1]);
+        acc[j][k][2] = __builtin_fma(rr, w, acc[j][k][2]);
+        acc[j][k][3] = __builtin_fma(tt, w, acc[j][k][3]);
+        acc[j][k][4] = __builtin_fma(rt, w, acc[j][k][4]);
+      }
+    }
+    // One loaded sample at a time: the straight-line body is one basic block, so without these
+    // the instruction DAG would hoist every load and product above every FMA. Passing the
+    // accumulators through an ordered no-op pins this step's FMAs before the next step's loads
+    // (and the memory clobber keeps weights from being held in registers across steps).
+#pragma unroll
+    for (int j = 0; j < kSsimC; j++) {
+      const int dxi = c - j;
+      if (dxi < 0 || dxi > 9) continue;
+#pragma unroll
+      for (int k = k_lo; k <= k_hi; k++)
+        asm volatile("" : "+v"(acc[j][k][0]), "+v"(acc[j][k][1]), "+v"(acc[j][k][2]), "+v"(acc[j][k][3]),
+                     "+v"(acc[j][k][4])::"memory");
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int... R>
+__device__ __forceinline__ void ssim_rows(const float* s_t, const float* s_r, const double* s_w, int base,
+                                          double (&acc)[kSsimC][kSsimK][5], std::integer_sequence<int, R...>) {
+  (ssim_row<R>(s_t, s_r, s_w, base, acc), ...);
+}
+
 // ---------------------------------------------------------------------------------------------
 // SSIM of one component: xSSIM::xCalcRowSSIM / xStructSim<flt64, CalcL>::CalcPel
 // (libs/pmbb-ivqm/src/xSSIM.cpp:114-141, libs/pmbb-ivqm/src/xStructSim.cpp:15-60).
@@ -45,10 +109,9 @@ template <int CALC_L>
 __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
   __shared__ float s_t[kSsimLH * kSsimLW];
   __shared__ float s_r[kSsimLH * kSsimLW];
-  // Weights transposed and zero-padded: s_w[dx * kWRow + dy + 7] = F[dy][dx] for dy in [0, 9], so
-  // the weights F[r - k][dx] of the lane's pels k = 0..7 for loaded row r are 8 consecutive doubles.
-  constexpr int kWRow = 24;
-  __shared__ double s_w[10 * kWRow];
+  // Weights transposed: s_w[dx * kSsimWRow + dy + 7] = F[dy][dx] for dy in [0, 9], so the weights
+  // F[r - k][dx] of the lane's pels k for loaded row r are consecutive doubles.
+  __shared__ __attribute__((aligned(16))) double s_w[10 * kSsimWRow];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int tx = blockIdx.x;
@@ -57,9 +120,9 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
   const int y0 = 5 + ty * kSsimTH;
   const ViewBuf T(a.tst), R(a.ref);
 
-  for (int i = threadIdx.x; i < 10 * kWRow; i += 256) {
-    const int dx = i / kWRow;
-    const int dy = i - dx * kWRow - 7;
+  for (int i = threadIdx.x; i < 10 * kSsimWRow; i += 256) {
+    const int dx = i / kSsimWRow;
+    const int dy = i - dx * kSsimWRow - 7;
     s_w[i] = (dy >= 0 && dy <= 9) ? a.f[dy * 10 + dx] : 0.0;
   }
   // Stage the window rows [y0 - 5, y0 + TH + 4) x [x0 - 5, x0 + TW + 4), clamped inside the
@@ -74,55 +137,38 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
   }
   __syncthreads();
 
-  double acc[kSsimK][5];
+  double acc[kSsimC][kSsimK][5];
 #pragma unroll
-  for (int k = 0; k < kSsimK; k++)
+  for (int j = 0; j < kSsimC; j++)
 #pragma unroll
-    for (int m = 0; m < 5; m++) acc[k][m] = 0.0;
+    for (int k = 0; k < kSsimK; k++)
+#pragma unroll
+      for (int m = 0; m < 5; m++) acc[j][k][m] = 0.0;
 
-#pragma unroll
-  for (int r = 0; r < kSsimK + 9; r++) {
-    const int base = (kSsimK * wave + r) * kSsimLW + lane;
-#pragma unroll
-    for (int dx = 0; dx < 10; dx++) {
-      const double tv = (double)s_t[base + dx];
-      const double rv = (double)s_r[base + dx];
-      const double rr = rv * rv, tt = tv * tv, rt = rv * tv;
-      const double* wcol = s_w + dx * kWRow + r + 7;  // wcol[-k] = F[r - k][dx]
-#pragma unroll
-      for (int k = 0; k < kSsimK; k++) {
-        const int dy = r - k;  // window row of pel k that this loaded row is
-        if (dy < 0 || dy > 9) continue;
-        const double c = wcol[-k];
-        acc[k][0] = __builtin_fma(rv, c, acc[k][0]);
-        acc[k][1] = __builtin_fma(tv, c, acc[k][1]);
-        acc[k][2] = __builtin_fma(rr, c, acc[k][2]);
-        acc[k][3] = __builtin_fma(tt, c, acc[k][3]);
-        acc[k][4] = __builtin_fma(rt, c, acc[k][4]);
-      }
-      // One loaded value at a time: bounds the live products and weights.
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    // Re-read weights and samples for every loaded row instead of keeping them in registers.
-    asm volatile("" ::: "memory");
-  }
+  // Every loaded row r = 0 .. K + 8 as its own compile-time step (the loop unroller gives up on
+  // this many instructions): straight-line code, constant LDS offsets, no branches.
+  ssim_rows(s_t, s_r, s_w, (kSsimK * wave) * kSsimLW + kSsimC * lane, acc,
+            std::make_integer_sequence<int, kSsimK + 9>{});
 
-  const int x = x0 + lane;
 #pragma unroll
   for (int k = 0; k < kSsimK; k++) {
     const int y = y0 + kSsimK * wave + k;
     double v = 0.0;
-    if (x < a.w - 5 && y < a.h - 5) {
-      const double aR = acc[k][0], aT = acc[k][1];
-      const double vR = acc[k][2] - aR * aR;
-      const double vT = acc[k][3] - aT * aT;
-      const double cov = acc[k][4] - aR * aT;
-      const double cs = (2 * cov + a.c2) / (vR + vT + a.c2);
-      if (CALC_L) {
-        const double l = (2 * aR * aT + a.c1) / (aR * aR + aT * aT + a.c1);
-        v = l * cs;
-      } else {
-        v = cs;
+#pragma unroll
+    for (int j = 0; j < kSsimC; j++) {
+      const int x = x0 + kSsimC * lane + j;
+      if (x < a.w - 5 && y < a.h - 5) {
+        const double aR = acc[j][k][0], aT = acc[j][k][1];
+        const double vR = acc[j][k][2] - aR * aR;
+        const double vT = acc[j][k][3] - aT * aT;
+        const double cov = acc[j][k][4] - aR * aT;
+        const double cs = (2 * cov + a.c2) / (vR + vT + a.c2);
+        if (CALC_L) {
+          const double l = (2 * aR * aT + a.c1) / (aR * aR + aT * aT + a.c1);
+          v += l * cs;
+        } else {
+          v += cs;
+        }
       }
     }
     v = wave_sum_f64(v);
@@ -130,27 +176,30 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
   }
 }
 
-// rows[y] = sum over tile columns of the partials (rows outside the interior are zero).
-__global__ __launch_bounds__(256) void k_ssim_rows(const double* __restrict__ part, int32_t tiles_x, int32_t h,
-                                                   double* __restrict__ rows) {
+// Row sums of every evaluation of a frame: grid (ceil(max_h / 256), evaluations).
+__global__ __launch_bounds__(256) void k_ssim_rows(SsimRowsJobs j) {
+  const int e = blockIdx.y;
   const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = j.h[e];
   if (y >= h) return;
+  const double* part = j.part[e];
+  const int tiles_x = j.tiles_x[e];
   double s = 0.0;
   if (y >= 5 && y < h - 5)
     for (int t = 0; t < tiles_x; t++) s += part[(int64_t)t * h + y];
-  rows[y] = s;
+  j.rows[e][y] = s;
 }
 
 // xPixelOpsSTD::DownsampleHV (libs/pmbb-core/src/xPixelOpsSTD.cpp:50-67): dst = (a + b + c + d + 2) >> 2
-// over 2x2 blocks of the source view; dst is dw x dh uint16 (stride dw).
-__global__ __launch_bounds__(256) void k_downsample(PlaneView src, int32_t dw, int32_t dh, uint16_t* __restrict__ dst) {
+// over 2x2 blocks of the source view; dst is dw x dh uint16 (stride dw). grid.z = plane.
+__global__ __launch_bounds__(256) void k_downsample(DownsampleJobs j) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)dw * dh) return;
-  const int y = (int)(i / dw);
-  const int x = (int)(i - (int64_t)y * dw);
-  const ViewBuf S(src);
+  if (i >= (int64_t)j.dw * j.dh) return;
+  const int y = (int)(i / j.dw);
+  const int x = (int)(i - (int64_t)y * j.dw);
+  const ViewBuf S(j.src[blockIdx.z]);
   const uint32_t v = S.at(2 * x, 2 * y) + S.at(2 * x + 1, 2 * y) + S.at(2 * x, 2 * y + 1) + S.at(2 * x + 1, 2 * y + 1);
-  dst[i] = (uint16_t)((v + 2) >> 2);
+  j.dst[blockIdx.z][i] = (uint16_t)((v + 2) >> 2);
 }
 
 }  // namespace
@@ -170,18 +219,20 @@ hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream) {
   return hipSuccess;
 }
 
-hipError_t launch_ssim_rows(const SsimArgs& a, double* rows, hipStream_t stream) {
-  if (a.h <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ssim_rows, dim3((a.h + 255) / 256), dim3(256), 0, stream, a.partials, a.tiles_x, a.h, rows);
+hipError_t launch_ssim_rows(const SsimRowsJobs& j, hipStream_t stream) {
+  if (j.n <= 0 || j.max_h <= 0) return hipSuccess;
+  if (j.n > kMaxSsimEvals) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ssim_rows, dim3((unsigned)((j.max_h + 255) / 256), (unsigned)j.n), dim3(256), 0, stream, j);
   return hipGetLastError();
 }
 
 int32_t ssim_tiles_x(int32_t w) { return w > 10 ? (w - 10 + kSsimTW - 1) / kSsimTW : 0; }
 
-hipError_t launch_downsample(const PlaneView& src, int32_t dw, int32_t dh, uint16_t* dst, hipStream_t stream) {
-  const int64_t n = (int64_t)dw * dh;
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_downsample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, dw, dh, dst);
+hipError_t launch_downsample(const DownsampleJobs& j, hipStream_t stream) {
+  const int64_t n = (int64_t)j.dw * j.dh;
+  if (n <= 0 || j.n <= 0) return hipSuccess;
+  if (j.n > 6) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_downsample, dim3((unsigned)((n + 255) / 256), 1, (unsigned)j.n), dim3(256), 0, stream, j);
   return hipGetLastError();
 }
 

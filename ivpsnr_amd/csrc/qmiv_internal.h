@@ -82,6 +82,10 @@ hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, ui
 void cps_fast_tile(int32_t* tile_w, int32_t* tile_h);
 // Resident workgroups of the fast CPS kernel per CU (from the HIP occupancy calculator).
 int cps_fast_blocks_per_cu(int bytes_per_sample);
+// The 4:2:0 fast CPS kernel (kernels_cps420.hip): chroma cells factored out of the key.
+hipError_t launch_cps_fast420(const CpsArgs& a, hipStream_t stream);
+void cps_fast420_tile(int32_t* tile_w, int32_t* tile_h);
+int cps_fast420_blocks_per_cu(int bytes_per_sample);
 
 // Nearest-neighbour 4:2:0 -> 4:4:4 upsampling (a copy for 4:4:4 sources) of a packed frame into a
 // planar 4:4:4 frame of the same sample type.
@@ -91,10 +95,31 @@ hipError_t launch_narrow(const uint16_t* src, int64_t n, uint8_t* dst, hipStream
 
 // Windowed moments of one evaluation into a.partials ([tiles_x][h] row sums per tile column).
 hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream);
-// rows[y] = sum over tile columns of a.partials, in tile order (zero outside the interior).
-hipError_t launch_ssim_rows(const SsimArgs& a, double* rows, hipStream_t stream);
 int32_t ssim_tiles_x(int32_t w);
-hipError_t launch_downsample(const PlaneView& src, int32_t dw, int32_t dh, uint16_t* dst, hipStream_t stream);
+
+// Per-row reduction of every SSIM evaluation of one frame in one launch:
+// rows_j[y] = sum over tile columns t of part_j[t * h_j + y], in tile order (zero outside the
+// interior rows [5, h_j - 5)).
+constexpr int kMaxSsimEvals = 32;
+struct SsimRowsJobs {
+  int32_t n;
+  int32_t max_h;
+  int32_t tiles_x[kMaxSsimEvals];
+  int32_t h[kMaxSsimEvals];
+  const double* part[kMaxSsimEvals];
+  double* rows[kMaxSsimEvals];
+};
+hipError_t launch_ssim_rows(const SsimRowsJobs& j, hipStream_t stream);
+
+// One MS-SSIM pyramid level for up to 6 planes in one launch:
+// dst_i[y][x] = (s(2x, 2y) + s(2x + 1, 2y) + s(2x, 2y + 1) + s(2x + 1, 2y + 1) + 2) >> 2.
+struct DownsampleJobs {
+  int32_t n;
+  int32_t dw, dh;
+  PlaneView src[6];
+  uint16_t* dst[6];
+};
+hipError_t launch_downsample(const DownsampleJobs& j, hipStream_t stream);
 
 // Tile shape of the fast CPS kernel (device code only; the host asks cps_fast_tile()).
 constexpr int kCpsTileW = 64;   // 64 lanes x 1 column
