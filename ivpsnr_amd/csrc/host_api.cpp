@@ -51,6 +51,14 @@ struct qmiv_pic {
 struct qmiv_ctx {
   int32_t device = 0;
   hipStream_t stream = nullptr;
+  // Results copies of a frame set run here, each after its frame's kernels (an event per frame),
+  // so that the next frame's kernels do not wait for the copy engine.
+  hipStream_t copy_stream = nullptr;
+  std::vector<hipEvent_t> copy_ev;
+  // A frame set alternates its frames over `stream` and `aux_stream` (independent scratch), so one
+  // frame's short kernels overlap the other's search; fork/join events order them with the rest.
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   // Device scratch (grown on demand; one frame evaluation at a time per context).
   void* d_scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -104,6 +112,8 @@ qmiv_status ensure_scratch(qmiv_ctx* ctx, size_t bytes) {
   if (ctx->scratch_bytes >= bytes) return QMIV_OK;
   if (ctx->d_scratch) {
     QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    QMIV_HIP(hipStreamSynchronize(ctx->aux_stream));
+    QMIV_HIP(hipStreamSynchronize(ctx->copy_stream));
     QMIV_HIP(hipFree(ctx->d_scratch));
     ctx->d_scratch = nullptr;
     ctx->scratch_bytes = 0;
@@ -118,6 +128,8 @@ qmiv_status ensure_results(qmiv_ctx* ctx, size_t bytes) {
   if (ctx->results_bytes >= bytes) return QMIV_OK;
   if (ctx->h_results) {
     QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    QMIV_HIP(hipStreamSynchronize(ctx->aux_stream));
+    QMIV_HIP(hipStreamSynchronize(ctx->copy_stream));
     QMIV_HIP(hipHostFree(ctx->h_results));
     ctx->h_results = nullptr;
     ctx->results_bytes = 0;
@@ -133,6 +145,8 @@ qmiv_status ensure_scp(qmiv_ctx* ctx, size_t bytes) {
   if (ctx->scp_bytes >= bytes) return QMIV_OK;
   if (ctx->d_scp) {
     QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    QMIV_HIP(hipStreamSynchronize(ctx->aux_stream));
+    QMIV_HIP(hipStreamSynchronize(ctx->copy_stream));
     QMIV_HIP(hipFree(ctx->d_scp));
     ctx->d_scp = nullptr;
     ctx->scp_bytes = 0;
@@ -459,7 +473,7 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
 // `scp` (2 x [3][h][w] uint16) and the results block copied into pinned `hbase`.
 qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params& prm,
                           const FrameWork& fw, const FramePlan& P, char* dbase, char* hbase, uint16_t* scp,
-                          hipStream_t st) {
+                          hipStream_t st, hipStream_t copy_st = nullptr, hipEvent_t copy_ev = nullptr) {
   const Geom& g = p0->g;
   const Layout& L = P.L;
   FrameSummary* sum = (FrameSummary*)(dbase + L.off_summary);
@@ -514,12 +528,20 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     }
   }
   if (!L.ssim.empty()) QMIV_TRY(enqueue_ssim(ctx, p0, p1, P, dbase, scp, st));
-  QMIV_HIP(hipMemcpyAsync(hbase, dbase, L.results_bytes, hipMemcpyDeviceToHost, st));
+  if (copy_st) {
+    QMIV_HIP(hipEventRecord(copy_ev, st));
+    QMIV_HIP(hipStreamWaitEvent(copy_st, copy_ev, 0));
+    QMIV_HIP(hipMemcpyAsync(hbase, dbase, L.results_bytes, hipMemcpyDeviceToHost, copy_st));
+  } else {
+    QMIV_HIP(hipMemcpyAsync(hbase, dbase, L.results_bytes, hipMemcpyDeviceToHost, st));
+  }
   return QMIV_OK;
 }
 
 qmiv_status wait_stream(qmiv_ctx* ctx) {
   QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->aux_stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->copy_stream));
   harvest_events(ctx);
   return QMIV_OK;
 }
@@ -863,7 +885,16 @@ qmiv_status qmiv_create(int32_t device, qmiv_ctx** out) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     ctx->num_cus = cus;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
   if (e != hipSuccess) {
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return fail(QMIV_ERR_HIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
   }
@@ -875,6 +906,11 @@ void qmiv_destroy(qmiv_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+  if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
+  for (hipEvent_t ev : ctx->copy_ev) (void)hipEventDestroy(ev);
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
   if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
   if (ctx->d_scp) (void)hipFree(ctx->d_scp);
   if (ctx->h_results) (void)hipHostFree(ctx->h_results);
@@ -882,6 +918,8 @@ void qmiv_destroy(qmiv_ctx* ctx) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
   }
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+  if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -994,17 +1032,46 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
   }
   QMIV_TRY(ensure_scratch(ctx, dtot));
   QMIV_TRY(ensure_results(ctx, htot));
-  // Frames run in stream order, so one set of shift-compensated pictures serves them all.
+  // Frames alternate over two streams; each stream's frames run in order, so one set of
+  // shift-compensated pictures per stream serves them.
+  const int32_t nstreams = n > 1 ? 2 : 1;
   uint16_t* scp = nullptr;
+  size_t scp_stride = 0;
   if (fw.need_scp) {
     size_t most = 0;
     for (int32_t i = 0; i < n; i++) most = std::max(most, (size_t)p0[i]->g.w * p0[i]->g.h);
-    QMIV_TRY(ensure_scp(ctx, 6 * most * sizeof(uint16_t)));
+    scp_stride = align_up(6 * most, 128);
+    QMIV_TRY(ensure_scp(ctx, scp_stride * nstreams * sizeof(uint16_t)));
     scp = (uint16_t*)ctx->d_scp;
   }
-  for (int32_t i = 0; i < n; i++)
-    QMIV_TRY(enqueue_frame(ctx, p0[i], p1[i], prm, fw, plans[(size_t)i], (char*)ctx->d_scratch + doff[(size_t)i],
-                           (char*)ctx->h_results + hoff[(size_t)i], scp, ctx->stream));
+  while ((int32_t)ctx->copy_ev.size() < n) {
+    hipEvent_t e;
+    QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->copy_ev.push_back(e);
+  }
+  const hipStream_t streams[2] = {ctx->stream, ctx->aux_stream};
+  if (nstreams > 1) {  // the aux stream starts after everything already on the context stream
+    QMIV_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+    QMIV_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_ev, 0));
+  }
+  for (int32_t i = 0; i < n; i++) {
+    const int32_t k = i % nstreams;
+    const qmiv_status st = enqueue_frame(ctx, p0[i], p1[i], prm, fw, plans[(size_t)i],
+                                         (char*)ctx->d_scratch + doff[(size_t)i], (char*)ctx->h_results + hoff[(size_t)i],
+                                         scp ? scp + scp_stride * (size_t)k : nullptr, streams[k], ctx->copy_stream,
+                                         ctx->copy_ev[(size_t)i]);
+    if (st != QMIV_OK) {
+      // Drain what was enqueued: scratch and result buffers must be idle before any later reuse.
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipStreamSynchronize(ctx->aux_stream);
+      (void)hipStreamSynchronize(ctx->copy_stream);
+      return st;
+    }
+  }
+  if (nstreams > 1) {  // later work on the context stream follows the whole set
+    QMIV_HIP(hipEventRecord(ctx->join_ev, ctx->aux_stream));
+    QMIV_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
+  }
   QMIV_TRY(wait_stream(ctx));
   qmiv_status first_err = QMIV_OK;
   for (int32_t i = 0; i < n; i++) {

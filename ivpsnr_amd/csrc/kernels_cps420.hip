@@ -29,7 +29,10 @@ namespace qmiv {
 namespace {
 
 constexpr int kTW = 128;          // tile columns: 64 lanes x 2
-constexpr int kKB = 2;            // 2 x 2 blocks per lane, stacked vertically
+#ifndef QMIV_CPS420_KB
+#define QMIV_CPS420_KB 2
+#endif
+constexpr int kKB = QMIV_CPS420_KB;  // 2 x 2 blocks per lane, stacked vertically
 constexpr int kTH = 4 * 2 * kKB;  // tile rows: 4 waves x 2 kKB
 constexpr int kLW = kTW + 8;      // luma record columns [x0 - 4, x0 + kTW + 4)
 constexpr int kLH = kTH + 4;      // luma record rows [y0 - 2, y0 + kTH + 2)
@@ -39,64 +42,77 @@ constexpr int kCH = kLH / 2;
 // Records of the searched picture Yp for the tile at (x0, y0). Coordinates are clamped into the
 // picture; with even picture sizes and even tile origins, the two luma columns (rows) of an LDS
 // cell always clamp into one chroma sample, so cells stay uniform at the edges too.
+// Each thread and step builds four luma records and two chroma cells of one LDS row: inside the
+// picture from one aligned 8-byte (u16) or 4-byte (u8) luma load and two chroma samples per
+// plane, at the edges from clamped single-sample loads; all steps' loads are issued together.
+constexpr int kQW = kLW / 4;
+constexpr int kFillItems = kLH * kQW;
+constexpr int kFillSteps = (kFillItems + 255) / 256;
+
 template <typename T>
 __device__ __forceinline__ void fill420(const Buf<T>& Yp, const Geom& g, int x0, int y0, int32_t w0, int32_t w1,
                                         int32_t w2, uint2* s_luma, uint32_t* s_cell) {
   const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
   const bool interior = x0 >= 4 && x0 + kTW + 4 <= g.w && y0 >= 2 && y0 + kTH + 2 <= g.h && (g.w & 3) == 0;
-  if (interior) {
-    // Four luma records per thread and step: one aligned 8-byte (u16) or 4-byte (u8) luma load
-    // and two chroma samples per plane, each shared by two records.
-    constexpr int QW = kLW / 4;
-    for (int i = threadIdx.x; i < kLH * QW; i += 256) {
-      const int ly = i / QW;
-      const int lx = (i - ly * QW) * 4;
+  uint32_t yv[kFillSteps][4], cu[kFillSteps][2], cv[kFillSteps][2];
+#pragma unroll
+  for (int k = 0; k < kFillSteps; k++) {
+    const int i = threadIdx.x + 256 * k;
+    if (kFillItems % 256 != 0 && k == kFillSteps - 1 && i >= kFillItems) break;
+    const int ly = i / kQW;
+    const int lx = (i - ly * kQW) * 4;
+    if (interior) {
       const int gy = y0 - 2 + ly;
       const int gx = x0 - 4 + lx;
-      uint32_t yv[4], u0, u1, v0, v1;
       const uint32_t lo = (uint32_t)(gy * g.w + gx);
       const uint32_t co = (uint32_t)((gy >> 1) * g.cw + (gx >> 1));
       if constexpr (sizeof(T) == 2) {
         const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(Yp.r, lo * 2u, 0, 0));
-        yv[0] = w.x & 0xffffu; yv[1] = w.x >> 16; yv[2] = w.y & 0xffffu; yv[3] = w.y >> 16;
+        yv[k][0] = w.x & 0xffffu; yv[k][1] = w.x >> 16; yv[k][2] = w.y & 0xffffu; yv[k][3] = w.y >> 16;
         const uint32_t wu = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, (cb_off + co) * 2u, 0, 0);
         const uint32_t wv = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, (cr_off + co) * 2u, 0, 0);
-        u0 = wu & 0xffffu; u1 = wu >> 16; v0 = wv & 0xffffu; v1 = wv >> 16;
+        cu[k][0] = wu & 0xffffu; cu[k][1] = wu >> 16; cv[k][0] = wv & 0xffffu; cv[k][1] = wv >> 16;
       } else {
         const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, lo, 0, 0);
-        yv[0] = w & 0xffu; yv[1] = (w >> 8) & 0xffu; yv[2] = (w >> 16) & 0xffu; yv[3] = w >> 24;
+        yv[k][0] = w & 0xffu; yv[k][1] = (w >> 8) & 0xffu; yv[k][2] = (w >> 16) & 0xffu; yv[k][3] = w >> 24;
         const uint32_t wu = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cb_off + co, 0, 0);
         const uint32_t wv = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cr_off + co, 0, 0);
-        u0 = wu & 0xffu; u1 = wu >> 8; v0 = wv & 0xffu; v1 = wv >> 8;
+        cu[k][0] = wu & 0xffu; cu[k][1] = wu >> 8; cv[k][0] = wv & 0xffu; cv[k][1] = wv >> 8;
       }
-      const int32_t sc0 = w1 * (int32_t)(u0 * u0) + w2 * (int32_t)(v0 * v0);
-      const int32_t sc1 = w1 * (int32_t)(u1 * u1) + w2 * (int32_t)(v1 * v1);
-      const int32_t h = 5 * ly + lx;
-      uint4 r01, r23;
-      r01.x = yv[0]; r01.y = (uint32_t)(32 * (w0 * (int32_t)(yv[0] * yv[0]) + sc0) + h);
-      r01.z = yv[1]; r01.w = (uint32_t)(32 * (w0 * (int32_t)(yv[1] * yv[1]) + sc0) + h + 1);
-      r23.x = yv[2]; r23.y = (uint32_t)(32 * (w0 * (int32_t)(yv[2] * yv[2]) + sc1) + h + 2);
-      r23.z = yv[3]; r23.w = (uint32_t)(32 * (w0 * (int32_t)(yv[3] * yv[3]) + sc1) + h + 3);
-      uint4* dst = reinterpret_cast<uint4*>(s_luma + ly * kLW + lx);
-      dst[0] = r01;
-      dst[1] = r23;
-      if ((ly & 1) == 0)
-        *reinterpret_cast<uint2*>(s_cell + (ly >> 1) * kCW + (lx >> 1)) = make_uint2(u0 | (v0 << 16), u1 | (v1 << 16));
-    }
-  } else {
-    for (int i = threadIdx.x; i < kLH * kLW; i += 256) {
-      const int ly = i / kLW;
-      const int lx = i - ly * kLW;
+    } else {
       const int gy = clampi(y0 - 2 + ly, 0, g.h - 1);
-      const int gx = clampi(x0 - 4 + lx, 0, g.w - 1);
-      const uint32_t yv = Yp.ld((uint32_t)(gy * g.w + gx));
-      const uint32_t co = (uint32_t)((gy >> 1) * g.cw + (gx >> 1));
-      const uint32_t uv = Yp.ld(cb_off + co);
-      const uint32_t vv = Yp.ld(cr_off + co);
-      const int32_t S = w0 * (int32_t)(yv * yv) + w1 * (int32_t)(uv * uv) + w2 * (int32_t)(vv * vv);
-      s_luma[i] = make_uint2(yv, (uint32_t)(S * 32 + 5 * ly + lx));
-      if (((ly | lx) & 1) == 0) s_cell[(ly >> 1) * kCW + (lx >> 1)] = uv | (vv << 16);
+      const uint32_t rowo = (uint32_t)(gy * g.w);
+      const uint32_t crow = (uint32_t)((gy >> 1) * g.cw);
+#pragma unroll
+      for (int m = 0; m < 4; m++) yv[k][m] = Yp.ld(rowo + (uint32_t)clampi(x0 - 4 + lx + m, 0, g.w - 1));
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const uint32_t co = crow + (uint32_t)(clampi(x0 - 4 + lx + 2 * c, 0, g.w - 1) >> 1);
+        cu[k][c] = Yp.ld(cb_off + co);
+        cv[k][c] = Yp.ld(cr_off + co);
+      }
     }
+  }
+#pragma unroll
+  for (int k = 0; k < kFillSteps; k++) {
+    const int i = threadIdx.x + 256 * k;
+    if (kFillItems % 256 != 0 && k == kFillSteps - 1 && i >= kFillItems) break;
+    const int ly = i / kQW;
+    const int lx = (i - ly * kQW) * 4;
+    const int32_t sc0 = w1 * (int32_t)(cu[k][0] * cu[k][0]) + w2 * (int32_t)(cv[k][0] * cv[k][0]);
+    const int32_t sc1 = w1 * (int32_t)(cu[k][1] * cu[k][1]) + w2 * (int32_t)(cv[k][1] * cv[k][1]);
+    const int32_t h = 5 * ly + lx;
+    uint4 r01, r23;
+    r01.x = yv[k][0]; r01.y = (uint32_t)(32 * (w0 * (int32_t)(yv[k][0] * yv[k][0]) + sc0) + h);
+    r01.z = yv[k][1]; r01.w = (uint32_t)(32 * (w0 * (int32_t)(yv[k][1] * yv[k][1]) + sc0) + h + 1);
+    r23.x = yv[k][2]; r23.y = (uint32_t)(32 * (w0 * (int32_t)(yv[k][2] * yv[k][2]) + sc1) + h + 2);
+    r23.z = yv[k][3]; r23.w = (uint32_t)(32 * (w0 * (int32_t)(yv[k][3] * yv[k][3]) + sc1) + h + 3);
+    uint4* dst = reinterpret_cast<uint4*>(s_luma + ly * kLW + lx);
+    dst[0] = r01;
+    dst[1] = r23;
+    if ((ly & 1) == 0)
+      *reinterpret_cast<uint2*>(s_cell + (ly >> 1) * kCW + (lx >> 1)) =
+          make_uint2(cu[k][0] | (cv[k][0] << 16), cu[k][1] | (cv[k][1] << 16));
   }
 }
 

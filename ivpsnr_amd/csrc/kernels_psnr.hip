@@ -52,32 +52,46 @@ __global__ __launch_bounds__(256) void k_stats(const T* __restrict__ p0, const T
   uint32_t m0 = 0, m1 = 0;
   constexpr int kVec = 16 / sizeof(T);  // samples per 16-byte load
   int32_t vec_end = 0;
+  // Adds one 16-byte group of both pictures to the lane's sums.
+  auto group = [&](const uint4& va, const uint4& vb) {
+    const uint32_t wa[4] = {va.x, va.y, va.z, va.w};
+    const uint32_t wb[4] = {vb.x, vb.y, vb.z, vb.w};
+    int32_t sd_l = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+#pragma unroll
+      for (int s = 0; s < (int)(4 / sizeof(T)); s++) {
+        const uint32_t sh = s * 8 * sizeof(T);
+        const uint32_t mask = sizeof(T) == 1 ? 0xffu : 0xffffu;
+        const uint32_t ea = (wa[k] >> sh) & mask;
+        const uint32_t eb = (wb[k] >> sh) & mask;
+        const int32_t d = (int32_t)eb - (int32_t)ea;
+        sd_l += d;
+        const uint32_t sq = (uint32_t)d * (uint32_t)d;
+        ssd += sq;
+        m0 = max(m0, ea);
+        m1 = max(m1, eb);
+      }
+    }
+    sd += sd_l;
+  };
   if ((((uintptr_t)a | (uintptr_t)b) & 15) == 0) {
     vec_end = (width / kVec) * kVec;
-    for (int32_t x = lane * kVec; x < vec_end; x += 64 * kVec) {
-      const uint4 va = *reinterpret_cast<const uint4*>(a + x);
-      const uint4 vb = *reinterpret_cast<const uint4*>(b + x);
-      const uint32_t wa[4] = {va.x, va.y, va.z, va.w};
-      const uint32_t wb[4] = {vb.x, vb.y, vb.z, vb.w};
-      int32_t sd_l = 0;
+    // Four groups per picture in flight per lane before any is used (HBM needs the bytes in flight).
+    constexpr int U = 4;
+    int32_t x = lane * kVec;
+    for (; x + (U - 1) * 64 * kVec < vec_end; x += U * 64 * kVec) {
+      uint4 va[U], vb[U];
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-#pragma unroll
-        for (int s = 0; s < (int)(4 / sizeof(T)); s++) {
-          const uint32_t sh = s * 8 * sizeof(T);
-          const uint32_t mask = sizeof(T) == 1 ? 0xffu : 0xffffu;
-          const uint32_t ea = (wa[k] >> sh) & mask;
-          const uint32_t eb = (wb[k] >> sh) & mask;
-          const int32_t d = (int32_t)eb - (int32_t)ea;
-          sd_l += d;
-          const uint32_t sq = (uint32_t)d * (uint32_t)d;
-          ssd += sq;
-          m0 = max(m0, ea);
-          m1 = max(m1, eb);
-        }
+      for (int u = 0; u < U; u++) {
+        va[u] = *reinterpret_cast<const uint4*>(a + x + u * 64 * kVec);
+        vb[u] = *reinterpret_cast<const uint4*>(b + x + u * 64 * kVec);
       }
-      sd += sd_l;
+#pragma unroll
+      for (int u = 0; u < U; u++) group(va[u], vb[u]);
     }
+    for (; x < vec_end; x += 64 * kVec)
+      group(*reinterpret_cast<const uint4*>(a + x), *reinterpret_cast<const uint4*>(b + x));
   }
   for (int32_t x = vec_end + lane; x < width; x += 64) {
     const uint32_t ea = ldpel(a, x), eb = ldpel(b, x);
@@ -106,7 +120,7 @@ __global__ __launch_bounds__(256) void k_stats(const T* __restrict__ p0, const T
 // (xDistortionSTD.cpp:18-28), is divided by the area in double precision and rounded half-even
 // (the SSE conversion of xCommonDefCORE.h:129-131), then clipped to +-maxdiff.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_summary(Geom g, const int64_t* __restrict__ row_sd,
+__global__ __launch_bounds__(1024) void k_summary(Geom g, const int64_t* __restrict__ row_sd,
                                                  const uint64_t* __restrict__ row_ssd,
                                                  const uint32_t* __restrict__ row_max, int32_t md0, int32_t md1,
                                                  int32_t md2, int32_t md3, int32_t compute_gcd,
@@ -536,7 +550,7 @@ hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* 
 
 hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* row_ssd, const uint32_t* row_max,
                           const int32_t maxdiff[4], int32_t compute_gcd, FrameSummary* sum, hipStream_t stream) {
-  hipLaunchKernelGGL(k_summary, dim3(1), dim3(256), 0, stream, g, row_sd, row_ssd, row_max, maxdiff[0], maxdiff[1],
+  hipLaunchKernelGGL(k_summary, dim3(1), dim3(1024), 0, stream, g, row_sd, row_ssd, row_max, maxdiff[0], maxdiff[1],
                      maxdiff[2], maxdiff[3], compute_gcd, sum);
   return hipGetLastError();
 }
