@@ -281,6 +281,31 @@ qmiv_status qmiv_batch_run(qmiv_batch* batch, qmiv_frame_result* out);
 qmiv_status qmiv_batch_get_info(const qmiv_batch* batch, qmiv_batch_info* info);
 void qmiv_batch_destroy(qmiv_batch* batch);
 
+/* ---------------------------------------------------------------------------------------------
+   Differentiable SSIM family (no counterpart in the reference, whose SSIM family is integer and
+   forward-only, libs/pmbb-ivqm/src/xSSIM.cpp; for training and tuning callers that need
+   gradients). Pictures are `planes` planar fp32 planes of width x height in device memory, values
+   x in [0, 1] standing for samples x * max_value; the forward is the reference's SSIM map on those
+   real-valued samples (10 x 10 window, C1 = (0.01 max)^2, C2 = (0.03 max)^2), so pictures holding
+   k / max_value give the reference's values. Gradients are with respect to x, written to the
+   non-null grad buffers (same shape as the inputs). Synchronous.
+   --------------------------------------------------------------------------------------------- */
+/* SSIM per plane: value[p] and d value[p] / d x of both pictures. */
+qmiv_status qmiv_ssim_backward(qmiv_ctx* ctx, int32_t width, int32_t height, int32_t planes, const float* tst,
+                               const float* ref, double max_value, int32_t calc_l, double* value, float* grad_tst,
+                               float* grad_ref);
+/* MS-SSIM per plane with a float 2 x 2 mean pyramid (the reference's integer pyramid rounds, so
+   this equals it at the first scale only); scales (nullable) receives [5][planes]. */
+qmiv_status qmiv_msssim_backward(qmiv_ctx* ctx, int32_t width, int32_t height, int32_t planes, const float* tst,
+                                 const float* ref, double max_value, double* value, double* scales, float* grad_tst,
+                                 float* grad_ref);
+/* IV-SSIM of two 3-plane pictures (P0 = tst, P1 = ref): the shift search and GCD run on the
+   pictures quantised to max_value = 2^bd - 1; the shift-compensated pictures gather the real
+   samples. Gradient: a subgradient that holds the shifts, the GCD and the minimum's direction. */
+qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t width, int32_t height, const float* tst, const float* ref,
+                                 double max_value, const qmiv_params* prm, double* ivssim, double* r2t, double* t2r,
+                                 float* grad_tst, float* grad_ref);
+
 #ifdef __cplusplus
 }
 #endif

@@ -120,6 +120,12 @@ _SIGNATURES = {
     "qmiv_batch_run": (C.c_int, [C.c_void_p, C.POINTER(FrameResult)]),
     "qmiv_batch_get_info": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
     "qmiv_batch_destroy": (None, [C.c_void_p]),
+    "qmiv_ssim_backward": (C.c_int, [_ctx_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_double,
+                                     C.c_int32, _f64p, C.c_void_p, C.c_void_p]),
+    "qmiv_msssim_backward": (C.c_int, [_ctx_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_double,
+                                       _f64p, _f64p, C.c_void_p, C.c_void_p]),
+    "qmiv_ivssim_backward": (C.c_int, [_ctx_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_double,
+                                       C.POINTER(Params), _f64p, _f64p, _f64p, C.c_void_p, C.c_void_p]),
 }
 EXPORTS.extend(_SIGNATURES)
 

@@ -59,6 +59,10 @@ struct qmiv_ctx {
   // frame's short kernels overlap the other's search; fork/join events order them with the rest.
   hipStream_t aux_stream = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // Differentiable SSIM family: float workspace and the quantised pictures of the shift search.
+  void* d_fwork = nullptr;
+  size_t fwork_bytes = 0;
+  qmiv_pic* fq[2] = {nullptr, nullptr};
   // Device scratch (grown on demand; one frame evaluation at a time per context).
   void* d_scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -337,6 +341,8 @@ struct FrameWork {
   bool need_cps = false;
   bool need_scp = false;
   uint32_t ssim_metrics = 0;     // QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM
+  int16_t* best0 = nullptr;      // optional per-pel winner indices of the two directions
+  int16_t* best1 = nullptr;
 };
 
 // Everything decided on the host before a frame's kernels are enqueued.
@@ -515,6 +521,8 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     a.partials = partials;
     a.scp0 = scp0;
     a.scp1 = scp1;
+    a.best0 = fw.best0;
+    a.best1 = fw.best1;
     a.maxv = (1 << g.bd) - 1;
     {
       KernelTimer t(ctx, QMIV_K_CPS, st);
@@ -844,6 +852,120 @@ qmiv_status batch_time(qmiv_batch* b, int32_t reps, double& ms) {
 
 }  // namespace
 
+// =============================================================================================
+// Differentiable SSIM family on float planes.
+// =============================================================================================
+namespace {
+
+qmiv_status ensure_fwork(qmiv_ctx* ctx, size_t bytes) {
+  if (ctx->fwork_bytes >= bytes) return QMIV_OK;
+  if (ctx->d_fwork) {
+    QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    QMIV_HIP(hipFree(ctx->d_fwork));
+    ctx->d_fwork = nullptr;
+    ctx->fwork_bytes = 0;
+  }
+  const size_t sz = align_up(bytes, 1 << 20);
+  if (hipMalloc(&ctx->d_fwork, sz) != hipSuccess) return fail(QMIV_ERR_OUT_OF_MEMORY, "float workspace allocation failed");
+  ctx->fwork_bytes = sz;
+  return QMIV_OK;
+}
+
+// Bump allocation inside the float workspace: sizes first (base = null), then pointers.
+struct Bump {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t n) {
+    T* p = base ? (T*)(base + off) : nullptr;
+    off = align_up(off + n * sizeof(T), 256);
+    return p;
+  }
+};
+
+void ssimf_consts(double max_value, double& c1, double& c2, double f[100]) {
+  const double k1 = 0.01 * max_value, k2 = 0.03 * max_value;
+  c1 = k1 * k1;
+  c2 = k2 * k2;
+  fin::ssim_window(f);
+}
+
+// One float SSIM evaluation on the context stream: row sums into rows_dev (h doubles), and when
+// maps is not null the gradient maps; then, when g.grad_t or g.grad_r is set, the gradient.
+qmiv_status enqueue_ssimf(qmiv_ctx* ctx, const void* tst, int32_t tst_f64, double scale_t, const void* ref,
+                          int32_t ref_f64, double scale_r, int32_t w, int32_t h, int32_t calc_l, double max_value,
+                          double* partials, double* rows_dev, double* maps, SsimGArgs* g) {
+  hipStream_t st = ctx->stream;
+  SsimFArgs a{};
+  a.tst = tst;
+  a.ref = ref;
+  a.tst_f64 = tst_f64;
+  a.ref_f64 = ref_f64;
+  a.scale_t = scale_t;
+  a.scale_r = scale_r;
+  a.w = w;
+  a.h = h;
+  a.tiles_x = ssimf_tiles_x(w);
+  ssimf_consts(max_value, a.c1, a.c2, a.f);
+  a.partials = partials;
+  a.maps = maps;
+  QMIV_HIP(launch_ssimf(a, calc_l, maps ? 1 : 0, st));
+  SsimRowsJobs j{};
+  j.n = 1;
+  j.max_h = h;
+  j.tiles_x[0] = a.tiles_x;
+  j.h[0] = h;
+  j.part[0] = partials;
+  j.rows[0] = rows_dev;
+  QMIV_HIP(launch_ssim_rows(j, st));
+  if (g && maps && (g->grad_t || g->grad_r)) {
+    g->tst = tst;
+    g->ref = ref;
+    g->tst_f64 = tst_f64;
+    g->ref_f64 = ref_f64;
+    g->scale_t = scale_t;
+    g->scale_r = scale_r;
+    g->w = w;
+    g->h = h;
+    g->maps = maps;
+    std::memcpy(g->f, a.f, sizeof(a.f));
+    QMIV_HIP(launch_ssimf_grad(*g, st));
+  }
+  return QMIV_OK;
+}
+
+qmiv_status check_fargs(qmiv_ctx* ctx, int32_t w, int32_t h, int32_t planes, const float* tst, const float* ref,
+                        double max_value) {
+  if (!ctx || !tst || !ref) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (w <= 0 || h <= 0 || planes < 1 || planes > 4) return fail(QMIV_ERR_INVALID_ARG, "bad picture shape");
+  if ((int64_t)w * h * planes >= ((int64_t)1 << 31)) return fail(QMIV_ERR_UNSUPPORTED, "picture too large");
+  if (!(max_value > 0)) return fail(QMIV_ERR_INVALID_ARG, "max_value must be positive");
+  return QMIV_OK;
+}
+
+// Quantised 4:4:4 picture of the context for the shift search (recreated on a geometry change).
+qmiv_status fq_picture(qmiv_ctx* ctx, int k, int32_t w, int32_t h, int32_t bd, qmiv_pic** out) {
+  qmiv_pic*& q = ctx->fq[k];
+  Geom g;
+  QMIV_TRY(make_geom(w, h, bd, 444, g));
+  if (q && !same_geom(q->g, g)) {
+    QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(q->dev);
+    delete q;
+    q = nullptr;
+  }
+  if (!q) {
+    void* dev = nullptr;
+    if (hipMalloc(&dev, align_up(g.frame_bytes, 256)) != hipSuccess)
+      return fail(QMIV_ERR_OUT_OF_MEMORY, "quantised picture allocation failed");
+    q = new qmiv_pic{ctx, g, dev, true};
+  }
+  *out = q;
+  return QMIV_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* qmiv_last_error(void) { return g_last_error.c_str(); }
@@ -913,6 +1035,12 @@ void qmiv_destroy(qmiv_ctx* ctx) {
   if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
   if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
   if (ctx->d_scp) (void)hipFree(ctx->d_scp);
+  if (ctx->d_fwork) (void)hipFree(ctx->d_fwork);
+  for (qmiv_pic* q : ctx->fq)
+    if (q) {
+      (void)hipFree(q->dev);
+      delete q;
+    }
   if (ctx->h_results) (void)hipHostFree(ctx->h_results);
   for (auto& pr : ctx->ev_pool) {
     (void)hipEventDestroy(pr.first);
@@ -1424,6 +1552,234 @@ void qmiv_batch_destroy(qmiv_batch* b) {
   if (b->d_scp) (void)hipFree(b->d_scp);
   if (b->h_results) (void)hipHostFree(b->h_results);
   delete b;
+}
+
+
+qmiv_status qmiv_ssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, int32_t planes, const float* tst, const float* ref,
+                               double max_value, int32_t calc_l, double* value, float* grad_tst, float* grad_ref) {
+  QMIV_TRY(check_fargs(ctx, w, h, planes, tst, ref, max_value));
+  if (!value) return fail(QMIV_ERR_INVALID_ARG, "null value");
+  QMIV_HIP(hipSetDevice(ctx->device));
+  const bool grad = grad_tst || grad_ref;
+  const size_t plane = (size_t)w * h;
+  Bump b{nullptr};
+  auto layout = [&](Bump& bb, double*& partials, double*& rows, double*& maps) {
+    partials = bb.take<double>(std::max<size_t>((size_t)ssimf_tiles_x(w) * h, 1));
+    rows = bb.take<double>((size_t)planes * h);
+    maps = grad ? bb.take<double>(4 * plane) : nullptr;
+  };
+  double *partials, *rows, *maps;
+  layout(b, partials, rows, maps);
+  QMIV_TRY(ensure_fwork(ctx, b.off));
+  QMIV_TRY(ensure_results(ctx, sizeof(double) * (size_t)planes * h));
+  Bump bb{(char*)ctx->d_fwork};
+  layout(bb, partials, rows, maps);
+  for (int32_t c = 0; c < planes; c++) {
+    SsimGArgs g{};
+    g.grad_t = grad_tst ? grad_tst + c * plane : nullptr;
+    g.grad_r = grad_ref ? grad_ref + c * plane : nullptr;
+    g.coef_t = g.coef_r = 1.0;
+    QMIV_TRY(enqueue_ssimf(ctx, tst + c * plane, 0, max_value, ref + c * plane, 0, max_value, w, h, calc_l, max_value,
+                           partials, rows + (size_t)c * h, maps, grad ? &g : nullptr));
+  }
+  QMIV_HIP(hipMemcpyAsync(ctx->h_results, rows, sizeof(double) * (size_t)planes * h, hipMemcpyDeviceToHost, ctx->stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  for (int32_t c = 0; c < planes; c++) value[c] = fin::ssim_from_rows((const double*)ctx->h_results + (size_t)c * h, h, w);
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_msssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, int32_t planes, const float* tst,
+                                 const float* ref, double max_value, double* value, double* scales, float* grad_tst,
+                                 float* grad_ref) {
+  QMIV_TRY(check_fargs(ctx, w, h, planes, tst, ref, max_value));
+  if (!value) return fail(QMIV_ERR_INVALID_ARG, "null value");
+  QMIV_HIP(hipSetDevice(ctx->device));
+  const bool grad = grad_tst || grad_ref;
+  const size_t plane = (size_t)w * h;
+  constexpr int kScales = 5;
+  // Level l >= 1 of every plane: pictures in fp64 (exact 2 x 2 means) and the level's gradients.
+  struct Lv {
+    const void* t;
+    const void* r;
+    float *gt, *gr;
+  };
+  auto layout = [&](Bump& bb, double*& partials, double*& rows, double*& maps, std::vector<Lv>& lv) {
+    partials = bb.take<double>(std::max<size_t>((size_t)ssimf_tiles_x(w) * h, 1));
+    rows = bb.take<double>((size_t)planes * kScales * h);
+    maps = grad ? bb.take<double>(4 * plane) : nullptr;
+    for (int32_t c = 0; c < planes; c++)
+      for (int l = 1; l < kScales; l++) {
+        const size_t n = std::max<size_t>((size_t)(w >> l) * (h >> l), 1);
+        Lv& v = lv[(size_t)c * kScales + l];
+        v.t = bb.take<double>(n);
+        v.r = bb.take<double>(n);
+        v.gt = grad ? bb.take<float>(n) : nullptr;
+        v.gr = grad ? bb.take<float>(n) : nullptr;
+      }
+  };
+  double *partials, *rows, *maps;
+  std::vector<Lv> lv((size_t)planes * kScales);
+  Bump b{nullptr};
+  layout(b, partials, rows, maps, lv);
+  QMIV_TRY(ensure_fwork(ctx, b.off));
+  QMIV_TRY(ensure_results(ctx, sizeof(double) * (size_t)planes * kScales * h));
+  Bump bb{(char*)ctx->d_fwork};
+  layout(bb, partials, rows, maps, lv);
+  hipStream_t st = ctx->stream;
+  for (int32_t c = 0; c < planes; c++) {
+    Lv* L = &lv[(size_t)c * kScales];
+    L[0].t = tst + c * plane;
+    L[0].r = ref + c * plane;
+    L[0].gt = grad_tst ? grad_tst + c * plane : nullptr;
+    L[0].gr = grad_ref ? grad_ref + c * plane : nullptr;
+    for (int l = 1; l < kScales; l++) {
+      QMIV_HIP(launch_down2(L[l - 1].t, l > 1, w >> (l - 1), h >> (l - 1), (double*)L[l].t, st));
+      QMIV_HIP(launch_down2(L[l - 1].r, l > 1, w >> (l - 1), h >> (l - 1), (double*)L[l].r, st));
+    }
+    for (int l = 0; l < kScales; l++) {
+      const int32_t wl = w >> l, hl = h >> l;
+      double* rl = rows + ((size_t)c * kScales + l) * h;
+      if (wl <= 0 || hl <= 0) continue;
+      SsimGArgs g{};
+      g.grad_t = L[l].gt;
+      g.grad_r = L[l].gr;
+      g.coef_t = g.coef_r = 1.0;
+      // CS at the first and last scale, L x CS between (libs/pmbb-ivqm/src/xSSIM.cpp:74,80)
+      const int32_t calc_l = (l == 0 || l == kScales - 1) ? 0 : 1;
+      QMIV_TRY(enqueue_ssimf(ctx, L[l].t, l > 0, max_value, L[l].r, l > 0, max_value, wl, hl, calc_l, max_value,
+                             partials, rl, maps, grad ? &g : nullptr));
+    }
+  }
+  QMIV_HIP(hipMemcpyAsync(ctx->h_results, rows, sizeof(double) * (size_t)planes * kScales * h, hipMemcpyDeviceToHost,
+                          st));
+  QMIV_HIP(hipStreamSynchronize(st));
+  static const double wts[kScales] = {0.0448, 0.2856, 0.3001, 0.2363, 0.1333};  // xStructSimConsts.h:27
+  for (int32_t c = 0; c < planes; c++) {
+    double sc[kScales];
+    for (int l = 0; l < kScales; l++)
+      sc[l] = fin::ssim_from_rows((const double*)ctx->h_results + ((size_t)c * kScales + l) * h, h >> l, w >> l);
+    value[c] = fin::msssim_combine(sc);
+    if (scales)
+      for (int l = 0; l < kScales; l++) scales[l * planes + c] = sc[l];
+    if (grad) {
+      // d(prod s_l^w_l)/d s_l = MS w_l / s_l; the pyramid adjoint folds coarse into fine levels.
+      Lv* L = &lv[(size_t)c * kScales];
+      for (int l = kScales - 1; l >= 0; l--) {
+        const int32_t wl = w >> l, hl = h >> l;
+        if (wl <= 0 || hl <= 0) continue;
+        const double coef = value[c] * wts[l] / sc[l];
+        const bool top = l == kScales - 1 || (w >> (l + 1)) <= 0 || (h >> (l + 1)) <= 0;
+        if (L[l].gt) QMIV_HIP(launch_up2_addf(top ? nullptr : L[l + 1].gt, coef, wl, hl, L[l].gt, st));
+        if (L[l].gr) QMIV_HIP(launch_up2_addf(top ? nullptr : L[l + 1].gr, coef, wl, hl, L[l].gr, st));
+      }
+    }
+  }
+  if (grad) QMIV_HIP(hipStreamSynchronize(st));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, const float* tst, const float* ref,
+                                 double max_value, const qmiv_params* prm_in, double* ivssim, double* r2t, double* t2r,
+                                 float* grad_tst, float* grad_ref) {
+  QMIV_TRY(check_fargs(ctx, w, h, 3, tst, ref, max_value));
+  int32_t bd = 0;
+  for (int b = 8; b <= 16; b++)
+    if (max_value == (double)((1 << b) - 1)) bd = b;
+  if (!bd) return fail(QMIV_ERR_UNSUPPORTED, "IV-SSIM needs max_value = 2^bd - 1 with bd in 8..16");
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  const bool grad = grad_tst || grad_ref;
+  const size_t plane = (size_t)w * h;
+  hipStream_t st = ctx->stream;
+  // Shift search on the quantised pictures (P0 = tst, P1 = ref), keeping each pel's winner.
+  qmiv_pic *q0, *q1;
+  QMIV_TRY(fq_picture(ctx, 0, w, h, bd, &q0));
+  QMIV_TRY(fq_picture(ctx, 1, w, h, bd, &q1));
+  auto layout = [&](Bump& bb, int16_t*& best, float*& scp, float*& gscp, double*& partials, double*& rows,
+                    double*& maps) {
+    best = bb.take<int16_t>(2 * plane);
+    scp = bb.take<float>(6 * plane);
+    gscp = grad ? bb.take<float>(3 * plane) : nullptr;
+    partials = bb.take<double>(std::max<size_t>((size_t)ssimf_tiles_x(w) * h, 1));
+    rows = bb.take<double>(6 * (size_t)h);
+    maps = grad ? bb.take<double>(4 * plane) : nullptr;
+  };
+  int16_t* best;
+  float *scp, *gscp;
+  double *partials, *rows, *maps;
+  Bump b{nullptr};
+  layout(b, best, scp, gscp, partials, rows, maps);
+  QMIV_TRY(ensure_fwork(ctx, b.off));
+  Bump bb{(char*)ctx->d_fwork};
+  layout(bb, best, scp, gscp, partials, rows, maps);
+  const int32_t bytes = bd <= 8 ? 1 : 2;
+  QMIV_HIP(launch_quantf(tst, 3 * (int64_t)plane, max_value, (1 << bd) - 1, q0->dev, bytes, st));
+  QMIV_HIP(launch_quantf(ref, 3 * (int64_t)plane, max_value, (1 << bd) - 1, q1->dev, bytes, st));
+  FrameWork fw;
+  fw.need_gcd = true;
+  fw.need_cps = true;
+  fw.best0 = best;
+  fw.best1 = best + plane;
+  Layout L;
+  bool fast;
+  QMIV_TRY(run_frame_kernels(ctx, q0, q1, prm, fw, L, fast));
+  const FrameSummary* sum = (const FrameSummary*)((const char*)ctx->h_results + L.off_summary);
+  int32_t gcd[3] = {sum->gcd[0], sum->gcd[1], sum->gcd[2]};
+  const int32_t maxv = (1 << bd) - 1;
+  // SCP1 = clip(P1(best0) - gcd) (R2T winners), SCP0 = clip(P0(best1) + gcd) (T2R winners), in samples.
+  ScpFArgs s1{ref, best, w, h, prm.search_range, max_value, 1, {gcd[0], gcd[1], gcd[2]}, maxv, scp + 3 * plane};
+  ScpFArgs s0{tst, best + plane, w, h, prm.search_range, max_value, -1, {gcd[0], gcd[1], gcd[2]}, maxv, scp};
+  QMIV_HIP(launch_scp_gatherf(s1, st));
+  QMIV_HIP(launch_scp_gatherf(s0, st));
+  // T2R_c = SSIM(P0_c, SCP1_c), R2T_c = SSIM(P1_c, SCP0_c) (libs/pmbb-ivqm/src/xSSIM.cpp:197-207).
+  for (int c = 0; c < 3; c++) {
+    QMIV_TRY(enqueue_ssimf(ctx, tst + c * plane, 0, max_value, scp + 3 * plane + c * plane, 0, 1.0, w, h, 1, max_value,
+                           partials, rows + (size_t)c * h, nullptr, nullptr));
+    QMIV_TRY(enqueue_ssimf(ctx, ref + c * plane, 0, max_value, scp + c * plane, 0, 1.0, w, h, 1, max_value, partials,
+                           rows + (size_t)(3 + c) * h, nullptr, nullptr));
+  }
+  QMIV_TRY(ensure_results(ctx, sizeof(double) * 6 * (size_t)h));
+  QMIV_HIP(hipMemcpyAsync(ctx->h_results, rows, sizeof(double) * 6 * (size_t)h, hipMemcpyDeviceToHost, st));
+  QMIV_HIP(hipStreamSynchronize(st));
+  double vt2r[3], vr2t[3];
+  for (int c = 0; c < 3; c++) {
+    vt2r[c] = fin::ssim_from_rows((const double*)ctx->h_results + (size_t)c * h, h, w);
+    vr2t[c] = fin::ssim_from_rows((const double*)ctx->h_results + (size_t)(3 + c) * h, h, w);
+  }
+  const double T2R = fin::weighted_average4(vt2r, prm.cmp_weights_average);
+  const double R2T = fin::weighted_average4(vr2t, prm.cmp_weights_average);
+  const double IV = fin::min2(T2R, R2T);
+  if (ivssim) *ivssim = IV;
+  if (t2r) *t2r = T2R;
+  if (r2t) *r2t = R2T;
+  if (!grad) return QMIV_OK;
+  // Subgradient: the shifts, the GCD and the direction of the minimum stay fixed.
+  if (grad_tst) QMIV_HIP(hipMemsetAsync(grad_tst, 0, 3 * plane * sizeof(float), st));
+  if (grad_ref) QMIV_HIP(hipMemsetAsync(grad_ref, 0, 3 * plane * sizeof(float), st));
+  const bool use_r2t = R2T < T2R;  // min2 keeps T2R on ties
+  const double wsum = (double)(prm.cmp_weights_average[0] + prm.cmp_weights_average[1] + prm.cmp_weights_average[2] +
+                               prm.cmp_weights_average[3]);
+  // Direct argument: P0 (T2R) or P1 (R2T); the shift-compensated argument is scattered back into
+  // the other picture through its winners.
+  const float* direct = use_r2t ? ref : tst;
+  float* g_direct = use_r2t ? grad_ref : grad_tst;
+  float* g_other = use_r2t ? grad_tst : grad_ref;
+  const float* scp_sel = use_r2t ? scp : scp + 3 * plane;
+  for (int c = 0; c < 3; c++) {
+    SsimGArgs g{};
+    g.grad_t = g_direct ? g_direct + c * plane : nullptr;
+    g.grad_r = g_other ? gscp + c * plane : nullptr;
+    g.coef_t = g.coef_r = (double)prm.cmp_weights_average[c] / wsum;
+    g.accumulate = 1;
+    if (g.grad_r) QMIV_HIP(hipMemsetAsync(g.grad_r, 0, plane * sizeof(float), st));
+    QMIV_TRY(enqueue_ssimf(ctx, direct + c * plane, 0, max_value, scp_sel + c * plane, 0, 1.0, w, h, 1, max_value,
+                           partials, rows + (size_t)c * h, maps, &g));
+  }
+  if (g_other) QMIV_HIP(launch_scp_scatterf(use_r2t ? s0 : s1, gscp, 1.0, g_other, st));
+  QMIV_HIP(hipStreamSynchronize(st));
+  return QMIV_OK;
 }
 
 }  // extern "C"

@@ -130,6 +130,7 @@ __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int
   const Buf<T> Xp(DIR == 0 ? a.p0 : a.p1, g.frame_bytes);
   const Buf<T> Yp(DIR == 0 ? a.p1 : a.p0, g.frame_bytes);
   uint16_t* scp_ptr = DIR == 0 ? a.scp1 : a.scp0;
+  int16_t* best_out = DIR == 0 ? a.best0 : a.best1;
   constexpr int32_t sgn = DIR == 0 ? 1 : -1;
   const int32_t w0 = a.ws[0], w1 = a.ws[1], w2 = a.ws[2];
   const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
@@ -255,6 +256,7 @@ __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int
           acc[2 * b + py][0] += (uint32_t)(d0 * d0);
           acc[2 * b + py][1] += (uint32_t)(d1 * d1);
           acc[2 * b + py][2] += (uint32_t)(d2 * d2);
+          if (best_out) best_out[y * g.w + x] = (int16_t)idx;
           if (scp_ptr) {
             const Buf<uint16_t> scp(scp_ptr, 6u * (uint32_t)g.w * (uint32_t)g.h);
             const uint32_t o = (uint32_t)(y * g.w + x);

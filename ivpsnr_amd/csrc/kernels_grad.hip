@@ -1,0 +1,365 @@
+// gfx950 kernels of the differentiable SSIM family (float pictures): the forward windowed moments
+// with the per-pel partial derivatives of the SSIM map, the gradient correlation, the float 2x2
+// pyramid and its adjoint, quantisation for the shift search, and the gather of the
+// shift-compensated pictures with its adjoint scatter.
+//
+// SSIM map of one interior pel p (xStructSim<flt64, CalcL>::CalcPel,
+// libs/pmbb-ivqm/src/xStructSim.cpp:15-60, on real-valued samples):
+//   muT = sum w T, muR = sum w R, qT = sum w T^2, qR = sum w R^2, x = sum w T R
+//   N = 2 (x - muT muR) + C2, D = (qT - muT^2) + (qR - muR^2) + C2, CS = N / D
+//   L = (2 muT muR + C1) / (muT^2 + muR^2 + C1),  s = L CS (or CS alone)
+// Its derivatives: ds/dmuT = A_T, ds/dmuR = A_R, ds/dqT = ds/dqR = B, ds/dx = Cm, with
+//   dCS/dmuT = (2 muT N - 2 muR D) / D^2,  dCS/dq = -N / D^2,  dCS/dx = 2 / D,
+//   dL/dmuT = (2 muR Ld - 2 muT Ln) / Ld^2   (and symmetric in T <-> R).
+// A sample T(q) enters the windows of pels p = q - o with weight w(o), hence
+//   dS/dT(q) = sum_o w(o) [A_T + 2 T(q) B + R(q) Cm](q - o) / (W H),
+// a correlation of the maps with the flipped window (k_ssimf_grad).
+#include <utility>
+
+#include "device_common.h"
+#include "qmiv_internal.h"
+
+namespace qmiv {
+namespace {
+
+constexpr int kGW = 64;          // output columns per workgroup (one per lane)
+constexpr int kGK = 4;           // output rows per lane
+constexpr int kGH = 4 * kGK;     // output rows per workgroup
+constexpr int kGLW = kGW + 9;    // window columns
+constexpr int kGLH = kGH + 9;
+constexpr int kGWRow = kGK + 16;  // stride of one transposed weight column in LDS
+
+// One loaded window row R of the lane's K output rows: the NV values at column c feed pels
+// k = max(0, R - 9) .. min(K - 1, R) with weight W[R - k][c] (fused multiply-add, in the window
+// order: rows outer, columns inner).
+template <int NV, int R, typename Load>
+__device__ __forceinline__ void win_row(const Load& load, const double* s_w, double (&acc)[kGK][NV]) {
+  constexpr int k_lo = R > 9 ? R - 9 : 0;
+  constexpr int k_hi = R < kGK - 1 ? R : kGK - 1;
+#pragma unroll
+  for (int c = 0; c < 10; c++) {
+    double v[NV];
+    load(R, c, v);
+#pragma unroll
+    for (int k = k_lo; k <= k_hi; k++) {
+      const double w = s_w[c * kGWRow + R - k + 7];
+#pragma unroll
+      for (int m = 0; m < NV; m++) acc[k][m] = __builtin_fma(v[m], w, acc[k][m]);
+    }
+    // Keep this step's FMAs ahead of the next step's loads (see kernels_ssim.hip).
+#pragma unroll
+    for (int k = k_lo; k <= k_hi; k++)
+#pragma unroll
+      for (int m = 0; m < NV; m++) asm volatile("" : "+v"(acc[k][m])::"memory");
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int NV, typename Load, int... R>
+__device__ __forceinline__ void win_rows(const Load& load, const double* s_w, double (&acc)[kGK][NV],
+                                         std::integer_sequence<int, R...>) {
+  (win_row<NV, R>(load, s_w, acc), ...);
+}
+
+__device__ __forceinline__ double ldsample(const void* p, int32_t f64, int64_t i) {
+  return f64 ? ((const double*)p)[i] : (double)((const float*)p)[i];
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Forward: SSIM map row sums of one plane pair, and (GRAD) the maps A_T, A_R, B, Cm of every
+// interior pel, divided by W H. Tile 64 x 16 over the interior, like the integer kernel.
+// ---------------------------------------------------------------------------------------------
+template <int CALC_L, int GRAD>
+__global__ __launch_bounds__(256) void k_ssimf(SsimFArgs a) {
+  __shared__ double s_t[kGLH * kGLW];
+  __shared__ double s_r[kGLH * kGLW];
+  __shared__ double s_w[10 * kGWRow];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int tx = blockIdx.x;
+  const int x0 = 5 + tx * kGW;
+  const int y0 = 5 + blockIdx.y * kGH;
+  for (int i = threadIdx.x; i < 10 * kGWRow; i += 256) {
+    const int dx = i / kGWRow;
+    const int dy = i - dx * kGWRow - 7;
+    s_w[i] = (dy >= 0 && dy <= 9) ? a.f[dy * 10 + dx] : 0.0;
+  }
+  for (int i = threadIdx.x; i < kGLH * kGLW; i += 256) {
+    const int ly = i / kGLW;
+    const int lx = i - ly * kGLW;
+    const int gy = min(y0 - 5 + ly, a.h - 1);
+    const int gx = min(x0 - 5 + lx, a.w - 1);
+    const int64_t o = (int64_t)gy * a.w + gx;
+    s_t[i] = ldsample(a.tst, a.tst_f64, o) * a.scale_t;
+    s_r[i] = ldsample(a.ref, a.ref_f64, o) * a.scale_r;
+  }
+  __syncthreads();
+
+  double acc[kGK][5];
+#pragma unroll
+  for (int k = 0; k < kGK; k++)
+#pragma unroll
+    for (int m = 0; m < 5; m++) acc[k][m] = 0.0;
+  const int base = (kGK * wave) * kGLW + lane;
+  auto load = [&](int R, int c, double (&v)[5]) {
+    const double tv = s_t[base + R * kGLW + c];
+    const double rv = s_r[base + R * kGLW + c];
+    v[0] = rv;
+    v[1] = tv;
+    v[2] = rv * rv;
+    v[3] = tv * tv;
+    v[4] = rv * tv;
+  };
+  win_rows<5>(load, s_w, acc, std::make_integer_sequence<int, kGK + 9>{});
+
+  const int x = x0 + lane;
+  const double inv_area = 1.0 / ((double)a.w * (double)a.h);
+#pragma unroll
+  for (int k = 0; k < kGK; k++) {
+    const int y = y0 + kGK * wave + k;
+    double sv = 0.0;
+    if (x < a.w - 5 && y < a.h - 5) {
+      const double muR = acc[k][0], muT = acc[k][1];
+      const double vR = acc[k][2] - muR * muR;
+      const double vT = acc[k][3] - muT * muT;
+      const double cov = acc[k][4] - muR * muT;
+      const double N = 2 * cov + a.c2;
+      const double D = vR + vT + a.c2;
+      const double cs = N / D;
+      double L = 1.0, Ln = 0.0, Ld = 1.0;
+      if (CALC_L) {
+        Ln = 2 * muR * muT + a.c1;
+        Ld = muR * muR + muT * muT + a.c1;
+        L = Ln / Ld;
+      }
+      sv = L * cs;
+      if (GRAD) {
+        const double D2 = D * D;
+        const double dcs_dmuT = (2 * muT * N - 2 * muR * D) / D2;
+        const double dcs_dmuR = (2 * muR * N - 2 * muT * D) / D2;
+        double aT = L * dcs_dmuT, aR = L * dcs_dmuR;
+        if (CALC_L) {
+          const double Ld2 = Ld * Ld;
+          aT += (2 * muR * Ld - 2 * muT * Ln) / Ld2 * cs;
+          aR += (2 * muT * Ld - 2 * muR * Ln) / Ld2 * cs;
+        }
+        const int64_t o = (int64_t)y * a.w + x;
+        const int64_t plane = (int64_t)a.w * a.h;
+        a.maps[o] = aT * inv_area;
+        a.maps[plane + o] = aR * inv_area;
+        a.maps[2 * plane + o] = -L * N / D2 * inv_area;
+        a.maps[3 * plane + o] = 2 * L / D * inv_area;
+      }
+    }
+    sv = wave_sum_d(sv);
+    if (lane == 0 && y < a.h - 5) a.partials[(int64_t)tx * a.h + y] = sv;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Gradient: gT(q) = coef_t scale_t [cA_T + 2 T(q) cB + R(q) cC](q), gR likewise, with c* the
+// correlations of the maps (zero outside the interior) with the flipped window. Written, or
+// added (accumulate), to the outputs that are not null. Tile 64 x 16 over the whole plane.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ssimf_grad(SsimGArgs a) {
+  __shared__ double s_m[4][kGLH * kGLW];
+  __shared__ double s_w[10 * kGWRow];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * kGW;
+  const int y0 = blockIdx.y * kGH;
+  // Flipped window F'[a][b] = F[9 - a][9 - b], offsets (a - 4, b - 4) from the output pel.
+  for (int i = threadIdx.x; i < 10 * kGWRow; i += 256) {
+    const int dx = i / kGWRow;
+    const int dy = i - dx * kGWRow - 7;
+    s_w[i] = (dy >= 0 && dy <= 9) ? a.f[(9 - dy) * 10 + (9 - dx)] : 0.0;
+  }
+  const int64_t plane = (int64_t)a.w * a.h;
+  for (int i = threadIdx.x; i < kGLH * kGLW; i += 256) {
+    const int ly = i / kGLW;
+    const int lx = i - ly * kGLW;
+    const int py = y0 - 4 + ly;
+    const int px = x0 - 4 + lx;
+    const bool in = py >= 5 && py < a.h - 5 && px >= 5 && px < a.w - 5;
+    const int64_t o = (int64_t)py * a.w + px;
+#pragma unroll
+    for (int m = 0; m < 4; m++) s_m[m][i] = in ? a.maps[m * plane + o] : 0.0;
+  }
+  __syncthreads();
+
+  double acc[kGK][4];
+#pragma unroll
+  for (int k = 0; k < kGK; k++)
+#pragma unroll
+    for (int m = 0; m < 4; m++) acc[k][m] = 0.0;
+  const int base = (kGK * wave) * kGLW + lane;
+  auto load = [&](int R, int c, double (&v)[4]) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) v[m] = s_m[m][base + R * kGLW + c];
+  };
+  win_rows<4>(load, s_w, acc, std::make_integer_sequence<int, kGK + 9>{});
+
+  const int x = x0 + lane;
+#pragma unroll
+  for (int k = 0; k < kGK; k++) {
+    const int y = y0 + kGK * wave + k;
+    if (x < a.w && y < a.h) {
+      const int64_t o = (int64_t)y * a.w + x;
+      const double tv = ldsample(a.tst, a.tst_f64, o) * a.scale_t;
+      const double rv = ldsample(a.ref, a.ref_f64, o) * a.scale_r;
+      if (a.grad_t) {
+        const double g = a.coef_t * a.scale_t * (acc[k][0] + 2 * tv * acc[k][2] + rv * acc[k][3]);
+        a.grad_t[o] = (float)(a.accumulate ? (double)a.grad_t[o] + g : g);
+      }
+      if (a.grad_r) {
+        const double g = a.coef_r * a.scale_r * (acc[k][1] + 2 * rv * acc[k][2] + tv * acc[k][3]);
+        a.grad_r[o] = (float)(a.accumulate ? (double)a.grad_r[o] + g : g);
+      }
+    }
+  }
+}
+
+// dst (w/2 x h/2) = mean of the 2 x 2 blocks of src (w x h), exact in fp64.
+__global__ __launch_bounds__(256) void k_down2(const void* __restrict__ src, int32_t src_f64, int32_t w, int32_t h,
+                                               double* __restrict__ dst) {
+  const int dw = w >> 1, dh = h >> 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)dw * dh) return;
+  const int y = (int)(i / dw);
+  const int x = (int)(i - (int64_t)y * dw);
+  const int64_t o = (int64_t)(2 * y) * w + 2 * x;
+  dst[i] = 0.25 * ((ldsample(src, src_f64, o) + ldsample(src, src_f64, o + 1)) +
+                   (ldsample(src, src_f64, o + w) + ldsample(src, src_f64, o + w + 1)));
+}
+
+// Pyramid adjoint: fine = coef * fine + coarse[y >> 1][x >> 1] / 4 where a coarse parent exists.
+__global__ __launch_bounds__(256) void k_up2_addf(const float* __restrict__ coarse, double coef, int32_t w, int32_t h,
+                                                  float* __restrict__ fine) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)w * h) return;
+  const int y = (int)(i / w);
+  const int x = (int)(i - (int64_t)y * w);
+  double v = coef * (double)fine[i];
+  const int dw = w >> 1, dh = h >> 1;
+  if (coarse && (y >> 1) < dh && (x >> 1) < dw) v += 0.25 * (double)coarse[(int64_t)(y >> 1) * dw + (x >> 1)];
+  fine[i] = (float)v;
+}
+
+// dst = rint(clamp(src * scale, 0, maxv)), 8- or 16-bit samples for the shift search.
+__global__ __launch_bounds__(256) void k_quantf(const float* __restrict__ src, int64_t n, double scale, int32_t maxv,
+                                                void* __restrict__ dst, int32_t bytes) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = fmin(fmax((double)src[i] * scale, 0.0), (double)maxv);
+  const uint32_t q = (uint32_t)__builtin_rint(v);
+  if (bytes == 1) ((uint8_t*)dst)[i] = (uint8_t)q;
+  else ((uint16_t*)dst)[i] = (uint16_t)q;
+}
+
+// Shift-compensated picture in sample units (xShftCompPic::xGenShftCompRow on real samples):
+// out_c(p) = clamp(src_c(clamp(p + o(best(p)))) * scale - sgn gcd_c, 0, maxv), 3 planes.
+__global__ __launch_bounds__(256) void k_scp_gatherf(ScpFArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t plane = (int64_t)a.w * a.h;
+  if (i >= plane) return;
+  const int y = (int)(i / a.w);
+  const int x = (int)(i - (int64_t)y * a.w);
+  const int n = 2 * a.B + 1;
+  const int idx = a.best[i];
+  const int sy = clampi(y + idx / n - a.B, 0, a.h - 1);
+  const int sx = clampi(x + idx % n - a.B, 0, a.w - 1);
+  const int64_t s = (int64_t)sy * a.w + sx;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double v = (double)a.src[c * plane + s] * a.scale - a.sgn * a.gcd[c];
+    a.out[c * plane + i] = (float)fmin(fmax(v, 0.0), (double)a.maxv);
+  }
+}
+
+// Adjoint of k_scp_gatherf: grad_src_c(clamp(p + o)) += coef * scale * g_c(p) where the clamp to
+// [0, maxv] is not active.
+__global__ __launch_bounds__(256) void k_scp_scatterf(ScpFArgs a, const float* __restrict__ g, double coef,
+                                                      float* __restrict__ grad_src) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t plane = (int64_t)a.w * a.h;
+  if (i >= plane) return;
+  const int y = (int)(i / a.w);
+  const int x = (int)(i - (int64_t)y * a.w);
+  const int n = 2 * a.B + 1;
+  const int idx = a.best[i];
+  const int sy = clampi(y + idx / n - a.B, 0, a.h - 1);
+  const int sx = clampi(x + idx % n - a.B, 0, a.w - 1);
+  const int64_t s = (int64_t)sy * a.w + sx;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double v = (double)a.src[c * plane + s] * a.scale - a.sgn * a.gcd[c];
+    if (v >= 0.0 && v <= (double)a.maxv)
+      atomicAdd(&grad_src[c * plane + s], (float)(coef * a.scale * (double)g[c * plane + i]));
+  }
+}
+
+}  // namespace
+
+int32_t ssimf_tiles_x(int32_t w) { return w > 10 ? (w - 10 + kGW - 1) / kGW : 0; }
+
+hipError_t launch_ssimf(const SsimFArgs& a, int32_t calc_l, int32_t grad, hipStream_t stream) {
+  const int tiles_x = ssimf_tiles_x(a.w);
+  const int tiles_y = a.h > 10 ? (a.h - 10 + kGH - 1) / kGH : 0;
+  if (tiles_x != a.tiles_x) return hipErrorInvalidValue;
+  if (tiles_x == 0 || tiles_y == 0) return hipSuccess;
+  const dim3 grid(tiles_x, tiles_y);
+  if (calc_l && grad) k_ssimf<1, 1><<<grid, dim3(256), 0, stream>>>(a);
+  else if (calc_l) k_ssimf<1, 0><<<grid, dim3(256), 0, stream>>>(a);
+  else if (grad) k_ssimf<0, 1><<<grid, dim3(256), 0, stream>>>(a);
+  else k_ssimf<0, 0><<<grid, dim3(256), 0, stream>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ssimf_grad(const SsimGArgs& a, hipStream_t stream) {
+  if (a.w <= 0 || a.h <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ssimf_grad, dim3((a.w + kGW - 1) / kGW, (a.h + kGH - 1) / kGH), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_down2(const void* src, int32_t src_f64, int32_t w, int32_t h, double* dst, hipStream_t stream) {
+  const int64_t n = (int64_t)(w >> 1) * (h >> 1);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_down2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, src_f64, w, h, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_up2_addf(const float* coarse, double coef, int32_t w, int32_t h, float* fine, hipStream_t stream) {
+  const int64_t n = (int64_t)w * h;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_up2_addf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, coarse, coef, w, h, fine);
+  return hipGetLastError();
+}
+
+hipError_t launch_quantf(const float* src, int64_t n, double scale, int32_t maxv, void* dst, int32_t bytes,
+                         hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_quantf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, n, scale, maxv, dst, bytes);
+  return hipGetLastError();
+}
+
+hipError_t launch_scp_gatherf(const ScpFArgs& a, hipStream_t stream) {
+  const int64_t n = (int64_t)a.w * a.h;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scp_gatherf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scp_scatterf(const ScpFArgs& a, const float* g, double coef, float* grad_src, hipStream_t stream) {
+  const int64_t n = (int64_t)a.w * a.h;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scp_scatterf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, g, coef, grad_src);
+  return hipGetLastError();
+}
+
+}  // namespace qmiv

@@ -299,6 +299,7 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
   const Buf<T> Xp(DIR == 0 ? a.p0 : a.p1, g.frame_bytes);
   const Buf<T> Yp(DIR == 0 ? a.p1 : a.p0, g.frame_bytes);
   uint16_t* scp_ptr = DIR == 0 ? a.scp1 : a.scp0;
+  int16_t* best_out = DIR == 0 ? a.best0 : a.best1;
   constexpr int32_t sgn = DIR == 0 ? 1 : -1;
   const int32_t w0 = a.ws[0], w1 = a.ws[1], w2 = a.ws[2];
   const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
@@ -387,6 +388,7 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
       acc[k][0] += (uint32_t)(d0 * d0);
       acc[k][1] += (uint32_t)(d1 * d1);
       acc[k][2] += (uint32_t)(d2 * d2);
+      if (best_out) best_out[y * g.w + x] = (int16_t)idx;
       if (scp_ptr) {
         const Buf<uint16_t> scp(scp_ptr, 6u * (uint32_t)g.w * (uint32_t)g.h);
         const uint32_t o = (uint32_t)(y * g.w + x);
@@ -465,6 +467,7 @@ __global__ __launch_bounds__(256) void k_cps_generic(CpsArgs a) {
   const T* __restrict__ Xp = (const T*)(dir == 0 ? a.p0 : a.p1);
   const T* __restrict__ Yp = (const T*)(dir == 0 ? a.p1 : a.p0);
   uint16_t* __restrict__ scp = dir == 0 ? a.scp1 : a.scp0;
+  int16_t* __restrict__ best_out = dir == 0 ? a.best0 : a.best1;
   uint32_t dist_best[3] = {0, 0, 0};
   if (x < g.w) {
     int32_t t[4];
@@ -475,6 +478,7 @@ __global__ __launch_bounds__(256) void k_cps_generic(CpsArgs a) {
     t[3] = sgn * sum->gcd[3];
     int32_t best_err = INT_MAX;
     int64_t best_q = -1, best_co = -1;
+    int32_t best_idx = 0, idx = 0;
     for (int dy = -a.B; dy <= a.B; dy++) {
       const int qy = clampi(y + dy, 0, g.h - 1);
       for (int dx = -a.B; dx <= a.B; dx++) {
@@ -497,9 +501,12 @@ __global__ __launch_bounds__(256) void k_cps_generic(CpsArgs a) {
           dist_best[2] = dist[2];
           best_q = q;
           best_co = qc;
+          best_idx = idx;
         }
+        idx++;
       }
     }
+    if (best_out) best_out[(int64_t)y * g.w + x] = (int16_t)best_idx;
     if (scp && best_q >= 0) {
       const int64_t o = (int64_t)y * g.w + x;
       const int64_t plane = (int64_t)g.w * g.h;

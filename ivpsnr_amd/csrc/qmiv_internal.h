@@ -47,6 +47,8 @@ struct CpsArgs {
   uint64_t* partials;        // [nseg][2][3][h] per-segment row sums
   uint16_t* scp0;            // optional [3][h][w]: clipU(P0(best of P1 - g) + g)   (SCP of InputFile0)
   uint16_t* scp1;            // optional [3][h][w]: clipU(P1(best of P0 + g) - g)   (SCP of InputFile1)
+  int16_t* best0;            // optional [h][w]: R2T winner, window raster index (dy + B)(2B + 1) + dx + B
+  int16_t* best1;            // optional [h][w]: T2R winner
   int32_t maxv;
 };
 
@@ -78,6 +80,57 @@ hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* 
 hipError_t launch_cps_fast(const CpsArgs& a, hipStream_t stream);
 hipError_t launch_cps_generic(const CpsArgs& a, hipStream_t stream);
 hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, uint64_t* rows, hipStream_t stream);
+// ---- Differentiable SSIM family on float planes (kernels_grad.hip) ----
+// Forward moments of one plane pair; samples are src * scale (src fp32, or fp64 when *_f64).
+struct SsimFArgs {
+  const void* tst;
+  const void* ref;
+  int32_t tst_f64, ref_f64;
+  double scale_t, scale_r;
+  int32_t w, h;
+  int32_t tiles_x;       // ssimf_tiles_x(w)
+  double c1, c2;
+  double* partials;      // [tiles_x][h] row sums of the SSIM map
+  double* maps;          // [4][h][w] A_T, A_R, B, Cm over W H (interior pels; others untouched), or null
+  double f[100];         // window weights F[dy + 5][dx + 5]
+};
+// Gradient from the maps: g = coef * scale * (...), stored or added (accumulate) where not null.
+struct SsimGArgs {
+  const void* tst;
+  const void* ref;
+  int32_t tst_f64, ref_f64;
+  double scale_t, scale_r;
+  int32_t w, h;
+  const double* maps;
+  float* grad_t;
+  float* grad_r;
+  double coef_t, coef_r;
+  int32_t accumulate;
+  double f[100];
+};
+// Shift-compensated float picture (3 planes) from the winners of one search direction.
+struct ScpFArgs {
+  const float* src;      // searched picture (3 planes)
+  const int16_t* best;   // [h][w] winner indices
+  int32_t w, h, B;
+  double scale;
+  int32_t sgn;           // +1: Y - gcd (SCP of InputFile1); -1: Y + gcd (SCP of InputFile0)
+  int32_t gcd[3];
+  int32_t maxv;
+  float* out;            // [3][h][w] in sample units
+};
+int32_t ssimf_tiles_x(int32_t w);
+hipError_t launch_ssimf(const SsimFArgs& a, int32_t calc_l, int32_t grad, hipStream_t stream);
+hipError_t launch_ssimf_grad(const SsimGArgs& a, hipStream_t stream);
+// dst (fp64, w/2 x h/2) = exact mean of the 2 x 2 blocks of src (fp32, or fp64 when src_f64).
+hipError_t launch_down2(const void* src, int32_t src_f64, int32_t w, int32_t h, double* dst, hipStream_t stream);
+// fine = coef * fine + coarse(parent) / 4 (coarse may be null: scaling only).
+hipError_t launch_up2_addf(const float* coarse, double coef, int32_t w, int32_t h, float* fine, hipStream_t stream);
+hipError_t launch_quantf(const float* src, int64_t n, double scale, int32_t maxv, void* dst, int32_t bytes,
+                         hipStream_t stream);
+hipError_t launch_scp_gatherf(const ScpFArgs& a, hipStream_t stream);
+hipError_t launch_scp_scatterf(const ScpFArgs& a, const float* g, double coef, float* grad_src, hipStream_t stream);
+
 // Tile shape of the fast CPS kernel (pels per workgroup tile).
 void cps_fast_tile(int32_t* tile_w, int32_t* tile_h);
 // Resident workgroups of the fast CPS kernel per CU (from the HIP occupancy calculator).

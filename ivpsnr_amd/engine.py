@@ -130,6 +130,31 @@ class Engine:
         self._batches.append(weakref.ref(bt))
         return bt
 
+    # ---- differentiable SSIM family on float planes in device memory (pointers) ----
+    def ssim_backward(self, w, h, planes, tst_ptr, ref_ptr, max_value, calc_l=True, grad_tst_ptr=0, grad_ref_ptr=0):
+        v = (C.c_double * planes)()
+        check(lib().qmiv_ssim_backward(self.ctx, w, h, planes, C.c_void_p(tst_ptr), C.c_void_p(ref_ptr), max_value,
+                                       1 if calc_l else 0, v, C.c_void_p(grad_tst_ptr or None),
+                                       C.c_void_p(grad_ref_ptr or None)), "qmiv_ssim_backward")
+        return list(v)
+
+    def msssim_backward(self, w, h, planes, tst_ptr, ref_ptr, max_value, grad_tst_ptr=0, grad_ref_ptr=0):
+        v = (C.c_double * planes)()
+        sc = (C.c_double * (5 * planes))()
+        check(lib().qmiv_msssim_backward(self.ctx, w, h, planes, C.c_void_p(tst_ptr), C.c_void_p(ref_ptr), max_value,
+                                         v, sc, C.c_void_p(grad_tst_ptr or None), C.c_void_p(grad_ref_ptr or None)),
+              "qmiv_msssim_backward")
+        return list(v), np.array(list(sc)).reshape(5, planes)
+
+    def ivssim_backward(self, w, h, tst_ptr, ref_ptr, max_value, params: Params | None = None, grad_tst_ptr=0,
+                        grad_ref_ptr=0):
+        iv, r2t, t2r = C.c_double(), C.c_double(), C.c_double()
+        check(lib().qmiv_ivssim_backward(self.ctx, w, h, C.c_void_p(tst_ptr), C.c_void_p(ref_ptr), max_value,
+                                         C.byref(params) if params else None, C.byref(iv), C.byref(r2t),
+                                         C.byref(t2r), C.c_void_p(grad_tst_ptr or None),
+                                         C.c_void_p(grad_ref_ptr or None)), "qmiv_ivssim_backward")
+        return {"ivssim": iv.value, "r2t": r2t.value, "t2r": t2r.value}
+
     def calc_gcd(self, tst: Picture, ref: Picture, params: Params | None = None) -> np.ndarray:
         g = (C.c_int32 * 4)()
         check(lib().qmiv_calc_gcd(self.ctx, tst.handle, ref.handle, C.byref(params) if params else None, g),

@@ -42,3 +42,16 @@ def assert_frame_equal(gpu, orc, metrics, ctx=""):
 
 def planes444(oracle, packed, w, h, bd, cf):
     return oracle.unpack(np.ascontiguousarray(packed), w, h, bd, cf)
+
+
+def ssim_window_table():
+    """The 10 x 10 SSIM window (rows dy, columns dx) as doubles: the normalised 11 x 11 Gaussian
+    (sigma 1.5) with each tap rounded to ten decimals and stored in fp32
+    (libs/pmbb-ivqm/src/xStructSimConsts.h:30-43), first 10 rows and columns
+    (libs/pmbb-ivqm/src/xStructSim.cpp:19-21). Pinned by tests/golden/ssim_filter_bits.json."""
+    import numpy as np
+    g = np.exp(-((np.arange(11) - 5.0) ** 2) / (2 * 1.5 * 1.5))
+    m = np.outer(g, g)
+    m = m / m.sum()
+    taps = np.array([np.float32(float("%.10f" % v)) for v in m.reshape(-1)], np.float32).reshape(11, 11)
+    return taps[:10, :10].astype(np.float64)

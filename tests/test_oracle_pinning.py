@@ -274,3 +274,13 @@ def test_int32_row_wrap_semantics(oracle):
     exact = w * 1023 * 1023
     assert int(rows64[0, 0]) == exact
     assert int(rows32[0, 0]) == int(np.uint64(np.int64(np.int32(np.uint32(exact % (1 << 32))))))
+
+
+def test_window_table_helper_matches_reference_bits():
+    """The test helper's SSIM window equals the reference's fp32 table bit for bit."""
+    import json, os
+    import numpy as np
+    from .helpers import ssim_window_table
+    bits = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ssim_filter_bits.json")))
+    table = np.array(bits, np.uint32).view(np.float32).reshape(11, 11)
+    np.testing.assert_array_equal(ssim_window_table(), table[:10, :10].astype(np.float64))
