@@ -8,7 +8,9 @@ and resident in HBM before the timed region starts.
 instead (its dominant kernel is the fp64 windowed-moment kernel). `--workload multiview24` times
 24 views of uneven size (8 x 4096x2048 ERP, 8 x 1920x1080, 8 x 2048x1088, yuv420p10le) x 4 frames,
 IV-PSNR + WS-PSNR + IV-SSIM, as one captured graph replay per frame set against the plain
-per-view call loop; `multiview24-ivpsnr` is the same without IV-SSIM.
+per-view call loop; `multiview24-ivpsnr` is the same without IV-SSIM. `--workload grad-ivssim1024`
+times the differentiable IV-SSIM (forward and gradients of both pictures) on 8 x 3 x 1024 x 1024
+fp32 pictures.
 
 Prints one JSON line:
   value      whole-job Mpix/s over the timed steps
@@ -214,12 +216,74 @@ def run_multiview(args, with_ssim):
     eng.close()
 
 
+def run_grad(args):
+    """IV-SSIM forward + gradients of 8 fp32 picture pairs 3 x 1024 x 1024 in [0, 1]: a smooth
+    random field, a copy shifted by up to 2 pels per 32 x 32 block with N(0, 0.02) noise."""
+    import torch  # first: the engine library then shares torch's HIP runtime (one runtime per process)
+    from ivpsnr_amd import Engine
+
+    n, c, h, w = 8, 3, 1024, 1024
+    maxv = 1023.0
+    g = torch.Generator(device="cuda").manual_seed(5)
+    yy = torch.arange(h, device="cuda", dtype=torch.float32)[:, None] / h
+    xx = torch.arange(w, device="cuda", dtype=torch.float32)[None, :] / w
+    ref = torch.zeros(n, c, h, w, device="cuda")
+    for _ in range(6):
+        a = torch.rand(n, c, 1, 1, device="cuda", generator=g) * 0.1 + 0.05
+        fx = torch.rand(n, c, 1, 1, device="cuda", generator=g) * 3.5 + 0.5
+        fy = torch.rand(n, c, 1, 1, device="cuda", generator=g) * 3.5 + 0.5
+        ph = torch.rand(n, c, 1, 1, device="cuda", generator=g) * 6.283
+        ref += a * torch.sin(6.283 * (fx * xx + fy * yy) + ph)
+    ref = (ref + 0.5).clamp(0, 1)
+    sh = torch.randint(-2, 3, (2, h // 32, w // 32), device="cuda", generator=g)
+    iy = (torch.arange(h, device="cuda")[:, None] + sh[0].repeat_interleave(32, 0).repeat_interleave(32, 1)).clamp(0, h - 1)
+    ix = (torch.arange(w, device="cuda")[None, :] + sh[1].repeat_interleave(32, 0).repeat_interleave(32, 1)).clamp(0, w - 1)
+    tst = (ref[:, :, iy, ix] + 0.02 * torch.randn(n, c, h, w, device="cuda", generator=g)).clamp(0, 1).contiguous()
+    ref = ref.contiguous()
+    gt, gr = torch.empty_like(tst), torch.empty_like(ref)
+    torch.cuda.synchronize()
+    eng = Engine(0)
+
+    def step():
+        return [eng.ivssim_backward(w, h, tst[i].data_ptr(), ref[i].data_ptr(), maxv, None, gt[i].data_ptr(),
+                                    gr[i].data_ptr())["ivssim"] for i in range(n)]
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vals = step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_per_step = el / args.steps * 1e3
+    mpix = n * h * w / 1e6
+    # fp64 flops per picture pair: 6 forward evaluations (value) + 3 forward with maps + 3 gradient
+    # correlations (4 maps x 100 taps x 2), per interior / whole pel
+    flops = n * c * ((6 + 3) * 1000 * (h - 10) * (w - 10) + 3 * 800 * h * w) / 3
+    achieved = flops / (ms_per_step / 1e3) / 1e12
+    out = {
+        "metric": "Differentiable IV-SSIM forward+backward Mpix/s @3x1024x1024 fp32, batch 8",
+        "value": round(mpix / (ms_per_step / 1e3), 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "IV-SSIM value + gradients of both pictures, 8 x 3 x 1024 x 1024 fp32 in [0,1], "
+                               "max 1023, B=2", "ivssim_first": round(vals[0], 6)},
+        "roofline": {"bound": "fp64-valu", "achieved": round(achieved, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "whole step (k_ssimf, k_ssimf_grad, CPS, gather/scatter)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["multiview24", "multiview24-ivpsnr"],
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["multiview24", "multiview24-ivpsnr", "grad-ivssim1024"],
                     default="ivpsnr-erp4k10")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--frames", type=int, default=0, help="frame pairs per step (default: the workload's)")
@@ -236,6 +300,9 @@ def main():
 
     if args.workload.startswith("multiview24"):
         run_multiview(args, with_ssim=args.workload == "multiview24")
+        return
+    if args.workload == "grad-ivssim1024":
+        run_grad(args)
         return
 
     from ivpsnr_amd import Engine, default_params

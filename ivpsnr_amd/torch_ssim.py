@@ -5,6 +5,10 @@ Inputs are float32 CUDA tensors of shape (planes, H, W) or (N, planes, H, W) wit
 [0, 1] standing for samples x * max_value. The engine computes the forward values and the
 gradients in one pass (fp64 inside); backward scales the saved gradients by the incoming ones.
 Values: per plane for SSIM / MS-SSIM, one scalar per picture for IV-SSIM.
+
+One HIP runtime serves a process: PyTorch wheels carry their own libamdhip64 under the soname the
+engine library asks for, so whichever loads first serves both. This module imports torch before
+the engine library is loaded, so the engine runs on torch's runtime here.
 """
 from __future__ import annotations
 
