@@ -94,6 +94,10 @@ def lib():
         L.orc_frame.restype = C.c_int
         L.orc_frame_average.argtypes = [f64p, C.c_size_t]
         L.orc_frame_average.restype = C.c_double
+        L.orc_colorspace_coeffs.argtypes = [C.c_int32, C.c_int32, i32p]
+        L.orc_colorspace_coeffs.restype = C.c_int
+        L.orc_colorspace.argtypes = [u16p, u16p, C.c_size_t, C.c_int32, C.c_int32, C.c_int32]
+        L.orc_colorspace.restype = C.c_int
         _lib = L
     return _lib
 
@@ -249,3 +253,20 @@ def msssim_plane(tst: np.ndarray, ref: np.ndarray, bd: int):
 def frame_average(values) -> float:
     v = np.ascontiguousarray(values, np.float64)
     return lib().orc_frame_average(v, v.size)
+
+
+def colorspace_coeffs(space: int, to_rgb: bool) -> np.ndarray:
+    """Q16 matrix (3 x 3) of colour space 0..3 (BT.601, BT.709, SMPTE 240M, BT.2020)."""
+    m = np.zeros(9, np.int32)
+    if lib().orc_colorspace_coeffs(space, int(to_rgb), m):
+        raise ValueError("unknown colour space")
+    return m.reshape(3, 3)
+
+
+def colorspace(planes: np.ndarray, bd: int, space: int, to_rgb: bool) -> np.ndarray:
+    """Three planar uint16 planes (3, h, w) converted RGB -> YCbCr (to_rgb False) or back."""
+    src = np.ascontiguousarray(planes, np.uint16)
+    dst = np.zeros_like(src)
+    if lib().orc_colorspace(src.reshape(-1), dst.reshape(-1), src[0].size, bd, space, int(to_rgb)):
+        raise ValueError("unknown colour space")
+    return dst

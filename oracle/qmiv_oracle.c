@@ -500,3 +500,74 @@ int orc_frame(const void* p0, const void* p1, const orc_params* prm, orc_frame_r
 }
 
 double orc_frame_average(const double* v, size_t n) { return n ? orc_kbns(v, n) / (double)n : 0.0; }
+
+/* ---------------------------------------------------------------------------------------------
+   Colour space (libs/pmbb-core/src/xColorSpaceSTD.cpp:195-272). Q16 matrices of the four
+   standard luma/chroma spaces (index = eClrSpcLC, xCommonDefCORE.h:356-370: BT.601, BT.709,
+   SMPTE 240M, BT.2020), derived from the standards' luma weights (Kr, Kg, Kb):
+     RGB -> YCbCr: Y = Kr R + Kg G + Kb B, Cb = (B - Y) / (2 (1 - Kb)), Cr = (R - Y) / (2 (1 - Kr))
+     YCbCr -> RGB: R = Y + 2 (1 - Kr) Cr, B = Y + 2 (1 - Kb) Cb,
+                   G = Y - 2 Kb (1 - Kb) / Kg Cb - 2 Kr (1 - Kr) / Kg Cr
+   each coefficient scaled by 65536 and rounded half away from zero.
+   --------------------------------------------------------------------------------------------- */
+static const double k_luma_weights[4][3] = {
+    {0.299, 0.587, 0.114},    /* ITU-R BT.601 / SMPTE 170M */
+    {0.2126, 0.7152, 0.0722}, /* ITU-R BT.709 */
+    {0.212, 0.701, 0.087},    /* SMPTE 240M */
+    {0.2627, 0.678, 0.0593},  /* ITU-R BT.2020 */
+};
+
+static int32_t q16(double v) { return v >= 0.0 ? (int32_t)(v * 65536.0 + 0.5) : -(int32_t)(-v * 65536.0 + 0.5); }
+
+int orc_colorspace_coeffs(int32_t space, int32_t to_rgb, int32_t m[9]) {
+  if (space < 0 || space > 3) return -1;
+  const double kr = k_luma_weights[space][0], kg = k_luma_weights[space][1], kb = k_luma_weights[space][2];
+  double f[9];
+  if (!to_rgb) {
+    f[0] = kr;                        f[1] = kg;                        f[2] = kb;
+    f[3] = -kr / (2.0 * (1.0 - kb));  f[4] = -kg / (2.0 * (1.0 - kb));  f[5] = 0.5;
+    f[6] = 0.5;                       f[7] = -kg / (2.0 * (1.0 - kr));  f[8] = -kb / (2.0 * (1.0 - kr));
+  } else {
+    f[0] = 1.0; f[1] = 0.0;                                f[2] = 2.0 * (1.0 - kr);
+    f[3] = 1.0; f[4] = -2.0 * kb * (1.0 - kb) / kg;        f[5] = -2.0 * kr * (1.0 - kr) / kg;
+    f[6] = 1.0; f[7] = 2.0 * (1.0 - kb);                   f[8] = 0.0;
+  }
+  for (int i = 0; i < 9; i++) m[i] = q16(f[i]);
+  return 0;
+}
+
+static int32_t clipu(int32_t v, int32_t maxv) { return v < 0 ? 0 : (v > maxv ? maxv : v); }
+/* int32 arithmetic with two's-complement wrap and arithmetic right shift, as the reference's
+   int32 expressions compile. */
+static int32_t wrap_add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+static int32_t wrap_mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+static int32_t wrap_shl(int32_t a, int s) { return (int32_t)((uint32_t)a << s); }
+
+int orc_colorspace(const uint16_t* src, uint16_t* dst, size_t n, int32_t bd, int32_t space, int32_t to_rgb) {
+  int32_t m[9];
+  if (orc_colorspace_coeffs(space, to_rgb, m)) return -1;
+  const int32_t mid = 1 << (bd - 1), maxv = (1 << bd) - 1, add = 1 << 15;
+  const uint16_t *s0 = src, *s1 = src + n, *s2 = src + 2 * n;
+  uint16_t *d0 = dst, *d1 = dst + n, *d2 = dst + 2 * n;
+  for (size_t i = 0; i < n; i++) {
+    if (!to_rgb) {  /* ConvertRGB2YCbCr_I32 (xColorSpaceSTD.cpp:195-235): the 0.5 taps as << 15 */
+      const int32_t r = s0[i], g = s1[i], b = s2[i];
+      const int32_t ty = wrap_add(wrap_add(wrap_add(wrap_mul(m[0], r), wrap_mul(m[1], g)), wrap_mul(m[2], b)), add) >> 16;
+      const int32_t tu = wrap_add(wrap_add(wrap_add(wrap_mul(m[3], r), wrap_mul(m[4], g)), wrap_shl(b, 15)), add) >> 16;
+      const int32_t tv = wrap_add(wrap_add(wrap_add(wrap_shl(r, 15), wrap_mul(m[7], g)), wrap_mul(m[8], b)), add) >> 16;
+      d0[i] = (uint16_t)clipu(ty, maxv);
+      d1[i] = (uint16_t)clipu(tu + mid, maxv);
+      d2[i] = (uint16_t)clipu(tv + mid, maxv);
+    } else {        /* ConvertYCbCr2RGB_I32 (xColorSpaceSTD.cpp:236-272) */
+      const int32_t iy = s0[i], iu = (int32_t)s1[i] - mid, iv = (int32_t)s2[i] - mid;
+      const int32_t sy = wrap_add(wrap_shl(iy, 16), add);
+      const int32_t r = wrap_add(sy, wrap_mul(m[2], iv)) >> 16;
+      const int32_t g = wrap_add(wrap_add(sy, wrap_mul(m[4], iu)), wrap_mul(m[5], iv)) >> 16;
+      const int32_t b = wrap_add(sy, wrap_mul(m[7], iu)) >> 16;
+      d0[i] = (uint16_t)clipu(r, maxv);
+      d1[i] = (uint16_t)clipu(g, maxv);
+      d2[i] = (uint16_t)clipu(b, maxv);
+    }
+  }
+  return 0;
+}

@@ -114,6 +114,12 @@ int orc_frame(const void* p0, const void* p1, const orc_params* prm, orc_frame_r
 /* KBNS average over frames (apps/QMIV/src/xAppQMIV.h:256-260). */
 double orc_frame_average(const double* v, size_t n);
 
+/* Colour space: Q16 matrix of space 0..3 (BT.601, BT.709, SMPTE 240M, BT.2020) for RGB -> YCbCr
+   (to_rgb = 0) or YCbCr -> RGB (to_rgb = 1), row-major; and the conversion of three planar planes
+   of n samples (xColorSpaceSTD.cpp:195-272). */
+int orc_colorspace_coeffs(int32_t space, int32_t to_rgb, int32_t m[9]);
+int orc_colorspace(const uint16_t* src, uint16_t* dst, size_t n, int32_t bd, int32_t space, int32_t to_rgb);
+
 #ifdef __cplusplus
 }
 #endif

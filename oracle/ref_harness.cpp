@@ -21,6 +21,8 @@
 #include "xThreadPool.h"
 #include "xMathUtils.h"
 #include "xPixelOps.h"
+#include "xColorSpaceSTD.h"
+#include "xColorSpaceCoeff.h"
 
 #include <chrono>
 #include <cstdio>
@@ -182,7 +184,55 @@ double ms(Clock::time_point a, Clock::time_point b) {
 
 }  // namespace
 
+// -cstables: the reference's Q16 integer colour matrices, [space][rgb2ycbcr | ycbcr2rgb][3][3].
+// -cs DIR SPACE W H BD IN OUT: DIR 0 = RGB -> YCbCr, 1 = YCbCr -> RGB on three planar uint16
+// planes (xColorSpaceSTD::ConvertRGB2YCbCr_I32 / ConvertYCbCr2RGB_I32).
+static int colour_modes(int argc, char** argv) {
+  const std::string M = argv[1];
+  if (M == "-cstables") {
+    fputc('[', stdout);
+    for (int s = 0; s < 4; s++) {
+      fprintf(stdout, "%s[", s ? "," : "");
+      for (int d = 0; d < 2; d++) {
+        const auto& T = d == 0 ? xColorSpaceCoeff<int32>::c_RGB2YCbCr[s] : xColorSpaceCoeff<int32>::c_YCbCr2RGB[s];
+        fprintf(stdout, "%s[", d ? "," : "");
+        for (int r = 0; r < 3; r++)
+          for (int c = 0; c < 3; c++) fprintf(stdout, "%s%d", (r || c) ? "," : "", (int)T[r][c]);
+        fputc(']', stdout);
+      }
+      fputc(']', stdout);
+    }
+    fputs("]\n", stdout);
+    return 0;
+  }
+  if (M == "-cs" && argc == 9) {
+    const int Dir = atoi(argv[2]), Space = atoi(argv[3]), W = atoi(argv[4]), H = atoi(argv[5]), BD = atoi(argv[6]);
+    const size_t N = (size_t)W * H;
+    std::vector<uint16> In(3 * N), Out(3 * N);
+    FILE* F = fopen(argv[7], "rb");
+    if (!F || fread(In.data(), 2, 3 * N, F) != 3 * N) { fprintf(stderr, "cannot read %s\n", argv[7]); return 1; }
+    fclose(F);
+    if (Dir == 0)
+      xColorSpaceSTD::ConvertRGB2YCbCr_I32(Out.data(), Out.data() + N, Out.data() + 2 * N, In.data(), In.data() + N,
+                                           In.data() + 2 * N, W, W, W, H, BD, (eClrSpcLC)Space);
+    else
+      xColorSpaceSTD::ConvertYCbCr2RGB_I32(Out.data(), Out.data() + N, Out.data() + 2 * N, In.data(), In.data() + N,
+                                           In.data() + 2 * N, W, W, W, H, BD, (eClrSpcLC)Space);
+    F = fopen(argv[8], "wb");
+    if (!F || fwrite(Out.data(), 2, 3 * N, F) != 3 * N) { fprintf(stderr, "cannot write %s\n", argv[8]); return 1; }
+    fclose(F);
+    return 0;
+  }
+  return -1;
+}
+
 int main(int argc, char** argv) {
+  if (argc >= 2 && (std::string(argv[1]) == "-cstables" || std::string(argv[1]) == "-cs")) {
+    const int r = colour_modes(argc, argv);
+    if (r >= 0) return r;
+    usage();
+    return 2;
+  }
   if (argc == 2 && std::string(argv[1]) == "-filter") {
     // The SSIM window table as the reference stores it (xStructSimConsts.h:30-43), fp32 bit patterns.
     fputc('[', stdout);

@@ -79,9 +79,47 @@ SMALL_CASES = [
 ]
 
 
+COLOUR_W, COLOUR_H = 56, 40
+
+
+def colour_input(bd, seed):
+    """Seeded planar uint16 planes (3, 40, 56) with samples in [0, 2^bd)."""
+    import numpy as np
+    return np.random.default_rng(seed).integers(0, 1 << bd, size=(3, COLOUR_H, COLOUR_W), dtype=np.uint16)
+
+
+def colour_fixtures():
+    """The reference's Q16 colour matrices and its conversions of seeded planes
+    (xColorSpaceSTD::ConvertRGB2YCbCr_I32 / ConvertYCbCr2RGB_I32)."""
+    import hashlib
+    import numpy as np
+    tables = json.loads(subprocess.run([REF, "-cstables"], capture_output=True, text=True, check=True).stdout)
+    cases = []
+    with tempfile.TemporaryDirectory() as td:
+        for bd in (8, 10, 14, 16):
+            for space in range(4):
+                for to_rgb in (0, 1):
+                    seed = 1000 * bd + 10 * space + to_rgb
+                    src = colour_input(bd, seed)
+                    fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+                    src.astype("<u2").tofile(fi)
+                    subprocess.run([REF, "-cs", str(to_rgb), str(space), str(COLOUR_W), str(COLOUR_H), str(bd), fi, fo],
+                                   check=True)
+                    out = np.fromfile(fo, "<u2")
+                    cases.append({"bit_depth": bd, "space": space, "to_rgb": to_rgb, "seed": seed,
+                                  "width": COLOUR_W, "height": COLOUR_H,
+                                  "sha256_out": hashlib.sha256(out.tobytes()).hexdigest(),
+                                  "head": out[:8].tolist()})
+    return {"tables": tables, "cases": cases}
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("oracle/_ref/qmiv_ref is not built (make -C oracle ref)")
+    if "--colour" in sys.argv:
+        with open(os.path.join(HERE, "colorspace.json"), "w") as f:
+            json.dump(colour_fixtures(), f)
+        return
     fixtures = {}
     with tempfile.TemporaryDirectory() as td:
         # 1. Small synthetic cases: every metric, full per-row arrays, both WS settings.

@@ -284,3 +284,61 @@ def test_window_table_helper_matches_reference_bits():
     bits = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ssim_filter_bits.json")))
     table = np.array(bits, np.uint32).view(np.float32).reshape(11, 11)
     np.testing.assert_array_equal(ssim_window_table(), table[:10, :10].astype(np.float64))
+
+
+COLOUR = load("colorspace")
+
+
+@pytest.mark.parametrize("space", range(4))
+@pytest.mark.parametrize("to_rgb", [0, 1])
+def test_colour_tables_equal_reference(oracle, space, to_rgb):
+    """The Q16 matrices equal those printed by the reference build (xColorSpaceCoeff.cpp)."""
+    want = np.array(COLOUR["tables"][space][to_rgb], np.int32).reshape(3, 3)
+    np.testing.assert_array_equal(oracle.colorspace_coeffs(space, bool(to_rgb)), want)
+
+
+def test_colour_tables_rows_follow_from_luma_weights(oracle):
+    """Invariants of any RGB<->YCbCr matrix: the luma row sums to 1.0 (within rounding), the chroma
+    rows sum to zero, the inverse maps grey to grey and its product with the forward matrix is the
+    identity to Q16 rounding."""
+    for space in range(4):
+        fwd = oracle.colorspace_coeffs(space, False).astype(np.float64) / 65536
+        inv = oracle.colorspace_coeffs(space, True).astype(np.float64) / 65536
+        assert abs(fwd[0].sum() - 1.0) < 2e-5
+        assert abs(fwd[1].sum()) < 2e-5 and abs(fwd[2].sum()) < 2e-5
+        assert fwd[1, 2] == 0.5 and fwd[2, 0] == 0.5
+        np.testing.assert_allclose(inv[:, 0], 1.0)
+        np.testing.assert_allclose(inv @ fwd, np.eye(3), atol=1e-4)
+
+
+@pytest.mark.parametrize("case", COLOUR["cases"], ids=lambda c: f"b{c['bit_depth']}s{c['space']}r{c['to_rgb']}")
+def test_colour_conversion_matches_reference_outputs(oracle, case):
+    """Seeded planes converted by the reference's ConvertRGB2YCbCr_I32 / ConvertYCbCr2RGB_I32."""
+    src = np.random.default_rng(case["seed"]).integers(0, 1 << case["bit_depth"],
+                                                        size=(3, case["height"], case["width"]), dtype=np.uint16)
+    out = oracle.colorspace(src, case["bit_depth"], case["space"], bool(case["to_rgb"]))
+    assert out.reshape(-1)[:8].tolist() == case["head"]
+    assert hashlib.sha256(out.astype("<u2").tobytes()).hexdigest() == case["sha256_out"]
+
+
+@pytest.mark.parametrize("space", range(4))
+@pytest.mark.parametrize("w", [63, 64, 65, 127, 128, 129])
+def test_colour_round_trip_14bit_gradients(oracle, space, w):
+    """RGB -> YCbCr -> RGB of horizontal gradients (offsets 0, 100, 200) at 14 bits returns every
+    sample within 1 (test-core-xColorspace.cpp:107-125)."""
+    h = 65
+    x = np.arange(w, dtype=np.int64)
+    src = np.stack([np.broadcast_to((x + off) & 0x3FFF, (h, w)) for off in (0, 100, 200)]).astype(np.uint16)
+    back = oracle.colorspace(oracle.colorspace(src, 14, space, False), 14, space, True)
+    assert np.abs(back.astype(np.int32) - src.astype(np.int32)).max() <= 1
+
+
+def test_colour_grey_maps_to_mid_chroma(oracle):
+    """R = G = B = v converts to Y = v and Cb = Cr = 2^(bd-1) (chroma rows sum to zero). Up to 14
+    bits: at 16 bits the reference's int32 products wrap (65536 * v > 2^31), which the 16-bit golden
+    cases above pin."""
+    for bd in (8, 10, 14):
+        v = np.arange(0, 1 << bd, max(1, (1 << bd) // 97), dtype=np.uint16)
+        out = oracle.colorspace(np.stack([v, v, v])[:, None, :], bd, 1, False)
+        assert np.abs(out[0, 0].astype(np.int64) - v).max() <= 1
+        assert np.abs(out[1:].astype(np.int64) - (1 << (bd - 1))).max() <= 1
