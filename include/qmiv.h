@@ -246,6 +246,23 @@ qmiv_status qmiv_calc_ivssim_scp(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_
    Asynchronous on the context stream. */
 qmiv_status qmiv_pic_upsample(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst);
 
+/* Colour spaces of the RGB <-> YCbCr conversions: eClrSpcLC (libs/pmbb-core/src/xCommonDefCORE.h:356-364;
+   SMPTE 170M shares BT.601's matrix). */
+enum { QMIV_CLR_BT601 = 0, QMIV_CLR_BT709 = 1, QMIV_CLR_SMPTE240M = 2, QMIV_CLR_BT2020 = 3 };
+
+/* Converts a 4:4:4 picture between R,G,B planes and Y,Cb,Cr planes (to_rgb 0: RGB -> YCbCr,
+   1: YCbCr -> RGB) in the reference's Q16 integer arithmetic, bit-exact with
+   xColorSpaceSTD::ConvertRGB2YCbCr_I32 / ConvertYCbCr2RGB_I32 (libs/pmbb-core/src/xColorSpaceSTD.cpp:195-272)
+   and the matrices of xColorSpaceCoeff (libs/pmbb-core/src/xColorSpaceCoeff.cpp). dst may be src
+   (the reference app converts in place: apps/QMIV/src/xAppQMIV.cpp:778-798); otherwise it must be
+   a distinct 4:4:4 picture of the same size and bit depth. Asynchronous on the context stream. */
+qmiv_status qmiv_pic_colorspace(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst, int32_t space, int32_t to_rgb);
+
+/* Exchanges planes a and b (0..2) of a picture with equal plane sizes (4:4:4): xPicP::swapComponents
+   (libs/pmbb-core/src/xPic.cpp:158-164), which the reference app uses to reorder BGR and GBR
+   input to RGB (apps/QMIV/src/xAppQMIV.cpp:800-813). Asynchronous on the context stream. */
+qmiv_status qmiv_pic_swap_planes(qmiv_ctx* ctx, qmiv_pic* pic, int32_t a, int32_t b);
+
 /* ---------------------------------------------------------------------------------------------
    Batched frame sets (no counterpart in the reference, which runs its frame loop,
    apps/QMIV/src/xAppQMIV.cpp:661-935, once per view and process). A batch binds n picture pairs

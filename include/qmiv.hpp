@@ -91,6 +91,13 @@ class xPicP {
   int32 getMaxPelValue() const { return (1 << m_BitDepth) - 1; }
   bool isCompatible(const xPicP* Ref) const { return Ref && Ref->m_Size == m_Size && Ref->m_BitDepth == m_BitDepth; }
   qmiv_pic* handle() const { return m_Pic; }
+  qmiv_ctx* context() const { return m_Ctx->get(); }
+
+  // xPicP::swapComponents (libs/pmbb-core/src/xPic.cpp:158-164): exchanges two planes (0..2); the
+  // data moves on the device, the handle stays.
+  bool swapComponents(int32 CmpIdA, int32 CmpIdB) {
+    return qmiv_pic_swap_planes(m_Ctx->get(), m_Pic, CmpIdA, CmpIdB) == QMIV_OK;
+  }
 
   // xSeqBase::readFrame's unpack (libs/pmbb-core/src/xSeq.cpp:179-249): one packed planar frame
   // as read from a raw file (ChromaFormat 420 or 444) becomes this 4:4:4 picture; 4:2:0 chroma is
@@ -125,6 +132,22 @@ class xPicP {
   qmiv_pic* m_Staging = nullptr;  // packed 4:2:0 upload buffer of fromPacked
   int32V2 m_Size = {0, 0};
   int32 m_BitDepth = 0;
+};
+
+// eClrSpcLC (libs/pmbb-core/src/xCommonDefCORE.h:356-364), the standardised YCbCr spaces.
+enum class eClrSpcLC : int32 { BT601 = 0, SMPTE170M = 0, BT709 = 1, SMPTE240M = 2, BT2020 = 3 };
+
+// xColorSpace::ConvertRGB2YCbCr / ConvertYCbCr2RGB over whole 4:4:4 pictures, as the reference
+// app calls them (apps/QMIV/src/xAppQMIV.cpp:778-798): planes (R, G, B) <-> (Y, Cb, Cr), bit-exact
+// Q16 integer arithmetic, Dst may be Src.
+class xColorSpace {
+ public:
+  static void ConvertRGB2YCbCr(xPicP* Dst, const xPicP* Src, eClrSpcLC ClrSpc) {
+    xCheck(qmiv_pic_colorspace(Src->context(), Src->handle(), Dst->handle(), (int32)ClrSpc, 0), "qmiv_pic_colorspace");
+  }
+  static void ConvertYCbCr2RGB(xPicP* Dst, const xPicP* Src, eClrSpcLC ClrSpc) {
+    xCheck(qmiv_pic_colorspace(Src->context(), Src->handle(), Dst->handle(), (int32)ClrSpc, 1), "qmiv_pic_colorspace");
+  }
 };
 
 // Parameters shared by the processors: xGlobClrDiffPrms (xGlobClrDiff.h:17-26),

@@ -113,6 +113,8 @@ _SIGNATURES = {
     "qmiv_calc_ivssim_scp": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), _f64p, _f64p,
                                        _f64p]),
     "qmiv_pic_upsample": (C.c_int, [_ctx_p, _pic_p, _pic_p]),
+    "qmiv_pic_colorspace": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.c_int32, C.c_int32]),
+    "qmiv_pic_swap_planes": (C.c_int, [_ctx_p, _pic_p, C.c_int32, C.c_int32]),
     "qmiv_batch_options_default": (None, [C.POINTER(BatchOptions)]),
     "qmiv_batch_create": (C.c_int, [_ctx_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                     C.POINTER(Params), C.POINTER(C.c_uint32), C.POINTER(BatchOptions),

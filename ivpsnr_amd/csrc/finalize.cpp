@@ -88,6 +88,29 @@ int32_t gcd_max_diff(float coef, int32_t bit_depth) {
 
 double min2(double a, double b) { return (b < a) ? b : a; }
 
+bool colour_matrix(int32_t space, bool to_rgb, int32_t m[9]) {
+  static const double kLuma[4][3] = {
+      {0.299, 0.587, 0.114}, {0.2126, 0.7152, 0.0722}, {0.212, 0.701, 0.087}, {0.2627, 0.678, 0.0593}};
+  if (space < 0 || space > 3) return false;
+  const double kr = kLuma[space][0], kg = kLuma[space][1], kb = kLuma[space][2];
+  double f[9];
+  if (!to_rgb) {
+    const double cb = 2.0 * (1.0 - kb), cr = 2.0 * (1.0 - kr);
+    const double fwd[9] = {kr, kg, kb, -kr / cb, -kg / cb, 0.5, 0.5, -kg / cr, -kb / cr};
+    for (int i = 0; i < 9; i++) f[i] = fwd[i];
+  } else {
+    const double inv[9] = {1.0, 0.0, 2.0 * (1.0 - kr),
+                           1.0, -2.0 * kb * (1.0 - kb) / kg, -2.0 * kr * (1.0 - kr) / kg,
+                           1.0, 2.0 * (1.0 - kb), 0.0};
+    for (int i = 0; i < 9; i++) f[i] = inv[i];
+  }
+  for (int i = 0; i < 9; i++) {
+    const double v = 65536.0 * f[i];
+    m[i] = v >= 0.0 ? (int32_t)(v + 0.5) : -(int32_t)(-v + 0.5);
+  }
+  return true;
+}
+
 void ssim_window(double out[100]) {
   double g[11];
   for (int i = 0; i < 11; i++) g[i] = std::exp(-((double)((i - 5) * (i - 5))) / (2.0 * 1.5 * 1.5));

@@ -53,5 +53,11 @@ double msssim_combine(const double scores[5]);
 // std::min argument order (xCommonDefCORE.h:93).
 double min2(double a, double b);
 
+// Q16 RGB -> YCbCr (to_rgb false) or YCbCr -> RGB matrix, row-major, of colour space 0..3 (BT.601,
+// BT.709, SMPTE 240M, BT.2020): the textbook matrices from the luma weights (Kr, Kg, Kb) in
+// double, each entry scaled by 65536 and rounded half away from zero
+// (libs/pmbb-core/src/xColorSpaceCoeff.h:36-100,171-213). Returns false for an unknown space.
+bool colour_matrix(int32_t space, bool to_rgb, int32_t m[9]);
+
 }  // namespace fin
 }  // namespace qmiv

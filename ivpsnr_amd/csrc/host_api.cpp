@@ -1434,6 +1434,37 @@ qmiv_status qmiv_pic_upsample(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst)
   return QMIV_OK;
 }
 
+qmiv_status qmiv_pic_colorspace(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst, int32_t space, int32_t to_rgb) {
+  if (!ctx || !src || !dst) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (src->ctx != ctx || dst->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
+  if (src->g.cs != 0 || dst->g.cs != 0) return fail(QMIV_ERR_INVALID_ARG, "colour conversion needs 4:4:4 pictures");
+  if (dst->g.w != src->g.w || dst->g.h != src->g.h || dst->g.bd != src->g.bd)
+    return fail(QMIV_ERR_INVALID_ARG, "destination must match the source's size and bit depth");
+  if (to_rgb != 0 && to_rgb != 1) return fail(QMIV_ERR_INVALID_ARG, "to_rgb must be 0 or 1");
+  const int64_t bytes = (int64_t)src->g.frame_bytes;
+  const char* s0 = (const char*)src->dev;
+  const char* d0 = (const char*)dst->dev;
+  if (d0 != s0 && d0 < s0 + bytes && s0 < d0 + bytes)
+    return fail(QMIV_ERR_INVALID_ARG, "destination overlaps the source without being the same picture");
+  ColourArgs k;
+  if (!fin::colour_matrix(space, to_rgb != 0, k.m)) return fail(QMIV_ERR_INVALID_ARG, "colour space must be 0..3");
+  k.mid = 1 << (src->g.bd - 1);
+  k.maxv = (1 << src->g.bd) - 1;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_HIP(launch_colour(src->dev, dst->dev, src->g, k, to_rgb != 0, ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_swap_planes(qmiv_ctx* ctx, qmiv_pic* pic, int32_t a, int32_t b) {
+  if (!ctx || !pic) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (pic->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "picture belongs to another context");
+  if (pic->g.cs != 0) return fail(QMIV_ERR_INVALID_ARG, "plane exchange needs a 4:4:4 picture");
+  if (a < 0 || a > 2 || b < 0 || b > 2) return fail(QMIV_ERR_INVALID_ARG, "plane index must be 0..2");
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_HIP(launch_swap_planes(pic->dev, pic->g, a, b, ctx->stream));
+  return QMIV_OK;
+}
+
 
 void qmiv_batch_options_default(qmiv_batch_options* o) {
   if (!o) return;

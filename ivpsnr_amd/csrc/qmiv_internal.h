@@ -142,6 +142,16 @@ int cps_fast420_blocks_per_cu(int bytes_per_sample);
 
 // Nearest-neighbour 4:2:0 -> 4:4:4 upsampling (a copy for 4:4:4 sources) of a packed frame into a
 // planar 4:4:4 frame of the same sample type.
+// Q16 matrix (row-major) and range of an RGB <-> YCbCr conversion.
+struct ColourArgs {
+  int32_t m[9];
+  int32_t mid;   // 2^(bd - 1)
+  int32_t maxv;  // 2^bd - 1
+};
+
+hipError_t launch_colour(const void* src, void* dst, const Geom& g, const ColourArgs& k, bool to_rgb,
+                         hipStream_t stream);
+hipError_t launch_swap_planes(void* pic, const Geom& g, int a, int b, hipStream_t stream);
 hipError_t launch_upsample(const void* src, const Geom& gs, void* dst, hipStream_t stream);
 // dst[i] = (uint8_t)src[i] for i < n.
 hipError_t launch_narrow(const uint16_t* src, int64_t n, uint8_t* dst, hipStream_t stream);

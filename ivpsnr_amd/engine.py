@@ -236,6 +236,21 @@ class Engine:
         self.sync()
         return dst
 
+    def colorspace(self, src: Picture, space: int, to_rgb: bool, dst: Picture | None = None) -> Picture:
+        """RGB -> YCbCr (to_rgb False) or YCbCr -> RGB of a 4:4:4 picture, into `dst` (default: in
+        place), in the reference's Q16 arithmetic (xColorSpace::ConvertRGB2YCbCr / ConvertYCbCr2RGB)."""
+        dst = src if dst is None else dst
+        check(lib().qmiv_pic_colorspace(self.ctx, src.handle, dst.handle, int(space), int(bool(to_rgb))),
+              "qmiv_pic_colorspace")
+        self.sync()
+        return dst
+
+    def swap_planes(self, pic: Picture, a: int, b: int) -> Picture:
+        """Exchanges planes a and b of a 4:4:4 picture in place (xPicP::swapComponents)."""
+        check(lib().qmiv_pic_swap_planes(self.ctx, pic.handle, int(a), int(b)), "qmiv_pic_swap_planes")
+        self.sync()
+        return pic
+
     def close(self):
         for ref in getattr(self, "_batches", []):
             bt = ref()

@@ -1,12 +1,13 @@
 // The reference application's per-frame processor calls (apps/QMIV/src/xAppQMIV.cpp:693-727,
 // 831-935), made through the C++ processor classes of include/qmiv.hpp on the MI355X engine.
 //
-//   test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1
+//   test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1 OUT_CLR
 //
 // FILE0 / FILE1 hold one packed planar frame each (InputFile0 / InputFile1). Prints one JSON
-// object with every metric (doubles in %.17g) and writes the two shift-compensated pictures as
-// planar 4:4:4 16-bit files. Checks the identities the processor API promises and exits 1 on a
+// object with every metric (doubles in %.17g) and writes the two shift-compensated pictures and
+// picture 0 converted RGB -> YCbCr (BT.709) as planar 4:4:4 16-bit files. Checks the identities the processor API promises and exits 1 on a
 // failed check.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -35,7 +36,7 @@ static std::vector<uint8_t> readFile(const char* Path, size_t Bytes) {
   return B;
 }
 
-static void writeScp(const xPicP& Pic, const char* Path) {
+static std::vector<uint16_t> planes16(const xPicP& Pic) {
   const size_t N = 3 * (size_t)Pic.getArea();
   std::vector<uint16_t> Out(N);
   if (Pic.getBitDepth() <= 8) {
@@ -45,6 +46,12 @@ static void writeScp(const xPicP& Pic, const char* Path) {
   } else {
     Pic.download(Out.data());
   }
+  return Out;
+}
+
+static void writeScp(const xPicP& Pic, const char* Path) {
+  const std::vector<uint16_t> Out = planes16(Pic);
+  const size_t N = Out.size();
   FILE* F = std::fopen(Path, "wb");
   if (!F || std::fwrite(Out.data(), 2, N, F) != N) std::exit(2);
   std::fclose(F);
@@ -55,8 +62,8 @@ static void printV4(const char* Name, const flt64V4& V) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 10) {
-    std::fprintf(stderr, "usage: test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1\n");
+  if (argc != 11) {
+    std::fprintf(stderr, "usage: test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1 OUT_CLR\n");
     return 2;
   }
   const int32 W = std::atoi(argv[1]), H = std::atoi(argv[2]), BD = std::atoi(argv[3]), CF = std::atoi(argv[4]);
@@ -122,6 +129,21 @@ int main(int argc, char** argv) {
       Threw = E.status() == QMIV_ERR_INVALID_ARG;
     }
     CHECK(Threw);
+
+    // xAppQMIV.cpp:778-813 with picture 0 taken as R, G, B planes: to YCbCr out of place; back to
+    // RGB in place stays within the conversion's rounding; two plane exchanges restore it.
+    xPicP Clr(Ctx, {W, H}, BD);
+    xColorSpace::ConvertRGB2YCbCr(&Clr, &PicInP0, eClrSpcLC::BT709);
+    writeScp(Clr, argv[10]);
+    const std::vector<uint16_t> Before = planes16(Clr);
+    CHECK(Clr.swapComponents(0, 2));
+    const std::vector<uint16_t> Swapped = planes16(Clr);
+    const size_t Area = (size_t)Clr.getArea();
+    CHECK(std::equal(Swapped.begin(), Swapped.begin() + Area, Before.begin() + 2 * Area));
+    CHECK(Clr.swapComponents(2, 0));
+    CHECK(planes16(Clr) == Before);
+    xColorSpace::ConvertYCbCr2RGB(&Clr, &Clr, eClrSpcLC::BT709);
+    CHECK(!(planes16(Clr) == Before));
 
     writeScp(PicSCP0, argv[8]);
     writeScp(PicSCP1, argv[9]);

@@ -1,6 +1,6 @@
 """The C++ processor classes (include/qmiv.hpp) on the GPU, in the reference application's call
 order, against the oracle: PSNR family and GCD bit-exact, SSIM family to 1e-9 relative, the
-shift-compensated pictures bit-exact. The program also asserts the processor API's own
+shift-compensated pictures and the RGB -> YCbCr conversion bit-exact. The program also asserts the processor API's own
 identities (auto-GCD IV-PSNR, two- vs four-argument IV-SSIM, error on incompatible pictures)."""
 import json
 import os
@@ -28,11 +28,12 @@ def test_processor_classes(oracle, tmp_path, w, h, bd, cf, erp, seed):
         pytest.fail(f"{BIN} not built (make -C tests/cpp)")
     pair = make_pair(w, h, bd, seed, chroma_format=cf)
     f0, f1 = tmp_path / "p0.yuv", tmp_path / "p1.yuv"
-    s0, s1 = tmp_path / "scp0.bin", tmp_path / "scp1.bin"
+    s0, s1, sc = tmp_path / "scp0.bin", tmp_path / "scp1.bin", tmp_path / "clr.bin"
     f0.write_bytes(pair.ref.tobytes())
     f1.write_bytes(pair.tst.tobytes())
     print(f"case {w}x{h} bd{bd} cf{cf} erp{int(erp)} seed{seed}", flush=True)
-    r = subprocess.run([BIN, str(w), str(h), str(bd), str(cf), str(f0), str(f1), str(int(erp)), str(s0), str(s1)],
+    r = subprocess.run([BIN, str(w), str(h), str(bd), str(cf), str(f0), str(f1), str(int(erp)), str(s0), str(s1),
+                        str(sc)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = json.loads(r.stdout)
@@ -51,3 +52,4 @@ def test_processor_classes(oracle, tmp_path, w, h, bd, cf, erp, seed):
     n = 3 * w * h
     np.testing.assert_array_equal(np.fromfile(s0, np.uint16, n).reshape(3, h, w), scp0)
     np.testing.assert_array_equal(np.fromfile(s1, np.uint16, n).reshape(3, h, w), scp1)
+    np.testing.assert_array_equal(np.fromfile(sc, np.uint16, n).reshape(3, h, w), oracle.colorspace(P0, bd, 1, False))
