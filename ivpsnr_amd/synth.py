@@ -133,6 +133,40 @@ def make_pair(width: int, height: int, bit_depth: int, seed: int, *, chroma_form
     return FramePair(width, height, bit_depth, chroma_format, ref, tst)
 
 
+def make_mask(width: int, height: int, bit_depth: int, seed: int, *, chroma_format: int = 420,
+              kind: str = "objects", objects: int = 4) -> np.ndarray:
+    """One packed mask frame whose luma plane selects the pels of masked mode.
+
+    kind "objects": a few random ellipses at 2^bd - 1 on zero (the binary object masks of masked
+    mode); "soft": the same ellipses holding random values in [1, 2^bd - 1]; "zero" / "full":
+    every pel 0 / 2^bd - 1. Chroma planes hold the luma plane's top-left samples.
+    """
+    if chroma_format == 420 and (width % 2 or height % 2):
+        raise ValueError("4:2:0 needs even width and height")
+    max_value = (1 << bit_depth) - 1
+    rng = np.random.default_rng(np.random.SeedSequence([int(seed), width, height, bit_depth, 77]))
+    if kind == "zero":
+        y = np.zeros((height, width), np.int64)
+    elif kind == "full":
+        y = np.full((height, width), max_value, np.int64)
+    elif kind in ("objects", "soft"):
+        yy, xx = np.mgrid[0:height, 0:width]
+        inside = np.zeros((height, width), bool)
+        for _ in range(objects):
+            cy, cx = rng.uniform(0, height), rng.uniform(0, width)
+            ry, rx = rng.uniform(0.08, 0.3) * height, rng.uniform(0.05, 0.2) * width
+            inside |= ((yy - cy) / ry) ** 2 + ((xx - cx) / rx) ** 2 <= 1.0
+        if kind == "objects":
+            y = np.where(inside, max_value, 0)
+        else:
+            y = np.where(inside, rng.integers(1, max_value + 1, size=(height, width)), 0)
+    else:
+        raise ValueError(f"unknown mask kind {kind!r}")
+    c = y[::2, ::2] if chroma_format == 420 else y
+    dt = np.uint8 if bit_depth <= 8 else np.uint16
+    return np.concatenate([y.astype(dt).ravel(), c.astype(dt).ravel(), c.astype(dt).ravel()])
+
+
 # Workload presets (frame size, bit depth, frames, generator knobs).
 CONFIGS = {
     # plain IV-PSNR + PSNR, 8-bit 1080p

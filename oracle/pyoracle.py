@@ -55,6 +55,18 @@ class FrameResult(C.Structure):
         return out
 
 
+class MaskedResult(C.Structure):
+    _fields_ = [
+        ("in_range", C.c_int32 * 3), ("exact", C.c_int32 * 3), ("num_non_masked", C.c_int32),
+        ("gcd", C.c_int32 * 4),
+        ("psnr", C.c_double * 3), ("psnr_pic", C.c_double),
+        ("wspsnr", C.c_double * 3), ("wspsnr_pic", C.c_double),
+        ("ivpsnr", C.c_double), ("r2t", C.c_double), ("t2r", C.c_double),
+        ("cps_sum", (C.c_uint64 * 3) * 2),
+    ]
+    as_dict = FrameResult.as_dict
+
+
 _lib = None
 
 
@@ -92,6 +104,14 @@ def lib():
         L.orc_frame.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Params), C.POINTER(FrameResult), C.c_void_p,
                                 C.c_void_p]
         L.orc_frame.restype = C.c_int
+        L.orc_frame_masked.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(Params),
+                                       C.POINTER(MaskedResult)]
+        L.orc_frame_masked.restype = C.c_int
+        L.orc_gcd_masked.argtypes = [P3, P3, u16p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                     C.POINTER(C.c_float * 4), i32p]
+        L.orc_cps_rows_masked.argtypes = [P3, P3, u16p, C.c_int32, C.c_int32, i32p, C.c_int32, i32p, u64p]
+        L.orc_psnr_from_masked_ssd.argtypes = [C.c_double, C.c_int32, C.c_int32, C.c_int32]
+        L.orc_psnr_from_masked_ssd.restype = C.c_double
         L.orc_frame_average.argtypes = [f64p, C.c_size_t]
         L.orc_frame_average.restype = C.c_double
         L.orc_colorspace_coeffs.argtypes = [C.c_int32, C.c_int32, i32p]
@@ -137,6 +157,42 @@ def frame(p0: np.ndarray, p1: np.ndarray, prm: Params, want_scp: bool = False):
     if want_scp:
         return r, scp0.reshape(3, prm.height, prm.width), scp1.reshape(3, prm.height, prm.width)
     return r
+
+
+def frame_masked(p0: np.ndarray, p1: np.ndarray, mask: np.ndarray, bdm: int, cfm: int, prm: Params) -> MaskedResult:
+    """Run one frame pair with a mask (packed arrays) through the oracle's masked flow."""
+    r = MaskedResult()
+    p0, p1, mask = (np.ascontiguousarray(a) for a in (p0, p1, mask))
+    rc = lib().orc_frame_masked(p0.ctypes.data, p1.ctypes.data, mask.ctypes.data, bdm, cfm, C.byref(prm), C.byref(r))
+    if rc != 0:
+        raise ValueError(f"orc_frame_masked failed ({rc})")
+    return r
+
+
+def cps_rows_masked(x: np.ndarray, y: np.ndarray, mask: np.ndarray, g, B: int = 2, wts=(4, 1, 1, 0)) -> np.ndarray:
+    """Masked CPS rows (3, h) of 4:4:4 planes x, y (3, h, w) with a mask plane (h, w)."""
+    _, h, w = x.shape
+    x = np.ascontiguousarray(x, np.uint16)
+    y = np.ascontiguousarray(y, np.uint16)
+    mask = np.ascontiguousarray(mask, np.uint16)
+    rows = np.zeros(3 * h, np.uint64)
+    lib().orc_cps_rows_masked(_planes3(x), _planes3(y), mask.reshape(-1), w, h, np.asarray(g, np.int32),
+                              B, np.asarray(wts, np.int32), rows)
+    return rows.reshape(3, h)
+
+
+def gcd_masked(p0: np.ndarray, p1: np.ndarray, mask: np.ndarray, bd: int, bdm: int, nnm: int,
+               ucoef=(0.01, 0.01, 0.01, 0.0)) -> list:
+    _, h, w = p0.shape
+    g = np.zeros(4, np.int32)
+    uc = (C.c_float * 4)(*ucoef)
+    lib().orc_gcd_masked(_planes3(np.ascontiguousarray(p0, np.uint16)), _planes3(np.ascontiguousarray(p1, np.uint16)),
+                         np.ascontiguousarray(mask, np.uint16).reshape(-1), w, h, bd, bdm, nnm, C.byref(uc), g)
+    return g.tolist()
+
+
+def psnr_from_masked_ssd(ssd: float, nnm: int, bd: int, bdm: int) -> float:
+    return lib().orc_psnr_from_masked_ssd(ssd, nnm, bd, bdm)
 
 
 def unpack(packed: np.ndarray, w: int, h: int, bd: int, cf: int = 420) -> np.ndarray:

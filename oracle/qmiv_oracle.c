@@ -502,6 +502,202 @@ int orc_frame(const void* p0, const void* p1, const orc_params* prm, orc_frame_r
 double orc_frame_average(const double* v, size_t n) { return n ? orc_kbns(v, n) / (double)n : 0.0; }
 
 /* ---------------------------------------------------------------------------------------------
+   Masked mode
+   --------------------------------------------------------------------------------------------- */
+
+/* _mm_cvtsd_si32 under round-to-nearest-even (xCommonDefCORE.h:130): NaN and results outside
+   int32 give the "integer indefinite" value INT32_MIN. */
+static int32_t cvtsd_si32(double v) {
+  const double r = rint(v);
+  if (!(r >= -2147483648.0 && r <= 2147483647.0)) return (int32_t)0x80000000u;
+  return (int32_t)r;
+}
+
+void orc_gcd_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* mask, int32_t w,
+                    int32_t h, int32_t bd, int32_t bdm, int32_t nnm, const float ucoef[4], int32_t g[4]) {
+  const int32_t mx = ORC_MAXV(bd);
+  const size_t n = (size_t)w * (size_t)h;
+  /* (flt64)((int64)NumNonMasked * (int64)MaxMsk) */
+  const double denom = (double)((int64_t)nnm * (int64_t)ORC_MAXV(bdm));
+  for (int c = 0; c < 4; c++) {
+    const int32_t maxdiff = (int32_t)rintf(ucoef[c] * (float)mx);
+    int64_t sd = 0;
+    if (c < 3) {
+      /* xDistortionSTD::CalcWeightedSD (xDistortionSTD.cpp:69-80): each (Tst - Ref) * Msk is an
+         int32 product (wrapping), accumulated in int64. Tst = P1, Ref = P0 here. */
+      for (size_t i = 0; i < n; i++) {
+        const int32_t d = (int32_t)p1[c][i] - (int32_t)p0[c][i];
+        sd += (int64_t)(int32_t)((uint32_t)d * (uint32_t)mask[i]);
+      }
+    }
+    const int32_t r = cvtsd_si32((double)sd / denom);
+    g[c] = imax(-maxdiff, imin(r, maxdiff));
+  }
+}
+
+void orc_cps_rows_masked(const uint16_t* const x[3], const uint16_t* const y[3], const uint16_t* mask, int32_t w,
+                         int32_t h, const int32_t g[4], int32_t B, const int32_t wts[4], uint64_t* rows) {
+  for (int32_t py = 0; py < h; py++) {
+    uint64_t row[3] = {0, 0, 0};
+    for (int32_t px = 0; px < w; px++) {
+      const size_t o = (size_t)py * w + px;
+      const uint32_t m = mask[o];
+      if (m == 0) continue; /* masked pel: skipped */
+      int32_t t[4];
+      for (int c = 0; c < 3; c++) t[c] = (int32_t)x[c][o] + g[c];
+      t[3] = g[3];
+      int32_t best_err = 0x7fffffff;
+      uint32_t best_dist[3] = {0, 0, 0};
+      for (int32_t dy = -B; dy <= B; dy++) {
+        const int32_t qy = clampi(py + dy, 0, h - 1);
+        for (int32_t dx = -B; dx <= B; dx++) {
+          const int32_t qx = clampi(px + dx, 0, w - 1);
+          const size_t q = (size_t)qy * w + qx;
+          if (mask[q] == 0) continue; /* masked candidate (the margin replicates the mask's edge) */
+          uint32_t dist[4], err = 0;
+          for (int c = 0; c < 4; c++) {
+            const int32_t r = c < 3 ? (int32_t)y[c][q] : 0;
+            const uint32_t d = (uint32_t)(t[c] - r);
+            dist[c] = d * d;                    /* (Diff).getVecPow2() in int32 */
+            err += dist[c] * (uint32_t)wts[c];  /* (Dist * CmpWeights).getSum() */
+          }
+          if ((int32_t)err < best_err) {
+            best_err = (int32_t)err;
+            for (int c = 0; c < 3; c++) best_dist[c] = dist[c];
+          }
+        }
+      }
+      /* RowDist += ((uint64V4)Dist) * CurrMskValue: sign-extending conversion, uint64 product */
+      for (int c = 0; c < 3; c++) row[c] += (uint64_t)(int64_t)(int32_t)best_dist[c] * (uint64_t)m;
+    }
+    for (int c = 0; c < 3; c++) rows[(size_t)c * h + py] = row[c];
+  }
+}
+
+double orc_psnr_from_masked_ssd(double ssd, int32_t nnm, int32_t bd, int32_t bdm) {
+  /* (int64)NumNonMasked * xPow2(MaxValuePic) * MaxValueMsk, int64 arithmetic (wrapping) */
+  const uint64_t mx = (uint64_t)(int64_t)nnm * ((uint64_t)ORC_MAXV(bd) * (uint64_t)ORC_MAXV(bd)) * (uint64_t)ORC_MAXV(bdm);
+  return ssd > 0 ? 10.0 * log10((double)(int64_t)mx / ssd) : 1.7976931348623157e308;
+}
+
+/* xDistortionSTD::CalcWeightedSSD per row (xDistortionSTD.cpp:81-86): (uint64)xPow2(int32 d) * m. */
+static void ssd_rows_masked(const uint16_t* a, const uint16_t* b, const uint16_t* mask, int32_t w, int32_t h,
+                            uint64_t* rows) {
+  for (int32_t yy = 0; yy < h; yy++) {
+    uint64_t s = 0;
+    for (int32_t xx = 0; xx < w; xx++) {
+      const size_t o = (size_t)yy * w + xx;
+      const uint32_t d = (uint32_t)((int32_t)a[o] - (int32_t)b[o]);
+      s += (uint64_t)(int64_t)(int32_t)(d * d) * (uint64_t)mask[o];
+    }
+    rows[yy] = s;
+  }
+}
+
+int orc_frame_masked(const void* p0, const void* p1, const void* mask, int32_t bdm, int32_t cfm,
+                     const orc_params* prm, orc_masked_result* out) {
+  const int32_t w = prm->width, h = prm->height, bd = prm->bit_depth;
+  const size_t n = (size_t)w * (size_t)h;
+  const int32_t area = w * h;
+  const int m = prm->metrics;
+  memset(out, 0, sizeof(*out));
+  if (m & (ORC_SSIM | ORC_MSSSIM | ORC_IVSSIM)) return -3; /* xAppQMIV.cpp:337 */
+  uint16_t* buf = (uint16_t*)malloc(sizeof(uint16_t) * n * 9);
+  if (!buf) return -2;
+  const uint16_t* P0[3] = {buf, buf + n, buf + 2 * n};
+  const uint16_t* P1[3] = {buf + 3 * n, buf + 4 * n, buf + 5 * n};
+  const uint16_t* M = buf + 6 * n; /* luma plane of the mask (eCmp::LM) */
+  if (orc_unpack(p0, w, h, bd, prm->chroma_format, buf, buf + n, buf + 2 * n) ||
+      orc_unpack(p1, w, h, bd, prm->chroma_format, buf + 3 * n, buf + 4 * n, buf + 5 * n) ||
+      orc_unpack(mask, w, h, bdm, cfm, buf + 6 * n, buf + 7 * n, buf + 8 * n)) {
+    free(buf);
+    return -1;
+  }
+  out->in_range[0] = orc_in_range(buf, (int32_t)(3 * n), bd);
+  out->in_range[1] = orc_in_range(buf + 3 * n, (int32_t)(3 * n), bd);
+  out->in_range[2] = orc_in_range(buf + 6 * n, (int32_t)(3 * n), bdm);
+  for (int c = 0; c < 3; c++) out->exact[c] = memcmp(P0[c], P1[c], n * sizeof(uint16_t)) == 0;
+  int32_t nnm = 0; /* xPixelOpsSTD::CountNonZero (xPixelOpsSTD.cpp:273-284) */
+  for (size_t i = 0; i < n; i++) nnm += M[i] != 0;
+  out->num_non_masked = nnm;
+
+  if (m & ORC_IVPSNR) orc_gcd_masked(P0, P1, M, w, h, bd, bdm, nnm, prm->unnoticeable_coef, out->gcd);
+  const int32_t* g = out->gcd;
+  const int32_t ng[4] = {-g[0], -g[1], -g[2], -g[3]};
+
+  const double fake = orc_psnr_from_ssd(1.0, area, bd);
+  double* ws = NULL;
+  double corr = 1.0;
+  if (prm->erp) {
+    ws = (double*)malloc(sizeof(double) * (size_t)h);
+    orc_ws_init(h, prm->lat_range_deg, ws, &corr);
+  }
+  uint64_t* rows = (uint64_t*)malloc(sizeof(uint64_t) * 3 * (size_t)h);
+  double plain[3] = {0, 0, 0};
+  if (m & (ORC_PSNR | ORC_WSPSNR)) {
+    /* xPSNR::calcPicPSNRM (libs/pmbb-ivqm/src/xPSNR.cpp:42-71,84-136) */
+    for (int c = 0; c < 3; c++) {
+      ssd_rows_masked(P1[c], P0[c], M, w, h, rows);
+      uint64_t s = 0;
+      for (int32_t yy = 0; yy < h; yy++) s += rows[yy];
+      plain[c] = orc_psnr_from_masked_ssd((double)s, nnm, bd, bdm);
+    }
+  }
+  if (m & ORC_PSNR) {
+    for (int c = 0; c < 3; c++) out->psnr[c] = out->exact[c] ? fake : plain[c];
+    out->psnr_pic = pic_value(out->psnr, prm->cmp_weights_average);
+  }
+  if (m & ORC_WSPSNR) {
+    /* xWSPSNR::calcPicWSPSNRM (libs/pmbb-ivqm/src/xWSPSNR.cpp:61-97,127-155): plain masked PSNR
+       without WS; with WS, KBNS over rows of rowSSD * w[y], times the correction. */
+    for (int c = 0; c < 3; c++) {
+      double v = plain[c];
+      if (ws) {
+        ssd_rows_masked(P1[c], P0[c], M, w, h, rows);
+        double s = 0.0, k = 0.0;
+        for (int32_t yy = 0; yy < h; yy++) kbns_acc(&s, &k, (double)rows[yy] * ws[yy]);
+        v = orc_psnr_from_masked_ssd((s + k) * corr, nnm, bd, bdm);
+      }
+      out->wspsnr[c] = out->exact[c] ? fake : v;
+    }
+    out->wspsnr_pic = pic_value(out->wspsnr, prm->cmp_weights_average);
+  }
+  if (m & ORC_IVPSNR) {
+    /* xIVPSNRM::calcPicIVPSNRM(I0, I1, Msk, NNM, GCD) (libs/pmbb-ivqm/src/xIVPSNR.cpp:152-209):
+       R2T = Q(P1 - g in P0), T2R = Q(P0 + g in P1). */
+    const int32_t* wa = prm->cmp_weights_average;
+    const double inv = 1.0 / (double)(wa[0] + wa[1] + wa[2] + wa[3]);
+    for (int d = 0; d < 2; d++) {
+      orc_cps_rows_masked(d == 0 ? P1 : P0, d == 0 ? P0 : P1, M, w, h, d == 0 ? ng : g, prm->search_range,
+                          prm->cmp_weights_search, rows);
+      double q[4] = {0, 0, 0, 0};
+      for (int c = 0; c < 3; c++) {
+        uint64_t s = 0;
+        for (int32_t yy = 0; yy < h; yy++) s += rows[(size_t)c * h + yy];
+        out->cps_sum[d][c] = s;
+        double err;
+        if (ws) {
+          double a = 0.0, k = 0.0;
+          for (int32_t yy = 0; yy < h; yy++) kbns_acc(&a, &k, (double)rows[(size_t)c * h + yy] * ws[yy]);
+          err = a + k;
+        } else {
+          err = (double)s;
+        }
+        q[c] = orc_psnr_from_masked_ssd(err > 0 ? err : 1.0, nnm, bd, bdm);
+      }
+      const double v = (((q[0] * (double)wa[0] + q[1] * (double)wa[1]) + q[2] * (double)wa[2]) + q[3] * (double)wa[3]) * inv;
+      if (d == 0) out->r2t = v;
+      else out->t2r = v;
+    }
+    out->ivpsnr = dmin(out->r2t, out->t2r);
+  }
+  free(rows);
+  free(ws);
+  free(buf);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------
    Colour space (libs/pmbb-core/src/xColorSpaceSTD.cpp:195-272). Q16 matrices of the four
    standard luma/chroma spaces (index = eClrSpcLC, xCommonDefCORE.h:356-370: BT.601, BT.709,
    SMPTE 240M, BT.2020), derived from the standards' luma weights (Kr, Kg, Kb):

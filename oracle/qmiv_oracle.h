@@ -111,6 +111,38 @@ double orc_msssim_plane(const uint16_t* tst, const uint16_t* ref, int32_t w, int
 int orc_frame(const void* p0, const void* p1, const orc_params* prm, orc_frame_result* out, uint16_t* scp0,
               uint16_t* scp1);
 
+/* Masked mode (apps/QMIV/src/xAppQMIV.cpp:319-337,695,825-889): PSNR, WS-PSNR and IV-PSNR over
+   the pels whose mask (luma plane of the mask picture) is non-zero, each weighted by the mask
+   value. r2t / t2r carry the masked processor's own labels (libs/pmbb-ivqm/src/xIVPSNR.cpp:152-165):
+   R2T searches P1 - g inside P0, T2R searches P0 + g inside P1. cps_sum[0] is R2T. */
+typedef struct {
+  int32_t in_range[3];        /* P0, P1, mask */
+  int32_t exact[3];
+  int32_t num_non_masked;
+  int32_t gcd[4];
+  double psnr[3], psnr_pic;
+  double wspsnr[3], wspsnr_pic;
+  double ivpsnr, r2t, t2r;
+  uint64_t cps_sum[2][3];
+} orc_masked_result;
+
+/* xGlobClrDiff::CalcGlobalColorDiffM (libs/pmbb-ivqm/src/xGlobClrDiff.cpp:52-83) as the processor
+   calls it (xGlobClrDiff.h:42): g = clip(cvt(sum((p1 - p0) * m) / (nnm * maxm))). */
+void orc_gcd_masked(const uint16_t* const p0[3], const uint16_t* const p1[3], const uint16_t* mask, int32_t w,
+                    int32_t h, int32_t bd, int32_t bdm, int32_t nnm, const float ucoef[4], int32_t g[4]);
+
+/* xCorrespPixelShiftSTD::CalcDistAsymmetricRowM + FindBestPixelWithinBlockM
+   (libs/pmbb-ivqm/src/xCorrespPixelShiftSTD.cpp:80-144): rows[c*h + y] (uint64). */
+void orc_cps_rows_masked(const uint16_t* const x[3], const uint16_t* const y[3], const uint16_t* mask, int32_t w,
+                         int32_t h, const int32_t g[4], int32_t B, const int32_t wts[4], uint64_t* rows);
+
+/* xPSNR::CalcPSNRfromMaskedSSD (libs/pmbb-ivqm/src/xPSNR.cpp:148-155). */
+double orc_psnr_from_masked_ssd(double ssd, int32_t nnm, int32_t bd, int32_t bdm);
+
+/* One frame of the masked flow. mask is a packed frame of bit depth bdm and chroma format cfm. */
+int orc_frame_masked(const void* p0, const void* p1, const void* mask, int32_t bdm, int32_t cfm,
+                     const orc_params* prm, orc_masked_result* out);
+
 /* KBNS average over frames (apps/QMIV/src/xAppQMIV.h:256-260). */
 double orc_frame_average(const double* v, size_t n);
 

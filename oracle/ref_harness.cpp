@@ -50,6 +50,12 @@ class ProbeIVPSNR : public xIVPSNRM {
     Rows = m_RowDistsV4;
     return Q;
   }
+  flt64 qualM(const xPicI* T, const xPicI* R, const xPicP* M, const int32V4& G, int32 NNM,
+              std::vector<uint64V4>& Rows) {
+    flt64 Q = xCalcQualAsymmetricPicM(T, R, M, G, NNM);
+    Rows = m_RowDistsV4;
+    return Q;
+  }
   const std::vector<uint64>& wsRows(int C) const { return m_RowDistortions[C]; }
   const std::vector<flt64>& weights() const { return m_EquirectangularWeights; }
   flt64 correction() const { return m_DistortionCorrection; }
@@ -74,7 +80,8 @@ class ProbeSSIM : public xIVSSIM {
 
 struct Args {
   int W = 0, H = 0, BD = 8, CF = 420;
-  std::string I0, I1;
+  std::string I0, I1, IM;  // IM: mask file (masked mode, xAppQMIV.cpp:319-337)
+  int BDM = 8, CFM = 420;
   int NumFrames = 0;
   bool Erp = false;
   int Lon = 360, Lat = 180;
@@ -107,7 +114,7 @@ bool parseFlt4(const char* S, flt32V4& V) {
 
 void usage() {
   fprintf(stderr,
-          "qmiv_ref -w W -h H -bd BD -cf 420|444 -i0 FILE -i1 FILE [-n FRAMES] [-erp LON LAT]\n"
+          "qmiv_ref -w W -h H -bd BD -cf 420|444 -i0 FILE -i1 FILE [-im MASK -bdm BD -cfm CF] [-n FRAMES] [-erp LON LAT]\n"
           "         [-sr B] [-cws a:b:c:d] [-cwa a:b:c:d] [-ucf a:b:c:d] [-nth T] [-ilp 0|1]\n"
           "         [-ml PSNR,WSPSNR,IVPSNR,SSIM,MSSSIM,IVSSIM|All] [-rows] [-pics] [-time]\n");
 }
@@ -122,6 +129,9 @@ bool parse(int argc, char** argv, Args& A) {
     else if (K == "-cf") { need(1); A.CF = atoi(argv[++i]); }
     else if (K == "-i0") { need(1); A.I0 = argv[++i]; }
     else if (K == "-i1") { need(1); A.I1 = argv[++i]; }
+    else if (K == "-im") { need(1); A.IM = argv[++i]; }
+    else if (K == "-bdm") { need(1); A.BDM = atoi(argv[++i]); }
+    else if (K == "-cfm") { need(1); A.CFM = atoi(argv[++i]); }
     else if (K == "-n") { need(1); A.NumFrames = atoi(argv[++i]); }
     else if (K == "-erp") { need(2); A.Erp = true; A.Lon = atoi(argv[++i]); A.Lat = atoi(argv[++i]); }
     else if (K == "-sr") { need(1); A.SR = atoi(argv[++i]); }
@@ -256,21 +266,28 @@ int main(int argc, char** argv) {
   const bool CalcPSNRs = A.M[0] || A.M[1] || A.M[2];
   const int32 Margin = xRoundUpToNearestMultiple(A.SR, 2);  // xAppQMIV.cpp:330
 
-  xSeq Seq0(Size, A.BD, CF), Seq1(Size, A.BD, CF);
-  if (!Seq0.openFile(A.I0, xSeq::eMode::Read) || !Seq1.openFile(A.I1, xSeq::eMode::Read)) {
+  const bool UseMask = !A.IM.empty();
+  if (UseMask && CalcSSIMs) { fprintf(stderr, "SSIM metrics cannot be combined with mask mode\n"); return 2; }
+  const eCrF CFM = A.CFM == 420 ? eCrF::CF420 : A.CFM == 444 ? eCrF::CF444 : A.CFM == 422 ? eCrF::CF422 : eCrF::CF400;
+  xSeq Seq0(Size, A.BD, CF), Seq1(Size, A.BD, CF), SeqM(Size, A.BDM, CFM);
+  if (!Seq0.openFile(A.I0, xSeq::eMode::Read) || !Seq1.openFile(A.I1, xSeq::eMode::Read) ||
+      (UseMask && !SeqM.openFile(A.IM, xSeq::eMode::Read))) {
     fprintf(stderr, "cannot open inputs\n");
     return 1;
   }
   int NumFrames = std::min(Seq0.getNumOfFrames(), Seq1.getNumOfFrames());
+  if (UseMask) NumFrames = std::min(NumFrames, SeqM.getNumOfFrames());
   if (A.NumFrames > 0) NumFrames = std::min(NumFrames, A.NumFrames);
   if (NumFrames <= 0) { fprintf(stderr, "no frames\n"); return 1; }
 
   xThreadPool* Pool = nullptr;
   if (A.Nth > 0) { Pool = new xThreadPool; Pool->create(A.Nth, A.H + 1); }
 
-  xPicP P0(Size, A.BD, Margin), P1(Size, A.BD, Margin);
+  xPicP P0(Size, A.BD, Margin), P1(Size, A.BD, Margin), PM;
+  if (UseMask) PM.create(Size, A.BDM, Margin);
   xPicI I0, I1;
-  const bool UsePicI = A.Ilp && A.M[2];
+  // Masked IV-PSNR exists only for interleaved pictures (xIVPSNR.h:47-49).
+  const bool UsePicI = (A.Ilp || UseMask) && A.M[2];
   if (UsePicI) { I0.create(Size, A.BD, Margin); I1.create(Size, A.BD, Margin); }
   xPicP SCP0, SCP1;
   if (CalcIVs) { SCP0.create(Size, A.BD, Margin); SCP1.create(Size, A.BD, Margin); }
@@ -317,18 +334,24 @@ int main(int argc, char** argv) {
   fprintf(F, "{\"width\":%d,\"height\":%d,\"bit_depth\":%d,\"chroma_format\":%d,\"frames\":[", A.W, A.H, A.BD, A.CF);
   for (int f = 0; f < NumFrames; f++) {
     auto t0 = Clock::now();
-    if (!Seq0.readFrame(&P0) || !Seq1.readFrame(&P1)) { fprintf(stderr, "read error\n"); return 1; }
+    if (!Seq0.readFrame(&P0) || !Seq1.readFrame(&P1) || (UseMask && !SeqM.readFrame(&PM))) {
+      fprintf(stderr, "read error\n");
+      return 1;
+    }
     auto t1 = Clock::now();
-    bool InRange0 = P0.check(A.I0), InRange1 = P1.check(A.I1);
+    bool InRange0 = P0.check(A.I0), InRange1 = P1.check(A.I1), InRangeM = UseMask ? PM.check(A.IM) : true;
     auto t2 = Clock::now();
     boolV4 Exact = xMakeVec4<bool>(false);
     for (int c = 0; c < 3; c++) Exact[c] = P0.equalCmp(&P1, (eCmp)c);
     P0.extend();
     P1.extend();
+    if (UseMask) PM.extend();
     if (UsePicI) { I0.rearrangeFromPlanar(&P0); I1.rearrangeFromPlanar(&P1); }
+    // xAppQMIV.cpp:825-829
+    const int32 NNM = UseMask ? xPixelOps::CountNonZero(PM.getAddr(eCmp::LM), PM.getStride(), PM.getWidth(), PM.getHeight()) : 0;
     auto t3 = Clock::now();
     int32V4 GCD = {0, 0, 0, 0};
-    if (CalcIVs) GCD = ProcGCD.CalcGlobalColorDiff(&P0, &P1);
+    if (CalcIVs) GCD = UseMask ? ProcGCD.CalcGlobalColorDiffM(&P0, &P1, &PM, NNM) : ProcGCD.CalcGlobalColorDiff(&P0, &P1);
     auto t4 = Clock::now();
     if (CalcSCP) ProcSCP.GenShftCompPics(&SCP1, &SCP0, &P1, &P0, GCD);
     auto t5 = Clock::now();
@@ -337,10 +360,11 @@ int main(int argc, char** argv) {
     if (f) fputc(',', F);
     fprintf(F, "{\"frame\":%d,\"in_range\":[%d,%d],\"exact\":[%d,%d,%d],\"gcd\":[%d,%d,%d,%d]", f, InRange0, InRange1,
             (int)Exact[0], (int)Exact[1], (int)Exact[2], GCD[0], GCD[1], GCD[2], GCD[3]);
+    if (UseMask) fprintf(F, ",\"in_range_mask\":%d,\"num_non_masked\":%d", InRangeM, NNM);
 
     if (A.M[0]) {
       auto a = Clock::now();
-      flt64V4 V = ProcPSNR.calcPicPSNR(&P0, &P1);
+      flt64V4 V = UseMask ? ProcPSNR.calcPicPSNRM(&P0, &P1, &PM, NNM) : ProcPSNR.calcPicPSNR(&P0, &P1);
       for (int c = 0; c < 3; c++) if (Exact[c]) V[c] = FakePSNR;
       T[5] += ms(a, Clock::now());
       CmpVals[0].push_back(V);
@@ -349,7 +373,7 @@ int main(int argc, char** argv) {
     }
     if (A.M[1]) {
       auto a = Clock::now();
-      flt64V4 V = ProcPSNR.calcPicWSPSNR(&P0, &P1);
+      flt64V4 V = UseMask ? ProcPSNR.calcPicWSPSNRM(&P0, &P1, &PM, NNM) : ProcPSNR.calcPicWSPSNR(&P0, &P1);
       for (int c = 0; c < 3; c++) if (Exact[c]) V[c] = FakePSNR;
       T[6] += ms(a, Clock::now());
       CmpVals[1].push_back(V);
@@ -370,11 +394,35 @@ int main(int argc, char** argv) {
     }
     if (A.M[2]) {
       auto a = Clock::now();
-      flt64 V = UsePicI ? ProcPSNR.calcPicIVPSNR(&I0, &I1, GCD) : ProcPSNR.calcPicIVPSNR(&P0, &P1, GCD);
+      flt64 MR2T = 0, MT2R = 0;
+      if (UseMask) ProcPSNR.setDebugCallbackQAP([&](flt64 R2T, flt64 T2R) { MR2T = R2T; MT2R = T2R; });
+      flt64 V = UseMask   ? ProcPSNR.calcPicIVPSNRM(&I0, &I1, &PM, NNM, GCD)
+                : UsePicI ? ProcPSNR.calcPicIVPSNR(&I0, &I1, GCD)
+                          : ProcPSNR.calcPicIVPSNR(&P0, &P1, GCD);
       T[7] += ms(a, Clock::now());
       PicVals[2].push_back(V);
       fprintf(F, ",\"ivpsnr\":"); pd(F, V);
-      if (A.Rows) {
+      if (UseMask) {
+        // The masked processor's own labels (xIVPSNR.cpp:152-165): R2T searches P1 - g in P0.
+        fprintf(F, ",\"r2t\":"); pd(F, MR2T); fprintf(F, ",\"t2r\":"); pd(F, MT2R);
+        if (A.Rows) {
+          std::vector<uint64V4> RowsR2T, RowsT2R;
+          ProcPSNR.qualM(&I1, &I0, &PM, -GCD, NNM, RowsR2T);
+          ProcPSNR.qualM(&I0, &I1, &PM, GCD, NNM, RowsT2R);
+          const std::vector<uint64V4>* Dirs[2] = {&RowsR2T, &RowsT2R};
+          fprintf(F, ",\"cps_rows\":[");
+          for (int d = 0; d < 2; d++) {
+            fprintf(F, "%s[", d ? "," : "");
+            for (int c = 0; c < 3; c++) {
+              fprintf(F, "%s[", c ? "," : "");
+              for (int y = 0; y < A.H; y++) fprintf(F, "%s%llu", y ? "," : "", (unsigned long long)(*Dirs[d])[y][c]);
+              fputc(']', F);
+            }
+            fputc(']', F);
+          }
+          fputc(']', F);
+        }
+      } else if (A.Rows) {
         // Both directions as calcPicIVPSNR forms them (xIVPSNR.cpp:45-57).
         std::vector<uint64V4> RowsR2T, RowsT2R;
         flt64 R2T, T2R;

@@ -21,7 +21,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
-from ivpsnr_amd.synth import make_pair, sha256  # noqa: E402
+from ivpsnr_amd.synth import make_mask, make_pair, sha256  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = os.path.join(ROOT, "oracle", "_ref", "qmiv_ref")
@@ -113,9 +113,48 @@ def colour_fixtures():
     return {"tables": tables, "cases": cases}
 
 
+MASK_CASES = [
+    # w, h, bd, chroma, seed, pair knobs, mask bit depth, mask chroma, mask kind
+    (64, 48, 8, 420, 1, {}, 8, 420, "objects"),
+    (130, 66, 10, 420, 2, {}, 8, 420, "objects"),
+    (67, 45, 8, 444, 3, {}, 8, 444, "soft"),
+    (256, 128, 10, 420, 4, {"luma_offset": 12.0}, 8, 420, "soft"),
+    (96, 70, 16, 420, 5, {}, 16, 420, "soft"),
+    (64, 48, 10, 420, 6, {}, 8, 420, "zero"),
+    (64, 48, 10, 420, 7, {}, 10, 444, "full"),
+    (1920, 1080, 8, 420, 8, {}, 8, 420, "objects"),
+]
+
+
+def masked_fixtures():
+    """Masked mode (PSNR, WS-PSNR, IV-PSNR with a mask picture) by the reference's processors."""
+    cases = []
+    with tempfile.TemporaryDirectory() as td:
+        for (w, h, bd, cf, seed, kw, bdm, cfm, kind) in MASK_CASES:
+            pair = make_pair(w, h, bd, seed, chroma_format=cf, **kw)
+            mask = make_mask(w, h, bdm, seed, chroma_format=cfm, kind=kind)
+            f0, f1 = write_pairs(td, [pair])
+            fm = os.path.join(td, "m.yuv")
+            mask.tofile(fm)
+            entry = {"width": w, "height": h, "bit_depth": bd, "chroma_format": cf, "seed": seed, "knobs": kw,
+                     "mask_bit_depth": bdm, "mask_chroma_format": cfm, "mask_kind": kind,
+                     "sha256_ref": sha256(pair.ref), "sha256_tst": sha256(pair.tst), "sha256_mask": sha256(mask)}
+            for erp in (False, True):
+                out = run_ref(w, h, bd, cf, f0, f1, "PSNR,WSPSNR,IVPSNR", erp,
+                              ["-rows", "-im", fm, "-bdm", str(bdm), "-cfm", str(cfm)])
+                entry["erp" if erp else "plain"] = compact_frame(out["frames"][0], keep_rows=h <= 128)
+            cases.append(entry)
+            print("masked", w, h, bd, cf, kind, file=sys.stderr)
+    return cases
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("oracle/_ref/qmiv_ref is not built (make -C oracle ref)")
+    if "--masked" in sys.argv:
+        with open(os.path.join(HERE, "masked.json"), "w") as f:
+            json.dump(masked_fixtures(), f)
+        return
     if "--colour" in sys.argv:
         with open(os.path.join(HERE, "colorspace.json"), "w") as f:
             json.dump(colour_fixtures(), f)
