@@ -246,6 +246,61 @@ qmiv_status qmiv_calc_ivssim_scp(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_
    Asynchronous on the context stream. */
 qmiv_status qmiv_pic_upsample(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst);
 
+/* ------------------------------------------------------------------------------------------ */
+/* Masked mode: metrics over the pels whose mask is non-zero, weighted by the mask value       */
+/* (apps/QMIV/src/xAppQMIV.cpp:319-337,695,825-889). The mask is a picture of the same size    */
+/* (any bit depth 8..16, 4:2:0 or 4:4:4) of which the luma plane is used; its margin is        */
+/* clamp-to-edge like the pictures'.                                                           */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct qmiv_masked_result {
+  int32_t in_range[3];     /* P0, P1, mask: all pels <= 2^bd - 1 (xPicP::check of the three inputs) */
+  int32_t exact[3];        /* per component: P0 and P1 identical over the whole picture */
+  int32_t num_non_masked;  /* xPixelOps::CountNonZero of the mask's luma plane (xAppQMIV.cpp:825-829) */
+  int32_t gcd[4];          /* masked GCD R2T (xAppQMIV.cpp:695), zeros without IV-PSNR */
+  double psnr[3], psnr_pic;
+  double wspsnr[3], wspsnr_pic;
+  /* The masked processor's labels (libs/pmbb-ivqm/src/xIVPSNR.cpp:152-165): R2T searches P1 - g
+     inside P0, T2R searches P0 + g inside P1 (the unmasked processor names them the other way). */
+  double ivpsnr, ivpsnr_r2t, ivpsnr_t2r;
+  uint64_t cps_sum[2][3];  /* [R2T, T2R][component]: sum over rows of the mask-weighted values */
+} qmiv_masked_result;
+
+/* One frame through the application's masked flow: validation of the three inputs, exact
+   components, the masked GCD (with IV-PSNR), then PSNR / WS-PSNR / IV-PSNR in their masked forms
+   (xPSNR::calcPicPSNRM, xPSNR.cpp:42-71; xWSPSNR::calcPicWSPSNRM, xWSPSNR.cpp:61-97;
+   xIVPSNRM::calcPicIVPSNRM, xIVPSNR.cpp:152-209). metrics may hold only QMIV_PSNR, QMIV_WSPSNR and
+   QMIV_IVPSNR (the application rejects SSIM metrics with a mask, xAppQMIV.cpp:337). Identical
+   components get the PSNR of SSD=1 when prm->fake_psnr_for_exact is set (xAppQMIV.cpp:837-871).
+   Returns QMIV_ERR_PEL_RANGE (with the flags filled) when a pel of any input exceeds its bit depth. */
+qmiv_status qmiv_frame_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
+                              const qmiv_params* prm, uint32_t metrics, qmiv_masked_result* out);
+
+/* xPixelOps::CountNonZero over the mask's luma plane (libs/pmbb-core/src/xPixelOpsSTD.cpp:273-284). */
+qmiv_status qmiv_count_nonzero(qmiv_ctx* ctx, const qmiv_pic* mask, int32_t* out);
+
+/* Processor-level masked calls. num_non_masked < 0 means "count it" (the reference's NOT_VALID).
+   prm->fake_psnr_for_exact selects xPSNR::setFakeValsForExact (PSNR of SSD=1 where the masked
+   error is 0); a null prm gives the defaults with that flag off, as the processors start. */
+
+/* xGlobClrDiffProc::CalcGlobalColorDiffM(Tst, Ref, Msk, NNM) (libs/pmbb-ivqm/src/xGlobClrDiff.h:42,
+   xGlobClrDiff.cpp:52-83): clip(rint(sum((Ref - Tst) * m) / (NNM * maxm)), +-rint(coef * max)). */
+qmiv_status qmiv_calc_gcd_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* mask,
+                                 const qmiv_params* prm, int32_t num_non_masked, int32_t gcd[4]);
+/* xPSNR::calcPicPSNRM (libs/pmbb-ivqm/src/xPSNR.cpp:42-71,84-136,148-155). */
+qmiv_status qmiv_calc_psnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* mask,
+                                  const qmiv_params* prm, int32_t num_non_masked, double psnr[3]);
+/* xWSPSNR::calcPicWSPSNRM (libs/pmbb-ivqm/src/xWSPSNR.cpp:61-97,127-155); without prm->erp the
+   plain masked PSNR with the mask counted again, as the reference does. */
+qmiv_status qmiv_calc_wspsnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* mask,
+                                    const qmiv_params* prm, int32_t num_non_masked, double wspsnr[3]);
+/* xIVPSNRM::calcPicIVPSNRM(Tst, Ref, Msk, NNM, GCD) (libs/pmbb-ivqm/src/xIVPSNR.cpp:152-209). With
+   gcd_ref2tst null: calcPicIVPSNRM(Tst, Ref, Msk, TstI, RefI) (:138-150), which counts the mask
+   and computes the masked GCD itself (num_non_masked is then ignored). Search ranges 0..12. */
+qmiv_status qmiv_calc_ivpsnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* mask,
+                                    const qmiv_params* prm, int32_t num_non_masked, const int32_t gcd_ref2tst[4],
+                                    double* ivpsnr, double* r2t, double* t2r);
+
 /* Colour spaces of the RGB <-> YCbCr conversions: eClrSpcLC (libs/pmbb-core/src/xCommonDefCORE.h:356-364;
    SMPTE 170M shares BT.601's matrix). */
 enum { QMIV_CLR_BT601 = 0, QMIV_CLR_BT709 = 1, QMIV_CLR_SMPTE240M = 2, QMIV_CLR_BT2020 = 3 };

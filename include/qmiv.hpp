@@ -193,7 +193,23 @@ class xGlobClrDiffProc : public xProcPrms {
     xCheck(qmiv_calc_gcd(m_Ctx->get(), Tst->handle(), Ref->handle(), &m_Prm, G.data()), "qmiv_calc_gcd");
     return G;
   }
+  // xGlobClrDiffProc::CalcGlobalColorDiffM (xGlobClrDiff.h:42, xGlobClrDiff.cpp:52-83):
+  // rint(sum((Ref - Tst) * Msk) / (NumNonMasked * MaxMsk)), clipped.
+  int32V4 CalcGlobalColorDiffM(const xPicP* Tst, const xPicP* Ref, const xPicP* Msk, int32 NumNonMasked) {
+    int32V4 G{};
+    xCheck(qmiv_calc_gcd_masked(m_Ctx->get(), Tst->handle(), Ref->handle(), Msk->handle(), &m_Prm, NumNonMasked,
+                                G.data()),
+           "qmiv_calc_gcd_masked");
+    return G;
+  }
 };
+
+// xPixelOps::CountNonZero over a mask picture's first plane (libs/pmbb-core/src/xPixelOpsSTD.cpp:273-284).
+inline int32 countNonZero(xContext& Ctx, const xPicP* Msk) {
+  int32 N = 0;
+  xCheck(qmiv_count_nonzero(Ctx.get(), Msk->handle(), &N), "qmiv_count_nonzero");
+  return N;
+}
 
 // xShftCompPicProc (libs/pmbb-ivqm/src/xShftCompPic.h:33-44).
 class xShftCompPicProc : public xProcPrms {
@@ -224,6 +240,52 @@ class xIVPSNR : public xProcPrms {
   // Plain PSNR unless initWS enabled the spherical weights (xWSPSNR.cpp:17-27).
   flt64V4 calcPicWSPSNR(const xPicP* Tst, const xPicP* Ref) { return xPerCmp(Tst, Ref, QMIV_WSPSNR); }
 
+  // Masked forms (xPSNR::calcPicPSNRM, xPSNR.cpp:42-71; xWSPSNR::calcPicWSPSNRM, xWSPSNR.cpp:61-97):
+  // NumNonMasked < 0 (NOT_VALID) counts the mask's non-zero pels.
+  flt64V4 calcPicPSNRM(const xPicP* Tst, const xPicP* Ref, const xPicP* Msk, int32 NumNonMasked = -1) {
+    flt64V4 V{};
+    const qmiv_params P = xMaskedPrm();
+    xCheck(qmiv_calc_psnr_masked(m_Ctx->get(), Tst->handle(), Ref->handle(), Msk->handle(), &P, NumNonMasked,
+                                 V.data()),
+           "qmiv_calc_psnr_masked");
+    xReportMsk(Msk, NumNonMasked);
+    return {V[0], V[1], V[2], 0.0};
+  }
+  flt64V4 calcPicWSPSNRM(const xPicP* Tst, const xPicP* Ref, const xPicP* Msk, int32 NumNonMasked = -1) {
+    flt64V4 V{};
+    const qmiv_params P = xMaskedPrm();
+    xCheck(qmiv_calc_wspsnr_masked(m_Ctx->get(), Tst->handle(), Ref->handle(), Msk->handle(), &P, NumNonMasked,
+                                   V.data()),
+           "qmiv_calc_wspsnr_masked");
+    xReportMsk(Msk, NumNonMasked);
+    return {V[0], V[1], V[2], 0.0};
+  }
+  // xIVPSNRM::calcPicIVPSNRM(Tst, Ref, Msk, NumNonMasked, GCD) (xIVPSNR.cpp:152-166).
+  flt64 calcPicIVPSNRM(const xPicP* Tst, const xPicP* Ref, const xPicP* Msk, int32 NumNonMasked,
+                       const int32V4& GlobalColorDiffRef2Tst) {
+    flt64 IV = 0, R2T = 0, T2R = 0;
+    const qmiv_params P = xMaskedPrm();
+    xCheck(qmiv_calc_ivpsnr_masked(m_Ctx->get(), Tst->handle(), Ref->handle(), Msk->handle(), &P, NumNonMasked,
+                                   GlobalColorDiffRef2Tst.data(), &IV, &R2T, &T2R),
+           "qmiv_calc_ivpsnr_masked");
+    if (m_DebugCallbackQAP) m_DebugCallbackQAP(R2T, T2R);
+    return IV;
+  }
+  // xIVPSNRM::calcPicIVPSNRM(Tst, Ref, Msk, TstI, RefI) (xIVPSNR.cpp:138-150): counts the mask and
+  // computes the masked colour difference itself.
+  flt64 calcPicIVPSNRM(const xPicP* Tst, const xPicP* Ref, const xPicP* Msk) {
+    const int32 NumNonMasked = countNonZero(*m_Ctx, Msk);
+    if (m_DebugCallbackMSK) m_DebugCallbackMSK(NumNonMasked);
+    int32V4 G{};
+    const qmiv_params P = xMaskedPrm();
+    xCheck(qmiv_calc_gcd_masked(m_Ctx->get(), Tst->handle(), Ref->handle(), Msk->handle(), &P, NumNonMasked,
+                                G.data()),
+           "qmiv_calc_gcd_masked");
+    if (m_DebugCallbackGCS) m_DebugCallbackGCS(G);
+    return calcPicIVPSNRM(Tst, Ref, Msk, NumNonMasked, G);
+  }
+  void setDebugCallbackMSK(std::function<void(int32)> Callback) { m_DebugCallbackMSK = std::move(Callback); }
+
   // xIVPSNR::calcPicIVPSNR(Tst, Ref) (xIVPSNR.cpp:18-31): the colour difference is
   // -CalcGlobalColorDiff(Tst, Ref) of the static form, i.e. minus the rounded sum(Tst - Ref) / Area.
   flt64 calcPicIVPSNR(const xPicP* Tst, const xPicP* Ref) {
@@ -252,9 +314,18 @@ class xIVPSNR : public xProcPrms {
     const double* V = Metric == QMIV_PSNR ? R.psnr : R.wspsnr;
     return {V[0], V[1], V[2], 0.0};
   }
+  qmiv_params xMaskedPrm() const {
+    qmiv_params P = m_Prm;
+    P.fake_psnr_for_exact = m_FakeValsForExact ? 1 : 0;
+    return P;
+  }
+  void xReportMsk(const xPicP* Msk, int32 NumNonMasked) {
+    if (m_DebugCallbackMSK) m_DebugCallbackMSK(NumNonMasked >= 0 ? NumNonMasked : countNonZero(*m_Ctx, Msk));
+  }
 
   tDCfGCS m_DebugCallbackGCS;
   tDCfQAP m_DebugCallbackQAP;
+  std::function<void(int32)> m_DebugCallbackMSK;
   bool m_FakeValsForExact = false;
 };
 

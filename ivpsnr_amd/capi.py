@@ -55,6 +55,18 @@ class FrameResult(C.Structure):
         return out
 
 
+class MaskedResult(C.Structure):
+    _fields_ = [
+        ("in_range", C.c_int32 * 3), ("exact", C.c_int32 * 3), ("num_non_masked", C.c_int32),
+        ("gcd", C.c_int32 * 4),
+        ("psnr", C.c_double * 3), ("psnr_pic", C.c_double),
+        ("wspsnr", C.c_double * 3), ("wspsnr_pic", C.c_double),
+        ("ivpsnr", C.c_double), ("ivpsnr_r2t", C.c_double), ("ivpsnr_t2r", C.c_double),
+        ("cps_sum", (C.c_uint64 * 3) * 2),
+    ]
+    as_dict = FrameResult.as_dict
+
+
 K_STATS, K_SUMMARY, K_CPS, K_CPS_ROWS, K_SSIM, K_PYRAMID, K_SSIM_ROWS, K_COUNT = 0, 1, 2, 3, 4, 5, 6, 8
 KERNEL_NAMES = {K_STATS: "stats", K_SUMMARY: "summary", K_CPS: "cps", K_CPS_ROWS: "cps_rows", K_SSIM: "ssim",
                 K_PYRAMID: "pyramid", K_SSIM_ROWS: "ssim_rows"}
@@ -113,6 +125,14 @@ _SIGNATURES = {
     "qmiv_calc_ivssim_scp": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), _f64p, _f64p,
                                        _f64p]),
     "qmiv_pic_upsample": (C.c_int, [_ctx_p, _pic_p, _pic_p]),
+    "qmiv_frame_masked": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), C.c_uint32,
+                                    C.POINTER(MaskedResult)]),
+    "qmiv_count_nonzero": (C.c_int, [_ctx_p, _pic_p, _i32p]),
+    "qmiv_calc_gcd_masked": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), C.c_int32, _i32p]),
+    "qmiv_calc_psnr_masked": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), C.c_int32, _f64p]),
+    "qmiv_calc_wspsnr_masked": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), C.c_int32, _f64p]),
+    "qmiv_calc_ivpsnr_masked": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), C.c_int32, _i32p,
+                                          _f64p, _f64p, _f64p]),
     "qmiv_pic_colorspace": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.c_int32, C.c_int32]),
     "qmiv_pic_swap_planes": (C.c_int, [_ctx_p, _pic_p, C.c_int32, C.c_int32]),
     "qmiv_batch_options_default": (None, [C.POINTER(BatchOptions)]),

@@ -43,24 +43,53 @@ void ws_weights(int32_t height, int32_t lat_range_deg, std::vector<double>& weig
   correction = height / kbns(weights.data(), weights.size());
 }
 
+namespace {
+// One component's error of the asymmetric quality: lane-wise xKBNS4 of row * w[y] from zero
+// with WS, else the uint64 sum of the rows (std::accumulate).
+double rows_error(const uint64_t* r, int32_t height, const double* ws) {
+  if (ws) {
+    double s = 0.0, k = 0.0;
+    for (int32_t y = 0; y < height; y++) neumaier(s, k, (double)r[y] * ws[y]);
+    return s + k;
+  }
+  uint64_t s = 0;
+  for (int32_t y = 0; y < height; y++) s += r[y];
+  return (double)s;
+}
+}  // namespace
+
 double qual_from_rows(const uint64_t* rows, int32_t height, const double* ws, int32_t area, int32_t bit_depth,
                       const int32_t wavg[4]) {
-  double err[3];
-  for (int c = 0; c < 3; c++) {
-    const uint64_t* r = rows + (size_t)c * (size_t)height;
-    if (ws) {
-      double s = 0.0, k = 0.0;  // lane-wise xKBNS4, starting from zero
-      for (int32_t y = 0; y < height; y++) neumaier(s, k, (double)r[y] * ws[y]);
-      err[c] = s + k;
-    } else {
-      uint64_t s = 0;
-      for (int32_t y = 0; y < height; y++) s += r[y];
-      err[c] = (double)s;
-    }
-  }
   double q[3];
-  for (int c = 0; c < 3; c++) q[c] = psnr_from_ssd(err[c] > 0 ? err[c] : 1.0, area, bit_depth);
+  for (int c = 0; c < 3; c++) {
+    const double err = rows_error(rows + (size_t)c * (size_t)height, height, ws);
+    q[c] = psnr_from_ssd(err > 0 ? err : 1.0, area, bit_depth);
+  }
   return weighted_average4(q, wavg);
+}
+
+double psnr_from_masked_ssd(double ssd, int32_t nnm, int32_t bit_depth, int32_t mask_bit_depth) {
+  const uint64_t maxv = (uint64_t)((1 << bit_depth) - 1);
+  const uint64_t maxm = (uint64_t)((1 << mask_bit_depth) - 1);
+  const uint64_t peak = (uint64_t)(int64_t)nnm * (maxv * maxv) * maxm;  // int64 products, wrapping
+  return ssd > 0 ? 10.0 * std::log10((double)(int64_t)peak / ssd) : DBL_MAX;
+}
+
+double qual_from_rows_masked(const uint64_t* rows, int32_t height, const double* ws, int32_t nnm, int32_t bit_depth,
+                             int32_t mask_bit_depth, const int32_t wavg[4]) {
+  double q[3];
+  for (int c = 0; c < 3; c++) {
+    const double err = rows_error(rows + (size_t)c * (size_t)height, height, ws);
+    q[c] = psnr_from_masked_ssd(err > 0 ? err : 1.0, nnm, bit_depth, mask_bit_depth);
+  }
+  return weighted_average4(q, wavg);
+}
+
+double wspsnr_from_rows_masked(const uint64_t* rows444, int32_t height, const double* ws, double correction,
+                               int32_t nnm, int32_t bit_depth, int32_t mask_bit_depth) {
+  double s = 0.0, k = 0.0;
+  for (int32_t y = 0; y < height; y++) neumaier(s, k, (double)rows444[y] * ws[y]);
+  return psnr_from_masked_ssd((s + k) * correction, nnm, bit_depth, mask_bit_depth);
 }
 
 double wspsnr_from_rows(const uint64_t* rows444, int32_t height, const double* ws, double correction,

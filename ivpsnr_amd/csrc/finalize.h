@@ -27,6 +27,19 @@ double qual_from_rows(const uint64_t* rows, int32_t height, const double* ws, in
 double wspsnr_from_rows(const uint64_t* rows444, int32_t height, const double* ws, double correction,
                         int32_t area, int32_t bit_depth);
 
+// xPSNR::CalcPSNRfromMaskedSSD (libs/pmbb-ivqm/src/xPSNR.cpp:148-155): peak = nnm * max^2 * maxm in
+// int64 arithmetic; DBL_MAX for SSD <= 0.
+double psnr_from_masked_ssd(double ssd, int32_t nnm, int32_t bit_depth, int32_t mask_bit_depth);
+
+// xIVPSNRM::xCalcQualAsymmetricPicM tail (libs/pmbb-ivqm/src/xIVPSNR.cpp:170-206): as qual_from_rows
+// with the masked peak.
+double qual_from_rows_masked(const uint64_t* rows, int32_t height, const double* ws, int32_t nnm, int32_t bit_depth,
+                             int32_t mask_bit_depth, const int32_t wavg[4]);
+
+// Masked WS-PSNR of one component from its per-row masked SSD (libs/pmbb-ivqm/src/xWSPSNR.cpp:127-155).
+double wspsnr_from_rows_masked(const uint64_t* rows444, int32_t height, const double* ws, double correction,
+                               int32_t nnm, int32_t bit_depth, int32_t mask_bit_depth);
+
 // Component-weighted picture value as xMetricStat::setPerCmpMeric (apps/QMIV/src/xAppQMIV.h:245-251).
 double pic_value(const double v[3], const int32_t wavg[4]);
 

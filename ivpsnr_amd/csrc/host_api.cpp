@@ -1813,4 +1813,279 @@ qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, const floa
   return QMIV_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Masked mode (apps/QMIV/src/xAppQMIV.cpp:319-337,695,825-889)
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+// Scratch of one masked evaluation.
+//   device: [results block | nnz | sd | max | diff | CPS partials]
+//   results block (copied back): MaskSummary | masked SSD rows [3][h] | CPS rows [2][3][h]
+struct MaskedLayout {
+  size_t off_sum = 0, off_ssd = 0, off_cps = 0, results_bytes = 0;
+  size_t off_nnz = 0, off_sd = 0, off_max = 0, off_diff = 0, off_part = 0, total = 0;
+  int32_t nseg = 0;
+};
+
+MaskedLayout masked_layout(const Geom& g, bool cps) {
+  MaskedLayout L;
+  const size_t h = (size_t)g.h;
+  size_t o = 0;
+  L.off_sum = o;
+  o = align_up(o + sizeof(MaskSummary), 256);
+  L.off_ssd = o;
+  o = align_up(o + 3 * h * sizeof(uint64_t), 256);
+  L.off_cps = o;
+  o = align_up(o + (cps ? 6 * h * sizeof(uint64_t) : 0), 256);
+  L.results_bytes = o;
+  L.off_nnz = o;
+  o = align_up(o + h * sizeof(int32_t), 256);
+  L.off_sd = o;
+  o = align_up(o + 3 * h * sizeof(int64_t), 256);
+  L.off_max = o;
+  o = align_up(o + 3 * h * sizeof(uint32_t), 256);
+  L.off_diff = o;
+  o = align_up(o + h * sizeof(uint32_t), 256);
+  L.nseg = masked_cps_segments(g);
+  L.off_part = o;
+  o = align_up(o + (cps ? (size_t)L.nseg * 6 * h * sizeof(uint64_t) : 0), 256);
+  L.total = o;
+  return L;
+}
+
+qmiv_status check_mask(const qmiv_pic* p0, const qmiv_pic* mask) {
+  if (!mask) return fail(QMIV_ERR_INVALID_ARG, "null mask picture");
+  if (mask->ctx != p0->ctx) return fail(QMIV_ERR_INVALID_ARG, "mask belongs to another context");
+  if (mask->g.w != p0->g.w || mask->g.h != p0->g.h)
+    return fail(QMIV_ERR_INVALID_ARG, "mask differs from the pictures in size");
+  return QMIV_OK;
+}
+
+// Masked statistics (and, with cps, the masked search of both directions) of P0 / P1 under the
+// mask; results at the start of ctx->h_results. gcd_given: null to compute the masked GCD.
+qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
+                       const qmiv_params& prm, bool cps, int32_t nnm_given, const int32_t* gcd_given,
+                       MaskedLayout& L) {
+  const Geom& g = p0->g;
+  if (cps && (prm.search_range < 0 || prm.search_range > kMaxMaskedSearchRange))
+    return fail(QMIV_ERR_UNSUPPORTED, "masked search supports search ranges 0..12");
+  L = masked_layout(g, cps);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_TRY(ensure_scratch(ctx, L.total));
+  QMIV_TRY(ensure_results(ctx, L.results_bytes));
+  char* d = (char*)ctx->d_scratch;
+  MaskRows r;
+  r.nnz = (int32_t*)(d + L.off_nnz);
+  r.sd = (int64_t*)(d + L.off_sd);
+  r.ssd = (uint64_t*)(d + L.off_ssd);
+  r.max = (uint32_t*)(d + L.off_max);
+  r.diff = (uint32_t*)(d + L.off_diff);
+  MaskSummary* sum = (MaskSummary*)(d + L.off_sum);
+  int32_t md[4];
+  for (int c = 0; c < 4; c++) md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
+  hipStream_t st = ctx->stream;
+  {
+    KernelTimer t(ctx, QMIV_K_STATS, st);
+    QMIV_HIP(launch_mstats(p0->dev, p1->dev, g, mask->dev, mask->g, r, st));
+  }
+  {
+    KernelTimer t(ctx, QMIV_K_SUMMARY, st);
+    QMIV_HIP(launch_msummary(g, (1 << mask->g.bd) - 1, r, md, nnm_given, gcd_given, sum, st));
+  }
+  if (cps) {
+    uint64_t* part = (uint64_t*)(d + L.off_part);
+    {
+      KernelTimer t(ctx, QMIV_K_CPS, st);
+      QMIV_HIP(launch_cps_masked(p0->dev, p1->dev, g, mask->dev, mask->g, sum, prm.search_range,
+                                 prm.cmp_weights_search, part, st));
+    }
+    {
+      KernelTimer t(ctx, QMIV_K_CPS_ROWS, st);
+      QMIV_HIP(launch_cps_rows(part, L.nseg, g.h, (uint64_t*)(d + L.off_cps), st));
+    }
+  }
+  QMIV_HIP(hipMemcpyAsync(ctx->h_results, d, L.results_bytes, hipMemcpyDeviceToHost, st));
+  return wait_stream(ctx);
+}
+
+const MaskSummary* masked_summary(const qmiv_ctx* ctx, const MaskedLayout& L) {
+  return (const MaskSummary*)((const char*)ctx->h_results + L.off_sum);
+}
+
+qmiv_params masked_params(const qmiv_params* prm_in, bool processor) {
+  qmiv_params p;
+  if (prm_in) {
+    p = *prm_in;
+  } else {
+    qmiv_params_default(&p);
+    if (processor) p.fake_psnr_for_exact = 0;  // xPSNR's default: no fake values
+  }
+  return p;
+}
+
+}  // namespace
+
+qmiv_status qmiv_frame_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
+                              const qmiv_params* prm_in, uint32_t metrics, qmiv_masked_result* out) {
+  if (!ctx || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(p0, p1));
+  QMIV_TRY(check_mask(p0, mask));
+  if (p0->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
+  if (metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM))
+    return fail(QMIV_ERR_INVALID_ARG, "structural similarity metrics cannot be combined with mask mode");
+  if (metrics & ~(uint32_t)(QMIV_PSNR | QMIV_WSPSNR | QMIV_IVPSNR)) return fail(QMIV_ERR_INVALID_ARG, "unknown metric bits");
+  const qmiv_params prm = masked_params(prm_in, false);
+  std::memset(out, 0, sizeof(*out));
+  const bool iv = (metrics & QMIV_IVPSNR) != 0;
+  const int32_t zero[4] = {0, 0, 0, 0};
+  MaskedLayout L;
+  // Without IV-PSNR the application computes no colour difference and reports zeros.
+  QMIV_TRY(run_masked(ctx, p0, p1, mask, prm, iv, -1, iv ? nullptr : zero, L));
+  const char* hb = (const char*)ctx->h_results;
+  const MaskSummary* sum = masked_summary(ctx, L);
+  const Geom& g = p0->g;
+  const int32_t bdm = mask->g.bd, nnm = sum->nnm;
+  for (int i = 0; i < 3; i++) out->in_range[i] = sum->in_range[i];
+  for (int c = 0; c < 3; c++) out->exact[c] = sum->exact[c];
+  for (int c = 0; c < 4; c++) out->gcd[c] = sum->gcd[c];
+  out->num_non_masked = nnm;
+  if (!sum->all_in_range) return fail(QMIV_ERR_PEL_RANGE, "input or mask contains pels above the bit-depth maximum");
+  const double fake = fin::psnr_from_ssd(1.0, g.w * g.h, g.bd);
+  double plain[3];
+  for (int c = 0; c < 3; c++) plain[c] = fin::psnr_from_masked_ssd((double)sum->ssd[c], nnm, g.bd, bdm);
+  std::vector<double> ws;
+  double corr = 1.0;
+  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  if (metrics & QMIV_PSNR) {
+    for (int c = 0; c < 3; c++) out->psnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : plain[c];
+    out->psnr_pic = fin::pic_value(out->psnr, prm.cmp_weights_average);
+  }
+  if (metrics & QMIV_WSPSNR) {
+    const uint64_t* rows = (const uint64_t*)(hb + L.off_ssd);
+    for (int c = 0; c < 3; c++) {
+      const double v = prm.erp ? fin::wspsnr_from_rows_masked(rows + (size_t)c * g.h, g.h, ws.data(), corr, nnm, g.bd, bdm)
+                               : plain[c];
+      out->wspsnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : v;
+    }
+    out->wspsnr_pic = fin::pic_value(out->wspsnr, prm.cmp_weights_average);
+  }
+  if (iv) {
+    const uint64_t* rows = (const uint64_t*)(hb + L.off_cps);
+    const double* w = prm.erp ? ws.data() : nullptr;
+    out->ivpsnr_r2t = fin::qual_from_rows_masked(rows, g.h, w, nnm, g.bd, bdm, prm.cmp_weights_average);
+    out->ivpsnr_t2r = fin::qual_from_rows_masked(rows + 3 * (size_t)g.h, g.h, w, nnm, g.bd, bdm, prm.cmp_weights_average);
+    out->ivpsnr = fin::min2(out->ivpsnr_r2t, out->ivpsnr_t2r);
+    for (int d = 0; d < 2; d++)
+      for (int c = 0; c < 3; c++) {
+        uint64_t s = 0;
+        for (int32_t y = 0; y < g.h; y++) s += rows[((size_t)d * 3 + c) * g.h + y];
+        out->cps_sum[d][c] = s;
+      }
+  }
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_count_nonzero(qmiv_ctx* ctx, const qmiv_pic* mask, int32_t* out) {
+  if (!ctx || !mask || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (mask->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "mask belongs to another context");
+  const qmiv_params prm = masked_params(nullptr, true);
+  const int32_t zero[4] = {0, 0, 0, 0};
+  MaskedLayout L;
+  QMIV_TRY(run_masked(ctx, mask, mask, mask, prm, false, -1, zero, L));
+  *out = masked_summary(ctx, L)->nnm;
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_gcd_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* mask,
+                                 const qmiv_params* prm_in, int32_t num_non_masked, int32_t gcd[4]) {
+  if (!ctx || !gcd) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(tst, ref));
+  QMIV_TRY(check_mask(tst, mask));
+  const qmiv_params prm = masked_params(prm_in, true);
+  MaskedLayout L;
+  // Processor form (xGlobClrDiff.h:42): the static function runs on (Ref, Tst): sum((Ref - Tst) * m).
+  QMIV_TRY(run_masked(ctx, tst, ref, mask, prm, false, num_non_masked, nullptr, L));
+  const MaskSummary* sum = masked_summary(ctx, L);
+  for (int c = 0; c < 4; c++) gcd[c] = sum->gcd[c];
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_psnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* mask,
+                                  const qmiv_params* prm_in, int32_t num_non_masked, double psnr[3]) {
+  if (!ctx || !psnr) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(tst, ref));
+  QMIV_TRY(check_mask(tst, mask));
+  const qmiv_params prm = masked_params(prm_in, true);
+  const int32_t zero[4] = {0, 0, 0, 0};
+  MaskedLayout L;
+  QMIV_TRY(run_masked(ctx, tst, ref, mask, prm, false, -1, zero, L));
+  const MaskSummary* sum = masked_summary(ctx, L);
+  const Geom& g = tst->g;
+  const int32_t nnm = num_non_masked >= 0 ? num_non_masked : sum->nnm;
+  for (int c = 0; c < 3; c++) {
+    psnr[c] = fin::psnr_from_masked_ssd((double)sum->ssd[c], nnm, g.bd, mask->g.bd);
+    // xPSNR::xCalcCmpPSNRM (libs/pmbb-ivqm/src/xPSNR.cpp:84-92)
+    if (prm.fake_psnr_for_exact && sum->ssd[c] == 0) psnr[c] = fin::psnr_from_ssd(1.0, g.w * g.h, g.bd);
+  }
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_wspsnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* mask,
+                                    const qmiv_params* prm_in, int32_t num_non_masked, double wspsnr[3]) {
+  if (!ctx || !wspsnr) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  const qmiv_params prm = masked_params(prm_in, true);
+  // Without WS the reference recounts the mask (calcPicPSNRM without a count, xWSPSNR.cpp:71-74).
+  if (!prm.erp) return qmiv_calc_psnr_masked(ctx, tst, ref, mask, &prm, -1, wspsnr);
+  QMIV_TRY(check_pair(tst, ref));
+  QMIV_TRY(check_mask(tst, mask));
+  const int32_t zero[4] = {0, 0, 0, 0};
+  MaskedLayout L;
+  QMIV_TRY(run_masked(ctx, tst, ref, mask, prm, false, -1, zero, L));
+  const MaskSummary* sum = masked_summary(ctx, L);
+  const uint64_t* rows = (const uint64_t*)((const char*)ctx->h_results + L.off_ssd);
+  const Geom& g = tst->g;
+  const int32_t nnm = num_non_masked >= 0 ? num_non_masked : sum->nnm;
+  std::vector<double> ws;
+  double corr;
+  fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  for (int c = 0; c < 3; c++) {
+    const uint64_t* r = rows + (size_t)c * g.h;
+    wspsnr[c] = fin::wspsnr_from_rows_masked(r, g.h, ws.data(), corr, nnm, g.bd, mask->g.bd);
+    // xWSPSNR::xCalcCmpWSPSNRM (libs/pmbb-ivqm/src/xWSPSNR.cpp:148-152): fake where the weighted
+    // error is 0, the one case in which the masked PSNR is DBL_MAX.
+    if (prm.fake_psnr_for_exact && wspsnr[c] == DBL_MAX) wspsnr[c] = fin::psnr_from_ssd(1.0, g.w * g.h, g.bd);
+  }
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_calc_ivpsnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_pic* mask,
+                                    const qmiv_params* prm_in, int32_t num_non_masked, const int32_t gcd_ref2tst[4],
+                                    double* ivpsnr, double* r2t, double* t2r) {
+  if (!ctx || !ivpsnr) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(tst, ref));
+  QMIV_TRY(check_mask(tst, mask));
+  const qmiv_params prm = masked_params(prm_in, true);
+  MaskedLayout L;
+  // gcd_ref2tst null: xIVPSNRM::calcPicIVPSNRM(Tst, Ref, Msk, TstI, RefI) counts the mask and
+  // computes the masked GCD itself (libs/pmbb-ivqm/src/xIVPSNR.cpp:138-150).
+  const int32_t nnm_arg = gcd_ref2tst ? num_non_masked : -1;
+  QMIV_TRY(run_masked(ctx, tst, ref, mask, prm, true, -1, gcd_ref2tst, L));
+  const MaskSummary* sum = masked_summary(ctx, L);
+  if (!sum->all_in_range) return fail(QMIV_ERR_PEL_RANGE, "input or mask contains pels above the bit-depth maximum");
+  const Geom& g = tst->g;
+  const int32_t nnm = nnm_arg >= 0 ? nnm_arg : sum->nnm;
+  std::vector<double> ws;
+  double corr;
+  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  const uint64_t* rows = (const uint64_t*)((const char*)ctx->h_results + L.off_cps);
+  const double* w = prm.erp ? ws.data() : nullptr;
+  const double a = fin::qual_from_rows_masked(rows, g.h, w, nnm, g.bd, mask->g.bd, prm.cmp_weights_average);
+  const double b = fin::qual_from_rows_masked(rows + 3 * (size_t)g.h, g.h, w, nnm, g.bd, mask->g.bd,
+                                              prm.cmp_weights_average);
+  if (r2t) *r2t = a;
+  if (t2r) *t2r = b;
+  *ivpsnr = fin::min2(a, b);
+  return QMIV_OK;
+}
+
 }  // extern "C"

@@ -142,6 +142,41 @@ int cps_fast420_blocks_per_cu(int bytes_per_sample);
 
 // Nearest-neighbour 4:2:0 -> 4:4:4 upsampling (a copy for 4:4:4 sources) of a packed frame into a
 // planar 4:4:4 frame of the same sample type.
+// Masked mode: per-row statistics of the 4:4:4 domain (device memory).
+struct MaskRows {
+  int32_t* nnz;    // [h] non-zero mask pels
+  int64_t* sd;     // [3][h] sum((P1 - P0) * m), int32 products
+  uint64_t* ssd;   // [3][h] sum(sq(P1 - P0) * m)
+  uint32_t* max;   // [3][h] largest pel of P0, P1, mask (all planes)
+  uint32_t* diff;  // [h] bit c: component c differs somewhere in the row
+};
+
+// Masked mode: plane totals, flags and the GCD used by the masked search (device memory, read
+// back by the host).
+struct MaskSummary {
+  int32_t gcd[4];
+  int32_t nnm;
+  int32_t in_range[3];     // P0, P1, mask
+  int32_t all_in_range;
+  int32_t exact[3];
+  int64_t sd[3];
+  uint64_t ssd[3];
+};
+
+hipError_t launch_mstats(const void* p0, const void* p1, const Geom& g, const void* mask, const Geom& gm,
+                         const MaskRows& r, hipStream_t st);
+// gcd_given: null to compute the masked GCD, else the four values to store; nnm_given >= 0
+// replaces the counted pels in the GCD's denominator.
+hipError_t launch_msummary(const Geom& g, int32_t maxm, const MaskRows& r, const int32_t maxdiff[4], int32_t nnm_given,
+                           const int32_t* gcd_given, MaskSummary* out, hipStream_t st);
+constexpr int32_t kMaxMaskedSearchRange = 12;
+// partials: [masked_cps_segments(g)][2][3][h], reduced by launch_cps_rows.
+hipError_t launch_cps_masked(const void* p0, const void* p1, const Geom& g, const void* mask, const Geom& gm,
+                             const MaskSummary* sum, int32_t B, const int32_t w[4], uint64_t* partials,
+                             hipStream_t st);
+size_t masked_cps_lds_bytes(int32_t B);
+int32_t masked_cps_segments(const Geom& g);
+
 // Q16 matrix (row-major) and range of an RGB <-> YCbCr conversion.
 struct ColourArgs {
   int32_t m[9];

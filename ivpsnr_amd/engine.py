@@ -229,6 +229,54 @@ class Engine:
                                          C.byref(t2r)), "qmiv_calc_ivssim_scp")
         return {"ivssim": iv.value, "r2t": r2t.value, "t2r": t2r.value}
 
+    # --- masked mode -----------------------------------------------------------------------------
+
+    def frame_masked(self, p0: Picture, p1: Picture, mask: Picture, metrics: int, params: Params | None = None,
+                     raise_on_range: bool = True) -> capi.MaskedResult:
+        """The application's masked flow (PSNR / WS-PSNR / IV-PSNR over the mask's non-zero pels)."""
+        r = capi.MaskedResult()
+        st = lib().qmiv_frame_masked(self.ctx, p0.handle, p1.handle, mask.handle, C.byref(params) if params else None,
+                                     metrics, C.byref(r))
+        if st != capi.QMIV_OK and (raise_on_range or st != 5):
+            check(st, "qmiv_frame_masked")
+        r.status = st
+        return r
+
+    def count_nonzero(self, mask: Picture) -> int:
+        n = C.c_int32()
+        check(lib().qmiv_count_nonzero(self.ctx, mask.handle, C.byref(n)), "qmiv_count_nonzero")
+        return n.value
+
+    def calc_gcd_masked(self, tst: Picture, ref: Picture, mask: Picture, nnm: int = -1,
+                        params: Params | None = None) -> np.ndarray:
+        g = (C.c_int32 * 4)()
+        check(lib().qmiv_calc_gcd_masked(self.ctx, tst.handle, ref.handle, mask.handle,
+                                         C.byref(params) if params else None, nnm, g), "qmiv_calc_gcd_masked")
+        return np.array(list(g), np.int32)
+
+    def calc_psnr_masked(self, tst: Picture, ref: Picture, mask: Picture, nnm: int = -1,
+                         params: Params | None = None) -> np.ndarray:
+        v = (C.c_double * 3)()
+        check(lib().qmiv_calc_psnr_masked(self.ctx, tst.handle, ref.handle, mask.handle,
+                                          C.byref(params) if params else None, nnm, v), "qmiv_calc_psnr_masked")
+        return np.array(list(v))
+
+    def calc_wspsnr_masked(self, tst: Picture, ref: Picture, mask: Picture, nnm: int = -1,
+                           params: Params | None = None) -> np.ndarray:
+        v = (C.c_double * 3)()
+        check(lib().qmiv_calc_wspsnr_masked(self.ctx, tst.handle, ref.handle, mask.handle,
+                                            C.byref(params) if params else None, nnm, v), "qmiv_calc_wspsnr_masked")
+        return np.array(list(v))
+
+    def calc_ivpsnr_masked(self, tst: Picture, ref: Picture, mask: Picture, gcd=None, nnm: int = -1,
+                           params: Params | None = None) -> dict:
+        g = (C.c_int32 * 4)(*[int(x) for x in gcd]) if gcd is not None else None
+        iv, r2t, t2r = C.c_double(), C.c_double(), C.c_double()
+        check(lib().qmiv_calc_ivpsnr_masked(self.ctx, tst.handle, ref.handle, mask.handle,
+                                            C.byref(params) if params else None, nnm, g, C.byref(iv), C.byref(r2t),
+                                            C.byref(t2r)), "qmiv_calc_ivpsnr_masked")
+        return {"ivpsnr": iv.value, "r2t": r2t.value, "t2r": t2r.value}
+
     def upsample(self, src: Picture) -> Picture:
         """A new 4:4:4 picture holding the nearest-neighbour upsampling of `src`."""
         dst = self.picture(src.width, src.height, src.bit_depth, 444)

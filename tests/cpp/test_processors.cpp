@@ -1,15 +1,17 @@
 // The reference application's per-frame processor calls (apps/QMIV/src/xAppQMIV.cpp:693-727,
 // 831-935), made through the C++ processor classes of include/qmiv.hpp on the MI355X engine.
 //
-//   test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1 OUT_CLR
+//   test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1 OUT_CLR [MASK BDM CFM]
 //
 // FILE0 / FILE1 hold one packed planar frame each (InputFile0 / InputFile1). Prints one JSON
 // object with every metric (doubles in %.17g) and writes the two shift-compensated pictures and
-// picture 0 converted RGB -> YCbCr (BT.709) as planar 4:4:4 16-bit files. Checks the identities the processor API promises and exits 1 on a
+// picture 0 converted RGB -> YCbCr (BT.709) as planar 4:4:4 16-bit files. With a mask file, the
+// masked processor calls of the application's masked flow are printed too ("masked"). Checks the identities the processor API promises and exits 1 on a
 // failed check.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "../../include/qmiv.hpp"
@@ -62,8 +64,8 @@ static void printV4(const char* Name, const flt64V4& V) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 11) {
-    std::fprintf(stderr, "usage: test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1 OUT_CLR\n");
+  if (argc != 11 && argc != 14) {
+    std::fprintf(stderr, "usage: test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1 OUT_CLR [MASK BDM CFM]\n");
     return 2;
   }
   const int32 W = std::atoi(argv[1]), H = std::atoi(argv[2]), BD = std::atoi(argv[3]), CF = std::atoi(argv[4]);
@@ -148,6 +150,36 @@ int main(int argc, char** argv) {
     writeScp(PicSCP0, argv[8]);
     writeScp(PicSCP1, argv[9]);
 
+    // Masked flow (xAppQMIV.cpp:695,825-889) through the masked processor methods.
+    std::string Masked;
+    if (argc == 14) {
+      const int32 BDM = std::atoi(argv[12]), CFM = std::atoi(argv[13]);
+      std::vector<uint8_t> FM = readFile(argv[11], qmiv_frame_bytes(W, H, BDM, CFM));
+      xPicP PicMsk(Ctx, {W, H}, BDM);
+      PicMsk.fromPacked(FM.data(), CFM);
+      const int32 NNM = countNonZero(Ctx, &PicMsk);
+      const int32V4 GM = ProcGCD.CalcGlobalColorDiffM(&PicInP0, &PicInP1, &PicMsk, NNM);
+      const flt64V4 PM = ProcPSNR.calcPicPSNRM(&PicInP0, &PicInP1, &PicMsk, NNM);
+      const flt64V4 WM = ProcPSNR.calcPicWSPSNRM(&PicInP0, &PicInP1, &PicMsk, NNM);
+      flt64 MR2T = 0, MT2R = 0;
+      int32 SeenNNM = -1;
+      ProcPSNR.setDebugCallbackQAP([&](flt64 A, flt64 B) { MR2T = A; MT2R = B; });
+      ProcPSNR.setDebugCallbackMSK([&](int32 N) { SeenNNM = N; });
+      const flt64 IM = ProcPSNR.calcPicIVPSNRM(&PicInP0, &PicInP1, &PicMsk, NNM, GM);
+      // The self-contained form counts the mask and finds the same colour difference.
+      int32V4 GAuto{};
+      ProcPSNR.setDebugCallbackGCS([&](const int32V4& G) { GAuto = G; });
+      CHECK(ProcPSNR.calcPicIVPSNRM(&PicInP0, &PicInP1, &PicMsk) == IM);
+      CHECK(GAuto == GM);
+      CHECK(SeenNNM == NNM);
+      char Buf[1024];
+      std::snprintf(Buf, sizeof(Buf),
+                    "\"masked\":{\"nnm\":%d,\"gcd\":[%d,%d,%d,%d],\"psnr\":[%.17g,%.17g,%.17g],"
+                    "\"wspsnr\":[%.17g,%.17g,%.17g],\"ivpsnr\":%.17g,\"r2t\":%.17g,\"t2r\":%.17g},",
+                    NNM, GM[0], GM[1], GM[2], GM[3], PM[0], PM[1], PM[2], WM[0], WM[1], WM[2], IM, MR2T, MT2R);
+      Masked = Buf;
+    }
+
     std::printf("{\"gcd\":[%d,%d,%d,%d],", GCD_R2T[0], GCD_R2T[1], GCD_R2T[2], GCD_R2T[3]);
     printV4("psnr", PSNR);
     printV4("wspsnr", WSPSNR);
@@ -156,7 +188,7 @@ int main(int argc, char** argv) {
     printV4("ssim_cs", CS);
     printV4("msssim", MSSSIM);
     std::printf("\"ivssim\":%.17g,\"ivssim_r2t\":%.17g,\"ivssim_t2r\":%.17g,", IVSSIM, SR2T, ST2R);
-    std::printf("\"failures\":%d}\n", g_Failures);
+    std::printf("%s\"failures\":%d}\n", Masked.c_str(), g_Failures);
   } catch (const xError& E) {
     std::fprintf(stderr, "error: %s\n", E.what());
     return 3;

@@ -9,7 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from ivpsnr_amd.synth import make_pair
+from ivpsnr_amd.synth import make_mask, make_pair
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -24,6 +24,10 @@ def close(a, b, rel=1e-9):
 @pytest.mark.parametrize("w,h,bd,cf,erp,seed", [(320, 192, 10, 420, True, 1), (256, 128, 8, 420, False, 2),
                                                 (96, 64, 12, 444, False, 3), (1920, 1080, 10, 420, False, 4)])
 def test_processor_classes(oracle, tmp_path, w, h, bd, cf, erp, seed):
+    run_processors(oracle, tmp_path, w, h, bd, cf, erp, seed, masked=seed % 2 == 1)
+
+
+def run_processors(oracle, tmp_path, w, h, bd, cf, erp, seed, masked):
     if not os.path.exists(BIN):
         pytest.fail(f"{BIN} not built (make -C tests/cpp)")
     pair = make_pair(w, h, bd, seed, chroma_format=cf)
@@ -32,8 +36,14 @@ def test_processor_classes(oracle, tmp_path, w, h, bd, cf, erp, seed):
     f0.write_bytes(pair.ref.tobytes())
     f1.write_bytes(pair.tst.tobytes())
     print(f"case {w}x{h} bd{bd} cf{cf} erp{int(erp)} seed{seed}", flush=True)
+    extra = []
+    if masked:
+        fm = tmp_path / "m.yuv"
+        mask = make_mask(w, h, 8, seed, chroma_format=420, kind="soft")
+        fm.write_bytes(mask.tobytes())
+        extra = [str(fm), "8", "420"]
     r = subprocess.run([BIN, str(w), str(h), str(bd), str(cf), str(f0), str(f1), str(int(erp)), str(s0), str(s1),
-                        str(sc)],
+                        str(sc)] + extra,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = json.loads(r.stdout)
@@ -53,3 +63,11 @@ def test_processor_classes(oracle, tmp_path, w, h, bd, cf, erp, seed):
     np.testing.assert_array_equal(np.fromfile(s0, np.uint16, n).reshape(3, h, w), scp0)
     np.testing.assert_array_equal(np.fromfile(s1, np.uint16, n).reshape(3, h, w), scp1)
     np.testing.assert_array_equal(np.fromfile(sc, np.uint16, n).reshape(3, h, w), oracle.colorspace(P0, bd, 1, False))
+    if masked:
+        # The processors start without fake values (xPSNR's default), the application flow with them
+        # for identical components: none here.
+        mo = oracle.frame_masked(pair.ref, pair.tst, mask, 8, 420, oracle.params(w, h, bd, cf, metrics=7, erp=erp))
+        gm = got["masked"]
+        assert gm["nnm"] == mo.num_non_masked and gm["gcd"] == list(mo.gcd)
+        assert gm["psnr"] == list(mo.psnr) and gm["wspsnr"] == list(mo.wspsnr)
+        assert [gm["ivpsnr"], gm["r2t"], gm["t2r"]] == [mo.ivpsnr, mo.r2t, mo.t2r]

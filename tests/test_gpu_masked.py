@@ -1,0 +1,158 @@
+"""Masked mode on the GPU: the frame flow and the processor-level calls against the reference's
+masked processors (tests/golden/masked.json, bit-exact) and against the oracle on larger, ragged
+and re-parameterised cases; the full mask against the unmasked flow; error paths."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ivpsnr_amd import default_params
+from ivpsnr_amd.capi import QmivError
+from ivpsnr_amd.synth import make_mask, make_pair, sha256
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "masked.json")
+with open(GOLDEN) as f:
+    MASKED = json.load(f)
+PSNRS = 1 | 2 | 4
+
+
+def upload(engine, case):
+    w, h, bd, cf = case["width"], case["height"], case["bit_depth"], case["chroma_format"]
+    pair = make_pair(w, h, bd, case["seed"], chroma_format=cf, **case["knobs"])
+    mask = make_mask(w, h, case["mask_bit_depth"], case["seed"], chroma_format=case["mask_chroma_format"],
+                     kind=case["mask_kind"])
+    assert sha256(pair.ref) == case["sha256_ref"] and sha256(mask) == case["sha256_mask"]
+    p0 = engine.picture(w, h, bd, cf, data=pair.ref)
+    p1 = engine.picture(w, h, bd, cf, data=pair.tst)
+    pm = engine.picture(w, h, case["mask_bit_depth"], case["mask_chroma_format"], data=mask)
+    return pair, mask, p0, p1, pm
+
+
+@pytest.mark.parametrize("case", MASKED, ids=lambda c: f"{c['width']}x{c['height']}b{c['bit_depth']}{c['mask_kind']}")
+@pytest.mark.parametrize("mode", ["plain", "erp"])
+def test_frame_masked_matches_reference(engine, case, mode):
+    print(f"masked {case['width']}x{case['height']} bd{case['bit_depth']} {case['mask_kind']} {mode}", flush=True)
+    _, _, p0, p1, pm = upload(engine, case)
+    want = case[mode]
+    r = engine.frame_masked(p0, p1, pm, PSNRS, default_params(erp=mode == "erp")).as_dict()
+    assert r["num_non_masked"] == want["num_non_masked"]
+    assert r["gcd"] == want["gcd"]
+    assert r["in_range"] == want["in_range"] + [want["in_range_mask"]]
+    assert r["exact"] == want["exact"]
+    for k in ("psnr", "wspsnr", "psnr_pic", "wspsnr_pic", "ivpsnr"):
+        assert r[k] == want[k], k
+    assert r["ivpsnr_r2t"] == want["r2t"] and r["ivpsnr_t2r"] == want["t2r"]
+    assert r["cps_sum"] == want["cps_rows_sum"]
+
+
+@pytest.mark.parametrize("case", MASKED[:5], ids=lambda c: f"{c['width']}x{c['height']}b{c['bit_depth']}")
+def test_processor_calls_match_reference(engine, case):
+    _, mask, p0, p1, pm = upload(engine, case)
+    plain, erp = case["plain"], case["erp"]
+    nnm = plain["num_non_masked"]
+    assert engine.count_nonzero(pm) == nnm
+    assert engine.calc_gcd_masked(p0, p1, pm).tolist() == plain["gcd"]
+    assert engine.calc_gcd_masked(p0, p1, pm, nnm).tolist() == plain["gcd"]
+    assert engine.calc_psnr_masked(p0, p1, pm, nnm).tolist() == plain["psnr"]
+    assert engine.calc_psnr_masked(p0, p1, pm).tolist() == plain["psnr"]
+    assert engine.calc_wspsnr_masked(p0, p1, pm, nnm).tolist() == plain["wspsnr"]
+    assert engine.calc_wspsnr_masked(p0, p1, pm, nnm, default_params(erp=True)).tolist() == erp["wspsnr"]
+    auto = engine.calc_ivpsnr_masked(p0, p1, pm)
+    assert [auto["ivpsnr"], auto["r2t"], auto["t2r"]] == [plain["ivpsnr"], plain["r2t"], plain["t2r"]]
+    given = engine.calc_ivpsnr_masked(p0, p1, pm, plain["gcd"], nnm, default_params(erp=True))
+    assert [given["ivpsnr"], given["r2t"], given["t2r"]] == [erp["ivpsnr"], erp["r2t"], erp["t2r"]]
+
+
+@pytest.mark.parametrize("w,h,bd,cf,bdm,kind,erp", [
+    (4096, 2048, 10, 420, 8, "objects", True),
+    (1920, 1080, 10, 420, 10, "soft", False),
+    (333, 199, 12, 444, 8, "soft", False),
+    (2, 2, 8, 420, 8, "full", False),
+])
+def test_frame_masked_matches_oracle(engine, oracle, w, h, bd, cf, bdm, kind, erp):
+    print(f"masked oracle {w}x{h} bd{bd} cf{cf} {kind}", flush=True)
+    pair = make_pair(w, h, bd, 31, chroma_format=cf, luma_offset=4.0)
+    mask = make_mask(w, h, bdm, 31, chroma_format=420 if w % 2 == 0 and h % 2 == 0 else 444, kind=kind)
+    cfm = 420 if w % 2 == 0 and h % 2 == 0 else 444
+    p0 = engine.picture(w, h, bd, cf, data=pair.ref)
+    p1 = engine.picture(w, h, bd, cf, data=pair.tst)
+    pm = engine.picture(w, h, bdm, cfm, data=mask)
+    r = engine.frame_masked(p0, p1, pm, PSNRS, default_params(erp=erp)).as_dict()
+    o = oracle.frame_masked(pair.ref, pair.tst, mask, bdm, cfm,
+                            oracle.params(w, h, bd, cf, metrics=PSNRS, erp=erp)).as_dict()
+    for k in ("num_non_masked", "gcd", "in_range", "exact", "psnr", "wspsnr", "psnr_pic", "wspsnr_pic", "ivpsnr",
+              "cps_sum"):
+        assert r[k] == o[k], k
+    assert r["ivpsnr_r2t"] == o["r2t"] and r["ivpsnr_t2r"] == o["t2r"]
+
+
+@pytest.mark.parametrize("B,wts", [(0, (4, 1, 1, 0)), (1, (4, 1, 1, 0)), (5, (1, 2, 3, 0)), (12, (4, 1, 1, 0))])
+def test_search_parameters_match_oracle(engine, oracle, B, wts):
+    w, h, bd = 150, 90, 10
+    pair = make_pair(w, h, bd, 41, luma_offset=6.0)
+    mask = make_mask(w, h, 8, 41, kind="soft")
+    P0, P1 = oracle.unpack(pair.ref, w, h, bd), oracle.unpack(pair.tst, w, h, bd)
+    M = oracle.unpack(mask, w, h, 8)[0]
+    p0 = engine.picture(w, h, bd, 420, data=pair.ref)
+    p1 = engine.picture(w, h, bd, 420, data=pair.tst)
+    pm = engine.picture(w, h, 8, 420, data=mask)
+    prm = default_params()
+    prm.search_range = B
+    for i in range(4):
+        prm.cmp_weights_search[i] = wts[i]
+    g = [3, -2, 1, 2]  # a non-zero fourth entry exercises the fourth lane of the error
+    r = engine.calc_ivpsnr_masked(p0, p1, pm, g, -1, prm)
+    nnm = int((M != 0).sum())
+    rows_r2t = oracle.cps_rows_masked(P1, P0, M, [-v for v in g], B, wts)
+    rows_t2r = oracle.cps_rows_masked(P0, P1, M, g, B, wts)
+    want = []
+    for rows in (rows_r2t, rows_t2r):
+        err = [float(rows[c].sum()) for c in range(3)]  # exact uint64 sums, then double
+        q = [oracle.psnr_from_masked_ssd(e if e > 0 else 1.0, nnm, bd, 8) for e in err]
+        want.append((((q[0] * 4 + q[1] * 1) + q[2] * 1) + 0.0 * 0) * (1.0 / 6))
+    assert r["r2t"] == want[0] and r["t2r"] == want[1]
+
+
+def test_full_mask_equals_unmasked_flow(engine):
+    w, h, bd = 320, 180, 10
+    pair = make_pair(w, h, bd, 51, luma_offset=5.0)
+    mask = make_mask(w, h, 8, 51, kind="full")
+    p0 = engine.picture(w, h, bd, 420, data=pair.ref)
+    p1 = engine.picture(w, h, bd, 420, data=pair.tst)
+    pm = engine.picture(w, h, 8, 420, data=mask)
+    for erp in (False, True):
+        prm = default_params(erp=erp)
+        prm.cps_int32_rows = 0
+        m = engine.frame_masked(p0, p1, pm, PSNRS, prm)
+        u = engine.frame(p0, p1, PSNRS, prm)
+        # Without WS every value is a quotient of the same integers scaled by 255: identical. With
+        # WS the weighted row sums round differently, so only to 1e-12.
+        same = (lambda a, b: a == b) if not erp else (lambda a, b: a == pytest.approx(b, rel=1e-12))
+        assert list(m.gcd) == list(u.gcd)
+        assert list(m.psnr) == list(u.psnr)
+        assert all(same(a, b) for a, b in zip(m.wspsnr, u.wspsnr))
+        assert same(m.ivpsnr_r2t, u.ivpsnr_t2r) and same(m.ivpsnr_t2r, u.ivpsnr_r2t) and same(m.ivpsnr, u.ivpsnr)
+        assert [list(x) for x in m.cps_sum] == [[v * 255 for v in u.cps_sum[1]], [v * 255 for v in u.cps_sum[0]]]
+
+
+def test_errors_and_range_flags(engine):
+    w, h = 64, 32
+    pair = make_pair(w, h, 10, 61)
+    p0 = engine.picture(w, h, 10, 420, data=pair.ref)
+    p1 = engine.picture(w, h, 10, 420, data=pair.tst)
+    bad = make_mask(w, h, 8, 61, kind="soft")
+    m9 = engine.picture(w, h, 9, 444, data=np.full(3 * w * h, 600, np.uint16))  # 600 > 511
+    r = engine.frame_masked(p0, p1, m9, PSNRS, raise_on_range=False)
+    assert r.status == 5 and list(r.in_range) == [1, 1, 0]
+    pm = engine.picture(w, h, 8, 420, data=bad)
+    with pytest.raises(QmivError):
+        engine.frame_masked(p0, p1, pm, PSNRS | 8)  # SSIM with a mask
+    with pytest.raises(QmivError):
+        engine.frame_masked(p0, p1, engine.picture(w + 2, h, 8, 420), PSNRS)
+    prm = default_params()
+    prm.search_range = 13
+    with pytest.raises(QmivError):
+        engine.frame_masked(p0, p1, pm, PSNRS, prm)
