@@ -10,7 +10,8 @@ instead (its dominant kernel is the fp64 windowed-moment kernel). `--workload mu
 IV-PSNR + WS-PSNR + IV-SSIM, as one captured graph replay per frame set against the plain
 per-view call loop; `multiview24-ivpsnr` is the same without IV-SSIM. `--workload grad-ivssim1024`
 times the differentiable IV-SSIM (forward and gradients of both pictures) on 8 x 3 x 1024 x 1024
-fp32 pictures.
+fp32 pictures. `--workload masked-1080p8` times masked mode (PSNR + WS-PSNR + IV-PSNR over the
+pels of an 8-bit object mask) on 8 frame pairs 1920x1080 yuv420p.
 
 Prints one JSON line:
   value      whole-job Mpix/s over the timed steps
@@ -278,12 +279,105 @@ def run_grad(args):
     eng.close()
 
 
+def run_masked(args):
+    """Masked PSNR + WS-PSNR + IV-PSNR of 8 frame pairs 1920x1080 yuv420p with an 8-bit object mask
+    (binary ellipses at 255 over about a third of the picture), one qmiv_frame_masked per frame."""
+    from ivpsnr_amd import Engine, default_params
+    from ivpsnr_amd.synth import make_mask, make_pair
+    W, H, BD, nframes = 1920, 1080, 8, args.frames or 8
+    print(f"[bench] masked-1080p8: generating {nframes} frame pairs and masks", file=sys.stderr, flush=True)
+    pairs = [make_pair(W, H, BD, seed=1 + f) for f in range(nframes)]
+    masks = [make_mask(W, H, 8, seed=1 + f, kind="objects") for f in range(nframes)]
+    prm = default_params()
+    eng = Engine(0)
+    pics = [(eng.picture(W, H, BD, 420, data=p.ref), eng.picture(W, H, BD, 420, data=p.tst),
+             eng.picture(W, H, 8, 420, data=m)) for p, m in zip(pairs, masks)]
+    eng.sync()
+    metrics = 1 | 2 | 4
+
+    def step():
+        return [eng.frame_masked(a, b, m, metrics, prm) for a, b, m in pics]
+
+    def timed(steps, warmup, profile):
+        for _ in range(warmup):
+            step()
+        eng.profile(profile)
+        eng.profile_read(reset=True)
+        eng.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = step()
+        eng.sync()
+        el = time.perf_counter() - t0
+        pr = eng.profile_read(reset=True)
+        eng.profile(False)
+        return el, pr, r
+
+    print(f"[bench] masked: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
+    elapsed, prof, res = timed(args.steps, args.warmup, True)
+    el2, _, _ = timed(args.steps, 0, False)
+    elapsed = min(elapsed, el2)
+    ms_per_step = elapsed / args.steps * 1e3
+    mpix = nframes * W * H / 1e6
+    k_ms, k_n = prof.get("cps", (0.0, 0))
+    k_avg_ms = k_ms / max(k_n, 1)
+    per_launch = frame_pair_bytes(W, H, BD) + W * H * 3 // 2  # two pictures and the mask frame
+    achieved = per_launch / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        ref_bin = os.path.join(ROOT, "oracle", "_ref", "qmiv_ref_v3")
+        cores = min(16, os.cpu_count() or 1)
+        if os.path.exists(ref_bin):
+            with tempfile.TemporaryDirectory() as td:
+                f0, f1, fm = (os.path.join(td, n) for n in ("p0.yuv", "p1.yuv", "m.yuv"))
+                sample = 4
+                with open(f0, "wb") as a, open(f1, "wb") as b, open(fm, "wb") as c:
+                    for i in range(sample):
+                        a.write(pairs[i].ref.tobytes())
+                        b.write(pairs[i].tst.tobytes())
+                        c.write(masks[i].tobytes())
+                cmd = [ref_bin, "-w", str(W), "-h", str(H), "-bd", str(BD), "-cf", "420", "-i0", f0, "-i1", f1,
+                       "-im", fm, "-bdm", "8", "-cfm", "420", "-ml", "PSNR,WSPSNR,IVPSNR", "-nth", str(cores),
+                       "-time"]
+                try:
+                    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+                    if r.returncode == 0:
+                        t = json.loads(r.stdout)["time_ms_per_frame"]
+                        ms = sum(t[k] for k in ("validate", "preproc", "gcd", "psnr", "wspsnr", "ivpsnr"))
+                        cpu = {"value": round(W * H / (ms * 1e3), 3), "unit": "Mpix/s", "cores": cores,
+                               "kind": "reference",
+                               "sample": f"{sample} frames {W}x{H} 8-bit with their masks (PSNR,WSPSNR,IVPSNR "
+                                         f"masked) incl. validation, preprocessing and GCD, {cores} threads; "
+                                         f"{ms:.1f} ms/frame"}
+                except (subprocess.TimeoutExpired, OSError, ValueError, KeyError):
+                    pass
+    out = {
+        "metric": "masked IV-PSNR Mpix/s @1920x1080 8b 420",
+        "value": round(mpix / (ms_per_step / 1e3), 2),
+        "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "masked PSNR+WS-PSNR+IV-PSNR, 8 frame pairs 1920x1080 yuv420p, 8-bit object "
+                               "masks, B=2, GCD on", "frames_per_step": nframes,
+                   "ivpsnr_last_frame": round(res[-1].ivpsnr, 6), "nnm_last_frame": res[-1].num_non_masked},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic("k_cps_masked_1920x1080_8b.txt"),
+                     "kernel": "k_cps_masked", "kernel_avg_us": round(k_avg_ms * 1e3, 2),
+                     "bytes_per_launch": per_launch},
+        "kernels_us_per_frame": {k: round(v[0] / nframes / args.steps * 1e3, 2) for k, v in prof.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["multiview24", "multiview24-ivpsnr", "grad-ivssim1024"],
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["multiview24", "multiview24-ivpsnr", "grad-ivssim1024",
+                                                               "masked-1080p8"],
                     default="ivpsnr-erp4k10")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--frames", type=int, default=0, help="frame pairs per step (default: the workload's)")
@@ -303,6 +397,9 @@ def main():
         return
     if args.workload == "grad-ivssim1024":
         run_grad(args)
+        return
+    if args.workload == "masked-1080p8":
+        run_masked(args)
         return
 
     from ivpsnr_amd import Engine, default_params

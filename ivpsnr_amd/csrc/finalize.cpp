@@ -46,26 +46,58 @@ void ws_weights(int32_t height, int32_t lat_range_deg, std::vector<double>& weig
 namespace {
 // One component's error of the asymmetric quality: lane-wise xKBNS4 of row * w[y] from zero
 // with WS, else the uint64 sum of the rows (std::accumulate).
-double rows_error(const uint64_t* r, int32_t height, const double* ws) {
-  if (ws) {
-    double s = 0.0, k = 0.0;
-    for (int32_t y = 0; y < height; y++) neumaier(s, k, (double)r[y] * ws[y]);
-    return s + k;
+// Three Neumaier sums from zero of r_c[y >> sh_c] << sh_c times ws[y], interleaved so that their
+// dependency chains overlap; each sum performs the operations of its own loop in the same order.
+void kbns3_rows(const uint64_t* const r[3], const int sh[3], int32_t height, const double* ws, double out[3]) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, k0 = 0.0, k1 = 0.0, k2 = 0.0;
+  for (int32_t y = 0; y < height; y++) {
+    const double w = ws[y];
+    const double v0 = (double)(r[0][y >> sh[0]] << sh[0]) * w;
+    const double v1 = (double)(r[1][y >> sh[1]] << sh[1]) * w;
+    const double v2 = (double)(r[2][y >> sh[2]] << sh[2]) * w;
+    const double t0 = s0 + v0, t1 = s1 + v1, t2 = s2 + v2;
+    k0 += std::fabs(s0) >= std::fabs(v0) ? ((s0 - t0) + v0) : ((v0 - t0) + s0);
+    k1 += std::fabs(s1) >= std::fabs(v1) ? ((s1 - t1) + v1) : ((v1 - t1) + s1);
+    k2 += std::fabs(s2) >= std::fabs(v2) ? ((s2 - t2) + v2) : ((v2 - t2) + s2);
+    s0 = t0;
+    s1 = t1;
+    s2 = t2;
   }
-  uint64_t s = 0;
-  for (int32_t y = 0; y < height; y++) s += r[y];
-  return (double)s;
+  out[0] = s0 + k0;
+  out[1] = s1 + k1;
+  out[2] = s2 + k2;
 }
+
+// Errors of the three components of an asymmetric quality from rows [3][height].
+void rows_errors3(const uint64_t* rows, int32_t height, const double* ws, double err[3]) {
+  const uint64_t* r[3] = {rows, rows + (size_t)height, rows + 2 * (size_t)height};
+  if (ws) {
+    const int sh[3] = {0, 0, 0};
+    kbns3_rows(r, sh, height, ws, err);
+    return;
+  }
+  for (int c = 0; c < 3; c++) {
+    uint64_t s = 0;
+    for (int32_t y = 0; y < height; y++) s += r[c][y];
+    err[c] = (double)s;
+  }
+}
+
 }  // namespace
 
 double qual_from_rows(const uint64_t* rows, int32_t height, const double* ws, int32_t area, int32_t bit_depth,
                       const int32_t wavg[4]) {
-  double q[3];
-  for (int c = 0; c < 3; c++) {
-    const double err = rows_error(rows + (size_t)c * (size_t)height, height, ws);
-    q[c] = psnr_from_ssd(err > 0 ? err : 1.0, area, bit_depth);
-  }
+  double err[3], q[3];
+  rows_errors3(rows, height, ws, err);
+  for (int c = 0; c < 3; c++) q[c] = psnr_from_ssd(err[c] > 0 ? err[c] : 1.0, area, bit_depth);
   return weighted_average4(q, wavg);
+}
+
+void wspsnr3_from_rows(const uint64_t* const rows[3], const int shift[3], int32_t height, const double* ws,
+                       double correction, int32_t area, int32_t bit_depth, double out[3]) {
+  double err[3];
+  kbns3_rows(rows, shift, height, ws, err);
+  for (int c = 0; c < 3; c++) out[c] = psnr_from_ssd(err[c] * correction, area, bit_depth);
 }
 
 double psnr_from_masked_ssd(double ssd, int32_t nnm, int32_t bit_depth, int32_t mask_bit_depth) {
@@ -77,11 +109,9 @@ double psnr_from_masked_ssd(double ssd, int32_t nnm, int32_t bit_depth, int32_t 
 
 double qual_from_rows_masked(const uint64_t* rows, int32_t height, const double* ws, int32_t nnm, int32_t bit_depth,
                              int32_t mask_bit_depth, const int32_t wavg[4]) {
-  double q[3];
-  for (int c = 0; c < 3; c++) {
-    const double err = rows_error(rows + (size_t)c * (size_t)height, height, ws);
-    q[c] = psnr_from_masked_ssd(err > 0 ? err : 1.0, nnm, bit_depth, mask_bit_depth);
-  }
+  double err[3], q[3];
+  rows_errors3(rows, height, ws, err);
+  for (int c = 0; c < 3; c++) q[c] = psnr_from_masked_ssd(err[c] > 0 ? err[c] : 1.0, nnm, bit_depth, mask_bit_depth);
   return weighted_average4(q, wavg);
 }
 

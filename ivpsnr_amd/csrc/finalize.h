@@ -40,6 +40,12 @@ double qual_from_rows_masked(const uint64_t* rows, int32_t height, const double*
 double wspsnr_from_rows_masked(const uint64_t* rows444, int32_t height, const double* ws, double correction,
                                int32_t nnm, int32_t bit_depth, int32_t mask_bit_depth);
 
+// WS-PSNR of the three components at once (their weighted KBNS sums interleaved, each one's
+// arithmetic unchanged): row y of component c is rows[c][y >> shift[c]] << shift[c] (a 4:2:0
+// chroma plane's row SSD doubled horizontally and repeated vertically onto the 4:4:4 rows).
+void wspsnr3_from_rows(const uint64_t* const rows[3], const int shift[3], int32_t height, const double* ws,
+                       double correction, int32_t area, int32_t bit_depth, double out[3]);
+
 // Component-weighted picture value as xMetricStat::setPerCmpMeric (apps/QMIV/src/xAppQMIV.h:245-251).
 double pic_value(const double v[3], const int32_t wavg[4]);
 

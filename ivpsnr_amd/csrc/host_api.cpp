@@ -55,6 +55,16 @@ struct qmiv_ctx {
   // so that the next frame's kernels do not wait for the copy engine.
   hipStream_t copy_stream = nullptr;
   std::vector<hipEvent_t> copy_ev;
+  // Recorded after each frame's results copy: the host finalises frame i while later frames run.
+  std::vector<hipEvent_t> done_ev;
+  // Spherical weights and correction per (height, latitude range): the host finalisation's cosines
+  // are computed once per size.
+  struct WsTable {
+    int32_t h, lat;
+    std::vector<double> w;
+    double corr;
+  };
+  std::vector<WsTable> ws_cache;
   // A frame set alternates its frames over `stream` and `aux_stream` (independent scratch), so one
   // frame's short kernels overlap the other's search; fork/join events order them with the rest.
   hipStream_t aux_stream = nullptr;
@@ -110,6 +120,18 @@ qmiv_status make_geom(int32_t w, int32_t h, int32_t bd, int32_t cf, Geom& g) {
 
 bool same_geom(const Geom& a, const Geom& b) {
   return a.w == b.w && a.h == b.h && a.bd == b.bd && a.cs == b.cs;
+}
+
+// xWeightedSpherically::initWS for this height and latitude range, computed once per context.
+const qmiv_ctx::WsTable& ws_table(qmiv_ctx* ctx, int32_t h, int32_t lat) {
+  for (const auto& t : ctx->ws_cache)
+    if (t.h == h && t.lat == lat) return t;
+  qmiv_ctx::WsTable t;
+  t.h = h;
+  t.lat = lat;
+  fin::ws_weights(h, lat, t.w, t.corr);
+  ctx->ws_cache.push_back(std::move(t));
+  return ctx->ws_cache.back();
 }
 
 qmiv_status ensure_scratch(qmiv_ctx* ctx, size_t bytes) {
@@ -479,7 +501,8 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
 // `scp` (2 x [3][h][w] uint16) and the results block copied into pinned `hbase`.
 qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params& prm,
                           const FrameWork& fw, const FramePlan& P, char* dbase, char* hbase, uint16_t* scp,
-                          hipStream_t st, hipStream_t copy_st = nullptr, hipEvent_t copy_ev = nullptr) {
+                          hipStream_t st, hipStream_t copy_st = nullptr, hipEvent_t copy_ev = nullptr,
+                          hipEvent_t done_ev = nullptr) {
   const Geom& g = p0->g;
   const Layout& L = P.L;
   FrameSummary* sum = (FrameSummary*)(dbase + L.off_summary);
@@ -540,8 +563,10 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     QMIV_HIP(hipEventRecord(copy_ev, st));
     QMIV_HIP(hipStreamWaitEvent(copy_st, copy_ev, 0));
     QMIV_HIP(hipMemcpyAsync(hbase, dbase, L.results_bytes, hipMemcpyDeviceToHost, copy_st));
+    if (done_ev) QMIV_HIP(hipEventRecord(done_ev, copy_st));
   } else {
     QMIV_HIP(hipMemcpyAsync(hbase, dbase, L.results_bytes, hipMemcpyDeviceToHost, st));
+    if (done_ev) QMIV_HIP(hipEventRecord(done_ev, st));
   }
   return QMIV_OK;
 }
@@ -596,8 +621,8 @@ qmiv_status check_pair(const qmiv_pic* a, const qmiv_pic* b) {
 }
 
 // Host finalisation of one frame from its results block (apps/QMIV/src/xAppQMIV.cpp:831-904).
-qmiv_status finalize_frame(const Geom& g, const qmiv_params& prm, uint32_t metrics, const FrameWork& fw,
-                           const Layout& L, const char* hb, qmiv_frame_result* out) {
+qmiv_status finalize_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, uint32_t metrics,
+                           const FrameWork& fw, const Layout& L, const char* hb, qmiv_frame_result* out) {
   std::memset(out, 0, sizeof(*out));
   const FrameSummary* sum = (const FrameSummary*)(hb + L.off_summary);
   const uint64_t* h_cps = (const uint64_t*)(hb + L.off_cps_rows);
@@ -612,9 +637,13 @@ qmiv_status finalize_frame(const Geom& g, const qmiv_params& prm, uint32_t metri
 
   const int32_t area = g.w * g.h;
   const double fake = fin::psnr_from_ssd(1.0, area, g.bd);
-  std::vector<double> ws;
+  const double* ws = nullptr;
   double corr = 1.0;
-  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  if (prm.erp) {
+    const qmiv_ctx::WsTable& t = ws_table(ctx, g.h, prm.lat_range_deg);
+    ws = t.w.data();
+    corr = t.corr;
+  }
   if (metrics & QMIV_PSNR) {
     for (int c = 0; c < 3; c++) {
       const double v = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
@@ -623,17 +652,17 @@ qmiv_status finalize_frame(const Geom& g, const qmiv_params& prm, uint32_t metri
     out->psnr_pic = fin::pic_value(out->psnr, prm.cmp_weights_average);
   }
   if (metrics & QMIV_WSPSNR) {
-    std::vector<uint64_t> r;
-    for (int c = 0; c < 3; c++) {
-      double v;
-      if (prm.erp) {
-        rows444(g, h_ssd, c, r);
-        v = fin::wspsnr_from_rows(r.data(), g.h, ws.data(), corr, area, g.bd);
-      } else {
-        v = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
-      }
-      out->wspsnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : v;
+    double v[3];
+    if (prm.erp) {
+      // Per-plane row SSDs onto the 4:4:4 rows: a 4:2:0 chroma row counts twice horizontally and
+      // serves two 4:4:4 rows.
+      const uint64_t* r[3] = {h_ssd, h_ssd + g.h, h_ssd + g.h + g.ch};
+      const int sh[3] = {0, g.cs, g.cs};
+      fin::wspsnr3_from_rows(r, sh, g.h, ws, corr, area, g.bd, v);
+    } else {
+      for (int c = 0; c < 3; c++) v[c] = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
     }
+    for (int c = 0; c < 3; c++) out->wspsnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : v[c];
     out->wspsnr_pic = fin::pic_value(out->wspsnr, prm.cmp_weights_average);
   }
   if (metrics & QMIV_IVPSNR) {
@@ -645,7 +674,7 @@ qmiv_status finalize_frame(const Geom& g, const qmiv_params& prm, uint32_t metri
         for (int32_t y = 0; y < g.h; y++) s += rows[((size_t)d * 3 + c) * g.h + y];
         out->cps_sum[d][c] = s;
       }
-    const double* wsp = prm.erp ? ws.data() : nullptr;
+    const double* wsp = ws;
     out->ivpsnr_r2t = fin::qual_from_rows(rows.data(), g.h, wsp, area, g.bd, prm.cmp_weights_average);
     out->ivpsnr_t2r = fin::qual_from_rows(rows.data() + 3 * (size_t)g.h, g.h, wsp, area, g.bd, prm.cmp_weights_average);
     out->ivpsnr = fin::min2(out->ivpsnr_r2t, out->ivpsnr_t2r);
@@ -1031,6 +1060,7 @@ void qmiv_destroy(qmiv_ctx* ctx) {
   if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
   if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
   for (hipEvent_t ev : ctx->copy_ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : ctx->done_ev) (void)hipEventDestroy(ev);
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
   if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
@@ -1177,6 +1207,11 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
     QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ctx->copy_ev.push_back(e);
   }
+  while ((int32_t)ctx->done_ev.size() < n) {
+    hipEvent_t e;
+    QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->done_ev.push_back(e);
+  }
   const hipStream_t streams[2] = {ctx->stream, ctx->aux_stream};
   if (nstreams > 1) {  // the aux stream starts after everything already on the context stream
     QMIV_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
@@ -1187,7 +1222,7 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
     const qmiv_status st = enqueue_frame(ctx, p0[i], p1[i], prm, fw, plans[(size_t)i],
                                          (char*)ctx->d_scratch + doff[(size_t)i], (char*)ctx->h_results + hoff[(size_t)i],
                                          scp ? scp + scp_stride * (size_t)k : nullptr, streams[k], ctx->copy_stream,
-                                         ctx->copy_ev[(size_t)i]);
+                                         ctx->copy_ev[(size_t)i], ctx->done_ev[(size_t)i]);
     if (st != QMIV_OK) {
       // Drain what was enqueued: scratch and result buffers must be idle before any later reuse.
       (void)hipStreamSynchronize(ctx->stream);
@@ -1200,13 +1235,16 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
     QMIV_HIP(hipEventRecord(ctx->join_ev, ctx->aux_stream));
     QMIV_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
   }
-  QMIV_TRY(wait_stream(ctx));
+  // Each frame is finalised on the host as soon as its results have arrived, while the device
+  // works on the frames after it.
   qmiv_status first_err = QMIV_OK;
   for (int32_t i = 0; i < n; i++) {
-    qmiv_status st = finalize_frame(p0[i]->g, prm, metrics, fw, plans[(size_t)i].L,
+    QMIV_HIP(hipEventSynchronize(ctx->done_ev[(size_t)i]));
+    qmiv_status st = finalize_frame(ctx, p0[i]->g, prm, metrics, fw, plans[(size_t)i].L,
                                     (const char*)ctx->h_results + hoff[(size_t)i], &out[i]);
     if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
   }
+  QMIV_TRY(wait_stream(ctx));
   return first_err;
 }
 
@@ -1266,9 +1304,9 @@ qmiv_status qmiv_calc_wspsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
     for (int c = 0; c < 3; c++) wspsnr[c] = fin::psnr_from_ssd((double)sum->ssd[c], area, g.bd);
     return QMIV_OK;
   }
-  std::vector<double> ws;
-  double corr;
-  fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  const qmiv_ctx::WsTable& wst = ws_table(ctx, g.h, prm.lat_range_deg);
+  const std::vector<double>& ws = wst.w;
+  const double corr = wst.corr;
   std::vector<uint64_t> r;
   for (int c = 0; c < 3; c++) {
     rows444(g, h_ssd, c, r);
@@ -1299,9 +1337,10 @@ qmiv_status qmiv_calc_ivpsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
   const Geom& g = tst->g;
   std::vector<uint64_t> rows((const uint64_t*)(hb + L.off_cps_rows), (const uint64_t*)(hb + L.off_cps_rows) + 6 * (size_t)g.h);
   apply_row_semantics(rows.data(), rows.size(), prm.cps_int32_rows != 0);
-  std::vector<double> ws;
-  double corr;
-  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  static const std::vector<double> kNoWs;
+  const qmiv_ctx::WsTable* wst = prm.erp ? &ws_table(ctx, g.h, prm.lat_range_deg) : nullptr;
+  const std::vector<double>& ws = wst ? wst->w : kNoWs;
+  const double corr = wst ? wst->corr : 1.0;
   const double* wsp = prm.erp ? ws.data() : nullptr;
   const int32_t area = g.w * g.h;
   const double q0 = fin::qual_from_rows(rows.data(), g.h, wsp, area, g.bd, prm.cmp_weights_average);
@@ -1558,7 +1597,7 @@ qmiv_status qmiv_batch_run(qmiv_batch* b, qmiv_frame_result* out) {
   qmiv_status first_err = QMIV_OK;
   for (int32_t i = 0; i < b->n; i++) {
     const size_t k = (size_t)i;
-    const qmiv_status st = finalize_frame(b->p0[k]->g, b->prm[k], b->metrics[k], b->fw[k], b->plans[k].L,
+    const qmiv_status st = finalize_frame(b->ctx, b->p0[k]->g, b->prm[k], b->metrics[k], b->fw[k], b->plans[k].L,
                                           (const char*)b->h_results + b->hoff[k], &out[i]);
     if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
   }
@@ -1953,9 +1992,10 @@ qmiv_status qmiv_frame_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic*
   const double fake = fin::psnr_from_ssd(1.0, g.w * g.h, g.bd);
   double plain[3];
   for (int c = 0; c < 3; c++) plain[c] = fin::psnr_from_masked_ssd((double)sum->ssd[c], nnm, g.bd, bdm);
-  std::vector<double> ws;
-  double corr = 1.0;
-  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  static const std::vector<double> kNoWs;
+  const qmiv_ctx::WsTable* wst = prm.erp ? &ws_table(ctx, g.h, prm.lat_range_deg) : nullptr;
+  const std::vector<double>& ws = wst ? wst->w : kNoWs;
+  const double corr = wst ? wst->corr : 1.0;
   if (metrics & QMIV_PSNR) {
     for (int c = 0; c < 3; c++) out->psnr[c] = (prm.fake_psnr_for_exact && out->exact[c]) ? fake : plain[c];
     out->psnr_pic = fin::pic_value(out->psnr, prm.cmp_weights_average);
@@ -2045,9 +2085,9 @@ qmiv_status qmiv_calc_wspsnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qm
   const uint64_t* rows = (const uint64_t*)((const char*)ctx->h_results + L.off_ssd);
   const Geom& g = tst->g;
   const int32_t nnm = num_non_masked >= 0 ? num_non_masked : sum->nnm;
-  std::vector<double> ws;
-  double corr;
-  fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  const qmiv_ctx::WsTable& wst = ws_table(ctx, g.h, prm.lat_range_deg);
+  const std::vector<double>& ws = wst.w;
+  const double corr = wst.corr;
   for (int c = 0; c < 3; c++) {
     const uint64_t* r = rows + (size_t)c * g.h;
     wspsnr[c] = fin::wspsnr_from_rows_masked(r, g.h, ws.data(), corr, nnm, g.bd, mask->g.bd);
@@ -2074,9 +2114,10 @@ qmiv_status qmiv_calc_ivpsnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qm
   if (!sum->all_in_range) return fail(QMIV_ERR_PEL_RANGE, "input or mask contains pels above the bit-depth maximum");
   const Geom& g = tst->g;
   const int32_t nnm = nnm_arg >= 0 ? nnm_arg : sum->nnm;
-  std::vector<double> ws;
-  double corr;
-  if (prm.erp) fin::ws_weights(g.h, prm.lat_range_deg, ws, corr);
+  static const std::vector<double> kNoWs;
+  const qmiv_ctx::WsTable* wst = prm.erp ? &ws_table(ctx, g.h, prm.lat_range_deg) : nullptr;
+  const std::vector<double>& ws = wst ? wst->w : kNoWs;
+  const double corr = wst ? wst->corr : 1.0;
   const uint64_t* rows = (const uint64_t*)((const char*)ctx->h_results + L.off_cps);
   const double* w = prm.erp ? ws.data() : nullptr;
   const double a = fin::qual_from_rows_masked(rows, g.h, w, nnm, g.bd, mask->g.bd, prm.cmp_weights_average);
