@@ -138,6 +138,29 @@ qmiv_status qmiv_pic_upload(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size
 qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, size_t bytes);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Raw planar YUV sequences (the read side of xSeq, libs/pmbb-core/src/xSeq.cpp)               */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct qmiv_seq qmiv_seq;
+
+/* Opens a raw planar file of packed frames (the layout of qmiv_pic_upload) for reading. The frame
+   count is floor(file size / frame bytes): a partial last frame is ignored (xSeq::xBackendOpen,
+   xSeq.cpp:451-470; frame bytes as xSeq::create, :408-428). Host-only: no device is touched. */
+qmiv_status qmiv_seq_open(const char* path, int32_t width, int32_t height, int32_t bit_depth,
+                          int32_t chroma_format, qmiv_seq** out);
+int32_t qmiv_seq_num_frames(const qmiv_seq* seq);
+size_t qmiv_seq_frame_bytes(const qmiv_seq* seq);
+/* Reads frame `index` (seek by frame, xSeq.cpp:519-525) into host memory of the frame's size. */
+qmiv_status qmiv_seq_read_host(qmiv_seq* seq, int32_t index, void* dst, size_t bytes);
+/* Reads frame `index` into a device picture of the sequence's geometry, ordered on the context's
+   stream (work enqueued afterwards sees the frame): the file is read into a pinned buffer (by a
+   background thread when the frame was prefetched: reading frame i prefetches frame i + 1) and
+   copied asynchronously. Replaces xSeqBase::readFrame (xSeq.cpp:155-174) plus the unpack, which
+   the engine does not need. A sequence serves one device. */
+qmiv_status qmiv_seq_read(qmiv_ctx* ctx, qmiv_seq* seq, int32_t index, qmiv_pic* pic);
+/* Waits for outstanding copies out of the sequence's buffers, then closes the file. */
+void qmiv_seq_close(qmiv_seq* seq);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Timing (replaces the reference's per-stage RDTSC timing, apps/QMIV/src/xAppQMIV.cpp:670-744) */
 /* ------------------------------------------------------------------------------------------ */
 

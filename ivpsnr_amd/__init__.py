@@ -5,8 +5,8 @@ include/qmiv.h); this package is the Python view of that ABI plus the synthetic 
 generator used by the tests and the benchmark.
 """
 from .capi import IVPSNR, IVSSIM, MSSSIM, PSNR, SSIM, WSPSNR  # noqa: F401
-from .engine import Engine, Picture, default_params  # noqa: F401
+from .engine import Engine, Picture, Sequence, default_params  # noqa: F401
 
 ALL_METRICS = PSNR | WSPSNR | IVPSNR | SSIM | MSSSIM | IVSSIM
 
-__all__ = ["Engine", "Picture", "default_params", "ALL_METRICS", "PSNR", "WSPSNR", "IVPSNR", "SSIM", "MSSSIM", "IVSSIM"]
+__all__ = ["Engine", "Picture", "Sequence", "default_params", "ALL_METRICS", "PSNR", "WSPSNR", "IVPSNR", "SSIM", "MSSSIM", "IVSSIM"]

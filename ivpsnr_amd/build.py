@@ -24,7 +24,7 @@ HOST_HIP_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__
 # The double-precision finalisation must round like the reference's ISO C++ build.
 FINALIZE_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-fno-fast-math"]
 
-HEADERS = ["qmiv_internal.h", "finalize.h"]
+HEADERS = ["qmiv_internal.h", "finalize.h", "host_internal.h"]
 
 
 def _units():

@@ -125,6 +125,12 @@ _SIGNATURES = {
     "qmiv_calc_ivssim_scp": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), _f64p, _f64p,
                                        _f64p]),
     "qmiv_pic_upsample": (C.c_int, [_ctx_p, _pic_p, _pic_p]),
+    "qmiv_seq_open": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "qmiv_seq_num_frames": (C.c_int32, [C.c_void_p]),
+    "qmiv_seq_frame_bytes": (C.c_size_t, [C.c_void_p]),
+    "qmiv_seq_read_host": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),
+    "qmiv_seq_read": (C.c_int, [_ctx_p, C.c_void_p, C.c_int32, _pic_p]),
+    "qmiv_seq_close": (None, [C.c_void_p]),
     "qmiv_frame_masked": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), C.c_uint32,
                                     C.POINTER(MaskedResult)]),
     "qmiv_count_nonzero": (C.c_int, [_ctx_p, _pic_p, _i32p]),

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "finalize.h"
+#include "host_internal.h"
 #include "qmiv_internal.h"
 
 using namespace qmiv;
@@ -41,56 +42,7 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 }  // namespace
 
-struct qmiv_pic {
-  qmiv_ctx* ctx;
-  Geom g;
-  void* dev;
-  bool owned;
-};
-
-struct qmiv_ctx {
-  int32_t device = 0;
-  hipStream_t stream = nullptr;
-  // Results copies of a frame set run here, each after its frame's kernels (an event per frame),
-  // so that the next frame's kernels do not wait for the copy engine.
-  hipStream_t copy_stream = nullptr;
-  std::vector<hipEvent_t> copy_ev;
-  // Recorded after each frame's results copy: the host finalises frame i while later frames run.
-  std::vector<hipEvent_t> done_ev;
-  // Spherical weights and correction per (height, latitude range): the host finalisation's cosines
-  // are computed once per size.
-  struct WsTable {
-    int32_t h, lat;
-    std::vector<double> w;
-    double corr;
-  };
-  std::vector<WsTable> ws_cache;
-  // A frame set alternates its frames over `stream` and `aux_stream` (independent scratch), so one
-  // frame's short kernels overlap the other's search; fork/join events order them with the rest.
-  hipStream_t aux_stream = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-  // Differentiable SSIM family: float workspace and the quantised pictures of the shift search.
-  void* d_fwork = nullptr;
-  size_t fwork_bytes = 0;
-  qmiv_pic* fq[2] = {nullptr, nullptr};
-  // Device scratch (grown on demand; one frame evaluation at a time per context).
-  void* d_scratch = nullptr;
-  size_t scratch_bytes = 0;
-  // Pinned host block receiving the per-frame results.
-  void* h_results = nullptr;
-  size_t results_bytes = 0;
-  // Shift-compensated pictures (IV-SSIM), 2 x [3][h][w] uint16.
-  void* d_scp = nullptr;
-  size_t scp_bytes = 0;
-  int32_t num_cus = 256;
-  int32_t cps_blocks_per_cu[2][2] = {{0, 0}, {0, 0}};  // [4:4:4 kernel, 4:2:0 kernel][u8, u16], queried on first use
-  // Kernel timing: event pairs recorded around launches since the last harvest.
-  bool profiling = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
-  std::vector<int> ev_kind;
-  size_t ev_used = 0;
-  qmiv_profile prof{};
-};
+qmiv_status qmiv::host_fail(qmiv_status st, const std::string& msg) { return fail(st, msg); }
 
 namespace {
 

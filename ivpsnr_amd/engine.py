@@ -4,6 +4,7 @@ benchmark. Every call goes through libqmiv.so; nothing here computes a metric.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import weakref
 
 import numpy as np
@@ -49,6 +50,45 @@ class Picture:
         if self.handle:
             lib().qmiv_pic_destroy(self.handle)
             self.handle = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Sequence:
+    """A raw planar YUV file of packed frames (qmiv_seq_*): frame count, host reads and reads into
+    device pictures through pinned buffers with one-frame prefetch."""
+
+    def __init__(self, path: str, width: int, height: int, bit_depth: int, chroma_format: int = 420):
+        h = C.c_void_p()
+        check(lib().qmiv_seq_open(os.fsencode(path), width, height, bit_depth, chroma_format, C.byref(h)),
+              "qmiv_seq_open")
+        self.handle = h
+        self.width, self.height, self.bit_depth, self.chroma_format = width, height, bit_depth, chroma_format
+        self.num_frames = lib().qmiv_seq_num_frames(h)
+        self.frame_bytes = lib().qmiv_seq_frame_bytes(h)
+
+    def __len__(self) -> int:
+        return self.num_frames
+
+    def read_host(self, index: int) -> np.ndarray:
+        dt = np.uint8 if self.bit_depth <= 8 else np.uint16
+        out = np.empty(self.frame_bytes // np.dtype(dt).itemsize, dt)
+        check(lib().qmiv_seq_read_host(self.handle, index, out.ctypes.data, out.nbytes), "qmiv_seq_read_host")
+        return out
+
+    def read(self, engine: "Engine", index: int, pic: "Picture") -> "Picture":
+        """Frame `index` into `pic`, ordered on the engine's stream (no wait)."""
+        check(lib().qmiv_seq_read(engine.ctx, self.handle, index, pic.handle), "qmiv_seq_read")
+        return pic
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().qmiv_seq_close(self.handle)
+        self.handle = None
 
     def __del__(self):
         try:
