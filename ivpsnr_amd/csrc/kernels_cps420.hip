@@ -49,12 +49,20 @@ constexpr int kQW = kLW / 4;
 constexpr int kFillItems = kLH * kQW;
 constexpr int kFillSteps = (kFillItems + 255) / 256;
 
+// The pels one thread loads for one fill, kept as the loaded words (a 4:2:0 row quad of luma,
+// two samples of each chroma plane) from the load, issued a whole search phase ahead, to the store
+// into LDS; nothing reads them in between, so the loads stay in flight across the search.
+//   16-bit samples: l0 = Y0 | Y1 << 16, l1 = Y2 | Y3 << 16, u = U0 | U1 << 16, v = V0 | V1 << 16
+//   8-bit samples:  l0 = Y0 | Y1 << 8 | Y2 << 16 | Y3 << 24, u = U0 | U1 << 8, v = V0 | V1 << 8
+struct FillRegs {
+  uint32_t l0[kFillSteps], l1[kFillSteps], u[kFillSteps], v[kFillSteps];
+};
+
 template <typename T>
-__device__ __forceinline__ void fill420(const Buf<T>& Yp, const Geom& g, int x0, int y0, int32_t w0, int32_t w1,
-                                        int32_t w2, uint2* s_luma, uint32_t* s_cell) {
+__device__ __forceinline__ void fill420_load(const Buf<T>& Yp, const Geom& g, int x0, int y0, FillRegs& fr) {
   const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
   const bool interior = x0 >= 4 && x0 + kTW + 4 <= g.w && y0 >= 2 && y0 + kTH + 2 <= g.h && (g.w & 3) == 0;
-  uint32_t yv[kFillSteps][4], cu[kFillSteps][2], cv[kFillSteps][2];
+  constexpr uint32_t sh = 8 * sizeof(T);
 #pragma unroll
   for (int k = 0; k < kFillSteps; k++) {
     const int i = threadIdx.x + 256 * k;
@@ -64,99 +72,174 @@ __device__ __forceinline__ void fill420(const Buf<T>& Yp, const Geom& g, int x0,
     if (interior) {
       const int gy = y0 - 2 + ly;
       const int gx = x0 - 4 + lx;
-      const uint32_t lo = (uint32_t)(gy * g.w + gx);
-      const uint32_t co = (uint32_t)((gy >> 1) * g.cw + (gx >> 1));
+      const uint32_t lo = __umul24((uint32_t)gy, (uint32_t)g.w) + (uint32_t)gx;
+      const uint32_t co = __umul24((uint32_t)(gy >> 1), (uint32_t)g.cw) + (uint32_t)(gx >> 1);
       if constexpr (sizeof(T) == 2) {
         const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(Yp.r, lo * 2u, 0, 0));
-        yv[k][0] = w.x & 0xffffu; yv[k][1] = w.x >> 16; yv[k][2] = w.y & 0xffffu; yv[k][3] = w.y >> 16;
-        const uint32_t wu = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, (cb_off + co) * 2u, 0, 0);
-        const uint32_t wv = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, (cr_off + co) * 2u, 0, 0);
-        cu[k][0] = wu & 0xffffu; cu[k][1] = wu >> 16; cv[k][0] = wv & 0xffffu; cv[k][1] = wv >> 16;
+        fr.l0[k] = w.x;
+        fr.l1[k] = w.y;
+        fr.u[k] = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, (cb_off + co) * 2u, 0, 0);
+        fr.v[k] = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, (cr_off + co) * 2u, 0, 0);
       } else {
-        const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, lo, 0, 0);
-        yv[k][0] = w & 0xffu; yv[k][1] = (w >> 8) & 0xffu; yv[k][2] = (w >> 16) & 0xffu; yv[k][3] = w >> 24;
-        const uint32_t wu = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cb_off + co, 0, 0);
-        const uint32_t wv = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cr_off + co, 0, 0);
-        cu[k][0] = wu & 0xffu; cu[k][1] = wu >> 8; cv[k][0] = wv & 0xffu; cv[k][1] = wv >> 8;
+        fr.l0[k] = __builtin_amdgcn_raw_buffer_load_b32(Yp.r, lo, 0, 0);
+        fr.l1[k] = 0u;
+        fr.u[k] = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cb_off + co, 0, 0);
+        fr.v[k] = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cr_off + co, 0, 0);
       }
     } else {
       const int gy = clampi(y0 - 2 + ly, 0, g.h - 1);
-      const uint32_t rowo = (uint32_t)(gy * g.w);
-      const uint32_t crow = (uint32_t)((gy >> 1) * g.cw);
+      const uint32_t rowo = __umul24((uint32_t)gy, (uint32_t)g.w);
+      const uint32_t crow = __umul24((uint32_t)(gy >> 1), (uint32_t)g.cw);
+      uint32_t yq[4], uq[2], vq[2];
 #pragma unroll
-      for (int m = 0; m < 4; m++) yv[k][m] = Yp.ld(rowo + (uint32_t)clampi(x0 - 4 + lx + m, 0, g.w - 1));
+      for (int m = 0; m < 4; m++) yq[m] = Yp.ld(rowo + (uint32_t)clampi(x0 - 4 + lx + m, 0, g.w - 1));
 #pragma unroll
       for (int c = 0; c < 2; c++) {
         const uint32_t co = crow + (uint32_t)(clampi(x0 - 4 + lx + 2 * c, 0, g.w - 1) >> 1);
-        cu[k][c] = Yp.ld(cb_off + co);
-        cv[k][c] = Yp.ld(cr_off + co);
+        uq[c] = Yp.ld(cb_off + co);
+        vq[c] = Yp.ld(cr_off + co);
       }
+      if constexpr (sizeof(T) == 2) {
+        fr.l0[k] = yq[0] | (yq[1] << 16);
+        fr.l1[k] = yq[2] | (yq[3] << 16);
+      } else {
+        fr.l0[k] = yq[0] | (yq[1] << 8) | (yq[2] << 16) | (yq[3] << 24);
+        fr.l1[k] = 0u;
+      }
+      fr.u[k] = uq[0] | (uq[1] << sh);
+      fr.v[k] = vq[0] | (vq[1] << sh);
     }
   }
+}
+
+// Builds the LDS records {Y0, C} and chroma cells of one fill from the loaded words.
+template <typename T>
+__device__ __forceinline__ void fill420_store(const FillRegs& fr, int32_t w0, int32_t w1, int32_t w2, uint2* s_luma,
+                                              uint32_t* s_cell) {
+  constexpr uint32_t sh = 8 * sizeof(T);
+  constexpr uint32_t mask = sizeof(T) == 2 ? 0xffffu : 0xffu;
 #pragma unroll
   for (int k = 0; k < kFillSteps; k++) {
     const int i = threadIdx.x + 256 * k;
     if (kFillItems % 256 != 0 && k == kFillSteps - 1 && i >= kFillItems) break;
     const int ly = i / kQW;
     const int lx = (i - ly * kQW) * 4;
-    const int32_t sc0 = w1 * (int32_t)(cu[k][0] * cu[k][0]) + w2 * (int32_t)(cv[k][0] * cv[k][0]);
-    const int32_t sc1 = w1 * (int32_t)(cu[k][1] * cu[k][1]) + w2 * (int32_t)(cv[k][1] * cv[k][1]);
+    uint32_t yv[4];
+    if constexpr (sizeof(T) == 2) {
+      yv[0] = fr.l0[k] & 0xffffu; yv[1] = fr.l0[k] >> 16; yv[2] = fr.l1[k] & 0xffffu; yv[3] = fr.l1[k] >> 16;
+    } else {
+      yv[0] = fr.l0[k] & 0xffu; yv[1] = (fr.l0[k] >> 8) & 0xffu; yv[2] = (fr.l0[k] >> 16) & 0xffu; yv[3] = fr.l0[k] >> 24;
+    }
+    const uint32_t cu[2] = {fr.u[k] & mask, (fr.u[k] >> sh) & mask};
+    const uint32_t cv[2] = {fr.v[k] & mask, (fr.v[k] >> sh) & mask};
+    // Squares and weights in unsigned 24-bit multiplies (full rate): the host's eligibility check
+    // (non-negative weights, keys below 2^31) keeps samples <= 4095, so every operand, a square
+    // included, is below 2^24 and every product below 2^31.
+    const int32_t sc0 = (int32_t)__umul24((uint32_t)w1, __umul24(cu[0], cu[0])) + (int32_t)__umul24((uint32_t)w2, __umul24(cv[0], cv[0]));
+    const int32_t sc1 = (int32_t)__umul24((uint32_t)w1, __umul24(cu[1], cu[1])) + (int32_t)__umul24((uint32_t)w2, __umul24(cv[1], cv[1]));
     const int32_t h = 5 * ly + lx;
     uint4 r01, r23;
-    r01.x = yv[k][0]; r01.y = (uint32_t)(32 * (w0 * (int32_t)(yv[k][0] * yv[k][0]) + sc0) + h);
-    r01.z = yv[k][1]; r01.w = (uint32_t)(32 * (w0 * (int32_t)(yv[k][1] * yv[k][1]) + sc0) + h + 1);
-    r23.x = yv[k][2]; r23.y = (uint32_t)(32 * (w0 * (int32_t)(yv[k][2] * yv[k][2]) + sc1) + h + 2);
-    r23.z = yv[k][3]; r23.w = (uint32_t)(32 * (w0 * (int32_t)(yv[k][3] * yv[k][3]) + sc1) + h + 3);
+    r01.x = yv[0]; r01.y = (uint32_t)(32 * ((int32_t)__umul24((uint32_t)w0, __umul24(yv[0], yv[0])) + sc0) + h);
+    r01.z = yv[1]; r01.w = (uint32_t)(32 * ((int32_t)__umul24((uint32_t)w0, __umul24(yv[1], yv[1])) + sc0) + h + 1);
+    r23.x = yv[2]; r23.y = (uint32_t)(32 * ((int32_t)__umul24((uint32_t)w0, __umul24(yv[2], yv[2])) + sc1) + h + 2);
+    r23.z = yv[3]; r23.w = (uint32_t)(32 * ((int32_t)__umul24((uint32_t)w0, __umul24(yv[3], yv[3])) + sc1) + h + 3);
     uint4* dst = reinterpret_cast<uint4*>(s_luma + ly * kLW + lx);
     dst[0] = r01;
     dst[1] = r23;
     if ((ly & 1) == 0)
       *reinterpret_cast<uint2*>(s_cell + (ly >> 1) * kCW + (lx >> 1)) =
-          make_uint2(cu[k][0] | (cv[k][0] << 16), cu[k][1] | (cv[k][1] << 16));
+          make_uint2(cu[0] | (cv[0] << 16), cu[1] | (cv[1] << 16));
   }
 }
 
 // One direction over one tile. acc[r][c]: the lane's per-row, per-component sums (both of its
 // columns) kept across the tile sweep.
-template <typename T, int DIR>
+// The tested pels of one lane for one phase, as loaded words: per block b, the two luma rows of
+// its 2 x 2 pels (one 32-bit load of two 16-bit samples or one 16-bit load of two 8-bit samples)
+// and its chroma cell's Cb and Cr samples. Loaded a phase ahead, like FillRegs.
+struct XRegs {
+  uint32_t row[kKB][2], cb[kKB], cr[kKB];
+};
+
+template <typename T>
+__device__ __forceinline__ void x420_load(const Buf<T>& Xp, const Geom& g, int x0, int y0, int lane, int wave,
+                                          XRegs& xr) {
+  const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
+  const int bx = min(x0 + 2 * lane, g.w - 2);  // block origin (even), clamped for the tile's tail
+#pragma unroll
+  for (int b = 0; b < kKB; b++) {
+    const int by = min(y0 + 2 * (kKB * wave + b), g.h - 2);
+    const uint32_t co = __umul24((uint32_t)(by >> 1), (uint32_t)g.cw) + (uint32_t)(bx >> 1);
+    xr.cb[b] = Xp.ld(cb_off + co);
+    xr.cr[b] = Xp.ld(cr_off + co);
+#pragma unroll
+    for (int py = 0; py < 2; py++) {
+      // Even width and even bx: the pair is aligned to its own size.
+      const uint32_t o = __umul24((uint32_t)(by + py), (uint32_t)g.w) + (uint32_t)bx;
+      if constexpr (sizeof(T) == 2) xr.row[b][py] = __builtin_amdgcn_raw_buffer_load_b32(Xp.r, o * 2u, 0, 0);
+      else xr.row[b][py] = __builtin_amdgcn_raw_buffer_load_b16(Xp.r, o, 0, 0);
+    }
+  }
+}
+
+// OUT: bit 0 writes the shift-compensated pels, bit 1 the winning offsets (both optional outputs
+// of the search, chosen at launch so that the plain IV-PSNR search carries neither).
+// fr and xr hold this phase's fill and tested pels, loaded during the previous phase; once they are
+// consumed the loads of the next phase's (fill of picture next_fill, tested pels of next_x, tile
+// column next_x0, when has_next) are issued into them, so their latency hides behind this phase's
+// search.
+template <typename T, int DIR, int OUT>
 __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int x0, int y0, int lane, int wave,
                                             const int32_t (&gcd)[3], uint2* s_luma, uint32_t* s_cell,
-                                            uint32_t (&acc)[2 * kKB][3]) {
+                                            uint32_t (&acc)[2 * kKB][3], FillRegs& fr, XRegs& xr,
+                                            const void* next_fill, const void* next_x, int next_x0, bool has_next) {
   // Opaque copies: keeps per-row address arithmetic from being hoisted out of the tile sweep and
   // held in registers across it.
   asm volatile("" : "+v"(lane));
   asm volatile("" : "+v"(wave));
   // R2T (DIR 0): P0 + gcd searched in P1; T2R (DIR 1): P1 - gcd searched in P0.
-  const Buf<T> Xp(DIR == 0 ? a.p0 : a.p1, g.frame_bytes);
-  const Buf<T> Yp(DIR == 0 ? a.p1 : a.p0, g.frame_bytes);
   uint16_t* scp_ptr = DIR == 0 ? a.scp1 : a.scp0;
   int16_t* best_out = DIR == 0 ? a.best0 : a.best1;
   constexpr int32_t sgn = DIR == 0 ? 1 : -1;
   const int32_t w0 = a.ws[0], w1 = a.ws[1], w2 = a.ws[2];
-  const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
-
-  __syncthreads();
-  fill420<T>(Yp, g, x0, y0, w0, w1, w2, s_luma, s_cell);
-  __syncthreads();
 
   // Tested values X = pel + sgn * gcd (not clipped: xc_CLIP_CURR_TST_RANGE = false,
-  // libs/pmbb-ivqm/src/xCommonDefIVQM.h:36) and the key coefficients a_c = -64 w_c X_c.
-  const int32_t m0 = -64 * w0, m1 = -64 * w1, m2 = -64 * w2;
-  const int bx = min(x0 + 2 * lane, g.w - 2);  // block origin, clamped for the tile's tail
+  // libs/pmbb-ivqm/src/xCommonDefIVQM.h:36).
+  constexpr uint32_t smask = sizeof(T) == 2 ? 0xffffu : 0xffu;
+  constexpr uint32_t ssh = 8 * sizeof(T);
   int32_t x0v[kKB][2][2], a0[kKB][2][2], x1v[kKB], x2v[kKB], a1[kKB], a2[kKB];
 #pragma unroll
   for (int b = 0; b < kKB; b++) {
-    const int by = min(y0 + 2 * (kKB * wave + b), g.h - 2);
-    const uint32_t co = (uint32_t)((by >> 1) * g.cw + (bx >> 1));
-    x1v[b] = (int32_t)Xp.ld(cb_off + co) + sgn * gcd[1];
-    x2v[b] = (int32_t)Xp.ld(cr_off + co) + sgn * gcd[2];
+    x1v[b] = (int32_t)xr.cb[b];
+    x2v[b] = (int32_t)xr.cr[b];
+#pragma unroll
+    for (int py = 0; py < 2; py++) {
+      x0v[b][py][0] = (int32_t)(xr.row[b][py] & smask);
+      x0v[b][py][1] = (int32_t)((xr.row[b][py] >> ssh) & smask);
+    }
+  }
+
+  __syncthreads();
+  fill420_store<T>(fr, w0, w1, w2, s_luma, s_cell);
+  __syncthreads();
+  if (has_next) {
+    fill420_load<T>(Buf<T>(next_fill, g.frame_bytes), g, next_x0, y0, fr);
+    x420_load<T>(Buf<T>(next_x, g.frame_bytes), g, next_x0, y0, lane, wave, xr);
+  }
+
+  // The key coefficients a_c = -64 w_c X_c.
+  const int32_t m0 = -64 * w0, m1 = -64 * w1, m2 = -64 * w2;
+#pragma unroll
+  for (int b = 0; b < kKB; b++) {
+    x1v[b] += sgn * gcd[1];
+    x2v[b] += sgn * gcd[2];
     a1[b] = __mul24(m1, x1v[b]);
     a2[b] = __mul24(m2, x2v[b]);
 #pragma unroll
     for (int py = 0; py < 2; py++)
 #pragma unroll
       for (int px = 0; px < 2; px++) {
-        x0v[b][py][px] = (int32_t)Xp.ld((uint32_t)((by + py) * g.w + bx + px)) + sgn * gcd[0];
+        x0v[b][py][px] += sgn * gcd[0];
         a0[b][py][px] = __mul24(m0, x0v[b][py][px]);
       }
   }
@@ -232,7 +315,9 @@ __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int
     __builtin_amdgcn_sched_barrier(0);
   }
 
-  // Winning candidates: per-component squared errors (and the shift-compensated pels).
+  // Winning candidates: per-component squared errors (and the optional outputs). Pels beyond the
+  // picture (a partial last tile or band) add nothing; interior tiles skip the test.
+  const bool full = x0 + kTW <= g.w && y0 + kTH <= g.h;
 #pragma unroll
   for (int b = 0; b < kKB; b++)
 #pragma unroll
@@ -241,35 +326,40 @@ __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int
       for (int px = 0; px < 2; px++) {
         const int ly = 2 + 2 * (kKB * wave + b) + py;
         const int lx = 4 + 2 * lane + px;
-        const int idx = (best[b][py][px] - (5 * ly + lx) + 12) & 31;
-        const int dy = idx / 5;
-        const int dx = idx - dy * 5;
+        const int idx = (best[b][py][px] - (5 * ly + lx) + 12) & 31;  // 5 (dy + 2) + (dx + 2)
+        const int dy = (int)(__umul24((uint32_t)idx, 13u) >> 6);     // idx / 5 for idx <= 24
+        const int dx = idx - 5 * dy;
         const int qy = ly - 2 + dy, qx = lx - 2 + dx;
         const uint32_t yv = s_luma[qy * kLW + qx].x;
         const uint32_t uv = s_cell[(qy >> 1) * kCW + (qx >> 1)];
+        // |d| < 2^23: the low 32 bits of the 24-bit products are the reference's int32 squares.
         const int32_t d0 = x0v[b][py][px] - (int32_t)yv;
         const int32_t d1 = x1v[b] - (int32_t)(uv & 0xffffu);
         const int32_t d2 = x2v[b] - (int32_t)(uv >> 16);
         const int y = y0 + 2 * (kKB * wave + b) + py;
         const int x = x0 + 2 * lane + px;
-        if (x < g.w && y < g.h) {
-          acc[2 * b + py][0] += (uint32_t)(d0 * d0);
-          acc[2 * b + py][1] += (uint32_t)(d1 * d1);
-          acc[2 * b + py][2] += (uint32_t)(d2 * d2);
-          if (best_out) best_out[y * g.w + x] = (int16_t)idx;
-          if (scp_ptr) {
-            const Buf<uint16_t> scp(scp_ptr, 6u * (uint32_t)g.w * (uint32_t)g.h);
-            const uint32_t o = (uint32_t)(y * g.w + x);
-            const uint32_t plane = (uint32_t)(g.w * g.h);
-            scp.st(o, (uint32_t)clampi((int32_t)yv - sgn * gcd[0], 0, a.maxv));
-            scp.st(plane + o, (uint32_t)clampi((int32_t)(uv & 0xffffu) - sgn * gcd[1], 0, a.maxv));
-            scp.st(2 * plane + o, (uint32_t)clampi((int32_t)(uv >> 16) - sgn * gcd[2], 0, a.maxv));
+        const bool inside = full || (x < g.w && y < g.h);
+        const uint32_t keep = inside ? 0xffffffffu : 0u;
+        acc[2 * b + py][0] += (uint32_t)__mul24(d0, d0) & keep;
+        acc[2 * b + py][1] += (uint32_t)__mul24(d1, d1) & keep;
+        acc[2 * b + py][2] += (uint32_t)__mul24(d2, d2) & keep;
+        if constexpr ((OUT & 3) != 0) {
+          if (inside) {
+            const uint32_t o = __umul24((uint32_t)y, (uint32_t)g.w) + (uint32_t)x;
+            if constexpr ((OUT & 2) != 0) best_out[o] = (int16_t)idx;
+            if constexpr ((OUT & 1) != 0) {
+              const Buf<uint16_t> scp(scp_ptr, 6u * (uint32_t)g.w * (uint32_t)g.h);
+              const uint32_t plane = (uint32_t)(g.w * g.h);
+              scp.st(o, (uint32_t)clampi((int32_t)yv - sgn * gcd[0], 0, a.maxv));
+              scp.st(plane + o, (uint32_t)clampi((int32_t)(uv & 0xffffu) - sgn * gcd[1], 0, a.maxv));
+              scp.st(2 * plane + o, (uint32_t)clampi((int32_t)(uv >> 16) - sgn * gcd[2], 0, a.maxv));
+            }
           }
         }
       }
 }
 
-template <typename T>
+template <typename T, int OUT>
 __global__ __launch_bounds__(256) void k_cps_fast420(CpsArgs a) {
   __shared__ __attribute__((aligned(16))) uint2 s_luma[kLH * kLW];
   __shared__ __attribute__((aligned(16))) uint32_t s_cell[kCH * kCW];
@@ -293,10 +383,19 @@ __global__ __launch_bounds__(256) void k_cps_fast420(CpsArgs a) {
 #pragma unroll
     for (int c = 0; c < 3; c++) acc0[r][c] = acc1[r][c] = 0;
 
+  // R2T searches P1, T2R searches P0: the fills alternate P1, P0 tile after tile, each loaded one
+  // phase ahead.
+  FillRegs fr;
+  XRegs xr;
+  if (tx_begin < tx_end) {
+    fill420_load<T>(Buf<T>(a.p1, g.frame_bytes), g, tx_begin * kTW, y0, fr);
+    x420_load<T>(Buf<T>(a.p0, g.frame_bytes), g, tx_begin * kTW, y0, lane, wave, xr);
+  }
   for (int tx = tx_begin; tx < tx_end; tx++) {
     const int x0 = tx * kTW;
-    tile_dir420<T, 0>(a, g, x0, y0, lane, wave, gcd, s_luma, s_cell, acc0);
-    tile_dir420<T, 1>(a, g, x0, y0, lane, wave, gcd, s_luma, s_cell, acc1);
+    tile_dir420<T, 0, OUT>(a, g, x0, y0, lane, wave, gcd, s_luma, s_cell, acc0, fr, xr, a.p0, a.p1, x0, true);
+    tile_dir420<T, 1, OUT>(a, g, x0, y0, lane, wave, gcd, s_luma, s_cell, acc1, fr, xr, a.p1, a.p0, x0 + kTW,
+                           tx + 1 < tx_end);
   }
 
   // One partial per (segment, direction, component, row): reduce over the wave's 64 lanes.
@@ -322,10 +421,25 @@ hipError_t launch_cps_fast420(const CpsArgs& a, hipStream_t stream) {
   const int bands = (a.g.h + kTH - 1) / kTH;
   if (a.tiles_x != (a.g.w + kTW - 1) / kTW) return hipErrorInvalidValue;
   dim3 grid(bands * a.nseg), block(256);
-  if (a.g.bytes_per_sample == 1)
-    hipLaunchKernelGGL(k_cps_fast420<uint8_t>, grid, block, 0, stream, a);
-  else
-    hipLaunchKernelGGL(k_cps_fast420<uint16_t>, grid, block, 0, stream, a);
+  // Both directions' outputs are requested together (SCP for IV-SSIM, offsets for inspection).
+  if ((a.scp0 == nullptr) != (a.scp1 == nullptr) || (a.best0 == nullptr) != (a.best1 == nullptr))
+    return hipErrorInvalidValue;
+  const int out = (a.scp0 ? 1 : 0) | (a.best0 ? 2 : 0);
+  if (a.g.bytes_per_sample == 1) {
+    switch (out) {
+      case 0: k_cps_fast420<uint8_t, 0><<<grid, block, 0, stream>>>(a); break;
+      case 1: k_cps_fast420<uint8_t, 1><<<grid, block, 0, stream>>>(a); break;
+      case 2: k_cps_fast420<uint8_t, 2><<<grid, block, 0, stream>>>(a); break;
+      default: k_cps_fast420<uint8_t, 3><<<grid, block, 0, stream>>>(a); break;
+    }
+  } else {
+    switch (out) {
+      case 0: k_cps_fast420<uint16_t, 0><<<grid, block, 0, stream>>>(a); break;
+      case 1: k_cps_fast420<uint16_t, 1><<<grid, block, 0, stream>>>(a); break;
+      case 2: k_cps_fast420<uint16_t, 2><<<grid, block, 0, stream>>>(a); break;
+      default: k_cps_fast420<uint16_t, 3><<<grid, block, 0, stream>>>(a); break;
+    }
+  }
   return hipGetLastError();
 }
 
@@ -337,8 +451,8 @@ void cps_fast420_tile(int32_t* tile_w, int32_t* tile_h) {
 int cps_fast420_blocks_per_cu(int bytes_per_sample) {
   int n = 0;
   hipError_t e = bytes_per_sample == 1
-                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast420<uint8_t>, 256, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast420<uint16_t>, 256, 0);
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast420<uint8_t, 0>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast420<uint16_t, 0>, 256, 0);
   return e == hipSuccess && n > 0 ? n : 1;
 }
 
