@@ -369,8 +369,15 @@ __global__ __launch_bounds__(256) void k_cps_fast420(CpsArgs a) {
   const Geom g = a.g;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int band = blockIdx.x / a.nseg;
-  const int seg = blockIdx.x - band * a.nseg;
+  // Workgroups are dealt to the XCDs round-robin; renumber them so that each XCD takes a contiguous
+  // run of (band, segment) work and vertically adjacent bands share their halo rows in one L2.
+#ifndef QMIV_CPS420_XCD
+#define QMIV_CPS420_XCD 1
+#endif
+  int bid = blockIdx.x;
+  if (QMIV_CPS420_XCD && (gridDim.x & 7) == 0) bid = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  const int band = bid / a.nseg;
+  const int seg = bid - band * a.nseg;
   const int y0 = band * kTH;
   const int tx_begin = seg * a.sweep;
   const int tx_end = min(a.tiles_x, tx_begin + a.sweep);
