@@ -17,9 +17,10 @@
 // Every key keeps its exact value (only the grouping of the minimum changes), so the winner and
 // its tie-break are identical. Per candidate: one 24-bit multiply-add and a minimum.
 //
-// Workgroup: 4 waves, tile 128 x 16 pels; each lane owns two vertically stacked 2 x 2 blocks
-// (one tested chroma cell each). LDS holds the searched picture's luma records {Y0, C} for
-// rows [y0 - 2, y0 + 18) x columns [x0 - 4, x0 + 132) and its chroma cells {U | V << 16}.
+// Workgroup: 4 waves, tile 128 x 8 pels; each lane owns one 2 x 2 block (one tested chroma cell;
+// QMIV_CPS420_KB stacks more blocks per lane, which measured slower: fewer waves fit a SIMD). LDS
+// holds the searched picture's luma records {Y0, C} for rows [y0 - 2, y0 + 10) x columns
+// [x0 - 4, x0 + 132) and its chroma cells {U | V << 16}.
 #include <climits>
 
 #include "device_common.h"
@@ -30,9 +31,9 @@ namespace {
 
 constexpr int kTW = 128;          // tile columns: 64 lanes x 2
 #ifndef QMIV_CPS420_KB
-#define QMIV_CPS420_KB 2
+#define QMIV_CPS420_KB 1
 #endif
-constexpr int kKB = QMIV_CPS420_KB;  // 2 x 2 blocks per lane, stacked vertically
+constexpr int kKB = QMIV_CPS420_KB;  // 2 x 2 blocks per lane, stacked vertically (1: 104 VGPRs, 4 waves per SIMD)
 constexpr int kTH = 4 * 2 * kKB;  // tile rows: 4 waves x 2 kKB
 constexpr int kLW = kTW + 8;      // luma record columns [x0 - 4, x0 + kTW + 4)
 constexpr int kLH = kTH + 4;      // luma record rows [y0 - 2, y0 + kTH + 2)
