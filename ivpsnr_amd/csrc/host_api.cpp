@@ -5,9 +5,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "finalize.h"
@@ -667,7 +673,72 @@ qmiv_status finalize_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm,
 // Batched frame sets: every view's work captured once into one HIP graph with concurrent
 // branches, replayed per frame set.
 // =============================================================================================
+namespace {
+
+// A few persistent host threads for a parallel loop over independent items (the host finalisation
+// of a frame set's views); the calling thread takes items too.
+class HostPool {
+ public:
+  explicit HostPool(int workers) {
+    for (int i = 0; i < workers; i++) th_.emplace_back([this] { loop(); });
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  // Runs f(0) .. f(n - 1), each once, and returns when all have finished.
+  void run(int n, const std::function<void(int)>& f) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &f;
+      n_ = n;
+      next_.store(0);
+      active_ = (int)th_.size();
+      gen_++;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [&] { return active_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void work() {
+    for (int i = next_.fetch_add(1); i < n_; i = next_.fetch_add(1)) (*job_)(i);
+  }
+  void loop() {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      lk.unlock();
+      work();
+      lk.lock();
+      if (--active_ == 0) done_cv_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)>* job_ = nullptr;
+  int n_ = 0;
+  std::atomic<int> next_{0};
+  int active_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+}  // namespace
+
 struct qmiv_batch {
+  std::unique_ptr<HostPool> pool;   // host finalisation of the views
   qmiv_ctx* ctx = nullptr;
   int32_t n = 0;
   std::vector<const qmiv_pic*> p0, p1;
@@ -1545,15 +1616,27 @@ qmiv_status qmiv_batch_run(qmiv_batch* b, qmiv_frame_result* out) {
   qmiv_ctx* ctx = b->ctx;
   QMIV_HIP(hipSetDevice(ctx->device));
   QMIV_HIP(hipGraphLaunch(b->exec, ctx->stream));
-  QMIV_HIP(hipStreamSynchronize(ctx->stream));
-  qmiv_status first_err = QMIV_OK;
-  for (int32_t i = 0; i < b->n; i++) {
-    const size_t k = (size_t)i;
-    const qmiv_status st = finalize_frame(b->ctx, b->p0[k]->g, b->prm[k], b->metrics[k], b->fw[k], b->plans[k].L,
-                                          (const char*)b->h_results + b->hoff[k], &out[i]);
-    if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
+  // The spherical weights every view needs exist before the views are finalised in parallel (the
+  // table is only read there).
+  for (int32_t i = 0; i < b->n; i++)
+    if (b->prm[(size_t)i].erp) (void)ws_table(ctx, b->p0[(size_t)i]->g.h, b->prm[(size_t)i].lat_range_deg);
+  if (!b->pool && b->n > 1) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    b->pool.reset(new HostPool((int)std::min<unsigned>(7u, std::min<unsigned>(hw, (unsigned)b->n) - 1u)));
   }
-  return first_err;
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<qmiv_status> st((size_t)b->n, QMIV_OK);
+  auto fin_view = [&](int i) {
+    const size_t k = (size_t)i;
+    st[k] = finalize_frame(b->ctx, b->p0[k]->g, b->prm[k], b->metrics[k], b->fw[k], b->plans[k].L,
+                           (const char*)b->h_results + b->hoff[k], &out[i]);
+  };
+  if (b->pool) b->pool->run(b->n, fin_view);
+  else for (int32_t i = 0; i < b->n; i++) fin_view(i);
+  for (int32_t i = 0; i < b->n; i++)
+    if (st[(size_t)i] != QMIV_OK)
+      return fail(st[(size_t)i], "view " + std::to_string(i) + ": input contains pels above the bit-depth maximum");
+  return QMIV_OK;
 }
 
 qmiv_status qmiv_batch_get_info(const qmiv_batch* b, qmiv_batch_info* info) {
