@@ -1,6 +1,8 @@
 """Builds the gfx950 engine library ivpsnr_amd/libqmiv.so in-tree (hipcc cross-compiles; no GPU needed).
 
     python -m ivpsnr_amd.build        # compile everything that changed, then link
+
+It also builds the command-line driver ivpsnr_amd/bin/qmiv (cli/qmiv_cli.cpp) against the library.
 """
 from __future__ import annotations
 
@@ -13,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqmiv.so")
+CLI_SRC = os.path.join(HERE, "cli", "qmiv_cli.cpp")
+CLI = os.path.join(HERE, "bin", "qmiv")
 ARCH = os.environ.get("QMIV_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -72,7 +76,20 @@ def build(force: bool = False, verbose: bool = False) -> str:
         tmp = LIB + ".tmp"
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
         os.replace(tmp, LIB)
+    build_cli(force=force)
     return LIB
+
+
+def build_cli(force: bool = False) -> str:
+    """Compile the command-line driver, linked to libqmiv.so beside it through an $ORIGIN rpath."""
+    deps = [CLI_SRC, LIB, os.path.join(HERE, "..", "include", "qmiv.h")]
+    if force or not os.path.exists(CLI) or any(os.path.getmtime(d) > os.path.getmtime(CLI) for d in deps):
+        os.makedirs(os.path.dirname(CLI), exist_ok=True)
+        tmp = CLI + ".tmp"
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-o", tmp, CLI_SRC, "-L" + HERE, "-lqmiv",
+              "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -1,0 +1,218 @@
+"""The command-line driver on the GPU: every frame line, average line and result-file line it prints
+equals the reference application's format (tests/cli_format.py) filled with the oracle's values for
+the same frames. PSNR-family lines must match as text (bit-exact values); SSIM-family values are
+parsed and compared within 1e-6. Covers start frames, ERP, the metric list, masked mode, YCbCr ->
+RGB and BGR -> RGB conversions, and identical components."""
+import numpy as np
+import pytest
+
+from ivpsnr_amd.synth import make_mask, make_pair
+from tests.cli_format import avg_lines, frame_cmp_line, frame_pic_line, pic_value, run_cli, write_seq
+
+pytestmark = pytest.mark.gpu
+
+ALL = 63
+
+
+def frames_for(w, h, bd, cf, n, seed, **knobs):
+    return [make_pair(w, h, bd, seed + i, chroma_format=cf, **knobs) for i in range(n)]
+
+
+def expected_psnr_family(results, names, mask=False, rgb=False, exact=None):
+    """Frame and average lines of PSNR, WSPSNR and IVPSNR from per-frame oracle results."""
+    from oracle import pyoracle
+    lines, avg = [], []
+    for name in names:
+        cmp_rows, pics = [], []
+        for f, r in enumerate(results):
+            ex = exact[f] if exact is not None else list(r.exact)
+            if name == "IVPSNR":
+                pics.append(r.ivpsnr)
+                lines.append(frame_pic_line(f, name, r.ivpsnr, mask, rgb))
+                continue
+            v = list(r.psnr if name == "PSNR" else r.wspsnr)
+            cmp_rows.append(v)
+            pics.append(pic_value(v))
+            lines.append(frame_cmp_line(f, name, v, mask, rgb, ex))
+            lines.append(frame_pic_line(f, name, pics[-1], mask, rgb))
+        n = len(results)
+        cmp_avg = [pyoracle.frame_average([c[i] for c in cmp_rows]) for i in range(3)] if cmp_rows else None
+        avg.append((name, cmp_avg, pyoracle.frame_average(pics)))
+    return lines, avg
+
+
+def check_output(r, lines, avg, mask=False, rgb=False, result_file=None):
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    out = r.stdout.splitlines()
+    for line in lines:
+        assert line in out, line
+    for name, cmp_avg, pic_avg in avg:
+        for line in avg_lines("Average      ", name, cmp_avg, pic_avg, mask, rgb):
+            assert line in out, line
+    assert out[-1] == "END-OF-LOG"
+    if result_file is not None:
+        text = result_file.read_text().splitlines()
+        assert text[0].startswith("FILE0  \"") and text[1].startswith("FILE1  \"")
+        body = [l for l in text if not l.startswith(("FILE", "TIME"))]
+        want = [l for name, c, p in avg for l in avg_lines("", name, c, p, mask, rgb)]
+        assert body == want
+
+
+def test_plain_all_metrics_start_frames_erp(tmp_path, oracle):
+    w, h, bd = 128, 64, 10
+    pairs = frames_for(w, h, bd, 420, 4, 100)
+    write_seq(tmp_path / "ref.yuv", [p.ref for p in pairs], bd)
+    write_seq(tmp_path / "tst.yuv", [p.tst for p in pairs[1:]], bd)  # tst frame k pairs with ref frame k + 1
+    print("cli plain 128x64 bd10 erp all metrics", flush=True)
+    r = run_cli(["-i0", "ref.yuv", "-i1", "tst.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-s0", 1, "-erp", "-ml", "All",
+                 "-v", 2, "-r", "res.txt"], tmp_path)
+    prm = oracle.params(w, h, bd, 420, metrics=ALL, erp=True)
+    res = [oracle.frame(pairs[f + 1].ref, pairs[f + 1].tst, prm) for f in range(3)]
+    lines, avg = expected_psnr_family(res, ["PSNR", "WSPSNR", "IVPSNR"])
+    check_output(r, lines, avg)
+    out = r.stdout.splitlines()
+    frame_tokens = [l.split() for l in out if l.startswith("Frame ")]
+    # normalized metrics: printed values within 1e-6 of the oracle's (6 decimals, SSIM within 1e-6)
+    for name, attr in (("SSIM", "ssim"), ("MSSSIM", "msssim")):
+        cmp_got = [list(map(float, t[4:7])) for t in frame_tokens if t[2] == name and t[3] == "Y:Cb:Cr"]
+        pic_got = [float(t[3]) for t in frame_tokens if t[2] == name + "-YCbCr"]
+        np.testing.assert_allclose(cmp_got, [list(getattr(x, attr)) for x in res], atol=1.5e-6)
+        np.testing.assert_allclose(pic_got, [pic_value(list(getattr(x, attr))) for x in res], atol=1.5e-6)
+    ivssim_got = [float(t[3]) for t in frame_tokens if t[2] == "IVSSIM"]
+    np.testing.assert_allclose(ivssim_got, [x.ivssim for x in res], atol=1.5e-6)
+    assert "FramesToProcess  = 3" in out
+    assert len((tmp_path / "res.txt").read_text().splitlines()) == 3 + 2 + 2 + 1 + 2 + 2 + 1
+
+
+def test_result_file_and_verbose0(tmp_path, oracle):
+    w, h, bd = 96, 48, 8
+    pairs = frames_for(w, h, bd, 444, 2, 200)
+    write_seq(tmp_path / "a.yuv", [p.ref for p in pairs], bd)
+    write_seq(tmp_path / "b.yuv", [p.tst for p in pairs], bd)
+    print("cli 444 bd8 result file", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-pf", "yuv444p", "-ps", f"{w}x{h}", "-ml", "PSNR,WSPSNR,IVPSNR",
+                 "-sr", 3, "-cws", "2:1:1:0", "-cwa", "4:1:1:0", "-v", 0, "-r", "res.txt"], tmp_path)
+    prm = oracle.params(w, h, bd, 444, search_range=3, weights_search=(2, 1, 1, 0))
+    res = [oracle.frame(p.ref, p.tst, prm) for p in pairs]
+    lines, avg = expected_psnr_family(res, ["PSNR", "WSPSNR", "IVPSNR"])
+    check_output(r, [], avg, result_file=tmp_path / "res.txt")
+    assert not any(l.startswith("Frame ") for l in r.stdout.splitlines())
+
+
+def test_masked(tmp_path, oracle):
+    w, h, bd, bdm = 160, 96, 10, 8
+    pairs = frames_for(w, h, bd, 420, 3, 300)
+    masks = [make_mask(w, h, bdm, 300 + i, kind="soft" if i % 2 else "objects") for i in range(3)]
+    write_seq(tmp_path / "a.yuv", [p.ref for p in pairs], bd)
+    write_seq(tmp_path / "b.yuv", [p.tst for p in pairs], bd)
+    write_seq(tmp_path / "m.yuv", masks, bdm)
+    print("cli masked 160x96 bd10 mask bd8", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-im", "m.yuv", "-bdm", bdm,
+                 "-ml", "PSNR, WSPSNR, IVPSNR", "-erp", "-v", 4, "-r", "res.txt"], tmp_path)
+    prm = oracle.params(w, h, bd, 420, erp=True)
+    res = [oracle.frame_masked(p.ref, p.tst, m, bdm, 420, prm) for p, m in zip(pairs, masks)]
+    lines, avg = expected_psnr_family(res, ["PSNR", "WSPSNR"], mask=True)
+    for f, x in enumerate(res):
+        lines.append(frame_pic_line(f, "IVPSNR", x.ivpsnr, mask=True) + f"    R2T {x.r2t:7.4f}  T2R {x.t2r:7.4f}")
+    avg.append(("IVPSNR", None, oracle.frame_average([x.ivpsnr for x in res])))
+    check_output(r, lines, avg, mask=True, result_file=tmp_path / "res.txt")
+    debug = [l for l in r.stdout.splitlines() if l.startswith("Frame") and "NNM" in l]
+    assert [int(l.split()[3]) for l in debug] == [x.num_non_masked for x in res]
+    assert [list(map(int, l.split()[5:9])) for l in debug] == [list(x.gcd) for x in res]
+
+
+def pack444(planes, bd):
+    return np.ascontiguousarray(planes.reshape(-1), np.uint8 if bd <= 8 else np.uint16)
+
+
+@pytest.mark.parametrize("space_name,space", [("YCbCr_BT709", 1), ("YCbCr_BT2020", 3), ("YCbCr_SMPTE170M", 0)])
+def test_ycbcr_to_rgb(tmp_path, oracle, space_name, space):
+    w, h, bd = 128, 64, 10
+    pairs = frames_for(w, h, bd, 420, 2, 400 + space)
+    write_seq(tmp_path / "a.yuv", [p.ref for p in pairs], bd)
+    write_seq(tmp_path / "b.yuv", [p.tst for p in pairs], bd)
+    print(f"cli {space_name} -> RGB", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-csi", space_name, "-csm", "RGB",
+                 "-ml", "PSNR,WSPSNR,IVPSNR", "-v", 2], tmp_path)
+    prm = oracle.params(w, h, bd, 444)
+    res = []
+    for p in pairs:
+        conv = [pack444(oracle.colorspace(oracle.unpack(x, w, h, bd, 420), bd, space, True), bd) for x in (p.ref, p.tst)]
+        res.append(oracle.frame(conv[0], conv[1], prm))
+    lines, avg = expected_psnr_family(res, ["PSNR", "WSPSNR", "IVPSNR"], rgb=True)
+    check_output(r, lines, avg, rgb=True)
+
+
+def test_bgr_reorder_and_rgb_to_ycbcr(tmp_path, oracle):
+    w, h, bd = 64, 48, 12
+    pairs = frames_for(w, h, bd, 444, 2, 500)
+    write_seq(tmp_path / "a.yuv", [p.ref for p in pairs], bd)
+    write_seq(tmp_path / "b.yuv", [p.tst for p in pairs], bd)
+    prm = oracle.params(w, h, bd, 444)
+    print("cli GBR -> RGB", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-cf", 444, "-csi", "GBR", "-csm",
+                 "RGB", "-ml", "PSNR,IVPSNR", "-v", 2], tmp_path)
+    gbr_to_rgb = lambda x: pack444(oracle.unpack(x, w, h, bd, 444)[[2, 0, 1]], bd)  # G,B,R planes -> R,G,B
+    res = [oracle.frame(gbr_to_rgb(p.ref), gbr_to_rgb(p.tst), prm) for p in pairs]
+    lines, avg = expected_psnr_family(res, ["PSNR", "IVPSNR"], rgb=True)
+    check_output(r, lines, avg, rgb=True)
+    print("cli RGB -> YCbCr_BT709", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-cf", 444, "-csi", "RGB", "-csm",
+                 "YCbCr_BT709", "-ml", "PSNR,IVPSNR", "-v", 2], tmp_path)
+    to_ycc = lambda x: pack444(oracle.colorspace(oracle.unpack(x, w, h, bd, 444), bd, 1, False), bd)
+    res = [oracle.frame(to_ycc(p.ref), to_ycc(p.tst), prm) for p in pairs]
+    lines, avg = expected_psnr_family(res, ["PSNR", "IVPSNR"])
+    check_output(r, lines, avg)
+
+
+def test_identical_components(tmp_path, oracle):
+    """Identical chroma gets the PSNR of SSD=1 and the ExactU / ExactV tags (xAppQMIV.cpp:837-871). With
+    a YCbCr -> RGB conversion the flags are those of the input pictures: identical luma alone marks R
+    although R differs after conversion."""
+    w, h, bd = 64, 32, 10
+    n = w * h
+    pair = make_pair(w, h, bd, 600)
+    tst_chroma_same = pair.tst.copy()
+    tst_chroma_same[n:] = pair.ref[n:]
+    tst_luma_same = pair.tst.copy()
+    tst_luma_same[:n] = pair.ref[:n]
+    write_seq(tmp_path / "a.yuv", [pair.ref, pair.ref], bd)
+    write_seq(tmp_path / "b.yuv", [tst_chroma_same, tst_luma_same], bd)
+    fake = oracle.psnr_from_ssd(1.0, n, bd)
+    print("cli identical components", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-ml", "PSNR,WSPSNR", "-v", 2], tmp_path)
+    prm = oracle.params(w, h, bd, 420)
+    res = [oracle.frame(pair.ref, t, prm) for t in (tst_chroma_same, tst_luma_same)]
+    assert list(res[0].exact) == [0, 1, 1] and res[0].psnr[1] == fake
+    lines, avg = expected_psnr_family(res, ["PSNR", "WSPSNR"])
+    check_output(r, lines, avg)
+    assert "ExactU ExactV" in r.stdout and "ExactY" in r.stdout
+
+    print("cli identical luma with YCbCr_BT709 -> RGB", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-csi", "YCbCr_BT709", "-csm", "RGB",
+                 "-ml", "PSNR", "-v", 2], tmp_path)
+    prm444 = oracle.params(w, h, bd, 444)
+    rows, exacts, pics = [], [], []
+    lines = []
+    for f, t in enumerate((tst_chroma_same, tst_luma_same)):
+        conv = [pack444(oracle.colorspace(oracle.unpack(x, w, h, bd, 420), bd, 1, True), bd) for x in (pair.ref, t)]
+        x = oracle.frame(conv[0], conv[1], prm444)
+        ex = list(res[f].exact)  # flags of the input pictures
+        v = [fake if ex[c] else x.psnr[c] for c in range(3)]
+        rows.append(v)
+        pics.append(pic_value(v))
+        lines += [frame_cmp_line(f, "PSNR", v, rgb=True, exact=ex), frame_pic_line(f, "PSNR", pics[-1], rgb=True)]
+    avg = [("PSNR", [oracle.frame_average([v[c] for v in rows]) for c in range(3)], oracle.frame_average(pics))]
+    check_output(r, lines, avg, rgb=True)
+
+
+def test_invalid_pel_stops(tmp_path):
+    w, h, bd = 32, 16, 10
+    pair = make_pair(w, h, bd, 700)
+    bad = pair.tst.copy()
+    bad[5] = 1500
+    write_seq(tmp_path / "a.yuv", [pair.ref], bd)
+    write_seq(tmp_path / "b.yuv", [bad], bd)
+    print("cli invalid pel", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-ml", "PSNR"], tmp_path)
+    assert r.returncode != 0 and "InputFile contains invalid values (b.yuv)" in r.stderr
