@@ -138,7 +138,8 @@ qmiv_status qmiv_pic_upload(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size
 qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, size_t bytes);
 
 /* ------------------------------------------------------------------------------------------ */
-/* Raw planar YUV sequences (the read side of xSeq, libs/pmbb-core/src/xSeq.cpp)               */
+/* Sequences: raw planar YUV files (the read side of xSeq, libs/pmbb-core/src/xSeq.cpp) and     */
+/* PNG file lists (xSeqPNG, libs/pmbb-spng/src/xSeqPNG.cpp)                                   */
 /* ------------------------------------------------------------------------------------------ */
 typedef struct qmiv_seq qmiv_seq;
 
@@ -147,6 +148,16 @@ typedef struct qmiv_seq qmiv_seq;
    xSeq.cpp:451-470; frame bytes as xSeq::create, :408-428). Host-only: no device is touched. */
 qmiv_status qmiv_seq_open(const char* path, int32_t width, int32_t height, int32_t bit_depth,
                           int32_t chroma_format, qmiv_seq** out);
+/* Opens a list of PNG files as a sequence: xSeqPNG (libs/pmbb-spng/src/xSeqPNG.cpp:55-148). The
+   frame file names are the pattern formatted with the frame's file index the way fmt::format does
+   for the fields {}, {:Nd} and {:0Nd} (e.g. "view_{:03d}.png"). The first index is the last of 0
+   and 1 whose file exists; the frames are the files from there on, up to the first missing index
+   (at most 65535). Frames are 8-bit 4:4:4 pictures with the planes R, G, B: each file must be
+   width x height at bit depth 8 (any colour type; grey is replicated, palettes looked up, alpha
+   dropped, as libspng's RGB8 output without transparency or gamma processing). Decoding happens on
+   the host at read time (qmiv_seq_read_host / qmiv_seq_read); a file that does not decode makes
+   that read fail. Host-only at open: no device is touched. */
+qmiv_status qmiv_seq_open_png(const char* pattern, int32_t width, int32_t height, qmiv_seq** out);
 int32_t qmiv_seq_num_frames(const qmiv_seq* seq);
 size_t qmiv_seq_frame_bytes(const qmiv_seq* seq);
 /* Reads frame `index` (seek by frame, xSeq.cpp:519-525) into host memory of the frame's size. */

@@ -28,7 +28,7 @@ HOST_HIP_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__
 # The double-precision finalisation must round like the reference's ISO C++ build.
 FINALIZE_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-fno-fast-math"]
 
-HEADERS = ["qmiv_internal.h", "finalize.h", "host_internal.h"]
+HEADERS = ["qmiv_internal.h", "finalize.h", "host_internal.h", "png_reader.h"]
 
 
 def _units():
@@ -74,7 +74,7 @@ def build(force: bool = False, verbose: bool = False) -> str:
     objs = [u[1] for u in units]
     if force or todo or not os.path.exists(LIB):
         tmp = LIB + ".tmp"
-        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
         os.replace(tmp, LIB)
     build_cli(force=force)
     return LIB

@@ -126,6 +126,7 @@ _SIGNATURES = {
                                        _f64p]),
     "qmiv_pic_upsample": (C.c_int, [_ctx_p, _pic_p, _pic_p]),
     "qmiv_seq_open": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "qmiv_seq_open_png": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "qmiv_seq_num_frames": (C.c_int32, [C.c_void_p]),
     "qmiv_seq_frame_bytes": (C.c_size_t, [C.c_void_p]),
     "qmiv_seq_read_host": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t]),

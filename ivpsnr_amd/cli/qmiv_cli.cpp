@@ -9,7 +9,7 @@
 //
 // Parameters are "-short value" on the command line or "LongName = value" lines in files given with
 // -c (a later occurrence overrides an earlier one):
-//   i0 InputFile0, i1 InputFile1, ff FileFormat (RAW), ps PictureSize (WxH), pw PictureWidth,
+//   i0 InputFile0, i1 InputFile1, ff FileFormat (RAW, PNG), ps PictureSize (WxH), pw PictureWidth,
 //   ph PictureHeight, pf PictureFormat (yuv420p, yuv444p10le, ...), bd BitDepth (8..14),
 //   cf ChromaFormat (420, 444), s0 StartFrame0, s1 StartFrame1, nf NumberOfFrames, r ResultFile,
 //   ml MetricList (comma list of PSNR WSPSNR IVPSNR SSIM MSSSIM IVSSIM, or All), im InputFileM,
@@ -19,7 +19,8 @@
 //   cwa CmpWeightsAverage, unc UnnoticeableCoef (a:b:c:d), ipa InvalidPelActn (STOP), nma
 //   NameMismatchActn, nth NumberOfThreads, ilp InterleavedPic, v VerboseLevel (0..4).
 // Differences from the reference application: chroma formats 4:2:0 and 4:4:4 only (4:2:2 and 4:0:0
-// are rejected); FileFormat PNG is not supported; InvalidPelActn CNCL (concealment) is rejected, and
+// are rejected); with FileFormat PNG, BitDepth and BitDepthM must be 8 (the files are decoded to
+// 8-bit R, G, B planes, xSeqPNG.cpp:14-36); InvalidPelActn CNCL (concealment) is rejected, and
 // under WARN or SKIP a frame with a pel above the bit-depth maximum stops the run, because the engine
 // computes no metric for such a frame; NumberOfThreads and InterleavedPic are reported and otherwise
 // ignored (the engine runs on one GPU and its results do not depend on either); the timing report of
@@ -91,7 +92,7 @@ int32_t lc_space(ClrSpc c) {  // xClrSpcAppToClrSpc (apps/QMIV/src/xAppQMIV.h:11
 
 std::string format(const char* f, ...) __attribute__((format(printf, 1, 2)));
 std::string format(const char* f, ...) {
-  char buf[1024];
+  char buf[4096];
   va_list ap;
   va_start(ap, f);
   std::vsnprintf(buf, sizeof(buf), f, ap);
@@ -321,8 +322,9 @@ int main(int argc, char** argv) {
   const std::string in_file[3] = {P.str("InputFile0"), P.str("InputFile1"), P.str("InputFileM")};
   if (in_file[0].empty()) P.log += "!  InputFile0 is empty\n";
   if (in_file[1].empty()) P.log += "!  InputFile1 is empty\n";
-  const std::string file_format = P.str("FileFormat", "RAW");
-  if (upper(file_format) != "RAW") P.log += "!  FileFormat " + file_format + " is not supported (RAW only)\n";
+  const std::string file_format = upper(P.str("FileFormat", "RAW"));
+  const bool png = file_format == "PNG";
+  if (file_format != "RAW" && !png) P.log += "!  Invalid FileFormat " + file_format + "\n";
   int32_t w = 0, h = 0;
   if (P.has("PictureSize")) {
     if (std::sscanf(P.str("PictureSize").c_str(), "%dx%d", &w, &h) != 2 || w <= 0 || h <= 0)
@@ -371,6 +373,7 @@ int main(int argc, char** argv) {
   const int32_t cfm = P.has("ChromaFormatM") ? str2cf(P.str("ChromaFormatM")) : cf;
   if (bdm < 8 || bdm > 14) P.log += "!  Invalid or unsuported BitDepthM value\n";
   if (cfm != 420 && cfm != 444) P.log += "!  Invalid or unsuported ChromaFormatM value (4:2:0 and 4:4:4 only)\n";
+  if (png && (bd != 8 || bdm != 8)) P.log += "!  PNG input is 8-bit: BitDepth and BitDepthM must be 8\n";
   qmiv_params prm;
   qmiv_params_default(&prm);
   prm.erp = P.num("Equirectangular", 0) != 0;
@@ -426,7 +429,7 @@ int main(int argc, char** argv) {
     std::string c = "Run-time configuration:\n";
     c += format("InputFile0        = %s\n", in_file[0].c_str());
     c += format("InputFile1        = %s\n", in_file[1].c_str());
-    c += "FileFormat        = RAW\n";
+    c += format("FileFormat        = %s\n", png ? "PNG" : "RAW");
     c += format("PictureSize       = %dx%d\n", w, h);
     c += format("BitDepth          = %d\n", bd);
     c += format("ChromaFormat      = CF%d\n", cf);
@@ -476,17 +479,24 @@ int main(int argc, char** argv) {
   // --- sequences: xAppQMIV::setupSeqAndBuffs (apps/QMIV/src/xAppQMIV.cpp:492-560) ---------------
   const char fid[3] = {'0', '1', 'M'};
   const int32_t num_inputs = use_mask ? 3 : 2;
-  const int32_t bds[3] = {bd, bd, bdm}, cfs[3] = {cf, cf, cfm};
+  // PNG frames are 4:4:4 R, G, B pictures whatever ChromaFormat says (xSeqPNG::create, xSeqPNG.cpp:14-35).
+  const int32_t bds[3] = {bd, bd, bdm};
+  const int32_t cfs[3] = {png ? 444 : cf, png ? 444 : cf, png ? 444 : cfm};
   Owned o;
-  for (int i = 0; i < num_inputs; i++)
+  for (int i = 0; i < num_inputs && !png; i++)
     if (!std::filesystem::exists(in_file[i]))
       fail(format("ERROR --> InputFile%c does not exist (%s)", fid[i], in_file[i].c_str()));
-  if (verbose >= 1)
+  if (verbose >= 1 && !png)
     for (int i = 0; i < num_inputs; i++)
       std::printf("SizeOfInputFile%c = %lld\n", fid[i], (long long)std::filesystem::file_size(in_file[i]));
   int32_t frames_in[3] = {0, 0, 0};
   for (int i = 0; i < num_inputs; i++) {
-    check(qmiv_seq_open(in_file[i].c_str(), w, h, bds[i], cfs[i], &o.seq[i]), "InputFile opening failure (" + in_file[i] + ")");
+    if (png) {
+      if (qmiv_seq_open_png(in_file[i].c_str(), w, h, &o.seq[i]) != QMIV_OK)
+        fail(format("ERROR --> InputFile%c does not exist (%s) [%s]", fid[i], in_file[i].c_str(), qmiv_last_error()));
+    } else {
+      check(qmiv_seq_open(in_file[i].c_str(), w, h, bds[i], cfs[i], &o.seq[i]), "InputFile opening failure (" + in_file[i] + ")");
+    }
     frames_in[i] = qmiv_seq_num_frames(o.seq[i]);
     if (verbose >= 1) std::printf("DetectedFrames%d  = %d\n", i, frames_in[i]);
     const int32_t first = i < 2 ? start[i] : 0;
@@ -504,7 +514,7 @@ int main(int argc, char** argv) {
   for (int i = 0; i < num_inputs; i++) check(qmiv_pic_create(o.ctx, w, h, bds[i], cfs[i], &o.in[i]), "picture allocation");
   // Colour conversion and plane reordering work on 4:4:4 pictures, as the reference's do (its
   // pictures are always 4:4:4 after reading: libs/pmbb-core/src/xSeq.cpp:197-213).
-  const bool up444 = convert && cf == 420;
+  const bool up444 = convert && cfs[0] == 420;
   if (up444)
     for (int i = 0; i < 2; i++) check(qmiv_pic_create(o.ctx, w, h, bd, 444, &o.up[i]), "picture allocation");
 

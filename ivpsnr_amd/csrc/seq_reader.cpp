@@ -1,8 +1,10 @@
-// Raw planar YUV sequence reader: the read path of xSeq (libs/pmbb-core/src/xSeq.cpp:408-428 frame
-// size, :451-470 frame count, :519-525 seek by frame) feeding device pictures. Packed frames are
-// read from the file into pinned host buffers by a background thread (the frame after the last one
-// requested is prefetched) and copied into the pictures asynchronously on the context's stream, so
-// disk reads, host-to-device copies and the metric kernels overlap.
+// Sequence readers feeding device pictures: raw planar YUV files, the read path of xSeq
+// (libs/pmbb-core/src/xSeq.cpp:408-428 frame size, :451-470 frame count, :519-525 seek by frame),
+// and lists of PNG files, the read path of xSeqPNG (libs/pmbb-spng/src/xSeqPNG.cpp:55-148). Packed
+// frames are read (and, for PNG, decoded) into pinned host buffers by a background thread (the
+// frame after the last one requested is prefetched) and copied into the pictures asynchronously on
+// the context's stream, so disk reads, decoding, host-to-device copies and the metric kernels
+// overlap.
 #include <fcntl.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -15,7 +17,10 @@
 #include <string>
 #include <thread>
 
+#include <vector>
+
 #include "host_internal.h"
+#include "png_reader.h"
 
 using namespace qmiv;
 
@@ -36,7 +41,8 @@ struct Slot {
 }  // namespace
 
 struct qmiv_seq {
-  int fd = -1;
+  int fd = -1;                     // raw sequence file
+  std::vector<std::string> png;    // PNG sequence: one file per frame
   int32_t w = 0, h = 0, bd = 0, cf = 0;
   size_t frame_bytes = 0;
   int32_t num_frames = 0;
@@ -52,14 +58,24 @@ struct qmiv_seq {
 
 namespace {
 
-// pread of one whole frame (EINTR and short reads retried).
-bool read_frame(const qmiv_seq* s, int32_t index, void* dst) {
+// One whole frame into dst: pread of a raw frame (EINTR and short reads retried), or the decoded
+// PNG file of the frame.
+bool read_frame(const qmiv_seq* s, int32_t index, void* dst, std::string* err = nullptr) {
+  if (!s->png.empty()) {
+    std::string e;
+    const bool ok = png::decode_file_rgb8_planar(s->png[(size_t)index], s->w, s->h, (uint8_t*)dst, e);
+    if (!ok && err) *err = e;
+    return ok;
+  }
   size_t done = 0;
   const off_t base = (off_t)s->frame_bytes * (off_t)index;
   while (done < s->frame_bytes) {
     const ssize_t r = pread(s->fd, (char*)dst + done, s->frame_bytes - done, base + (off_t)done);
     if (r < 0 && errno == EINTR) continue;
-    if (r <= 0) return false;
+    if (r <= 0) {
+      if (err) *err = "short read";
+      return false;
+    }
     done += (size_t)r;
   }
   return true;
@@ -162,6 +178,42 @@ qmiv_status qmiv_seq_open(const char* path, int32_t width, int32_t height, int32
   return QMIV_OK;
 }
 
+qmiv_status qmiv_seq_open_png(const char* pattern, int32_t width, int32_t height, qmiv_seq** out) {
+  if (!pattern || !out) return host_fail(QMIV_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (width <= 0 || height <= 0 || (int64_t)width * height > INT32_MAX / 3)
+    return host_fail(QMIV_ERR_INVALID_ARG, "unsupported frame geometry");
+  // xSeqPNG::xBackendOpen (xSeqPNG.cpp:55-78): the first index is the last of 0 and 1 whose file
+  // exists; frames are the files from there on, up to the first missing index or 65535.
+  auto exists = [](const std::string& f) {
+    struct stat st;
+    return stat(f.c_str(), &st) == 0;
+  };
+  std::string name, err;
+  int32_t first = -1;
+  for (int32_t i = 0; i <= 1; i++) {
+    if (!png::format_index(pattern, i, name, err)) return host_fail(QMIV_ERR_INVALID_ARG, err);
+    if (exists(name)) first = i;
+  }
+  if (first < 0) return host_fail(QMIV_ERR_INVALID_ARG, std::string("First Idx=(0 or 1) not found for ") + pattern);
+  std::vector<std::string> files;
+  for (int32_t i = first; i < 65535; i++) {
+    if (!png::format_index(pattern, i, name, err)) return host_fail(QMIV_ERR_INVALID_ARG, err);
+    if (!exists(name)) break;
+    files.push_back(name);
+  }
+  qmiv_seq* s = new qmiv_seq;
+  s->png = std::move(files);
+  s->w = width;
+  s->h = height;
+  s->bd = 8;
+  s->cf = 444;
+  s->frame_bytes = (size_t)3 * (size_t)width * (size_t)height;
+  s->num_frames = (int32_t)s->png.size();
+  *out = s;
+  return QMIV_OK;
+}
+
 int32_t qmiv_seq_num_frames(const qmiv_seq* seq) { return seq ? seq->num_frames : -1; }
 
 size_t qmiv_seq_frame_bytes(const qmiv_seq* seq) { return seq ? seq->frame_bytes : 0; }
@@ -170,7 +222,8 @@ qmiv_status qmiv_seq_read_host(qmiv_seq* seq, int32_t index, void* dst, size_t b
   if (!seq || !dst) return host_fail(QMIV_ERR_INVALID_ARG, "null argument");
   if (index < 0 || index >= seq->num_frames) return host_fail(QMIV_ERR_INVALID_ARG, "frame index out of range");
   if (bytes != seq->frame_bytes) return host_fail(QMIV_ERR_INVALID_ARG, "buffer size differs from the frame size");
-  if (!read_frame(seq, index, dst)) return host_fail(QMIV_ERR_INVALID_ARG, "short read");
+  std::string err;
+  if (!read_frame(seq, index, dst, &err)) return host_fail(QMIV_ERR_INVALID_ARG, err);
   return QMIV_OK;
 }
 
@@ -205,12 +258,13 @@ qmiv_status qmiv_seq_read(qmiv_ctx* ctx, qmiv_seq* seq, int32_t index, qmiv_pic*
     sl.ready = false;
     sl.frame = index;
     lk.unlock();
-    const bool ok = drain_copy(seq, i) == hipSuccess && read_frame(seq, index, sl.host);
+    std::string err = "device copy out of the frame buffer failed";
+    const bool ok = drain_copy(seq, i) == hipSuccess && read_frame(seq, index, sl.host, &err);
     lk.lock();
     sl.reading = false;
     sl.ready = ok;
     seq->cv.notify_all();
-    if (!ok) return host_fail(QMIV_ERR_INVALID_ARG, "short read");
+    if (!ok) return host_fail(QMIV_ERR_INVALID_ARG, err);
   }
   Slot& sl = seq->slot[i];
   sl.last_use = ++seq->tick;

@@ -59,8 +59,9 @@ class Picture:
 
 
 class Sequence:
-    """A raw planar YUV file of packed frames (qmiv_seq_*): frame count, host reads and reads into
-    device pictures through pinned buffers with one-frame prefetch."""
+    """A raw planar YUV file of packed frames, or a list of PNG files (Sequence.png), read through
+    qmiv_seq_*: frame count, host reads and reads into device pictures through pinned buffers with
+    one-frame prefetch."""
 
     def __init__(self, path: str, width: int, height: int, bit_depth: int, chroma_format: int = 420):
         h = C.c_void_p()
@@ -70,6 +71,19 @@ class Sequence:
         self.width, self.height, self.bit_depth, self.chroma_format = width, height, bit_depth, chroma_format
         self.num_frames = lib().qmiv_seq_num_frames(h)
         self.frame_bytes = lib().qmiv_seq_frame_bytes(h)
+
+    @classmethod
+    def png(cls, pattern: str, width: int, height: int) -> "Sequence":
+        """A list of PNG files named by `pattern` ({} / {:0Nd} fields take the file index), read as
+        8-bit 4:4:4 R, G, B frames (qmiv_seq_open_png)."""
+        h = C.c_void_p()
+        check(lib().qmiv_seq_open_png(os.fsencode(pattern), width, height, C.byref(h)), "qmiv_seq_open_png")
+        self = cls.__new__(cls)
+        self.handle = h
+        self.width, self.height, self.bit_depth, self.chroma_format = width, height, 8, 444
+        self.num_frames = lib().qmiv_seq_num_frames(h)
+        self.frame_bytes = lib().qmiv_seq_frame_bytes(h)
+        return self
 
     def __len__(self) -> int:
         return self.num_frames

@@ -6,7 +6,8 @@ on seeded synthetic inputs and on the reference's example frames, and stores inp
 plus every output as JSON. The example YUV files themselves are not copied; only the numbers
 the reference computes on them are stored.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # metric fixtures
+    python tests/golden/make_golden.py --png      # PNG decoding (oracle/_ref/libspng_ref.so)
 """
 from __future__ import annotations
 
@@ -148,7 +149,32 @@ def masked_fixtures():
     return cases
 
 
+def png_fixtures():
+    """The reference's PNG read step (libspng as xSeqPNG calls it, oracle/spng_ref.c) on the encoded
+    cases of tests/png_writer.py: decode status and the SHA-256 of the R, G, B planes."""
+    import ctypes as C
+    from tests import png_writer
+    spng = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libspng_ref.so"))
+    out = []
+    with tempfile.TemporaryDirectory() as td:
+        for case in png_writer.cases():
+            data, (w, h), _ = png_writer.build_case(case)
+            path = os.path.join(td, case["name"] + ".png")
+            with open(path, "wb") as f:
+                f.write(data)
+            planes = np.zeros(3 * w * h, np.uint8)
+            err = C.create_string_buffer(256)
+            rc = spng.spng_ref_decode(path.encode(), w, h, planes.ctypes.data_as(C.c_void_p), err, 256)
+            out.append({"name": case["name"], "ok": rc == 0, "error": err.value.decode(),
+                        "sha256": hashlib.sha256(planes.tobytes()).hexdigest() if rc == 0 else None})
+    return {"cases": out}
+
+
 def main():
+    if "--png" in sys.argv:
+        with open(os.path.join(HERE, "png.json"), "w") as f:
+            json.dump(png_fixtures(), f, indent=0)
+        return
     if not os.path.exists(REF):
         sys.exit("oracle/_ref/qmiv_ref is not built (make -C oracle ref)")
     if "--masked" in sys.argv:

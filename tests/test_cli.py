@@ -66,6 +66,19 @@ def test_configuration_file_and_override(seqs):
     assert "SizeOfInputFileM = 768" in r.stdout
 
 
+def test_png_sequences_are_counted(seqs):
+    from tests.png_writer import png_bytes
+    for i in range(1, 4):
+        (seqs / f"a{i}.png").write_bytes(png_bytes(np.zeros((8, 16, 3), np.uint8), 2))
+    for i in range(0, 6):
+        (seqs / f"b{i:02d}.png").write_bytes(png_bytes(np.zeros((8, 16, 3), np.uint8), 2))
+    r = run_cli(["-i0", "a{}.png", "-i1", "b{:02d}.png", "-ff", "PNG", "-ps", "16x8", "-s1", "1"], seqs)
+    cfg = config_block(r.stdout)
+    assert cfg["FileFormat"] == "PNG" and "SizeOfInputFile0" not in cfg
+    # a: files 1..3; b: files 0..5 start at 1 (the last of 0 and 1 that exists), so 5 frames
+    assert cfg["DetectedFrames0"] == "3" and cfg["DetectedFrames1"] == "5" and cfg["FramesToProcess"] == "3"
+
+
 @pytest.mark.parametrize("args,message", [
     (["-i0", "a.yuv", "-ps", "16x8"], "InputFile1 is empty"),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-ml", "PSNR,XYZ"], 'Token "XYZ" in not a metric'),
@@ -79,7 +92,9 @@ def test_configuration_file_and_override(seqs):
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-im", "m.yuv", "-ml", "PSNR,IVSSIM"],
      "Structural Similarity metrics cannot be combined with Mask mode"),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-ipa", "CNCL"], "CNCL is not supported"),
-    (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-ff", "PNG"], "FileFormat PNG is not supported"),
+    (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-ff", "TIFF"], "Invalid FileFormat TIFF"),
+    (["-i0", "a{}.png", "-i1", "b{}.png", "-ps", "16x8", "-ff", "PNG", "-bd", "10"], "PNG input is 8-bit"),
+    (["-i0", "none{}.png", "-i1", "b{}.png", "-ps", "16x8", "-ff", "PNG"], "InputFile0 does not exist (none{}.png)"),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-xyz", "1"], "Unknown parameter -xyz"),
     (["-i0", "a.yuv", "-i1", "nope.yuv", "-ps", "16x8"], "InputFile1 does not exist (nope.yuv)"),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-bd", "10", "-s1", "3"],

@@ -2,7 +2,7 @@
 equals the reference application's format (tests/cli_format.py) filled with the oracle's values for
 the same frames. PSNR-family lines must match as text (bit-exact values); SSIM-family values are
 parsed and compared within 1e-6. Covers start frames, ERP, the metric list, masked mode, YCbCr ->
-RGB and BGR -> RGB conversions, and identical components."""
+RGB and BGR -> RGB conversions, identical components and PNG-list input."""
 import numpy as np
 import pytest
 
@@ -216,3 +216,33 @@ def test_invalid_pel_stops(tmp_path):
     print("cli invalid pel", flush=True)
     r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-ml", "PSNR"], tmp_path)
     assert r.returncode != 0 and "InputFile contains invalid values (b.yuv)" in r.stderr
+
+
+def test_png_sequences(tmp_path, oracle):
+    """FileFormat PNG (xSeqPNG): frames decoded to 8-bit R, G, B planes, metrics on them as they are
+    (ColorSpaceInput YCbCr: no conversion) and after an RGB -> YCbCr_BT709 conversion."""
+    from tests.png_writer import expected_rgb8, png_bytes, random_image
+    w, h = 48, 32
+    planes = {0: [], 1: []}
+    for f in range(3):
+        for i, colour in ((0, 2), (1, 6)):
+            pix, _ = random_image(w, h, colour, 800 + 10 * f + i)
+            if i == 1:  # the tested sequence: the reference frame plus a shift and noise, with alpha
+                base = planes[0][f].transpose(1, 2, 0).astype(np.int64)
+                pix[..., :3] = np.clip(np.roll(base, 1, axis=1) + (pix[..., :3] % 5) - 2, 0, 255).astype(np.uint8)
+            (tmp_path / f"{'ab'[i]}_{f + 1:03d}.png").write_bytes(png_bytes(pix, colour, interlace=f == 1))
+            planes[i].append(expected_rgb8(pix, colour))
+    print("cli PNG sequences", flush=True)
+    r = run_cli(["-i0", "a_{:03d}.png", "-i1", "b_{:03d}.png", "-ff", "PNG", "-ps", f"{w}x{h}",
+                 "-ml", "PSNR,WSPSNR,IVPSNR", "-v", 2], tmp_path)
+    prm = oracle.params(w, h, 8, 444)
+    res = [oracle.frame(planes[0][f].reshape(-1), planes[1][f].reshape(-1), prm) for f in range(3)]
+    lines, avg = expected_psnr_family(res, ["PSNR", "WSPSNR", "IVPSNR"])
+    check_output(r, lines, avg)
+    print("cli PNG sequences RGB -> YCbCr_BT709", flush=True)
+    r = run_cli(["-i0", "a_{:03d}.png", "-i1", "b_{:03d}.png", "-ff", "PNG", "-ps", f"{w}x{h}", "-csi", "RGB",
+                 "-csm", "YCbCr_BT709", "-ml", "PSNR,IVPSNR", "-v", 2], tmp_path)
+    conv = lambda p: oracle.colorspace(p, 8, 1, False).reshape(-1).astype(np.uint8)
+    res = [oracle.frame(conv(planes[0][f]), conv(planes[1][f]), prm) for f in range(3)]
+    lines, avg = expected_psnr_family(res, ["PSNR", "IVPSNR"])
+    check_output(r, lines, avg)

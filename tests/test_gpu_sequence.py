@@ -64,3 +64,26 @@ def test_geometry_mismatch_rejected(engine, tmp_path):
     with pytest.raises(QmivError):
         seq.read(engine, 2, engine.picture(64, 32, 8, 420))
     seq.close()
+
+
+def test_png_frames_land_in_device_pictures(engine, tmp_path):
+    """PNG-list sequences (qmiv_seq_open_png) read into 8-bit 4:4:4 device pictures through the same
+    pinned buffers and prefetch: every frame equals its host decode and the pixels it was encoded from."""
+    from tests.png_writer import expected_rgb8, png_bytes, random_image
+    w, h = 96, 40
+    want = []
+    for i in range(4):
+        colour = (0, 2, 3, 6)[i]
+        pix, pal = random_image(w, h, colour, 900 + i)
+        (tmp_path / f"v{i + 1}.png").write_bytes(png_bytes(pix, colour, palette=pal, interlace=i % 2 == 1))
+        want.append(expected_rgb8(pix, colour, pal).reshape(-1))
+    seq = Sequence.png(str(tmp_path / "v{}.png"), w, h)
+    assert len(seq) == 4
+    pic = engine.picture(w, h, 8, 444)
+    for i in (0, 1, 2, 3, 1):
+        print(f"png frame {i}", flush=True)
+        seq.read(engine, i, pic)
+        engine.sync()
+        np.testing.assert_array_equal(pic.download(), want[i])
+        np.testing.assert_array_equal(seq.read_host(i), want[i])
+    seq.close()
