@@ -11,6 +11,7 @@
     xShftCompPicProc     (xShftCompPic.h:33-44)        xShftCompPicProc
     xPSNR / xWSPSNR / xIVPSNR (xPSNR.h, xWSPSNR.h, xIVPSNR.h)   xIVPSNR (PSNR, WS-PSNR, IV-PSNR)
     xSSIM / xIVSSIM      (xSSIM.h:25-79)               xIVSSIM (SSIM, MS-SSIM, IV-SSIM)
+    xSeq / xSeqPNG       (xSeq.h, xSeqPNG.h)           xSeq / xSeqPNG (read side)
 */
 #ifndef QMIV_HPP
 #define QMIV_HPP
@@ -132,6 +133,117 @@ class xPicP {
   qmiv_pic* m_Staging = nullptr;  // packed 4:2:0 upload buffer of fromPacked
   int32V2 m_Size = {0, 0};
   int32 m_BitDepth = 0;
+};
+
+// Read side of the reference's sequences: xSeqBase (libs/pmbb-core/src/xSeq.h:17-150,
+// xSeq.cpp:28-174) with the raw backend xSeq and the PNG-list backend xSeqPNG
+// (libs/pmbb-spng/src/xSeqPNG.h:14-38). Frames are read by a background thread into pinned buffers
+// and copied to the device asynchronously on the picture's context (qmiv_seq_*); readFrame gives
+// the 4:4:4 picture the reference's unpack gives (4:2:0 chroma upsampled on the device). Only
+// eMode::Read is implemented.
+class xSeqBase {
+ public:
+  enum class eMode : int32 { Unknown, Read, Write, Append };
+  enum class eRetv : int32 { Success, EndOfFile, Error, WrongArg, NotImplemented };
+  class tResult {
+   public:
+    tResult(eRetv Result, const std::string& Message = std::string()) : m_Result(Result), m_Message(Message) {}
+    explicit operator bool() const { return m_Result == eRetv::Success; }
+    bool operator==(eRetv Res) const { return m_Result == Res; }
+    bool operator!=(eRetv Res) const { return m_Result != Res; }
+    eRetv getResult() const { return m_Result; }
+    const std::string& getMessage() const { return m_Message; }
+
+   private:
+    eRetv m_Result;
+    std::string m_Message;
+  };
+
+  virtual ~xSeqBase() { (void)closeFile(); }
+  xSeqBase(const xSeqBase&) = delete;
+  xSeqBase& operator=(const xSeqBase&) = delete;
+
+  tResult openFile(const std::string& FileName, eMode OpMode) {
+    if (OpMode != eMode::Read) return eRetv::NotImplemented;
+    (void)closeFile();
+    const qmiv_status St = xBackendOpen(FileName);
+    if (St != QMIV_OK) return tResult(eRetv::Error, qmiv_last_error());
+    m_NumOfFrames = qmiv_seq_num_frames(m_Seq);
+    m_CurrFrameIdx = 0;
+    return eRetv::Success;
+  }
+  tResult closeFile() {
+    qmiv_pic_destroy(m_Staging);
+    qmiv_seq_close(m_Seq);
+    m_Staging = nullptr;
+    m_Seq = nullptr;
+    m_NumOfFrames = 0;
+    m_CurrFrameIdx = 0;
+    return eRetv::Success;
+  }
+  int32 getNumOfFrames() const { return m_NumOfFrames; }
+  int32 getCurrFrameIdx() const { return m_CurrFrameIdx; }
+  tResult seekFrame(int32 FrameNumber) {
+    if (!m_Seq) return eRetv::Error;
+    if (FrameNumber >= m_NumOfFrames) return eRetv::WrongArg;
+    m_CurrFrameIdx = FrameNumber;
+    return eRetv::Success;
+  }
+  tResult skipFrame(int32 NumFrames) { return seekFrame(m_CurrFrameIdx + NumFrames); }
+  // The current frame into Pic (of the sequence's size and bit depth), then the next frame.
+  tResult readFrame(xPicP* Pic) {
+    if (!m_Seq) return tResult(eRetv::Error, "OpMode does not allow Read");
+    if (m_CurrFrameIdx >= m_NumOfFrames) return eRetv::EndOfFile;
+    if (!Pic || Pic->getSize() != m_Size || Pic->getBitDepth() != m_BitDepth) return eRetv::WrongArg;
+    qmiv_pic* Dst = Pic->handle();
+    if (m_ChromaFormat != 444) {
+      if (!m_Staging && qmiv_pic_create(Pic->context(), m_Size[0], m_Size[1], m_BitDepth, m_ChromaFormat,
+                                        &m_Staging) != QMIV_OK)
+        return tResult(eRetv::Error, qmiv_last_error());
+      Dst = m_Staging;
+    }
+    if (qmiv_seq_read(Pic->context(), m_Seq, m_CurrFrameIdx, Dst) != QMIV_OK ||
+        (Dst != Pic->handle() && qmiv_pic_upsample(Pic->context(), Dst, Pic->handle()) != QMIV_OK))
+      return tResult(eRetv::Error, qmiv_last_error());
+    m_CurrFrameIdx++;
+    return eRetv::Success;
+  }
+
+ protected:
+  xSeqBase(int32V2 Size, int32 BitDepth, int32 ChromaFormat) : m_Size(Size), m_BitDepth(BitDepth), m_ChromaFormat(ChromaFormat) {}
+  virtual qmiv_status xBackendOpen(const std::string& FileName) = 0;
+
+  qmiv_seq* m_Seq = nullptr;
+  qmiv_pic* m_Staging = nullptr;  // 4:2:0 frame before the upsampling of readFrame
+  int32V2 m_Size;
+  int32 m_BitDepth;
+  int32 m_ChromaFormat;
+  int32 m_NumOfFrames = 0;
+  int32 m_CurrFrameIdx = 0;
+};
+
+// Raw planar file of packed frames (ChromaFormat 420 or 444), xSeq(Size, BitDepth, ChromaFormat).
+class xSeq : public xSeqBase {
+ public:
+  xSeq(int32V2 Size, int32 BitDepth, int32 ChromaFormat) : xSeqBase(Size, BitDepth, ChromaFormat) {}
+
+ protected:
+  qmiv_status xBackendOpen(const std::string& FileName) override {
+    return qmiv_seq_open(FileName.c_str(), m_Size[0], m_Size[1], m_BitDepth, m_ChromaFormat, &m_Seq);
+  }
+};
+
+// List of 8-bit PNG files named by an fmt-style pattern ("frame_{:03d}.png"), read as R, G, B
+// planes: xSeqPNG(Size, MaxNumFiles). The engine stops at 65535 files, the bound the reference
+// application passes.
+class xSeqPNG : public xSeqBase {
+ public:
+  explicit xSeqPNG(int32V2 Size, int32 /*MaxNumFiles*/ = 65535) : xSeqBase(Size, 8, 444) {}
+
+ protected:
+  qmiv_status xBackendOpen(const std::string& FileName) override {
+    return qmiv_seq_open_png(FileName.c_str(), m_Size[0], m_Size[1], &m_Seq);
+  }
 };
 
 // eClrSpcLC (libs/pmbb-core/src/xCommonDefCORE.h:356-364), the standardised YCbCr spaces.

@@ -3,7 +3,8 @@
 //
 //   test_processors W H BD CF FILE0 FILE1 ERP OUT_SCP0 OUT_SCP1 OUT_CLR [MASK BDM CFM]
 //
-// FILE0 / FILE1 hold one packed planar frame each (InputFile0 / InputFile1). Prints one JSON
+// FILE0 / FILE1 hold one packed planar frame each (InputFile0 / InputFile1), read through xPicP::fromPacked
+// and through xSeq. Prints one JSON
 // object with every metric (doubles in %.17g) and writes the two shift-compensated pictures and
 // picture 0 converted RGB -> YCbCr (BT.709) as planar 4:4:4 16-bit files. With a mask file, the
 // masked processor calls of the application's masked flow are printed too ("masked"). Checks the identities the processor API promises and exits 1 on a
@@ -78,8 +79,22 @@ int main(int argc, char** argv) {
     xContext Ctx(0);
     xPicP PicInP0(Ctx, {W, H}, BD), PicInP1(Ctx, {W, H}, BD);
     xPicP PicSCP0(Ctx, {W, H}, BD), PicSCP1(Ctx, {W, H}, BD);
+    // InputFile0 through the unpack of a packed frame, InputFile1 through the sequence reader
+    // (xSeq::openFile / readFrame, xAppQMIV.cpp:537-541, :674); both paths must give the same picture.
     PicInP0.fromPacked(F0.data(), CF);
-    PicInP1.fromPacked(F1.data(), CF);
+    xSeq Seq0({W, H}, BD, CF), Seq1({W, H}, BD, CF);
+    CHECK(Seq0.openFile(argv[5], xSeq::eMode::Read) && Seq1.openFile(argv[6], xSeq::eMode::Read));
+    CHECK(Seq1.getNumOfFrames() == 1 && Seq1.seekFrame(1) == xSeq::eRetv::WrongArg);
+    CHECK(Seq1.readFrame(&PicInP1));
+    CHECK(Seq1.readFrame(&PicInP1) == xSeq::eRetv::EndOfFile);
+    {
+      xPicP Check0(Ctx, {W, H}, BD);
+      CHECK(Seq0.readFrame(&Check0));
+      CHECK(planes16(Check0) == planes16(PicInP0));
+      xPicP Check1(Ctx, {W, H}, BD);
+      Check1.fromPacked(F1.data(), CF);
+      CHECK(planes16(Check1) == planes16(PicInP1));
+    }
 
     xGlobClrDiffProc ProcGCD(Ctx);
     xShftCompPicProc ProcSCP(Ctx);
