@@ -70,18 +70,24 @@ def ssim_flops_per_frame(w, h):
     return 3 * per_cmp * 1000, 3 * (1 + 5 + 2)
 
 
-def read_traffic(name):
-    """HBM bytes per launch of the dominant kernel from its committed rocprofv3 PMC summary
-    (line `traffic_bytes_per_launch: N`), or None."""
+def read_profile_value(name, key):
+    """A figure of the dominant kernel from its committed rocprofv3 PMC summary (a line `key: N`),
+    or None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return int(float(line.split(":", 1)[1]))
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def read_traffic(name):
+    """HBM bytes per launch of the dominant kernel (line `traffic_bytes_per_launch: N`), or None."""
+    v = read_profile_value(name, "traffic_bytes_per_launch")
+    return None if v is None else int(v)
 
 
 def cpu_baseline(wl, pairs, sample_frames):
@@ -477,7 +483,11 @@ def main():
         achieved = per_launch / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic(wl["traffic_file"]),
-                "kernel": "k_cps_fast420", "kernel_avg_us": round(k_avg_ms * 1e3, 2), "bytes_per_launch": per_launch}
+                "kernel": "k_cps_fast420", "kernel_avg_us": round(k_avg_ms * 1e3, 2), "bytes_per_launch": per_launch,
+                # The search is bound by int32 VALU issue, not HBM: 25 candidates x 2 directions per
+                # pel, and the VALU busy fraction of the same kernel from its PMC summary.
+                "candidate_evals_per_s": round(50 * W * H / (k_avg_ms / 1e3), -6) if k_avg_ms > 0 else 0.0,
+                "valu_busy_frac": read_profile_value(wl["traffic_file"], "valu_busy_frac")}
     else:
         flops, launches = ssim_flops_per_frame(W, H)
         per_launch = flops / launches
