@@ -46,55 +46,87 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
   return v;
 }
 
-// One loaded window row R of the lane's K x C output pels. Sample column c (relative to the lane's
-// first window column) feeds output column j with window column dxi = c - j, and output row k
-// with window row dy = R - k, weight F[dy][dxi]; k runs over max(0, R - 9) .. min(K - 1, R).
-template <int R>
-__device__ __forceinline__ void ssim_row(const float* s_t, const float* s_r, const double* s_w, int base,
-                                         double (&acc)[kSsimC][kSsimK][5]) {
+// The window is walked as steps (R, c): loaded window row R = 0 .. K + 8 of the lane's K x C output
+// pels, sample column c = 0 .. 8 + C relative to the lane's first window column. Sample column c
+// feeds output column j with window column dxi = c - j and output row k with window row dy = R - k,
+// weight F[dy][dxi]; k runs over max(0, R - 9) .. min(K - 1, R).
+constexpr int kSsimSW = 9 + kSsimC;               // sample columns per loaded row
+constexpr int kSsimSteps = (kSsimK + 9) * kSsimSW;
+
+// One step's operands: the two samples and the weights of every (j, k) they feed.
+struct SsimStep {
+  float t, r;
+  double w[kSsimC][kSsimK];
+};
+
+template <int S>
+__device__ __forceinline__ void ssim_load(const float* s_t, const float* s_r, const double* s_w, int base,
+                                          SsimStep& d) {
+  constexpr int R = S / kSsimSW, c = S % kSsimSW;
   constexpr int k_lo = R > 9 ? R - 9 : 0;
   constexpr int k_hi = R < kSsimK - 1 ? R : kSsimK - 1;
-  const int row = base + R * kSsimLW;
+  d.t = s_t[base + R * kSsimLW + c];
+  d.r = s_r[base + R * kSsimLW + c];
 #pragma unroll
-  for (int c = 0; c < 9 + kSsimC; c++) {
-    const double tv = (double)s_t[row + c];
-    const double rv = (double)s_r[row + c];
-    const double rr = rv * rv, tt = tv * tv, rt = rv * tv;
+  for (int j = 0; j < kSsimC; j++) {
+    constexpr int unused = 0;
+    (void)unused;
+    const int dxi = c - j;
+    if (dxi < 0 || dxi > 9) continue;
 #pragma unroll
-    for (int j = 0; j < kSsimC; j++) {
-      const int dxi = c - j;
-      if (dxi < 0 || dxi > 9) continue;
-#pragma unroll
-      for (int k = k_lo; k <= k_hi; k++) {
-        const double w = s_w[dxi * kSsimWRow + R - k + 7];  // F[R - k][dxi]
-        acc[j][k][0] = __builtin_fma(rv, w, acc[j][k][0]);
-        acc[j][k][1] = __builtin_fma(tv, w, acc[j][k][1]);
-        acc[j][k][2] = __builtin_fma(rr, w, acc[j][k][2]);
-        acc[j][k][3] = __builtin_fma(tt, w, acc[j][k][3]);
-        acc[j][k][4] = __builtin_fma(rt, w, acc[j][k][4]);
-      }
-    }
-    // One loaded sample at a time: the straight-line body is one basic block, so without these
-    // the instruction DAG would hoist every load and product above every FMA. Passing the
-    // accumulators through an ordered no-op pins this step's FMAs before the next step's loads
-    // (and the memory clobber keeps weights from being held in registers across steps).
-#pragma unroll
-    for (int j = 0; j < kSsimC; j++) {
-      const int dxi = c - j;
-      if (dxi < 0 || dxi > 9) continue;
-#pragma unroll
-      for (int k = k_lo; k <= k_hi; k++)
-        asm volatile("" : "+v"(acc[j][k][0]), "+v"(acc[j][k][1]), "+v"(acc[j][k][2]), "+v"(acc[j][k][3]),
-                     "+v"(acc[j][k][4])::"memory");
-    }
-    __builtin_amdgcn_sched_barrier(0);
+    for (int k = k_lo; k <= k_hi; k++) d.w[j][k] = s_w[dxi * kSsimWRow + R - k + 7];  // F[R - k][dxi]
   }
 }
 
-template <int... R>
-__device__ __forceinline__ void ssim_rows(const float* s_t, const float* s_r, const double* s_w, int base,
-                                          double (&acc)[kSsimC][kSsimK][5], std::integer_sequence<int, R...>) {
-  (ssim_row<R>(s_t, s_r, s_w, base, acc), ...);
+template <int S>
+__device__ __forceinline__ void ssim_fma(const SsimStep& d, double (&acc)[kSsimC][kSsimK][5]) {
+  constexpr int R = S / kSsimSW, c = S % kSsimSW;
+  constexpr int k_lo = R > 9 ? R - 9 : 0;
+  constexpr int k_hi = R < kSsimK - 1 ? R : kSsimK - 1;
+  const double tv = (double)d.t, rv = (double)d.r;
+  const double rr = rv * rv, tt = tv * tv, rt = rv * tv;
+#pragma unroll
+  for (int j = 0; j < kSsimC; j++) {
+    const int dxi = c - j;
+    if (dxi < 0 || dxi > 9) continue;
+#pragma unroll
+    for (int k = k_lo; k <= k_hi; k++) {
+      const double w = d.w[j][k];
+      acc[j][k][0] = __builtin_fma(rv, w, acc[j][k][0]);
+      acc[j][k][1] = __builtin_fma(tv, w, acc[j][k][1]);
+      acc[j][k][2] = __builtin_fma(rr, w, acc[j][k][2]);
+      acc[j][k][3] = __builtin_fma(tt, w, acc[j][k][3]);
+      acc[j][k][4] = __builtin_fma(rt, w, acc[j][k][4]);
+    }
+  }
+  // Passing the step's accumulators through an ordered no-op pins its FMAs in place: the body is
+  // one basic block, and without these the instruction DAG would hoist every load and product above
+  // every FMA (the memory clobber keeps the next loads below, the weights out of registers).
+#pragma unroll
+  for (int j = 0; j < kSsimC; j++) {
+    const int dxi = c - j;
+    if (dxi < 0 || dxi > 9) continue;
+#pragma unroll
+    for (int k = k_lo; k <= k_hi; k++)
+      asm volatile("" : "+v"(acc[j][k][0]), "+v"(acc[j][k][1]), "+v"(acc[j][k][2]), "+v"(acc[j][k][3]),
+                   "+v"(acc[j][k][4])::"memory");
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Software pipeline over the steps: the operands of step S + 1 are loaded from LDS before the FMAs
+// of step S, so the LDS latency of every step overlaps the previous step's arithmetic.
+template <int S>
+__device__ __forceinline__ void ssim_steps(const float* s_t, const float* s_r, const double* s_w, int base,
+                                           double (&acc)[kSsimC][kSsimK][5], const SsimStep& cur) {
+  if constexpr (S + 1 < kSsimSteps) {
+    SsimStep nxt;
+    ssim_load<S + 1>(s_t, s_r, s_w, base, nxt);
+    ssim_fma<S>(cur, acc);
+    ssim_steps<S + 1>(s_t, s_r, s_w, base, acc, nxt);
+  } else {
+    ssim_fma<S>(cur, acc);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -145,10 +177,14 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
 #pragma unroll
       for (int m = 0; m < 5; m++) acc[j][k][m] = 0.0;
 
-  // Every loaded row r = 0 .. K + 8 as its own compile-time step (the loop unroller gives up on
-  // this many instructions): straight-line code, constant LDS offsets, no branches.
-  ssim_rows(s_t, s_r, s_w, (kSsimK * wave) * kSsimLW + kSsimC * lane, acc,
-            std::make_integer_sequence<int, kSsimK + 9>{});
+  // Every step as its own compile-time instance (the loop unroller gives up on this many
+  // instructions): straight-line code, constant LDS offsets, no branches.
+  {
+    const int base = (kSsimK * wave) * kSsimLW + kSsimC * lane;
+    SsimStep first;
+    ssim_load<0>(s_t, s_r, s_w, base, first);
+    ssim_steps<0>(s_t, s_r, s_w, base, acc, first);
+  }
 
 #pragma unroll
   for (int k = 0; k < kSsimK; k++) {
