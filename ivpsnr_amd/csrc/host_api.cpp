@@ -405,7 +405,47 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
   a.c2 = k2 * k2;
   const uint16_t* scp0 = scp;
   const uint16_t* scp1 = scp0 ? scp0 + 3 * (size_t)g.w * g.h : nullptr;
-  for (const SsimEval& e : L.ssim) {
+  // Full-resolution evaluations of one component that share pictures run as one fused pass
+  // (launch_ssim_multi): SSIM(P0, P1), its CS (MS-SSIM level 0 or SSIM without L), and IV-SSIM's
+  // SSIM(P0, SCP1) and SSIM(P1, SCP0). The rest run one by one.
+  std::vector<char> fused(L.ssim.size(), 0);
+  for (int c = 0; c < 3; c++) {
+    int i_lcs = -1, i_cs = -1, i_c = -1, i_d = -1;
+    for (size_t i = 0; i < L.ssim.size(); i++) {
+      const SsimEval& e = L.ssim[i];
+      if (e.cmp != c || e.level != 0) continue;
+      if (e.kind == 0) i_lcs = (int)i;
+      else if ((e.kind == 4 || e.kind == 1) && i_cs < 0) i_cs = (int)i;
+      else if (e.kind == 2 && scp1) i_c = (int)i;
+      else if (e.kind == 3 && scp0) i_d = (int)i;
+    }
+    const int n = (i_lcs >= 0) + (i_cs >= 0) + (i_c >= 0) + (i_d >= 0);
+    if ((i_lcs < 0 && i_cs < 0) || n < 2) continue;
+    SsimMultiArgs m{};
+    std::memcpy(m.f, a.f, sizeof(m.f));
+    m.c1 = a.c1;
+    m.c2 = a.c2;
+    m.w = g.w;
+    m.h = g.h;
+    m.tiles_x = ssim_tiles_x(g.w);
+    const size_t plane = (size_t)g.w * g.h;
+    m.p0 = pic_view(p0, c);
+    m.p1 = pic_view(p1, c);
+    if (i_c >= 0) m.scp1 = u16_view(scp1 + c * plane, g.w, g.h);
+    if (i_d >= 0) m.scp0 = u16_view(scp0 + c * plane, g.w, g.h);
+    auto part = [&](int i) { return i >= 0 ? (double*)(dbase + L.ssim[i].off_part) : nullptr; };
+    m.part_lcs = part(i_lcs);
+    m.part_cs = part(i_cs);
+    m.part_c = part(i_c);
+    m.part_d = part(i_d);
+    for (int i : {i_lcs, i_cs, i_c, i_d})
+      if (i >= 0) fused[i] = 1;
+    KernelTimer t(ctx, QMIV_K_SSIM, st);
+    QMIV_HIP(launch_ssim_multi(m, st));
+  }
+  for (size_t ei = 0; ei < L.ssim.size(); ei++) {
+    if (fused[ei]) continue;
+    const SsimEval& e = L.ssim[ei];
     a.w = e.w;
     a.h = e.h;
     a.tiles_x = ssim_tiles_x(e.w);

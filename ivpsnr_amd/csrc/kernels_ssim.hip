@@ -21,7 +21,6 @@ constexpr int kSsimTW = 64 * kSsimC;                 // output columns per workg
 constexpr int kSsimTH = 4 * kSsimK;                  // output rows per workgroup
 constexpr int kSsimLW = (kSsimTW + 9 + 1) & ~1;      // window columns dx in [-5, 4], even row stride
 constexpr int kSsimLH = kSsimTH + 9;
-constexpr int kSsimWRow = kSsimK + 16;               // stride of one transposed weight column in LDS
 static_assert(kSsimC == 1 || kSsimC == 2, "one or two columns per lane");
 
 constexpr int32_t kBufDword3 = 0x00020000;
@@ -53,33 +52,38 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 constexpr int kSsimSW = 9 + kSsimC;               // sample columns per loaded row
 constexpr int kSsimSteps = (kSsimK + 9) * kSsimSW;
 
-// One step's operands: the two samples and the weights of every (j, k) they feed.
+// The window F is symmetric: F[a][b] = F[b][a], and F[5 + d][b] = F[5 - d][b] (a Gaussian of the
+// offsets -5 .. 4). Its 100 taps hold 21 distinct weights, kept in scalar registers for the whole
+// kernel: tap (dy, dx) uses weight sym_id(dy, dx). launch_ssim checks the symmetry of the table.
+constexpr int kSsimSym = 21;
+__host__ __device__ constexpr int sym_off(int i) { return i < 5 ? 5 - i : i - 5; }  // |offset| of tap index i
+__host__ __device__ constexpr int sym_id(int dy, int dx) {
+  const int a = sym_off(dy) < sym_off(dx) ? sym_off(dy) : sym_off(dx);
+  const int b = sym_off(dy) < sym_off(dx) ? sym_off(dx) : sym_off(dy);
+  return b * (b + 1) / 2 + a;  // pairs a <= b of 0 .. 5
+}
+__host__ __device__ constexpr int sym_rep(int u) {  // a tap index (dy * 10 + dx) of weight u
+  int b = 0;
+  while ((b + 1) * (b + 2) / 2 <= u) b++;
+  const int a = u - b * (b + 1) / 2;
+  return (5 - a) * 10 + (5 - b);
+}
+
+// One step's operands: the two samples.
 struct SsimStep {
   float t, r;
-  double w[kSsimC][kSsimK];
 };
 
 template <int S>
-__device__ __forceinline__ void ssim_load(const float* s_t, const float* s_r, const double* s_w, int base,
-                                          SsimStep& d) {
+__device__ __forceinline__ void ssim_load(const float* s_t, const float* s_r, int base, SsimStep& d) {
   constexpr int R = S / kSsimSW, c = S % kSsimSW;
-  constexpr int k_lo = R > 9 ? R - 9 : 0;
-  constexpr int k_hi = R < kSsimK - 1 ? R : kSsimK - 1;
   d.t = s_t[base + R * kSsimLW + c];
   d.r = s_r[base + R * kSsimLW + c];
-#pragma unroll
-  for (int j = 0; j < kSsimC; j++) {
-    constexpr int unused = 0;
-    (void)unused;
-    const int dxi = c - j;
-    if (dxi < 0 || dxi > 9) continue;
-#pragma unroll
-    for (int k = k_lo; k <= k_hi; k++) d.w[j][k] = s_w[dxi * kSsimWRow + R - k + 7];  // F[R - k][dxi]
-  }
 }
 
 template <int S>
-__device__ __forceinline__ void ssim_fma(const SsimStep& d, double (&acc)[kSsimC][kSsimK][5]) {
+__device__ __forceinline__ void ssim_fma(const SsimStep& d, const double (&wv)[kSsimSym],
+                                         double (&acc)[kSsimC][kSsimK][5]) {
   constexpr int R = S / kSsimSW, c = S % kSsimSW;
   constexpr int k_lo = R > 9 ? R - 9 : 0;
   constexpr int k_hi = R < kSsimK - 1 ? R : kSsimK - 1;
@@ -91,7 +95,7 @@ __device__ __forceinline__ void ssim_fma(const SsimStep& d, double (&acc)[kSsimC
     if (dxi < 0 || dxi > 9) continue;
 #pragma unroll
     for (int k = k_lo; k <= k_hi; k++) {
-      const double w = d.w[j][k];
+      const double w = wv[sym_id(R - k, dxi)];  // F[R - k][dxi]
       acc[j][k][0] = __builtin_fma(rv, w, acc[j][k][0]);
       acc[j][k][1] = __builtin_fma(tv, w, acc[j][k][1]);
       acc[j][k][2] = __builtin_fma(rr, w, acc[j][k][2]);
@@ -101,7 +105,7 @@ __device__ __forceinline__ void ssim_fma(const SsimStep& d, double (&acc)[kSsimC
   }
   // Passing the step's accumulators through an ordered no-op pins its FMAs in place: the body is
   // one basic block, and without these the instruction DAG would hoist every load and product above
-  // every FMA (the memory clobber keeps the next loads below, the weights out of registers).
+  // every FMA (the memory clobber keeps the next loads below).
 #pragma unroll
   for (int j = 0; j < kSsimC; j++) {
     const int dxi = c - j;
@@ -117,15 +121,15 @@ __device__ __forceinline__ void ssim_fma(const SsimStep& d, double (&acc)[kSsimC
 // Software pipeline over the steps: the operands of step S + 1 are loaded from LDS before the FMAs
 // of step S, so the LDS latency of every step overlaps the previous step's arithmetic.
 template <int S>
-__device__ __forceinline__ void ssim_steps(const float* s_t, const float* s_r, const double* s_w, int base,
+__device__ __forceinline__ void ssim_steps(const float* s_t, const float* s_r, int base, const double (&wv)[kSsimSym],
                                            double (&acc)[kSsimC][kSsimK][5], const SsimStep& cur) {
   if constexpr (S + 1 < kSsimSteps) {
     SsimStep nxt;
-    ssim_load<S + 1>(s_t, s_r, s_w, base, nxt);
-    ssim_fma<S>(cur, acc);
-    ssim_steps<S + 1>(s_t, s_r, s_w, base, acc, nxt);
+    ssim_load<S + 1>(s_t, s_r, base, nxt);
+    ssim_fma<S>(cur, wv, acc);
+    ssim_steps<S + 1>(s_t, s_r, base, wv, acc, nxt);
   } else {
-    ssim_fma<S>(cur, acc);
+    ssim_fma<S>(cur, wv, acc);
   }
 }
 
@@ -141,9 +145,6 @@ template <int CALC_L>
 __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
   __shared__ float s_t[kSsimLH * kSsimLW];
   __shared__ float s_r[kSsimLH * kSsimLW];
-  // Weights transposed: s_w[dx * kSsimWRow + dy + 7] = F[dy][dx] for dy in [0, 9], so the weights
-  // F[r - k][dx] of the lane's pels k for loaded row r are consecutive doubles.
-  __shared__ __attribute__((aligned(16))) double s_w[10 * kSsimWRow];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int tx = blockIdx.x;
@@ -152,10 +153,12 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
   const int y0 = 5 + ty * kSsimTH;
   const ViewBuf T(a.tst), R(a.ref);
 
-  for (int i = threadIdx.x; i < 10 * kSsimWRow; i += 256) {
-    const int dx = i / kSsimWRow;
-    const int dy = i - dx * kSsimWRow - 7;
-    s_w[i] = (dy >= 0 && dy <= 9) ? a.f[dy * 10 + dx] : 0.0;
+  // The 21 distinct weights, read once from the kernel arguments and held in scalar registers.
+  double wv[kSsimSym];
+#pragma unroll
+  for (int u = 0; u < kSsimSym; u++) {
+    wv[u] = a.f[sym_rep(u)];
+    asm volatile("" : "+s"(wv[u]));
   }
   // Stage the window rows [y0 - 5, y0 + TH + 4) x [x0 - 5, x0 + TW + 4), clamped inside the
   // picture (clamped values only feed pels outside the interior, which are discarded).
@@ -182,8 +185,8 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
   {
     const int base = (kSsimK * wave) * kSsimLW + kSsimC * lane;
     SsimStep first;
-    ssim_load<0>(s_t, s_r, s_w, base, first);
-    ssim_steps<0>(s_t, s_r, s_w, base, acc, first);
+    ssim_load<0>(s_t, s_r, base, first);
+    ssim_steps<0>(s_t, s_r, base, wv, acc, first);
   }
 
 #pragma unroll
@@ -209,6 +212,194 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
     }
     v = wave_sum_f64(v);
     if (lane == 0 && y < a.h - 5) a.partials[(int64_t)tx * a.h + y] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Several full-resolution evaluations of one component in one pass. SSIM(P0, P1) with L·CS, its
+// CS alone (MS-SSIM level 0, or SSIM without L), IV-SSIM's SSIM(P0, SCP1) and SSIM(P1, SCP0)
+// (xSSIM.cpp:54-66,67-87,192-212) share window moments: μ and E[x²] of P0 and P1 serve all of
+// them, so the set needs 5 + 3 + 3 moment sums where the evaluations one by one need 5 each. Every
+// moment is accumulated with the same fused multiply-adds in the same order as in k_ssim, so each
+// output is bit-identical to its separate evaluation. Outputs with a null partials pointer are
+// skipped; HAS_C / HAS_D add the SCP1 / SCP0 moments.
+// ---------------------------------------------------------------------------------------------
+template <int KM>
+struct MultiTile {
+  static constexpr int TH = 4 * KM;
+  static constexpr int LH = TH + 9;
+  static constexpr int Steps = (KM + 9) * 10;
+};
+
+template <bool HAS_C, bool HAS_D>
+struct MultiStep {
+  float p0, p1, sc, sd;
+};
+
+template <int KM, bool HAS_C, bool HAS_D, int S>
+__device__ __forceinline__ void multi_load(const float* s0, const float* s1, const float* sc, const float* sd,
+                                           int base, MultiStep<HAS_C, HAS_D>& d) {
+  constexpr int R = S / 10, c = S % 10;
+  const int o = base + R * kSsimLW + c;
+  d.p0 = s0[o];
+  d.p1 = s1[o];
+  if (HAS_C) d.sc = sc[o];
+  if (HAS_D) d.sd = sd[o];
+}
+
+// Moment slots per output row k: 0 Σw·P0, 1 Σw·P1, 2 Σw·P0², 3 Σw·P1², 4 Σw·P0·P1, then for SCP1
+// 5 Σw·S1, 6 Σw·S1², 7 Σw·S1·P0, then for SCP0 8 (or 5) Σw·S0, Σw·S0², Σw·S0·P1.
+template <bool HAS_C, bool HAS_D>
+constexpr int multi_moments() { return 5 + (HAS_C ? 3 : 0) + (HAS_D ? 3 : 0); }
+
+template <int KM, bool HAS_C, bool HAS_D, int S>
+__device__ __forceinline__ void multi_fma(const MultiStep<HAS_C, HAS_D>& d, const double (&wv)[kSsimSym],
+                                          double (&acc)[KM][multi_moments<HAS_C, HAS_D>()]) {
+  constexpr int R = S / 10, c = S % 10;
+  constexpr int k_lo = R > 9 ? R - 9 : 0;
+  constexpr int k_hi = R < KM - 1 ? R : KM - 1;
+  constexpr int oD = HAS_C ? 8 : 5;
+  const double v0 = (double)d.p0, v1 = (double)d.p1;
+  const double q00 = v0 * v0, q11 = v1 * v1, q10 = v1 * v0;  // k_ssim's rr, tt, rt as (ref, tst) = (P1, P0)
+  double vc = 0, qcc = 0, qc0 = 0, vd = 0, qdd = 0, qd1 = 0;
+  if (HAS_C) {
+    vc = (double)d.sc;
+    qcc = vc * vc;
+    qc0 = vc * v0;  // (ref, tst) = (SCP1, P0)
+  }
+  if (HAS_D) {
+    vd = (double)d.sd;
+    qdd = vd * vd;
+    qd1 = vd * v1;  // (ref, tst) = (SCP0, P1)
+  }
+#pragma unroll
+  for (int k = k_lo; k <= k_hi; k++) {
+    const double w = wv[sym_id(R - k, c)];
+    acc[k][0] = __builtin_fma(v0, w, acc[k][0]);
+    acc[k][1] = __builtin_fma(v1, w, acc[k][1]);
+    acc[k][2] = __builtin_fma(q00, w, acc[k][2]);
+    acc[k][3] = __builtin_fma(q11, w, acc[k][3]);
+    acc[k][4] = __builtin_fma(q10, w, acc[k][4]);
+    if (HAS_C) {
+      acc[k][5] = __builtin_fma(vc, w, acc[k][5]);
+      acc[k][6] = __builtin_fma(qcc, w, acc[k][6]);
+      acc[k][7] = __builtin_fma(qc0, w, acc[k][7]);
+    }
+    if (HAS_D) {
+      acc[k][oD] = __builtin_fma(vd, w, acc[k][oD]);
+      acc[k][oD + 1] = __builtin_fma(qdd, w, acc[k][oD + 1]);
+      acc[k][oD + 2] = __builtin_fma(qd1, w, acc[k][oD + 2]);
+    }
+  }
+#pragma unroll
+  for (int k = k_lo; k <= k_hi; k++)
+#pragma unroll
+    for (int m = 0; m < multi_moments<HAS_C, HAS_D>(); m++) asm volatile("" : "+v"(acc[k][m])::"memory");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int KM, bool HAS_C, bool HAS_D, int S>
+__device__ __forceinline__ void multi_steps(const float* s0, const float* s1, const float* sc, const float* sd,
+                                            int base, const double (&wv)[kSsimSym],
+                                            double (&acc)[KM][multi_moments<HAS_C, HAS_D>()],
+                                            const MultiStep<HAS_C, HAS_D>& cur) {
+  if constexpr (S + 1 < MultiTile<KM>::Steps) {
+    MultiStep<HAS_C, HAS_D> nxt;
+    multi_load<KM, HAS_C, HAS_D, S + 1>(s0, s1, sc, sd, base, nxt);
+    multi_fma<KM, HAS_C, HAS_D, S>(cur, wv, acc);
+    multi_steps<KM, HAS_C, HAS_D, S + 1>(s0, s1, sc, sd, base, wv, acc, nxt);
+  } else {
+    multi_fma<KM, HAS_C, HAS_D, S>(cur, wv, acc);
+  }
+}
+
+// One output of the set from its (ref, tst) moments, as k_ssim's epilogue computes it.
+__device__ __forceinline__ double ssim_value(double aR, double aT, double eRR, double eTT, double eRT, double c1,
+                                             double c2, bool calc_l) {
+  const double vR = eRR - aR * aR;
+  const double vT = eTT - aT * aT;
+  const double cov = eRT - aR * aT;
+  const double cs = (2 * cov + c2) / (vR + vT + c2);
+  if (!calc_l) return cs;
+  const double l = (2 * aR * aT + c1) / (aR * aR + aT * aT + c1);
+  return l * cs;
+}
+
+template <int KM, bool HAS_C, bool HAS_D>
+__global__ __launch_bounds__(256) void k_ssim_multi(SsimMultiArgs a) {
+  using Tile = MultiTile<KM>;
+  constexpr int NM = multi_moments<HAS_C, HAS_D>();
+  constexpr int oD = HAS_C ? 8 : 5;
+  __shared__ float s0[Tile::LH * kSsimLW];
+  __shared__ float s1[Tile::LH * kSsimLW];
+  __shared__ float sc[HAS_C ? Tile::LH * kSsimLW : 1];
+  __shared__ float sd[HAS_D ? Tile::LH * kSsimLW : 1];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int tx = blockIdx.x;
+  const int ty = blockIdx.y;
+  const int x0 = 5 + tx * kSsimTW;
+  const int y0 = 5 + ty * Tile::TH;
+  const ViewBuf P0(a.p0), P1(a.p1), SC(a.scp1), SD(a.scp0);
+
+  double wv[kSsimSym];
+#pragma unroll
+  for (int u = 0; u < kSsimSym; u++) {
+    wv[u] = a.f[sym_rep(u)];
+    asm volatile("" : "+s"(wv[u]));
+  }
+  for (int i = threadIdx.x; i < Tile::LH * kSsimLW; i += 256) {
+    const int ly = i / kSsimLW;
+    const int lx = i - ly * kSsimLW;
+    const int gy = min(y0 - 5 + ly, a.h - 1);
+    const int gx = min(x0 - 5 + lx, a.w - 1);
+    s0[i] = (float)P0.at(gx, gy);
+    s1[i] = (float)P1.at(gx, gy);
+    if (HAS_C) sc[i] = (float)SC.at(gx, gy);
+    if (HAS_D) sd[i] = (float)SD.at(gx, gy);
+  }
+  __syncthreads();
+
+  double acc[KM][NM];
+#pragma unroll
+  for (int k = 0; k < KM; k++)
+#pragma unroll
+    for (int m = 0; m < NM; m++) acc[k][m] = 0.0;
+  {
+    const int base = (KM * wave) * kSsimLW + lane;
+    MultiStep<HAS_C, HAS_D> first;
+    multi_load<KM, HAS_C, HAS_D, 0>(s0, s1, sc, sd, base, first);
+    multi_steps<KM, HAS_C, HAS_D, 0>(s0, s1, sc, sd, base, wv, acc, first);
+  }
+
+  const int x = x0 + lane;
+#pragma unroll
+  for (int k = 0; k < KM; k++) {
+    const int y = y0 + KM * wave + k;
+    const bool in = x < a.w - 5 && y < a.h - 5;
+    const int64_t o = (int64_t)tx * a.h + y;
+    // (ref, tst) = (P1, P0)
+    if (a.part_lcs) {
+      double v = in ? ssim_value(acc[k][1], acc[k][0], acc[k][3], acc[k][2], acc[k][4], a.c1, a.c2, true) : 0.0;
+      v = wave_sum_f64(v);
+      if (lane == 0 && y < a.h - 5) a.part_lcs[o] = v;
+    }
+    if (a.part_cs) {
+      double v = in ? ssim_value(acc[k][1], acc[k][0], acc[k][3], acc[k][2], acc[k][4], a.c1, a.c2, false) : 0.0;
+      v = wave_sum_f64(v);
+      if (lane == 0 && y < a.h - 5) a.part_cs[o] = v;
+    }
+    if (HAS_C) {  // (ref, tst) = (SCP1, P0)
+      double v = in ? ssim_value(acc[k][5], acc[k][0], acc[k][6], acc[k][2], acc[k][7], a.c1, a.c2, true) : 0.0;
+      v = wave_sum_f64(v);
+      if (lane == 0 && y < a.h - 5) a.part_c[o] = v;
+    }
+    if (HAS_D) {  // (ref, tst) = (SCP0, P1)
+      double v = in ? ssim_value(acc[k][oD], acc[k][1], acc[k][oD + 1], acc[k][3], acc[k][oD + 2], a.c1, a.c2, true)
+                    : 0.0;
+      v = wave_sum_f64(v);
+      if (lane == 0 && y < a.h - 5) a.part_d[o] = v;
+    }
   }
 }
 
@@ -241,6 +432,8 @@ __global__ __launch_bounds__(256) void k_downsample(DownsampleJobs j) {
 }  // namespace
 
 hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream) {
+  for (int i = 0; i < 100; i++)  // the kernel holds only the 21 distinct weights of a symmetric window
+    if (a.f[i] != a.f[sym_rep(sym_id(i / 10, i % 10))]) return hipErrorInvalidValue;
   const int tiles_x = a.w > 10 ? (a.w - 10 + kSsimTW - 1) / kSsimTW : 0;
   const int tiles_y = a.h > 10 ? (a.h - 10 + kSsimTH - 1) / kSsimTH : 0;
   if (tiles_x != a.tiles_x) return hipErrorInvalidValue;
@@ -253,6 +446,27 @@ hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream) {
     return hipGetLastError();
   }
   return hipSuccess;
+}
+
+#ifndef QMIV_SSIM_MULTI_ROWS
+#define QMIV_SSIM_MULTI_ROWS 4
+#endif
+constexpr int kMultiK = QMIV_SSIM_MULTI_ROWS;
+
+hipError_t launch_ssim_multi(const SsimMultiArgs& a, hipStream_t stream) {
+  for (int i = 0; i < 100; i++)
+    if (a.f[i] != a.f[sym_rep(sym_id(i / 10, i % 10))]) return hipErrorInvalidValue;
+  const int tiles_x = a.w > 10 ? (a.w - 10 + kSsimTW - 1) / kSsimTW : 0;
+  const int tiles_y = a.h > 10 ? (a.h - 10 + MultiTile<kMultiK>::TH - 1) / MultiTile<kMultiK>::TH : 0;
+  if (tiles_x != a.tiles_x || (!a.part_lcs && !a.part_cs)) return hipErrorInvalidValue;
+  if (tiles_x == 0 || tiles_y == 0) return hipSuccess;
+  const dim3 grid(tiles_x, tiles_y), block(256);
+  const bool c = a.part_c != nullptr, d = a.part_d != nullptr;
+  if (c && d) hipLaunchKernelGGL((k_ssim_multi<kMultiK, true, true>), grid, block, 0, stream, a);
+  else if (c) hipLaunchKernelGGL((k_ssim_multi<kMultiK, true, false>), grid, block, 0, stream, a);
+  else if (d) hipLaunchKernelGGL((k_ssim_multi<kMultiK, false, true>), grid, block, 0, stream, a);
+  else hipLaunchKernelGGL((k_ssim_multi<kMultiK, false, false>), grid, block, 0, stream, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_ssim_rows(const SsimRowsJobs& j, hipStream_t stream) {

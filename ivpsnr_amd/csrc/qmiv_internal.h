@@ -72,6 +72,20 @@ struct SsimArgs {
   double f[100];         // window weights F[dy + 5][dx + 5], dy, dx in [-5, 4]
 };
 
+// Arguments of the fused full-resolution SSIM kernel: one component, pictures P0, P1 and the
+// shift-compensated SCP1 / SCP0 (IV-SSIM), and one tile-partials array per output (null: not asked).
+struct SsimMultiArgs {
+  PlaneView p0, p1, scp1, scp0;
+  int32_t w, h;
+  int32_t tiles_x;       // ssim_tiles_x(w)
+  double c1, c2;
+  double* part_lcs;      // SSIM(P0, P1), L·CS
+  double* part_cs;       // CS(P0, P1) (MS-SSIM level 0, or SSIM without L)
+  double* part_c;        // SSIM(P0, SCP1), L·CS
+  double* part_d;        // SSIM(P1, SCP0), L·CS
+  double f[100];
+};
+
 // Kernel launchers (kernels.hip). All launch on `stream` and return hipSuccess or the launch error.
 hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* row_sd, uint64_t* row_ssd,
                         uint32_t* row_max, hipStream_t stream);
@@ -193,6 +207,7 @@ hipError_t launch_narrow(const uint16_t* src, int64_t n, uint8_t* dst, hipStream
 
 // Windowed moments of one evaluation into a.partials ([tiles_x][h] row sums per tile column).
 hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream);
+hipError_t launch_ssim_multi(const SsimMultiArgs& a, hipStream_t stream);
 int32_t ssim_tiles_x(int32_t w);
 
 // Per-row reduction of every SSIM evaluation of one frame in one launch:

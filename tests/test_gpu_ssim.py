@@ -131,3 +131,24 @@ def test_scp_against_oracle_with_generic_path(engine, oracle):
     P0, P1 = oracle.unpack(pair.ref, w, h, bd), oracle.unpack(pair.tst, w, h, bd)
     np.testing.assert_array_equal(dst_ref, oracle.scp(P1, P0, bd, g, B=3))
     np.testing.assert_array_equal(dst_tst, oracle.scp(P0, P1, bd, -g, B=3))
+
+
+@pytest.mark.parametrize("w,h,bd,cf", [(320, 180, 10, 420), (257, 131, 8, 444), (1920, 1080, 10, 420)])
+def test_fused_full_resolution_pass_equals_single_evaluations(engine, w, h, bd, cf):
+    """The full-resolution evaluations that share pictures run as one fused pass when requested
+    together (SSIM, MS-SSIM level 0, IV-SSIM's two); every value must be bit-identical to the one a
+    request of that metric alone computes, whatever subset is fused."""
+    pair = make_pair(w, h, bd, 23, chroma_format=cf)
+    p0, p1 = _pics(engine, pair)
+    prm = default_params()
+    alone = {m: engine.frame(p0, p1, m, prm) for m in (SSIM, MSSSIM, IVSSIM)}
+    for combo in (SSIM | MSSSIM, SSIM | IVSSIM, MSSSIM | IVSSIM, SSIM_ALL):
+        r = engine.frame(p0, p1, combo, prm)
+        if combo & SSIM:
+            assert list(r.ssim) == list(alone[SSIM].ssim), combo
+        if combo & MSSSIM:
+            assert np.array_equal(np.array(r.msssim_scales), np.array(alone[MSSSIM].msssim_scales), equal_nan=True)
+            assert np.array_equal(list(r.msssim), list(alone[MSSSIM].msssim), equal_nan=True), combo
+        if combo & IVSSIM:
+            a = alone[IVSSIM]
+            assert [r.ivssim, r.ivssim_r2t, r.ivssim_t2r] == [a.ivssim, a.ivssim_r2t, a.ivssim_t2r], combo
