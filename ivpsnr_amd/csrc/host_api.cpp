@@ -443,6 +443,9 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
     KernelTimer t(ctx, QMIV_K_SSIM, st);
     QMIV_HIP(launch_ssim_multi(m, st));
   }
+  // The remaining evaluations (mostly the MS-SSIM pyramid levels, each too small to fill the GPU)
+  // go into batched launches of up to kMaxSsimBatch evaluations; a lone one runs by itself.
+  std::vector<SsimArgs> rest;
   for (size_t ei = 0; ei < L.ssim.size(); ei++) {
     if (fused[ei]) continue;
     const SsimEval& e = L.ssim[ei];
@@ -476,8 +479,32 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
         break;
     }
     a.partials = (double*)(dbase + e.off_part);
+    rest.push_back(a);
+  }
+  for (size_t i0 = 0; i0 < rest.size(); i0 += kMaxSsimBatch) {
+    const size_t n = std::min<size_t>(kMaxSsimBatch, rest.size() - i0);
     KernelTimer t(ctx, QMIV_K_SSIM, st);
-    QMIV_HIP(launch_ssim(a, st));
+    if (n == 1) {
+      QMIV_HIP(launch_ssim(rest[i0], st));
+      continue;
+    }
+    SsimBatchArgs b{};
+    b.n = (int32_t)n;
+    b.c1 = a.c1;
+    b.c2 = a.c2;
+    std::memcpy(b.f, a.f, sizeof(b.f));
+    for (size_t k = 0; k < n; k++) {
+      const SsimArgs& r = rest[i0 + k];
+      SsimBatchJob& J = b.job[k];
+      J.tst = r.tst;
+      J.ref = r.ref;
+      J.w = r.w;
+      J.h = r.h;
+      J.tiles_x = r.tiles_x;
+      J.calc_l = r.calc_l;
+      J.partials = r.partials;
+    }
+    QMIV_HIP(launch_ssim_batch(b, st));
   }
   SsimRowsJobs j{};
   for (const SsimEval& e : L.ssim) {

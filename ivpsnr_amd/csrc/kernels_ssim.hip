@@ -141,23 +141,22 @@ __device__ __forceinline__ void ssim_steps(const float* s_t, const float* s_r, i
 // in the reference's order (dy outer, dx inner; fused multiply-add). Output: one partial row sum
 // per (tile column, row).
 // ---------------------------------------------------------------------------------------------
-template <int CALC_L>
-__global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
-  __shared__ float s_t[kSsimLH * kSsimLW];
-  __shared__ float s_r[kSsimLH * kSsimLW];
+// One tile (kSsimTW x kSsimTH output pels) of one evaluation: stage the window, accumulate the
+// moments, write the tile's per-row partial sums.
+__device__ __forceinline__ void ssim_tile(const PlaneView& tst, const PlaneView& ref, int32_t w, int32_t h,
+                                          bool calc_l, double c1, double c2, const double* f, double* partials,
+                                          int tx, int ty, float* s_t, float* s_r) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int tx = blockIdx.x;
-  const int ty = blockIdx.y;
   const int x0 = 5 + tx * kSsimTW;  // first output column of the tile
   const int y0 = 5 + ty * kSsimTH;
-  const ViewBuf T(a.tst), R(a.ref);
+  const ViewBuf T(tst), R(ref);
 
   // The 21 distinct weights, read once from the kernel arguments and held in scalar registers.
   double wv[kSsimSym];
 #pragma unroll
   for (int u = 0; u < kSsimSym; u++) {
-    wv[u] = a.f[sym_rep(u)];
+    wv[u] = f[sym_rep(u)];
     asm volatile("" : "+s"(wv[u]));
   }
   // Stage the window rows [y0 - 5, y0 + TH + 4) x [x0 - 5, x0 + TW + 4), clamped inside the
@@ -165,8 +164,8 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
   for (int i = threadIdx.x; i < kSsimLH * kSsimLW; i += 256) {
     const int ly = i / kSsimLW;
     const int lx = i - ly * kSsimLW;
-    const int gy = min(y0 - 5 + ly, a.h - 1);
-    const int gx = min(x0 - 5 + lx, a.w - 1);
+    const int gy = min(y0 - 5 + ly, h - 1);
+    const int gx = min(x0 - 5 + lx, w - 1);
     s_t[i] = (float)T.at(gx, gy);
     s_r[i] = (float)R.at(gx, gy);
   }
@@ -196,14 +195,14 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
 #pragma unroll
     for (int j = 0; j < kSsimC; j++) {
       const int x = x0 + kSsimC * lane + j;
-      if (x < a.w - 5 && y < a.h - 5) {
+      if (x < w - 5 && y < h - 5) {
         const double aR = acc[j][k][0], aT = acc[j][k][1];
         const double vR = acc[j][k][2] - aR * aR;
         const double vT = acc[j][k][3] - aT * aT;
         const double cov = acc[j][k][4] - aR * aT;
-        const double cs = (2 * cov + a.c2) / (vR + vT + a.c2);
-        if (CALC_L) {
-          const double l = (2 * aR * aT + a.c1) / (aR * aR + aT * aT + a.c1);
+        const double cs = (2 * cov + c2) / (vR + vT + c2);
+        if (calc_l) {
+          const double l = (2 * aR * aT + c1) / (aR * aR + aT * aT + c1);
           v += l * cs;
         } else {
           v += cs;
@@ -211,8 +210,30 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
       }
     }
     v = wave_sum_f64(v);
-    if (lane == 0 && y < a.h - 5) a.partials[(int64_t)tx * a.h + y] = v;
+    if (lane == 0 && y < h - 5) partials[(int64_t)tx * h + y] = v;
   }
+}
+
+template <int CALC_L>
+__global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
+  __shared__ float s_t[kSsimLH * kSsimLW];
+  __shared__ float s_r[kSsimLH * kSsimLW];
+  ssim_tile(a.tst, a.ref, a.w, a.h, CALC_L != 0, a.c1, a.c2, a.f, a.partials, blockIdx.x, blockIdx.y, s_t, s_r);
+}
+
+// Many small evaluations (the MS-SSIM pyramid levels of all components) in one launch: the
+// workgroups are the evaluations' tiles back to back, job j owning tiles [tile0, tile0 + tiles).
+__global__ __launch_bounds__(256) void k_ssim_batch(SsimBatchArgs a) {
+  __shared__ float s_t[kSsimLH * kSsimLW];
+  __shared__ float s_r[kSsimLH * kSsimLW];
+  const int t = blockIdx.x;
+  int j = 0;
+  while (j + 1 < a.n && t >= a.job[j + 1].tile0) j++;
+  const SsimBatchJob& J = a.job[j];
+  const int local = t - J.tile0;
+  const int ty = local / J.tiles_x;
+  const int tx = local - ty * J.tiles_x;
+  ssim_tile(J.tst, J.ref, J.w, J.h, J.calc_l != 0, a.c1, a.c2, a.f, J.partials, tx, ty, s_t, s_r);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -466,6 +487,24 @@ hipError_t launch_ssim_multi(const SsimMultiArgs& a, hipStream_t stream) {
   else if (c) hipLaunchKernelGGL((k_ssim_multi<kMultiK, true, false>), grid, block, 0, stream, a);
   else if (d) hipLaunchKernelGGL((k_ssim_multi<kMultiK, false, true>), grid, block, 0, stream, a);
   else hipLaunchKernelGGL((k_ssim_multi<kMultiK, false, false>), grid, block, 0, stream, a);
+  return hipGetLastError();
+}
+
+int32_t ssim_tiles_y(int32_t h) { return h > 10 ? (h - 10 + kSsimTH - 1) / kSsimTH : 0; }
+
+hipError_t launch_ssim_batch(SsimBatchArgs& a, hipStream_t stream) {
+  for (int i = 0; i < 100; i++)
+    if (a.f[i] != a.f[sym_rep(sym_id(i / 10, i % 10))]) return hipErrorInvalidValue;
+  if (a.n <= 0 || a.n > kMaxSsimBatch) return hipErrorInvalidValue;
+  int32_t tiles = 0;
+  for (int j = 0; j < a.n; j++) {
+    SsimBatchJob& J = a.job[j];
+    if (J.tiles_x != ssim_tiles_x(J.w)) return hipErrorInvalidValue;
+    J.tile0 = tiles;
+    tiles += J.tiles_x * ssim_tiles_y(J.h);
+  }
+  if (tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ssim_batch, dim3((unsigned)tiles), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 

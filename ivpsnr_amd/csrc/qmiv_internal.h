@@ -86,6 +86,24 @@ struct SsimMultiArgs {
   double f[100];
 };
 
+// Arguments of the batched SSIM kernel: several evaluations (pyramid levels, components) in one
+// launch. tile0 is filled in by the launcher.
+constexpr int kMaxSsimBatch = 16;
+struct SsimBatchJob {
+  PlaneView tst, ref;
+  int32_t w, h;
+  int32_t tiles_x;       // ssim_tiles_x(w)
+  int32_t calc_l;        // 1: L x CS; 0: CS only
+  int32_t tile0;         // first workgroup of this job
+  double* partials;      // [tiles_x][h]
+};
+struct SsimBatchArgs {
+  int32_t n;
+  double c1, c2;
+  double f[100];
+  SsimBatchJob job[kMaxSsimBatch];
+};
+
 // Kernel launchers (kernels.hip). All launch on `stream` and return hipSuccess or the launch error.
 hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* row_sd, uint64_t* row_ssd,
                         uint32_t* row_max, hipStream_t stream);
@@ -208,6 +226,7 @@ hipError_t launch_narrow(const uint16_t* src, int64_t n, uint8_t* dst, hipStream
 // Windowed moments of one evaluation into a.partials ([tiles_x][h] row sums per tile column).
 hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream);
 hipError_t launch_ssim_multi(const SsimMultiArgs& a, hipStream_t stream);
+hipError_t launch_ssim_batch(SsimBatchArgs& a, hipStream_t stream);
 int32_t ssim_tiles_x(int32_t w);
 
 // Per-row reduction of every SSIM evaluation of one frame in one launch:
