@@ -383,19 +383,16 @@ qmiv_status enqueue_ssim(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, 
   for (const SsimEval& e : L.ssim) need_pyr |= e.kind == 1;
   if (need_pyr) {
     KernelTimer t(ctx, QMIV_K_PYRAMID, st);
-    for (int l = 1; l < 5; l++) {
-      DownsampleJobs j{};
-      j.n = 6;
-      j.dw = g.w >> l;
-      j.dh = g.h >> l;
-      for (int pic = 0; pic < 2; pic++)
-        for (int c = 0; c < 3; c++) {
-          j.src[pic * 3 + c] = l == 1 ? pic_view(pic ? p1 : p0, c)
-                                      : u16_view(level_ptr(pic, c, l - 1), g.w >> (l - 1), g.h >> (l - 1));
-          j.dst[pic * 3 + c] = level_ptr(pic, c, l);
-        }
-      QMIV_HIP(launch_downsample(j, st));
-    }
+    PyramidJobs j{};
+    j.n = 6;
+    j.w = g.w;
+    j.h = g.h;
+    for (int pic = 0; pic < 2; pic++)
+      for (int c = 0; c < 3; c++) {
+        j.src[pic * 3 + c] = pic_view(pic ? p1 : p0, c);
+        for (int l = 1; l < 5; l++) j.dst[pic * 3 + c][l - 1] = level_ptr(pic, c, l);
+      }
+    QMIV_HIP(launch_pyramid(j, st));
   }
   SsimArgs a{};
   fin::ssim_window(a.f);

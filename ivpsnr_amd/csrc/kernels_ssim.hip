@@ -438,19 +438,61 @@ __global__ __launch_bounds__(256) void k_ssim_rows(SsimRowsJobs j) {
   j.rows[e][y] = s;
 }
 
-// xPixelOpsSTD::DownsampleHV (libs/pmbb-core/src/xPixelOpsSTD.cpp:50-67): dst = (a + b + c + d + 2) >> 2
-// over 2x2 blocks of the source view; dst is dw x dh uint16 (stride dw). grid.z = plane.
-__global__ __launch_bounds__(256) void k_downsample(DownsampleJobs j) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)j.dw * j.dh) return;
-  const int y = (int)(i / j.dw);
-  const int x = (int)(i - (int64_t)y * j.dw);
-  const ViewBuf S(j.src[blockIdx.z]);
-  const uint32_t v = S.at(2 * x, 2 * y) + S.at(2 * x + 1, 2 * y) + S.at(2 * x, 2 * y + 1) + S.at(2 * x + 1, 2 * y + 1);
-  j.dst[blockIdx.z][i] = (uint16_t)((v + 2) >> 2);
+// All four MS-SSIM pyramid levels of up to six planes in one launch. A workgroup owns a 16 x 16
+// block of level 1 (a 32 x 32 block of the source view) and the 8 x 8, 4 x 4 and 2 x 2 blocks of
+// levels 2-4 below it, each level from the previous one in LDS with DownsampleHV's rounding
+// (libs/pmbb-core/src/xPixelOpsSTD.cpp:50-67), so the cascade is the same as level by level. A level
+// pel is written when it lies inside that level's (W >> l) x (H >> l); its inputs then always lie
+// inside the previous level. grid.z = plane.
+__global__ __launch_bounds__(256) void k_pyramid(PyramidJobs j) {
+  __shared__ uint32_t l1[16][17], l2[8][9], l3[4][5];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int p = blockIdx.z;
+  const ViewBuf S(j.src[p]);
+  const int x1 = blockIdx.x * 16 + tx, y1 = blockIdx.y * 16 + ty;
+  const int w1 = j.w >> 1, h1 = j.h >> 1;
+  uint32_t v = 0;
+  if (x1 < w1 && y1 < h1) {
+    v = (S.at(2 * x1, 2 * y1) + S.at(2 * x1 + 1, 2 * y1) + S.at(2 * x1, 2 * y1 + 1) + S.at(2 * x1 + 1, 2 * y1 + 1) + 2) >> 2;
+    j.dst[p][0][(int64_t)y1 * w1 + x1] = (uint16_t)v;
+  }
+  l1[ty][tx] = v;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int u = threadIdx.x & 7, t = threadIdx.x >> 3;
+    const int x2 = blockIdx.x * 8 + u, y2 = blockIdx.y * 8 + t;
+    const int w2 = j.w >> 2, h2 = j.h >> 2;
+    const uint32_t v2 = (l1[2 * t][2 * u] + l1[2 * t][2 * u + 1] + l1[2 * t + 1][2 * u] + l1[2 * t + 1][2 * u + 1] + 2) >> 2;
+    l2[t][u] = v2;
+    if (x2 < w2 && y2 < h2) j.dst[p][1][(int64_t)y2 * w2 + x2] = (uint16_t)v2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int u = threadIdx.x & 3, t = threadIdx.x >> 2;
+    const int x3 = blockIdx.x * 4 + u, y3 = blockIdx.y * 4 + t;
+    const int w3 = j.w >> 3, h3 = j.h >> 3;
+    const uint32_t v3 = (l2[2 * t][2 * u] + l2[2 * t][2 * u + 1] + l2[2 * t + 1][2 * u] + l2[2 * t + 1][2 * u + 1] + 2) >> 2;
+    l3[t][u] = v3;
+    if (x3 < w3 && y3 < h3) j.dst[p][2][(int64_t)y3 * w3 + x3] = (uint16_t)v3;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int u = threadIdx.x & 1, t = threadIdx.x >> 1;
+    const int x4 = blockIdx.x * 2 + u, y4 = blockIdx.y * 2 + t;
+    const int w4 = j.w >> 4, h4 = j.h >> 4;
+    const uint32_t v4 = (l3[2 * t][2 * u] + l3[2 * t][2 * u + 1] + l3[2 * t + 1][2 * u] + l3[2 * t + 1][2 * u + 1] + 2) >> 2;
+    if (x4 < w4 && y4 < h4) j.dst[p][3][(int64_t)y4 * w4 + x4] = (uint16_t)v4;
+  }
 }
 
 }  // namespace
+
+hipError_t launch_pyramid(const PyramidJobs& j, hipStream_t stream) {
+  if (j.n <= 0 || j.n > 6 || (j.w >> 1) <= 0 || (j.h >> 1) <= 0) return j.n > 6 ? hipErrorInvalidValue : hipSuccess;
+  const dim3 grid((unsigned)(((j.w >> 1) + 15) / 16), (unsigned)(((j.h >> 1) + 15) / 16), (unsigned)j.n);
+  hipLaunchKernelGGL(k_pyramid, grid, dim3(256), 0, stream, j);
+  return hipGetLastError();
+}
 
 hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream) {
   for (int i = 0; i < 100; i++)  // the kernel holds only the 21 distinct weights of a symmetric window
@@ -517,12 +559,5 @@ hipError_t launch_ssim_rows(const SsimRowsJobs& j, hipStream_t stream) {
 
 int32_t ssim_tiles_x(int32_t w) { return w > 10 ? (w - 10 + kSsimTW - 1) / kSsimTW : 0; }
 
-hipError_t launch_downsample(const DownsampleJobs& j, hipStream_t stream) {
-  const int64_t n = (int64_t)j.dw * j.dh;
-  if (n <= 0 || j.n <= 0) return hipSuccess;
-  if (j.n > 6) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_downsample, dim3((unsigned)((n + 255) / 256), 1, (unsigned)j.n), dim3(256), 0, stream, j);
-  return hipGetLastError();
-}
 
 }  // namespace qmiv

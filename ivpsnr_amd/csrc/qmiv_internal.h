@@ -245,13 +245,15 @@ hipError_t launch_ssim_rows(const SsimRowsJobs& j, hipStream_t stream);
 
 // One MS-SSIM pyramid level for up to 6 planes in one launch:
 // dst_i[y][x] = (s(2x, 2y) + s(2x + 1, 2y) + s(2x, 2y + 1) + s(2x + 1, 2y + 1) + 2) >> 2.
-struct DownsampleJobs {
+// MS-SSIM levels 1-4 of up to six planes in one launch: dst[p][l - 1] is level l, (w >> l) x (h >> l)
+// uint16 (stride w >> l), of the 4:4:4 view src[p] of size w x h.
+struct PyramidJobs {
   int32_t n;
-  int32_t dw, dh;
+  int32_t w, h;
   PlaneView src[6];
-  uint16_t* dst[6];
+  uint16_t* dst[6][4];
 };
-hipError_t launch_downsample(const DownsampleJobs& j, hipStream_t stream);
+hipError_t launch_pyramid(const PyramidJobs& j, hipStream_t stream);
 
 // Tile shape of the fast CPS kernel (device code only; the host asks cps_fast_tile()).
 constexpr int kCpsTileW = 64;   // 64 lanes x 1 column
