@@ -1426,8 +1426,7 @@ qmiv_status qmiv_calc_ivpsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
   apply_row_semantics(rows.data(), rows.size(), prm.cps_int32_rows != 0);
   static const std::vector<double> kNoWs;
   const qmiv_ctx::WsTable* wst = prm.erp ? &ws_table(ctx, g.h, prm.lat_range_deg) : nullptr;
-  const std::vector<double>& ws = wst ? wst->w : kNoWs;
-  const double corr = wst ? wst->corr : 1.0;
+  const std::vector<double>& ws = wst ? wst->w : kNoWs;  // IV-PSNR's WS sums take no H / sum(w) correction
   const double* wsp = prm.erp ? ws.data() : nullptr;
   const int32_t area = g.w * g.h;
   const double q0 = fin::qual_from_rows(rows.data(), g.h, wsp, area, g.bd, prm.cmp_weights_average);
@@ -2215,8 +2214,7 @@ qmiv_status qmiv_calc_ivpsnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qm
   const int32_t nnm = nnm_arg >= 0 ? nnm_arg : sum->nnm;
   static const std::vector<double> kNoWs;
   const qmiv_ctx::WsTable* wst = prm.erp ? &ws_table(ctx, g.h, prm.lat_range_deg) : nullptr;
-  const std::vector<double>& ws = wst ? wst->w : kNoWs;
-  const double corr = wst ? wst->corr : 1.0;
+  const std::vector<double>& ws = wst ? wst->w : kNoWs;  // IV-PSNR's WS sums take no H / sum(w) correction
   const uint64_t* rows = (const uint64_t*)((const char*)ctx->h_results + L.off_cps);
   const double* w = prm.erp ? ws.data() : nullptr;
   const double a = fin::qual_from_rows_masked(rows, g.h, w, nnm, g.bd, mask->g.bd, prm.cmp_weights_average);
