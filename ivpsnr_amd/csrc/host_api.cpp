@@ -1964,7 +1964,8 @@ struct MaskedLayout {
   int32_t nseg = 0;
 };
 
-MaskedLayout masked_layout(const Geom& g, bool cps) {
+// nseg: CPS partial segments per row (of the kernel that runs the search).
+MaskedLayout masked_layout(const Geom& g, bool cps, int32_t nseg) {
   MaskedLayout L;
   const size_t h = (size_t)g.h;
   size_t o = 0;
@@ -1983,7 +1984,7 @@ MaskedLayout masked_layout(const Geom& g, bool cps) {
   o = align_up(o + 3 * h * sizeof(uint32_t), 256);
   L.off_diff = o;
   o = align_up(o + h * sizeof(uint32_t), 256);
-  L.nseg = masked_cps_segments(g);
+  L.nseg = nseg;
   L.off_part = o;
   o = align_up(o + (cps ? (size_t)L.nseg * 6 * h * sizeof(uint64_t) : 0), 256);
   L.total = o;
@@ -1998,6 +1999,21 @@ qmiv_status check_mask(const qmiv_pic* p0, const qmiv_pic* mask) {
   return QMIV_OK;
 }
 
+// The masked search on the fast 4:2:0 kernel: besides the unmasked kernel's conditions, a masked
+// candidate's record must lose to every unmasked one, which the kernel ensures when every real key
+// and every chroma term T stays below 2^29 in magnitude. gcd_bound: |GCD| per component, at most.
+bool fast_masked_cps_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) {
+  if (g.cs != 1 || !fast_cps_ok(g, p, gcd_bound)) return false;
+  const int64_t maxv = (1 << g.bd) - 1;
+  int64_t m = 0;
+  for (int c = 0; c < 3; c++) m = std::max<int64_t>(m, maxv + std::abs((int64_t)gcd_bound[c]));
+  const int64_t w0 = p.cmp_weights_search[0], w1 = p.cmp_weights_search[1], w2 = p.cmp_weights_search[2];
+  // key = 32 sum_c w_c Y_c (Y_c - 2 X_c) + h, with Y in [0, maxv] and |X| <= m: each term lies in
+  // [-m^2, m^2]; T = -64 (w1 X1 U + w2 X2 V).
+  const int64_t lim = (int64_t)1 << 29;
+  return 32 * (w0 + w1 + w2) * m * m + 32 < lim && 64 * (w1 + w2) * m * maxv < lim;
+}
+
 // Masked statistics (and, with cps, the masked search of both directions) of P0 / P1 under the
 // mask; results at the start of ctx->h_results. gcd_given: null to compute the masked GCD.
 qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
@@ -2006,7 +2022,20 @@ qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, co
   const Geom& g = p0->g;
   if (cps && (prm.search_range < 0 || prm.search_range > kMaxMaskedSearchRange))
     return fail(QMIV_ERR_UNSUPPORTED, "masked search supports search ranges 0..12");
-  L = masked_layout(g, cps);
+  // The computed GCD is clamped to the unnoticeable-difference bound, and its fourth entry is 0.
+  int32_t md[4], gbound[4];
+  for (int c = 0; c < 4; c++) md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
+  for (int c = 0; c < 4; c++) gbound[c] = gcd_given ? gcd_given[c] : (c < 3 ? md[c] : 0);
+  const bool fast = cps && prm.search_range == 2 && fast_masked_cps_ok(g, prm, gbound);
+  CpsArgs fa{};
+  if (fast) {
+    int32_t tw, th;
+    cps_tile(true, &tw, &th);
+    fa.tiles_x = (g.w + tw - 1) / tw;
+    fa.sweep = pick_sweep(ctx, g, true, gbound);
+    fa.nseg = (fa.tiles_x + fa.sweep - 1) / fa.sweep;
+  }
+  L = masked_layout(g, cps, fast ? fa.nseg : masked_cps_segments(g));
   QMIV_HIP(hipSetDevice(ctx->device));
   QMIV_TRY(ensure_scratch(ctx, L.total));
   QMIV_TRY(ensure_results(ctx, L.results_bytes));
@@ -2018,8 +2047,6 @@ qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, co
   r.max = (uint32_t*)(d + L.off_max);
   r.diff = (uint32_t*)(d + L.off_diff);
   MaskSummary* sum = (MaskSummary*)(d + L.off_sum);
-  int32_t md[4];
-  for (int c = 0; c < 4; c++) md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
   hipStream_t st = ctx->stream;
   {
     KernelTimer t(ctx, QMIV_K_STATS, st);
@@ -2033,8 +2060,23 @@ qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, co
     uint64_t* part = (uint64_t*)(d + L.off_part);
     {
       KernelTimer t(ctx, QMIV_K_CPS, st);
-      QMIV_HIP(launch_cps_masked(p0->dev, p1->dev, g, mask->dev, mask->g, sum, prm.search_range,
-                                 prm.cmp_weights_search, part, st));
+      if (fast) {
+        fa.p0 = p0->dev;
+        fa.p1 = p1->dev;
+        fa.g = g;
+        for (int c = 0; c < 4; c++) fa.ws[c] = prm.cmp_weights_search[c];
+        fa.B = prm.search_range;
+        fa.partials = part;
+        fa.maxv = (1 << g.bd) - 1;
+        fa.mask = mask->dev;
+        fa.mask_bytes = (uint32_t)((size_t)g.w * g.h * mask->g.bytes_per_sample);  // its luma plane
+        fa.mask_bytes_per_sample = mask->g.bytes_per_sample;
+        fa.msum = sum;
+        QMIV_HIP(launch_cps_fast420(fa, st));
+      } else {
+        QMIV_HIP(launch_cps_masked(p0->dev, p1->dev, g, mask->dev, mask->g, sum, prm.search_range,
+                                   prm.cmp_weights_search, part, st));
+      }
     }
     {
       KernelTimer t(ctx, QMIV_K_CPS_ROWS, st);

@@ -22,6 +22,7 @@
 // holds the searched picture's luma records {Y0, C} for rows [y0 - 2, y0 + 10) x columns
 // [x0 - 4, x0 + 132) and its chroma cells {U | V << 16}.
 #include <climits>
+#include <type_traits>
 
 #include "device_common.h"
 #include "qmiv_internal.h"
@@ -57,10 +58,21 @@ constexpr int kFillSteps = (kFillItems + 255) / 256;
 //   8-bit samples:  l0 = Y0 | Y1 << 8 | Y2 << 16 | Y3 << 24, u = U0 | U1 << 8, v = V0 | V1 << 8
 struct FillRegs {
   uint32_t l0[kFillSteps], l1[kFillSteps], u[kFillSteps], v[kFillSteps];
+  uint32_t m0[kFillSteps], m1[kFillSteps];  // masked search: the quad's mask samples, as loaded
 };
 
-template <typename T>
-__device__ __forceinline__ void fill420_load(const Buf<T>& Yp, const Geom& g, int x0, int y0, FillRegs& fr) {
+// Masked search: a searched pel whose mask is 0 gets the record {Y0 = 0, C = kMaskedC + (h mod 32)}
+// (h: its candidate index, so a winner still decodes to a window position). Its keys are then
+// kMaskedC + (h mod 32) + T(P, Q) > 2^29 while every real key lies in (-2^29, 2^29) and
+// |T| < 2^29 (the host checks both bounds), so it never beats an unmasked candidate. A tested pel
+// whose mask is 0 contributes nothing; any other has itself as an unmasked candidate, so its
+// winner is the first strict minimum over the unmasked ones, as in
+// xCorrespPixelShiftSTD::FindBestPixelWithinBlockM (libs/pmbb-ivqm/src/xCorrespPixelShiftSTD.cpp).
+constexpr int32_t kMaskedC = 1 << 30;
+
+template <typename T, typename TM, bool MASKED>
+__device__ __forceinline__ void fill420_load(const Buf<T>& Yp, const Buf<TM>& Mp, const Geom& g, int x0, int y0,
+                                             FillRegs& fr) {
   const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
   const bool interior = x0 >= 4 && x0 + kTW + 4 <= g.w && y0 >= 2 && y0 + kTH + 2 <= g.h && (g.w & 3) == 0;
   constexpr uint32_t sh = 8 * sizeof(T);
@@ -87,6 +99,15 @@ __device__ __forceinline__ void fill420_load(const Buf<T>& Yp, const Geom& g, in
         fr.u[k] = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cb_off + co, 0, 0);
         fr.v[k] = __builtin_amdgcn_raw_buffer_load_b16(Yp.r, cr_off + co, 0, 0);
       }
+      if constexpr (MASKED) {
+        if constexpr (sizeof(TM) == 2) {
+          const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(Mp.r, lo * 2u, 0, 0));
+          fr.m0[k] = w.x;
+          fr.m1[k] = w.y;
+        } else {
+          fr.m0[k] = __builtin_amdgcn_raw_buffer_load_b32(Mp.r, lo, 0, 0);
+        }
+      }
     } else {
       const int gy = clampi(y0 - 2 + ly, 0, g.h - 1);
       const uint32_t rowo = __umul24((uint32_t)gy, (uint32_t)g.w);
@@ -109,12 +130,23 @@ __device__ __forceinline__ void fill420_load(const Buf<T>& Yp, const Geom& g, in
       }
       fr.u[k] = uq[0] | (uq[1] << sh);
       fr.v[k] = vq[0] | (vq[1] << sh);
+      if constexpr (MASKED) {
+        uint32_t mq[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) mq[m] = Mp.ld(rowo + (uint32_t)clampi(x0 - 4 + lx + m, 0, g.w - 1));
+        if constexpr (sizeof(TM) == 2) {
+          fr.m0[k] = mq[0] | (mq[1] << 16);
+          fr.m1[k] = mq[2] | (mq[3] << 16);
+        } else {
+          fr.m0[k] = mq[0] | (mq[1] << 8) | (mq[2] << 16) | (mq[3] << 24);
+        }
+      }
     }
   }
 }
 
 // Builds the LDS records {Y0, C} and chroma cells of one fill from the loaded words.
-template <typename T>
+template <typename T, typename TM, bool MASKED>
 __device__ __forceinline__ void fill420_store(const FillRegs& fr, int32_t w0, int32_t w1, int32_t w2, uint2* s_luma,
                                               uint32_t* s_cell) {
   constexpr uint32_t sh = 8 * sizeof(T);
@@ -144,6 +176,20 @@ __device__ __forceinline__ void fill420_store(const FillRegs& fr, int32_t w0, in
     r01.z = yv[1]; r01.w = (uint32_t)(32 * ((int32_t)__umul24((uint32_t)w0, __umul24(yv[1], yv[1])) + sc0) + h + 1);
     r23.x = yv[2]; r23.y = (uint32_t)(32 * ((int32_t)__umul24((uint32_t)w0, __umul24(yv[2], yv[2])) + sc1) + h + 2);
     r23.z = yv[3]; r23.w = (uint32_t)(32 * ((int32_t)__umul24((uint32_t)w0, __umul24(yv[3], yv[3])) + sc1) + h + 3);
+    if constexpr (MASKED) {
+      bool on[4];
+      if constexpr (sizeof(TM) == 2) {
+        on[0] = (fr.m0[k] & 0xffffu) != 0; on[1] = (fr.m0[k] >> 16) != 0;
+        on[2] = (fr.m1[k] & 0xffffu) != 0; on[3] = (fr.m1[k] >> 16) != 0;
+      } else {
+        on[0] = (fr.m0[k] & 0xffu) != 0; on[1] = (fr.m0[k] & 0xff00u) != 0;
+        on[2] = (fr.m0[k] & 0xff0000u) != 0; on[3] = (fr.m0[k] >> 24) != 0;
+      }
+      if (!on[0]) { r01.x = 0u; r01.y = (uint32_t)kMaskedC | (r01.y & 31u); }
+      if (!on[1]) { r01.z = 0u; r01.w = (uint32_t)kMaskedC | (r01.w & 31u); }
+      if (!on[2]) { r23.x = 0u; r23.y = (uint32_t)kMaskedC | (r23.y & 31u); }
+      if (!on[3]) { r23.z = 0u; r23.w = (uint32_t)kMaskedC | (r23.w & 31u); }
+    }
     uint4* dst = reinterpret_cast<uint4*>(s_luma + ly * kLW + lx);
     dst[0] = r01;
     dst[1] = r23;
@@ -160,11 +206,12 @@ __device__ __forceinline__ void fill420_store(const FillRegs& fr, int32_t w0, in
 // and its chroma cell's Cb and Cr samples. Loaded a phase ahead, like FillRegs.
 struct XRegs {
   uint32_t row[kKB][2], cb[kKB], cr[kKB];
+  uint32_t mrow[kKB][2];  // masked search: the block's mask samples, two per row
 };
 
-template <typename T>
-__device__ __forceinline__ void x420_load(const Buf<T>& Xp, const Geom& g, int x0, int y0, int lane, int wave,
-                                          XRegs& xr) {
+template <typename T, typename TM, bool MASKED>
+__device__ __forceinline__ void x420_load(const Buf<T>& Xp, const Buf<TM>& Mp, const Geom& g, int x0, int y0, int lane,
+                                          int wave, XRegs& xr) {
   const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
   const int bx = min(x0 + 2 * lane, g.w - 2);  // block origin (even), clamped for the tile's tail
 #pragma unroll
@@ -179,6 +226,10 @@ __device__ __forceinline__ void x420_load(const Buf<T>& Xp, const Geom& g, int x
       const uint32_t o = __umul24((uint32_t)(by + py), (uint32_t)g.w) + (uint32_t)bx;
       if constexpr (sizeof(T) == 2) xr.row[b][py] = __builtin_amdgcn_raw_buffer_load_b32(Xp.r, o * 2u, 0, 0);
       else xr.row[b][py] = __builtin_amdgcn_raw_buffer_load_b16(Xp.r, o, 0, 0);
+      if constexpr (MASKED) {
+        if constexpr (sizeof(TM) == 2) xr.mrow[b][py] = __builtin_amdgcn_raw_buffer_load_b32(Mp.r, o * 2u, 0, 0);
+        else xr.mrow[b][py] = __builtin_amdgcn_raw_buffer_load_b16(Mp.r, o, 0, 0);
+      }
     }
   }
 }
@@ -189,10 +240,13 @@ __device__ __forceinline__ void x420_load(const Buf<T>& Xp, const Geom& g, int x
 // consumed the loads of the next phase's (fill of picture next_fill, tested pels of next_x, tile
 // column next_x0, when has_next) are issued into them, so their latency hides behind this phase's
 // search.
-template <typename T, int DIR, int OUT>
+template <bool MASKED>
+using AccT = typename std::conditional<MASKED, uint64_t, uint32_t>::type;
+
+template <typename T, typename TM, bool MASKED, int DIR, int OUT>
 __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int x0, int y0, int lane, int wave,
                                             const int32_t (&gcd)[3], uint2* s_luma, uint32_t* s_cell,
-                                            uint32_t (&acc)[2 * kKB][3], FillRegs& fr, XRegs& xr,
+                                            AccT<MASKED> (&acc)[2 * kKB][3], FillRegs& fr, XRegs& xr,
                                             const void* next_fill, const void* next_x, int next_x0, bool has_next) {
   // Opaque copies: keeps per-row address arithmetic from being hoisted out of the tile sweep and
   // held in registers across it.
@@ -209,6 +263,7 @@ __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int
   constexpr uint32_t smask = sizeof(T) == 2 ? 0xffffu : 0xffu;
   constexpr uint32_t ssh = 8 * sizeof(T);
   int32_t x0v[kKB][2][2], a0[kKB][2][2], x1v[kKB], x2v[kKB], a1[kKB], a2[kKB];
+  uint32_t mv[kKB][2][2];  // masked search: the tested pels' mask values
 #pragma unroll
   for (int b = 0; b < kKB; b++) {
     x1v[b] = (int32_t)xr.cb[b];
@@ -217,15 +272,22 @@ __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int
     for (int py = 0; py < 2; py++) {
       x0v[b][py][0] = (int32_t)(xr.row[b][py] & smask);
       x0v[b][py][1] = (int32_t)((xr.row[b][py] >> ssh) & smask);
+      if constexpr (MASKED) {
+        constexpr uint32_t mmask = sizeof(TM) == 2 ? 0xffffu : 0xffu;
+        constexpr uint32_t msh = 8 * sizeof(TM);
+        mv[b][py][0] = xr.mrow[b][py] & mmask;
+        mv[b][py][1] = (xr.mrow[b][py] >> msh) & mmask;
+      }
     }
   }
 
   __syncthreads();
-  fill420_store<T>(fr, w0, w1, w2, s_luma, s_cell);
+  fill420_store<T, TM, MASKED>(fr, w0, w1, w2, s_luma, s_cell);
   __syncthreads();
   if (has_next) {
-    fill420_load<T>(Buf<T>(next_fill, g.frame_bytes), g, next_x0, y0, fr);
-    x420_load<T>(Buf<T>(next_x, g.frame_bytes), g, next_x0, y0, lane, wave, xr);
+    const Buf<TM> mbuf(a.mask, a.mask_bytes);
+    fill420_load<T, TM, MASKED>(Buf<T>(next_fill, g.frame_bytes), mbuf, g, next_x0, y0, fr);
+    x420_load<T, TM, MASKED>(Buf<T>(next_x, g.frame_bytes), mbuf, g, next_x0, y0, lane, wave, xr);
   }
 
   // The key coefficients a_c = -64 w_c X_c.
@@ -340,10 +402,19 @@ __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int
         const int y = y0 + 2 * (kKB * wave + b) + py;
         const int x = x0 + 2 * lane + px;
         const bool inside = full || (x < g.w && y < g.h);
-        const uint32_t keep = inside ? 0xffffffffu : 0u;
-        acc[2 * b + py][0] += (uint32_t)__mul24(d0, d0) & keep;
-        acc[2 * b + py][1] += (uint32_t)__mul24(d1, d1) & keep;
-        acc[2 * b + py][2] += (uint32_t)__mul24(d2, d2) & keep;
+        if constexpr (MASKED) {
+          // xCorrespPixelShiftSTD::CalcDistAsymmetricRowM: the winner's squared errors (int32,
+          // sign-extended) times the tested pel's mask value; masked pels add nothing.
+          const uint64_t m = inside ? (uint64_t)mv[b][py][px] : 0u;
+          acc[2 * b + py][0] += (uint64_t)(uint32_t)__mul24(d0, d0) * m;
+          acc[2 * b + py][1] += (uint64_t)(uint32_t)__mul24(d1, d1) * m;
+          acc[2 * b + py][2] += (uint64_t)(uint32_t)__mul24(d2, d2) * m;
+        } else {
+          const uint32_t keep = inside ? 0xffffffffu : 0u;
+          acc[2 * b + py][0] += (uint32_t)__mul24(d0, d0) & keep;
+          acc[2 * b + py][1] += (uint32_t)__mul24(d1, d1) & keep;
+          acc[2 * b + py][2] += (uint32_t)__mul24(d2, d2) & keep;
+        }
         if constexpr ((OUT & 3) != 0) {
           if (inside) {
             const uint32_t o = __umul24((uint32_t)y, (uint32_t)g.w) + (uint32_t)x;
@@ -360,13 +431,20 @@ __device__ __forceinline__ void tile_dir420(const CpsArgs& a, const Geom& g, int
       }
 }
 
-template <typename T, int OUT>
+template <typename T, typename TM, bool MASKED, int OUT>
 __global__ __launch_bounds__(256) void k_cps_fast420(CpsArgs a) {
   __shared__ __attribute__((aligned(16))) uint2 s_luma[kLH * kLW];
   __shared__ __attribute__((aligned(16))) uint32_t s_cell[kCH * kCW];
 
-  const FrameSummary* __restrict__ sum = a.sum;
-  if (!sum->in_range) return;
+  // Nothing to do when a pel exceeds the bit depth: the host reports that error.
+  const int32_t* gsrc;
+  if constexpr (MASKED) {
+    if (!a.msum->all_in_range) return;
+    gsrc = a.msum->gcd;
+  } else {
+    if (!a.sum->in_range) return;
+    gsrc = a.sum->gcd;
+  }
   const Geom g = a.g;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -382,10 +460,11 @@ __global__ __launch_bounds__(256) void k_cps_fast420(CpsArgs a) {
   const int y0 = band * kTH;
   const int tx_begin = seg * a.sweep;
   const int tx_end = min(a.tiles_x, tx_begin + a.sweep);
-  const int32_t gcd[3] = {sum->gcd[0], sum->gcd[1], sum->gcd[2]};
+  const int32_t gcd[3] = {gsrc[0], gsrc[1], gsrc[2]};
 
-  // Per-lane row sums over the sweep (the host bounds sweep * 2 * max squared error below 2^32).
-  uint32_t acc0[2 * kKB][3], acc1[2 * kKB][3];
+  // Per-lane row sums over the sweep (the host bounds sweep * 2 * max squared error below 2^32;
+  // masked sums, each square times a mask value of up to 16 bits, are kept in 64 bits).
+  AccT<MASKED> acc0[2 * kKB][3], acc1[2 * kKB][3];
 #pragma unroll
   for (int r = 0; r < 2 * kKB; r++)
 #pragma unroll
@@ -396,14 +475,16 @@ __global__ __launch_bounds__(256) void k_cps_fast420(CpsArgs a) {
   FillRegs fr;
   XRegs xr;
   if (tx_begin < tx_end) {
-    fill420_load<T>(Buf<T>(a.p1, g.frame_bytes), g, tx_begin * kTW, y0, fr);
-    x420_load<T>(Buf<T>(a.p0, g.frame_bytes), g, tx_begin * kTW, y0, lane, wave, xr);
+    const Buf<TM> mbuf(a.mask, a.mask_bytes);
+    fill420_load<T, TM, MASKED>(Buf<T>(a.p1, g.frame_bytes), mbuf, g, tx_begin * kTW, y0, fr);
+    x420_load<T, TM, MASKED>(Buf<T>(a.p0, g.frame_bytes), mbuf, g, tx_begin * kTW, y0, lane, wave, xr);
   }
   for (int tx = tx_begin; tx < tx_end; tx++) {
     const int x0 = tx * kTW;
-    tile_dir420<T, 0, OUT>(a, g, x0, y0, lane, wave, gcd, s_luma, s_cell, acc0, fr, xr, a.p0, a.p1, x0, true);
-    tile_dir420<T, 1, OUT>(a, g, x0, y0, lane, wave, gcd, s_luma, s_cell, acc1, fr, xr, a.p1, a.p0, x0 + kTW,
-                           tx + 1 < tx_end);
+    tile_dir420<T, TM, MASKED, 0, OUT>(a, g, x0, y0, lane, wave, gcd, s_luma, s_cell, acc0, fr, xr, a.p0, a.p1, x0,
+                                       true);
+    tile_dir420<T, TM, MASKED, 1, OUT>(a, g, x0, y0, lane, wave, gcd, s_luma, s_cell, acc1, fr, xr, a.p1, a.p0,
+                                       x0 + kTW, tx + 1 < tx_end);
   }
 
   // One partial per (segment, direction, component, row): reduce over the wave's 64 lanes.
@@ -414,9 +495,12 @@ __global__ __launch_bounds__(256) void k_cps_fast420(CpsArgs a) {
     for (int c = 0; c < 3; c++) {
       const uint64_t v0 = wave_sum((uint64_t)acc0[r][c]);
       const uint64_t v1 = wave_sum((uint64_t)acc1[r][c]);
+      // The masked processor's R2T is the search of P1 - g in P0 (xIVPSNR.cpp:152-165): the
+      // second direction here.
+      const int d0 = MASKED ? 1 : 0, d1 = MASKED ? 0 : 1;
       if (lane == 0 && y < g.h) {
-        a.partials[(((int64_t)seg * 2 + 0) * 3 + c) * g.h + y] = v0;
-        a.partials[(((int64_t)seg * 2 + 1) * 3 + c) * g.h + y] = v1;
+        a.partials[(((int64_t)seg * 2 + d0) * 3 + c) * g.h + y] = v0;
+        a.partials[(((int64_t)seg * 2 + d1) * 3 + c) * g.h + y] = v1;
       }
     }
   }
@@ -433,19 +517,31 @@ hipError_t launch_cps_fast420(const CpsArgs& a, hipStream_t stream) {
   if ((a.scp0 == nullptr) != (a.scp1 == nullptr) || (a.best0 == nullptr) != (a.best1 == nullptr))
     return hipErrorInvalidValue;
   const int out = (a.scp0 ? 1 : 0) | (a.best0 ? 2 : 0);
+  if (a.mask) {  // masked search: no optional outputs
+    if (out != 0 || !a.msum || (a.mask_bytes_per_sample != 1 && a.mask_bytes_per_sample != 2))
+      return hipErrorInvalidValue;
+    if (a.g.bytes_per_sample == 1) {
+      if (a.mask_bytes_per_sample == 1) k_cps_fast420<uint8_t, uint8_t, true, 0><<<grid, block, 0, stream>>>(a);
+      else k_cps_fast420<uint8_t, uint16_t, true, 0><<<grid, block, 0, stream>>>(a);
+    } else {
+      if (a.mask_bytes_per_sample == 1) k_cps_fast420<uint16_t, uint8_t, true, 0><<<grid, block, 0, stream>>>(a);
+      else k_cps_fast420<uint16_t, uint16_t, true, 0><<<grid, block, 0, stream>>>(a);
+    }
+    return hipGetLastError();
+  }
   if (a.g.bytes_per_sample == 1) {
     switch (out) {
-      case 0: k_cps_fast420<uint8_t, 0><<<grid, block, 0, stream>>>(a); break;
-      case 1: k_cps_fast420<uint8_t, 1><<<grid, block, 0, stream>>>(a); break;
-      case 2: k_cps_fast420<uint8_t, 2><<<grid, block, 0, stream>>>(a); break;
-      default: k_cps_fast420<uint8_t, 3><<<grid, block, 0, stream>>>(a); break;
+      case 0: k_cps_fast420<uint8_t, uint8_t, false, 0><<<grid, block, 0, stream>>>(a); break;
+      case 1: k_cps_fast420<uint8_t, uint8_t, false, 1><<<grid, block, 0, stream>>>(a); break;
+      case 2: k_cps_fast420<uint8_t, uint8_t, false, 2><<<grid, block, 0, stream>>>(a); break;
+      default: k_cps_fast420<uint8_t, uint8_t, false, 3><<<grid, block, 0, stream>>>(a); break;
     }
   } else {
     switch (out) {
-      case 0: k_cps_fast420<uint16_t, 0><<<grid, block, 0, stream>>>(a); break;
-      case 1: k_cps_fast420<uint16_t, 1><<<grid, block, 0, stream>>>(a); break;
-      case 2: k_cps_fast420<uint16_t, 2><<<grid, block, 0, stream>>>(a); break;
-      default: k_cps_fast420<uint16_t, 3><<<grid, block, 0, stream>>>(a); break;
+      case 0: k_cps_fast420<uint16_t, uint8_t, false, 0><<<grid, block, 0, stream>>>(a); break;
+      case 1: k_cps_fast420<uint16_t, uint8_t, false, 1><<<grid, block, 0, stream>>>(a); break;
+      case 2: k_cps_fast420<uint16_t, uint8_t, false, 2><<<grid, block, 0, stream>>>(a); break;
+      default: k_cps_fast420<uint16_t, uint8_t, false, 3><<<grid, block, 0, stream>>>(a); break;
     }
   }
   return hipGetLastError();
@@ -459,8 +555,8 @@ void cps_fast420_tile(int32_t* tile_w, int32_t* tile_h) {
 int cps_fast420_blocks_per_cu(int bytes_per_sample) {
   int n = 0;
   hipError_t e = bytes_per_sample == 1
-                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast420<uint8_t, 0>, 256, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast420<uint16_t, 0>, 256, 0);
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast420<uint8_t, uint8_t, false, 0>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast420<uint16_t, uint8_t, false, 0>, 256, 0);
   return e == hipSuccess && n > 0 ? n : 1;
 }
 

@@ -81,6 +81,141 @@ __global__ __launch_bounds__(256) void k_mstats(const T* __restrict__ p0, const 
   }
 }
 
+// Buffer offset past every frame (frames are below 2 GiB): a raw buffer load there returns 0.
+constexpr uint32_t kOob = 0x80000000u;
+
+// Four adjacent samples from sample index i (i * sizeof(T) aligned to 4 * sizeof(T)); zeros when !ok.
+template <typename T>
+__device__ __forceinline__ void ld4(const Buf<T>& b, uint32_t i, bool ok, uint32_t (&v)[4]) {
+  const uint32_t off = ok ? i * (uint32_t)sizeof(T) : kOob;
+  if constexpr (sizeof(T) == 2) {
+    const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(b.r, off, 0, 0));
+    v[0] = w.x & 0xffffu; v[1] = w.x >> 16; v[2] = w.y & 0xffffu; v[3] = w.y >> 16;
+  } else {
+    const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(b.r, off, 0, 0);
+    v[0] = w & 0xffu; v[1] = (w >> 8) & 0xffu; v[2] = (w >> 16) & 0xffu; v[3] = w >> 24;
+  }
+}
+
+// Two adjacent samples from sample index i (aligned to 2 * sizeof(T)); zeros when !ok.
+template <typename T>
+__device__ __forceinline__ void ld2(const Buf<T>& b, uint32_t i, bool ok, uint32_t (&v)[4]) {
+  const uint32_t off = ok ? i * (uint32_t)sizeof(T) : kOob;
+  if constexpr (sizeof(T) == 2) {
+    const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(b.r, off, 0, 0);
+    v[0] = w & 0xffffu; v[1] = w >> 16;
+  } else {
+    const uint32_t w = __builtin_amdgcn_raw_buffer_load_b16(b.r, off, 0, 0);
+    v[0] = w & 0xffu; v[1] = w >> 8;
+  }
+  v[2] = v[3] = 0u;
+}
+
+// Chroma samples of four adjacent pels from x (x divisible by 4): two for 4:2:0, four for 4:4:4.
+template <typename T, int CS>
+__device__ __forceinline__ void ldc(const Buf<T>& b, uint32_t i, bool ok, uint32_t (&v)[4]) {
+  if constexpr (CS == 1) ld2<T>(b, i, ok, v);
+  else ld4<T>(b, i, ok, v);
+}
+
+// Block totals of the eleven per-pel statistics over a 256-lane workgroup: q 0 counted pels,
+// 1..3 SD, 4..6 SSD (64-bit sums; two's complement, so one adder serves the signed SD), 7..9 maxima
+// of P0, P1 and the mask, 10 the OR of the difference flags. The lanes' values are transposed
+// through LDS and each quantity is reduced by 16 lanes (16 values each, conflict-free reads), then
+// across those 16. tot[q] is valid in every lane on return.
+constexpr int kStatQ = 11;
+__device__ __forceinline__ uint64_t stat_comb(int q, uint64_t a, uint64_t b) {
+  return q < 7 ? a + b : (q < 10 ? max(a, b) : (a | b));
+}
+__device__ __forceinline__ void block_stats(const uint64_t (&v)[kStatQ], uint64_t (*s)[256], uint64_t* tot) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < kStatQ; q++) s[q][t] = v[q];
+  __syncthreads();
+  if (t < kStatQ * 16) {
+    const int q = t >> 4, p = t & 15;
+    uint64_t acc = s[q][p];
+#pragma unroll
+    for (int i = 1; i < 16; i++) acc = stat_comb(q, acc, s[q][16 * i + p]);
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) acc = stat_comb(q, acc, __shfl_xor(acc, o));
+    if (p == 0) tot[q] = acc;
+  }
+  __syncthreads();
+}
+
+// k_mstats for widths divisible by 4, the same sums in the same integer types: one workgroup per
+// row, each lane four adjacent pels per step (one load per plane and picture, 1024 pels per step),
+// two steps in flight per pass. Lanes past the row read zeros (the buffer's bounds check), which
+// add nothing: mask 0, equal pels, no larger maximum. grid = h.
+template <typename T, typename TM, int CS, int CSM>
+__global__ __launch_bounds__(256) void k_mstats_v(const T* __restrict__ p0, const T* __restrict__ p1, Geom g,
+                                                  const TM* __restrict__ mask, Geom gm, MaskRows r) {
+  __shared__ uint64_t s_red[kStatQ][256];
+  __shared__ uint64_t s_tot[kStatQ];
+  const int y = blockIdx.x;
+  const Buf<T> b0(p0, g.frame_bytes), b1(p1, g.frame_bytes);
+  const Buf<TM> bm(mask, gm.frame_bytes);
+  const uint32_t row = (uint32_t)y * (uint32_t)g.w;
+  const uint32_t crow = (uint32_t)(y >> CS) * (uint32_t)g.cw;
+  const uint32_t mrow = (uint32_t)(y >> CSM) * (uint32_t)gm.cw;
+  const uint32_t cb = (uint32_t)g.cb_off, cr = (uint32_t)g.cr_off;
+  const uint32_t mcb = (uint32_t)gm.cb_off, mcr = (uint32_t)gm.cr_off;
+  int32_t nnz = 0;
+  int64_t sd[3] = {0, 0, 0};
+  uint64_t ssd[3] = {0, 0, 0};
+  uint32_t mx0 = 0, mx1 = 0, mxm = 0, diff = 0;
+  for (int base = 0; base < g.w; base += 2048) {
+    uint32_t m[2][4], mc[2][2][4], a[2][3][4], b[2][3][4];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const uint32_t x = (uint32_t)(base + 1024 * s + 4 * (int)threadIdx.x);
+      const bool ok = x < (uint32_t)g.w;
+      const uint32_t cx = x >> CS, mcx = x >> CSM;
+      ld4<TM>(bm, row + x, ok, m[s]);
+      ldc<TM, CSM>(bm, mcb + mrow + mcx, ok, mc[s][0]);
+      ldc<TM, CSM>(bm, mcr + mrow + mcx, ok, mc[s][1]);
+      ld4<T>(b0, row + x, ok, a[s][0]);
+      ldc<T, CS>(b0, cb + crow + cx, ok, a[s][1]);
+      ldc<T, CS>(b0, cr + crow + cx, ok, a[s][2]);
+      ld4<T>(b1, row + x, ok, b[s][0]);
+      ldc<T, CS>(b1, cb + crow + cx, ok, b[s][1]);
+      ldc<T, CS>(b1, cr + crow + cx, ok, b[s][2]);
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t mk = m[s][k];
+        mxm = max(mxm, max(mk, max(mc[s][0][k >> CSM], mc[s][1][k >> CSM])));
+        nnz += mk != 0u;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const int kc = c == 0 ? k : (k >> CS);
+          const uint32_t av = a[s][c][kc], bv = b[s][c][kc];
+          mx0 = max(mx0, av);
+          mx1 = max(mx1, bv);
+          const uint32_t d = bv - av;
+          diff |= (d != 0u) << c;
+          sd[c] += (int64_t)(int32_t)(d * mk);
+          ssd[c] += (uint64_t)(int64_t)(int32_t)(d * d) * (uint64_t)mk;
+        }
+      }
+  }
+  const uint64_t v[kStatQ] = {(uint64_t)nnz, (uint64_t)sd[0], (uint64_t)sd[1], (uint64_t)sd[2], ssd[0], ssd[1], ssd[2],
+                               mx0, mx1, mxm, diff};
+  block_stats(v, s_red, s_tot);
+  if (threadIdx.x < kStatQ) {
+    const int q = threadIdx.x;
+    const uint64_t t = s_tot[q];
+    if (q == 0) r.nnz[y] = (int32_t)t;
+    else if (q < 4) r.sd[(q - 1) * g.h + y] = (int64_t)t;
+    else if (q < 7) r.ssd[(q - 4) * g.h + y] = t;
+    else if (q < 10) r.max[(q - 7) * g.h + y] = (uint32_t)t;
+    else r.diff[y] = (uint32_t)t;
+  }
+}
+
 // _mm_cvtsd_si32 under round-to-nearest-even (libs/pmbb-core/src/xCommonDefCORE.h:130): NaN and
 // results outside int32 give INT32_MIN.
 __device__ __forceinline__ int32_t cvt_rne_x86(double v) {
@@ -93,64 +228,49 @@ __device__ __forceinline__ int32_t cvt_rne_x86(double v) {
 // libs/pmbb-ivqm/src/xGlobClrDiff.cpp:52-83, as the processor calls it, xGlobClrDiff.h:42):
 // g_c = clip(cvt(SD_c / (double)(nnm * maxm)), +-maxdiff_c); the fourth entry has SD 0. With
 // gcd_given the caller's values are stored instead; nnm_given >= 0 replaces the counted pels in
-// the GCD's denominator (the processor form takes the count as an argument). One block of 1024.
-__global__ __launch_bounds__(1024) void k_msummary(Geom g, int32_t maxv, int32_t maxm, MaskRows r, int32_t md0,
-                                                   int32_t md1, int32_t md2, int32_t md3, int32_t nnm_given,
-                                                   int32_t gcd_given,
-                                                   int32_t g0, int32_t g1, int32_t g2, int32_t g3,
-                                                   MaskSummary* __restrict__ out) {
-  __shared__ int64_t s_sd[3][16];
-  __shared__ uint64_t s_ssd[3][16];
-  __shared__ int64_t s_n[16];
-  __shared__ uint32_t s_max[3][16], s_diff[16];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int64_t n = 0, sd[3] = {0, 0, 0};
-  uint64_t ssd[3] = {0, 0, 0};
-  uint32_t mx[3] = {0, 0, 0}, diff = 0;
-  for (int y = threadIdx.x; y < g.h; y += blockDim.x) {
-    n += r.nnz[y];
+// the GCD's denominator (the processor form takes the count as an argument). One block of 256;
+// each lane sums every 256th row, four rows' loads in flight.
+__global__ __launch_bounds__(256) void k_msummary(Geom g, int32_t maxv, int32_t maxm, MaskRows r, int32_t md0,
+                                                  int32_t md1, int32_t md2, int32_t md3, int32_t nnm_given,
+                                                  int32_t gcd_given,
+                                                  int32_t g0, int32_t g1, int32_t g2, int32_t g3,
+                                                  MaskSummary* __restrict__ out) {
+  __shared__ uint64_t s_red[kStatQ][256];
+  __shared__ uint64_t s_tot[kStatQ];
+  uint64_t v[kStatQ];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      sd[c] += r.sd[c * g.h + y];
-      ssd[c] += r.ssd[c * g.h + y];
-      mx[c] = max(mx[c], r.max[c * g.h + y]);
-    }
-    diff |= r.diff[y];
-  }
-  n = wave_sum(n);
+  for (int q = 0; q < kStatQ; q++) v[q] = 0;
+  // Raw buffer loads, the offset of a row past h replaced by one past every array: such loads
+  // return 0 and need no branch, so all four rows' loads issue together.
+  const uint32_t h = (uint32_t)g.h;
+  const __amdgpu_buffer_rsrc_t rn = __builtin_amdgcn_make_buffer_rsrc(r.nnz, 0, (int)(4 * h), kBufDword3);
+  const __amdgpu_buffer_rsrc_t rsd = __builtin_amdgcn_make_buffer_rsrc(r.sd, 0, (int)(24 * h), kBufDword3);
+  const __amdgpu_buffer_rsrc_t rss = __builtin_amdgcn_make_buffer_rsrc(r.ssd, 0, (int)(24 * h), kBufDword3);
+  const __amdgpu_buffer_rsrc_t rmx = __builtin_amdgcn_make_buffer_rsrc(r.max, 0, (int)(12 * h), kBufDword3);
+  const __amdgpu_buffer_rsrc_t rdf = __builtin_amdgcn_make_buffer_rsrc(r.diff, 0, (int)(4 * h), kBufDword3);
+  for (uint32_t y0 = 0; y0 < h; y0 += 4 * 256) {
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    sd[c] = wave_sum(sd[c]);
-    ssd[c] = wave_sum(ssd[c]);
-    mx[c] = wave_max(mx[c]);
-  }
+    for (int k = 0; k < 4; k++) {
+      const uint32_t y = y0 + 256u * k + threadIdx.x;
+      const bool ok = y < h;
+      const uint32_t o4 = ok ? 4u * y : kOob, o8 = ok ? 8u * y : kOob;
+      v[0] += (uint64_t)(int64_t)(int32_t)__builtin_amdgcn_raw_buffer_load_b32(rn, o4, 0, 0);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) diff |= (uint32_t)__shfl_xor((int)diff, o);
-  if (lane == 0) {
-    s_n[wave] = n;
-    s_diff[wave] = diff;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      s_sd[c][wave] = sd[c];
-      s_ssd[c][wave] = ssd[c];
-      s_max[c][wave] = mx[c];
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = (int)(blockDim.x >> 6);
-    int64_t tn = 0, tsd[3] = {0, 0, 0};
-    uint64_t tssd[3] = {0, 0, 0};
-    uint32_t tmx[3] = {0, 0, 0}, tdiff = 0;
-    for (int w = 0; w < nw; w++) {
-      tn += s_n[w];
-      tdiff |= s_diff[w];
       for (int c = 0; c < 3; c++) {
-        tsd[c] += s_sd[c][w];
-        tssd[c] += s_ssd[c][w];
-        tmx[c] = max(tmx[c], s_max[c][w]);
+        v[1 + c] += __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rsd, o8 + (ok ? 8u * c * h : 0u), 0, 0));
+        v[4 + c] += __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rss, o8 + (ok ? 8u * c * h : 0u), 0, 0));
+        v[7 + c] = max(v[7 + c], (uint64_t)__builtin_amdgcn_raw_buffer_load_b32(rmx, o4 + (ok ? 4u * c * h : 0u), 0, 0));
       }
+      v[10] |= __builtin_amdgcn_raw_buffer_load_b32(rdf, o4, 0, 0);
     }
+  }
+  block_stats(v, s_red, s_tot);
+  const int64_t tn = (int64_t)s_tot[0];
+  const int64_t tsd[3] = {(int64_t)s_tot[1], (int64_t)s_tot[2], (int64_t)s_tot[3]};
+  const uint64_t tssd[3] = {s_tot[4], s_tot[5], s_tot[6]};
+  const uint32_t tmx[3] = {(uint32_t)s_tot[7], (uint32_t)s_tot[8], (uint32_t)s_tot[9]};
+  const uint32_t tdiff = (uint32_t)s_tot[10];
+  if (threadIdx.x == 0) {
     out->nnm = (int32_t)tn;
     out->in_range[0] = tmx[0] <= (uint32_t)maxv;
     out->in_range[1] = tmx[1] <= (uint32_t)maxv;
@@ -281,6 +401,22 @@ __global__ __launch_bounds__(256) void k_cps_masked(const T* __restrict__ p0, co
 template <typename T, typename TM>
 hipError_t mstats_t(const void* p0, const void* p1, const Geom& g, const void* mask, const Geom& gm, const MaskRows& r,
                     hipStream_t st) {
+#ifndef QMIV_MSTATS_VECTOR
+#define QMIV_MSTATS_VECTOR 1
+#endif
+  if (QMIV_MSTATS_VECTOR && g.w % 4 == 0) {
+    const dim3 grid((unsigned)g.h), block(256);
+    const T* a = (const T*)p0;
+    const T* b = (const T*)p1;
+    const TM* m = (const TM*)mask;
+    if (g.cs == 1)
+      gm.cs == 1 ? k_mstats_v<T, TM, 1, 1><<<grid, block, 0, st>>>(a, b, g, m, gm, r)
+                 : k_mstats_v<T, TM, 1, 0><<<grid, block, 0, st>>>(a, b, g, m, gm, r);
+    else
+      gm.cs == 1 ? k_mstats_v<T, TM, 0, 1><<<grid, block, 0, st>>>(a, b, g, m, gm, r)
+                 : k_mstats_v<T, TM, 0, 0><<<grid, block, 0, st>>>(a, b, g, m, gm, r);
+    return hipGetLastError();
+  }
   k_mstats<T, TM><<<dim3((unsigned)((g.h + 3) / 4)), dim3(256), 0, st>>>((const T*)p0, (const T*)p1, g,
                                                                          (const TM*)mask, gm, r);
   return hipGetLastError();
@@ -316,7 +452,7 @@ hipError_t launch_msummary(const Geom& g, int32_t maxm, const MaskRows& r, const
                            const int32_t* gcd_given, MaskSummary* out, hipStream_t st) {
   const int32_t z[4] = {0, 0, 0, 0};
   const int32_t* gg = gcd_given ? gcd_given : z;
-  k_msummary<<<dim3(1), dim3(1024), 0, st>>>(g, (1 << g.bd) - 1, maxm, r, maxdiff[0], maxdiff[1], maxdiff[2],
+  k_msummary<<<dim3(1), dim3(256), 0, st>>>(g, (1 << g.bd) - 1, maxm, r, maxdiff[0], maxdiff[1], maxdiff[2],
                                              maxdiff[3], nnm_given, gcd_given ? 1 : 0, gg[0], gg[1], gg[2], gg[3],
                                              out);
   return hipGetLastError();

@@ -33,6 +33,8 @@ struct FrameSummary {
 // Row layout of the per-row statistics: luma rows [0, h), Cb rows [h, h + ch), Cr rows [h + ch, h + 2ch).
 inline int32_t stat_rows(const Geom& g) { return g.h + 2 * g.ch; }
 
+struct MaskSummary;
+
 // Arguments of the corresponding-pixel-shift kernels.
 struct CpsArgs {
   const void* p0;
@@ -50,6 +52,12 @@ struct CpsArgs {
   int16_t* best0;            // optional [h][w]: R2T winner, window raster index (dy + B)(2B + 1) + dx + B
   int16_t* best1;            // optional [h][w]: T2R winner
   int32_t maxv;
+  // Masked search (4:2:0 fast kernel only): the mask picture (its luma plane at offset 0, one sample
+  // per pel), the masked summary (GCD, range flags) in place of sum, no optional outputs.
+  const void* mask = nullptr;
+  uint32_t mask_bytes = 0;
+  int32_t mask_bytes_per_sample = 0;
+  const MaskSummary* msum = nullptr;
 };
 
 // One plane read through its 4:4:4 view: value(x, y) = base[(y >> shift) * stride + (x >> shift)].

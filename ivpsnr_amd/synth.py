@@ -138,8 +138,10 @@ def make_mask(width: int, height: int, bit_depth: int, seed: int, *, chroma_form
     """One packed mask frame whose luma plane selects the pels of masked mode.
 
     kind "objects": a few random ellipses at 2^bd - 1 on zero (the binary object masks of masked
-    mode); "soft": the same ellipses holding random values in [1, 2^bd - 1]; "zero" / "full":
-    every pel 0 / 2^bd - 1. Chroma planes hold the luma plane's top-left samples.
+    mode); "soft": the same ellipses holding random values in [1, 2^bd - 1]; "speckle": every pel
+    independently 0 or a random value in [1, 2^bd - 1], half and half (most search windows mix
+    masked and unmasked candidates); "zero" / "full": every pel 0 / 2^bd - 1. Chroma planes hold
+    the luma plane's top-left samples.
     """
     if chroma_format == 420 and (width % 2 or height % 2):
         raise ValueError("4:2:0 needs even width and height")
@@ -160,6 +162,8 @@ def make_mask(width: int, height: int, bit_depth: int, seed: int, *, chroma_form
             y = np.where(inside, max_value, 0)
         else:
             y = np.where(inside, rng.integers(1, max_value + 1, size=(height, width)), 0)
+    elif kind == "speckle":
+        y = np.where(rng.random((height, width)) < 0.5, 0, rng.integers(1, max_value + 1, size=(height, width)))
     else:
         raise ValueError(f"unknown mask kind {kind!r}")
     c = y[::2, ::2] if chroma_format == 420 else y

@@ -71,6 +71,10 @@ def test_processor_calls_match_reference(engine, case):
     (1920, 1080, 10, 420, 10, "soft", False),
     (333, 199, 12, 444, 8, "soft", False),
     (2, 2, 8, 420, 8, "full", False),
+    (640, 360, 10, 420, 8, "speckle", False),
+    (516, 262, 8, 420, 16, "speckle", True),
+    (1920, 1080, 10, 420, 10, "speckle", False),
+    (130, 66, 10, 420, 8, "zero", False),
 ])
 def test_frame_masked_matches_oracle(engine, oracle, w, h, bd, cf, bdm, kind, erp):
     print(f"masked oracle {w}x{h} bd{bd} cf{cf} {kind}", flush=True)
@@ -89,7 +93,28 @@ def test_frame_masked_matches_oracle(engine, oracle, w, h, bd, cf, bdm, kind, er
     assert r["ivpsnr_r2t"] == o["r2t"] and r["ivpsnr_t2r"] == o["t2r"]
 
 
-@pytest.mark.parametrize("B,wts", [(0, (4, 1, 1, 0)), (1, (4, 1, 1, 0)), (5, (1, 2, 3, 0)), (12, (4, 1, 1, 0))])
+@pytest.mark.parametrize("w,h,cf,cfm,bd,bdm", [
+    (256, 96, 420, 444, 10, 8), (256, 96, 444, 420, 10, 8), (256, 96, 444, 444, 8, 16), (252, 98, 444, 444, 12, 10),
+    (254, 98, 420, 420, 10, 8),
+])
+def test_chroma_formats_match_oracle(engine, oracle, w, h, cf, cfm, bd, bdm):
+    """Every pairing of picture and mask chroma formats, at widths that take the statistics
+    kernel's four-pel loads (divisible by 4) and one that does not (254)."""
+    print(f"masked formats {w}x{h} cf{cf} cfm{cfm} bd{bd} bdm{bdm}", flush=True)
+    pair = make_pair(w, h, bd, 61, chroma_format=cf, luma_offset=3.0)
+    mask = make_mask(w, h, bdm, 61, chroma_format=cfm, kind="speckle")
+    p0 = engine.picture(w, h, bd, cf, data=pair.ref)
+    p1 = engine.picture(w, h, bd, cf, data=pair.tst)
+    pm = engine.picture(w, h, bdm, cfm, data=mask)
+    r = engine.frame_masked(p0, p1, pm, PSNRS, default_params()).as_dict()
+    o = oracle.frame_masked(pair.ref, pair.tst, mask, bdm, cfm, oracle.params(w, h, bd, cf, metrics=PSNRS)).as_dict()
+    for k in ("num_non_masked", "gcd", "in_range", "exact", "psnr", "wspsnr", "psnr_pic", "wspsnr_pic", "ivpsnr",
+              "cps_sum"):
+        assert r[k] == o[k], k
+
+
+@pytest.mark.parametrize("B,wts", [(0, (4, 1, 1, 0)), (1, (4, 1, 1, 0)), (2, (4, 1, 1, 0)), (2, (1, 2, 3, 0)),
+                                   (2, (4, 1, 1, 1)), (5, (1, 2, 3, 0)), (12, (4, 1, 1, 0))])
 def test_search_parameters_match_oracle(engine, oracle, B, wts):
     w, h, bd = 150, 90, 10
     pair = make_pair(w, h, bd, 41, luma_offset=6.0)
