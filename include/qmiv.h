@@ -366,7 +366,7 @@ typedef struct qmiv_batch qmiv_batch;
 typedef struct qmiv_batch_options {
   int32_t branches;   /* concurrent graph branches, 1..16; 0: autotuned over {1, 2, 4, 8} */
   int32_t cps_sweep;  /* cap on CPS tiles per workgroup; 0: occupancy rule; -1: autotuned over {rule, 1, 2, 4} */
-  int32_t tune_reps;  /* timed replays per autotuner candidate (default 3) */
+  int32_t tune_reps;  /* timed replays per autotuner candidate and round (default 3); the fastest counts */
 } qmiv_batch_options;
 
 typedef struct qmiv_batch_info {

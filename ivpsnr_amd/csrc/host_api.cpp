@@ -945,24 +945,33 @@ qmiv_status batch_build(qmiv_batch* b, int32_t branches, int32_t sweep) {
 }
 
 // Average replay time of the current graph over `reps` launches (after one untimed launch).
+// The fastest of `reps` replays (after one untimed), each between its own event pair: the minimum
+// is robust against clock ramps and other work on the device, which a mean is not.
 qmiv_status batch_time(qmiv_batch* b, int32_t reps, double& ms) {
   qmiv_ctx* ctx = b->ctx;
-  hipEvent_t t0, t1;
-  QMIV_HIP(hipEventCreate(&t0));
-  QMIV_HIP(hipEventCreate(&t1));
+  reps = std::max(reps, 1);
+  std::vector<hipEvent_t> ev((size_t)reps + 1, nullptr);
   qmiv_status st = QMIV_OK;
-  if (hipGraphLaunch(b->exec, ctx->stream) != hipSuccess || hipEventRecord(t0, ctx->stream) != hipSuccess)
+  for (auto& e : ev)
+    if (st == QMIV_OK && hipEventCreate(&e) != hipSuccess) st = fail(QMIV_ERR_HIP, "timing events while tuning");
+  if (st == QMIV_OK && (hipGraphLaunch(b->exec, ctx->stream) != hipSuccess ||
+                        hipEventRecord(ev[0], ctx->stream) != hipSuccess))
     st = fail(QMIV_ERR_HIP, "graph launch while tuning");
   for (int32_t r = 0; r < reps && st == QMIV_OK; r++)
-    if (hipGraphLaunch(b->exec, ctx->stream) != hipSuccess) st = fail(QMIV_ERR_HIP, "graph launch while tuning");
-  float el = 0.f;
-  if (st == QMIV_OK && (hipEventRecord(t1, ctx->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
-                        hipEventElapsedTime(&el, t0, t1) != hipSuccess))
+    if (hipGraphLaunch(b->exec, ctx->stream) != hipSuccess || hipEventRecord(ev[(size_t)r + 1], ctx->stream) != hipSuccess)
+      st = fail(QMIV_ERR_HIP, "graph launch while tuning");
+  if (st == QMIV_OK && hipEventSynchronize(ev[(size_t)reps]) != hipSuccess)
     st = fail(QMIV_ERR_HIP, "timing events while tuning");
+  ms = 1e300;
+  for (int32_t r = 0; r < reps && st == QMIV_OK; r++) {
+    float el = 0.f;
+    if (hipEventElapsedTime(&el, ev[(size_t)r], ev[(size_t)r + 1]) != hipSuccess)
+      st = fail(QMIV_ERR_HIP, "timing events while tuning");
+    ms = std::min(ms, (double)el);
+  }
   (void)hipStreamSynchronize(ctx->stream);
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
-  ms = (double)el / std::max(reps, 1);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
   return st;
 }
 
@@ -1647,22 +1656,34 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
   double best_ms = 0.0;
   qmiv_status st = QMIV_OK;
   if (tune) {
-    best_ms = 1e300;
-    auto try_one = [&](int32_t br, int32_t sw) -> qmiv_status {
+    // Each stage times its candidates in two rounds and keeps each one's best time: a candidate
+    // timed once while the clocks ramp can lose by more than the candidates differ.
+    auto time_one = [&](int32_t br, int32_t sw, double& ms) -> qmiv_status {
       QMIV_TRY(batch_build(b, br, sw));
-      double ms = 0.0;
-      QMIV_TRY(batch_time(b, opt.tune_reps, ms));
+      double t = 0.0;
+      QMIV_TRY(batch_time(b, opt.tune_reps, t));
       b->info.candidates_tried++;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best_br = br;
-        best_sw = sw;
-      }
+      ms = std::min(ms, t);
       return QMIV_OK;
     };
-    for (int32_t br : br_cands)
-      if ((st = try_one(br, sw_cands[0])) != QMIV_OK) break;
-    for (size_t k = 1; k < sw_cands.size() && st == QMIV_OK; k++) st = try_one(best_br, sw_cands[k]);
+    std::vector<double> br_ms(br_cands.size(), 1e300);
+    for (int round = 0; round < 2 && st == QMIV_OK; round++)
+      for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) st = time_one(br_cands[i], sw_cands[0], br_ms[i]);
+    best_ms = 1e300;
+    for (size_t i = 0; i < br_cands.size(); i++)
+      if (br_ms[i] < best_ms) {
+        best_ms = br_ms[i];
+        best_br = br_cands[i];
+      }
+    std::vector<double> sw_ms(sw_cands.size(), 1e300);
+    sw_ms[0] = best_ms;
+    for (int round = 0; round < 2 && st == QMIV_OK; round++)
+      for (size_t k = 1; k < sw_cands.size() && st == QMIV_OK; k++) st = time_one(best_br, sw_cands[k], sw_ms[k]);
+    for (size_t k = 0; k < sw_cands.size(); k++)
+      if (sw_ms[k] < best_ms) {
+        best_ms = sw_ms[k];
+        best_sw = sw_cands[k];
+      }
   }
   if (st == QMIV_OK) st = batch_build(b, best_br, best_sw);
   if (st != QMIV_OK) {
