@@ -289,7 +289,8 @@ def run_grad(args):
 
 def run_masked(args):
     """Masked PSNR + WS-PSNR + IV-PSNR of 8 frame pairs 1920x1080 yuv420p with an 8-bit object mask
-    (binary ellipses at 255 over about a third of the picture), one qmiv_frame_masked per frame."""
+    (binary ellipses at 255 over about a third of the picture): the frame set as one
+    qmiv_frames_masked call (timed), and one qmiv_frame_masked per frame beside it."""
     from ivpsnr_amd import Engine, default_params
     from ivpsnr_amd.synth import make_mask, make_pair
     W, H, BD, nframes = 1920, 1080, 8, args.frames or 8
@@ -303,10 +304,13 @@ def run_masked(args):
     eng.sync()
     metrics = 1 | 2 | 4
 
-    def step():
+    def step_set():
+        return eng.frames_masked(pics, metrics, prm)[0]
+
+    def step_loop():
         return [eng.frame_masked(a, b, m, metrics, prm) for a, b, m in pics]
 
-    def timed(steps, warmup, profile):
+    def timed(steps, warmup, profile, step=step_set):
         for _ in range(warmup):
             step()
         eng.profile(profile)
@@ -327,6 +331,9 @@ def run_masked(args):
     elapsed = min(elapsed, el2)
     ms_per_step = elapsed / args.steps * 1e3
     mpix = nframes * W * H / 1e6
+    el_loop, _, res_loop = timed(args.steps, 1, False, step_loop)
+    plain_mpix_s = mpix / (el_loop / args.steps)
+    assert [r.as_dict() for r in res] == [r.as_dict() for r in res_loop], "frame set differs from the per-frame calls"
     k_ms, k_n = prof.get("cps", (0.0, 0))
     k_avg_ms = k_ms / max(k_n, 1)
     per_launch = frame_pair_bytes(W, H, BD) + W * H  # two pictures and the mask's luma plane
@@ -367,6 +374,8 @@ def run_masked(args):
         "dtype": "int32", "data": "synthetic",
         "config": {"workload": "masked PSNR+WS-PSNR+IV-PSNR, 8 frame pairs 1920x1080 yuv420p, 8-bit object "
                                "masks, B=2, GCD on", "frames_per_step": nframes,
+                   "mode": "stream (qmiv_frames_masked: frame set enqueued over two streams, one wait)",
+                   "plain_loop_mpix_s": round(plain_mpix_s, 2), "stream_vs_plain": round(mpix / (ms_per_step / 1e3) / plain_mpix_s, 3),
                    "ivpsnr_last_frame": round(res[-1].ivpsnr, 6), "nnm_last_frame": res[-1].num_non_masked},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic("k_cps_fast420_masked_1920x1080_8b.txt"),

@@ -310,6 +310,15 @@ typedef struct qmiv_masked_result {
 qmiv_status qmiv_frame_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
                               const qmiv_params* prm, uint32_t metrics, qmiv_masked_result* out);
 
+/* The masked flow for n frames (sizes may differ between frames, each with its own mask): every
+   frame's kernels are enqueued back to back (alternating over two streams), then the host
+   finalisation of each as its results arrive. Replaces the reference application's frame loop in
+   mask mode (apps/QMIV/src/xAppQMIV.cpp:666-748 with the masked calls of :825-889). out[i]
+   receives frame i; the first failing frame's status is returned. */
+qmiv_status qmiv_frames_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                               const qmiv_pic* const* mask, const qmiv_params* prm, uint32_t metrics,
+                               qmiv_masked_result* out);
+
 /* xPixelOps::CountNonZero over the mask's luma plane (libs/pmbb-core/src/xPixelOpsSTD.cpp:273-284). */
 qmiv_status qmiv_count_nonzero(qmiv_ctx* ctx, const qmiv_pic* mask, int32_t* out);
 

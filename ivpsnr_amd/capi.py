@@ -134,6 +134,8 @@ _SIGNATURES = {
     "qmiv_seq_close": (None, [C.c_void_p]),
     "qmiv_frame_masked": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), C.c_uint32,
                                     C.POINTER(MaskedResult)]),
+    "qmiv_frames_masked": (C.c_int, [_ctx_p, C.c_int32, C.POINTER(_pic_p), C.POINTER(_pic_p), C.POINTER(_pic_p),
+                                     C.POINTER(Params), C.c_uint32, C.POINTER(MaskedResult)]),
     "qmiv_count_nonzero": (C.c_int, [_ctx_p, _pic_p, _i32p]),
     "qmiv_calc_gcd_masked": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), C.c_int32, _i32p]),
     "qmiv_calc_psnr_masked": (C.c_int, [_ctx_p, _pic_p, _pic_p, _pic_p, C.POINTER(Params), C.c_int32, _f64p]),

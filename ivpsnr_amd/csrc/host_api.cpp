@@ -2035,32 +2035,46 @@ bool fast_masked_cps_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_
   return 32 * (w0 + w1 + w2) * m * m + 32 < lim && 64 * (w1 + w2) * m * maxv < lim;
 }
 
-// Masked statistics (and, with cps, the masked search of both directions) of P0 / P1 under the
-// mask; results at the start of ctx->h_results. gcd_given: null to compute the masked GCD.
-qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
-                       const qmiv_params& prm, bool cps, int32_t nnm_given, const int32_t* gcd_given,
-                       MaskedLayout& L) {
-  const Geom& g = p0->g;
+// Everything decided on the host before a masked frame's kernels are enqueued.
+struct MaskedPlan {
+  MaskedLayout L;
+  bool cps = false;
+  bool fast = false;  // the search runs on the 4:2:0 fast kernel
+  CpsArgs fa{};       // its tiling (tiles_x, sweep, nseg) when fast
+  int32_t md[4];
+};
+
+// gcd_given: null to compute the masked GCD.
+qmiv_status plan_masked(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, bool cps, const int32_t* gcd_given,
+                        MaskedPlan& P) {
   if (cps && (prm.search_range < 0 || prm.search_range > kMaxMaskedSearchRange))
     return fail(QMIV_ERR_UNSUPPORTED, "masked search supports search ranges 0..12");
+  P = MaskedPlan{};
+  P.cps = cps;
   // The computed GCD is clamped to the unnoticeable-difference bound, and its fourth entry is 0.
-  int32_t md[4], gbound[4];
-  for (int c = 0; c < 4; c++) md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
-  for (int c = 0; c < 4; c++) gbound[c] = gcd_given ? gcd_given[c] : (c < 3 ? md[c] : 0);
-  const bool fast = cps && prm.search_range == 2 && fast_masked_cps_ok(g, prm, gbound);
-  CpsArgs fa{};
-  if (fast) {
+  int32_t gbound[4];
+  for (int c = 0; c < 4; c++) P.md[c] = fin::gcd_max_diff(prm.unnoticeable_coef[c], g.bd);
+  for (int c = 0; c < 4; c++) gbound[c] = gcd_given ? gcd_given[c] : (c < 3 ? P.md[c] : 0);
+  P.fast = cps && prm.search_range == 2 && fast_masked_cps_ok(g, prm, gbound);
+  if (P.fast) {
     int32_t tw, th;
     cps_tile(true, &tw, &th);
-    fa.tiles_x = (g.w + tw - 1) / tw;
-    fa.sweep = pick_sweep(ctx, g, true, gbound);
-    fa.nseg = (fa.tiles_x + fa.sweep - 1) / fa.sweep;
+    P.fa.tiles_x = (g.w + tw - 1) / tw;
+    P.fa.sweep = pick_sweep(ctx, g, true, gbound);
+    P.fa.nseg = (P.fa.tiles_x + P.fa.sweep - 1) / P.fa.sweep;
   }
-  L = masked_layout(g, cps, fast ? fa.nseg : masked_cps_segments(g));
-  QMIV_HIP(hipSetDevice(ctx->device));
-  QMIV_TRY(ensure_scratch(ctx, L.total));
-  QMIV_TRY(ensure_results(ctx, L.results_bytes));
-  char* d = (char*)ctx->d_scratch;
+  P.L = masked_layout(g, cps, P.fast ? P.fa.nseg : masked_cps_segments(g));
+  return QMIV_OK;
+}
+
+// Enqueues the masked statistics (and, with P.cps, the masked search of both directions) of
+// P0 / P1 under the mask on stream st, over scratch d (P.L.total bytes), then the copy of the
+// results block into the pinned host block h.
+qmiv_status enqueue_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
+                           const qmiv_params& prm, MaskedPlan& P, int32_t nnm_given, const int32_t* gcd_given,
+                           char* d, char* h, hipStream_t st) {
+  const Geom& g = p0->g;
+  const MaskedLayout& L = P.L;
   MaskRows r;
   r.nnz = (int32_t*)(d + L.off_nnz);
   r.sd = (int64_t*)(d + L.off_sd);
@@ -2068,20 +2082,20 @@ qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, co
   r.max = (uint32_t*)(d + L.off_max);
   r.diff = (uint32_t*)(d + L.off_diff);
   MaskSummary* sum = (MaskSummary*)(d + L.off_sum);
-  hipStream_t st = ctx->stream;
   {
     KernelTimer t(ctx, QMIV_K_STATS, st);
     QMIV_HIP(launch_mstats(p0->dev, p1->dev, g, mask->dev, mask->g, r, st));
   }
   {
     KernelTimer t(ctx, QMIV_K_SUMMARY, st);
-    QMIV_HIP(launch_msummary(g, (1 << mask->g.bd) - 1, r, md, nnm_given, gcd_given, sum, st));
+    QMIV_HIP(launch_msummary(g, (1 << mask->g.bd) - 1, r, P.md, nnm_given, gcd_given, sum, st));
   }
-  if (cps) {
+  if (P.cps) {
     uint64_t* part = (uint64_t*)(d + L.off_part);
     {
       KernelTimer t(ctx, QMIV_K_CPS, st);
-      if (fast) {
+      if (P.fast) {
+        CpsArgs& fa = P.fa;
         fa.p0 = p0->dev;
         fa.p1 = p1->dev;
         fa.g = g;
@@ -2104,7 +2118,22 @@ qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, co
       QMIV_HIP(launch_cps_rows(part, L.nseg, g.h, (uint64_t*)(d + L.off_cps), st));
     }
   }
-  QMIV_HIP(hipMemcpyAsync(ctx->h_results, d, L.results_bytes, hipMemcpyDeviceToHost, st));
+  QMIV_HIP(hipMemcpyAsync(h, d, L.results_bytes, hipMemcpyDeviceToHost, st));
+  return QMIV_OK;
+}
+
+// One masked evaluation on the context stream, waited for; results at the start of ctx->h_results.
+qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
+                       const qmiv_params& prm, bool cps, int32_t nnm_given, const int32_t* gcd_given,
+                       MaskedLayout& L) {
+  MaskedPlan P;
+  QMIV_TRY(plan_masked(ctx, p0->g, prm, cps, gcd_given, P));
+  L = P.L;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_TRY(ensure_scratch(ctx, L.total));
+  QMIV_TRY(ensure_results(ctx, L.results_bytes));
+  QMIV_TRY(enqueue_masked(ctx, p0, p1, mask, prm, P, nnm_given, gcd_given, (char*)ctx->d_scratch,
+                          (char*)ctx->h_results, ctx->stream));
   return wait_stream(ctx);
 }
 
@@ -2125,26 +2154,15 @@ qmiv_params masked_params(const qmiv_params* prm_in, bool processor) {
 
 }  // namespace
 
-qmiv_status qmiv_frame_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
-                              const qmiv_params* prm_in, uint32_t metrics, qmiv_masked_result* out) {
-  if (!ctx || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
-  QMIV_TRY(check_pair(p0, p1));
-  QMIV_TRY(check_mask(p0, mask));
-  if (p0->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
-  if (metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM))
-    return fail(QMIV_ERR_INVALID_ARG, "structural similarity metrics cannot be combined with mask mode");
-  if (metrics & ~(uint32_t)(QMIV_PSNR | QMIV_WSPSNR | QMIV_IVPSNR)) return fail(QMIV_ERR_INVALID_ARG, "unknown metric bits");
-  const qmiv_params prm = masked_params(prm_in, false);
-  std::memset(out, 0, sizeof(*out));
+namespace {
+
+// Host finalisation of one masked frame from its results block hb (the application's masked
+// flow, xAppQMIV.cpp:825-889).
+qmiv_status finalize_masked(qmiv_ctx* ctx, const Geom& g, int32_t bdm, const qmiv_params& prm, uint32_t metrics,
+                            const MaskedLayout& L, const char* hb, qmiv_masked_result* out) {
   const bool iv = (metrics & QMIV_IVPSNR) != 0;
-  const int32_t zero[4] = {0, 0, 0, 0};
-  MaskedLayout L;
-  // Without IV-PSNR the application computes no colour difference and reports zeros.
-  QMIV_TRY(run_masked(ctx, p0, p1, mask, prm, iv, -1, iv ? nullptr : zero, L));
-  const char* hb = (const char*)ctx->h_results;
-  const MaskSummary* sum = masked_summary(ctx, L);
-  const Geom& g = p0->g;
-  const int32_t bdm = mask->g.bd, nnm = sum->nnm;
+  const MaskSummary* sum = (const MaskSummary*)(hb + L.off_sum);
+  const int32_t nnm = sum->nnm;
   for (int i = 0; i < 3; i++) out->in_range[i] = sum->in_range[i];
   for (int c = 0; c < 3; c++) out->exact[c] = sum->exact[c];
   for (int c = 0; c < 4; c++) out->gcd[c] = sum->gcd[c];
@@ -2184,6 +2202,90 @@ qmiv_status qmiv_frame_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic*
       }
   }
   return QMIV_OK;
+}
+
+}  // namespace
+
+qmiv_status qmiv_frame_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_pic* mask,
+                              const qmiv_params* prm_in, uint32_t metrics, qmiv_masked_result* out) {
+  if (!ctx || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  const qmiv_pic* a[1] = {p0};
+  const qmiv_pic* b[1] = {p1};
+  const qmiv_pic* m[1] = {mask};
+  return qmiv_frames_masked(ctx, 1, a, b, m, prm_in, metrics, out);
+}
+
+qmiv_status qmiv_frames_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                               const qmiv_pic* const* mask, const qmiv_params* prm_in, uint32_t metrics,
+                               qmiv_masked_result* out) {
+  if (!ctx || !out || !p0 || !p1 || !mask || n < 0) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return QMIV_OK;
+  for (int32_t i = 0; i < n; i++) {
+    QMIV_TRY(check_pair(p0[i], p1[i]));
+    QMIV_TRY(check_mask(p0[i], mask[i]));
+    if (p0[i]->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
+  }
+  if (metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM))
+    return fail(QMIV_ERR_INVALID_ARG, "structural similarity metrics cannot be combined with mask mode");
+  if (metrics & ~(uint32_t)(QMIV_PSNR | QMIV_WSPSNR | QMIV_IVPSNR)) return fail(QMIV_ERR_INVALID_ARG, "unknown metric bits");
+  const qmiv_params prm = masked_params(prm_in, false);
+  for (int32_t i = 0; i < n; i++) std::memset(&out[i], 0, sizeof(out[i]));
+  const bool iv = (metrics & QMIV_IVPSNR) != 0;
+  // Without IV-PSNR the application computes no colour difference and reports zeros.
+  const int32_t zero[4] = {0, 0, 0, 0};
+  const int32_t* gcd_given = iv ? nullptr : zero;
+  // Plan every frame, then lay their scratch and result blocks end to end.
+  std::vector<MaskedPlan> plans((size_t)n);
+  std::vector<size_t> doff((size_t)n), hoff((size_t)n);
+  size_t dtot = 0, htot = 0;
+  for (int32_t i = 0; i < n; i++) {
+    QMIV_TRY(plan_masked(ctx, p0[i]->g, prm, iv, gcd_given, plans[(size_t)i]));
+    doff[(size_t)i] = dtot;
+    hoff[(size_t)i] = htot;
+    dtot += align_up(plans[(size_t)i].L.total, 256);
+    htot += align_up(plans[(size_t)i].L.results_bytes, 256);
+  }
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_TRY(ensure_scratch(ctx, dtot));
+  QMIV_TRY(ensure_results(ctx, htot));
+  while ((int32_t)ctx->done_ev.size() < n) {
+    hipEvent_t e;
+    QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->done_ev.push_back(e);
+  }
+  // Frames alternate over two streams, so one frame's statistics overlap the other's search.
+  const int32_t nstreams = n > 1 ? 2 : 1;
+  const hipStream_t streams[2] = {ctx->stream, ctx->aux_stream};
+  if (nstreams > 1) {  // the aux stream starts after everything already on the context stream
+    QMIV_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+    QMIV_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_ev, 0));
+  }
+  for (int32_t i = 0; i < n; i++) {
+    const hipStream_t st = streams[i % nstreams];
+    qmiv_status e = enqueue_masked(ctx, p0[i], p1[i], mask[i], prm, plans[(size_t)i], -1, gcd_given,
+                                   (char*)ctx->d_scratch + doff[(size_t)i], (char*)ctx->h_results + hoff[(size_t)i], st);
+    if (e == QMIV_OK && hipEventRecord(ctx->done_ev[(size_t)i], st) != hipSuccess)
+      e = fail(QMIV_ERR_HIP, "event record failed");
+    if (e != QMIV_OK) {
+      // Drain what was enqueued: scratch and result buffers must be idle before any later reuse.
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipStreamSynchronize(ctx->aux_stream);
+      return e;
+    }
+  }
+  if (nstreams > 1) {  // later work on the context stream follows the whole set
+    QMIV_HIP(hipEventRecord(ctx->join_ev, ctx->aux_stream));
+    QMIV_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
+  }
+  // Each frame is finalised on the host as soon as its results have arrived.
+  qmiv_status first_err = QMIV_OK;
+  for (int32_t i = 0; i < n; i++) {
+    QMIV_HIP(hipEventSynchronize(ctx->done_ev[(size_t)i]));
+    const qmiv_status st = finalize_masked(ctx, p0[i]->g, mask[i]->g.bd, prm, metrics, plans[(size_t)i].L,
+                                           (const char*)ctx->h_results + hoff[(size_t)i], &out[i]);
+    if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
+  }
+  return first_err;
 }
 
 qmiv_status qmiv_count_nonzero(qmiv_ctx* ctx, const qmiv_pic* mask, int32_t* out) {

@@ -296,6 +296,19 @@ class Engine:
         r.status = st
         return r
 
+    def frames_masked(self, triples, metrics: int, params: Params | None = None, raise_on_error: bool = True):
+        """All (p0, p1, mask) frames enqueued back to back, then each finalised as its results
+        arrive (qmiv_frames_masked). Returns the results and the call's status."""
+        n = len(triples)
+        a = (C.c_void_p * n)(*[t[0].handle.value for t in triples])
+        b = (C.c_void_p * n)(*[t[1].handle.value for t in triples])
+        m = (C.c_void_p * n)(*[t[2].handle.value for t in triples])
+        out = (capi.MaskedResult * n)()
+        st = lib().qmiv_frames_masked(self.ctx, n, a, b, m, C.byref(params) if params else None, metrics, out)
+        if raise_on_error:
+            check(st, "qmiv_frames_masked")
+        return list(out), st
+
     def count_nonzero(self, mask: Picture) -> int:
         n = C.c_int32()
         check(lib().qmiv_count_nonzero(self.ctx, mask.handle, C.byref(n)), "qmiv_count_nonzero")

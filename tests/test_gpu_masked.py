@@ -113,6 +113,40 @@ def test_chroma_formats_match_oracle(engine, oracle, w, h, cf, cfm, bd, bdm):
         assert r[k] == o[k], k
 
 
+def test_frames_masked_equal_single_frames(engine):
+    """A masked frame set (qmiv_frames_masked, two streams) gives each frame exactly what the
+    single-frame call gives, over frames of different sizes, formats, masks and bit depths; a
+    frame with a pel above its bit depth fails alone and the others are still finalised."""
+    specs = [(320, 180, 10, 420, 8, "objects"), (256, 96, 8, 444, 16, "speckle"), (130, 66, 10, 420, 10, "soft"),
+             (640, 360, 8, 420, 8, "speckle"), (96, 70, 16, 420, 16, "soft")]
+    triples = []
+    for k, (w, h, bd, cf, bdm, kind) in enumerate(specs):
+        pair = make_pair(w, h, bd, 70 + k, chroma_format=cf, luma_offset=2.0)
+        mask = make_mask(w, h, bdm, 70 + k, chroma_format=cf, kind=kind)
+        triples.append((engine.picture(w, h, bd, cf, data=pair.ref), engine.picture(w, h, bd, cf, data=pair.tst),
+                        engine.picture(w, h, bdm, cf, data=mask)))
+    for erp in (False, True):
+        prm = default_params(erp=erp)
+        single = [engine.frame_masked(a, b, m, PSNRS, prm).as_dict() for a, b, m in triples]
+        res, st = engine.frames_masked(triples, PSNRS, prm)
+        assert st == 0
+        for r, want in zip(res, single):
+            got = r.as_dict()
+            for k in want:
+                if k != "status":
+                    assert got[k] == want[k], k
+    # frame 2 out of range: pel 1023 + 1 in a 10-bit picture
+    w, h = 130, 66
+    bad = make_pair(w, h, 10, 99, luma_offset=0.0).tst.copy()
+    bad[5] = 1024
+    triples[2] = (triples[2][0], engine.picture(w, h, 10, 420, data=bad), triples[2][2])
+    res, st = engine.frames_masked(triples, PSNRS, default_params(), raise_on_error=False)
+    assert st == 5
+    assert res[2].as_dict()["in_range"][1] == 0
+    for k in (0, 1, 3, 4):
+        assert res[k].as_dict()["ivpsnr"] == engine.frame_masked(*triples[k], PSNRS, default_params()).ivpsnr
+
+
 @pytest.mark.parametrize("B,wts", [(0, (4, 1, 1, 0)), (1, (4, 1, 1, 0)), (2, (4, 1, 1, 0)), (2, (1, 2, 3, 0)),
                                    (2, (4, 1, 1, 1)), (5, (1, 2, 3, 0)), (12, (4, 1, 1, 0))])
 def test_search_parameters_match_oracle(engine, oracle, B, wts):
