@@ -2020,11 +2020,12 @@ qmiv_status check_mask(const qmiv_pic* p0, const qmiv_pic* mask) {
   return QMIV_OK;
 }
 
-// The masked search on the fast 4:2:0 kernel: besides the unmasked kernel's conditions, a masked
-// candidate's record must lose to every unmasked one, which the kernel ensures when every real key
-// and every chroma term T stays below 2^29 in magnitude. gcd_bound: |GCD| per component, at most.
+// The masked search on the fast kernels (4:2:0 and 4:4:4): besides the unmasked kernels'
+// conditions, a masked candidate's record must lose to every unmasked one, which the kernels ensure
+// when every real key (and, in the 4:2:0 kernel, every chroma term T added to it) stays below 2^29
+// in magnitude. gcd_bound: |GCD| per component, at most.
 bool fast_masked_cps_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) {
-  if (g.cs != 1 || !fast_cps_ok(g, p, gcd_bound)) return false;
+  if (!fast_cps_ok(g, p, gcd_bound)) return false;
   const int64_t maxv = (1 << g.bd) - 1;
   int64_t m = 0;
   for (int c = 0; c < 3; c++) m = std::max<int64_t>(m, maxv + std::abs((int64_t)gcd_bound[c]));
@@ -2039,7 +2040,7 @@ bool fast_masked_cps_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_
 struct MaskedPlan {
   MaskedLayout L;
   bool cps = false;
-  bool fast = false;  // the search runs on the 4:2:0 fast kernel
+  bool fast = false;  // the search runs on a fast kernel (the 4:2:0 one when g.cs == 1)
   CpsArgs fa{};       // its tiling (tiles_x, sweep, nseg) when fast
   int32_t md[4];
 };
@@ -2057,10 +2058,11 @@ qmiv_status plan_masked(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, bo
   for (int c = 0; c < 4; c++) gbound[c] = gcd_given ? gcd_given[c] : (c < 3 ? P.md[c] : 0);
   P.fast = cps && prm.search_range == 2 && fast_masked_cps_ok(g, prm, gbound);
   if (P.fast) {
+    const bool f420 = g.cs == 1;
     int32_t tw, th;
-    cps_tile(true, &tw, &th);
+    cps_tile(f420, &tw, &th);
     P.fa.tiles_x = (g.w + tw - 1) / tw;
-    P.fa.sweep = pick_sweep(ctx, g, true, gbound);
+    P.fa.sweep = pick_sweep(ctx, g, f420, gbound);
     P.fa.nseg = (P.fa.tiles_x + P.fa.sweep - 1) / P.fa.sweep;
   }
   P.L = masked_layout(g, cps, P.fast ? P.fa.nseg : masked_cps_segments(g));
@@ -2107,7 +2109,8 @@ qmiv_status enqueue_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1
         fa.mask_bytes = (uint32_t)((size_t)g.w * g.h * mask->g.bytes_per_sample);  // its luma plane
         fa.mask_bytes_per_sample = mask->g.bytes_per_sample;
         fa.msum = sum;
-        QMIV_HIP(launch_cps_fast420(fa, st));
+        if (g.cs == 1) QMIV_HIP(launch_cps_fast420(fa, st));
+        else QMIV_HIP(launch_cps_fast(fa, st));
       } else {
         QMIV_HIP(launch_cps_masked(p0->dev, p1->dev, g, mask->dev, mask->g, sum, prm.search_range,
                                    prm.cmp_weights_search, part, st));

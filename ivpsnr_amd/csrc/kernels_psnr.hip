@@ -9,6 +9,7 @@
 #include "qmiv_internal.h"
 #include "device_common.h"
 
+#include <type_traits>
 #include <climits>
 
 namespace qmiv {
@@ -215,11 +216,22 @@ __global__ __launch_bounds__(1024) void k_summary(Geom g, const int64_t* __restr
 constexpr int kCpsLW = kCpsTileW + 8;
 constexpr int kCpsLH = kCpsTileH + 4;
 
+// Masked search: a searched pel whose mask is 0 gets the record {0, 0, 0, kCpsMaskedC + (h mod 32)}.
+// Its key is then above 2^30 while every real key lies in (-2^29, 2^29) (the host checks the
+// bound), so it never beats an unmasked candidate, and a winner still decodes to a window
+// position. The tested pel is always its own unmasked candidate, as in
+// xCorrespPixelShiftSTD::FindBestPixelWithinBlockM (libs/pmbb-ivqm/src/xCorrespPixelShiftSTD.cpp).
+constexpr uint32_t kCpsMaskedC = 1u << 30;
+__device__ __forceinline__ uint4 masked_rec(uint4 r, uint32_t m) {
+  return m != 0u ? r : make_uint4(0u, 0u, 0u, kCpsMaskedC | (r.w & 31u));
+}
+
 // Fills the LDS records of picture Yp for the tile at (x0, y0):
-//   rec = {Y, U, V, C},  C = 32 * (w0 Y^2 + w1 U^2 + w2 V^2) + 5 ly + lx.
-template <typename T>
-__device__ __forceinline__ void cps_fill(const Buf<T>& Yp, const Geom& g, int x0, int y0, int32_t w0, int32_t w1,
-                                         int32_t w2, uint4* s_rec) {
+//   rec = {Y, U, V, C},  C = 32 * (w0 Y^2 + w1 U^2 + w2 V^2) + 5 ly + lx;
+// with MASKED, the masked record wherever the mask Mp (its luma plane) is 0.
+template <typename T, typename TM, bool MASKED>
+__device__ __forceinline__ void cps_fill(const Buf<T>& Yp, const Buf<TM>& Mp, const Geom& g, int x0, int y0, int32_t w0,
+                                         int32_t w1, int32_t w2, uint4* s_rec) {
   constexpr int LW = kCpsLW, LH = kCpsLH;
   const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
   const bool interior = x0 >= 4 && x0 + kCpsTileW + 4 <= g.w && y0 >= 2 && y0 + kCpsTileH + 2 <= g.h && (g.w & 3) == 0;
@@ -262,10 +274,21 @@ __device__ __forceinline__ void cps_fill(const Buf<T>& Yp, const Geom& g, int x0
           vv[m] = Yp.ld(cr_off + lo + m);
         }
       }
+      uint32_t mk[4] = {1u, 1u, 1u, 1u};
+      if constexpr (MASKED) {
+        if constexpr (sizeof(TM) == 2) {
+          const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(Mp.r, lo * 2u, 0, 0));
+          mk[0] = w.x & 0xffffu; mk[1] = w.x >> 16; mk[2] = w.y & 0xffffu; mk[3] = w.y >> 16;
+        } else {
+          const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(Mp.r, lo, 0, 0);
+          mk[0] = w & 0xffu; mk[1] = (w >> 8) & 0xffu; mk[2] = (w >> 16) & 0xffu; mk[3] = w >> 24;
+        }
+      }
 #pragma unroll
       for (int m = 0; m < 4; m++) {
         const int32_t S = w0 * (int32_t)(yv[m] * yv[m]) + w1 * (int32_t)(uv[m] * uv[m]) + w2 * (int32_t)(vv[m] * vv[m]);
-        s_rec[ly * LW + lx + m] = make_uint4(yv[m], uv[m], vv[m], (uint32_t)(S * 32 + 5 * ly + lx + m));
+        const uint4 r = make_uint4(yv[m], uv[m], vv[m], (uint32_t)(S * 32 + 5 * ly + lx + m));
+        s_rec[ly * LW + lx + m] = MASKED ? masked_rec(r, mk[m]) : r;
       }
     }
   } else {
@@ -279,17 +302,21 @@ __device__ __forceinline__ void cps_fill(const Buf<T>& Yp, const Geom& g, int x0
       const uint32_t uv = Yp.ld(cb_off + co);
       const uint32_t vv = Yp.ld(cr_off + co);
       const int32_t S = w0 * (int32_t)(yv * yv) + w1 * (int32_t)(uv * uv) + w2 * (int32_t)(vv * vv);
-      s_rec[i] = make_uint4(yv, uv, vv, (uint32_t)(S * 32 + 5 * ly + lx));
+      const uint4 r = make_uint4(yv, uv, vv, (uint32_t)(S * 32 + 5 * ly + lx));
+      s_rec[i] = MASKED ? masked_rec(r, Mp.ld((uint32_t)(gy * g.w + gx))) : r;
     }
   }
 }
 
 // One direction over one tile: each of the lane's kCpsTileH / 4 pels finds its best candidate; the winner's
 // per-component squared error is added to acc (per-lane row sums kept across the tile sweep).
-template <typename T, int DIR>
+template <bool MASKED>
+using CpsAcc = typename std::conditional<MASKED, uint64_t, uint32_t>::type;
+
+template <typename T, typename TM, bool MASKED, int DIR>
 __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, int x0, int y0, int lane, int wave,
                                              const int32_t (&gcd)[3], uint4* s_rec,
-                                             uint32_t (&acc)[kCpsTileH / 4][3]) {
+                                             CpsAcc<MASKED> (&acc)[kCpsTileH / 4][3]) {
   // Opaque copies: keeps per-row address arithmetic from being hoisted out of the tile sweep and
   // held in registers across it.
   asm volatile("" : "+v"(lane));
@@ -304,8 +331,9 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
   const int32_t w0 = a.ws[0], w1 = a.ws[1], w2 = a.ws[2];
   const uint32_t cb_off = (uint32_t)g.cb_off, cr_off = (uint32_t)g.cr_off;
 
+  const Buf<TM> Mp(a.mask, a.mask_bytes);
   __syncthreads();
-  cps_fill<T>(Yp, g, x0, y0, w0, w1, w2, s_rec);
+  cps_fill<T, TM, MASKED>(Yp, Mp, g, x0, y0, w0, w1, w2, s_rec);
   __syncthreads();
 
   // The lane's pels: X = pel + sgn * gcd (not clipped: xc_CLIP_CURR_TST_RANGE = false,
@@ -318,9 +346,11 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
   int32_t a0[K], a1[K], a2[K];
   uint32_t x01[K];
   int32_t x2[K];
+  uint32_t mx[K];  // masked search: the tested pels' mask values
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int y = min(y0 + K * wave + k, g.h - 1);
+    if constexpr (MASKED) mx[k] = Mp.ld((uint32_t)(y * g.w + xc));
     const uint32_t co = (uint32_t)((y >> g.cs) * g.cw + (xc >> g.cs));
     const int32_t v0 = (int32_t)Xp.ld((uint32_t)(y * g.w + xc)) + sgn * gcd[0];
     const int32_t v1 = (int32_t)Xp.ld(cb_off + co) + sgn * gcd[1];
@@ -385,9 +415,17 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
     const int32_t d0 = xv0 - (int32_t)r.x, d1 = xv1 - (int32_t)r.y, d2 = x2[k] - (int32_t)r.z;
     const int y = y0 + K * wave + k;
     if (x < g.w && y < g.h) {
-      acc[k][0] += (uint32_t)(d0 * d0);
-      acc[k][1] += (uint32_t)(d1 * d1);
-      acc[k][2] += (uint32_t)(d2 * d2);
+      if constexpr (MASKED) {
+        // xCorrespPixelShiftSTD::CalcDistAsymmetricRowM: the winner's int32 squared errors times the
+        // tested pel's mask value, in 64 bits (the host keeps the squares below 2^31).
+        acc[k][0] += (uint64_t)(uint32_t)(d0 * d0) * mx[k];
+        acc[k][1] += (uint64_t)(uint32_t)(d1 * d1) * mx[k];
+        acc[k][2] += (uint64_t)(uint32_t)(d2 * d2) * mx[k];
+      } else {
+        acc[k][0] += (uint32_t)(d0 * d0);
+        acc[k][1] += (uint32_t)(d1 * d1);
+        acc[k][2] += (uint32_t)(d2 * d2);
+      }
       if (best_out) best_out[y * g.w + x] = (int16_t)idx;
       if (scp_ptr) {
         const Buf<uint16_t> scp(scp_ptr, 6u * (uint32_t)g.w * (uint32_t)g.h);
@@ -401,12 +439,19 @@ __device__ __forceinline__ void cps_tile_dir(const CpsArgs& a, const Geom& g, in
   }
 }
 
-template <typename T>
+template <typename T, typename TM, bool MASKED>
 __global__ __launch_bounds__(256) void k_cps_fast(CpsArgs a) {
   __shared__ uint4 s_rec[kCpsLH * kCpsLW];
 
-  const FrameSummary* __restrict__ sum = a.sum;
-  if (!sum->in_range) return;
+  // Nothing to do when a pel exceeds the bit depth: the host reports that error.
+  const int32_t* gsrc;
+  if constexpr (MASKED) {
+    if (!a.msum->all_in_range) return;
+    gsrc = a.msum->gcd;
+  } else {
+    if (!a.sum->in_range) return;
+    gsrc = a.sum->gcd;
+  }
   const Geom g = a.g;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -415,11 +460,12 @@ __global__ __launch_bounds__(256) void k_cps_fast(CpsArgs a) {
   const int y0 = band * kCpsTileH;
   const int tx_begin = seg * a.sweep;
   const int tx_end = min(a.tiles_x, tx_begin + a.sweep);
-  const int32_t gcd[3] = {sum->gcd[0], sum->gcd[1], sum->gcd[2]};
+  const int32_t gcd[3] = {gsrc[0], gsrc[1], gsrc[2]};
 
-  // Per-lane row sums over the sweep (the host bounds sweep * max squared error below 2^32).
+  // Per-lane row sums over the sweep (the host bounds sweep * max squared error below 2^32;
+  // masked sums are kept in 64 bits).
   constexpr int K = kCpsTileH / 4;
-  uint32_t acc0[K][3], acc1[K][3];
+  CpsAcc<MASKED> acc0[K][3], acc1[K][3];
 #pragma unroll
   for (int k = 0; k < K; k++)
 #pragma unroll
@@ -427,8 +473,8 @@ __global__ __launch_bounds__(256) void k_cps_fast(CpsArgs a) {
 
   for (int tx = tx_begin; tx < tx_end; tx++) {
     const int x0 = tx * kCpsTileW;
-    cps_tile_dir<T, 0>(a, g, x0, y0, lane, wave, gcd, s_rec, acc0);
-    cps_tile_dir<T, 1>(a, g, x0, y0, lane, wave, gcd, s_rec, acc1);
+    cps_tile_dir<T, TM, MASKED, 0>(a, g, x0, y0, lane, wave, gcd, s_rec, acc0);
+    cps_tile_dir<T, TM, MASKED, 1>(a, g, x0, y0, lane, wave, gcd, s_rec, acc1);
   }
 
   // One partial per (segment, direction, component, row): reduce over the wave's 64 lanes.
@@ -439,9 +485,12 @@ __global__ __launch_bounds__(256) void k_cps_fast(CpsArgs a) {
     for (int c = 0; c < 3; c++) {
       const uint64_t v0 = wave_sum((uint64_t)acc0[k][c]);
       const uint64_t v1 = wave_sum((uint64_t)acc1[k][c]);
+      // The masked processor's R2T is the search of P1 - g in P0 (xIVPSNR.cpp:152-165): the
+      // second direction here.
+      const int d0 = MASKED ? 1 : 0, d1 = MASKED ? 0 : 1;
       if (lane == 0 && y < g.h) {
-        a.partials[(((int64_t)seg * 2 + 0) * 3 + c) * g.h + y] = v0;
-        a.partials[(((int64_t)seg * 2 + 1) * 3 + c) * g.h + y] = v1;
+        a.partials[(((int64_t)seg * 2 + d0) * 3 + c) * g.h + y] = v0;
+        a.partials[(((int64_t)seg * 2 + d1) * 3 + c) * g.h + y] = v1;
       }
     }
   }
@@ -566,10 +615,22 @@ hipError_t launch_cps_fast(const CpsArgs& a, hipStream_t stream) {
   const int bands = (a.g.h + kCpsTileH - 1) / kCpsTileH;
   if (a.tiles_x != (a.g.w + kCpsTileW - 1) / kCpsTileW) return hipErrorInvalidValue;
   dim3 grid(bands * a.nseg), block(256);
+  if (a.mask) {  // masked search: no optional outputs
+    if (a.scp0 || a.scp1 || a.best0 || a.best1 || !a.msum || (a.mask_bytes_per_sample != 1 && a.mask_bytes_per_sample != 2))
+      return hipErrorInvalidValue;
+    if (a.g.bytes_per_sample == 1) {
+      if (a.mask_bytes_per_sample == 1) hipLaunchKernelGGL((k_cps_fast<uint8_t, uint8_t, true>), grid, block, 0, stream, a);
+      else hipLaunchKernelGGL((k_cps_fast<uint8_t, uint16_t, true>), grid, block, 0, stream, a);
+    } else {
+      if (a.mask_bytes_per_sample == 1) hipLaunchKernelGGL((k_cps_fast<uint16_t, uint8_t, true>), grid, block, 0, stream, a);
+      else hipLaunchKernelGGL((k_cps_fast<uint16_t, uint16_t, true>), grid, block, 0, stream, a);
+    }
+    return hipGetLastError();
+  }
   if (a.g.bytes_per_sample == 1)
-    hipLaunchKernelGGL(k_cps_fast<uint8_t>, grid, block, 0, stream, a);
+    hipLaunchKernelGGL((k_cps_fast<uint8_t, uint8_t, false>), grid, block, 0, stream, a);
   else
-    hipLaunchKernelGGL(k_cps_fast<uint16_t>, grid, block, 0, stream, a);
+    hipLaunchKernelGGL((k_cps_fast<uint16_t, uint8_t, false>), grid, block, 0, stream, a);
   return hipGetLastError();
 }
 
@@ -581,8 +642,8 @@ void cps_fast_tile(int32_t* tile_w, int32_t* tile_h) {
 int cps_fast_blocks_per_cu(int bytes_per_sample) {
   int n = 0;
   hipError_t e = bytes_per_sample == 1
-                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast<uint8_t>, 256, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast<uint16_t>, 256, 0);
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast<uint8_t, uint8_t, false>, 256, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps_fast<uint16_t, uint8_t, false>, 256, 0);
   return e == hipSuccess && n > 0 ? n : 1;
 }
 

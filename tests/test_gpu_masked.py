@@ -95,11 +95,13 @@ def test_frame_masked_matches_oracle(engine, oracle, w, h, bd, cf, bdm, kind, er
 
 @pytest.mark.parametrize("w,h,cf,cfm,bd,bdm", [
     (256, 96, 420, 444, 10, 8), (256, 96, 444, 420, 10, 8), (256, 96, 444, 444, 8, 16), (252, 98, 444, 444, 12, 10),
-    (254, 98, 420, 420, 10, 8),
+    (254, 98, 420, 420, 10, 8), (130, 40, 444, 420, 8, 8),
 ])
 def test_chroma_formats_match_oracle(engine, oracle, w, h, cf, cfm, bd, bdm):
     """Every pairing of picture and mask chroma formats, at widths that take the statistics
-    kernel's four-pel loads (divisible by 4) and one that does not (254)."""
+    kernel's four-pel loads and the search kernels' aligned fills (divisible by 4) and ones that
+    do not (254, 130). 8- and 10-bit 4:4:4 pictures take the masked 4:4:4 fast search, 12-bit the
+    general one."""
     print(f"masked formats {w}x{h} cf{cf} cfm{cfm} bd{bd} bdm{bdm}", flush=True)
     pair = make_pair(w, h, bd, 61, chroma_format=cf, luma_offset=3.0)
     mask = make_mask(w, h, bdm, 61, chroma_format=cfm, kind="speckle")
