@@ -177,13 +177,14 @@ def test_search_parameters_match_oracle(engine, oracle, B, wts):
     assert r["r2t"] == want[0] and r["t2r"] == want[1]
 
 
-def test_full_mask_equals_unmasked_flow(engine):
+@pytest.mark.parametrize("cf", [420, 444])
+def test_full_mask_equals_unmasked_flow(engine, cf):
     w, h, bd = 320, 180, 10
-    pair = make_pair(w, h, bd, 51, luma_offset=5.0)
-    mask = make_mask(w, h, 8, 51, kind="full")
-    p0 = engine.picture(w, h, bd, 420, data=pair.ref)
-    p1 = engine.picture(w, h, bd, 420, data=pair.tst)
-    pm = engine.picture(w, h, 8, 420, data=mask)
+    pair = make_pair(w, h, bd, 51, chroma_format=cf, luma_offset=5.0)
+    mask = make_mask(w, h, 8, 51, chroma_format=cf, kind="full")
+    p0 = engine.picture(w, h, bd, cf, data=pair.ref)
+    p1 = engine.picture(w, h, bd, cf, data=pair.tst)
+    pm = engine.picture(w, h, 8, cf, data=mask)
     for erp in (False, True):
         prm = default_params(erp=erp)
         prm.cps_int32_rows = 0
