@@ -115,6 +115,30 @@ def test_chroma_formats_match_oracle(engine, oracle, w, h, cf, cfm, bd, bdm):
         assert r[k] == o[k], k
 
 
+@pytest.mark.parametrize("bd,cf,g", [(8, 420, [200, -150, 90, 0]), (10, 420, [-300, 250, -280, 0]),
+                                     (10, 444, [300, -260, 120, 0])])
+def test_large_given_gcd_matches_oracle(engine, oracle, bd, cf, g):
+    """A caller-given GCD far beyond the unnoticeable bound: tested values well outside [0, max]
+    (no clipping, xc_CLIP_CURR_TST_RANGE = false) still fit the fast kernels' key bounds, so the
+    fast masked search runs and must equal the oracle's search."""
+    w, h = 200, 72
+    pair = make_pair(w, h, bd, 81, chroma_format=cf, luma_offset=1.0)
+    mask = make_mask(w, h, 8, 81, chroma_format=cf, kind="speckle")
+    P0, P1 = oracle.unpack(pair.ref, w, h, bd, cf), oracle.unpack(pair.tst, w, h, bd, cf)
+    M = oracle.unpack(mask, w, h, 8, cf)[0]
+    p0 = engine.picture(w, h, bd, cf, data=pair.ref)
+    p1 = engine.picture(w, h, bd, cf, data=pair.tst)
+    pm = engine.picture(w, h, 8, cf, data=mask)
+    r = engine.calc_ivpsnr_masked(p0, p1, pm, g, -1, default_params())
+    nnm = int((M != 0).sum())
+    want = []
+    for rows in (oracle.cps_rows_masked(P1, P0, M, [-v for v in g]), oracle.cps_rows_masked(P0, P1, M, g)):
+        err = [float(rows[c].sum()) for c in range(3)]
+        q = [oracle.psnr_from_masked_ssd(e if e > 0 else 1.0, nnm, bd, 8) for e in err]
+        want.append((((q[0] * 4 + q[1] * 1) + q[2] * 1) + 0.0 * 0) * (1.0 / 6))
+    assert r["r2t"] == want[0] and r["t2r"] == want[1]
+
+
 def test_frames_masked_equal_single_frames(engine):
     """A masked frame set (qmiv_frames_masked, two streams) gives each frame exactly what the
     single-frame call gives, over frames of different sizes, formats, masks and bit depths; a
