@@ -282,26 +282,117 @@ __global__ __launch_bounds__(256) void k_scp_gatherf(ScpFArgs a) {
   }
 }
 
-// Adjoint of k_scp_gatherf: grad_src_c(clamp(p + o)) += coef * scale * g_c(p) where the clamp to
-// [0, maxv] is not active.
+// Adjoint of k_scp_gatherf: grad_src_c(clamp(p + o(p))) += coef * scale * g_c(p) wherever the clamp
+// to [0, maxv] is not active. Computed as a gather, one thread per source pel q: every p whose
+// winner clamp(p + o(p)) is q lies within B of q (the window clamp only moves a target towards
+// the border it crossed), so q visits its (2B + 1)^2 neighbours in raster order and adds the
+// contributions of those that chose it. Each q is written once: no atomics, and the same sums in
+// the same order on every run. A target clamped into the picture lies on its first or last row or
+// column, so a q inside those is chosen by p = q + e exactly when p's window index is
+// (B - ey) n + (B - ex): one compare, no division.
+struct ScpAdj {
+  int32_t y, x, B, n;
+  bool border;
+  __device__ __forceinline__ bool chose(const ScpFArgs& a, int ey, int ex, int idx) const {
+    if (!border) return idx == (B - ey) * n + (B - ex);
+    if (idx < 0) return false;
+    const int sy = clampi(y + ey + idx / n - B, 0, a.h - 1);
+    const int sx = clampi(x + ex + idx % n - B, 0, a.w - 1);
+    return sy == y && sx == x;
+  }
+};
+
+// The clamp test of q's three shifted values; false when none is open (nothing to add).
+__device__ __forceinline__ bool scp_open(const ScpFArgs& a, int64_t q, int64_t plane, bool (&open)[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double v = (double)a.src[c * plane + q] * a.scale - a.sgn * a.gcd[c];
+    open[c] = v >= 0.0 && v <= (double)a.maxv;
+  }
+  return open[0] || open[1] || open[2];
+}
+
+// Any search range: the neighbours straight from memory.
 __global__ __launch_bounds__(256) void k_scp_scatterf(ScpFArgs a, const float* __restrict__ g, double coef,
                                                       float* __restrict__ grad_src) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t plane = (int64_t)a.w * a.h;
   if (i >= plane) return;
-  const int y = (int)(i / a.w);
-  const int x = (int)(i - (int64_t)y * a.w);
-  const int n = 2 * a.B + 1;
-  const int idx = a.best[i];
-  const int sy = clampi(y + idx / n - a.B, 0, a.h - 1);
-  const int sx = clampi(x + idx % n - a.B, 0, a.w - 1);
-  const int64_t s = (int64_t)sy * a.w + sx;
+  ScpAdj q;
+  q.y = (int)((uint32_t)i / (uint32_t)a.w);  // planes hold fewer than 2^31 pels
+  q.x = (int)((uint32_t)i - (uint32_t)q.y * (uint32_t)a.w);
+  q.B = a.B;
+  q.n = 2 * a.B + 1;
+  q.border = q.y == 0 || q.y == a.h - 1 || q.x == 0 || q.x == a.w - 1;
+  bool open[3];
+  if (!scp_open(a, i, plane, open)) return;
+  const double k = coef * a.scale;
+  float acc[3] = {0.f, 0.f, 0.f};
+  bool hit = false;
+  for (int py = max(q.y - q.B, 0); py <= min(q.y + q.B, a.h - 1); py++)
+    for (int px = max(q.x - q.B, 0); px <= min(q.x + q.B, a.w - 1); px++) {
+      const int64_t p = (int64_t)py * a.w + px;
+      if (!q.chose(a, py - q.y, px - q.x, a.best[p])) continue;
+      hit = true;
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const double v = (double)a.src[c * plane + s] * a.scale - a.sgn * a.gcd[c];
-    if (v >= 0.0 && v <= (double)a.maxv)
-      atomicAdd(&grad_src[c * plane + s], (float)(coef * a.scale * (double)g[c * plane + i]));
+      for (int c = 0; c < 3; c++)
+        if (open[c]) acc[c] += (float)(k * (double)g[c * plane + p]);
+    }
+  if (!hit) return;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    if (open[c]) grad_src[c * plane + i] += acc[c];
+}
+
+// B = 2: a 64 x 4 tile of q per workgroup, its neighbourhood's winners and incoming gradients
+// staged in LDS (pels outside the picture hold the index -1, chosen by nobody), so that the
+// neighbours' reads in the data-dependent branches are LDS reads. grid = (ceil(w / 64), ceil(h / 4)).
+constexpr int kAdjTW = 64, kAdjTH = 4, kAdjLW = kAdjTW + 4, kAdjLH = kAdjTH + 4;
+__global__ __launch_bounds__(256) void k_scp_scatterf_b2(ScpFArgs a, const float* __restrict__ g, double coef,
+                                                         float* __restrict__ grad_src) {
+  __shared__ int32_t s_best[kAdjLH * kAdjLW];
+  __shared__ float s_g[3][kAdjLH * kAdjLW];
+  const int64_t plane = (int64_t)a.w * a.h;
+  const int x0 = blockIdx.x * kAdjTW, y0 = blockIdx.y * kAdjTH;
+  for (int k = threadIdx.x; k < kAdjLH * kAdjLW; k += 256) {
+    const int ly = k / kAdjLW, lx = k - ly * kAdjLW;
+    const int py = y0 - 2 + ly, px = x0 - 2 + lx;
+    const bool in = py >= 0 && py < a.h && px >= 0 && px < a.w;
+    const int64_t p = in ? (int64_t)py * a.w + px : 0;
+    s_best[k] = in ? (int32_t)a.best[p] : -1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) s_g[c][k] = in ? g[c * plane + p] : 0.f;
   }
+  __syncthreads();
+  const int tx = threadIdx.x & (kAdjTW - 1), ty = threadIdx.x / kAdjTW;
+  ScpAdj q;
+  q.y = y0 + ty;
+  q.x = x0 + tx;
+  if (q.y >= a.h || q.x >= a.w) return;
+  q.B = 2;
+  q.n = 5;
+  q.border = q.y == 0 || q.y == a.h - 1 || q.x == 0 || q.x == a.w - 1;
+  const int64_t i = (int64_t)q.y * a.w + q.x;
+  bool open[3];
+  if (!scp_open(a, i, plane, open)) return;
+  const double k = coef * a.scale;
+  float acc[3] = {0.f, 0.f, 0.f};
+  bool hit = false;
+#pragma unroll
+  for (int ey = -2; ey <= 2; ey++)
+#pragma unroll
+    for (int ex = -2; ex <= 2; ex++) {
+      const int l = (ty + 2 + ey) * kAdjLW + tx + 2 + ex;
+      if (!q.chose(a, ey, ex, s_best[l])) continue;
+      hit = true;
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        if (open[c]) acc[c] += (float)(k * (double)s_g[c][l]);
+    }
+  if (!hit) return;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    if (open[c]) grad_src[c * plane + i] += acc[c];
 }
 
 }  // namespace
@@ -358,7 +449,12 @@ hipError_t launch_scp_gatherf(const ScpFArgs& a, hipStream_t stream) {
 hipError_t launch_scp_scatterf(const ScpFArgs& a, const float* g, double coef, float* grad_src, hipStream_t stream) {
   const int64_t n = (int64_t)a.w * a.h;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scp_scatterf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, g, coef, grad_src);
+  if (a.B == 2) {
+    const dim3 grid((unsigned)((a.w + kAdjTW - 1) / kAdjTW), (unsigned)((a.h + kAdjTH - 1) / kAdjTH));
+    hipLaunchKernelGGL(k_scp_scatterf_b2, grid, dim3(256), 0, stream, a, g, coef, grad_src);
+  } else {
+    hipLaunchKernelGGL(k_scp_scatterf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, g, coef, grad_src);
+  }
   return hipGetLastError();
 }
 
