@@ -134,7 +134,8 @@ def engine_grads(engine, kind, tst, ref, **kw):
     elif kind == "msssim":
         v, _ = engine.msssim_backward(w, h, c, t.data_ptr(), r.data_ptr(), MAXV, gt.data_ptr(), gr.data_ptr())
     else:
-        v = engine.ivssim_backward(w, h, t.data_ptr(), r.data_ptr(), MAXV, None, gt.data_ptr(), gr.data_ptr())
+        v = engine.ivssim_backward(w, h, t.data_ptr(), r.data_ptr(), MAXV, kw.get("params"), gt.data_ptr(),
+                                   gr.data_ptr())
     return v, gt.double().cpu().numpy(), gr.double().cpu().numpy()
 
 
@@ -188,18 +189,21 @@ def test_msssim_gradient_against_fp64_autograd(engine):
         assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
 
 
-def test_ivssim_gradient_against_fp64_autograd(engine):
+def test_ivssim_gradient_against_fp64_autograd(engine, B=2):
+    """B = 2 takes the adjoint's LDS-tiled gather, other search ranges its general form."""
     c, h, w = 3, 48, 64
     tst, ref = (quantised(a) for a in field((c, h, w), 4))
-    iv, gt, gr = engine_grads(engine, "ivssim", tst, ref)
+    prm = default_params()
+    prm.search_range = B
+    iv, gt, gr = engine_grads(engine, "ivssim", tst, ref, params=prm)
     q0 = np.rint(np.float32(tst) * MAXV).astype(np.int64)
     q1 = np.rint(np.float32(ref) * MAXV).astype(np.int64)
     gcd = gcd_of(q0, q1)
-    best0 = cps_winners(q0, q1, gcd, +1)
-    best1 = cps_winners(q1, q0, gcd, -1)
+    best0 = cps_winners(q0, q1, gcd, +1, B)
+    best1 = cps_winners(q1, q0, gcd, -1, B)
     t = torch.tensor(tst, dtype=torch.float32).double().requires_grad_()
     r = torch.tensor(ref, dtype=torch.float32).double().requires_grad_()
-    vr, t2r, r2t = ivssim_ref(t, r, best0, best1, gcd)
+    vr, t2r, r2t = ivssim_ref(t, r, best0, best1, gcd, B)
     np.testing.assert_allclose([iv["ivssim"], iv["t2r"], iv["r2t"]], [vr.item(), t2r.item(), r2t.item()], rtol=1e-6)
     vr.backward()
     for got, want in ((gt, t.grad.numpy()), (gr, r.grad.numpy())):
@@ -257,6 +261,8 @@ CHECKS = [
     ("ssim_gradient_fp64_autograd[CS]", test_ssim_gradient_against_fp64_autograd, {"calc_l": False}),
     ("msssim_gradient_fp64_autograd", test_msssim_gradient_against_fp64_autograd, {}),
     ("ivssim_gradient_fp64_autograd", test_ivssim_gradient_against_fp64_autograd, {}),
+    ("ivssim_gradient_fp64_autograd[B=1]", test_ivssim_gradient_against_fp64_autograd, {"B": 1}),
+    ("ivssim_gradient_fp64_autograd[B=3]", test_ivssim_gradient_against_fp64_autograd, {"B": 3}),
     ("finite_differences[ssim]", test_gradient_against_finite_differences, {"kind": "ssim", "shape": (3, 40, 48)}),
     ("finite_differences[msssim]", test_gradient_against_finite_differences,
      {"kind": "msssim", "shape": (1, 180, 184)}),
