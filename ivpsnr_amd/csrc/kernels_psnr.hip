@@ -138,12 +138,32 @@ __global__ __launch_bounds__(1024) void k_summary(Geom g, const int64_t* __restr
   int64_t sd[3] = {0, 0, 0};
   uint64_t ssd[3] = {0, 0, 0};
   uint32_t m0 = 0, m1 = 0;
-  for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
-    const int c = r < g.h ? 0 : (r < g.h + g.ch ? 1 : 2);
-    sd[c] += row_sd[r];
-    ssd[c] += row_ssd[r];
-    m0 = max(m0, row_max[r]);
-    m1 = max(m1, row_max[nrows + r]);
+  // Raw buffer loads, the offset of a row past nrows replaced by one past every array (frames are
+  // below 2 GiB, so 2^31 is): such loads return 0 and need no branch, so four rows' loads per lane
+  // issue together.
+  const uint32_t nr = (uint32_t)nrows;
+  const __amdgpu_buffer_rsrc_t rsd = __builtin_amdgcn_make_buffer_rsrc(const_cast<int64_t*>(row_sd), 0, (int)(8 * nr), kBufDword3);
+  const __amdgpu_buffer_rsrc_t rss = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(row_ssd), 0, (int)(8 * nr), kBufDword3);
+  const __amdgpu_buffer_rsrc_t rmx = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(row_max), 0, (int)(8 * nr), kBufDword3);
+  for (uint32_t r0 = 0; r0 < nr; r0 += 4 * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t r = r0 + (uint32_t)k * blockDim.x + threadIdx.x;
+      const bool ok = r < nr;
+      const uint32_t o8 = ok ? 8u * r : 0x80000000u, o4 = ok ? 4u * r : 0x80000000u;
+      const int c = (int)r < g.h ? 0 : ((int)r < g.h + g.ch ? 1 : 2);
+      const int64_t vsd = __builtin_bit_cast(int64_t, __builtin_amdgcn_raw_buffer_load_b64(rsd, o8, 0, 0));
+      const uint64_t vss = __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(rss, o8, 0, 0));
+      const uint32_t v0 = __builtin_amdgcn_raw_buffer_load_b32(rmx, o4, 0, 0);
+      const uint32_t v1 = __builtin_amdgcn_raw_buffer_load_b32(rmx, ok ? 4u * (nr + r) : 0x80000000u, 0, 0);
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) {
+        sd[cc] += cc == c ? vsd : 0;
+        ssd[cc] += cc == c ? vss : 0;
+      }
+      m0 = max(m0, v0);
+      m1 = max(m1, v1);
+    }
   }
 #pragma unroll
   for (int c = 0; c < 3; c++) {
