@@ -208,6 +208,20 @@ qmiv_status qmiv_profile_read(qmiv_ctx* ctx, qmiv_profile* out, int32_t reset);
 qmiv_status qmiv_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm,
                        uint32_t metrics, qmiv_frame_result* out);
 
+/* qmiv_frame without allocation: no counterpart in the reference. For callers that cannot
+   allocate memory while they run (serving loops that forbid hipMalloc, harnesses that pre-reserve
+   all memory). Same flow, arguments, results and errors as qmiv_frame, on the context's stream,
+   but the call allocates no device or pinned host memory: the frame's device scratch (and, with
+   QMIV_IVSSIM, its two shift-compensated pictures) lies in one fixed 64 MiB block per device,
+   reserved when the library is loaded onto the device and held until the process ends, and its
+   results return through a 2 MiB pinned block allocated by qmiv_create. A frame needing more of
+   either returns QMIV_ERR_UNSUPPORTED with the sizes in qmiv_last_error (e.g. 4096 x 2048 with
+   IV-SSIM; every metric but IV-SSIM fits 4096 x 2048, all six fit 1920 x 1080). The block keeps
+   the last call's contents and nothing in it needs clearing. Every context on one device shares
+   that device's block: calls on different contexts of one device must not overlap (they race). */
+qmiv_status qmiv_frame_noalloc(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm,
+                               uint32_t metrics, qmiv_frame_result* out);
+
 /* The same flow for n frame pairs (sizes may differ between pairs): every frame's kernels are
    enqueued back to back on the context's stream, then one wait, then the host finalisation of
    each. Replaces the reference application's frame loop (apps/QMIV/src/xAppQMIV.cpp:666-748).

@@ -46,6 +46,9 @@ qmiv_status fail(qmiv_status st, const std::string& msg) {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Pinned results block of the allocation-free frame entry point, allocated with each context.
+constexpr size_t kFixedResultsBytes = (size_t)2 << 20;
+
 }  // namespace
 
 qmiv_status qmiv::host_fail(qmiv_status st, const std::string& msg) { return fail(st, msg); }
@@ -324,6 +327,16 @@ struct FrameWork {
   int16_t* best0 = nullptr;      // optional per-pel winner indices of the two directions
   int16_t* best1 = nullptr;
 };
+
+// The device work a frame's metrics need (the application's per-frame flow, xAppQMIV.cpp:686-704).
+FrameWork frame_work(uint32_t metrics) {
+  FrameWork fw;
+  fw.need_gcd = (metrics & (QMIV_IVPSNR | QMIV_IVSSIM)) != 0;
+  fw.need_cps = (metrics & QMIV_IVPSNR) != 0;
+  fw.need_scp = (metrics & QMIV_IVSSIM) != 0;
+  fw.ssim_metrics = metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM | kSsimNoLuminance);
+  return fw;
+}
 
 // Everything decided on the host before a frame's kernels are enqueued.
 struct FramePlan {
@@ -1136,7 +1149,17 @@ qmiv_status qmiv_create(int32_t device, qmiv_ctx** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
+  // The allocation-free entry point's buffers: the device's fixed scratch (its address resolved with
+  // this device current) and the pinned block its results return through.
+  if (e == hipSuccess) {
+    ctx->fixed_err = fixed_scratch_setup(device, &ctx->d_fixed);
+    if (ctx->fixed_err == hipSuccess) ctx->fixed_bytes = fixed_scratch_bytes();
+    else ctx->d_fixed = nullptr;
+    e = hipHostMalloc(&ctx->h_fixed, kFixedResultsBytes, hipHostMallocDefault);
+    if (e != hipSuccess) ctx->h_fixed = nullptr;
+  }
   if (e != hipSuccess) {
+    if (ctx->h_fixed) (void)hipHostFree(ctx->h_fixed);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
@@ -1168,6 +1191,7 @@ void qmiv_destroy(qmiv_ctx* ctx) {
       delete q;
     }
   if (ctx->h_results) (void)hipHostFree(ctx->h_results);
+  if (ctx->h_fixed) (void)hipHostFree(ctx->h_fixed);
   for (auto& pr : ctx->ev_pool) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
@@ -1268,11 +1292,7 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
   if (prm_in) prm = *prm_in;
   else qmiv_params_default(&prm);
   QMIV_HIP(hipSetDevice(ctx->device));
-  FrameWork fw;
-  fw.need_gcd = (metrics & (QMIV_IVPSNR | QMIV_IVSSIM)) != 0;
-  fw.need_cps = (metrics & QMIV_IVPSNR) != 0;
-  fw.need_scp = (metrics & QMIV_IVSSIM) != 0;
-  fw.ssim_metrics = metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM | kSsimNoLuminance);
+  const FrameWork fw = frame_work(metrics);
   // Plan every frame, then lay their scratch and result blocks end to end.
   std::vector<FramePlan> plans((size_t)n);
   std::vector<size_t> doff((size_t)n), hoff((size_t)n);
@@ -1342,6 +1362,42 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
   }
   QMIV_TRY(wait_stream(ctx));
   return first_err;
+}
+
+qmiv_status qmiv_frame_noalloc(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm_in,
+                               uint32_t metrics, qmiv_frame_result* out) {
+  if (!ctx || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_pair(p0, p1));
+  if (!ctx->d_fixed)
+    return fail(QMIV_ERR_HIP, std::string("fixed scratch unavailable on this device: ") + hipGetErrorString(ctx->fixed_err));
+  if (!ctx->h_fixed) return fail(QMIV_ERR_HIP, "no pinned results block on this context");
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  const FrameWork fw = frame_work(metrics);
+  const Geom& g = p0->g;
+  const FramePlan P = plan_frame(ctx, g, prm, fw);
+  // Device block: the frame's scratch, then (IV-SSIM) the two shift-compensated pictures.
+  const size_t scratch = align_up(P.L.total, 256);
+  const size_t scp_bytes = fw.need_scp ? 6 * (size_t)g.w * g.h * sizeof(uint16_t) : 0;
+  if (scratch + scp_bytes > ctx->fixed_bytes)
+    return fail(QMIV_ERR_UNSUPPORTED, "frame needs " + std::to_string(scratch + scp_bytes) +
+                                          " bytes of device scratch; the allocation-free entry point holds " +
+                                          std::to_string(ctx->fixed_bytes));
+  if (P.L.results_bytes > kFixedResultsBytes)
+    return fail(QMIV_ERR_UNSUPPORTED, "frame needs " + std::to_string(P.L.results_bytes) +
+                                          " bytes of results; the allocation-free entry point holds " +
+                                          std::to_string(kFixedResultsBytes));
+  char* dbase = (char*)ctx->d_fixed;
+  uint16_t* scp = fw.need_scp ? (uint16_t*)(dbase + scratch) : nullptr;
+  // Nothing in the block needs clearing: every kernel of the flow writes all it reads later.
+  const qmiv_status st =
+      enqueue_frame(ctx, p0, p1, prm, fw, P, dbase, (char*)ctx->h_fixed, scp, ctx->stream);
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  harvest_events(ctx);
+  QMIV_TRY(st);
+  return finalize_frame(ctx, g, prm, metrics, fw, P.L, (const char*)ctx->h_fixed, out);
 }
 
 qmiv_status qmiv_calc_gcd(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm_in,

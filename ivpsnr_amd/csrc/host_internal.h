@@ -57,6 +57,13 @@ struct qmiv_ctx {
   // Shift-compensated pictures (IV-SSIM), 2 x [3][h][w] uint16.
   void* d_scp = nullptr;
   size_t scp_bytes = 0;
+  // Allocation-free frame entry point (qmiv_frame_noalloc): the device's fixed scratch (shared by
+  // every context on the device; null when its set-up failed, with the HIP error kept) and a pinned
+  // results block of kFixedResultsBytes allocated with the context.
+  void* d_fixed = nullptr;
+  size_t fixed_bytes = 0;
+  hipError_t fixed_err = hipSuccess;
+  void* h_fixed = nullptr;
   int32_t num_cus = 256;
   int32_t cps_blocks_per_cu[2][2] = {{0, 0}, {0, 0}};  // [4:4:4 kernel, 4:2:0 kernel][u8, u16], queried on first use
   // Kernel timing: event pairs recorded around launches since the last harvest.

@@ -271,4 +271,10 @@ constexpr int kCpsTileW = 64;   // 64 lanes x 1 column
 constexpr int kCpsTileH = QMIV_CPS_TILE_H;  // 4 waves x (kCpsTileH / 4) rows
 constexpr int kGenericCols = 256;
 
+// Fixed device scratch of the allocation-free frame entry point (fixed_scratch.hip): its size, and
+// its device address on `device`, which must be the current device. The address is resolved once
+// per device and kept; later calls return it.
+size_t fixed_scratch_bytes();
+hipError_t fixed_scratch_setup(int device, void** ptr);
+
 }  // namespace qmiv

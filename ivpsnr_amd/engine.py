@@ -165,6 +165,15 @@ class Engine:
         r.status = st
         return r
 
+    def frame_noalloc(self, p0: Picture, p1: Picture, metrics: int, params: Params | None = None) -> FrameResult:
+        """qmiv_frame through the allocation-free entry point (the device's fixed scratch block)."""
+        r = FrameResult()
+        st = lib().qmiv_frame_noalloc(self.ctx, p0.handle, p1.handle, C.byref(params) if params else None, metrics,
+                                      C.byref(r))
+        check(st, "qmiv_frame_noalloc")
+        r.status = st
+        return r
+
     def frames(self, pairs, metrics: int, params: Params | None = None, raise_on_error: bool = True):
         """All (p0, p1) pairs enqueued back to back on the stream, one wait (qmiv_frames)."""
         n = len(pairs)
