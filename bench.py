@@ -230,25 +230,11 @@ def run_grad(args):
     random field, a copy shifted by up to 2 pels per 32 x 32 block with N(0, 0.02) noise."""
     import torch  # first: the engine library then shares torch's HIP runtime (one runtime per process)
     from ivpsnr_amd import Engine
+    from ivpsnr_amd.synth import grad_batch
 
     n, c, h, w = 8, 3, 1024, 1024
     maxv = 1023.0
-    g = torch.Generator(device="cuda").manual_seed(5)
-    yy = torch.arange(h, device="cuda", dtype=torch.float32)[:, None] / h
-    xx = torch.arange(w, device="cuda", dtype=torch.float32)[None, :] / w
-    ref = torch.zeros(n, c, h, w, device="cuda")
-    for _ in range(6):
-        a = torch.rand(n, c, 1, 1, device="cuda", generator=g) * 0.1 + 0.05
-        fx = torch.rand(n, c, 1, 1, device="cuda", generator=g) * 3.5 + 0.5
-        fy = torch.rand(n, c, 1, 1, device="cuda", generator=g) * 3.5 + 0.5
-        ph = torch.rand(n, c, 1, 1, device="cuda", generator=g) * 6.283
-        ref += a * torch.sin(6.283 * (fx * xx + fy * yy) + ph)
-    ref = (ref + 0.5).clamp(0, 1)
-    sh = torch.randint(-2, 3, (2, h // 32, w // 32), device="cuda", generator=g)
-    iy = (torch.arange(h, device="cuda")[:, None] + sh[0].repeat_interleave(32, 0).repeat_interleave(32, 1)).clamp(0, h - 1)
-    ix = (torch.arange(w, device="cuda")[None, :] + sh[1].repeat_interleave(32, 0).repeat_interleave(32, 1)).clamp(0, w - 1)
-    tst = (ref[:, :, iy, ix] + 0.02 * torch.randn(n, c, h, w, device="cuda", generator=g)).clamp(0, 1).contiguous()
-    ref = ref.contiguous()
+    tst, ref = grad_batch(n, c, h, w, seed=5)
     gt, gr = torch.empty_like(tst), torch.empty_like(ref)
     torch.cuda.synchronize()
     eng = Engine(0)

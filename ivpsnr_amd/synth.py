@@ -24,7 +24,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-__all__ = ["FramePair", "make_pair", "config_pair", "sha256", "CONFIGS"]
+__all__ = ["FramePair", "make_pair", "make_mask", "grad_batch", "config_pair", "sha256", "CONFIGS"]
 
 
 @dataclass
@@ -169,6 +169,34 @@ def make_mask(width: int, height: int, bit_depth: int, seed: int, *, chroma_form
     c = y[::2, ::2] if chroma_format == 420 else y
     dt = np.uint8 if bit_depth <= 8 else np.uint16
     return np.concatenate([y.astype(dt).ravel(), c.astype(dt).ravel(), c.astype(dt).ravel()])
+
+
+def grad_batch(n: int = 8, planes: int = 3, height: int = 1024, width: int = 1024, seed: int = 5):
+    """The differentiable workload's float pictures: (tst, ref), each an (n, planes, h, w) float32
+    CUDA tensor in [0, 1]. ref is a smooth random field (six sinusoids per plane around 0.5); tst is
+    ref shifted by up to 2 pels per 32 x 32 block (clamp-to-edge) with N(0, 0.02) noise. Drawn with
+    torch's CUDA generator (imported here; the caller imports torch before the engine)."""
+    import torch
+    dev = "cuda"
+    g = torch.Generator(device=dev).manual_seed(seed)
+    yy = torch.arange(height, device=dev, dtype=torch.float32)[:, None] / height
+    xx = torch.arange(width, device=dev, dtype=torch.float32)[None, :] / width
+    ref = torch.zeros(n, planes, height, width, device=dev)
+    for _ in range(6):
+        a = torch.rand(n, planes, 1, 1, device=dev, generator=g) * 0.1 + 0.05
+        fx = torch.rand(n, planes, 1, 1, device=dev, generator=g) * 3.5 + 0.5
+        fy = torch.rand(n, planes, 1, 1, device=dev, generator=g) * 3.5 + 0.5
+        ph = torch.rand(n, planes, 1, 1, device=dev, generator=g) * 6.283
+        ref += a * torch.sin(6.283 * (fx * xx + fy * yy) + ph)
+    ref = (ref + 0.5).clamp(0, 1)
+    nby, nbx = (height + 31) // 32, (width + 31) // 32
+    sh = torch.randint(-2, 3, (2, nby, nbx), device=dev, generator=g)
+    dy = sh[0].repeat_interleave(32, 0).repeat_interleave(32, 1)[:height, :width]
+    dx = sh[1].repeat_interleave(32, 0).repeat_interleave(32, 1)[:height, :width]
+    iy = (torch.arange(height, device=dev)[:, None] + dy).clamp(0, height - 1)
+    ix = (torch.arange(width, device=dev)[None, :] + dx).clamp(0, width - 1)
+    tst = (ref[:, :, iy, ix] + 0.02 * torch.randn(n, planes, height, width, device=dev, generator=g)).clamp(0, 1)
+    return tst.contiguous(), ref.contiguous()
 
 
 # Workload presets (frame size, bit depth, frames, generator knobs).

@@ -8,6 +8,8 @@ the reference computes on them are stored.
 
     python tests/golden/make_golden.py            # metric fixtures
     python tests/golden/make_golden.py --png      # PNG decoding (oracle/_ref/libspng_ref.so)
+    python tests/golden/make_golden.py --multiview   # 24-view frame sets (4096x2048 ERP, 1920x1080, 2048x1088)
+    python tests/golden/make_golden.py --ssimsets    # SSIM-family frame sets (16 x 1080p10, mixed sizes)
 """
 from __future__ import annotations
 
@@ -149,6 +151,62 @@ def masked_fixtures():
     return cases
 
 
+# Config 4: 24 views of uneven size, 10-bit 4:2:0; view i is the synthetic pair of seed 11 + i and
+# its frame f that pair circularly shifted by 64 f samples (the benchmark's multi-view frame sets).
+MULTIVIEW = [(4096, 2048, True)] * 8 + [(1920, 1080, False)] * 8 + [(2048, 1088, False)] * 8
+MULTIVIEW_FRAMES = 2
+MULTIVIEW_METRICS = "PSNR,WSPSNR,IVPSNR,IVSSIM"
+
+
+def multiview_frame(pair, f):
+    return np.roll(pair.ref, 64 * f), np.roll(pair.tst, 64 * f)
+
+
+def multiview_fixtures():
+    """The 24 views' first two frame sets (IV-PSNR, WS-PSNR, PSNR, IV-SSIM) by the reference."""
+    views = []
+    with tempfile.TemporaryDirectory() as td:
+        for i, (w, h, erp) in enumerate(MULTIVIEW):
+            pair = make_pair(w, h, 10, seed=11 + i)
+            f0, f1 = os.path.join(td, "p0.yuv"), os.path.join(td, "p1.yuv")
+            sh = []
+            with open(f0, "wb") as a, open(f1, "wb") as b:
+                for f in range(MULTIVIEW_FRAMES):
+                    r, t = multiview_frame(pair, f)
+                    a.write(r.tobytes())
+                    b.write(t.tobytes())
+                    sh.append([sha256(r), sha256(t)])
+            out = run_ref(w, h, 10, 420, f0, f1, MULTIVIEW_METRICS, erp, ["-rows"])
+            views.append({"width": w, "height": h, "bit_depth": 10, "erp": erp, "seed": 11 + i, "sha256": sh,
+                          "frames": [compact_frame(fr, keep_rows=False) for fr in out["frames"]]})
+            print("multiview", i, w, h, file=sys.stderr)
+    return {"metrics": MULTIVIEW_METRICS, "views": views}
+
+
+# SSIM-family frame sets: 16 frames 1920x1080 10-bit (seeds 101..116), and five frames of mixed
+# sizes and bit depths (seeds 201..205), every metric.
+SSIM_SET_1080 = [(1920, 1080, 10, 101 + f) for f in range(16)]
+SSIM_SET_MIXED = [(1920, 1080, 10, 201), (640, 360, 8, 202), (2048, 1088, 10, 203), (1280, 720, 10, 204),
+                  (352, 288, 8, 205)]
+
+
+def ssim_set_fixtures():
+    sets = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, frames, ml in (("1080p10", SSIM_SET_1080, "SSIM,MSSSIM,IVSSIM"), ("mixed", SSIM_SET_MIXED, "All")):
+            out = []
+            for (w, h, bd, seed) in frames:
+                pair = make_pair(w, h, bd, seed=seed)
+                f0, f1 = write_pairs(td, [pair])
+                fr = run_ref(w, h, bd, 420, f0, f1, ml, False, ["-rows"])["frames"][0]
+                fr = {k: v for k, v in compact_frame(fr, keep_rows=False).items() if k != "ssim_rows_cr"}
+                out.append({"width": w, "height": h, "bit_depth": bd, "seed": seed, "sha256_ref": sha256(pair.ref),
+                            "sha256_tst": sha256(pair.tst), "frame": fr})
+                print("ssim set", name, w, h, bd, seed, file=sys.stderr)
+            sets[name] = {"metrics": ml, "frames": out}
+    return sets
+
+
 def png_fixtures():
     """The reference's PNG read step (libspng as xSeqPNG calls it, oracle/spng_ref.c) on the encoded
     cases of tests/png_writer.py: decode status and the SHA-256 of the R, G, B planes."""
@@ -180,6 +238,14 @@ def main():
     if "--masked" in sys.argv:
         with open(os.path.join(HERE, "masked.json"), "w") as f:
             json.dump(masked_fixtures(), f)
+        return
+    if "--multiview" in sys.argv:
+        with open(os.path.join(HERE, "multiview24.json"), "w") as f:
+            json.dump(multiview_fixtures(), f, separators=(",", ":"))
+        return
+    if "--ssimsets" in sys.argv:
+        with open(os.path.join(HERE, "ssim_sets.json"), "w") as f:
+            json.dump(ssim_set_fixtures(), f, separators=(",", ":"))
         return
     if "--colour" in sys.argv:
         with open(os.path.join(HERE, "colorspace.json"), "w") as f:
