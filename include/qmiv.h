@@ -129,10 +129,13 @@ qmiv_status qmiv_pic_wrap(qmiv_ctx* ctx, int32_t width, int32_t height, int32_t 
 /* Frees an engine-owned picture (waits for outstanding device work); valid after qmiv_destroy. */
 void qmiv_pic_destroy(qmiv_pic* pic);
 void* qmiv_pic_device_ptr(qmiv_pic* pic);
-/* Copies one packed frame from host memory (qmiv_frame_bytes bytes) onto the device, ordered on
-   the context's stream. Pinned host memory is copied asynchronously; pageable memory is staged
-   through the context's pinned buffer first. Replaces xSeqBase::readFrame's unpack step
-   (libs/pmbb-core/src/xSeq.cpp:179-249): the engine keeps the packed layout. */
+/* Copies one packed frame from host memory (qmiv_frame_bytes bytes) onto the device with one
+   hipMemcpyAsync ordered on the context's stream (work enqueued afterwards sees the frame). From
+   pinned host memory the copy is asynchronous: keep the memory unchanged until the stream has
+   passed it (qmiv_sync). From pageable memory the HIP runtime stages the data through its own
+   pinned buffers and the call returns once the source may be reused. Replaces
+   xSeqBase::readFrame's unpack step (libs/pmbb-core/src/xSeq.cpp:179-249): the engine keeps the
+   packed layout. */
 qmiv_status qmiv_pic_upload(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes);
 /* Copies a device picture back to host memory (packed layout). */
 qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, size_t bytes);
@@ -179,11 +182,12 @@ void qmiv_seq_close(qmiv_seq* seq);
 enum {
   QMIV_K_STATS = 0,     /* per-row SD / SSD / range statistics */
   QMIV_K_SUMMARY = 1,   /* plane totals and GCD */
-  QMIV_K_CPS = 2,       /* corresponding-pixel-shift search (IV-PSNR, SCP) */
+  QMIV_K_CPS = 2,       /* corresponding-pixel-shift search (IV-PSNR, SCP): the fast kernels */
   QMIV_K_CPS_ROWS = 3,  /* per-row reduction of the CPS partial sums */
   QMIV_K_SSIM = 4,      /* SSIM windowed moments */
   QMIV_K_PYRAMID = 5,   /* MS-SSIM 2x2 pyramid */
   QMIV_K_SSIM_ROWS = 6, /* per-row reduction of the SSIM tile partial sums */
+  QMIV_K_CPS_GENERAL = 7, /* the search's general kernels (any search range, weights, bit depth) */
   QMIV_K_COUNT = 8
 };
 

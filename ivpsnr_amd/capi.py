@@ -67,9 +67,9 @@ class MaskedResult(C.Structure):
     as_dict = FrameResult.as_dict
 
 
-K_STATS, K_SUMMARY, K_CPS, K_CPS_ROWS, K_SSIM, K_PYRAMID, K_SSIM_ROWS, K_COUNT = 0, 1, 2, 3, 4, 5, 6, 8
+K_STATS, K_SUMMARY, K_CPS, K_CPS_ROWS, K_SSIM, K_PYRAMID, K_SSIM_ROWS, K_CPS_GENERAL, K_COUNT = 0, 1, 2, 3, 4, 5, 6, 7, 8
 KERNEL_NAMES = {K_STATS: "stats", K_SUMMARY: "summary", K_CPS: "cps", K_CPS_ROWS: "cps_rows", K_SSIM: "ssim",
-                K_PYRAMID: "pyramid", K_SSIM_ROWS: "ssim_rows"}
+                K_PYRAMID: "pyramid", K_SSIM_ROWS: "ssim_rows", K_CPS_GENERAL: "cps_general"}
 
 
 class Profile(C.Structure):

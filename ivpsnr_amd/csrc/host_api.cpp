@@ -583,7 +583,7 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     a.best1 = fw.best1;
     a.maxv = (1 << g.bd) - 1;
     {
-      KernelTimer t(ctx, QMIV_K_CPS, st);
+      KernelTimer t(ctx, P.fast ? QMIV_K_CPS : QMIV_K_CPS_GENERAL, st);
       if (P.fast420) QMIV_HIP(launch_cps_fast420(a, st));
       else if (P.fast) QMIV_HIP(launch_cps_fast(a, st));
       else QMIV_HIP(launch_cps_generic(a, st));
@@ -2151,7 +2151,7 @@ qmiv_status enqueue_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1
   if (P.cps) {
     uint64_t* part = (uint64_t*)(d + L.off_part);
     {
-      KernelTimer t(ctx, QMIV_K_CPS, st);
+      KernelTimer t(ctx, P.fast ? QMIV_K_CPS : QMIV_K_CPS_GENERAL, st);
       if (P.fast) {
         CpsArgs& fa = P.fa;
         fa.p0 = p0->dev;
@@ -2344,6 +2344,7 @@ qmiv_status qmiv_frames_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* 
                                            (const char*)ctx->h_results + hoff[(size_t)i], &out[i]);
     if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
   }
+  QMIV_TRY(wait_stream(ctx));  // both streams idle; the set's kernel timings are harvested
   return first_err;
 }
 

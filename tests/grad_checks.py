@@ -189,9 +189,10 @@ def test_msssim_gradient_against_fp64_autograd(engine):
         assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
 
 
-def test_ivssim_gradient_against_fp64_autograd(engine, B=2):
-    """B = 2 takes the adjoint's LDS-tiled gather, other search ranges its general form."""
-    c, h, w = 3, 48, 64
+def test_ivssim_gradient_against_fp64_autograd(engine, B=2, h=48, w=64):
+    """B = 2 takes the adjoint's LDS-tiled gather (64 x 4 tiles with a 2-pel halo), other search
+    ranges its general form. 200 x 72 spans three full tiles and a partial one in x."""
+    c = 3
     tst, ref = (quantised(a) for a in field((c, h, w), 4))
     prm = default_params()
     prm.search_range = B
@@ -426,6 +427,7 @@ CHECKS = [
     ("ssim_gradient_fp64_autograd[CS]", test_ssim_gradient_against_fp64_autograd, {"calc_l": False}),
     ("msssim_gradient_fp64_autograd", test_msssim_gradient_against_fp64_autograd, {}),
     ("ivssim_gradient_fp64_autograd", test_ivssim_gradient_against_fp64_autograd, {}),
+    ("ivssim_gradient_fp64_autograd[200x72]", test_ivssim_gradient_against_fp64_autograd, {"h": 72, "w": 200}),
     ("ivssim_gradient_fp64_autograd[B=1]", test_ivssim_gradient_against_fp64_autograd, {"B": 1}),
     ("ivssim_gradient_fp64_autograd[B=3]", test_ivssim_gradient_against_fp64_autograd, {"B": 3}),
     ("finite_differences[ssim]", test_gradient_against_finite_differences, {"kind": "ssim", "shape": (3, 40, 48)}),

@@ -129,7 +129,13 @@ def test_large_given_gcd_matches_oracle(engine, oracle, bd, cf, g):
     p0 = engine.picture(w, h, bd, cf, data=pair.ref)
     p1 = engine.picture(w, h, bd, cf, data=pair.tst)
     pm = engine.picture(w, h, 8, cf, data=mask)
+    engine.profile(True)
+    engine.profile_read(reset=True)
     r = engine.calc_ivpsnr_masked(p0, p1, pm, g, -1, default_params())
+    prof = engine.profile_read(reset=True)
+    engine.profile(False)
+    # the fast masked search ran, not the general kernel
+    assert prof.get("cps", (0, 0))[1] == 1 and "cps_general" not in prof, prof
     nnm = int((M != 0).sum())
     want = []
     for rows in (oracle.cps_rows_masked(P1, P0, M, [-v for v in g]), oracle.cps_rows_masked(P0, P1, M, g)):

@@ -27,6 +27,9 @@ CASES = [
     (200, 8, 10, 420, 7, {"max_shift": 2}),         # fewer rows than a tile band
     (8, 200, 8, 420, 8, {}),                        # fewer columns than a tile
     (1920, 1080, 8, 420, 1, {}),                    # 1080p 8-bit frame of the plain workload
+    # more than 4096 statistics rows (h + 2 ch): the summary kernel's row loop takes a second pass
+    (2048, 1400, 10, 444, 9, {}),                   # 4200 rows
+    (1024, 2304, 10, 420, 10, {}),                  # 4608 rows
 ]
 
 
