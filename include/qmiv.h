@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define QMIV_ABI_VERSION 1
+#define QMIV_ABI_VERSION 2
 
 typedef enum qmiv_status {
   QMIV_OK = 0,
@@ -79,7 +79,18 @@ typedef struct qmiv_params {
   /* 1 (default): qmiv_frame replaces the PSNR / WS-PSNR of identical components by the
      PSNR of SSD=1, as the application does (apps/QMIV/src/xAppQMIV.cpp:837-844,864-871). */
   int32_t fake_psnr_for_exact;
+  /* What the frame flow does with a pel above 2^bd - 1, the application's InvalidPelActn
+     (apps/QMIV/src/xAppQMIV.cpp:681-683,753-770): QMIV_PEL_STOP (default) returns
+     QMIV_ERR_PEL_RANGE; QMIV_PEL_WARN and QMIV_PEL_SKIP compute every metric over the samples as
+     they are (the in_range flags still report the check); QMIV_PEL_CNCL clips the offending
+     pictures to 2^bd - 1 in place (xPicP::conceal, ClipToRange, libs/pmbb-core/src/xPixelOpsSTD.cpp:198-209)
+     and computes the metrics over the clipped pictures (in_range reports the pictures as read).
+     Processor-level calls (qmiv_calc_*, qmiv_gen_scp*) never validate, like the reference's
+     processors. */
+  int32_t invalid_pel_action;
 } qmiv_params;
+
+enum { QMIV_PEL_STOP = 0, QMIV_PEL_WARN = 1, QMIV_PEL_SKIP = 2, QMIV_PEL_CNCL = 3 };
 
 typedef struct qmiv_frame_result {
   int32_t in_range[2];             /* per input: all pels <= 2^bd - 1 (xPicP::check) */
@@ -114,13 +125,15 @@ qmiv_status qmiv_sync(qmiv_ctx* ctx);
 
 void qmiv_params_default(qmiv_params* p);
 
-/* Bytes of one packed frame: xSeq::create (libs/pmbb-core/src/xSeq.cpp:408-428). */
+/* Bytes of one packed frame of a raw file in chroma format 400, 420, 422 or 444 (any width and
+   height): xSeq::create (libs/pmbb-core/src/xSeq.cpp:408-428). 0 for an invalid format. */
 size_t qmiv_frame_bytes(int32_t width, int32_t height, int32_t bit_depth, int32_t chroma_format);
 
 /* Allocates a device picture for one packed frame (bit_depth 8..16; chroma_format 420 with even
-   width and height, or 444). Replaces xPicP::create (libs/pmbb-core/src/xPic.h:31) plus the
-   margin (the engine addresses edges by clamping instead, identical to ExtendMargin,
-   libs/pmbb-core/src/xPixelOpsSTD.cpp:210-236). */
+   width and height, 444, or 400: a luma plane only, for masks). Replaces xPicP::create
+   (libs/pmbb-core/src/xPic.h:31) plus the margin (the engine addresses edges by clamping instead,
+   identical to ExtendMargin, libs/pmbb-core/src/xPixelOpsSTD.cpp:210-236). Frames of other file
+   formats (4:2:2, 4:2:0 of odd width) go into 4:4:4 pictures through qmiv_pic_upload_unpack. */
 qmiv_status qmiv_pic_create(qmiv_ctx* ctx, int32_t width, int32_t height, int32_t bit_depth,
                             int32_t chroma_format, qmiv_pic** out);
 /* Wraps caller-owned device memory holding one packed frame (not freed by the engine). */
@@ -139,6 +152,41 @@ void* qmiv_pic_device_ptr(qmiv_pic* pic);
 qmiv_status qmiv_pic_upload(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes);
 /* Copies a device picture back to host memory (packed layout). */
 qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, size_t bytes);
+/* xSeqBase::xUnpackFrame (libs/pmbb-core/src/xSeq.cpp:179-249) on the device: one packed raw-file
+   frame (qmiv_frame_bytes of file_chroma_format bytes, of the picture's size and bit depth) becomes
+   the picture. The same format is a plain copy (qmiv_pic_upload); 400, 420 (any width) and 422
+   frames unpack into a 4:4:4 picture: chroma nearest-neighbour upsampled from rows of width >> 1
+   samples (libs/pmbb-core/src/xSeq.cpp:197-229; for an odd width the last column reads the next
+   chroma row's first sample, as the reference does; a read past the frame's end gives 0), 4:0:0
+   chroma set to 2^(bd - 1). Asynchronous on the context's stream as qmiv_pic_upload; the frame goes
+   through a context-owned device staging buffer. */
+qmiv_status qmiv_pic_upload_unpack(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes,
+                                   int32_t file_chroma_format);
+/* The reference's own picture layouts, for callers that hold them. xPicP
+   (libs/pmbb-core/src/xPicCommon.cpp:16-33, xPic.h:18-74): three uint16 planes whose first interior
+   sample planes[c] points at, rows `stride` samples apart (W + 2 margin); the margin is neither
+   read nor written. xPicI (xPic.h:79-113, xPic.cpp:169-175,201-207): 4 x uint16 per pel (Y, Cb,
+   Cr, A), origin at the first interior pel, rows `stride` pels apart; A is ignored on upload and
+   written 0. The device picture is 4:4:4 of the same size and bit depth; an 8-bit picture takes
+   samples up to 255 (else QMIV_ERR_PEL_RANGE). Uploads are ordered on the context's stream and
+   return once the host memory may be reused; downloads wait for the data. */
+qmiv_status qmiv_pic_upload_planar(qmiv_ctx* ctx, qmiv_pic* pic, const uint16_t* const planes[3], int32_t stride);
+qmiv_status qmiv_pic_download_planar(qmiv_ctx* ctx, const qmiv_pic* pic, uint16_t* const planes[3], int32_t stride);
+/* xPicI::rearrangeFromPlanar's input side (libs/pmbb-core/src/xPic.cpp:201-207) from host memory. */
+qmiv_status qmiv_pic_upload_interleaved(qmiv_ctx* ctx, qmiv_pic* pic, const uint16_t* origin, int32_t stride);
+qmiv_status qmiv_pic_download_interleaved(qmiv_ctx* ctx, const qmiv_pic* pic, uint16_t* origin, int32_t stride);
+/* xPicP::copy (libs/pmbb-core/src/xPic.cpp:46-51) between device pictures of one geometry;
+   asynchronous. */
+qmiv_status qmiv_pic_copy(qmiv_ctx* ctx, qmiv_pic* dst, const qmiv_pic* src);
+/* xPicP::check (libs/pmbb-core/src/xPic.cpp:62-82, xPixelOpsSTD::CheckIfInRange, :163-174): per
+   component, every sample <= 2^bd - 1. Synchronous. */
+qmiv_status qmiv_pic_check(qmiv_ctx* ctx, const qmiv_pic* pic, int32_t in_range[3]);
+/* xPicP::conceal (libs/pmbb-core/src/xPic.cpp:83-90, xPixelOpsSTD::ClipToRange, :198-209): samples
+   above 2^bd - 1 become 2^bd - 1, in place. Asynchronous on the context's stream. */
+qmiv_status qmiv_pic_conceal(qmiv_ctx* ctx, qmiv_pic* pic);
+/* xPicP::equalCmp per component (libs/pmbb-core/src/xPic.cpp:103-107, CompareEqual,
+   xPixelOpsSTD.cpp:285-295). Synchronous. */
+qmiv_status qmiv_pic_equal(qmiv_ctx* ctx, const qmiv_pic* a, const qmiv_pic* b, int32_t equal[3]);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Sequences: raw planar YUV files (the read side of xSeq, libs/pmbb-core/src/xSeq.cpp) and     */
@@ -146,9 +194,10 @@ qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, si
 /* ------------------------------------------------------------------------------------------ */
 typedef struct qmiv_seq qmiv_seq;
 
-/* Opens a raw planar file of packed frames (the layout of qmiv_pic_upload) for reading. The frame
-   count is floor(file size / frame bytes): a partial last frame is ignored (xSeq::xBackendOpen,
-   xSeq.cpp:451-470; frame bytes as xSeq::create, :408-428). Host-only: no device is touched. */
+/* Opens a raw planar file of packed frames (chroma format 400, 420, 422 or 444, any size) for
+   reading. The frame count is floor(file size / frame bytes): a partial last frame is ignored
+   (xSeq::xBackendOpen, xSeq.cpp:451-470; frame bytes as xSeq::create, :408-428). Host-only: no
+   device is touched. */
 qmiv_status qmiv_seq_open(const char* path, int32_t width, int32_t height, int32_t bit_depth,
                           int32_t chroma_format, qmiv_seq** out);
 /* Opens a list of PNG files as a sequence: xSeqPNG (libs/pmbb-spng/src/xSeqPNG.cpp:55-148). The
@@ -165,11 +214,12 @@ int32_t qmiv_seq_num_frames(const qmiv_seq* seq);
 size_t qmiv_seq_frame_bytes(const qmiv_seq* seq);
 /* Reads frame `index` (seek by frame, xSeq.cpp:519-525) into host memory of the frame's size. */
 qmiv_status qmiv_seq_read_host(qmiv_seq* seq, int32_t index, void* dst, size_t bytes);
-/* Reads frame `index` into a device picture of the sequence's geometry, ordered on the context's
-   stream (work enqueued afterwards sees the frame): the file is read into a pinned buffer (by a
-   background thread when the frame was prefetched: reading frame i prefetches frame i + 1) and
-   copied asynchronously. Replaces xSeqBase::readFrame (xSeq.cpp:155-174) plus the unpack, which
-   the engine does not need. A sequence serves one device. */
+/* Reads frame `index` into a device picture of the sequence's size and bit depth, ordered on the
+   context's stream (work enqueued afterwards sees the frame): the file is read into a pinned buffer
+   (by a background thread when the frame was prefetched: reading frame i prefetches frame i + 1)
+   and copied asynchronously. Replaces xSeqBase::readFrame (xSeq.cpp:155-174): a picture of the
+   file's own format takes the frame as it is; 4:2:2, 4:0:0 and odd-width 4:2:0 frames unpack into
+   a 4:4:4 picture as qmiv_pic_upload_unpack does. A sequence serves one device. */
 qmiv_status qmiv_seq_read(qmiv_ctx* ctx, qmiv_seq* seq, int32_t index, qmiv_pic* pic);
 /* Waits for outstanding copies out of the sequence's buffers, then closes the file. */
 void qmiv_seq_close(qmiv_seq* seq);

@@ -15,6 +15,7 @@ QMIV_OK = 0
 STATUS_NAMES = {0: "QMIV_OK", 1: "QMIV_ERR_INVALID_ARG", 2: "QMIV_ERR_UNSUPPORTED", 3: "QMIV_ERR_HIP",
                 4: "QMIV_ERR_OUT_OF_MEMORY", 5: "QMIV_ERR_PEL_RANGE"}
 PSNR, WSPSNR, IVPSNR, SSIM, MSSSIM, IVSSIM = 1, 2, 4, 8, 16, 32
+PEL_STOP, PEL_WARN, PEL_SKIP, PEL_CNCL = 0, 1, 2, 3
 
 # Every entry point declared in include/qmiv.h; the CPU tests check the library exports them all.
 EXPORTS = []
@@ -29,6 +30,7 @@ class Params(C.Structure):
         ("erp", C.c_int32), ("lon_range_deg", C.c_int32), ("lat_range_deg", C.c_int32),
         ("cps_int32_rows", C.c_int32),
         ("fake_psnr_for_exact", C.c_int32),
+        ("invalid_pel_action", C.c_int32),
     ]
 
 
@@ -109,6 +111,15 @@ _SIGNATURES = {
     "qmiv_pic_device_ptr": (C.c_void_p, [_pic_p]),
     "qmiv_pic_upload": (C.c_int, [_ctx_p, _pic_p, C.c_void_p, C.c_size_t]),
     "qmiv_pic_download": (C.c_int, [_ctx_p, _pic_p, C.c_void_p, C.c_size_t]),
+    "qmiv_pic_upload_unpack": (C.c_int, [_ctx_p, _pic_p, C.c_void_p, C.c_size_t, C.c_int32]),
+    "qmiv_pic_upload_planar": (C.c_int, [_ctx_p, _pic_p, C.POINTER(C.c_void_p), C.c_int32]),
+    "qmiv_pic_download_planar": (C.c_int, [_ctx_p, _pic_p, C.POINTER(C.c_void_p), C.c_int32]),
+    "qmiv_pic_upload_interleaved": (C.c_int, [_ctx_p, _pic_p, C.c_void_p, C.c_int32]),
+    "qmiv_pic_download_interleaved": (C.c_int, [_ctx_p, _pic_p, C.c_void_p, C.c_int32]),
+    "qmiv_pic_copy": (C.c_int, [_ctx_p, _pic_p, _pic_p]),
+    "qmiv_pic_check": (C.c_int, [_ctx_p, _pic_p, _i32p]),
+    "qmiv_pic_conceal": (C.c_int, [_ctx_p, _pic_p]),
+    "qmiv_pic_equal": (C.c_int, [_ctx_p, _pic_p, _pic_p, _i32p]),
     "qmiv_frame": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), C.c_uint32, C.POINTER(FrameResult)]),
     "qmiv_frames": (C.c_int, [_ctx_p, C.c_int32, C.POINTER(_pic_p), C.POINTER(_pic_p), C.POINTER(Params),
                               C.c_uint32, C.POINTER(FrameResult)]),

@@ -53,35 +53,66 @@ constexpr size_t kFixedResultsBytes = (size_t)2 << 20;
 
 qmiv_status qmiv::host_fail(qmiv_status st, const std::string& msg) { return fail(st, msg); }
 
+// Bytes of one packed frame of a raw file: xSeq::create (libs/pmbb-core/src/xSeq.cpp:408-428), for
+// any size (odd 4:2:0 and 4:2:2 widths included); 0 for an invalid format.
+size_t qmiv::file_frame_bytes(int32_t w, int32_t h, int32_t bd, int32_t cf) {
+  if (w <= 0 || h <= 0 || bd < 8 || bd > 16) return 0;
+  const size_t cmp = (size_t)w * h * (bd <= 8 ? 1 : 2);
+  switch (cf) {
+    case 444: return 3 * cmp;
+    case 422: return cmp << 1;
+    case 420: return cmp + (cmp >> 1);
+    case 400: return cmp;
+    default: return 0;
+  }
+}
+
+qmiv_status qmiv::file_route(const qmiv_pic* pic, int32_t w, int32_t h, int32_t bd, int32_t fcf, bool& unpack) {
+  if (!pic) return fail(QMIV_ERR_INVALID_ARG, "null picture");
+  if (file_frame_bytes(w, h, bd, fcf) == 0) return fail(QMIV_ERR_INVALID_ARG, "file format must be 400, 420, 422 or 444");
+  if (pic->g.w != w || pic->g.h != h || pic->g.bd != bd)
+    return fail(QMIV_ERR_INVALID_ARG, "picture size or bit depth differs from the frame's");
+  unpack = pic->g.cf != fcf;
+  if (unpack && pic->g.cf != 444)
+    return fail(QMIV_ERR_INVALID_ARG, "frames of another chroma format unpack into 4:4:4 pictures only");
+  return QMIV_OK;
+}
+
 namespace {
 
 qmiv_status make_geom(int32_t w, int32_t h, int32_t bd, int32_t cf, Geom& g) {
   if (w <= 0 || h <= 0) return fail(QMIV_ERR_INVALID_ARG, "width and height must be positive");
   if (bd < 8 || bd > 16) return fail(QMIV_ERR_INVALID_ARG, "bit depth must be within 8..16");
-  if (cf != 420 && cf != 444) return fail(QMIV_ERR_UNSUPPORTED, "chroma format must be 420 or 444");
+  if (cf != 420 && cf != 444 && cf != 400)
+    return fail(QMIV_ERR_UNSUPPORTED, "device pictures are 4:2:0, 4:4:4 or 4:0:0 (other file formats unpack into 4:4:4)");
   if (cf == 420 && ((w & 1) || (h & 1)))
-    return fail(QMIV_ERR_UNSUPPORTED, "4:2:0 pictures need even width and height");
+    return fail(QMIV_ERR_UNSUPPORTED, "4:2:0 device pictures need even width and height (odd sizes unpack into 4:4:4)");
   if (h > 65535) return fail(QMIV_ERR_UNSUPPORTED, "height above 65535");
   g = Geom{};
   g.w = w;
   g.h = h;
+  g.cf = cf;
   g.cs = cf == 420 ? 1 : 0;
   g.cw = w >> g.cs;
   g.ch = h >> g.cs;
   g.bd = bd;
   g.bytes_per_sample = bd <= 8 ? 1 : 2;
-  const size_t samples = (size_t)w * h + 2 * (size_t)g.cw * g.ch;
+  const size_t samples = (size_t)w * h + (cf == 400 ? 0 : 2 * (size_t)g.cw * g.ch);
   const size_t bytes = samples * (size_t)g.bytes_per_sample;
   if (bytes >= (size_t)1 << 31) return fail(QMIV_ERR_UNSUPPORTED, "frame larger than 2 GiB");
   g.frame_bytes = (uint32_t)bytes;
-  g.cb_off = (int64_t)w * h;
-  g.cr_off = g.cb_off + (int64_t)g.cw * g.ch;
+  g.cb_off = cf == 400 ? 0 : (int64_t)w * h;
+  g.cr_off = cf == 400 ? 0 : g.cb_off + (int64_t)g.cw * g.ch;
   return QMIV_OK;
 }
 
 bool same_geom(const Geom& a, const Geom& b) {
-  return a.w == b.w && a.h == b.h && a.bd == b.bd && a.cs == b.cs;
+  return a.w == b.w && a.h == b.h && a.bd == b.bd && a.cf == b.cf;
 }
+
+bool is444(const Geom& g) { return g.cf == 444; }
+
+
 
 // xWeightedSpherically::initWS for this height and latitude range, computed once per context.
 const qmiv_ctx::WsTable& ws_table(qmiv_ctx* ctx, int32_t h, int32_t lat) {
@@ -128,6 +159,20 @@ qmiv_status ensure_results(qmiv_ctx* ctx, size_t bytes) {
   return QMIV_OK;
 }
 
+qmiv_status ensure_stage(qmiv_ctx* ctx, size_t bytes) {
+  if (ctx->stage_bytes >= bytes) return QMIV_OK;
+  if (ctx->d_stage) {
+    QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    QMIV_HIP(hipFree(ctx->d_stage));
+    ctx->d_stage = nullptr;
+    ctx->stage_bytes = 0;
+  }
+  const size_t sz = align_up(bytes, 1 << 20);
+  if (hipMalloc(&ctx->d_stage, sz) != hipSuccess) return fail(QMIV_ERR_OUT_OF_MEMORY, "staging buffer allocation failed");
+  ctx->stage_bytes = sz;
+  return QMIV_OK;
+}
+
 qmiv_status ensure_scp(qmiv_ctx* ctx, size_t bytes) {
   if (ctx->scp_bytes >= bytes) return QMIV_OK;
   if (ctx->d_scp) {
@@ -166,6 +211,12 @@ struct Layout {
   size_t off_summary, off_cps_rows, off_row_ssd, results_bytes;
   size_t off_row_sd, off_row_max, off_partials, total;
   int32_t nseg;
+  // The summary every later kernel and the host read: off_summary, or under concealment the summary
+  // of the statistics over the clipped pictures (off_summary then keeps the range flags as read).
+  size_t off_summary2 = 0;
+  // The general search's partials when it runs as the fallback of an out-of-range frame.
+  size_t off_partials_gen = 0;
+  int32_t nseg_gen = 0;
   std::vector<SsimEval> ssim;
   size_t off_pyr = 0;         // device: MS-SSIM pyramid levels 1..4, [picture][component][level]
   size_t pyr_level_off[5] = {0, 0, 0, 0, 0};  // element offsets of the levels within one plane chain
@@ -224,7 +275,10 @@ int32_t pick_sweep(qmiv_ctx* ctx, const Geom& g, bool fast420, const int32_t* gc
   return (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles_x, sweep));
 }
 
-Layout make_layout(const Geom& g, bool fast, bool fast420, int32_t sweep, uint32_t ssim_metrics) {
+// conceal: a second summary (concealment); fallback: partials of the general search beside the
+// fast one's (a frame flow that must search frames with out-of-range pels).
+Layout make_layout(const Geom& g, bool fast, bool fast420, int32_t sweep, uint32_t ssim_metrics, bool conceal = false,
+                   bool fallback = false) {
   Layout L{};
   const int32_t nrows = g.h + 2 * g.ch;
   int32_t tw, th;
@@ -234,6 +288,11 @@ Layout make_layout(const Geom& g, bool fast, bool fast420, int32_t sweep, uint32
   size_t o = 0;
   L.off_summary = o;
   o = align_up(o + sizeof(FrameSummary), 256);
+  L.off_summary2 = L.off_summary;
+  if (conceal) {
+    L.off_summary2 = o;
+    o = align_up(o + sizeof(FrameSummary), 256);
+  }
   L.off_cps_rows = o;
   o = align_up(o + sizeof(uint64_t) * 6 * (size_t)g.h, 256);
   L.off_row_ssd = o;
@@ -262,6 +321,11 @@ Layout make_layout(const Geom& g, bool fast, bool fast420, int32_t sweep, uint32
   o = align_up(o + sizeof(uint32_t) * 2 * (size_t)nrows, 256);
   L.off_partials = o;
   o = align_up(o + sizeof(uint64_t) * 6 * (size_t)g.h * (size_t)L.nseg, 256);
+  if (fallback) {
+    L.nseg_gen = (g.w + kGenericCols - 1) / kGenericCols;
+    L.off_partials_gen = o;
+    o = align_up(o + sizeof(uint64_t) * 6 * (size_t)g.h * (size_t)L.nseg_gen, 256);
+  }
   for (SsimEval& e : L.ssim) {
     e.off_part = o;
     o = align_up(o + sizeof(double) * std::max<size_t>((size_t)ssim_tiles_x(e.w) * (size_t)std::max(e.h, 1), 1), 256);
@@ -319,6 +383,7 @@ void harvest_events(qmiv_ctx* ctx) {
 }
 
 struct FrameWork {
+  int32_t ipa = QMIV_PEL_STOP;   // invalid-pel action of the frame flow
   bool need_gcd = false;         // compute the GCD on the device
   const int32_t* gcd = nullptr;  // or use this one
   bool need_cps = false;
@@ -329,8 +394,9 @@ struct FrameWork {
 };
 
 // The device work a frame's metrics need (the application's per-frame flow, xAppQMIV.cpp:686-704).
-FrameWork frame_work(uint32_t metrics) {
+FrameWork frame_work(uint32_t metrics, const qmiv_params& prm) {
   FrameWork fw;
+  fw.ipa = prm.invalid_pel_action;
   fw.need_gcd = (metrics & (QMIV_IVPSNR | QMIV_IVSSIM)) != 0;
   fw.need_cps = (metrics & QMIV_IVPSNR) != 0;
   fw.need_scp = (metrics & QMIV_IVSSIM) != 0;
@@ -345,7 +411,12 @@ struct FramePlan {
   bool fast420 = false;  // ... its 4:2:0 form
   int32_t sweep = 1;
   int32_t md[4];
+  bool conceal = false;   // CNCL: clip out-of-range pictures, then the statistics again
+  bool fallback = false;  // WARN / SKIP: the general search, for a frame with out-of-range pels
 };
+
+// Samples of this geometry can exceed the bit-depth maximum (16-bit storage below 16 bits).
+bool may_exceed(const Geom& g) { return g.bytes_per_sample == 2 && g.bd < 16; }
 
 // sweep_override > 0 caps the CPS tiles per workgroup (the batch autotuner's knob); 0 keeps
 // the occupancy rule of pick_sweep.
@@ -358,7 +429,10 @@ FramePlan plan_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, const
   P.fast420 = P.fast && g.cs == 1;
   P.sweep = pick_sweep(ctx, g, P.fast420, gbound);
   if (sweep_override > 0) P.sweep = std::min(P.sweep, sweep_override);
-  P.L = make_layout(g, P.fast, P.fast420, P.sweep, fw.ssim_metrics);
+  P.conceal = fw.ipa == QMIV_PEL_CNCL && may_exceed(g);
+  P.fallback = (fw.ipa == QMIV_PEL_WARN || fw.ipa == QMIV_PEL_SKIP) && may_exceed(g) && P.fast &&
+               (fw.need_cps || fw.need_scp);
+  P.L = make_layout(g, P.fast, P.fast420, P.sweep, fw.ssim_metrics, P.conceal, P.fallback);
   return P;
 }
 
@@ -547,6 +621,7 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
   uint32_t* row_max = (uint32_t*)(dbase + L.off_row_max);
   uint64_t* partials = (uint64_t*)(dbase + L.off_partials);
 
+  FrameSummary* sum2 = (FrameSummary*)(dbase + L.off_summary2);
   if (fw.gcd) QMIV_HIP(hipMemcpyAsync(sum->gcd, fw.gcd, sizeof(int32_t) * 4, hipMemcpyHostToDevice, st));
   {
     KernelTimer t(ctx, QMIV_K_STATS, st);
@@ -555,6 +630,24 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
   {
     KernelTimer t(ctx, QMIV_K_SUMMARY, st);
     QMIV_HIP(launch_summary(g, row_sd, row_ssd, row_max, P.md, fw.need_gcd ? 1 : 0, sum, st));
+  }
+  if (P.conceal) {
+    // xAppQMIV::validateFrames with CNCL (apps/QMIV/src/xAppQMIV.cpp:753-762): a picture holding a
+    // pel above the maximum is clipped in place, then the statistics and the GCD are computed
+    // again over the clipped pictures. Every step reads the first summary and does nothing when the
+    // frame was in range, so the frame needs no wait on the host.
+    const int32_t maxv = (1 << g.bd) - 1;
+    QMIV_HIP(launch_clip(p0->dev, g, maxv, &sum->max_pel[0], st));
+    QMIV_HIP(launch_clip(p1->dev, g, maxv, &sum->max_pel[1], st));
+    if (fw.gcd) QMIV_HIP(hipMemcpyAsync(sum2->gcd, fw.gcd, sizeof(int32_t) * 4, hipMemcpyHostToDevice, st));
+    {
+      KernelTimer t(ctx, QMIV_K_STATS, st);
+      QMIV_HIP(launch_stats(p0->dev, p1->dev, g, row_sd, row_ssd, row_max, st, sum));
+    }
+    {
+      KernelTimer t(ctx, QMIV_K_SUMMARY, st);
+      QMIV_HIP(launch_summary(g, row_sd, row_ssd, row_max, P.md, fw.need_gcd ? 1 : 0, sum2, st, sum));
+    }
   }
   if (fw.need_cps || fw.need_scp) {
     uint16_t* scp0 = nullptr;
@@ -568,7 +661,7 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     a.p0 = p0->dev;
     a.p1 = p1->dev;
     a.g = g;
-    a.sum = sum;
+    a.sum = sum2;
     for (int c = 0; c < 4; c++) a.ws[c] = prm.cmp_weights_search[c];
     a.B = prm.search_range;
     int32_t tw, th;
@@ -582,6 +675,11 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     a.best0 = fw.best0;
     a.best1 = fw.best1;
     a.maxv = (1 << g.bd) - 1;
+    // WARN / SKIP search frames with pels above the maximum too: the fast kernels run when the frame
+    // is in range, the general kernel (any 16-bit samples, the reference's wrapping int32 errors)
+    // when it is not; each gate reads the summary's range flag.
+    const bool any_range = fw.ipa == QMIV_PEL_WARN || fw.ipa == QMIV_PEL_SKIP;
+    if (!P.fast) a.gate = any_range ? kGateAlways : kGateInRange;
     {
       KernelTimer t(ctx, P.fast ? QMIV_K_CPS : QMIV_K_CPS_GENERAL, st);
       if (P.fast420) QMIV_HIP(launch_cps_fast420(a, st));
@@ -590,7 +688,19 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     }
     {
       KernelTimer t(ctx, QMIV_K_CPS_ROWS, st);
-      QMIV_HIP(launch_cps_rows(partials, L.nseg, g.h, cps_rows, st));
+      QMIV_HIP(launch_cps_rows(partials, L.nseg, g.h, cps_rows, st, &sum2->in_range, P.fallback ? 1 : 0));
+    }
+    if (P.fallback) {
+      CpsArgs b = a;
+      b.gate = kGateOutOfRange;
+      b.nseg = L.nseg_gen;
+      b.partials = (uint64_t*)(dbase + L.off_partials_gen);
+      {
+        KernelTimer t(ctx, QMIV_K_CPS_GENERAL, st);
+        QMIV_HIP(launch_cps_generic(b, st));
+      }
+      KernelTimer t(ctx, QMIV_K_CPS_ROWS, st);
+      QMIV_HIP(launch_cps_rows(b.partials, L.nseg_gen, g.h, cps_rows, st, &sum2->in_range, 2));
     }
   }
   if (!L.ssim.empty()) QMIV_TRY(enqueue_ssim(ctx, p0, p1, P, dbase, scp, st));
@@ -615,8 +725,12 @@ qmiv_status wait_stream(qmiv_ctx* ctx) {
 }
 
 // Single frame through enqueue + wait, results at the start of ctx->h_results.
+// The processor-level calls validate nothing (the reference's processors compute over the samples
+// as they are): their frames take the WARN / SKIP search paths.
 qmiv_status run_frame_kernels(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params& prm,
-                              const FrameWork& fw, Layout& L, bool& used_fast) {
+                              const FrameWork& fw_in, Layout& L, bool& used_fast) {
+  FrameWork fw = fw_in;
+  fw.ipa = QMIV_PEL_SKIP;
   const FramePlan P = plan_frame(ctx, p0->g, prm, fw);
   L = P.L;
   used_fast = P.fast;
@@ -651,6 +765,7 @@ void apply_row_semantics(uint64_t* rows, size_t n, bool int32_rows) {
 qmiv_status check_pair(const qmiv_pic* a, const qmiv_pic* b) {
   if (!a || !b) return fail(QMIV_ERR_INVALID_ARG, "null picture");
   if (!same_geom(a->g, b->g)) return fail(QMIV_ERR_INVALID_ARG, "pictures differ in size, bit depth or chroma format");
+  if (a->g.cf == 400) return fail(QMIV_ERR_INVALID_ARG, "4:0:0 pictures serve as masks only");
   if (a->ctx != b->ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to different contexts");
   return QMIV_OK;
 }
@@ -659,16 +774,20 @@ qmiv_status check_pair(const qmiv_pic* a, const qmiv_pic* b) {
 qmiv_status finalize_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, uint32_t metrics,
                            const FrameWork& fw, const Layout& L, const char* hb, qmiv_frame_result* out) {
   std::memset(out, 0, sizeof(*out));
-  const FrameSummary* sum = (const FrameSummary*)(hb + L.off_summary);
+  // The range flags of the pictures as read; everything else from the summary the kernels used
+  // (under concealment: of the clipped pictures).
+  const FrameSummary* read = (const FrameSummary*)(hb + L.off_summary);
+  const FrameSummary* sum = (const FrameSummary*)(hb + L.off_summary2);
   const uint64_t* h_cps = (const uint64_t*)(hb + L.off_cps_rows);
   const uint64_t* h_ssd = (const uint64_t*)(hb + L.off_row_ssd);
   const int32_t maxv = (1 << g.bd) - 1;
-  out->in_range[0] = g.bd >= 16 || sum->max_pel[0] <= maxv;
-  out->in_range[1] = g.bd >= 16 || sum->max_pel[1] <= maxv;
+  out->in_range[0] = g.bd >= 16 || read->max_pel[0] <= maxv;
+  out->in_range[1] = g.bd >= 16 || read->max_pel[1] <= maxv;
   for (int c = 0; c < 3; c++) out->exact[c] = sum->ssd[c] == 0;
   if (fw.need_gcd)
     for (int c = 0; c < 4; c++) out->gcd[c] = sum->gcd[c];
-  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
+  if (!read->in_range && fw.ipa == QMIV_PEL_STOP)
+    return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
 
   const int32_t area = g.w * g.h;
   const double fake = fin::psnr_from_ssd(1.0, area, g.bd);
@@ -1127,9 +1246,7 @@ void qmiv_params_default(qmiv_params* p) {
 }
 
 size_t qmiv_frame_bytes(int32_t width, int32_t height, int32_t bit_depth, int32_t chroma_format) {
-  Geom g;
-  if (make_geom(width, height, bit_depth, chroma_format, g) != QMIV_OK) return 0;
-  return g.frame_bytes;
+  return file_frame_bytes(width, height, bit_depth, chroma_format);
 }
 
 qmiv_status qmiv_create(int32_t device, qmiv_ctx** out) {
@@ -1268,6 +1385,193 @@ qmiv_status qmiv_pic_upload(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size
   return QMIV_OK;
 }
 
+qmiv_status qmiv_pic_upload_unpack(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes,
+                                   int32_t file_chroma_format) {
+  if (!ctx || !pic || !host) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (pic->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "picture belongs to another context");
+  return upload_file_frame(ctx, pic, host, bytes, file_chroma_format);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Checks shared by the reference-layout transfers: a 4:4:4 picture of this context, a stride of at
+// least the width; 8-bit pictures hold samples up to 255 only.
+qmiv_status check_ref_layout(qmiv_ctx* ctx, const qmiv_pic* pic, int32_t stride) {
+  if (pic->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "picture belongs to another context");
+  if (!is444(pic->g)) return fail(QMIV_ERR_INVALID_ARG, "reference-layout pictures are 4:4:4");
+  if (stride < pic->g.w) return fail(QMIV_ERR_INVALID_ARG, "stride below the picture width");
+  return QMIV_OK;
+}
+
+qmiv_status check_8bit_rows(const uint16_t* base, int32_t stride, int32_t w, int32_t h, int32_t step) {
+  for (int32_t y = 0; y < h; y++) {
+    const uint16_t* r = base + (size_t)y * stride * step;
+    for (int32_t x = 0; x < w * step; x++)
+      if ((step == 1 || x % 4 != 3) && r[x] > 255)
+        return fail(QMIV_ERR_PEL_RANGE, "sample above 255 for an 8-bit picture");
+  }
+  return QMIV_OK;
+}
+
+// Per-row statistics of (a, b) into the context scratch, row maxima and SSDs copied back:
+// host rows [0, n) row SSD (uint64), then row maxima of a (uint32).
+qmiv_status picture_rows(qmiv_ctx* ctx, const qmiv_pic* a, const qmiv_pic* b, const uint64_t** ssd,
+                         const uint32_t** maxa) {
+  const Geom& g = a->g;
+  const size_t n = (size_t)stat_rows(g);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_TRY(ensure_scratch(ctx, 24 * n));
+  QMIV_TRY(ensure_results(ctx, 12 * n));
+  char* d = (char*)ctx->d_scratch;
+  int64_t* row_sd = (int64_t*)d;
+  uint64_t* row_ssd = (uint64_t*)(d + 8 * n);
+  uint32_t* row_max = (uint32_t*)(d + 16 * n);
+  QMIV_HIP(launch_stats(a->dev, b->dev, g, row_sd, row_ssd, row_max, ctx->stream));
+  char* h = (char*)ctx->h_results;
+  QMIV_HIP(hipMemcpyAsync(h, row_ssd, 8 * n, hipMemcpyDeviceToHost, ctx->stream));
+  QMIV_HIP(hipMemcpyAsync(h + 8 * n, row_max, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  *ssd = (const uint64_t*)h;
+  *maxa = (const uint32_t*)(h + 8 * n);
+  return QMIV_OK;
+}
+
+// Rows of plane c in the statistics' row order (luma, then Cb, then Cr rows).
+void plane_rows(const Geom& g, int c, size_t& r0, size_t& r1) {
+  r0 = c == 0 ? 0 : (size_t)g.h + (size_t)(c - 1) * g.ch;
+  r1 = c == 0 ? (size_t)g.h : r0 + (size_t)g.ch;
+}
+
+}  // namespace
+
+extern "C" {
+
+qmiv_status qmiv_pic_upload_planar(qmiv_ctx* ctx, qmiv_pic* pic, const uint16_t* const planes[3], int32_t stride) {
+  if (!ctx || !pic || !planes || !planes[0] || !planes[1] || !planes[2]) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_ref_layout(ctx, pic, stride));
+  const Geom& g = pic->g;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  const size_t plane = (size_t)g.w * g.h;
+  if (g.bytes_per_sample == 2) {
+    for (int c = 0; c < 3; c++)
+      QMIV_HIP(hipMemcpy2DAsync((uint16_t*)pic->dev + c * plane, 2 * (size_t)g.w, planes[c], 2 * (size_t)stride,
+                                2 * (size_t)g.w, (size_t)g.h, hipMemcpyHostToDevice, ctx->stream));
+    return QMIV_OK;
+  }
+  for (int c = 0; c < 3; c++) QMIV_TRY(check_8bit_rows(planes[c], stride, g.w, g.h, 1));
+  QMIV_TRY(ensure_stage(ctx, 6 * plane));
+  uint16_t* st = (uint16_t*)ctx->d_stage;
+  for (int c = 0; c < 3; c++)
+    QMIV_HIP(hipMemcpy2DAsync(st + c * plane, 2 * (size_t)g.w, planes[c], 2 * (size_t)stride, 2 * (size_t)g.w,
+                              (size_t)g.h, hipMemcpyHostToDevice, ctx->stream));
+  QMIV_HIP(launch_narrow(st, (int64_t)(3 * plane), (uint8_t*)pic->dev, ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_download_planar(qmiv_ctx* ctx, const qmiv_pic* pic, uint16_t* const planes[3], int32_t stride) {
+  if (!ctx || !pic || !planes || !planes[0] || !planes[1] || !planes[2]) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_ref_layout(ctx, pic, stride));
+  const Geom& g = pic->g;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  const size_t plane = (size_t)g.w * g.h;
+  const uint16_t* src = (const uint16_t*)pic->dev;
+  if (g.bytes_per_sample == 1) {
+    QMIV_TRY(ensure_stage(ctx, 6 * plane));
+    QMIV_HIP(launch_widen((const uint8_t*)pic->dev, (int64_t)(3 * plane), (uint16_t*)ctx->d_stage, ctx->stream));
+    src = (const uint16_t*)ctx->d_stage;
+  }
+  for (int c = 0; c < 3; c++)
+    QMIV_HIP(hipMemcpy2DAsync(planes[c], 2 * (size_t)stride, src + c * plane, 2 * (size_t)g.w, 2 * (size_t)g.w,
+                              (size_t)g.h, hipMemcpyDeviceToHost, ctx->stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_upload_interleaved(qmiv_ctx* ctx, qmiv_pic* pic, const uint16_t* origin, int32_t stride) {
+  if (!ctx || !pic || !origin) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_ref_layout(ctx, pic, stride));
+  const Geom& g = pic->g;
+  if (g.bytes_per_sample == 1) QMIV_TRY(check_8bit_rows(origin, stride, g.w, g.h, 4));
+  QMIV_HIP(hipSetDevice(ctx->device));
+  const size_t row = 8 * (size_t)g.w;  // 4 x uint16 per pel
+  QMIV_TRY(ensure_stage(ctx, row * g.h));
+  QMIV_HIP(hipMemcpy2DAsync(ctx->d_stage, row, origin, 8 * (size_t)stride, row, (size_t)g.h, hipMemcpyHostToDevice,
+                            ctx->stream));
+  QMIV_HIP(launch_interleave(pic->dev, g, (uint16_t*)ctx->d_stage, false, ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_download_interleaved(qmiv_ctx* ctx, const qmiv_pic* pic, uint16_t* origin, int32_t stride) {
+  if (!ctx || !pic || !origin) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(check_ref_layout(ctx, pic, stride));
+  const Geom& g = pic->g;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  const size_t row = 8 * (size_t)g.w;
+  QMIV_TRY(ensure_stage(ctx, row * g.h));
+  QMIV_HIP(launch_interleave(pic->dev, g, (uint16_t*)ctx->d_stage, true, ctx->stream));
+  QMIV_HIP(hipMemcpy2DAsync(origin, 8 * (size_t)stride, ctx->d_stage, row, row, (size_t)g.h, hipMemcpyDeviceToHost,
+                            ctx->stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_copy(qmiv_ctx* ctx, qmiv_pic* dst, const qmiv_pic* src) {
+  if (!ctx || !dst || !src) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (!same_geom(dst->g, src->g)) return fail(QMIV_ERR_INVALID_ARG, "pictures differ in size, bit depth or chroma format");
+  if (dst->ctx != ctx || src->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
+  if (dst->dev == src->dev) return QMIV_OK;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_HIP(hipMemcpyAsync(dst->dev, src->dev, src->g.frame_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_check(qmiv_ctx* ctx, const qmiv_pic* pic, int32_t in_range[3]) {
+  if (!ctx || !pic || !in_range) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (pic->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "picture belongs to another context");
+  const Geom& g = pic->g;
+  const uint64_t* ssd;
+  const uint32_t* mx;
+  QMIV_TRY(picture_rows(ctx, pic, pic, &ssd, &mx));
+  const uint32_t maxv = (uint32_t)((1 << g.bd) - 1);
+  for (int c = 0; c < 3; c++) {
+    size_t r0, r1;
+    plane_rows(g, c, r0, r1);
+    uint32_t m = 0;
+    for (size_t r = r0; r < r1; r++) m = std::max(m, mx[r]);
+    // CheckIfInRange: every sample fits at 16 bits; a 4:0:0 picture has no chroma to check.
+    in_range[c] = g.bd >= 16 || (c > 0 && g.cf == 400) || m <= maxv;
+  }
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_conceal(qmiv_ctx* ctx, qmiv_pic* pic) {
+  if (!ctx || !pic) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (pic->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "picture belongs to another context");
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_HIP(launch_clip(pic->dev, pic->g, (1 << pic->g.bd) - 1, nullptr, ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_pic_equal(qmiv_ctx* ctx, const qmiv_pic* a, const qmiv_pic* b, int32_t equal[3]) {
+  if (!ctx || !equal) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (!a || !b) return fail(QMIV_ERR_INVALID_ARG, "null picture");
+  if (!same_geom(a->g, b->g)) return fail(QMIV_ERR_INVALID_ARG, "pictures differ in size, bit depth or chroma format");
+  if (a->ctx != ctx || b->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
+  const uint64_t* ssd;
+  const uint32_t* mx;
+  QMIV_TRY(picture_rows(ctx, a, b, &ssd, &mx));
+  for (int c = 0; c < 3; c++) {
+    size_t r0, r1;
+    plane_rows(a->g, c, r0, r1);
+    uint64_t s = 0;
+    for (size_t r = r0; r < r1; r++) s |= ssd[r];  // a sum of squares is 0 only if every term is
+    equal[c] = s == 0;
+  }
+  return QMIV_OK;
+}
+
 qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, size_t bytes) {
   if (!ctx || !pic || !host) return fail(QMIV_ERR_INVALID_ARG, "null argument");
   if (bytes != pic->g.frame_bytes) return fail(QMIV_ERR_INVALID_ARG, "byte count differs from the frame size");
@@ -1275,6 +1579,27 @@ qmiv_status qmiv_pic_download(qmiv_ctx* ctx, const qmiv_pic* pic, void* host, si
   QMIV_HIP(hipStreamSynchronize(ctx->stream));
   return QMIV_OK;
 }
+
+}  // extern "C"
+
+qmiv_status qmiv::upload_file_frame(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes, int32_t fcf) {
+  const Geom& g = pic->g;
+  bool unpack = false;
+  QMIV_TRY(file_route(pic, g.w, g.h, g.bd, fcf, unpack));
+  if (bytes != file_frame_bytes(g.w, g.h, g.bd, fcf))
+    return fail(QMIV_ERR_INVALID_ARG, "byte count differs from the file frame size");
+  QMIV_HIP(hipSetDevice(ctx->device));
+  if (!unpack) {
+    QMIV_HIP(hipMemcpyAsync(pic->dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return QMIV_OK;
+  }
+  QMIV_TRY(ensure_stage(ctx, bytes));
+  QMIV_HIP(hipMemcpyAsync(ctx->d_stage, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  QMIV_HIP(launch_unpack(ctx->d_stage, bytes, g.w, g.h, g.bd, fcf, pic->dev, ctx->stream));
+  return QMIV_OK;
+}
+
+extern "C" {
 
 qmiv_status qmiv_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm_in,
                        uint32_t metrics, qmiv_frame_result* out) {
@@ -1292,7 +1617,9 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
   if (prm_in) prm = *prm_in;
   else qmiv_params_default(&prm);
   QMIV_HIP(hipSetDevice(ctx->device));
-  const FrameWork fw = frame_work(metrics);
+  if (prm.invalid_pel_action < QMIV_PEL_STOP || prm.invalid_pel_action > QMIV_PEL_CNCL)
+    return fail(QMIV_ERR_INVALID_ARG, "invalid_pel_action must be 0..3");
+  const FrameWork fw = frame_work(metrics, prm);
   // Plan every frame, then lay their scratch and result blocks end to end.
   std::vector<FramePlan> plans((size_t)n);
   std::vector<size_t> doff((size_t)n), hoff((size_t)n);
@@ -1375,7 +1702,9 @@ qmiv_status qmiv_frame_noalloc(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic
   if (prm_in) prm = *prm_in;
   else qmiv_params_default(&prm);
   QMIV_HIP(hipSetDevice(ctx->device));
-  const FrameWork fw = frame_work(metrics);
+  if (prm.invalid_pel_action < QMIV_PEL_STOP || prm.invalid_pel_action > QMIV_PEL_CNCL)
+    return fail(QMIV_ERR_INVALID_ARG, "invalid_pel_action must be 0..3");
+  const FrameWork fw = frame_work(metrics, prm);
   const Geom& g = p0->g;
   const FramePlan P = plan_frame(ctx, g, prm, fw);
   // Device block: the frame's scratch, then (IV-SSIM) the two shift-compensated pictures.
@@ -1484,8 +1813,6 @@ qmiv_status qmiv_calc_ivpsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
   // R2T searches Tst + gcd in Ref: the kernel's direction 0 with P0 = Tst, P1 = Ref.
   QMIV_TRY(run_frame_kernels(ctx, tst, ref, prm, fw, L, fast));
   const char* hb = (const char*)ctx->h_results;
-  const FrameSummary* sum = (const FrameSummary*)(hb + L.off_summary);
-  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
   const Geom& g = tst->g;
   std::vector<uint64_t> rows((const uint64_t*)(hb + L.off_cps_rows), (const uint64_t*)(hb + L.off_cps_rows) + 6 * (size_t)g.h);
   apply_row_semantics(rows.data(), rows.size(), prm.cps_int32_rows != 0);
@@ -1503,11 +1830,21 @@ qmiv_status qmiv_calc_ivpsnr(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
   return QMIV_OK;
 }
 
+// Parameters of a processor-level call through the frame flow: no validation.
+static qmiv_params processor_params(const qmiv_params* prm) {
+  qmiv_params p;
+  if (prm) p = *prm;
+  else qmiv_params_default(&p);
+  p.invalid_pel_action = QMIV_PEL_SKIP;
+  return p;
+}
+
 qmiv_status qmiv_calc_ssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, int32_t calc_l, double ssim[3]) {
   if (!ssim) return fail(QMIV_ERR_INVALID_ARG, "null argument");
   qmiv_frame_result r;
   const uint32_t m = QMIV_SSIM | (calc_l ? 0u : kSsimNoLuminance);
-  QMIV_TRY(qmiv_frame(ctx, tst, ref, nullptr, m, &r));
+  const qmiv_params prm = processor_params(nullptr);
+  QMIV_TRY(qmiv_frame(ctx, tst, ref, &prm, m, &r));
   for (int c = 0; c < 3; c++) ssim[c] = r.ssim[c];
   return QMIV_OK;
 }
@@ -1516,7 +1853,8 @@ qmiv_status qmiv_calc_msssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
                              double* scales) {
   if (!msssim) return fail(QMIV_ERR_INVALID_ARG, "null argument");
   qmiv_frame_result r;
-  QMIV_TRY(qmiv_frame(ctx, tst, ref, nullptr, QMIV_MSSSIM, &r));
+  const qmiv_params prm = processor_params(nullptr);
+  QMIV_TRY(qmiv_frame(ctx, tst, ref, &prm, QMIV_MSSSIM, &r));
   for (int c = 0; c < 3; c++) msssim[c] = r.msssim[c];
   if (scales) std::memcpy(scales, r.msssim_scales, sizeof(r.msssim_scales));
   return QMIV_OK;
@@ -1525,7 +1863,8 @@ qmiv_status qmiv_calc_msssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic*
 qmiv_status qmiv_calc_ivssim(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_pic* ref, const qmiv_params* prm,
                              double* ivssim, double* r2t, double* t2r) {
   qmiv_frame_result r;
-  QMIV_TRY(qmiv_frame(ctx, tst, ref, prm, QMIV_IVSSIM, &r));
+  const qmiv_params p = processor_params(prm);
+  QMIV_TRY(qmiv_frame(ctx, tst, ref, &p, QMIV_IVSSIM, &r));
   if (ivssim) *ivssim = r.ivssim;
   if (r2t) *r2t = r.ivssim_r2t;
   if (t2r) *t2r = r.ivssim_t2r;
@@ -1548,8 +1887,6 @@ qmiv_status qmiv_gen_scp(qmiv_ctx* ctx, const qmiv_pic* src_ref, const qmiv_pic*
   // DstRef is rebuilt from SrcTst + gcd searched in SrcRef: the kernel's direction 0 with
   // P0 = SrcTst, P1 = SrcRef, i.e. its SCP of P1; DstTst is its SCP of P0.
   QMIV_TRY(run_frame_kernels(ctx, src_tst, src_ref, prm, fw, L, fast));
-  const FrameSummary* sum = (const FrameSummary*)((const char*)ctx->h_results + L.off_summary);
-  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
   const size_t plane3 = 3 * (size_t)src_tst->g.w * src_tst->g.h;
   const uint16_t* scp0 = (const uint16_t*)ctx->d_scp;
   QMIV_HIP(hipMemcpyAsync(dst_tst, scp0, plane3 * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -1564,7 +1901,7 @@ qmiv_status qmiv_gen_scp_pics(qmiv_ctx* ctx, const qmiv_pic* src_ref, const qmiv
   QMIV_TRY(check_pair(src_tst, src_ref));
   const Geom& gs = src_tst->g;
   for (const qmiv_pic* d : {(const qmiv_pic*)dst_ref, (const qmiv_pic*)dst_tst})
-    if (d->g.w != gs.w || d->g.h != gs.h || d->g.bd != gs.bd || d->g.cs != 0)
+    if (d->g.w != gs.w || d->g.h != gs.h || d->g.bd != gs.bd || !is444(d->g))
       return fail(QMIV_ERR_INVALID_ARG, "destination pictures must be 4:4:4 of the sources' size and bit depth");
   if (dst_ref == dst_tst) return fail(QMIV_ERR_INVALID_ARG, "destination pictures must differ");
   qmiv_params prm;
@@ -1577,8 +1914,6 @@ qmiv_status qmiv_gen_scp_pics(qmiv_ctx* ctx, const qmiv_pic* src_ref, const qmiv
   Layout L;
   bool fast;
   QMIV_TRY(run_frame_kernels(ctx, src_tst, src_ref, prm, fw, L, fast));
-  const FrameSummary* sum = (const FrameSummary*)((const char*)ctx->h_results + L.off_summary);
-  if (!sum->in_range) return fail(QMIV_ERR_PEL_RANGE, "input contains pels above the bit-depth maximum");
   const int64_t plane3 = 3 * (int64_t)gs.w * gs.h;
   const uint16_t* scp0 = (const uint16_t*)ctx->d_scp;
   // The frame kernels' SCP of P0 (= SrcTst) is DstTst; their SCP of P1 (= SrcRef) is DstRef.
@@ -1600,9 +1935,7 @@ qmiv_status qmiv_calc_ivssim_scp(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_
   QMIV_TRY(check_pair(tst, ref));
   QMIV_TRY(check_pair(tst, ref_scp));
   QMIV_TRY(check_pair(ref, tst_scp));
-  qmiv_params prm;
-  if (prm_in) prm = *prm_in;
-  else qmiv_params_default(&prm);
+  const qmiv_params prm = processor_params(prm_in);
   qmiv_frame_result a, b;
   QMIV_TRY(qmiv_frame(ctx, tst, ref_scp, &prm, QMIV_SSIM, &a));  // SSIMs_T2R = calcPicSSIM(Tst, RefSCP)
   QMIV_TRY(qmiv_frame(ctx, ref, tst_scp, &prm, QMIV_SSIM, &b));  // SSIMs_R2T = calcPicSSIM(Ref, TstSCP)
@@ -1616,8 +1949,9 @@ qmiv_status qmiv_calc_ivssim_scp(qmiv_ctx* ctx, const qmiv_pic* tst, const qmiv_
 
 qmiv_status qmiv_pic_upsample(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst) {
   if (!ctx || !src || !dst) return fail(QMIV_ERR_INVALID_ARG, "null argument");
-  if (dst->g.w != src->g.w || dst->g.h != src->g.h || dst->g.bd != src->g.bd || dst->g.cs != 0)
+  if (dst->g.w != src->g.w || dst->g.h != src->g.h || dst->g.bd != src->g.bd || !is444(dst->g))
     return fail(QMIV_ERR_INVALID_ARG, "destination must be a 4:4:4 picture of the source's size and bit depth");
+  if (src->g.cf == 400) return fail(QMIV_ERR_INVALID_ARG, "a 4:0:0 source has no chroma to upsample");
   if (dst == src || dst->dev == src->dev) return fail(QMIV_ERR_INVALID_ARG, "destination must differ from the source");
   QMIV_HIP(hipSetDevice(ctx->device));
   QMIV_HIP(launch_upsample(src->dev, src->g, dst->dev, ctx->stream));
@@ -1627,7 +1961,7 @@ qmiv_status qmiv_pic_upsample(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst)
 qmiv_status qmiv_pic_colorspace(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* dst, int32_t space, int32_t to_rgb) {
   if (!ctx || !src || !dst) return fail(QMIV_ERR_INVALID_ARG, "null argument");
   if (src->ctx != ctx || dst->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
-  if (src->g.cs != 0 || dst->g.cs != 0) return fail(QMIV_ERR_INVALID_ARG, "colour conversion needs 4:4:4 pictures");
+  if (!is444(src->g) || !is444(dst->g)) return fail(QMIV_ERR_INVALID_ARG, "colour conversion needs 4:4:4 pictures");
   if (dst->g.w != src->g.w || dst->g.h != src->g.h || dst->g.bd != src->g.bd)
     return fail(QMIV_ERR_INVALID_ARG, "destination must match the source's size and bit depth");
   if (to_rgb != 0 && to_rgb != 1) return fail(QMIV_ERR_INVALID_ARG, "to_rgb must be 0 or 1");
@@ -1648,7 +1982,7 @@ qmiv_status qmiv_pic_colorspace(qmiv_ctx* ctx, const qmiv_pic* src, qmiv_pic* ds
 qmiv_status qmiv_pic_swap_planes(qmiv_ctx* ctx, qmiv_pic* pic, int32_t a, int32_t b) {
   if (!ctx || !pic) return fail(QMIV_ERR_INVALID_ARG, "null argument");
   if (pic->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "picture belongs to another context");
-  if (pic->g.cs != 0) return fail(QMIV_ERR_INVALID_ARG, "plane exchange needs a 4:4:4 picture");
+  if (!is444(pic->g)) return fail(QMIV_ERR_INVALID_ARG, "plane exchange needs a 4:4:4 picture");
   if (a < 0 || a > 2 || b < 0 || b > 2) return fail(QMIV_ERR_INVALID_ARG, "plane index must be 0..2");
   QMIV_HIP(hipSetDevice(ctx->device));
   QMIV_HIP(launch_swap_planes(pic->dev, pic->g, a, b, ctx->stream));
@@ -1692,11 +2026,12 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
   for (int32_t i = 0; i < n; i++) {
     if (prm) b->prm[(size_t)i] = prm[i];
     else qmiv_params_default(&b->prm[(size_t)i]);
-    FrameWork& f = b->fw[(size_t)i];
-    f.need_gcd = (metrics[i] & (QMIV_IVPSNR | QMIV_IVSSIM)) != 0;
-    f.need_cps = (metrics[i] & QMIV_IVPSNR) != 0;
-    f.need_scp = (metrics[i] & QMIV_IVSSIM) != 0;
-    f.ssim_metrics = metrics[i] & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM);
+    const qmiv_params& pi = b->prm[(size_t)i];
+    if (pi.invalid_pel_action < QMIV_PEL_STOP || pi.invalid_pel_action > QMIV_PEL_CNCL) {
+      delete b;
+      return fail(QMIV_ERR_INVALID_ARG, "invalid_pel_action must be 0..3");
+    }
+    b->fw[(size_t)i] = frame_work(metrics[i] & (QMIV_PSNR | QMIV_WSPSNR | QMIV_IVPSNR | QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM), pi);
   }
   b->info.views = n;
   // Candidates: branch counts first (with the occupancy sweep rule), then sweep caps.
@@ -2039,15 +2374,25 @@ struct MaskedLayout {
   size_t off_sum = 0, off_ssd = 0, off_cps = 0, results_bytes = 0;
   size_t off_nnz = 0, off_sd = 0, off_max = 0, off_diff = 0, off_part = 0, total = 0;
   int32_t nseg = 0;
+  // As in Layout: the summary the search and the host read (the clipped inputs' under concealment),
+  // and the general search's partials when it is the fallback of an out-of-range frame.
+  size_t off_sum2 = 0, off_part_gen = 0;
+  int32_t nseg_gen = 0;
 };
 
-// nseg: CPS partial segments per row (of the kernel that runs the search).
-MaskedLayout masked_layout(const Geom& g, bool cps, int32_t nseg) {
+// nseg: CPS partial segments per row (of the kernel that runs the search); conceal: a second
+// summary; nseg_gen > 0: partials of the general search as the out-of-range fallback.
+MaskedLayout masked_layout(const Geom& g, bool cps, int32_t nseg, bool conceal = false, int32_t nseg_gen = 0) {
   MaskedLayout L;
   const size_t h = (size_t)g.h;
   size_t o = 0;
   L.off_sum = o;
   o = align_up(o + sizeof(MaskSummary), 256);
+  L.off_sum2 = L.off_sum;
+  if (conceal) {
+    L.off_sum2 = o;
+    o = align_up(o + sizeof(MaskSummary), 256);
+  }
   L.off_ssd = o;
   o = align_up(o + 3 * h * sizeof(uint64_t), 256);
   L.off_cps = o;
@@ -2064,6 +2409,11 @@ MaskedLayout masked_layout(const Geom& g, bool cps, int32_t nseg) {
   L.nseg = nseg;
   L.off_part = o;
   o = align_up(o + (cps ? (size_t)L.nseg * 6 * h * sizeof(uint64_t) : 0), 256);
+  if (cps && nseg_gen > 0) {
+    L.nseg_gen = nseg_gen;
+    L.off_part_gen = o;
+    o = align_up(o + (size_t)nseg_gen * 6 * h * sizeof(uint64_t), 256);
+  }
   L.total = o;
   return L;
 }
@@ -2099,11 +2449,15 @@ struct MaskedPlan {
   bool fast = false;  // the search runs on a fast kernel (the 4:2:0 one when g.cs == 1)
   CpsArgs fa{};       // its tiling (tiles_x, sweep, nseg) when fast
   int32_t md[4];
+  int32_t ipa = QMIV_PEL_STOP;
+  bool conceal = false;   // CNCL: clip out-of-range inputs, then the statistics again
+  bool fallback = false;  // WARN / SKIP: the general search, for a frame with out-of-range pels
 };
 
-// gcd_given: null to compute the masked GCD.
-qmiv_status plan_masked(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, bool cps, const int32_t* gcd_given,
-                        MaskedPlan& P) {
+// gcd_given: null to compute the masked GCD; gm: the mask's geometry; ipa: the invalid-pel action
+// (processor-level calls pass QMIV_PEL_SKIP: no validation).
+qmiv_status plan_masked(qmiv_ctx* ctx, const Geom& g, const Geom& gm, const qmiv_params& prm, int32_t ipa, bool cps,
+                        const int32_t* gcd_given, MaskedPlan& P) {
   if (cps && (prm.search_range < 0 || prm.search_range > kMaxMaskedSearchRange))
     return fail(QMIV_ERR_UNSUPPORTED, "masked search supports search ranges 0..12");
   P = MaskedPlan{};
@@ -2121,7 +2475,12 @@ qmiv_status plan_masked(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, bo
     P.fa.sweep = pick_sweep(ctx, g, f420, gbound);
     P.fa.nseg = (P.fa.tiles_x + P.fa.sweep - 1) / P.fa.sweep;
   }
-  P.L = masked_layout(g, cps, P.fast ? P.fa.nseg : masked_cps_segments(g));
+  P.ipa = ipa;
+  const bool exceed = may_exceed(g) || may_exceed(gm);
+  P.conceal = ipa == QMIV_PEL_CNCL && exceed;
+  P.fallback = (ipa == QMIV_PEL_WARN || ipa == QMIV_PEL_SKIP) && exceed && P.fast;
+  P.L = masked_layout(g, cps, P.fast ? P.fa.nseg : masked_cps_segments(g), P.conceal,
+                      P.fallback ? masked_cps_segments(g) : 0);
   return QMIV_OK;
 }
 
@@ -2140,16 +2499,33 @@ qmiv_status enqueue_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1
   r.max = (uint32_t*)(d + L.off_max);
   r.diff = (uint32_t*)(d + L.off_diff);
   MaskSummary* sum = (MaskSummary*)(d + L.off_sum);
+  MaskSummary* sum2 = (MaskSummary*)(d + L.off_sum2);
+  const int32_t maxm = (1 << mask->g.bd) - 1;
   {
     KernelTimer t(ctx, QMIV_K_STATS, st);
     QMIV_HIP(launch_mstats(p0->dev, p1->dev, g, mask->dev, mask->g, r, st));
   }
   {
     KernelTimer t(ctx, QMIV_K_SUMMARY, st);
-    QMIV_HIP(launch_msummary(g, (1 << mask->g.bd) - 1, r, P.md, nnm_given, gcd_given, sum, st));
+    QMIV_HIP(launch_msummary(g, maxm, r, P.md, nnm_given, gcd_given, sum, st));
+  }
+  if (P.conceal) {
+    // CNCL (xAppQMIV.cpp:753-762) over the three inputs: clip those holding a pel above their
+    // maximum, then the masked statistics again; each step is a no-op for an in-range frame.
+    const int32_t maxv = (1 << g.bd) - 1;
+    QMIV_HIP(launch_clip(p0->dev, g, maxv, &sum->max_pel[0], st));
+    QMIV_HIP(launch_clip(p1->dev, g, maxv, &sum->max_pel[1], st));
+    QMIV_HIP(launch_clip(mask->dev, mask->g, maxm, &sum->max_pel[2], st));
+    {
+      KernelTimer t(ctx, QMIV_K_STATS, st);
+      QMIV_HIP(launch_mstats(p0->dev, p1->dev, g, mask->dev, mask->g, r, st, sum));
+    }
+    KernelTimer t(ctx, QMIV_K_SUMMARY, st);
+    QMIV_HIP(launch_msummary(g, maxm, r, P.md, nnm_given, gcd_given, sum2, st, sum));
   }
   if (P.cps) {
     uint64_t* part = (uint64_t*)(d + L.off_part);
+    const bool any_range = P.ipa == QMIV_PEL_WARN || P.ipa == QMIV_PEL_SKIP;
     {
       KernelTimer t(ctx, P.fast ? QMIV_K_CPS : QMIV_K_CPS_GENERAL, st);
       if (P.fast) {
@@ -2164,17 +2540,28 @@ qmiv_status enqueue_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1
         fa.mask = mask->dev;
         fa.mask_bytes = (uint32_t)((size_t)g.w * g.h * mask->g.bytes_per_sample);  // its luma plane
         fa.mask_bytes_per_sample = mask->g.bytes_per_sample;
-        fa.msum = sum;
+        fa.msum = sum2;
         if (g.cs == 1) QMIV_HIP(launch_cps_fast420(fa, st));
         else QMIV_HIP(launch_cps_fast(fa, st));
       } else {
-        QMIV_HIP(launch_cps_masked(p0->dev, p1->dev, g, mask->dev, mask->g, sum, prm.search_range,
-                                   prm.cmp_weights_search, part, st));
+        QMIV_HIP(launch_cps_masked(p0->dev, p1->dev, g, mask->dev, mask->g, sum2, prm.search_range,
+                                   prm.cmp_weights_search, part, st, any_range ? kGateAlways : kGateInRange));
       }
     }
     {
       KernelTimer t(ctx, QMIV_K_CPS_ROWS, st);
-      QMIV_HIP(launch_cps_rows(part, L.nseg, g.h, (uint64_t*)(d + L.off_cps), st));
+      QMIV_HIP(launch_cps_rows(part, L.nseg, g.h, (uint64_t*)(d + L.off_cps), st, &sum2->all_in_range,
+                               P.fallback ? 1 : 0));
+    }
+    if (P.fallback) {
+      uint64_t* pg = (uint64_t*)(d + L.off_part_gen);
+      {
+        KernelTimer t(ctx, QMIV_K_CPS_GENERAL, st);
+        QMIV_HIP(launch_cps_masked(p0->dev, p1->dev, g, mask->dev, mask->g, sum2, prm.search_range,
+                                   prm.cmp_weights_search, pg, st, kGateOutOfRange));
+      }
+      KernelTimer t(ctx, QMIV_K_CPS_ROWS, st);
+      QMIV_HIP(launch_cps_rows(pg, L.nseg_gen, g.h, (uint64_t*)(d + L.off_cps), st, &sum2->all_in_range, 2));
     }
   }
   QMIV_HIP(hipMemcpyAsync(h, d, L.results_bytes, hipMemcpyDeviceToHost, st));
@@ -2186,7 +2573,7 @@ qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, co
                        const qmiv_params& prm, bool cps, int32_t nnm_given, const int32_t* gcd_given,
                        MaskedLayout& L) {
   MaskedPlan P;
-  QMIV_TRY(plan_masked(ctx, p0->g, prm, cps, gcd_given, P));
+  QMIV_TRY(plan_masked(ctx, p0->g, mask->g, prm, QMIV_PEL_SKIP, cps, gcd_given, P));
   L = P.L;
   QMIV_HIP(hipSetDevice(ctx->device));
   QMIV_TRY(ensure_scratch(ctx, L.total));
@@ -2197,7 +2584,7 @@ qmiv_status run_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, co
 }
 
 const MaskSummary* masked_summary(const qmiv_ctx* ctx, const MaskedLayout& L) {
-  return (const MaskSummary*)((const char*)ctx->h_results + L.off_sum);
+  return (const MaskSummary*)((const char*)ctx->h_results + L.off_sum2);
 }
 
 qmiv_params masked_params(const qmiv_params* prm_in, bool processor) {
@@ -2220,13 +2607,16 @@ namespace {
 qmiv_status finalize_masked(qmiv_ctx* ctx, const Geom& g, int32_t bdm, const qmiv_params& prm, uint32_t metrics,
                             const MaskedLayout& L, const char* hb, qmiv_masked_result* out) {
   const bool iv = (metrics & QMIV_IVPSNR) != 0;
-  const MaskSummary* sum = (const MaskSummary*)(hb + L.off_sum);
+  // Range flags of the inputs as read; the rest from the summary the search used.
+  const MaskSummary* read = (const MaskSummary*)(hb + L.off_sum);
+  const MaskSummary* sum = (const MaskSummary*)(hb + L.off_sum2);
   const int32_t nnm = sum->nnm;
-  for (int i = 0; i < 3; i++) out->in_range[i] = sum->in_range[i];
+  for (int i = 0; i < 3; i++) out->in_range[i] = read->in_range[i];
   for (int c = 0; c < 3; c++) out->exact[c] = sum->exact[c];
   for (int c = 0; c < 4; c++) out->gcd[c] = sum->gcd[c];
   out->num_non_masked = nnm;
-  if (!sum->all_in_range) return fail(QMIV_ERR_PEL_RANGE, "input or mask contains pels above the bit-depth maximum");
+  if (!read->all_in_range && prm.invalid_pel_action == QMIV_PEL_STOP)
+    return fail(QMIV_ERR_PEL_RANGE, "input or mask contains pels above the bit-depth maximum");
   const double fake = fin::psnr_from_ssd(1.0, g.w * g.h, g.bd);
   double plain[3];
   for (int c = 0; c < 3; c++) plain[c] = fin::psnr_from_masked_ssd((double)sum->ssd[c], nnm, g.bd, bdm);
@@ -2297,8 +2687,10 @@ qmiv_status qmiv_frames_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* 
   std::vector<MaskedPlan> plans((size_t)n);
   std::vector<size_t> doff((size_t)n), hoff((size_t)n);
   size_t dtot = 0, htot = 0;
+  if (prm.invalid_pel_action < QMIV_PEL_STOP || prm.invalid_pel_action > QMIV_PEL_CNCL)
+    return fail(QMIV_ERR_INVALID_ARG, "invalid_pel_action must be 0..3");
   for (int32_t i = 0; i < n; i++) {
-    QMIV_TRY(plan_masked(ctx, p0[i]->g, prm, iv, gcd_given, plans[(size_t)i]));
+    QMIV_TRY(plan_masked(ctx, p0[i]->g, mask[i]->g, prm, prm.invalid_pel_action, iv, gcd_given, plans[(size_t)i]));
     doff[(size_t)i] = dtot;
     hoff[(size_t)i] = htot;
     dtot += align_up(plans[(size_t)i].L.total, 256);
@@ -2434,7 +2826,6 @@ qmiv_status qmiv_calc_ivpsnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qm
   const int32_t nnm_arg = gcd_ref2tst ? num_non_masked : -1;
   QMIV_TRY(run_masked(ctx, tst, ref, mask, prm, true, -1, gcd_ref2tst, L));
   const MaskSummary* sum = masked_summary(ctx, L);
-  if (!sum->all_in_range) return fail(QMIV_ERR_PEL_RANGE, "input or mask contains pels above the bit-depth maximum");
   const Geom& g = tst->g;
   const int32_t nnm = nnm_arg >= 0 ? nnm_arg : sum->nnm;
   static const std::vector<double> kNoWs;

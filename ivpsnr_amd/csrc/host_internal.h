@@ -11,9 +11,20 @@
 #include "../../include/qmiv.h"
 #include "qmiv_internal.h"
 
+struct qmiv_pic;
+struct qmiv_ctx;
+
 namespace qmiv {
 // Records msg as the calling thread's last error (qmiv_last_error) and returns st.
 qmiv_status host_fail(qmiv_status st, const std::string& msg);
+// Bytes of one packed raw-file frame (xSeq::create), 0 for an invalid format.
+size_t file_frame_bytes(int32_t w, int32_t h, int32_t bd, int32_t cf);
+// Whether frames of file chroma format fcf go into pic by a plain copy (same format) or by the
+// device unpack into a 4:4:4 picture; QMIV_ERR_INVALID_ARG when neither.
+qmiv_status file_route(const qmiv_pic* pic, int32_t w, int32_t h, int32_t bd, int32_t fcf, bool& unpack);
+// Enqueues on the context's stream the upload of one packed raw-file frame (host memory) into pic:
+// one copy, then the unpack when the formats differ (through the context's device staging buffer).
+qmiv_status upload_file_frame(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes, int32_t fcf);
 }  // namespace qmiv
 
 struct qmiv_pic {
@@ -54,6 +65,10 @@ struct qmiv_ctx {
   // Pinned host block receiving the per-frame results.
   void* h_results = nullptr;
   size_t results_bytes = 0;
+  // Device staging of raw-file frames that unpack into 4:4:4 pictures, and of reference-layout
+  // (planar or interleaved uint16) uploads and downloads.
+  void* d_stage = nullptr;
+  size_t stage_bytes = 0;
   // Shift-compensated pictures (IV-SSIM), 2 x [3][h][w] uint16.
   void* d_scp = nullptr;
   size_t scp_bytes = 0;

@@ -23,9 +23,13 @@ __device__ __forceinline__ uint32_t ldu(const T* __restrict__ p, int64_t i) {
 // plus the largest pel of P0, P1 and the mask (all planes; xPicP::check) and whether any d_c is
 // non-zero (xPicP::equalCmp). A 4:2:0 chroma sample is read at (y >> 1, x >> 1).
 // One wave per row, 4 rows per block; grid = ceil(h / 4).
+// gate (nullable): a first pass's summary; when it found every input in range this pass has nothing
+// to redo (the concealment flow re-runs the statistics only after clipping).
 template <typename T, typename TM>
 __global__ __launch_bounds__(256) void k_mstats(const T* __restrict__ p0, const T* __restrict__ p1, Geom g,
-                                                const TM* __restrict__ mask, Geom gm, MaskRows r) {
+                                                const TM* __restrict__ mask, Geom gm, MaskRows r,
+                                                const MaskSummary* __restrict__ gate) {
+  if (gate && gate->all_in_range) return;
   const int lane = threadIdx.x & 63;
   const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (y >= g.h) return;
@@ -150,9 +154,11 @@ __device__ __forceinline__ void block_stats(const uint64_t (&v)[kStatQ], uint64_
 // add nothing: mask 0, equal pels, no larger maximum. grid = h.
 template <typename T, typename TM, int CS, int CSM>
 __global__ __launch_bounds__(256) void k_mstats_v(const T* __restrict__ p0, const T* __restrict__ p1, Geom g,
-                                                  const TM* __restrict__ mask, Geom gm, MaskRows r) {
+                                                  const TM* __restrict__ mask, Geom gm, MaskRows r,
+                                                  const MaskSummary* __restrict__ gate) {
   __shared__ uint64_t s_red[kStatQ][256];
   __shared__ uint64_t s_tot[kStatQ];
+  if (gate && gate->all_in_range) return;  // uniform over the workgroup
   const int y = blockIdx.x;
   const Buf<T> b0(p0, g.frame_bytes), b1(p1, g.frame_bytes);
   const Buf<TM> bm(mask, gm.frame_bytes);
@@ -234,9 +240,14 @@ __global__ __launch_bounds__(256) void k_msummary(Geom g, int32_t maxv, int32_t 
                                                   int32_t md1, int32_t md2, int32_t md3, int32_t nnm_given,
                                                   int32_t gcd_given,
                                                   int32_t g0, int32_t g1, int32_t g2, int32_t g3,
-                                                  MaskSummary* __restrict__ out) {
+                                                  MaskSummary* __restrict__ out, const MaskSummary* __restrict__ gate) {
   __shared__ uint64_t s_red[kStatQ][256];
   __shared__ uint64_t s_tot[kStatQ];
+  // Concealment flow: a first pass that found every input in range is this pass's result.
+  if (gate && gate->all_in_range) {
+    if (threadIdx.x == 0) *out = *gate;
+    return;
+  }
   uint64_t v[kStatQ];
 #pragma unroll
   for (int q = 0; q < kStatQ; q++) v[q] = 0;
@@ -276,6 +287,7 @@ __global__ __launch_bounds__(256) void k_msummary(Geom g, int32_t maxv, int32_t 
     out->in_range[1] = tmx[1] <= (uint32_t)maxv;
     out->in_range[2] = tmx[2] <= (uint32_t)maxm;
     out->all_in_range = out->in_range[0] && out->in_range[1] && out->in_range[2];
+    for (int i = 0; i < 3; i++) out->max_pel[i] = (int32_t)tmx[i];
     for (int c = 0; c < 3; c++) {
       out->sd[c] = tsd[c];
       out->ssd[c] = tssd[c];
@@ -318,9 +330,9 @@ template <typename T, typename TM>
 __global__ __launch_bounds__(256) void k_cps_masked(const T* __restrict__ p0, const T* __restrict__ p1, Geom g,
                                                     const TM* __restrict__ mask, const MaskSummary* __restrict__ sum,
                                                     int32_t B, int32_t w0, int32_t w1, int32_t w2, int32_t w3,
-                                                    uint64_t* __restrict__ partials) {
+                                                    uint64_t* __restrict__ partials, int32_t gate) {
   extern __shared__ uint16_t s_lds[];
-  if (!sum->all_in_range) return;
+  if (!cps_gate_open(gate, sum->all_in_range)) return;
   const int lw = kMTileW + 2 * B, lh = kMTileH + 2 * B;
   const int lsz = lw * lh;
   uint16_t* s_pic[2][3] = {{s_lds, s_lds + lsz, s_lds + 2 * lsz}, {s_lds + 3 * lsz, s_lds + 4 * lsz, s_lds + 5 * lsz}};
@@ -400,7 +412,7 @@ __global__ __launch_bounds__(256) void k_cps_masked(const T* __restrict__ p0, co
 
 template <typename T, typename TM>
 hipError_t mstats_t(const void* p0, const void* p1, const Geom& g, const void* mask, const Geom& gm, const MaskRows& r,
-                    hipStream_t st) {
+                    hipStream_t st, const MaskSummary* gate) {
 #ifndef QMIV_MSTATS_VECTOR
 #define QMIV_MSTATS_VECTOR 1
 #endif
@@ -410,24 +422,24 @@ hipError_t mstats_t(const void* p0, const void* p1, const Geom& g, const void* m
     const T* b = (const T*)p1;
     const TM* m = (const TM*)mask;
     if (g.cs == 1)
-      gm.cs == 1 ? k_mstats_v<T, TM, 1, 1><<<grid, block, 0, st>>>(a, b, g, m, gm, r)
-                 : k_mstats_v<T, TM, 1, 0><<<grid, block, 0, st>>>(a, b, g, m, gm, r);
+      gm.cs == 1 ? k_mstats_v<T, TM, 1, 1><<<grid, block, 0, st>>>(a, b, g, m, gm, r, gate)
+                 : k_mstats_v<T, TM, 1, 0><<<grid, block, 0, st>>>(a, b, g, m, gm, r, gate);
     else
-      gm.cs == 1 ? k_mstats_v<T, TM, 0, 1><<<grid, block, 0, st>>>(a, b, g, m, gm, r)
-                 : k_mstats_v<T, TM, 0, 0><<<grid, block, 0, st>>>(a, b, g, m, gm, r);
+      gm.cs == 1 ? k_mstats_v<T, TM, 0, 1><<<grid, block, 0, st>>>(a, b, g, m, gm, r, gate)
+                 : k_mstats_v<T, TM, 0, 0><<<grid, block, 0, st>>>(a, b, g, m, gm, r, gate);
     return hipGetLastError();
   }
   k_mstats<T, TM><<<dim3((unsigned)((g.h + 3) / 4)), dim3(256), 0, st>>>((const T*)p0, (const T*)p1, g,
-                                                                         (const TM*)mask, gm, r);
+                                                                         (const TM*)mask, gm, r, gate);
   return hipGetLastError();
 }
 
 template <typename T, typename TM>
 hipError_t cps_t(const void* p0, const void* p1, const Geom& g, const void* mask, const MaskSummary* sum, int32_t B,
-                 const int32_t w[4], uint64_t* partials, hipStream_t st) {
+                 const int32_t w[4], uint64_t* partials, hipStream_t st, int32_t gate) {
   const dim3 grid((unsigned)((g.w + kMTileW - 1) / kMTileW), (unsigned)((g.h + kMTileH - 1) / kMTileH));
   k_cps_masked<T, TM><<<grid, dim3(256), masked_cps_lds_bytes(B), st>>>(
-      (const T*)p0, (const T*)p1, g, (const TM*)mask, sum, B, w[0], w[1], w[2], w[3], partials);
+      (const T*)p0, (const T*)p1, g, (const TM*)mask, sum, B, w[0], w[1], w[2], w[3], partials, gate);
   return hipGetLastError();
 }
 
@@ -440,32 +452,33 @@ size_t masked_cps_lds_bytes(int32_t B) {
 int32_t masked_cps_segments(const Geom& g) { return (g.w + kMTileW - 1) / kMTileW; }
 
 hipError_t launch_mstats(const void* p0, const void* p1, const Geom& g, const void* mask, const Geom& gm,
-                         const MaskRows& r, hipStream_t st) {
+                         const MaskRows& r, hipStream_t st, const MaskSummary* gate) {
   if (g.bytes_per_sample == 1)
-    return gm.bytes_per_sample == 1 ? mstats_t<uint8_t, uint8_t>(p0, p1, g, mask, gm, r, st)
-                                    : mstats_t<uint8_t, uint16_t>(p0, p1, g, mask, gm, r, st);
-  return gm.bytes_per_sample == 1 ? mstats_t<uint16_t, uint8_t>(p0, p1, g, mask, gm, r, st)
-                                  : mstats_t<uint16_t, uint16_t>(p0, p1, g, mask, gm, r, st);
+    return gm.bytes_per_sample == 1 ? mstats_t<uint8_t, uint8_t>(p0, p1, g, mask, gm, r, st, gate)
+                                    : mstats_t<uint8_t, uint16_t>(p0, p1, g, mask, gm, r, st, gate);
+  return gm.bytes_per_sample == 1 ? mstats_t<uint16_t, uint8_t>(p0, p1, g, mask, gm, r, st, gate)
+                                  : mstats_t<uint16_t, uint16_t>(p0, p1, g, mask, gm, r, st, gate);
 }
 
 hipError_t launch_msummary(const Geom& g, int32_t maxm, const MaskRows& r, const int32_t maxdiff[4], int32_t nnm_given,
-                           const int32_t* gcd_given, MaskSummary* out, hipStream_t st) {
+                           const int32_t* gcd_given, MaskSummary* out, hipStream_t st, const MaskSummary* gate) {
+  if (gate == out) return hipErrorInvalidValue;
   const int32_t z[4] = {0, 0, 0, 0};
   const int32_t* gg = gcd_given ? gcd_given : z;
   k_msummary<<<dim3(1), dim3(256), 0, st>>>(g, (1 << g.bd) - 1, maxm, r, maxdiff[0], maxdiff[1], maxdiff[2],
                                              maxdiff[3], nnm_given, gcd_given ? 1 : 0, gg[0], gg[1], gg[2], gg[3],
-                                             out);
+                                             out, gate);
   return hipGetLastError();
 }
 
 hipError_t launch_cps_masked(const void* p0, const void* p1, const Geom& g, const void* mask, const Geom& gm,
                              const MaskSummary* sum, int32_t B, const int32_t w[4], uint64_t* partials,
-                             hipStream_t st) {
+                             hipStream_t st, int32_t gate) {
   if (g.bytes_per_sample == 1)
-    return gm.bytes_per_sample == 1 ? cps_t<uint8_t, uint8_t>(p0, p1, g, mask, sum, B, w, partials, st)
-                                    : cps_t<uint8_t, uint16_t>(p0, p1, g, mask, sum, B, w, partials, st);
-  return gm.bytes_per_sample == 1 ? cps_t<uint16_t, uint8_t>(p0, p1, g, mask, sum, B, w, partials, st)
-                                  : cps_t<uint16_t, uint16_t>(p0, p1, g, mask, sum, B, w, partials, st);
+    return gm.bytes_per_sample == 1 ? cps_t<uint8_t, uint8_t>(p0, p1, g, mask, sum, B, w, partials, st, gate)
+                                    : cps_t<uint8_t, uint16_t>(p0, p1, g, mask, sum, B, w, partials, st, gate);
+  return gm.bytes_per_sample == 1 ? cps_t<uint16_t, uint8_t>(p0, p1, g, mask, sum, B, w, partials, st, gate)
+                                  : cps_t<uint16_t, uint16_t>(p0, p1, g, mask, sum, B, w, partials, st, gate);
 }
 
 }  // namespace qmiv

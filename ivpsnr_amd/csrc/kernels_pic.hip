@@ -1,6 +1,7 @@
-// gfx950 kernels on whole pictures: 4:2:0 -> 4:4:4 nearest-neighbour upsampling, the store of
-// the engine's 16-bit shift-compensated planes into a caller's picture, RGB <-> YCbCr conversion
-// and plane exchange.
+// gfx950 kernels on whole pictures: 4:2:0 -> 4:4:4 nearest-neighbour upsampling, the unpack of raw
+// file frames (4:2:0 of any width, 4:2:2, 4:0:0) into 4:4:4 pictures, the store of the engine's
+// 16-bit shift-compensated planes into a caller's picture, RGB <-> YCbCr conversion, plane exchange
+// and the clip of out-of-range samples.
 #include "device_common.h"
 #include "qmiv_internal.h"
 
@@ -121,7 +122,126 @@ __global__ __launch_bounds__(256) void k_swap(T* __restrict__ a, T* __restrict__
   }
 }
 
+// xSeqBase::xUnpackFrame (libs/pmbb-core/src/xSeq.cpp:179-249) of one packed file frame (bytes at
+// src, src_bytes of them) into a 4:4:4 picture of the same sample size: luma copied; 4:2:0 and 4:2:2
+// chroma upsampled nearest-neighbour from rows of W >> 1 samples (UpsampleHV / UpsampleH,
+// libs/pmbb-core/src/xPixelOpsSTD.cpp:30-49, with the file stride of xSeq.cpp:200,216), so for an odd
+// width the last column takes sample W >> 1 of its row, which is the next row's first; 4:4:4 chroma
+// copied; 4:0:0 chroma set to the mid value 2^(bd - 1) (xSeq.cpp:399-400). Chroma planes begin at
+// byte offsets cb_byte / cr_byte, which may be odd for 16-bit samples, so samples are read bytewise;
+// a sample past the end of the frame reads 0. grid = (ceil(w / 256), h, 3).
+template <typename T>
+__global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ src, uint64_t src_bytes, int32_t w,
+                                                int32_t h, int32_t fcf, uint64_t cb_byte, uint64_t cr_byte,
+                                                uint32_t mid, T* __restrict__ dst) {
+  const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t y = blockIdx.y;
+  const int c = blockIdx.z;
+  if (x >= w) return;
+  uint32_t v;
+  if (c != 0 && fcf == 400) {
+    v = mid;
+  } else {
+    int64_t idx;
+    uint64_t base = 0;
+    if (c == 0 || fcf == 444) idx = (int64_t)y * w + x;
+    else if (fcf == 422) idx = (int64_t)y * (w >> 1) + (x >> 1);
+    else idx = (int64_t)(y >> 1) * (w >> 1) + (x >> 1);
+    if (c == 1) base = cb_byte;
+    if (c == 2) base = cr_byte;
+    const uint64_t b = base + (uint64_t)idx * sizeof(T);
+    v = b < src_bytes ? (uint32_t)src[b] : 0u;
+    if constexpr (sizeof(T) == 2) v |= (b + 1 < src_bytes ? (uint32_t)src[b + 1] : 0u) << 8;
+  }
+  dst[(int64_t)c * w * h + (int64_t)y * w + x] = (T)v;
+}
+
+// dst (16-bit planes) = src (8-bit planes), n samples: a picture read back into the reference's
+// uint16 layout.
+__global__ __launch_bounds__(256) void k_widen(const uint8_t* __restrict__ src, int64_t n, uint16_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// The reference's interleaved picture (xPicI: 4 x uint16 per pel, Y Cb Cr A, libs/pmbb-core/src/xPic.h:79-113,
+// xPic.cpp:201-207), its rows packed one after another (4 w samples each), to and from a planar 4:4:4
+// frame of w x h samples; A is 0 in the interleaved form. grid = (ceil(w / 256), h).
+template <typename T>
+__global__ __launch_bounds__(256) void k_deinterleave(const uint16_t* __restrict__ src, int32_t w, int32_t h,
+                                                      T* __restrict__ dst) {
+  const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t y = blockIdx.y;
+  if (x >= w) return;
+  const ushort4 v = *reinterpret_cast<const ushort4*>(src + 4 * ((int64_t)y * w + x));
+  const int64_t plane = (int64_t)w * h, o = (int64_t)y * w + x;
+  dst[o] = (T)v.x;
+  dst[plane + o] = (T)v.y;
+  dst[2 * plane + o] = (T)v.z;
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_interleave(const T* __restrict__ src, int32_t w, int32_t h,
+                                                    uint16_t* __restrict__ dst) {
+  const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t y = blockIdx.y;
+  if (x >= w) return;
+  const int64_t plane = (int64_t)w * h, o = (int64_t)y * w + x;
+  ushort4 v;
+  v.x = src[o];
+  v.y = src[plane + o];
+  v.z = src[2 * plane + o];
+  v.w = 0;
+  *reinterpret_cast<ushort4*>(dst + 4 * o) = v;
+}
+
+// ClipToRange of n 16-bit samples, 8 per thread, when *max_pel (the frame's largest sample from the
+// statistics pass; null: unconditional) exceeds maxv. Every thread reads the flag and leaves at once
+// when the frame is in range, so the launch costs almost nothing then.
+__global__ __launch_bounds__(256) void k_clip(uint16_t* __restrict__ p, int64_t n, int32_t maxv,
+                                              const int32_t* __restrict__ max_pel, int vec) {
+  if (max_pel && *max_pel <= maxv) return;
+  const int64_t i0 = 8 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (i0 >= n) return;
+  const uint16_t m = (uint16_t)maxv;
+  if (vec) {
+    Vec8<uint16_t> x = *(const Vec8<uint16_t>*)(p + i0);
+#pragma unroll
+    for (int j = 0; j < 8; j++) x.v[j] = x.v[j] > m ? m : x.v[j];
+    *(Vec8<uint16_t>*)(p + i0) = x;
+  } else {
+    const int k = n - i0 < 8 ? (int)(n - i0) : 8;
+    for (int j = 0; j < k; j++)
+      if (p[i0 + j] > m) p[i0 + j] = m;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_clip(void* pic, const Geom& g, int32_t maxv, const int32_t* max_pel, hipStream_t stream) {
+  if (g.bytes_per_sample != 2) return hipSuccess;  // 8-bit samples never exceed an 8-bit maximum
+  const int64_t n = (int64_t)g.frame_bytes / 2;
+  if (n <= 0) return hipSuccess;
+  const int vec = (n % 8 == 0) && ((uintptr_t)pic % 16 == 0);
+  const dim3 grid((unsigned)((n + 8 * 256 - 1) / (8 * 256)));
+  k_clip<<<grid, 256, 0, stream>>>((uint16_t*)pic, n, maxv, max_pel, vec);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack(const void* src, uint64_t src_bytes, int32_t w, int32_t h, int32_t bd, int32_t fcf,
+                         void* dst, hipStream_t stream) {
+  const uint64_t cmp = (uint64_t)w * h * (bd <= 8 ? 1 : 2);
+  uint64_t cb = cmp, cr = cmp;
+  if (fcf == 420) cr = cmp + (cmp >> 2);
+  else if (fcf == 422) cr = cmp + (cmp >> 1);
+  else if (fcf == 444) cr = 2 * cmp;
+  else if (fcf != 400) return hipErrorInvalidValue;
+  const uint32_t mid = 1u << (bd - 1);
+  const dim3 grid((unsigned)((w + 255) / 256), (unsigned)h, 3);
+  if (bd <= 8)
+    k_unpack<uint8_t><<<grid, 256, 0, stream>>>((const uint8_t*)src, src_bytes, w, h, fcf, cb, cr, mid, (uint8_t*)dst);
+  else
+    k_unpack<uint16_t><<<grid, 256, 0, stream>>>((const uint8_t*)src, src_bytes, w, h, fcf, cb, cr, mid, (uint16_t*)dst);
+  return hipGetLastError();
+}
 
 hipError_t launch_colour(const void* src, void* dst, const Geom& g, const ColourArgs& k, bool to_rgb,
                          hipStream_t stream) {
@@ -165,6 +285,25 @@ hipError_t launch_upsample(const void* src, const Geom& gs, void* dst, hipStream
     hipLaunchKernelGGL(k_upsample<uint8_t>, grid, dim3(256), 0, stream, (const uint8_t*)src, gs, (uint8_t*)dst);
   else
     hipLaunchKernelGGL(k_upsample<uint16_t>, grid, dim3(256), 0, stream, (const uint16_t*)src, gs, (uint16_t*)dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_widen(const uint8_t* src, int64_t n, uint16_t* dst, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, n, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_interleave(const void* planar, const Geom& g, uint16_t* inter, bool to_interleaved,
+                             hipStream_t stream) {
+  const dim3 grid((unsigned)((g.w + 255) / 256), (unsigned)g.h);
+  if (to_interleaved) {
+    if (g.bytes_per_sample == 1) k_interleave<uint8_t><<<grid, 256, 0, stream>>>((const uint8_t*)planar, g.w, g.h, inter);
+    else k_interleave<uint16_t><<<grid, 256, 0, stream>>>((const uint16_t*)planar, g.w, g.h, inter);
+  } else {
+    if (g.bytes_per_sample == 1) k_deinterleave<uint8_t><<<grid, 256, 0, stream>>>(inter, g.w, g.h, (uint8_t*)planar);
+    else k_deinterleave<uint16_t><<<grid, 256, 0, stream>>>(inter, g.w, g.h, (uint16_t*)planar);
+  }
   return hipGetLastError();
 }
 

@@ -26,10 +26,13 @@ __device__ __forceinline__ uint32_t ldpel(const T* __restrict__ p, int64_t i) {
 // Restates xDistortion::CalcSD / CalcSSD (libs/pmbb-core/src/xDistortionSTD.cpp:18-62) on the
 // 4:2:0 planes: the 4:4:4 sums are 4x the chroma-plane sums (each chroma sample appears 4 times).
 // ---------------------------------------------------------------------------------------------
+// gate (nullable): the summary of a first pass; when it found both pictures in range this pass has
+// nothing to redo (the concealment flow re-runs the statistics only after clipping).
 template <typename T>
 __global__ __launch_bounds__(256) void k_stats(const T* __restrict__ p0, const T* __restrict__ p1, Geom g,
                                                int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd,
-                                               uint32_t* __restrict__ row_max) {
+                                               uint32_t* __restrict__ row_max, const FrameSummary* __restrict__ gate) {
+  if (gate && gate->in_range) return;
   const int lane = threadIdx.x & 63;
   const int nrows = g.h + 2 * g.ch;
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -125,7 +128,13 @@ __global__ __launch_bounds__(1024) void k_summary(Geom g, const int64_t* __restr
                                                  const uint64_t* __restrict__ row_ssd,
                                                  const uint32_t* __restrict__ row_max, int32_t md0, int32_t md1,
                                                  int32_t md2, int32_t md3, int32_t compute_gcd,
-                                                 FrameSummary* __restrict__ sum) {
+                                                 FrameSummary* __restrict__ sum,
+                                                 const FrameSummary* __restrict__ gate) {
+  // Concealment flow: a first pass that found both pictures in range is this pass's result.
+  if (gate && gate->in_range) {
+    if (threadIdx.x == 0) *sum = *gate;
+    return;
+  }
   __shared__ unsigned long long s_sd[3], s_ssd[3];
   __shared__ unsigned int s_max[2];
   if (threadIdx.x < 3) {
@@ -527,7 +536,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_cps_generic(CpsArgs a) {
   __shared__ uint64_t red[3][256];
   const FrameSummary* __restrict__ sum = a.sum;
-  if (!sum->in_range) return;
+  if (!cps_gate_open(a.gate, sum->in_range)) return;
   const Geom g = a.g;
   const int y = blockIdx.y;
   const int dir = blockIdx.z;
@@ -585,7 +594,9 @@ __global__ __launch_bounds__(256) void k_cps_generic(CpsArgs a) {
         scp[c * plane + o] = (uint16_t)clampi(sel[c] - sgn * sum->gcd[c], 0, a.maxv);
     }
   }
-  for (int c = 0; c < 3; c++) red[c][threadIdx.x] = dist_best[c];
+  // The winner's int32 squares, sign-extended into the 64-bit row sums (xCorrespPixelShiftSTD.cpp:35-36:
+  // a square wraps only for differences of 46341 or more, i.e. samples beyond the bit depth).
+  for (int c = 0; c < 3; c++) red[c][threadIdx.x] = (uint64_t)(int64_t)(int32_t)dist_best[c];
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
@@ -599,8 +610,12 @@ __global__ __launch_bounds__(256) void k_cps_generic(CpsArgs a) {
 }
 
 // rows[d][c][y] = sum over segments of the partials.
+// flag / mode: 0 always; 1 only when *flag != 0; 2 only when *flag == 0 (the range-gated search paths).
 __global__ __launch_bounds__(256) void k_cps_rows(const uint64_t* __restrict__ part, int32_t nseg, int32_t h,
-                                                  uint64_t* __restrict__ rows) {
+                                                  uint64_t* __restrict__ rows, const int32_t* __restrict__ flag,
+                                                  int32_t mode) {
+  if (mode == 1 && *flag == 0) return;
+  if (mode == 2 && *flag != 0) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = 6 * (int64_t)h;
   if (i >= n) return;
@@ -612,22 +627,24 @@ __global__ __launch_bounds__(256) void k_cps_rows(const uint64_t* __restrict__ p
 }  // namespace
 
 hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* row_max, hipStream_t stream) {
+                        uint32_t* row_max, hipStream_t stream, const FrameSummary* gate) {
   const int nrows = g.h + 2 * g.ch;
   dim3 grid((nrows + 3) / 4), block(256);
   if (g.bytes_per_sample == 1)
     hipLaunchKernelGGL(k_stats<uint8_t>, grid, block, 0, stream, (const uint8_t*)p0, (const uint8_t*)p1, g, row_sd,
-                       row_ssd, row_max);
+                       row_ssd, row_max, gate);
   else
     hipLaunchKernelGGL(k_stats<uint16_t>, grid, block, 0, stream, (const uint16_t*)p0, (const uint16_t*)p1, g,
-                       row_sd, row_ssd, row_max);
+                       row_sd, row_ssd, row_max, gate);
   return hipGetLastError();
 }
 
 hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* row_ssd, const uint32_t* row_max,
-                          const int32_t maxdiff[4], int32_t compute_gcd, FrameSummary* sum, hipStream_t stream) {
+                          const int32_t maxdiff[4], int32_t compute_gcd, FrameSummary* sum, hipStream_t stream,
+                          const FrameSummary* gate) {
+  if (gate == sum) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_summary, dim3(1), dim3(1024), 0, stream, g, row_sd, row_ssd, row_max, maxdiff[0], maxdiff[1],
-                     maxdiff[2], maxdiff[3], compute_gcd, sum);
+                     maxdiff[2], maxdiff[3], compute_gcd, sum, gate);
   return hipGetLastError();
 }
 
@@ -668,6 +685,7 @@ int cps_fast_blocks_per_cu(int bytes_per_sample) {
 }
 
 hipError_t launch_cps_generic(const CpsArgs& a, hipStream_t stream) {
+  if ((int64_t)a.nseg * kGenericCols < a.g.w) return hipErrorInvalidValue;  // every column needs a block
   dim3 grid(a.nseg, a.g.h, 2), block(kGenericCols);
   if (a.g.bytes_per_sample == 1)
     hipLaunchKernelGGL(k_cps_generic<uint8_t>, grid, block, 0, stream, a);
@@ -676,9 +694,12 @@ hipError_t launch_cps_generic(const CpsArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, uint64_t* rows, hipStream_t stream) {
+hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, uint64_t* rows, hipStream_t stream,
+                           const int32_t* flag, int32_t mode) {
+  if (mode != 0 && !flag) return hipErrorInvalidValue;
   const int64_t n = 6 * (int64_t)h;
-  hipLaunchKernelGGL(k_cps_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, partials, nseg, h, rows);
+  hipLaunchKernelGGL(k_cps_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, partials, nseg, h, rows,
+                     flag, mode);
   return hipGetLastError();
 }
 

@@ -18,6 +18,9 @@ struct Geom {
   uint32_t frame_bytes;  // bytes of one packed frame (< 4 GiB)
   int64_t cb_off;    // sample offset of the Cb plane
   int64_t cr_off;    // sample offset of the Cr plane
+  // 420, 444, or 400: a luma plane only (a mask picture), whose chroma views alias the luma plane
+  // (cs 0, cw x ch = w x h, both chroma offsets 0).
+  int32_t cf;
 };
 
 // Per-frame summary written by the statistics pass (device memory, read back by the host).
@@ -58,7 +61,15 @@ struct CpsArgs {
   uint32_t mask_bytes = 0;
   int32_t mask_bytes_per_sample = 0;
   const MaskSummary* msum = nullptr;
+  // When the search runs, by the summary's range flag (all inputs within the bit depth):
+  // kGateInRange (the fast kernels' only mode), kGateOutOfRange (the general kernels as the fallback
+  // of a frame that holds pels above the bit depth) or kGateAlways.
+  int32_t gate = 0;
 };
+constexpr int32_t kGateInRange = 0, kGateOutOfRange = 1, kGateAlways = 2;
+__host__ __device__ inline bool cps_gate_open(int32_t gate, int32_t in_range) {
+  return gate == kGateAlways || (gate == kGateInRange ? in_range != 0 : in_range == 0);
+}
 
 // One plane read through its 4:4:4 view: value(x, y) = base[(y >> shift) * stride + (x >> shift)].
 struct PlaneView {
@@ -113,13 +124,18 @@ struct SsimBatchArgs {
 };
 
 // Kernel launchers (kernels.hip). All launch on `stream` and return hipSuccess or the launch error.
+// gate (nullable): a first pass's summary; a pass whose gate found both pictures in range does
+// nothing (statistics) or copies the gate (summary): the statistics re-run after concealment.
 hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* row_max, hipStream_t stream);
+                        uint32_t* row_max, hipStream_t stream, const FrameSummary* gate = nullptr);
 hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* row_ssd, const uint32_t* row_max,
-                          const int32_t maxdiff[4], int32_t compute_gcd, FrameSummary* sum, hipStream_t stream);
+                          const int32_t maxdiff[4], int32_t compute_gcd, FrameSummary* sum, hipStream_t stream,
+                          const FrameSummary* gate = nullptr);
 hipError_t launch_cps_fast(const CpsArgs& a, hipStream_t stream);
 hipError_t launch_cps_generic(const CpsArgs& a, hipStream_t stream);
-hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, uint64_t* rows, hipStream_t stream);
+// flag / mode: 0 always; 1 only when *flag != 0; 2 only when *flag == 0.
+hipError_t launch_cps_rows(const uint64_t* partials, int32_t nseg, int32_t h, uint64_t* rows, hipStream_t stream,
+                           const int32_t* flag = nullptr, int32_t mode = 0);
 // ---- Differentiable SSIM family on float planes (kernels_grad.hip) ----
 // Forward moments of one plane pair; samples are src * scale (src fp32, or fp64 when *_f64).
 struct SsimFArgs {
@@ -201,19 +217,22 @@ struct MaskSummary {
   int32_t exact[3];
   int64_t sd[3];
   uint64_t ssd[3];
+  int32_t max_pel[3];      // largest pel of P0, P1 (all planes) and the mask
 };
 
+// gate (nullable): as launch_stats / launch_summary, by all_in_range.
 hipError_t launch_mstats(const void* p0, const void* p1, const Geom& g, const void* mask, const Geom& gm,
-                         const MaskRows& r, hipStream_t st);
+                         const MaskRows& r, hipStream_t st, const MaskSummary* gate = nullptr);
 // gcd_given: null to compute the masked GCD, else the four values to store; nnm_given >= 0
 // replaces the counted pels in the GCD's denominator.
 hipError_t launch_msummary(const Geom& g, int32_t maxm, const MaskRows& r, const int32_t maxdiff[4], int32_t nnm_given,
-                           const int32_t* gcd_given, MaskSummary* out, hipStream_t st);
+                           const int32_t* gcd_given, MaskSummary* out, hipStream_t st,
+                           const MaskSummary* gate = nullptr);
 constexpr int32_t kMaxMaskedSearchRange = 12;
 // partials: [masked_cps_segments(g)][2][3][h], reduced by launch_cps_rows.
 hipError_t launch_cps_masked(const void* p0, const void* p1, const Geom& g, const void* mask, const Geom& gm,
                              const MaskSummary* sum, int32_t B, const int32_t w[4], uint64_t* partials,
-                             hipStream_t st);
+                             hipStream_t st, int32_t gate = kGateInRange);
 size_t masked_cps_lds_bytes(int32_t B);
 int32_t masked_cps_segments(const Geom& g);
 
@@ -227,9 +246,22 @@ struct ColourArgs {
 hipError_t launch_colour(const void* src, void* dst, const Geom& g, const ColourArgs& k, bool to_rgb,
                          hipStream_t stream);
 hipError_t launch_swap_planes(void* pic, const Geom& g, int a, int b, hipStream_t stream);
+// xPixelOps::ClipToRange over every sample of a 16-bit frame (libs/pmbb-core/src/xPixelOpsSTD.cpp:198-209),
+// when *max_pel exceeds maxv (null: always).
+hipError_t launch_clip(void* pic, const Geom& g, int32_t maxv, const int32_t* max_pel, hipStream_t stream);
 hipError_t launch_upsample(const void* src, const Geom& gs, void* dst, hipStream_t stream);
+// One packed raw-file frame (src: src_bytes in device memory, file chroma format fcf 400 / 420 / 422 /
+// 444, any w and h) unpacked into a 4:4:4 frame of the same sample size (xSeqBase::xUnpackFrame).
+hipError_t launch_unpack(const void* src, uint64_t src_bytes, int32_t w, int32_t h, int32_t bd, int32_t fcf,
+                         void* dst, hipStream_t stream);
 // dst[i] = (uint8_t)src[i] for i < n.
 hipError_t launch_narrow(const uint16_t* src, int64_t n, uint8_t* dst, hipStream_t stream);
+// dst[i] = src[i] for i < n.
+hipError_t launch_widen(const uint8_t* src, int64_t n, uint16_t* dst, hipStream_t stream);
+// A planar 4:4:4 frame of g to (to_interleaved) or from the reference's interleaved layout (4 x uint16
+// per pel, rows packed, A = 0).
+hipError_t launch_interleave(const void* planar, const Geom& g, uint16_t* inter, bool to_interleaved,
+                             hipStream_t stream);
 
 // Windowed moments of one evaluation into a.partials ([tiles_x][h] row sums per tile column).
 hipError_t launch_ssim(const SsimArgs& a, hipStream_t stream);

@@ -231,8 +231,9 @@ qmiv_status qmiv_seq_read(qmiv_ctx* ctx, qmiv_seq* seq, int32_t index, qmiv_pic*
   if (!ctx || !seq || !pic) return host_fail(QMIV_ERR_INVALID_ARG, "null argument");
   if (pic->ctx != ctx) return host_fail(QMIV_ERR_INVALID_ARG, "picture belongs to another context");
   if (index < 0 || index >= seq->num_frames) return host_fail(QMIV_ERR_INVALID_ARG, "frame index out of range");
-  if (pic->g.w != seq->w || pic->g.h != seq->h || pic->g.bd != seq->bd || pic->g.cs != (seq->cf == 420 ? 1 : 0))
-    return host_fail(QMIV_ERR_INVALID_ARG, "picture geometry differs from the sequence's");
+  bool unpack = false;
+  const qmiv_status rs = file_route(pic, seq->w, seq->h, seq->bd, seq->cf, unpack);
+  if (rs != QMIV_OK) return rs;
   const qmiv_status es = ensure_slots(seq, ctx);
   if (es != QMIV_OK) return es;
   std::unique_lock<std::mutex> lk(seq->mu);
@@ -269,8 +270,11 @@ qmiv_status qmiv_seq_read(qmiv_ctx* ctx, qmiv_seq* seq, int32_t index, qmiv_pic*
   Slot& sl = seq->slot[i];
   sl.last_use = ++seq->tick;
   if (hipSetDevice(ctx->device) != hipSuccess) return host_fail(QMIV_ERR_HIP, "hipSetDevice failed");
-  hipError_t e = hipMemcpyAsync(pic->dev, sl.host, seq->frame_bytes, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipEventRecord(sl.copied, ctx->stream);
+  // The pinned frame goes straight into the picture, or (another chroma format) through the
+  // context's staging buffer and the device unpack into the 4:4:4 picture.
+  const qmiv_status us = upload_file_frame(ctx, pic, sl.host, seq->frame_bytes, seq->cf);
+  if (us != QMIV_OK) return us;
+  const hipError_t e = hipEventRecord(sl.copied, ctx->stream);
   if (e != hipSuccess) return host_fail(QMIV_ERR_HIP, std::string("frame copy: ") + hipGetErrorString(e));
   sl.copy_pending = true;
   if (index + 1 < seq->num_frames) {

@@ -36,6 +36,64 @@ class Picture:
         self.engine.sync()
         return self
 
+    def upload_unpack(self, frame: np.ndarray, file_chroma_format: int) -> "Picture":
+        """One raw-file frame (bytes) of another chroma format unpacked into this 4:4:4 picture on
+        the device (qmiv_pic_upload_unpack)."""
+        a = np.ascontiguousarray(frame)
+        check(lib().qmiv_pic_upload_unpack(self.engine.ctx, self.handle, a.ctypes.data, a.nbytes, file_chroma_format),
+              "qmiv_pic_upload_unpack")
+        self.engine.sync()
+        return self
+
+    def upload_planar(self, planes: np.ndarray, margin: int = 0) -> "Picture":
+        """The reference's planar layout: planes (3, H + 2 margin, W + 2 margin) uint16, margin not read."""
+        a = np.ascontiguousarray(planes, np.uint16)
+        stride = a.shape[2]
+        ptrs = (C.c_void_p * 3)(*[a[c, margin:, margin:].ctypes.data for c in range(3)])
+        check(lib().qmiv_pic_upload_planar(self.engine.ctx, self.handle, ptrs, stride), "qmiv_pic_upload_planar")
+        self.engine.sync()
+        return self
+
+    def download_planar(self, margin: int = 0, fill: int = 0xFFFF) -> np.ndarray:
+        out = np.full((3, self.height + 2 * margin, self.width + 2 * margin), fill, np.uint16)
+        ptrs = (C.c_void_p * 3)(*[out[c, margin:, margin:].ctypes.data for c in range(3)])
+        check(lib().qmiv_pic_download_planar(self.engine.ctx, self.handle, ptrs, out.shape[2]), "qmiv_pic_download_planar")
+        return out
+
+    def upload_interleaved(self, pels: np.ndarray, margin: int = 0) -> "Picture":
+        """The reference's interleaved layout: (H + 2 margin, W + 2 margin, 4) uint16, margin not read."""
+        a = np.ascontiguousarray(pels, np.uint16)
+        check(lib().qmiv_pic_upload_interleaved(self.engine.ctx, self.handle, a[margin:, margin:].ctypes.data,
+                                                a.shape[1]), "qmiv_pic_upload_interleaved")
+        self.engine.sync()
+        return self
+
+    def download_interleaved(self, margin: int = 0, fill: int = 0xFFFF) -> np.ndarray:
+        out = np.full((self.height + 2 * margin, self.width + 2 * margin, 4), fill, np.uint16)
+        check(lib().qmiv_pic_download_interleaved(self.engine.ctx, self.handle, out[margin:, margin:].ctypes.data,
+                                                  out.shape[1]), "qmiv_pic_download_interleaved")
+        return out
+
+    def check_range(self) -> list:
+        v = (C.c_int32 * 3)()
+        check(lib().qmiv_pic_check(self.engine.ctx, self.handle, v), "qmiv_pic_check")
+        return list(v)
+
+    def conceal(self) -> "Picture":
+        check(lib().qmiv_pic_conceal(self.engine.ctx, self.handle), "qmiv_pic_conceal")
+        self.engine.sync()
+        return self
+
+    def equal(self, other: "Picture") -> list:
+        v = (C.c_int32 * 3)()
+        check(lib().qmiv_pic_equal(self.engine.ctx, self.handle, other.handle, v), "qmiv_pic_equal")
+        return list(v)
+
+    def copy_from(self, other: "Picture") -> "Picture":
+        check(lib().qmiv_pic_copy(self.engine.ctx, self.handle, other.handle), "qmiv_pic_copy")
+        self.engine.sync()
+        return self
+
     def download(self) -> np.ndarray:
         dt = np.uint8 if self.bit_depth <= 8 else np.uint16
         out = np.empty(self.nbytes // np.dtype(dt).itemsize, dt)

@@ -24,7 +24,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-__all__ = ["FramePair", "make_pair", "make_mask", "grad_batch", "config_pair", "sha256", "CONFIGS"]
+__all__ = ["FramePair", "make_pair", "make_mask", "file_frame", "inject_out_of_range", "grad_batch", "config_pair",
+           "sha256", "CONFIGS"]
 
 
 @dataclass
@@ -169,6 +170,52 @@ def make_mask(width: int, height: int, bit_depth: int, seed: int, *, chroma_form
     c = y[::2, ::2] if chroma_format == 420 else y
     dt = np.uint8 if bit_depth <= 8 else np.uint16
     return np.concatenate([y.astype(dt).ravel(), c.astype(dt).ravel(), c.astype(dt).ravel()])
+
+
+def file_frame(planes: np.ndarray, bit_depth: int, chroma_format: int) -> np.ndarray:
+    """One raw-file frame (bytes, uint8) of chroma format 400, 420, 422 or 444 from 4:4:4 planes
+    (3, h, w), in the layout the reference's reader takes (libs/pmbb-core/src/xSeq.cpp:179-249,
+    408-428): luma, then each chroma plane as (luma bytes >> 2) bytes for 4:2:0 and (luma bytes >> 1)
+    for 4:2:2, holding rows of w >> 1 samples taken at even columns (and even rows for 4:2:0),
+    little-endian above 8 bits; the frame is xSeq::create's size. Odd sizes keep the reference's row
+    stride of w >> 1; bytes left over at a plane's or the frame's end are 0."""
+    _, h, w = planes.shape
+    bps = 1 if bit_depth <= 8 else 2
+    dt = np.dtype("u1") if bps == 1 else np.dtype("<u2")
+    luma = np.ascontiguousarray(planes[0].astype(dt)).tobytes()
+    if chroma_format == 400:
+        return np.frombuffer(luma, np.uint8).copy()
+    if chroma_format == 444:
+        return np.frombuffer(b"".join(np.ascontiguousarray(p.astype(dt)).tobytes() for p in planes), np.uint8).copy()
+    cmp = w * h * bps
+    nbytes = cmp >> 2 if chroma_format == 420 else cmp >> 1
+    k = w >> 1
+    out = [luma]
+    for c in (1, 2):
+        sub = planes[c][::2, 0:2 * k:2] if chroma_format == 420 else planes[c][:, 0:2 * k:2]
+        raw = np.ascontiguousarray(sub.astype(dt)).tobytes()[:nbytes]
+        out.append(raw + bytes(nbytes - len(raw)))
+    # The frame is xSeq::create's size: luma + (luma >> 1) bytes for 4:2:0, twice the luma for 4:2:2
+    # (one byte more than the two planes when the luma byte count is odd).
+    total = cmp + (cmp >> 1) if chroma_format == 420 else 2 * cmp
+    data = b"".join(out)
+    return np.frombuffer(data + bytes(total - len(data)), np.uint8).copy()
+
+
+def inject_out_of_range(packed: np.ndarray, bit_depth: int, seed: int, count: int = 200,
+                        top: int | None = None) -> np.ndarray:
+    """A copy of a packed 16-bit frame with `count` samples (anywhere in its planes) set to values
+    in (2^bd - 1, top]: pels above the bit depth, as a broken file holds them. top (default
+    max(4095, 2^(bd + 1) - 1)) stays below 2^14 so that every difference fits the reference's
+    16-bit SIMD lanes."""
+    if top is None:
+        top = max(4095, (2 << bit_depth) - 1)
+    top = min(top, (1 << 14) - 1)
+    rng = np.random.default_rng(np.random.SeedSequence([int(seed), 991]))
+    out = packed.copy()
+    idx = rng.choice(out.size, size=count, replace=False)
+    out[idx] = rng.integers((1 << bit_depth), top + 1, size=count).astype(out.dtype)
+    return out
 
 
 def grad_batch(n: int = 8, planes: int = 3, height: int = 1024, width: int = 1024, seed: int = 5):

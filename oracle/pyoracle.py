@@ -29,6 +29,7 @@ class Params(C.Structure):
         ("erp", C.c_int32), ("lon_range_deg", C.c_int32), ("lat_range_deg", C.c_int32),
         ("cps_int32_rows", C.c_int32),
         ("metrics", C.c_int32),
+        ("conceal", C.c_int32),
     ]
 
 
@@ -124,7 +125,8 @@ def lib():
 
 def params(width, height, bit_depth, chroma_format=420, *, metrics=PSNR | WSPSNR | IVPSNR, erp=False,
            lat_range=180, lon_range=360, search_range=2, weights_search=(4, 1, 1, 0),
-           weights_average=(4, 1, 1, 0), unnoticeable=(0.01, 0.01, 0.01, 0.0), int32_rows=True) -> Params:
+           weights_average=(4, 1, 1, 0), unnoticeable=(0.01, 0.01, 0.01, 0.0), int32_rows=True,
+           conceal=False) -> Params:
     p = Params()
     lib().orc_default_params(C.byref(p), width, height, bit_depth, chroma_format)
     p.metrics = metrics
@@ -137,6 +139,7 @@ def params(width, height, bit_depth, chroma_format=420, *, metrics=PSNR | WSPSNR
         p.cmp_weights_average[i] = weights_average[i]
         p.unnoticeable_coef[i] = unnoticeable[i]
     p.cps_int32_rows = int(int32_rows)
+    p.conceal = int(conceal)
     return p
 
 

@@ -52,33 +52,60 @@ void orc_default_params(orc_params* p, int32_t width, int32_t height, int32_t bi
 
 int orc_unpack(const void* packed, int32_t w, int32_t h, int32_t bd, int32_t cf, uint16_t* y, uint16_t* cb,
                uint16_t* cr) {
+  /* File layout of xSeq::create (xSeq.cpp:408-428): luma, then the chroma planes, each
+     (luma bytes >> 2) for 4:2:0 and (luma bytes >> 1) for 4:2:2; chroma rows of w >> 1 samples
+     (xSeq.cpp:200,216), so for an odd width the last column reads the next row's first sample. A
+     byte past the frame reads as 0. 4:0:0 has no chroma planes; they are set to the mid value. */
+  if (w <= 0 || h <= 0 || bd < 8 || bd > 16) return -1;
   const size_t n = (size_t)w * (size_t)h;
-  size_t cn;
-  int32_t cw;
+  const size_t bps = bd <= 8 ? 1 : 2;
+  const size_t cmp = n * bps;
+  size_t cr_byte, frame;
   if (cf == 420) {
-    if ((w & 1) || (h & 1)) return -1;
-    cw = w / 2;
-    cn = n / 4;
+    cr_byte = cmp + (cmp >> 2);
+    frame = cmp + (cmp >> 1);
+  } else if (cf == 422) {
+    cr_byte = cmp + (cmp >> 1);
+    frame = cmp << 1;
   } else if (cf == 444) {
-    cw = w;
-    cn = n;
+    cr_byte = 2 * cmp;
+    frame = 3 * cmp;
+  } else if (cf == 400) {
+    cr_byte = cmp;
+    frame = cmp;
   } else {
     return -1;
   }
+  const uint8_t* b = (const uint8_t*)packed;
   uint16_t* dst[3] = {y, cb, cr};
   for (int c = 0; c < 3; c++) {
-    const size_t base = c == 0 ? 0 : n + (size_t)(c - 1) * cn;
+    const size_t base = c == 0 ? 0 : (c == 1 ? cmp : cr_byte);
     for (int32_t yy = 0; yy < h; yy++) {
       for (int32_t xx = 0; xx < w; xx++) {
-        size_t src;
-        if (c == 0 || cf == 444) src = base + (size_t)yy * w + xx;
-        else src = base + (size_t)(yy >> 1) * cw + (xx >> 1); /* UpsampleHV, xPixelOpsSTD.cpp:30-49 */
-        uint16_t v = bd <= 8 ? ((const uint8_t*)packed)[src] : ((const uint16_t*)packed)[src];
+        uint16_t v;
+        if (c > 0 && cf == 400) {
+          v = (uint16_t)(1u << (bd - 1));
+        } else {
+          size_t idx;
+          if (c == 0 || cf == 444) idx = (size_t)yy * w + xx;
+          else if (cf == 422) idx = (size_t)yy * (size_t)(w >> 1) + (size_t)(xx >> 1); /* UpsampleH */
+          else idx = (size_t)(yy >> 1) * (size_t)(w >> 1) + (size_t)(xx >> 1);          /* UpsampleHV, xPixelOpsSTD.cpp:30-49 */
+          const size_t o = base + idx * bps;
+          v = o < frame ? b[o] : 0;
+          if (bps == 2) v = (uint16_t)(v | (o + 1 < frame ? (uint16_t)b[o + 1] << 8 : 0));
+        }
         dst[c][(size_t)yy * w + xx] = v;
       }
     }
   }
   return 0;
+}
+
+/* xPixelOpsSTD::ClipToRange (xPixelOpsSTD.cpp:198-209), the concealment of xPicP::conceal. */
+static void clip_to_range(uint16_t* p, size_t n, int32_t bd) {
+  const uint16_t mx = (uint16_t)ORC_MAXV(bd);
+  for (size_t i = 0; i < n; i++)
+    if (p[i] > mx) p[i] = mx;
 }
 
 int orc_in_range(const uint16_t* p, int32_t n, int32_t bd) {
@@ -394,6 +421,12 @@ int orc_frame(const void* p0, const void* p1, const orc_params* prm, orc_frame_r
   }
   out->in_range[0] = orc_in_range(buf, (int32_t)(3 * n), bd);
   out->in_range[1] = orc_in_range(buf + 3 * n, (int32_t)(3 * n), bd);
+  /* InvalidPelActn CNCL (xAppQMIV.cpp:753-762): the pictures that fail the check are clipped and the
+     frame goes on with them; otherwise (WARN, SKIP) with the samples as read. */
+  if (prm->conceal) {
+    if (!out->in_range[0]) clip_to_range(buf, 3 * n, bd);
+    if (!out->in_range[1]) clip_to_range(buf + 3 * n, 3 * n, bd);
+  }
   for (int c = 0; c < 3; c++) out->exact[c] = memcmp(P0[c], P1[c], n * sizeof(uint16_t)) == 0;
 
   const int calc_ivs = (m & (ORC_IVPSNR | ORC_IVSSIM)) != 0;
@@ -616,6 +649,11 @@ int orc_frame_masked(const void* p0, const void* p1, const void* mask, int32_t b
   out->in_range[0] = orc_in_range(buf, (int32_t)(3 * n), bd);
   out->in_range[1] = orc_in_range(buf + 3 * n, (int32_t)(3 * n), bd);
   out->in_range[2] = orc_in_range(buf + 6 * n, (int32_t)(3 * n), bdm);
+  if (prm->conceal) {
+    if (!out->in_range[0]) clip_to_range(buf, 3 * n, bd);
+    if (!out->in_range[1]) clip_to_range(buf + 3 * n, 3 * n, bd);
+    if (!out->in_range[2]) clip_to_range(buf + 6 * n, 3 * n, bdm);
+  }
   for (int c = 0; c < 3; c++) out->exact[c] = memcmp(P0[c], P1[c], n * sizeof(uint16_t)) == 0;
   int32_t nnm = 0; /* xPixelOpsSTD::CountNonZero (xPixelOpsSTD.cpp:273-284) */
   for (size_t i = 0; i < n; i++) nnm += M[i] != 0;

@@ -26,7 +26,7 @@ enum {
 };
 
 typedef struct {
-  int32_t width, height, bit_depth, chroma_format; /* chroma_format: 420 or 444 */
+  int32_t width, height, bit_depth, chroma_format; /* chroma_format: 400, 420, 422 or 444 (file layout) */
   int32_t search_range;                           /* B */
   int32_t cmp_weights_search[4];
   int32_t cmp_weights_average[4];
@@ -35,6 +35,7 @@ typedef struct {
   int32_t lon_range_deg, lat_range_deg;
   int32_t cps_int32_rows;                         /* 1: SSE-path int32 per-row sums (app default), 0: uint64 */
   int32_t metrics;                                /* ORC_* bits */
+  int32_t conceal;                                /* 1: InvalidPelActn CNCL, else the samples as read */
 } orc_params;
 
 typedef struct {
@@ -53,7 +54,8 @@ typedef struct {
 
 void orc_default_params(orc_params* p, int32_t width, int32_t height, int32_t bit_depth, int32_t chroma_format);
 
-/* xSeqBase::xUnpackFrame (libs/pmbb-core/src/xSeq.cpp:179-249): packed planar frame -> 4:4:4 planes. */
+/* xSeqBase::xUnpackFrame (libs/pmbb-core/src/xSeq.cpp:179-249): packed planar file frame (400, 420,
+   422 or 444, any size) -> 4:4:4 planes. */
 int orc_unpack(const void* packed, int32_t w, int32_t h, int32_t bd, int32_t cf, uint16_t* y, uint16_t* cb, uint16_t* cr);
 
 /* xPixelOpsSTD::CheckIfInRange (xPixelOpsSTD.cpp:163-174). */

@@ -93,6 +93,8 @@ struct Args {
   bool Ilp = true;
   bool Rows = false;       // dump per-row arrays (small pictures)
   bool Pics = false;       // dump SCP / pyramid pictures (tiny pictures)
+  bool Conceal = false;    // InvalidPelActn CNCL: conceal the pictures that fail the check
+  std::string Dump;        // write frame 0's unpacked pictures (P0, P1, mask: 3 planes W x H uint16 each)
   bool Time = false;       // per-stage timing
   bool M[6] = {true, true, true, false, false, false};  // PSNR WSPSNR IVPSNR SSIM MSSSIM IVSSIM
 };
@@ -116,7 +118,8 @@ void usage() {
   fprintf(stderr,
           "qmiv_ref -w W -h H -bd BD -cf 420|444 -i0 FILE -i1 FILE [-im MASK -bdm BD -cfm CF] [-n FRAMES] [-erp LON LAT]\n"
           "         [-sr B] [-cws a:b:c:d] [-cwa a:b:c:d] [-ucf a:b:c:d] [-nth T] [-ilp 0|1]\n"
-          "         [-ml PSNR,WSPSNR,IVPSNR,SSIM,MSSSIM,IVSSIM|All] [-rows] [-pics] [-time]\n");
+          "         [-ml PSNR,WSPSNR,IVPSNR,SSIM,MSSSIM,IVSSIM|All] [-rows] [-pics] [-time]\n"
+          "         [-ipa STOP|WARN|SKIP|CNCL] [-dump FILE]\n");
 }
 
 bool parse(int argc, char** argv, Args& A) {
@@ -143,6 +146,8 @@ bool parse(int argc, char** argv, Args& A) {
     else if (K == "-rows") { A.Rows = true; }
     else if (K == "-pics") { A.Pics = true; }
     else if (K == "-time") { A.Time = true; }
+    else if (K == "-ipa") { need(1); A.Conceal = std::string(argv[++i]) == "CNCL"; }
+    else if (K == "-dump") { need(1); A.Dump = argv[++i]; }
     else if (K == "-ml") {
       need(1);
       std::string L = argv[++i];
@@ -340,6 +345,22 @@ int main(int argc, char** argv) {
     }
     auto t1 = Clock::now();
     bool InRange0 = P0.check(A.I0), InRange1 = P1.check(A.I1), InRangeM = UseMask ? PM.check(A.IM) : true;
+    // xAppQMIV::validateFrames (xAppQMIV.cpp:753-770): CNCL conceals; WARN / SKIP go on as read.
+    if (A.Conceal) {
+      if (!InRange0) P0.conceal();
+      if (!InRange1) P1.conceal();
+      if (UseMask && !InRangeM) PM.conceal();
+    }
+    if (f == 0 && !A.Dump.empty()) {
+      FILE* D = fopen(A.Dump.c_str(), "wb");
+      if (!D) { fprintf(stderr, "cannot write %s\n", A.Dump.c_str()); return 1; }
+      for (xPicP* Pic : {&P0, &P1, &PM}) {
+        if (Pic == &PM && !UseMask) break;
+        for (int c = 0; c < 3; c++)
+          for (int y = 0; y < A.H; y++) fwrite(Pic->getAddr((eCmp)c) + y * Pic->getStride(), 2, A.W, D);
+      }
+      fclose(D);
+    }
     auto t2 = Clock::now();
     boolV4 Exact = xMakeVec4<bool>(false);
     for (int c = 0; c < 3; c++) Exact[c] = P0.equalCmp(&P1, (eCmp)c);

@@ -10,6 +10,7 @@ the reference computes on them are stored.
     python tests/golden/make_golden.py --png      # PNG decoding (oracle/_ref/libspng_ref.so)
     python tests/golden/make_golden.py --multiview   # 24-view frame sets (4096x2048 ERP, 1920x1080, 2048x1088)
     python tests/golden/make_golden.py --ssimsets    # SSIM-family frame sets (16 x 1080p10, mixed sizes)
+    python tests/golden/make_golden.py --formats     # 4:2:2, odd-width 4:2:0, 4:0:0 masks, out-of-range pels
 """
 from __future__ import annotations
 
@@ -39,7 +40,14 @@ def run_ref(w, h, bd, cf, f0, f1, metrics, erp=False, extra=()):
     cmd = [REF, "-w", str(w), "-h", str(h), "-bd", str(bd), "-cf", str(cf), "-i0", f0, "-i1", f1,
            "-ml", metrics, "-nth", "8"] + (["-erp", "360", "180"] if erp else []) + list(extra)
     r = subprocess.run(cmd, capture_output=True, text=True, check=True)
-    return json.loads(r.stdout)
+    # xPicP::check prints its findings on stdout (libs/pmbb-core/src/xPic.cpp:62-82), inside the
+    # harness's JSON text: split them off, keeping them as the check's report.
+    import re
+    report = re.findall(r"(?:FILE BROKEN|DETECTED)[^\n]*\n", r.stdout)
+    out = json.loads(re.sub(r"(?:FILE BROKEN|DETECTED)[^\n]*\n", "", r.stdout))
+    if report:
+        out["check_report"] = "".join(report)
+    return out
 
 
 def write_pairs(td, pairs):
@@ -207,6 +215,63 @@ def ssim_set_fixtures():
     return sets
 
 
+def format_fixtures():
+    """Raw-file formats the reference's reader unpacks into 4:4:4 (xSeq.cpp:179-249): 4:2:2, 4:2:0 of
+    odd width, a 4:0:0 mask; and pictures holding pels above the bit depth, computed as read
+    (InvalidPelActn WARN / SKIP) and concealed (CNCL). Inputs: tests/inputs.py."""
+    from tests.inputs import FORMAT_CASES, MASK400_CASES, RANGE_CASES, format_pair, mask400_frame, range_mask, range_pair
+    out = {"formats": [], "mask400": [], "range": []}
+    with tempfile.TemporaryDirectory() as td:
+        f0, f1, fm, fd = (os.path.join(td, n) for n in ("p0.yuv", "p1.yuv", "m.yuv", "dump.bin"))
+        for (w, h, bd, fcf, seed) in FORMAT_CASES:
+            a, b = format_pair(w, h, bd, fcf, seed)
+            a.tofile(f0)
+            b.tofile(f1)
+            r = run_ref(w, h, bd, fcf, f0, f1, "All", False, ["-rows", "-dump", fd])
+            planes = np.fromfile(fd, "<u2")
+            out["formats"].append({"width": w, "height": h, "bit_depth": bd, "chroma_format": fcf, "seed": seed,
+                                   "sha256_p0": hashlib.sha256(a.tobytes()).hexdigest(),
+                                   "sha256_unpacked": hashlib.sha256(planes.tobytes()).hexdigest(),
+                                   "frame": compact_frame(r["frames"][0], keep_rows=False)})
+            print("format", w, h, bd, fcf, file=sys.stderr)
+        for (w, h, bd, bdm, seed) in MASK400_CASES:
+            pair = make_pair(w, h, bd, seed)
+            write_pairs(td, [pair])
+            mf = mask400_frame(w, h, bdm, seed)
+            mf.tofile(fm)
+            e = {"width": w, "height": h, "bit_depth": bd, "mask_bit_depth": bdm, "seed": seed,
+                 "sha256_mask": hashlib.sha256(mf.tobytes()).hexdigest()}
+            for erp in (False, True):
+                r = run_ref(w, h, bd, 420, f0, f1, "PSNR,WSPSNR,IVPSNR", erp,
+                            ["-rows", "-im", fm, "-bdm", str(bdm), "-cfm", "400"])
+                e["erp" if erp else "plain"] = compact_frame(r["frames"][0], keep_rows=False)
+            out["mask400"].append(e)
+            print("mask400", w, h, bd, bdm, file=sys.stderr)
+        for (w, h, bd, cf, seed, inj0, inj1, bdm) in RANGE_CASES:
+            a, b = range_pair(w, h, bd, cf, seed, inj0, inj1)
+            a.tofile(f0)
+            b.tofile(f1)
+            e = {"width": w, "height": h, "bit_depth": bd, "chroma_format": cf, "seed": seed, "inject": [inj0, inj1],
+                 "mask_bit_depth": bdm, "sha256_p0": sha256(a), "sha256_p1": sha256(b)}
+            extra = ["-rows"]
+            ml = "All"
+            if bdm:
+                mk = range_mask(w, h, bdm, cf, seed)
+                mk.tofile(fm)
+                e["sha256_mask"] = sha256(mk)
+                extra += ["-im", fm, "-bdm", str(bdm), "-cfm", str(cf)]
+                ml = "PSNR,WSPSNR,IVPSNR"
+            for mode in ("WARN", "CNCL"):
+                r = run_ref(w, h, bd, cf, f0, f1, ml, False, extra + ["-ipa", mode])
+                e[mode] = compact_frame(r["frames"][0], keep_rows=False)
+            # The check's report with the file names as placeholders ({i0}, {i1}, {im}).
+            rep = r.get("check_report", "").replace(f0, "{i0}").replace(f1, "{i1}").replace(fm, "{im}")
+            e["check_report"] = rep
+            out["range"].append(e)
+            print("range", w, h, bd, cf, file=sys.stderr)
+    return out
+
+
 def png_fixtures():
     """The reference's PNG read step (libspng as xSeqPNG calls it, oracle/spng_ref.c) on the encoded
     cases of tests/png_writer.py: decode status and the SHA-256 of the R, G, B planes."""
@@ -242,6 +307,10 @@ def main():
     if "--multiview" in sys.argv:
         with open(os.path.join(HERE, "multiview24.json"), "w") as f:
             json.dump(multiview_fixtures(), f, separators=(",", ":"))
+        return
+    if "--formats" in sys.argv:
+        with open(os.path.join(HERE, "formats.json"), "w") as f:
+            json.dump(format_fixtures(), f, separators=(",", ":"))
         return
     if "--ssimsets" in sys.argv:
         with open(os.path.join(HERE, "ssim_sets.json"), "w") as f:

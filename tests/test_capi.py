@@ -32,11 +32,15 @@ def test_python_binding_covers_header():
 
 def test_cpu_only_calls():
     lib = capi.lib()
-    assert lib.qmiv_abi_version() == 1
+    assert lib.qmiv_abi_version() == 2
     assert lib.qmiv_frame_bytes(1920, 1080, 8, 420) == 1920 * 1080 * 3 // 2
     assert lib.qmiv_frame_bytes(4096, 2048, 10, 420) == 4096 * 2048 * 3
     assert lib.qmiv_frame_bytes(67, 45, 8, 444) == 67 * 45 * 3
-    assert lib.qmiv_frame_bytes(67, 45, 8, 420) == 0  # odd 4:2:0 sizes are rejected
+    # raw-file frames of every chroma format, odd sizes too (xSeq::create: luma + chroma bytes)
+    assert lib.qmiv_frame_bytes(67, 45, 8, 420) == 67 * 45 + (67 * 45) // 2
+    assert lib.qmiv_frame_bytes(67, 45, 10, 422) == 2 * (2 * 67 * 45)
+    assert lib.qmiv_frame_bytes(67, 45, 10, 400) == 2 * 67 * 45
+    assert lib.qmiv_frame_bytes(64, 48, 8, 411) == 0  # not a chroma format
     p = capi.default_params()
     assert p.search_range == 2 and list(p.cmp_weights_search) == [4, 1, 1, 0]
     assert abs(p.unnoticeable_coef[0] - 0.01) < 1e-9 and p.cps_int32_rows == 1

@@ -134,8 +134,10 @@ def test_large_given_gcd_matches_oracle(engine, oracle, bd, cf, g):
     r = engine.calc_ivpsnr_masked(p0, p1, pm, g, -1, default_params())
     prof = engine.profile_read(reset=True)
     engine.profile(False)
-    # the fast masked search ran, not the general kernel
-    assert prof.get("cps", (0, 0))[1] == 1 and "cps_general" not in prof, prof
+    # the fast masked search was planned (a case that drops to the general search enqueues no fast
+    # kernel); processor-level calls validate nothing, so the general kernel is enqueued beside it,
+    # gated by the range flag, and returns at once for this in-range frame
+    assert prof.get("cps", (0, 0))[1] == 1, prof
     nnm = int((M != 0).sum())
     want = []
     for rows in (oracle.cps_rows_masked(P1, P0, M, [-v for v in g]), oracle.cps_rows_masked(P0, P1, M, g)):

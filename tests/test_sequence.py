@@ -51,7 +51,10 @@ def test_errors(tmp_path):
     path = tmp_path / "s.yuv"
     write_frames(path, 64, 32, 8, 420, 2)
     with pytest.raises(QmivError):
-        Sequence(str(path), 63, 32, 8, 420)  # 4:2:0 needs even sizes
+        Sequence(str(path), 64, 32, 8, 411)  # not a chroma format
+    odd = Sequence(str(path), 63, 32, 8, 420)  # odd widths open as the reference's reader does
+    assert odd.frame_bytes == 63 * 32 + (63 * 32) // 2
+    odd.close()
     with pytest.raises(QmivError):
         Sequence(str(path), 64, 32, 7, 420)
     seq = Sequence(str(path), 64, 32, 8, 420)
