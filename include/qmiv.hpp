@@ -7,6 +7,7 @@
 
     reference (namespace PMBB)                       here (namespace qmiv)
     xPicP                (libs/pmbb-core/src/xPic.h)   xPicP: planar 4:4:4 picture in device memory
+    xPicI                (libs/pmbb-core/src/xPic.h)   xPicI: the interleaved picture's role (a device picture)
     xGlobClrDiffProc     (xGlobClrDiff.h:38-43)        xGlobClrDiffProc
     xShftCompPicProc     (xShftCompPic.h:33-44)        xShftCompPicProc
     xPSNR / xWSPSNR / xIVPSNR (xPSNR.h, xWSPSNR.h, xIVPSNR.h)   xIVPSNR (PSNR, WS-PSNR, IV-PSNR)
@@ -18,9 +19,11 @@
 
 #include <array>
 #include <cstdint>
+#include <cstdio>
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "qmiv.h"
 
@@ -63,25 +66,32 @@ class xContext {
   qmiv_ctx* m_Ctx = nullptr;
 };
 
-// Planar 4:4:4 picture in device memory (the reference's xPicP without margins: edges are
-// clamped by the kernels). Samples are 8-bit at bit depth 8, else 16-bit.
+// Picture components (eCmp, libs/pmbb-core/src/xCommonDefCORE.h): Y, Cb, Cr or R, G, B planes.
+enum class eCmp : int32 { LM = 0, CB = 1, CR = 2, C0 = 0, C1 = 1, C2 = 2, R = 0, G = 1, B = 2 };
+using boolV4 = std::array<bool, 4>;
+
+// Planar 4:4:4 picture in device memory (the reference's xPicP, xPic.h:18-74). The device copy has
+// no margin: the kernels clamp at the edges, which reads what ExtendMargin replicates
+// (xPixelOpsSTD.cpp:210-236), so extend() has nothing to do. Margin and getStride() describe the
+// reference's host layout (uint16 planes, rows W + 2 Margin apart) that fromPlanar / toPlanar
+// exchange. Device samples are 8-bit at bit depth 8, else 16-bit.
 class xPicP {
  public:
-  xPicP(xContext& Ctx, int32V2 Size, int32 BitDepth) : m_Ctx(&Ctx) { create(Size, BitDepth); }
+  xPicP(xContext& Ctx, int32V2 Size, int32 BitDepth, int32 Margin = 4) : m_Ctx(&Ctx) { create(Size, BitDepth, Margin); }
   ~xPicP() { destroy(); }
   xPicP(const xPicP&) = delete;
   xPicP& operator=(const xPicP&) = delete;
 
-  void create(int32V2 Size, int32 BitDepth) {
+  void create(int32V2 Size, int32 BitDepth, int32 Margin = 4) {
     destroy();
     m_Size = Size;
     m_BitDepth = BitDepth;
+    m_Margin = Margin;
     xCheck(qmiv_pic_create(m_Ctx->get(), Size[0], Size[1], BitDepth, 444, &m_Pic), "qmiv_pic_create");
   }
   void destroy() {
     qmiv_pic_destroy(m_Pic);
-    qmiv_pic_destroy(m_Staging);
-    m_Pic = m_Staging = nullptr;
+    m_Pic = nullptr;
   }
 
   int32V2 getSize() const { return m_Size; }
@@ -90,6 +100,9 @@ class xPicP {
   int32 getArea() const { return m_Size[0] * m_Size[1]; }
   int32 getBitDepth() const { return m_BitDepth; }
   int32 getMaxPelValue() const { return (1 << m_BitDepth) - 1; }
+  int32 getMargin() const { return m_Margin; }
+  int32 getStride() const { return m_Size[0] + 2 * m_Margin; }
+  int32 getNumCmps() const { return 3; }
   bool isCompatible(const xPicP* Ref) const { return Ref && Ref->m_Size == m_Size && Ref->m_BitDepth == m_BitDepth; }
   qmiv_pic* handle() const { return m_Pic; }
   qmiv_ctx* context() const { return m_Ctx->get(); }
@@ -99,23 +112,66 @@ class xPicP {
   bool swapComponents(int32 CmpIdA, int32 CmpIdB) {
     return qmiv_pic_swap_planes(m_Ctx->get(), m_Pic, CmpIdA, CmpIdB) == QMIV_OK;
   }
+  bool swapComponents(eCmp CmpIdA, eCmp CmpIdB) { return swapComponents((int32)CmpIdA, (int32)CmpIdB); }
+
+  // xPicP::copy (xPic.cpp:46-51).
+  void copy(const xPicP* Src) { xCheck(qmiv_pic_copy(m_Ctx->get(), m_Pic, Src->m_Pic), "qmiv_pic_copy"); }
+  // xPicP::extend (xPic.cpp:91-95): the device kernels clamp at the edges instead.
+  void extend() {}
+  // xPicP::check (xPic.cpp:62-82): false when a component holds a sample above 2^BitDepth - 1, after
+  // printing, as the reference does, "FILE BROKEN <Name> (CMP=c)" for the first such component and
+  // one "DETECTED out-of-range value" line per offending sample (xPixelOpsSTD::FindOutOfRange,
+  // xPixelOpsSTD.cpp:175-197) to stdout.
+  bool check(const std::string& Name) const {
+    int32 Ok[3] = {1, 1, 1};
+    xCheck(qmiv_pic_check(m_Ctx->get(), m_Pic, Ok), "qmiv_pic_check");
+    if (Ok[0] && Ok[1] && Ok[2]) return true;
+    std::vector<uint16_t> P = toPlanarBuffer();
+    const int32 W = getWidth(), H = getHeight(), Max = getMaxPelValue();
+    for (int32 c = 0; c < 3; c++) {
+      if (Ok[c]) continue;
+      std::printf("FILE BROKEN %s (CMP=%d)\n", Name.c_str(), c);
+      const uint16_t* S = P.data() + (size_t)c * W * H;
+      for (int32 y = 0; y < H; y++)
+        for (int32 x = 0; x < W; x++)
+          if (S[(size_t)y * W + x] > Max)
+            std::printf("DETECTED out-of-range value (y=%d, x=%d, VALUE=%d, Expected=[0-%d])\n", y, x,
+                        (int)S[(size_t)y * W + x], Max);
+      return false;
+    }
+    return false;
+  }
+  // xPicP::conceal (xPic.cpp:83-90): samples above 2^BitDepth - 1 clipped to it, on the device.
+  void conceal() { xCheck(qmiv_pic_conceal(m_Ctx->get(), m_Pic), "qmiv_pic_conceal"); }
+  // xPicP::equalCmp / equalCmps (xPic.cpp:103-114).
+  bool equalCmp(const xPicP* Src, eCmp CmpId) const { return equalCmps(Src)[(int32)CmpId]; }
+  boolV4 equalCmps(const xPicP* Src) const {
+    int32 E[3] = {0, 0, 0};
+    xCheck(qmiv_pic_equal(m_Ctx->get(), m_Pic, Src->m_Pic, E), "qmiv_pic_equal");
+    return {E[0] != 0, E[1] != 0, E[2] != 0, true};
+  }
+  bool equalPic(const xPicP* Src) const {
+    const boolV4 E = equalCmps(Src);
+    return E[0] && E[1] && E[2];
+  }
+
+  // The reference's host layout (xPicCommon.cpp:16-33): Origins[c] points at plane c's first
+  // interior sample, rows Stride samples apart (getStride() for this picture's margin); margins are
+  // neither read nor written.
+  void fromPlanar(const uint16_t* const Origins[3], int32 Stride) {
+    xCheck(qmiv_pic_upload_planar(m_Ctx->get(), m_Pic, Origins, Stride), "qmiv_pic_upload_planar");
+  }
+  void toPlanar(uint16_t* const Origins[3], int32 Stride) const {
+    xCheck(qmiv_pic_download_planar(m_Ctx->get(), m_Pic, Origins, Stride), "qmiv_pic_download_planar");
+  }
 
   // xSeqBase::readFrame's unpack (libs/pmbb-core/src/xSeq.cpp:179-249): one packed planar frame
-  // as read from a raw file (ChromaFormat 420 or 444) becomes this 4:4:4 picture; 4:2:0 chroma is
-  // upsampled nearest-neighbour on the device.
+  // as read from a raw file (ChromaFormat 400, 420, 422 or 444, any size) becomes this 4:4:4
+  // picture; chroma is upsampled nearest-neighbour on the device.
   void fromPacked(const void* Packed, int32 ChromaFormat) {
-    if (ChromaFormat == 444) {
-      xCheck(qmiv_pic_upload(m_Ctx->get(), m_Pic, Packed, qmiv_frame_bytes(getWidth(), getHeight(), m_BitDepth, 444)),
-             "qmiv_pic_upload");
-      return;
-    }
-    if (!m_Staging)
-      xCheck(qmiv_pic_create(m_Ctx->get(), getWidth(), getHeight(), m_BitDepth, ChromaFormat, &m_Staging),
-             "qmiv_pic_create");
-    xCheck(qmiv_pic_upload(m_Ctx->get(), m_Staging, Packed,
-                           qmiv_frame_bytes(getWidth(), getHeight(), m_BitDepth, ChromaFormat)),
-           "qmiv_pic_upload");
-    xCheck(qmiv_pic_upsample(m_Ctx->get(), m_Staging, m_Pic), "qmiv_pic_upsample");
+    xCheck(qmiv_pic_upload_unpack(m_Ctx->get(), m_Pic, Packed,
+                                  qmiv_frame_bytes(getWidth(), getHeight(), m_BitDepth, ChromaFormat), ChromaFormat),
+           "qmiv_pic_upload_unpack");
   }
   // Planar 4:4:4 samples (Y, Cb, Cr; W x H each; 1 or 2 bytes per sample) in and out.
   void upload(const void* Planes) {
@@ -128,19 +184,69 @@ class xPicP {
   }
 
  private:
+  std::vector<uint16_t> toPlanarBuffer() const {
+    const size_t A = (size_t)getArea();
+    std::vector<uint16_t> P(3 * A);
+    uint16_t* const O[3] = {P.data(), P.data() + A, P.data() + 2 * A};
+    toPlanar(O, getWidth());
+    return P;
+  }
+
   xContext* m_Ctx;
   qmiv_pic* m_Pic = nullptr;
-  qmiv_pic* m_Staging = nullptr;  // packed 4:2:0 upload buffer of fromPacked
   int32V2 m_Size = {0, 0};
   int32 m_BitDepth = 0;
+  int32 m_Margin = 4;
+};
+
+// The reference's interleaved picture (xPicI, xPic.h:79-113: 4 x uint16 per pel), the form its
+// interleaved IV-PSNR processors take (xIVPSNR.h:35,49; the application's default,
+// xAppQMIV.cpp:314,892). Interleaving is a CPU memory layout; on the device the picture is the
+// same planar 4:4:4 picture, so rearrangeFromPlanar is a device copy. fromInterleaved /
+// toInterleaved exchange the reference's host layout (origin at the first interior pel, rows Stride
+// pels apart, A written 0).
+class xPicI {
+ public:
+  xPicI(xContext& Ctx, int32V2 Size, int32 BitDepth, int32 Margin = 4)
+      : m_Ctx(&Ctx), m_Size(Size), m_BitDepth(BitDepth), m_Margin(Margin) {
+    xCheck(qmiv_pic_create(Ctx.get(), Size[0], Size[1], BitDepth, 444, &m_Pic), "qmiv_pic_create");
+  }
+  ~xPicI() { qmiv_pic_destroy(m_Pic); }
+  xPicI(const xPicI&) = delete;
+  xPicI& operator=(const xPicI&) = delete;
+
+  int32V2 getSize() const { return m_Size; }
+  int32 getWidth() const { return m_Size[0]; }
+  int32 getHeight() const { return m_Size[1]; }
+  int32 getBitDepth() const { return m_BitDepth; }
+  int32 getMargin() const { return m_Margin; }
+  int32 getStride() const { return m_Size[0] + 2 * m_Margin; }
+  bool isCompatible(const xPicI* Ref) const { return Ref && Ref->m_Size == m_Size && Ref->m_BitDepth == m_BitDepth; }
+  qmiv_pic* handle() const { return m_Pic; }
+
+  // xPicI::rearrangeFromPlanar (xPic.cpp:201-207).
+  void rearrangeFromPlanar(const xPicP* Src) { xCheck(qmiv_pic_copy(m_Ctx->get(), m_Pic, Src->handle()), "qmiv_pic_copy"); }
+  void fromInterleaved(const uint16_t* Origin, int32 Stride) {
+    xCheck(qmiv_pic_upload_interleaved(m_Ctx->get(), m_Pic, Origin, Stride), "qmiv_pic_upload_interleaved");
+  }
+  void toInterleaved(uint16_t* Origin, int32 Stride) const {
+    xCheck(qmiv_pic_download_interleaved(m_Ctx->get(), m_Pic, Origin, Stride), "qmiv_pic_download_interleaved");
+  }
+
+ private:
+  xContext* m_Ctx;
+  qmiv_pic* m_Pic = nullptr;
+  int32V2 m_Size;
+  int32 m_BitDepth;
+  int32 m_Margin;
 };
 
 // Read side of the reference's sequences: xSeqBase (libs/pmbb-core/src/xSeq.h:17-150,
 // xSeq.cpp:28-174) with the raw backend xSeq and the PNG-list backend xSeqPNG
 // (libs/pmbb-spng/src/xSeqPNG.h:14-38). Frames are read by a background thread into pinned buffers
 // and copied to the device asynchronously on the picture's context (qmiv_seq_*); readFrame gives
-// the 4:4:4 picture the reference's unpack gives (4:2:0 chroma upsampled on the device). Only
-// eMode::Read is implemented.
+// the 4:4:4 picture the reference's unpack gives (chroma of 4:2:0, 4:2:2 and 4:0:0 files unpacked
+// on the device). Only eMode::Read is implemented.
 class xSeqBase {
  public:
   enum class eMode : int32 { Unknown, Read, Write, Append };
@@ -173,9 +279,7 @@ class xSeqBase {
     return eRetv::Success;
   }
   tResult closeFile() {
-    qmiv_pic_destroy(m_Staging);
     qmiv_seq_close(m_Seq);
-    m_Staging = nullptr;
     m_Seq = nullptr;
     m_NumOfFrames = 0;
     m_CurrFrameIdx = 0;
@@ -195,15 +299,8 @@ class xSeqBase {
     if (!m_Seq) return tResult(eRetv::Error, "OpMode does not allow Read");
     if (m_CurrFrameIdx >= m_NumOfFrames) return eRetv::EndOfFile;
     if (!Pic || Pic->getSize() != m_Size || Pic->getBitDepth() != m_BitDepth) return eRetv::WrongArg;
-    qmiv_pic* Dst = Pic->handle();
-    if (m_ChromaFormat != 444) {
-      if (!m_Staging && qmiv_pic_create(Pic->context(), m_Size[0], m_Size[1], m_BitDepth, m_ChromaFormat,
-                                        &m_Staging) != QMIV_OK)
-        return tResult(eRetv::Error, qmiv_last_error());
-      Dst = m_Staging;
-    }
-    if (qmiv_seq_read(Pic->context(), m_Seq, m_CurrFrameIdx, Dst) != QMIV_OK ||
-        (Dst != Pic->handle() && qmiv_pic_upsample(Pic->context(), Dst, Pic->handle()) != QMIV_OK))
+    // The engine unpacks 4:2:0, 4:2:2 and 4:0:0 frames into the 4:4:4 picture on the device.
+    if (qmiv_seq_read(Pic->context(), m_Seq, m_CurrFrameIdx, Pic->handle()) != QMIV_OK)
       return tResult(eRetv::Error, qmiv_last_error());
     m_CurrFrameIdx++;
     return eRetv::Success;
@@ -214,7 +311,6 @@ class xSeqBase {
   virtual qmiv_status xBackendOpen(const std::string& FileName) = 0;
 
   qmiv_seq* m_Seq = nullptr;
-  qmiv_pic* m_Staging = nullptr;  // 4:2:0 frame before the upsampling of readFrame
   int32V2 m_Size;
   int32 m_BitDepth;
   int32 m_ChromaFormat;
@@ -222,7 +318,7 @@ class xSeqBase {
   int32 m_CurrFrameIdx = 0;
 };
 
-// Raw planar file of packed frames (ChromaFormat 420 or 444), xSeq(Size, BitDepth, ChromaFormat).
+// Raw planar file of packed frames (ChromaFormat 400, 420, 422 or 444), xSeq(Size, BitDepth, ChromaFormat).
 class xSeq : public xSeqBase {
  public:
   xSeq(int32V2 Size, int32 BitDepth, int32 ChromaFormat) : xSeqBase(Size, BitDepth, ChromaFormat) {}
@@ -375,13 +471,7 @@ class xIVPSNR : public xProcPrms {
   // xIVPSNRM::calcPicIVPSNRM(Tst, Ref, Msk, NumNonMasked, GCD) (xIVPSNR.cpp:152-166).
   flt64 calcPicIVPSNRM(const xPicP* Tst, const xPicP* Ref, const xPicP* Msk, int32 NumNonMasked,
                        const int32V4& GlobalColorDiffRef2Tst) {
-    flt64 IV = 0, R2T = 0, T2R = 0;
-    const qmiv_params P = xMaskedPrm();
-    xCheck(qmiv_calc_ivpsnr_masked(m_Ctx->get(), Tst->handle(), Ref->handle(), Msk->handle(), &P, NumNonMasked,
-                                   GlobalColorDiffRef2Tst.data(), &IV, &R2T, &T2R),
-           "qmiv_calc_ivpsnr_masked");
-    if (m_DebugCallbackQAP) m_DebugCallbackQAP(R2T, T2R);
-    return IV;
+    return xIVM(Tst->handle(), Ref->handle(), Msk, NumNonMasked, GlobalColorDiffRef2Tst);
   }
   // xIVPSNRM::calcPicIVPSNRM(Tst, Ref, Msk, TstI, RefI) (xIVPSNR.cpp:138-150): counts the mask and
   // computes the masked colour difference itself.
@@ -409,22 +499,44 @@ class xIVPSNR : public xProcPrms {
   }
   // xIVPSNR::calcPicIVPSNR(Tst, Ref, GlobalColorDiffRef2Tst) (xIVPSNR.cpp:33-45).
   flt64 calcPicIVPSNR(const xPicP* Tst, const xPicP* Ref, const int32V4& GlobalColorDiffRef2Tst) {
-    flt64 IV = 0, R2T = 0, T2R = 0;
-    xCheck(qmiv_calc_ivpsnr(m_Ctx->get(), Tst->handle(), Ref->handle(), &m_Prm, GlobalColorDiffRef2Tst.data(), &IV,
-                            &R2T, &T2R, nullptr),
-           "qmiv_calc_ivpsnr");
-    if (m_DebugCallbackQAP) m_DebugCallbackQAP(R2T, T2R);
-    return IV;
+    return xIV(Tst->handle(), Ref->handle(), GlobalColorDiffRef2Tst);
+  }
+  // The interleaved form the application calls (xIVPSNR.h:35, xIVPSNR.cpp:45-57; xAppQMIV.cpp:892).
+  flt64 calcPicIVPSNR(const xPicI* Tst, const xPicI* Ref, const int32V4& GlobalColorDiffRef2Tst) {
+    return xIV(Tst->handle(), Ref->handle(), GlobalColorDiffRef2Tst);
+  }
+  // xIVPSNRM::calcPicIVPSNRM(const xPicI* Tst, const xPicI* Ref, const xPicP* Msk, NNM, GCD)
+  // (xIVPSNR.h:49, xIVPSNR.cpp:152-166), the application's masked call (xAppQMIV.cpp:889).
+  flt64 calcPicIVPSNRM(const xPicI* Tst, const xPicI* Ref, const xPicP* Msk, int32 NumNonMasked,
+                       const int32V4& GlobalColorDiffRef2Tst) {
+    return xIVM(Tst->handle(), Ref->handle(), Msk, NumNonMasked, GlobalColorDiffRef2Tst);
   }
 
  private:
   flt64V4 xPerCmp(const xPicP* Tst, const xPicP* Ref, uint32_t Metric) {
     qmiv_params P = m_Prm;
     P.fake_psnr_for_exact = m_FakeValsForExact ? 1 : 0;
+    P.invalid_pel_action = QMIV_PEL_SKIP;  // processors validate nothing; xPicP::check is the caller's
     qmiv_frame_result R;
     xCheck(qmiv_frame(m_Ctx->get(), Tst->handle(), Ref->handle(), &P, Metric, &R), "qmiv_frame");
     const double* V = Metric == QMIV_PSNR ? R.psnr : R.wspsnr;
     return {V[0], V[1], V[2], 0.0};
+  }
+  flt64 xIV(qmiv_pic* Tst, qmiv_pic* Ref, const int32V4& GlobalColorDiffRef2Tst) {
+    flt64 IV = 0, R2T = 0, T2R = 0;
+    xCheck(qmiv_calc_ivpsnr(m_Ctx->get(), Tst, Ref, &m_Prm, GlobalColorDiffRef2Tst.data(), &IV, &R2T, &T2R, nullptr),
+           "qmiv_calc_ivpsnr");
+    if (m_DebugCallbackQAP) m_DebugCallbackQAP(R2T, T2R);
+    return IV;
+  }
+  flt64 xIVM(qmiv_pic* Tst, qmiv_pic* Ref, const xPicP* Msk, int32 NumNonMasked, const int32V4& GlobalColorDiffRef2Tst) {
+    flt64 IV = 0, R2T = 0, T2R = 0;
+    const qmiv_params P = xMaskedPrm();
+    xCheck(qmiv_calc_ivpsnr_masked(m_Ctx->get(), Tst, Ref, Msk->handle(), &P, NumNonMasked,
+                                   GlobalColorDiffRef2Tst.data(), &IV, &R2T, &T2R),
+           "qmiv_calc_ivpsnr_masked");
+    if (m_DebugCallbackQAP) m_DebugCallbackQAP(R2T, T2R);
+    return IV;
   }
   qmiv_params xMaskedPrm() const {
     qmiv_params P = m_Prm;

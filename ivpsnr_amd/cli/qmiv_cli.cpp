@@ -11,20 +11,21 @@
 // -c (a later occurrence overrides an earlier one):
 //   i0 InputFile0, i1 InputFile1, ff FileFormat (RAW, PNG), ps PictureSize (WxH), pw PictureWidth,
 //   ph PictureHeight, pf PictureFormat (yuv420p, yuv444p10le, ...), bd BitDepth (8..14),
-//   cf ChromaFormat (420, 444), s0 StartFrame0, s1 StartFrame1, nf NumberOfFrames, r ResultFile,
+//   cf ChromaFormat (420, 422, 444), s0 StartFrame0, s1 StartFrame1, nf NumberOfFrames, r ResultFile,
 //   ml MetricList (comma list of PSNR WSPSNR IVPSNR SSIM MSSSIM IVSSIM, or All), im InputFileM,
 //   bdm BitDepthM, cfm ChromaFormatM, erp Equirectangular (flag), lor LonRangeDeg, lar LatRangeDeg,
 //   csi ColorSpaceInput, csm ColorSpaceMetric (RGB BGR GBR YCbCr YCbCr_BT601 YCbCr_SMPTE170M
 //   YCbCr_BT709 YCbCr_SMPTE240M YCbCr_BT2020), sr SearchRange, cws CmpWeightsSearch (a:b:c:d),
-//   cwa CmpWeightsAverage, unc UnnoticeableCoef (a:b:c:d), ipa InvalidPelActn (STOP), nma
+//   cwa CmpWeightsAverage, unc UnnoticeableCoef (a:b:c:d), ipa InvalidPelActn (STOP WARN SKIP CNCL), nma
 //   NameMismatchActn, nth NumberOfThreads, ilp InterleavedPic, v VerboseLevel (0..4).
-// Differences from the reference application: chroma formats 4:2:0 and 4:4:4 only (4:2:2 and 4:0:0
-// are rejected); with FileFormat PNG, BitDepth and BitDepthM must be 8 (the files are decoded to
-// 8-bit R, G, B planes, xSeqPNG.cpp:14-36); InvalidPelActn CNCL (concealment) is rejected, and
-// under WARN or SKIP a frame with a pel above the bit-depth maximum stops the run, because the engine
-// computes no metric for such a frame; NumberOfThreads and InterleavedPic are reported and otherwise
-// ignored (the engine runs on one GPU and its results do not depend on either); the timing report of
-// VerboseLevel 3 gives the engine's average wall time per frame instead of per-stage CPU times.
+// Chroma formats 4:2:0 (any size), 4:2:2 and 4:4:4 (the application's parser reads "400" as 4:2:0,
+// xMiscUtilsCORE.cpp:13-21); InvalidPelActn STOP, WARN, SKIP and CNCL with the reference's check
+// report (FILE BROKEN / DETECTED lines). VerboseLevel 3 runs each frame stage by stage and prints the
+// application's AvgTime table (LOAD, VALIDATE, PREPROC, GCD, SCP, per metric), the stages timed on
+// the GPU up to their results. Differences from the reference application: with FileFormat PNG,
+// BitDepth and BitDepthM must be 8 (the files are decoded to 8-bit R, G, B planes,
+// xSeqPNG.cpp:14-36); NumberOfThreads and InterleavedPic are reported and otherwise ignored (the
+// engine runs on one GPU and its results do not depend on either).
 #include <algorithm>
 #include <cctype>
 #include <chrono>
@@ -302,10 +303,12 @@ struct Owned {
   qmiv_seq* seq[3] = {nullptr, nullptr, nullptr};
   qmiv_pic* in[3] = {nullptr, nullptr, nullptr};
   qmiv_pic* up[2] = {nullptr, nullptr};
+  qmiv_pic* scp[2] = {nullptr, nullptr};  // the staged route's shift-compensated pictures
   ~Owned() {
     for (qmiv_seq* s : seq) qmiv_seq_close(s);
     for (qmiv_pic* p : in) qmiv_pic_destroy(p);
     for (qmiv_pic* p : up) qmiv_pic_destroy(p);
+    for (qmiv_pic* p : scp) qmiv_pic_destroy(p);
     qmiv_destroy(ctx);
   }
 };
@@ -343,7 +346,7 @@ int main(int argc, char** argv) {
     cf = str2cf(P.str("ChromaFormat", "420"));
   }
   if (bd < 8 || bd > 14) P.log += "!  Invalid or unsuported BitDepth value\n";
-  if (cf != 420 && cf != 444) P.log += "!  Invalid or unsuported ChromaFormat value (4:2:0 and 4:4:4 only)\n";
+  if (cf != 420 && cf != 422 && cf != 444) P.log += "!  Invalid or unsuported ChromaFormat value\n";
   const int32_t start[2] = {P.num("StartFrame0", 0), P.num("StartFrame1", 0)};
   if (start[0] < 0 || start[1] < 0) P.log += "!  StartFrame value cannot be negative\n";
   const int32_t number_of_frames = P.num("NumberOfFrames", -1);
@@ -372,7 +375,7 @@ int main(int argc, char** argv) {
   const int32_t bdm = P.num("BitDepthM", bd);
   const int32_t cfm = P.has("ChromaFormatM") ? str2cf(P.str("ChromaFormatM")) : cf;
   if (bdm < 8 || bdm > 14) P.log += "!  Invalid or unsuported BitDepthM value\n";
-  if (cfm != 420 && cfm != 444) P.log += "!  Invalid or unsuported ChromaFormatM value (4:2:0 and 4:4:4 only)\n";
+  if (cfm != 420 && cfm != 422 && cfm != 444) P.log += "!  Invalid or unsuported ChromaFormatM value\n";
   if (png && (bd != 8 || bdm != 8)) P.log += "!  PNG input is 8-bit: BitDepth and BitDepthM must be 8\n";
   qmiv_params prm;
   qmiv_params_default(&prm);
@@ -401,7 +404,8 @@ int main(int argc, char** argv) {
   const std::string ipa = upper(P.str("InvalidPelActn", "STOP")), nma = upper(P.str("NameMismatchActn", "WARN"));
   for (const std::string* a : {&ipa, &nma})
     if (*a != "SKIP" && *a != "WARN" && *a != "STOP" && *a != "CNCL") P.log += "!  Invalid action " + *a + "\n";
-  if (ipa == "CNCL") P.log += "!  InvalidPelActn CNCL is not supported\n";
+  prm.invalid_pel_action = ipa == "WARN" ? QMIV_PEL_WARN : ipa == "SKIP" ? QMIV_PEL_SKIP
+                           : ipa == "CNCL" ? QMIV_PEL_CNCL : QMIV_PEL_STOP;
   const int32_t threads = P.num("NumberOfThreads", -2);
   const int32_t interleaved = P.num("InterleavedPic", 1);
   const int verbose = P.num("VerboseLevel", 1);
@@ -482,6 +486,11 @@ int main(int argc, char** argv) {
   // PNG frames are 4:4:4 R, G, B pictures whatever ChromaFormat says (xSeqPNG::create, xSeqPNG.cpp:14-35).
   const int32_t bds[3] = {bd, bd, bdm};
   const int32_t cfs[3] = {png ? 444 : cf, png ? 444 : cf, png ? 444 : cfm};
+  // Device pictures: 4:2:0 of even size and 4:4:4 as read; 4:2:2 and odd-size 4:2:0 frames are
+  // unpacked on the device into 4:4:4 pictures, as the reference's reader unpacks every frame
+  // (libs/pmbb-core/src/xSeq.cpp:179-249).
+  int32_t dcf[3];
+  for (int i = 0; i < 3; i++) dcf[i] = cfs[i] == 420 && !(w & 1) && !(h & 1) ? 420 : 444;
   Owned o;
   for (int i = 0; i < num_inputs && !png; i++)
     if (!std::filesystem::exists(in_file[i]))
@@ -511,10 +520,11 @@ int main(int argc, char** argv) {
   if (use_mask && frames > frames_in[2]) fail("ERROR --> FramesToProcess > NumOfFramesM");
 
   check(qmiv_create(0, &o.ctx), "device initialisation");
-  for (int i = 0; i < num_inputs; i++) check(qmiv_pic_create(o.ctx, w, h, bds[i], cfs[i], &o.in[i]), "picture allocation");
-  // Colour conversion and plane reordering work on 4:4:4 pictures, as the reference's do (its
-  // pictures are always 4:4:4 after reading: libs/pmbb-core/src/xSeq.cpp:197-213).
-  const bool up444 = convert && cfs[0] == 420;
+  for (int i = 0; i < num_inputs; i++) check(qmiv_pic_create(o.ctx, w, h, bds[i], dcf[i], &o.in[i]), "picture allocation");
+  // Colour conversion, plane reordering and the stage-by-stage route (VerboseLevel 3) work on 4:4:4
+  // pictures, as the reference's do (its pictures are always 4:4:4 after reading:
+  // libs/pmbb-core/src/xSeq.cpp:197-213).
+  const bool up444 = (convert || verbose >= 3) && dcf[0] == 420;
   if (up444)
     for (int i = 0; i < 2; i++) check(qmiv_pic_create(o.ctx, w, h, bd, 444, &o.up[i]), "picture allocation");
 
@@ -531,25 +541,102 @@ int main(int argc, char** argv) {
 
   // --- frame loop: xAppQMIV::processAllFrames (apps/QMIV/src/xAppQMIV.cpp:661-759) ---------------
   const bool print_frame = verbose >= 2, debug = verbose >= 4;
+  // VerboseLevel 3 gathers per-stage times as the reference does (xAppQMIV.cpp:333, :670-744): each
+  // frame then runs stage by stage through the processor-level calls, each stage timed up to its
+  // results; below that level one fused engine call computes the whole frame.
+  const bool staged = verbose >= 3;
+  const int32_t act = prm.invalid_pel_action;
+  const bool need_gcd = calc[kIVPSNR] || calc[kIVSSIM];
+  double t_stage[5] = {0, 0, 0, 0, 0};  // LOAD, VALIDATE, PREPROC, GCD, SCP (ms, summed over frames)
+  double t_metric[kNumMetrics] = {0, 0, 0, 0, 0, 0};
+  if (staged && calc[kIVSSIM])
+    for (int i = 0; i < 2; i++) check(qmiv_pic_create(o.ctx, w, h, bd, 444, &o.scp[i]), "picture allocation");
+  auto ms_since = [](Clock::time_point a) { return std::chrono::duration<double, std::milli>(Clock::now() - a).count(); };
+  // xPicP::check's report (libs/pmbb-core/src/xPic.cpp:62-82, FindOutOfRange, xPixelOpsSTD.cpp:175-197)
+  // of input i as read: the first component holding a sample above the maximum, then every such sample
+  // of it in raster order of the 4:4:4 picture.
+  auto report = [&](int i) {
+    const int32_t fmt_cf = dcf[i], bdi = bds[i];
+    const size_t nb = qmiv_frame_bytes(w, h, bdi, fmt_cf);
+    std::vector<uint8_t> raw(nb);
+    check(qmiv_pic_download(o.ctx, o.in[i], raw.data(), nb), "picture download");
+    const int32_t maxv = (1 << bdi) - 1;
+    const int32_t cs = fmt_cf == 420 ? 1 : 0, cw = w >> cs;
+    auto sample = [&](int c, int32_t y, int32_t x) -> int32_t {
+      const size_t off = c == 0 ? (size_t)y * w + x
+                                : (size_t)w * h + (size_t)(c - 1) * (size_t)cw * (h >> cs) + (size_t)(y >> cs) * cw + (x >> cs);
+      return bdi <= 8 ? raw[off] : (int32_t)raw[2 * off] | ((int32_t)raw[2 * off + 1] << 8);
+    };
+    for (int c = 0; c < 3; c++) {
+      bool broken = false;
+      for (int32_t y = 0; y < h && !broken; y++)
+        for (int32_t x = 0; x < w && !broken; x++) broken = sample(c, y, x) > maxv;
+      if (!broken) continue;
+      std::printf("FILE BROKEN %s (CMP=%d)\n", in_file[i].c_str(), c);
+      for (int32_t y = 0; y < h; y++)
+        for (int32_t x = 0; x < w; x++)
+          if (sample(c, y, x) > maxv)
+            std::printf("DETECTED out-of-range value (y=%d, x=%d, VALUE=%d, Expected=[0-%d])\n", y, x, sample(c, y, x), maxv);
+      return;
+    }
+  };
+  auto invalid = [&](const int32_t* ok, int n) {
+    for (int i = 0; i < n; i++)
+      if (!ok[i]) fail("ERROR --> InputFile contains invalid values (" + in_file[i] + ")");
+    fail(std::string("ERROR --> ") + qmiv_last_error());
+  };
   const auto prc_beg = Clock::now();
   for (int32_t f = 0; f < frames; f++) {
     if (debug) std::printf("Frame %08d  ", f);
+    auto ts = Clock::now();
     for (int i = 0; i < num_inputs; i++) {
       const int32_t idx = (i < 2 ? start[i] : 0) + f;
       check(qmiv_seq_read(o.ctx, o.seq[i], idx, o.in[i]), "InputFile read error (" + in_file[i] + ")");
     }
-    auto invalid = [&](const int32_t* ok, int n) {
+    if (staged) check(qmiv_sync(o.ctx), "InputFile read");
+    t_stage[0] += ms_since(ts);
+    ts = Clock::now();
+    // Validation (xAppQMIV.cpp:681-683, :753-770). The fused call checks the frame itself (its
+    // report is printed below, from the pictures as read); concealment clips the pictures in place,
+    // so it and the staged route check them here first.
+    bool checked = false;
+    int32_t in_ok[3] = {1, 1, 1};
+    if (act != QMIV_PEL_SKIP && (staged || act == QMIV_PEL_CNCL)) {
+      checked = true;
+      for (int i = 0; i < num_inputs; i++) {
+        int32_t c3[3];
+        check(qmiv_pic_check(o.ctx, o.in[i], c3), "input validation");
+        in_ok[i] = c3[0] && c3[1] && c3[2];
+        if (!in_ok[i]) report(i);
+      }
+      if (act == QMIV_PEL_CNCL)
+        for (int i = 0; i < num_inputs; i++)
+          if (!in_ok[i]) check(qmiv_pic_conceal(o.ctx, o.in[i]), "concealment");
+      if (act == QMIV_PEL_STOP && !(in_ok[0] && in_ok[1] && in_ok[2])) invalid(in_ok, num_inputs);
+    }
+    if (staged) t_stage[1] += ms_since(ts);
+    ts = Clock::now();
+    // After a failed fused call's check: the report, then STOP ends the run.
+    auto after_check = [&](const int32_t* ok, int n) {
+      if (checked || act == QMIV_PEL_SKIP) return;
+      bool all_ok = true;
       for (int i = 0; i < n; i++)
-        if (!ok[i]) fail("ERROR --> InputFile contains invalid values (" + in_file[i] + ")");
-      fail(std::string("ERROR --> ") + qmiv_last_error());
+        if (!ok[i]) {
+          report(i);
+          all_ok = false;
+        }
+      if (act == QMIV_PEL_STOP && !all_ok) invalid(ok, n);
     };
     int32_t exact[3] = {0, 0, 0};
-    if (convert) {
+    if (staged) {
+      check(qmiv_pic_equal(o.ctx, o.in[0], o.in[1], exact), "exact comparison");
+    } else if (convert) {
       qmiv_frame_result pre;
       const qmiv_status rc = qmiv_frame(o.ctx, o.in[0], o.in[1], &prm, QMIV_PSNR, &pre);
-      if (rc == QMIV_ERR_PEL_RANGE) invalid(pre.in_range, 2);
-      check(rc, "input validation");
+      after_check(pre.in_range, 2);
+      if (rc != QMIV_ERR_PEL_RANGE) check(rc, "input validation");
       std::copy(pre.exact, pre.exact + 3, exact);
+      checked = true;
     }
     qmiv_pic* p[2] = {o.in[0], o.in[1]};
     if (up444)
@@ -569,9 +656,52 @@ int main(int argc, char** argv) {
     double psnr[3] = {0, 0, 0}, wspsnr[3] = {0, 0, 0}, ssim[3] = {0, 0, 0}, msssim[3] = {0, 0, 0};
     double ivpsnr = 0, r2t = 0, t2r = 0, ivssim = 0, s_r2t = 0, s_t2r = 0;
     int32_t gcd[4] = {0, 0, 0, 0};
-    if (use_mask) {
+    int32_t nnm = 0;
+    if (staged) {
+      // xAppQMIV.cpp:686-744 stage by stage; the processors validate nothing and substitute
+      // nothing, so the exact-component rule below applies as in the fused route.
+      qmiv_params sp = prm;
+      sp.fake_psnr_for_exact = 0;
+      if (use_mask) check(qmiv_count_nonzero(o.ctx, o.in[2], &nnm), "mask count");
+      check(qmiv_sync(o.ctx), "preprocessing");
+      t_stage[2] += ms_since(ts);
+      ts = Clock::now();
+      if (need_gcd) {
+        if (use_mask) check(qmiv_calc_gcd_masked(o.ctx, p[0], p[1], o.in[2], &sp, nnm, gcd), "GCD");
+        else check(qmiv_calc_gcd(o.ctx, p[0], p[1], &sp, gcd), "GCD");
+      }
+      t_stage[3] += ms_since(ts);
+      ts = Clock::now();
+      if (calc[kIVSSIM]) check(qmiv_gen_scp_pics(o.ctx, p[1], p[0], &sp, gcd, o.scp[1], o.scp[0]), "SCP");
+      t_stage[4] += ms_since(ts);
+      for (int m = 0; m < kNumMetrics; m++) {
+        if (!calc[m]) continue;
+        ts = Clock::now();
+        switch (m) {
+          case kPSNR:
+            check(use_mask ? qmiv_calc_psnr_masked(o.ctx, p[0], p[1], o.in[2], &sp, nnm, psnr)
+                           : qmiv_calc_psnr(o.ctx, p[0], p[1], psnr), "PSNR");
+            break;
+          case kWSPSNR:
+            check(use_mask ? qmiv_calc_wspsnr_masked(o.ctx, p[0], p[1], o.in[2], &sp, nnm, wspsnr)
+                           : qmiv_calc_wspsnr(o.ctx, p[0], p[1], &sp, wspsnr), "WS-PSNR");
+            break;
+          case kIVPSNR:
+            check(use_mask ? qmiv_calc_ivpsnr_masked(o.ctx, p[0], p[1], o.in[2], &sp, nnm, gcd, &ivpsnr, &r2t, &t2r)
+                           : qmiv_calc_ivpsnr(o.ctx, p[0], p[1], &sp, gcd, &ivpsnr, &r2t, &t2r, nullptr), "IV-PSNR");
+            break;
+          case kSSIM: check(qmiv_calc_ssim(o.ctx, p[0], p[1], 1, ssim), "SSIM"); break;
+          case kMSSSIM: check(qmiv_calc_msssim(o.ctx, p[0], p[1], msssim, nullptr), "MS-SSIM"); break;
+          default:
+            check(qmiv_calc_ivssim_scp(o.ctx, p[0], p[1], o.scp[0], o.scp[1], &sp, &ivssim, &s_r2t, &s_t2r), "IV-SSIM");
+        }
+        t_metric[m] += ms_since(ts);
+      }
+      if (debug && use_mask) std::printf("NNM %d    ", nnm);
+    } else if (use_mask) {
       qmiv_masked_result r;
       const qmiv_status rc = qmiv_frame_masked(o.ctx, p[0], p[1], o.in[2], &run_prm, metrics, &r);
+      after_check(r.in_range, 3);
       if (rc == QMIV_ERR_PEL_RANGE) invalid(r.in_range, 3);
       check(rc, "metric computation");
       std::copy(r.psnr, r.psnr + 3, psnr);
@@ -583,6 +713,7 @@ int main(int argc, char** argv) {
     } else {
       qmiv_frame_result r;
       const qmiv_status rc = qmiv_frame(o.ctx, p[0], p[1], &run_prm, metrics, &r);
+      after_check(r.in_range, 2);
       if (rc == QMIV_ERR_PEL_RANGE) invalid(r.in_range, 2);
       check(rc, "metric computation");
       std::copy(r.psnr, r.psnr + 3, psnr);
@@ -598,7 +729,7 @@ int main(int argc, char** argv) {
       if (metrics & (QMIV_IVPSNR | QMIV_IVSSIM)) std::printf("GCD-R2T %d %d %d %d    ", gcd[0], gcd[1], gcd[2], gcd[3]);
       std::printf("\n");
     }
-    if (convert)  // xAppQMIV.cpp:837-844, :864-871
+    if (convert || staged)  // xAppQMIV.cpp:837-844, :864-871
       for (int c = 0; c < 3; c++)
         if (exact[c]) psnr[c] = wspsnr[c] = fake_psnr;
 
@@ -658,7 +789,24 @@ int main(int argc, char** argv) {
   std::string out;
   for (int m = 0; m < kNumMetrics; m++)
     if (calc[m]) out += st[m].avg_lines("Average      ") + "\n";
-  if (verbose >= 3) out += format("\nAvgTime         FRAME %9.2f ms\n", frames > 0 ? 1000.0 * prc_s / frames : 0.0);
+  if (staged && frames > 0) {  // xAppQMIV::formatResultsStdOut (apps/QMIV/src/xAppQMIV.cpp:995-1030)
+    const double n = (double)frames;
+    const double pre = t_stage[2] / n, gcd_t = t_stage[3] / n, scp_t = t_stage[4] / n;
+    out += "\n";
+    out += format("AvgTime          LOAD %9.2f ms\n", t_stage[0] / n);
+    out += format("AvgTime      VALIDATE %9.2f ms\n", t_stage[1] / n);
+    out += format("AvgTime       PREPROC %9.2f ms\n", pre);
+    if (need_gcd) out += format("AvgTime           GCD %9.2f ms\n", gcd_t);
+    if (calc[kIVSSIM]) out += format("AvgTime           SCP %9.2f ms\n", scp_t);
+    for (int m = 0; m < kNumMetrics; m++) {
+      if (!calc[m]) continue;
+      const double avg = t_metric[m] / n;
+      const double before = pre + (m == kIVPSNR ? gcd_t : 0.0) + (m == kIVSSIM ? gcd_t + scp_t : 0.0);
+      out += format("AvgTime      %8s %9.2f ms", kMetricName[m], avg);
+      if (before > 0) out += format("   Total %7.2f ms", avg + before);
+      out += "\n";
+    }
+  }
   std::printf("%s\n", out.c_str());
   std::printf("TotalProcessingTime  = %.3f s\n", prc_s);
   std::printf("TotalApplicationTime = %.3f s\n", std::chrono::duration<double>(Clock::now() - app_beg).count());

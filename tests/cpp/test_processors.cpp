@@ -119,6 +119,35 @@ int main(int argc, char** argv) {
     ProcSSIM.setDebugCallbackQAP([&](flt64 A, flt64 B) { SR2T = A; ST2R = B; });
     const flt64 IVSSIM = ProcSSIM.calcPicIVSSIM(&PicInP0, &PicInP1, &PicSCP0, &PicSCP1);
 
+    // The application's default, interleaved pictures (xAppQMIV.cpp:314,856-857,892):
+    // rearrangeFromPlanar, then calcPicIVPSNR(const xPicI*, const xPicI*, GCD) - the same value.
+    xPicI PicInI0(Ctx, {W, H}, BD), PicInI1(Ctx, {W, H}, BD);
+    PicInI0.rearrangeFromPlanar(&PicInP0);
+    PicInI1.rearrangeFromPlanar(&PicInP1);
+    const flt64 IVPSNRI = ProcPSNR.calcPicIVPSNR(&PicInI0, &PicInI1, GCD_R2T);
+    CHECK(IVPSNRI == IVPSNR);
+    // The reference's host layouts, margins included: xPicP planes and xPicI pels out and back in.
+    {
+      const int32 M = PicInP0.getMargin(), S = PicInP0.getStride();
+      CHECK(M == 4 && S == W + 8);
+      std::vector<uint16_t> Buf(3 * (size_t)S * (H + 2 * M), 0xFFFF);
+      uint16_t* O[3];
+      for (int c = 0; c < 3; c++) O[c] = Buf.data() + (size_t)c * S * (H + 2 * M) + (size_t)M * S + M;
+      PicInP0.toPlanar(O, S);
+      CHECK(Buf[0] == 0xFFFF);  // the margin is left alone
+      xPicP Back(Ctx, {W, H}, BD);
+      Back.fromPlanar(O, S);
+      CHECK(Back.equalPic(&PicInP0) && !Back.equalPic(&PicInP1));
+      CHECK(Back.equalCmp(&PicInP0, eCmp::CB));
+      const int32 SI = PicInI1.getStride();
+      std::vector<uint16_t> Pels(4 * (size_t)SI * (H + 2 * M), 0);
+      PicInI1.toInterleaved(Pels.data() + 4 * ((size_t)M * SI + M), SI);
+      xPicI BackI(Ctx, {W, H}, BD);
+      BackI.fromInterleaved(Pels.data() + 4 * ((size_t)M * SI + M), SI);
+      CHECK(ProcPSNR.calcPicIVPSNR(&PicInI0, &BackI, GCD_R2T) == IVPSNR);
+      CHECK(PicInP0.check("InputFile0"));  // in range: true, nothing printed
+    }
+
     // Identities of the processor API.
     CHECK(IVPSNR == (R2T < T2R ? R2T : T2R));
     CHECK(IVSSIM == (ST2R < SR2T ? ST2R : SR2T));
@@ -181,6 +210,8 @@ int main(int argc, char** argv) {
       ProcPSNR.setDebugCallbackQAP([&](flt64 A, flt64 B) { MR2T = A; MT2R = B; });
       ProcPSNR.setDebugCallbackMSK([&](int32 N) { SeenNNM = N; });
       const flt64 IM = ProcPSNR.calcPicIVPSNRM(&PicInP0, &PicInP1, &PicMsk, NNM, GM);
+      // the application's masked call takes interleaved pictures (xAppQMIV.cpp:889)
+      CHECK(ProcPSNR.calcPicIVPSNRM(&PicInI0, &PicInI1, &PicMsk, NNM, GM) == IM);
       // The self-contained form counts the mask and finds the same colour difference.
       int32V4 GAuto{};
       ProcPSNR.setDebugCallbackGCS([&](const int32V4& G) { GAuto = G; });
@@ -198,7 +229,8 @@ int main(int argc, char** argv) {
     std::printf("{\"gcd\":[%d,%d,%d,%d],", GCD_R2T[0], GCD_R2T[1], GCD_R2T[2], GCD_R2T[3]);
     printV4("psnr", PSNR);
     printV4("wspsnr", WSPSNR);
-    std::printf("\"ivpsnr\":%.17g,\"ivpsnr_r2t\":%.17g,\"ivpsnr_t2r\":%.17g,", IVPSNR, R2T, T2R);
+    std::printf("\"ivpsnr\":%.17g,\"ivpsnr_r2t\":%.17g,\"ivpsnr_t2r\":%.17g,\"ivpsnr_interleaved\":%.17g,", IVPSNR,
+                R2T, T2R, IVPSNRI);
     printV4("ssim", SSIM);
     printV4("ssim_cs", CS);
     printV4("msssim", MSSSIM);

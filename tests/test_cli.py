@@ -83,7 +83,7 @@ def test_png_sequences_are_counted(seqs):
     (["-i0", "a.yuv", "-ps", "16x8"], "InputFile1 is empty"),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-ml", "PSNR,XYZ"], 'Token "XYZ" in not a metric'),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-bd", "16"], "Invalid or unsuported BitDepth value"),
-    (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-cf", "422"], "ChromaFormat"),
+    (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-cf", "411"], "Invalid or unsuported ChromaFormat value"),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-sr", "0"], "SearchRange value must non-zero"),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-csi", "YCbCr_BT709", "-csm", "YCbCr_BT2020"],
      "YCbCr to YCbCr conversion is not supported."),
@@ -91,7 +91,7 @@ def test_png_sequences_are_counted(seqs):
      "Generic YCbCr cannot be used for colorspace conversion."),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-im", "m.yuv", "-ml", "PSNR,IVSSIM"],
      "Structural Similarity metrics cannot be combined with Mask mode"),
-    (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-ipa", "CNCL"], "CNCL is not supported"),
+    (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-ipa", "HALT"], "Invalid action HALT"),
     (["-i0", "a.yuv", "-i1", "b.yuv", "-ps", "16x8", "-ff", "TIFF"], "Invalid FileFormat TIFF"),
     (["-i0", "a{}.png", "-i1", "b{}.png", "-ps", "16x8", "-ff", "PNG", "-bd", "10"], "PNG input is 8-bit"),
     (["-i0", "none{}.png", "-i1", "b{}.png", "-ps", "16x8", "-ff", "PNG"], "InputFile0 does not exist (none{}.png)"),

@@ -246,3 +246,87 @@ def test_png_sequences(tmp_path, oracle):
     res = [oracle.frame(conv(planes[0][f]), conv(planes[1][f]), prm) for f in range(3)]
     lines, avg = expected_psnr_family(res, ["PSNR", "IVPSNR"])
     check_output(r, lines, avg)
+
+
+def _cli_values(stdout):
+    """The frame and average lines of a run (the timing lines aside)."""
+    return [l for l in stdout.splitlines() if l.startswith(("Frame ", "Average "))]
+
+
+@pytest.mark.parametrize("i", [1, 3, 5])
+def test_file_formats(tmp_path, oracle, i):
+    """4:2:2 and odd-width 4:2:0 files (xSeq's unpack, libs/pmbb-core/src/xSeq.cpp:179-249)."""
+    from tests.inputs import FORMAT_CASES, format_pair
+    w, h, bd, fcf, seed = FORMAT_CASES[i]
+    frames = [format_pair(w, h, bd, fcf, seed + k) for k in range(2)]
+    (tmp_path / "a.yuv").write_bytes(b"".join(f[0].tobytes() for f in frames))
+    (tmp_path / "b.yuv").write_bytes(b"".join(f[1].tobytes() for f in frames))
+    print(f"cli format {w}x{h} bd{bd} cf{fcf}", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-cf", fcf, "-ml", "PSNR,WSPSNR,IVPSNR",
+                 "-v", 2], tmp_path)
+    prm = oracle.params(w, h, bd, fcf)
+    res = [oracle.frame(a, b, prm) for a, b in frames]
+    lines, avg = expected_psnr_family(res, ["PSNR", "WSPSNR", "IVPSNR"])
+    check_output(r, lines, avg)
+
+
+@pytest.mark.parametrize("action", ["WARN", "SKIP", "CNCL", "STOP"])
+def test_invalid_pel_actions(tmp_path, oracle, action):
+    """InvalidPelActn (xAppQMIV.cpp:681-683, :753-770): the check's report is the reference's own text
+    (tests/golden/formats.json, xPicP::check); WARN and SKIP compute over the samples as read, CNCL
+    over the clipped pictures, STOP ends the run."""
+    import json
+    import os
+    from tests.inputs import RANGE_CASES, range_pair
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "formats.json")) as f:
+        want = json.load(f)["range"][0]
+    w, h, bd, cf, seed, inj0, inj1, _ = RANGE_CASES[0]
+    a, b = range_pair(w, h, bd, cf, seed, inj0, inj1)
+    write_seq(tmp_path / "a.yuv", [a], bd)
+    write_seq(tmp_path / "b.yuv", [b], bd)
+    print(f"cli invalid pels {action}", flush=True)
+    r = run_cli(["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-ml", "PSNR,WSPSNR,IVPSNR", "-ipa",
+                 action, "-v", 2], tmp_path)
+    report = want["check_report"].replace("{i0}", "a.yuv").replace("{i1}", "b.yuv")
+    if action == "SKIP":
+        assert "FILE BROKEN" not in r.stdout and "DETECTED" not in r.stdout
+    else:
+        assert report in r.stdout
+    if action == "STOP":
+        assert r.returncode != 0 and "InputFile contains invalid values (a.yuv)" in r.stderr
+        return
+    res = [oracle.frame(a, b, oracle.params(w, h, bd, cf, conceal=action == "CNCL"))]
+    lines, avg = expected_psnr_family(res, ["PSNR", "WSPSNR", "IVPSNR"])
+    check_output(r, lines, avg)
+
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_staged_timing_report(tmp_path, masked):
+    """VerboseLevel 3: the frame runs stage by stage (processor-level calls) and the application's
+    AvgTime table is printed (xAppQMIV.cpp:995-1030); every frame and average line equals the fused
+    run's."""
+    w, h, bd = 128, 64, 10
+    pairs = frames_for(w, h, bd, 420, 3, 900)
+    write_seq(tmp_path / "a.yuv", [p.ref for p in pairs], bd)
+    write_seq(tmp_path / "b.yuv", [p.tst for p in pairs], bd)
+    args = ["-i0", "a.yuv", "-i1", "b.yuv", "-ps", f"{w}x{h}", "-bd", bd, "-erp"]
+    if masked:
+        write_seq(tmp_path / "m.yuv", [make_mask(w, h, 8, 900 + i, kind="soft") for i in range(3)], 8)
+        args += ["-im", "m.yuv", "-bdm", 8, "-ml", "PSNR,WSPSNR,IVPSNR"]
+        names = ["PSNR", "WSPSNR", "IVPSNR"]
+    else:
+        args += ["-ml", "All"]
+        names = ["PSNR", "WSPSNR", "IVPSNR", "SSIM", "MSSSIM", "IVSSIM"]
+    print(f"cli staged timing masked={masked}", flush=True)
+    fused = run_cli(args + ["-v", 2], tmp_path)
+    staged = run_cli(args + ["-v", 3], tmp_path)
+    assert fused.returncode == 0 and staged.returncode == 0, staged.stdout[-2000:] + staged.stderr
+    assert _cli_values(staged.stdout) == _cli_values(fused.stdout)
+    out = staged.stdout.splitlines()
+    for stage in ("LOAD", "VALIDATE", "PREPROC", "GCD") + (() if masked else ("SCP",)):
+        assert any(l.startswith("AvgTime ") and l.split()[1] == stage and l.endswith(" ms") for l in out), stage
+    for name in names:
+        line = [l for l in out if l.startswith("AvgTime ") and l.split()[1] == name]
+        assert len(line) == 1 and " ms" in line[0], name
+        if name in ("IVPSNR", "IVSSIM"):
+            assert "   Total " in line[0]

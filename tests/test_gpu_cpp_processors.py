@@ -55,6 +55,7 @@ def run_processors(oracle, tmp_path, w, h, bd, cf, erp, seed, masked):
     assert got["gcd"] == list(o.gcd)
     assert got["psnr"] == list(o.psnr) and got["wspsnr"] == list(o.wspsnr)
     assert [got["ivpsnr"], got["ivpsnr_r2t"], got["ivpsnr_t2r"]] == [o.ivpsnr, o.r2t, o.t2r]
+    assert got["ivpsnr_interleaved"] == o.ivpsnr  # calcPicIVPSNR(const xPicI*, const xPicI*, GCD)
     assert close(got["ssim"], list(o.ssim)) and close(got["msssim"], list(o.msssim))
     assert close([got["ivssim"], got["ivssim_r2t"], got["ivssim_t2r"]], [o.ivssim, o.ivssim_r2t, o.ivssim_t2r])
     P0, P1 = oracle.unpack(pair.ref, w, h, bd, cf), oracle.unpack(pair.tst, w, h, bd, cf)
