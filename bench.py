@@ -386,6 +386,9 @@ def main():
     ap.add_argument("--frames", type=int, default=0, help="frame pairs per step (default: the workload's)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 512 MiB (calibrates rocprof byte counters)")
+    ap.add_argument("--legs", choices=["all", "stream"], default="all",
+                    help="stream: run only the timed frame-set steps (no plain loop, no upload leg), so that "
+                         "a kernel trace of the run holds exactly the launches the roofline figure is taken from")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -447,17 +450,21 @@ def main():
         eng.profile(False)
         return el, pr, r
 
-    print(f"[bench] plain loop: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
-    el_plain, _, res_plain = timed(step_plain, args.steps, args.warmup, False)
+    only_stream = args.legs == "stream"
+    key = "ivpsnr" if "IVPSNR" in wl["ml"] else "ivssim"
+    el_plain = None
+    if not only_stream:
+        print(f"[bench] plain loop: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
+        el_plain, _, res_plain = timed(step_plain, args.steps, args.warmup, False)
     print(f"[bench] stream: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
     elapsed, prof, res = timed(step_stream, args.steps, args.warmup, True)
-    key = "ivpsnr" if "IVPSNR" in wl["ml"] else "ivssim"
-    assert [getattr(r, key) for r in res] == [getattr(r, key) for r in res_plain]
-    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s (plain loop {el_plain:.3f} s)", file=sys.stderr,
-          flush=True)
-    # Kernel events serialise nothing but add a few microseconds per launch: time once more without them.
-    el_noprof, _, _ = timed(step_stream, args.steps, 0, False)
-    elapsed = min(elapsed, el_noprof)
+    if not only_stream:
+        assert [getattr(r, key) for r in res] == [getattr(r, key) for r in res_plain]
+        print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s (plain loop {el_plain:.3f} s)", file=sys.stderr,
+              flush=True)
+        # Kernel events serialise nothing but add a few microseconds per launch: time once more without them.
+        el_noprof, _, _ = timed(step_stream, args.steps, 0, False)
+        elapsed = min(elapsed, el_noprof)
 
     # Host-to-device inclusive rate (not `value`): every step first copies its frame pairs from
     # pageable host arrays (staged through the context's pinned buffer, one wait per picture).
@@ -468,7 +475,7 @@ def main():
         return eng.frames(pics, metrics, prm)
 
     up_steps = max(1, min(args.steps, 5))
-    el_up, _, _ = timed(step_upload, up_steps, 1, False)
+    el_up = None if only_stream else timed(step_upload, up_steps, 1, False)[0]
 
     ms_per_step = elapsed / args.steps * 1e3
     mpix = nframes * W * H / 1e6
@@ -480,7 +487,9 @@ def main():
         achieved = per_launch / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic(wl["traffic_file"]),
-                "kernel": "k_cps_fast420", "kernel_avg_us": round(k_avg_ms * 1e3, 2), "bytes_per_launch": per_launch,
+                "kernel": "k_cps_fast420", "kernel_avg_us": round(k_avg_ms * 1e3, 2),
+                # the kernel's launches of one timed step, summed (at most ms_per_step: they run in order)
+                "kernel_us_per_step": round(k_ms / args.steps * 1e3, 1), "bytes_per_launch": per_launch,
                 # The search is bound by int32 VALU issue, not HBM: 25 candidates x 2 directions per
                 # pel, and the VALU busy fraction of the same kernel from its PMC summary.
                 "candidate_evals_per_s": round(50 * W * H / (k_avg_ms / 1e3), -6) if k_avg_ms > 0 else 0.0,
@@ -512,9 +521,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl["desc"], "frames_per_step": nframes,
                    "mode": "stream (qmiv_frames: frame set enqueued, one wait)",
-                   "plain_loop_mpix_s": round(mpix / (el_plain / args.steps), 2),
-                   "stream_vs_plain": round(el_plain / elapsed, 3),
-                   "upload_inclusive_mpix_s": round(mpix / (el_up / up_steps), 2),
+                   "plain_loop_mpix_s": round(mpix / (el_plain / args.steps), 2) if el_plain else None,
+                   "stream_vs_plain": round(el_plain / elapsed, 3) if el_plain else None,
+                   "upload_inclusive_mpix_s": round(mpix / (el_up / up_steps), 2) if el_up else None,
                    key + "_last_frame": round(getattr(res[-1], key), 6)},
         "roofline": roof,
         "kernels_us_per_frame": {k: round(v[0] / nframes / args.steps * 1e3, 2) for k, v in prof.items()},

@@ -622,10 +622,14 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
   uint64_t* partials = (uint64_t*)(dbase + L.off_partials);
 
   FrameSummary* sum2 = (FrameSummary*)(dbase + L.off_summary2);
+  // The 4:2:0 fast search adds its row sums straight into cps_rows, which the statistics pass
+  // clears first; the other searches write per-segment partials that a reduction pass sums.
+  const bool fused_rows = (fw.need_cps || fw.need_scp) && P.fast420;
   if (fw.gcd) QMIV_HIP(hipMemcpyAsync(sum->gcd, fw.gcd, sizeof(int32_t) * 4, hipMemcpyHostToDevice, st));
   {
     KernelTimer t(ctx, QMIV_K_STATS, st);
-    QMIV_HIP(launch_stats(p0->dev, p1->dev, g, row_sd, row_ssd, row_max, st));
+    QMIV_HIP(launch_stats(p0->dev, p1->dev, g, row_sd, row_ssd, row_max, st, nullptr, fused_rows ? cps_rows : nullptr,
+                          fused_rows ? 6 * g.h : 0));
   }
   {
     KernelTimer t(ctx, QMIV_K_SUMMARY, st);
@@ -670,6 +674,7 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     a.sweep = P.sweep;
     a.nseg = L.nseg;
     a.partials = partials;
+    if (fused_rows) a.rows = cps_rows;
     a.scp0 = scp0;
     a.scp1 = scp1;
     a.best0 = fw.best0;
@@ -686,12 +691,13 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
       else if (P.fast) QMIV_HIP(launch_cps_fast(a, st));
       else QMIV_HIP(launch_cps_generic(a, st));
     }
-    {
+    if (!fused_rows) {
       KernelTimer t(ctx, QMIV_K_CPS_ROWS, st);
       QMIV_HIP(launch_cps_rows(partials, L.nseg, g.h, cps_rows, st, &sum2->in_range, P.fallback ? 1 : 0));
     }
     if (P.fallback) {
       CpsArgs b = a;
+      b.rows = nullptr;
       b.gate = kGateOutOfRange;
       b.nseg = L.nseg_gen;
       b.partials = (uint64_t*)(dbase + L.off_partials_gen);
@@ -1645,11 +1651,6 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
     QMIV_TRY(ensure_scp(ctx, scp_stride * nstreams * sizeof(uint16_t)));
     scp = (uint16_t*)ctx->d_scp;
   }
-  while ((int32_t)ctx->copy_ev.size() < n) {
-    hipEvent_t e;
-    QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    ctx->copy_ev.push_back(e);
-  }
   while ((int32_t)ctx->done_ev.size() < n) {
     hipEvent_t e;
     QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1660,32 +1661,37 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
     QMIV_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
     QMIV_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_ev, 0));
   }
+  // Each frame is finalised on the host as soon as its results have arrived, while the device
+  // works on the frames after it: between two enqueues, every frame whose results are in.
+  qmiv_status first_err = QMIV_OK;
+  int32_t done = 0;
+  auto finalize_next = [&]() {
+    const int32_t i = done++;
+    const qmiv_status st = finalize_frame(ctx, p0[i]->g, prm, metrics, fw, plans[(size_t)i].L,
+                                          (const char*)ctx->h_results + hoff[(size_t)i], &out[i]);
+    if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
+  };
   for (int32_t i = 0; i < n; i++) {
     const int32_t k = i % nstreams;
     const qmiv_status st = enqueue_frame(ctx, p0[i], p1[i], prm, fw, plans[(size_t)i],
                                          (char*)ctx->d_scratch + doff[(size_t)i], (char*)ctx->h_results + hoff[(size_t)i],
-                                         scp ? scp + scp_stride * (size_t)k : nullptr, streams[k], ctx->copy_stream,
-                                         ctx->copy_ev[(size_t)i], ctx->done_ev[(size_t)i]);
+                                         scp ? scp + scp_stride * (size_t)k : nullptr, streams[k], nullptr, nullptr,
+                                         ctx->done_ev[(size_t)i]);
     if (st != QMIV_OK) {
       // Drain what was enqueued: scratch and result buffers must be idle before any later reuse.
       (void)hipStreamSynchronize(ctx->stream);
       (void)hipStreamSynchronize(ctx->aux_stream);
-      (void)hipStreamSynchronize(ctx->copy_stream);
       return st;
     }
+    while (done < i && hipEventQuery(ctx->done_ev[(size_t)done]) == hipSuccess) finalize_next();
   }
   if (nstreams > 1) {  // later work on the context stream follows the whole set
     QMIV_HIP(hipEventRecord(ctx->join_ev, ctx->aux_stream));
     QMIV_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
   }
-  // Each frame is finalised on the host as soon as its results have arrived, while the device
-  // works on the frames after it.
-  qmiv_status first_err = QMIV_OK;
-  for (int32_t i = 0; i < n; i++) {
-    QMIV_HIP(hipEventSynchronize(ctx->done_ev[(size_t)i]));
-    qmiv_status st = finalize_frame(ctx, p0[i]->g, prm, metrics, fw, plans[(size_t)i].L,
-                                    (const char*)ctx->h_results + hoff[(size_t)i], &out[i]);
-    if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
+  while (done < n) {
+    QMIV_HIP(hipEventSynchronize(ctx->done_ev[(size_t)done]));
+    finalize_next();
   }
   QMIV_TRY(wait_stream(ctx));
   return first_err;
@@ -2711,6 +2717,15 @@ qmiv_status qmiv_frames_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* 
     QMIV_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
     QMIV_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_ev, 0));
   }
+  // Each frame is finalised on the host as soon as its results have arrived, between enqueues too.
+  qmiv_status first_err = QMIV_OK;
+  int32_t done = 0;
+  auto finalize_next = [&]() {
+    const int32_t i = done++;
+    const qmiv_status st = finalize_masked(ctx, p0[i]->g, mask[i]->g.bd, prm, metrics, plans[(size_t)i].L,
+                                           (const char*)ctx->h_results + hoff[(size_t)i], &out[i]);
+    if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
+  };
   for (int32_t i = 0; i < n; i++) {
     const hipStream_t st = streams[i % nstreams];
     qmiv_status e = enqueue_masked(ctx, p0[i], p1[i], mask[i], prm, plans[(size_t)i], -1, gcd_given,
@@ -2723,18 +2738,15 @@ qmiv_status qmiv_frames_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* 
       (void)hipStreamSynchronize(ctx->aux_stream);
       return e;
     }
+    while (done < i && hipEventQuery(ctx->done_ev[(size_t)done]) == hipSuccess) finalize_next();
   }
   if (nstreams > 1) {  // later work on the context stream follows the whole set
     QMIV_HIP(hipEventRecord(ctx->join_ev, ctx->aux_stream));
     QMIV_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
   }
-  // Each frame is finalised on the host as soon as its results have arrived.
-  qmiv_status first_err = QMIV_OK;
-  for (int32_t i = 0; i < n; i++) {
-    QMIV_HIP(hipEventSynchronize(ctx->done_ev[(size_t)i]));
-    const qmiv_status st = finalize_masked(ctx, p0[i]->g, mask[i]->g.bd, prm, metrics, plans[(size_t)i].L,
-                                           (const char*)ctx->h_results + hoff[(size_t)i], &out[i]);
-    if (st != QMIV_OK && first_err == QMIV_OK) first_err = st;
+  while (done < n) {
+    QMIV_HIP(hipEventSynchronize(ctx->done_ev[(size_t)done]));
+    finalize_next();
   }
   QMIV_TRY(wait_stream(ctx));  // both streams idle; the set's kernel timings are harvested
   return first_err;

@@ -538,7 +538,8 @@ __global__ __launch_bounds__(256) void k_cps_fast420(CpsArgs a) {
     search_dir<T, TM, MASKED, 1, OUT>(a, g, x0, y0, lane, wave, gcd, s_pic, s_tbl, s_mask, acc1);
   }
 
-  // One partial per (segment, direction, component, row): reduce over the wave's 64 lanes.
+  // One sum per (segment, direction, component, row): reduce over the wave's 64 lanes, then either
+  // write the segment's partial or add it into the row (integer sums: any order gives the same).
 #pragma unroll
   for (int r = 0; r < 2; r++) {
     const int y = y0 + 2 * wave + r;
@@ -550,8 +551,13 @@ __global__ __launch_bounds__(256) void k_cps_fast420(CpsArgs a) {
       // second direction here.
       const int d0 = MASKED ? 1 : 0, d1 = MASKED ? 0 : 1;
       if (lane == 0 && y < g.h) {
-        a.partials[(((int64_t)seg * 2 + d0) * 3 + c) * g.h + y] = v0;
-        a.partials[(((int64_t)seg * 2 + d1) * 3 + c) * g.h + y] = v1;
+        if (a.rows) {
+          atomicAdd(reinterpret_cast<unsigned long long*>(&a.rows[((int64_t)d0 * 3 + c) * g.h + y]), (unsigned long long)v0);
+          atomicAdd(reinterpret_cast<unsigned long long*>(&a.rows[((int64_t)d1 * 3 + c) * g.h + y]), (unsigned long long)v1);
+        } else {
+          a.partials[(((int64_t)seg * 2 + d0) * 3 + c) * g.h + y] = v0;
+          a.partials[(((int64_t)seg * 2 + d1) * 3 + c) * g.h + y] = v1;
+        }
       }
     }
   }

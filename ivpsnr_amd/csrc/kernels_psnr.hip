@@ -9,6 +9,7 @@
 #include "qmiv_internal.h"
 #include "device_common.h"
 
+#include <algorithm>
 #include <type_traits>
 #include <climits>
 
@@ -28,14 +29,31 @@ __device__ __forceinline__ uint32_t ldpel(const T* __restrict__ p, int64_t i) {
 // ---------------------------------------------------------------------------------------------
 // gate (nullable): the summary of a first pass; when it found both pictures in range this pass has
 // nothing to redo (the concealment flow re-runs the statistics only after clipping).
+// zero / nzero: an array the pass clears in any case (the CPS kernel's per-row sums, which it adds
+// into after this pass).
+//
+// 16-bit samples of at most 12-bit depth take packed 16-bit arithmetic: per two samples one
+// v_pk_sub_i16 difference, one v_dot2 for the two squares, one v_dot2 against (1, 1) for the two
+// differences and one v_pk_max_u16 per picture. Exact while every sample is below 2^12 (differences
+// below 2^12 in magnitude, squares below 2^24, one group's eight dot products below 2^28); a row
+// holding a larger sample (a pel above the bit depth) is summed again by the scalar loop.
+typedef short qmiv_s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short qmiv_u16x2 __attribute__((ext_vector_type(2)));
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_stats(const T* __restrict__ p0, const T* __restrict__ p1, Geom g,
                                                int64_t* __restrict__ row_sd, uint64_t* __restrict__ row_ssd,
-                                               uint32_t* __restrict__ row_max, const FrameSummary* __restrict__ gate) {
+                                               uint32_t* __restrict__ row_max, const FrameSummary* __restrict__ gate,
+                                               uint64_t* __restrict__ zero, int32_t nzero) {
+  {
+    const int32_t zi = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (zi < nzero) zero[zi] = 0;
+  }
   if (gate && gate->in_range) return;
   const int lane = threadIdx.x & 63;
   const int nrows = g.h + 2 * g.ch;
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // The row is the wave's: scalar registers hold its address.
+  const int r = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (r >= nrows) return;
   int64_t off;
   int32_t width;
@@ -55,60 +73,120 @@ __global__ __launch_bounds__(256) void k_stats(const T* __restrict__ p0, const T
   uint64_t ssd = 0;
   uint32_t m0 = 0, m1 = 0;
   constexpr int kVec = 16 / sizeof(T);  // samples per 16-byte load
-  int32_t vec_end = 0;
-  // Adds one 16-byte group of both pictures to the lane's sums.
-  auto group = [&](const uint4& va, const uint4& vb) {
-    const uint32_t wa[4] = {va.x, va.y, va.z, va.w};
-    const uint32_t wb[4] = {vb.x, vb.y, vb.z, vb.w};
-    int32_t sd_l = 0;
+  const bool aligned = (((uintptr_t)a | (uintptr_t)b) & 15) == 0;
+  bool scalar = true;
+  if constexpr (sizeof(T) == 2) {
+    if (g.bd <= 12 && aligned) {
+      const int32_t vec_end = (width / kVec) * kVec;
+      qmiv_u16x2 mx0 = {0, 0}, mx1 = {0, 0};
+      // Two groups per picture in flight per lane.
+      int32_t x = lane * kVec;
+      auto group = [&](const uint4& va, const uint4& vb, int32_t& sq, int32_t& df) {
+        const uint32_t wa[4] = {va.x, va.y, va.z, va.w};
+        const uint32_t wb[4] = {vb.x, vb.y, vb.z, vb.w};
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-#pragma unroll
-      for (int s = 0; s < (int)(4 / sizeof(T)); s++) {
-        const uint32_t sh = s * 8 * sizeof(T);
-        const uint32_t mask = sizeof(T) == 1 ? 0xffu : 0xffffu;
-        const uint32_t ea = (wa[k] >> sh) & mask;
-        const uint32_t eb = (wb[k] >> sh) & mask;
+        for (int k = 0; k < 4; k++) {
+          const qmiv_u16x2 ua = __builtin_bit_cast(qmiv_u16x2, wa[k]);
+          const qmiv_u16x2 ub = __builtin_bit_cast(qmiv_u16x2, wb[k]);
+          const qmiv_s16x2 d = __builtin_bit_cast(qmiv_s16x2, wb[k]) - __builtin_bit_cast(qmiv_s16x2, wa[k]);
+          sq = __builtin_amdgcn_sdot2(d, d, sq, false);
+          df = __builtin_amdgcn_sdot2(d, (qmiv_s16x2){1, 1}, df, false);
+          mx0 = __builtin_elementwise_max(mx0, ua);
+          mx1 = __builtin_elementwise_max(mx1, ub);
+        }
+      };
+      for (; x + 64 * kVec < vec_end; x += 2 * 64 * kVec) {
+        const uint4 va0 = *reinterpret_cast<const uint4*>(a + x), vb0 = *reinterpret_cast<const uint4*>(b + x);
+        const uint4 va1 = *reinterpret_cast<const uint4*>(a + x + 64 * kVec);
+        const uint4 vb1 = *reinterpret_cast<const uint4*>(b + x + 64 * kVec);
+        int32_t sq = 0, df = 0;
+        group(va0, vb0, sq, df);
+        group(va1, vb1, sq, df);
+        ssd += (uint32_t)sq;
+        sd += df;
+      }
+      if (x < vec_end) {
+        int32_t sq = 0, df = 0;
+        group(*reinterpret_cast<const uint4*>(a + x), *reinterpret_cast<const uint4*>(b + x), sq, df);
+        ssd += (uint32_t)sq;
+        sd += df;
+      }
+      for (int32_t t = vec_end + lane; t < width; t += 64) {
+        const uint32_t ea = ldpel(a, t), eb = ldpel(b, t);
         const int32_t d = (int32_t)eb - (int32_t)ea;
-        sd_l += d;
-        const uint32_t sq = (uint32_t)d * (uint32_t)d;
-        ssd += sq;
+        sd += d;
+        ssd += (uint32_t)(d * d);
         m0 = max(m0, ea);
         m1 = max(m1, eb);
       }
-    }
-    sd += sd_l;
-  };
-  if ((((uintptr_t)a | (uintptr_t)b) & 15) == 0) {
-    vec_end = (width / kVec) * kVec;
-    // Four groups per picture in flight per lane before any is used (HBM needs the bytes in flight).
-    constexpr int U = 4;
-    int32_t x = lane * kVec;
-    for (; x + (U - 1) * 64 * kVec < vec_end; x += U * 64 * kVec) {
-      uint4 va[U], vb[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        va[u] = *reinterpret_cast<const uint4*>(a + x + u * 64 * kVec);
-        vb[u] = *reinterpret_cast<const uint4*>(b + x + u * 64 * kVec);
+      m0 = max(m0, (uint32_t)max(mx0.x, mx0.y));
+      m1 = max(m1, (uint32_t)max(mx1.x, mx1.y));
+      m0 = wave_max(m0);
+      m1 = wave_max(m1);
+      scalar = max(m0, m1) > 4095u;  // a sample outside the packed path's range: sum the row again
+      if (scalar) {
+        sd = 0;
+        ssd = 0;
       }
-#pragma unroll
-      for (int u = 0; u < U; u++) group(va[u], vb[u]);
     }
-    for (; x < vec_end; x += 64 * kVec)
-      group(*reinterpret_cast<const uint4*>(a + x), *reinterpret_cast<const uint4*>(b + x));
   }
-  for (int32_t x = vec_end + lane; x < width; x += 64) {
-    const uint32_t ea = ldpel(a, x), eb = ldpel(b, x);
-    const int32_t d = (int32_t)eb - (int32_t)ea;
-    sd += d;
-    ssd += (uint32_t)d * (uint32_t)d;
-    m0 = max(m0, ea);
-    m1 = max(m1, eb);
+  if (scalar) {
+    int32_t vec_end = 0;
+    // Adds one 16-byte group of both pictures to the lane's sums.
+    auto group = [&](const uint4& va, const uint4& vb) {
+      const uint32_t wa[4] = {va.x, va.y, va.z, va.w};
+      const uint32_t wb[4] = {vb.x, vb.y, vb.z, vb.w};
+      int32_t sd_l = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+#pragma unroll
+        for (int s = 0; s < (int)(4 / sizeof(T)); s++) {
+          const uint32_t sh = s * 8 * sizeof(T);
+          const uint32_t mask = sizeof(T) == 1 ? 0xffu : 0xffffu;
+          const uint32_t ea = (wa[k] >> sh) & mask;
+          const uint32_t eb = (wb[k] >> sh) & mask;
+          const int32_t d = (int32_t)eb - (int32_t)ea;
+          sd_l += d;
+          const uint32_t sq = (uint32_t)d * (uint32_t)d;
+          ssd += sq;
+          m0 = max(m0, ea);
+          m1 = max(m1, eb);
+        }
+      }
+      sd += sd_l;
+    };
+    if (aligned) {
+      vec_end = (width / kVec) * kVec;
+      // 8-bit samples: four groups per picture in flight per lane before any is used (HBM needs the
+      // bytes in flight); 16-bit samples reach here only for rare rows and keep registers low.
+      constexpr int U = sizeof(T) == 1 ? 4 : 1;
+      int32_t x = lane * kVec;
+      for (; x + (U - 1) * 64 * kVec < vec_end; x += U * 64 * kVec) {
+        uint4 va[U], vb[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          va[u] = *reinterpret_cast<const uint4*>(a + x + u * 64 * kVec);
+          vb[u] = *reinterpret_cast<const uint4*>(b + x + u * 64 * kVec);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) group(va[u], vb[u]);
+      }
+      for (; x < vec_end; x += 64 * kVec)
+        group(*reinterpret_cast<const uint4*>(a + x), *reinterpret_cast<const uint4*>(b + x));
+    }
+    for (int32_t x = vec_end + lane; x < width; x += 64) {
+      const uint32_t ea = ldpel(a, x), eb = ldpel(b, x);
+      const int32_t d = (int32_t)eb - (int32_t)ea;
+      sd += d;
+      ssd += (uint32_t)d * (uint32_t)d;
+      m0 = max(m0, ea);
+      m1 = max(m1, eb);
+    }
+    m0 = wave_max(m0);
+    m1 = wave_max(m1);
   }
   sd = wave_sum(sd);
   ssd = wave_sum(ssd);
-  m0 = wave_max(m0);
-  m1 = wave_max(m1);
   if (lane == 0) {
     row_sd[r] = sd;
     row_ssd[r] = ssd;
@@ -627,15 +705,18 @@ __global__ __launch_bounds__(256) void k_cps_rows(const uint64_t* __restrict__ p
 }  // namespace
 
 hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* row_max, hipStream_t stream, const FrameSummary* gate) {
+                        uint32_t* row_max, hipStream_t stream, const FrameSummary* gate, uint64_t* zero,
+                        int32_t nzero) {
   const int nrows = g.h + 2 * g.ch;
-  dim3 grid((nrows + 3) / 4), block(256);
+  if (nzero < 0 || (nzero > 0 && !zero)) return hipErrorInvalidValue;
+  const int blocks = std::max((nrows + 3) / 4, (nzero + 255) / 256);
+  dim3 grid(blocks), block(256);
   if (g.bytes_per_sample == 1)
     hipLaunchKernelGGL(k_stats<uint8_t>, grid, block, 0, stream, (const uint8_t*)p0, (const uint8_t*)p1, g, row_sd,
-                       row_ssd, row_max, gate);
+                       row_ssd, row_max, gate, zero, nzero);
   else
     hipLaunchKernelGGL(k_stats<uint16_t>, grid, block, 0, stream, (const uint16_t*)p0, (const uint16_t*)p1, g,
-                       row_sd, row_ssd, row_max, gate);
+                       row_sd, row_ssd, row_max, gate, zero, nzero);
   return hipGetLastError();
 }
 

@@ -50,6 +50,9 @@ struct CpsArgs {
   int32_t sweep;             // tiles per workgroup
   int32_t nseg;              // segments per band (= ceil(tiles_x / sweep)); generic kernel: column blocks
   uint64_t* partials;        // [nseg][2][3][h] per-segment row sums
+  // 4:2:0 fast kernel: when set, the row sums are added atomically into rows[2][3][h] (cleared
+  // beforehand) instead of being written to partials, so no reduction pass follows.
+  uint64_t* rows = nullptr;
   uint16_t* scp0;            // optional [3][h][w]: clipU(P0(best of P1 - g) + g)   (SCP of InputFile0)
   uint16_t* scp1;            // optional [3][h][w]: clipU(P1(best of P0 + g) - g)   (SCP of InputFile1)
   int16_t* best0;            // optional [h][w]: R2T winner, window raster index (dy + B)(2B + 1) + dx + B
@@ -126,8 +129,11 @@ struct SsimBatchArgs {
 // Kernel launchers (kernels.hip). All launch on `stream` and return hipSuccess or the launch error.
 // gate (nullable): a first pass's summary; a pass whose gate found both pictures in range does
 // nothing (statistics) or copies the gate (summary): the statistics re-run after concealment.
+// zero / nzero (launch_stats): an array that the statistics pass also clears (the per-row CPS sums
+// that the 4:2:0 fast kernel adds into), whatever the gate says.
 hipError_t launch_stats(const void* p0, const void* p1, const Geom& g, int64_t* row_sd, uint64_t* row_ssd,
-                        uint32_t* row_max, hipStream_t stream, const FrameSummary* gate = nullptr);
+                        uint32_t* row_max, hipStream_t stream, const FrameSummary* gate = nullptr,
+                        uint64_t* zero = nullptr, int32_t nzero = 0);
 hipError_t launch_summary(const Geom& g, const int64_t* row_sd, const uint64_t* row_ssd, const uint32_t* row_max,
                           const int32_t maxdiff[4], int32_t compute_gcd, FrameSummary* sum, hipStream_t stream,
                           const FrameSummary* gate = nullptr);
