@@ -283,6 +283,25 @@ qmiv_status qmiv_frame_noalloc(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic
 qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
                         const qmiv_params* prm, uint32_t metrics, qmiv_frame_result* out);
 
+/* qmiv_frames with each frame's upload: the packed frames h0[i], h1[i] (qmiv_frame_bytes each, in
+   the file layout qmiv_pic_upload takes) are copied into p0[i], p1[i] on the context's copy stream,
+   and frame i's kernels wait for its own two copies only, so the copies of frame i + 1 run while
+   frame i is computed. Replaces the reference application's read-then-process frame loop
+   (libs/pmbb-core/src/xSeq.cpp:155-174 per frame, apps/QMIV/src/xAppQMIV.cpp:666-748). The copies
+   run at the link's rate from pinned memory (qmiv_host_alloc); from ordinary memory the runtime
+   stages them and the call returns later. No picture may appear twice among the 2n pictures
+   (QMIV_ERR_INVALID_ARG). The sources are no longer read once the call has returned. */
+qmiv_status qmiv_frames_upload(qmiv_ctx* ctx, int32_t n, qmiv_pic* const* p0, qmiv_pic* const* p1,
+                               const void* const* h0, const void* const* h1, const qmiv_params* prm,
+                               uint32_t metrics, qmiv_frame_result* out);
+
+/* Pinned (page-locked) host memory of the context's device, for frames handed to the engine:
+   host-to-device copies from it run asynchronously at the link's rate. No counterpart in the
+   reference, whose reader fills ordinary memory (libs/pmbb-core/src/xSeq.cpp:39-56).
+   qmiv_host_free reads nothing of ctx and may run after qmiv_destroy. */
+qmiv_status qmiv_host_alloc(qmiv_ctx* ctx, size_t bytes, void** ptr);
+qmiv_status qmiv_host_free(qmiv_ctx* ctx, void* ptr);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Processor-level calls (same argument meaning as the reference's processor methods)          */
 /* ------------------------------------------------------------------------------------------ */

@@ -125,6 +125,10 @@ _SIGNATURES = {
                               C.c_uint32, C.POINTER(FrameResult)]),
     "qmiv_frame_noalloc": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), C.c_uint32,
                                      C.POINTER(FrameResult)]),
+    "qmiv_frames_upload": (C.c_int, [_ctx_p, C.c_int32, C.POINTER(_pic_p), C.POINTER(_pic_p), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_void_p), C.POINTER(Params), C.c_uint32, C.POINTER(FrameResult)]),
+    "qmiv_host_alloc": (C.c_int, [_ctx_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "qmiv_host_free": (C.c_int, [_ctx_p, C.c_void_p]),
     "qmiv_calc_gcd": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _i32p]),
     "qmiv_calc_psnr": (C.c_int, [_ctx_p, _pic_p, _pic_p, _f64p]),
     "qmiv_calc_wspsnr": (C.c_int, [_ctx_p, _pic_p, _pic_p, C.POINTER(Params), _f64p]),

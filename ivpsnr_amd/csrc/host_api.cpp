@@ -1605,6 +1605,12 @@ qmiv_status qmiv::upload_file_frame(qmiv_ctx* ctx, qmiv_pic* pic, const void* ho
   return QMIV_OK;
 }
 
+namespace {
+qmiv_status frames_impl(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                        const void* const* h0, const void* const* h1, const qmiv_params* prm_in, uint32_t metrics,
+                        qmiv_frame_result* out);
+}  // namespace
+
 extern "C" {
 
 qmiv_status qmiv_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm_in,
@@ -1617,6 +1623,38 @@ qmiv_status qmiv_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, co
 qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
                         const qmiv_params* prm_in, uint32_t metrics, qmiv_frame_result* out) {
   if (!ctx || !out || !p0 || !p1 || n < 0) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  return frames_impl(ctx, n, p0, p1, nullptr, nullptr, prm_in, metrics, out);
+}
+
+qmiv_status qmiv_frames_upload(qmiv_ctx* ctx, int32_t n, qmiv_pic* const* p0, qmiv_pic* const* p1,
+                               const void* const* h0, const void* const* h1, const qmiv_params* prm_in,
+                               uint32_t metrics, qmiv_frame_result* out) {
+  if (!ctx || !out || !p0 || !p1 || !h0 || !h1 || n < 0) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  // Every picture is written by its frame's upload while earlier frames compute: no picture may
+  // serve two frames of the set.
+  std::vector<const qmiv_pic*> all;
+  all.reserve(2 * (size_t)n);
+  for (int32_t i = 0; i < n; i++) {
+    if (!h0[i] || !h1[i]) return fail(QMIV_ERR_INVALID_ARG, "null source frame");
+    all.push_back(p0[i]);
+    all.push_back(p1[i]);
+  }
+  std::sort(all.begin(), all.end());
+  if (std::adjacent_find(all.begin(), all.end()) != all.end())
+    return fail(QMIV_ERR_INVALID_ARG, "a picture appears twice in the frame set");
+  std::vector<const qmiv_pic*> c0(p0, p0 + n), c1(p1, p1 + n);
+  return frames_impl(ctx, n, c0.data(), c1.data(), h0, h1, prm_in, metrics, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+// The frame-set loop of qmiv_frames and qmiv_frames_upload. h0 / h1 (nullable): packed host frames
+// copied into p0[i] / p1[i] on the copy stream, each frame's kernels waiting for its own copies.
+qmiv_status frames_impl(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                        const void* const* h0, const void* const* h1, const qmiv_params* prm_in, uint32_t metrics,
+                        qmiv_frame_result* out) {
   if (n == 0) return QMIV_OK;
   for (int32_t i = 0; i < n; i++) QMIV_TRY(check_pair(p0[i], p1[i]));
   qmiv_params prm;
@@ -1656,10 +1694,16 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
     QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ctx->done_ev.push_back(e);
   }
+  while (h0 && (int32_t)ctx->copy_ev.size() < n) {
+    hipEvent_t e;
+    QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->copy_ev.push_back(e);
+  }
   const hipStream_t streams[2] = {ctx->stream, ctx->aux_stream};
-  if (nstreams > 1) {  // the aux stream starts after everything already on the context stream
+  if (nstreams > 1 || h0) {  // the aux and copy streams start after everything on the context stream
     QMIV_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
-    QMIV_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_ev, 0));
+    if (nstreams > 1) QMIV_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_ev, 0));
+    if (h0) QMIV_HIP(hipStreamWaitEvent(ctx->copy_stream, ctx->fork_ev, 0));
   }
   // Each frame is finalised on the host as soon as its results have arrived, while the device
   // works on the frames after it: between two enqueues, every frame whose results are in.
@@ -1673,14 +1717,24 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
   };
   for (int32_t i = 0; i < n; i++) {
     const int32_t k = i % nstreams;
-    const qmiv_status st = enqueue_frame(ctx, p0[i], p1[i], prm, fw, plans[(size_t)i],
-                                         (char*)ctx->d_scratch + doff[(size_t)i], (char*)ctx->h_results + hoff[(size_t)i],
-                                         scp ? scp + scp_stride * (size_t)k : nullptr, streams[k], nullptr, nullptr,
-                                         ctx->done_ev[(size_t)i]);
+    qmiv_status st = QMIV_OK;
+    if (h0) {  // xSeq::readFrame per frame (libs/pmbb-core/src/xSeq.cpp:155-174), here an async copy
+      hipError_t e = hipMemcpyAsync(p0[i]->dev, h0[i], p0[i]->g.frame_bytes, hipMemcpyHostToDevice, ctx->copy_stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(p1[i]->dev, h1[i], p1[i]->g.frame_bytes, hipMemcpyHostToDevice, ctx->copy_stream);
+      if (e == hipSuccess) e = hipEventRecord(ctx->copy_ev[(size_t)i], ctx->copy_stream);
+      if (e == hipSuccess) e = hipStreamWaitEvent(streams[k], ctx->copy_ev[(size_t)i], 0);
+      if (e != hipSuccess) st = fail(QMIV_ERR_HIP, std::string("frame upload: ") + hipGetErrorString(e));
+    }
+    if (st == QMIV_OK)
+      st = enqueue_frame(ctx, p0[i], p1[i], prm, fw, plans[(size_t)i], (char*)ctx->d_scratch + doff[(size_t)i],
+                         (char*)ctx->h_results + hoff[(size_t)i], scp ? scp + scp_stride * (size_t)k : nullptr,
+                         streams[k], nullptr, nullptr, ctx->done_ev[(size_t)i]);
     if (st != QMIV_OK) {
       // Drain what was enqueued: scratch and result buffers must be idle before any later reuse.
       (void)hipStreamSynchronize(ctx->stream);
       (void)hipStreamSynchronize(ctx->aux_stream);
+      (void)hipStreamSynchronize(ctx->copy_stream);
       return st;
     }
     while (done < i && hipEventQuery(ctx->done_ev[(size_t)done]) == hipSuccess) finalize_next();
@@ -1695,6 +1749,26 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
   }
   QMIV_TRY(wait_stream(ctx));
   return first_err;
+}
+
+}  // namespace
+
+extern "C" {
+
+qmiv_status qmiv_host_alloc(qmiv_ctx* ctx, size_t bytes, void** ptr) {
+  if (!ctx || !ptr || bytes == 0) return fail(QMIV_ERR_INVALID_ARG, "null argument or zero size");
+  *ptr = nullptr;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  QMIV_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+  return QMIV_OK;
+}
+
+// Reads nothing of the context: pinned memory may outlive the context it was allocated through.
+qmiv_status qmiv_host_free(qmiv_ctx* ctx, void* ptr) {
+  (void)ctx;
+  if (!ptr) return QMIV_OK;
+  QMIV_HIP(hipHostFree(ptr));
+  return QMIV_OK;
 }
 
 qmiv_status qmiv_frame_noalloc(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1, const qmiv_params* prm_in,

@@ -243,6 +243,38 @@ class Engine:
             check(st, "qmiv_frames")
         return list(out)
 
+    def frames_upload(self, pairs, sources, metrics: int, params: Params | None = None, raise_on_error: bool = True):
+        """The (p0, p1) pairs computed as in frames(), each frame first uploaded from its (a, b) packed
+        host frames in `sources` on the copy stream, overlapping the frames before it
+        (qmiv_frames_upload). Sources from host_array() are copied at the link's rate."""
+        n = len(pairs)
+        if len(sources) != n:
+            raise ValueError("one (a, b) source pair per frame")
+        srcs = [(np.ascontiguousarray(a), np.ascontiguousarray(b)) for a, b in sources]
+        for (p0, p1), (a, b) in zip(pairs, srcs):
+            if a.nbytes != p0.nbytes or b.nbytes != p1.nbytes:
+                raise ValueError("source frame size differs from its picture's")
+        pa = (C.c_void_p * n)(*[p[0].handle.value for p in pairs])
+        pb = (C.c_void_p * n)(*[p[1].handle.value for p in pairs])
+        ha = (C.c_void_p * n)(*[a.ctypes.data for a, _ in srcs])
+        hb = (C.c_void_p * n)(*[b.ctypes.data for _, b in srcs])
+        out = (FrameResult * n)()
+        st = lib().qmiv_frames_upload(self.ctx, n, pa, pb, ha, hb, C.byref(params) if params else None, metrics, out)
+        if raise_on_error:
+            check(st, "qmiv_frames_upload")
+        return list(out)
+
+    def host_array(self, nbytes: int) -> np.ndarray:
+        """A uint8 array of nbytes in pinned host memory of this engine's device (qmiv_host_alloc),
+        freed when the array (and every view of it) is gone."""
+        ptr = C.c_void_p()
+        check(lib().qmiv_host_alloc(self.ctx, nbytes, C.byref(ptr)), "qmiv_host_alloc")
+        buf = (C.c_uint8 * nbytes).from_address(ptr.value)
+        arr = np.frombuffer(buf, dtype=np.uint8)
+        ctx = self.ctx
+        weakref.finalize(buf, lambda: lib().qmiv_host_free(ctx, ptr))
+        return arr
+
     def batch(self, pairs, metrics, params=None, branches: int = 0, cps_sweep: int = -1,
               tune_reps: int = 3) -> "Batch":
         """A captured frame set over the (p0, p1) pairs (qmiv_batch_create). `metrics` and

@@ -1,0 +1,76 @@
+"""Frame sets with their uploads (qmiv_frames_upload): every frame's copies on the copy stream,
+overlapping the frames before it, from pinned (qmiv_host_alloc) and from ordinary host memory.
+
+The results must equal the resident path's (qmiv_frames over pictures uploaded beforehand) field
+by field, every double included; one frame is checked against the oracle as well.
+"""
+import numpy as np
+import pytest
+
+from ivpsnr_amd import IVPSNR, IVSSIM, MSSSIM, PSNR, SSIM, WSPSNR, default_params
+from ivpsnr_amd.capi import QmivError
+from ivpsnr_amd.synth import make_pair
+from tests.helpers import assert_frame_equal, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+ALL = PSNR | WSPSNR | IVPSNR | SSIM | MSSSIM | IVSSIM
+FRAMES = [(256, 128, 10, 420, 1), (130, 66, 10, 420, 2), (67, 45, 8, 444, 3), (256, 128, 10, 420, 4),
+          (1920, 1080, 10, 420, 5)]
+
+
+def _same(a, b):
+    da, db = a.as_dict(), b.as_dict()
+    assert da.keys() == db.keys()
+    for k in da:
+        va, vb = np.asarray(da[k], np.float64), np.asarray(db[k], np.float64)
+        assert np.array_equal(np.isnan(va), np.isnan(vb)) and np.array_equal(va[~np.isnan(va)], vb[~np.isnan(vb)]), k
+
+
+def _pics(engine, pairs):
+    return [(engine.picture(p.width, p.height, p.bit_depth, p.chroma_format),
+             engine.picture(p.width, p.height, p.bit_depth, p.chroma_format)) for p in pairs]
+
+
+@pytest.mark.parametrize("pinned", [True, False])
+@pytest.mark.parametrize("erp", [False, True])
+def test_matches_resident_path(engine, pinned, erp):
+    print(f"frames_upload pinned={pinned} erp={erp}", flush=True)
+    pairs = [make_pair(w, h, bd, s, chroma_format=cf) for w, h, bd, cf, s in FRAMES]
+    prm = default_params(erp=erp)
+    resident = [(engine.picture(p.width, p.height, p.bit_depth, p.chroma_format, data=p.ref),
+                 engine.picture(p.width, p.height, p.bit_depth, p.chroma_format, data=p.tst)) for p in pairs]
+    want = engine.frames(resident, ALL, prm)
+    srcs = []
+    for p in pairs:
+        a, b = np.ascontiguousarray(p.ref).view(np.uint8).reshape(-1), np.ascontiguousarray(p.tst).view(np.uint8).reshape(-1)
+        if pinned:
+            ha, hb = engine.host_array(a.nbytes), engine.host_array(b.nbytes)
+            ha[:] = a
+            hb[:] = b
+            a, b = ha, hb
+        srcs.append((a, b))
+    pics = _pics(engine, pairs)
+    # twice in a row into the same pictures: the second set's copies follow the first set's kernels
+    for rep in range(2):
+        got = engine.frames_upload(pics, srcs, ALL, prm)
+        for i, (g, w_) in enumerate(zip(got, want)):
+            print(f"  rep {rep} frame {i} {FRAMES[i]}", flush=True)
+            _same(g, w_)
+
+
+def test_matches_oracle(engine, oracle):
+    w, h, bd, cf, s = 130, 66, 10, 420, 7
+    pair = make_pair(w, h, bd, s)
+    prm = default_params(erp=True)
+    pics = _pics(engine, [pair])
+    got = engine.frames_upload(pics, [(pair.ref, pair.tst)], PSNR | WSPSNR | IVPSNR, prm)[0]
+    orc = oracle.frame(pair.ref, pair.tst, oracle_params(oracle, w, h, bd, cf, PSNR | WSPSNR | IVPSNR, prm))
+    assert_frame_equal(got, orc, PSNR | WSPSNR | IVPSNR, "frames_upload")
+
+
+def test_picture_twice_is_rejected(engine):
+    pair = make_pair(64, 48, 8, 1)
+    p0, p1 = _pics(engine, [pair])[0]
+    with pytest.raises(QmivError, match="twice"):
+        engine.frames_upload([(p0, p1), (p0, p1)], [(pair.ref, pair.tst)] * 2, PSNR)
