@@ -460,17 +460,24 @@ qmiv_status qmiv_pic_swap_planes(qmiv_ctx* ctx, qmiv_pic* pic, int32_t a, int32_
 typedef struct qmiv_batch qmiv_batch;
 
 typedef struct qmiv_batch_options {
-  int32_t branches;   /* concurrent graph branches, 1..16; 0: autotuned over {1, 2, 4, 8} */
+  int32_t branches;   /* concurrent graph branches, 1..16; 0: autotuned over {1, 2, 4, 8, 16} */
   int32_t cps_sweep;  /* cap on CPS tiles per workgroup; 0: occupancy rule; -1: autotuned over {rule, 1, 2, 4} */
   int32_t tune_reps;  /* timed replays per autotuner candidate and round (default 3); the fastest counts */
 } qmiv_batch_options;
+/* The autotuner times every candidate in 5 interleaved rounds and ranks candidates by the median
+   of their rounds; a candidate with more branches (or the occupancy rule for the sweep) is kept
+   unless another one is more than 2% faster. Its choice is remembered per context and batch
+   signature (the views' sizes, formats, metrics and parameters): a later qmiv_batch_create with the
+   same signature and autotuned options takes it without timing again (from_cache = 1). */
 
 typedef struct qmiv_batch_info {
   int32_t views;
   int32_t branches;          /* chosen branch count */
   int32_t cps_sweep;         /* chosen sweep cap (0 = occupancy rule) */
-  int32_t candidates_tried;  /* autotuner candidates timed */
-  double tuned_ms;           /* replay time of the chosen graph while tuning (0 when not tuned) */
+  int32_t candidates_tried;  /* autotuner candidate timings (rounds included) */
+  double tuned_ms;           /* median replay time of the chosen graph while tuning (0 when not tuned) */
+  int32_t tune_rounds;       /* rounds per candidate (0 when not tuned) */
+  int32_t from_cache;        /* 1: the choice came from an earlier batch of the same signature */
 } qmiv_batch_info;
 
 void qmiv_batch_options_default(qmiv_batch_options* opt);

@@ -2118,43 +2118,82 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
   std::vector<int32_t> br_cands, sw_cands;
   if (opt.branches > 0) br_cands = {std::min(opt.branches, n)};
   else
-    for (int32_t c : {1, 2, 4, 8})
+    for (int32_t c : {1, 2, 4, 8, 16})
       if (c <= n) br_cands.push_back(c);
   if (opt.cps_sweep >= 0) sw_cands = {opt.cps_sweep};
   else sw_cands = {0, 1, 2, 4};
   int32_t best_br = br_cands[0], best_sw = sw_cands[0];
-  const bool tune = br_cands.size() > 1 || sw_cands.size() > 1;
+  bool tune = br_cands.size() > 1 || sw_cands.size() > 1;
   double best_ms = 0.0;
   qmiv_status st = QMIV_OK;
+  // The batch signature: every view's geometry, metrics and parameters, and which knobs are tuned.
+  std::string sig;
+  {
+    auto put = [&](const void* p, size_t k) { sig.append((const char*)p, k); };
+    const int32_t knobs[2] = {opt.branches, opt.cps_sweep};
+    put(knobs, sizeof(knobs));
+    for (int32_t i = 0; i < n; i++) {
+      put(&p0[i]->g, sizeof(Geom));
+      put(&metrics[i], sizeof(uint32_t));
+      put(&b->prm[(size_t)i], sizeof(qmiv_params));
+    }
+  }
+  if (tune)
+    for (const auto& e : ctx->batch_tuned)
+      if (e.first == sig) {
+        best_br = e.second.first;
+        best_sw = e.second.second;
+        b->info.from_cache = 1;
+        tune = false;
+        break;
+      }
   if (tune) {
-    // Each stage times its candidates in two rounds and keeps each one's best time: a candidate
-    // timed once while the clocks ramp can lose by more than the candidates differ.
-    auto time_one = [&](int32_t br, int32_t sw, double& ms) -> qmiv_status {
+    // Every candidate is timed in kRounds interleaved rounds (so that clock ramps and other work
+    // spread over all of them) and ranked by the median of its rounds.
+    constexpr int kRounds = 5;
+    b->info.tune_rounds = kRounds;
+    auto median = [](std::vector<double> v) {
+      std::sort(v.begin(), v.end());
+      return v[v.size() / 2];
+    };
+    auto time_one = [&](int32_t br, int32_t sw, std::vector<double>& samples) -> qmiv_status {
       QMIV_TRY(batch_build(b, br, sw));
       double t = 0.0;
       QMIV_TRY(batch_time(b, opt.tune_reps, t));
       b->info.candidates_tried++;
-      ms = std::min(ms, t);
+      samples.push_back(t);
       return QMIV_OK;
     };
-    std::vector<double> br_ms(br_cands.size(), 1e300);
-    for (int round = 0; round < 2 && st == QMIV_OK; round++)
-      for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) st = time_one(br_cands[i], sw_cands[0], br_ms[i]);
-    best_ms = 1e300;
-    for (size_t i = 0; i < br_cands.size(); i++)
-      if (br_ms[i] < best_ms) {
-        best_ms = br_ms[i];
+    std::vector<std::vector<double>> br_t(br_cands.size());
+    for (int round = 0; round < kRounds && st == QMIV_OK; round++)
+      for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) st = time_one(br_cands[i], sw_cands[0], br_t[i]);
+    std::vector<double> br_med(br_cands.size(), 1e300);
+    double fastest = 1e300;
+    for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) fastest = std::min(fastest, br_med[i] = median(br_t[i]));
+    // More branches unless fewer are more than 2% faster: the choice does not flip on noise.
+    for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++)
+      if (br_med[i] <= 1.02 * fastest) {
         best_br = br_cands[i];
+        best_ms = br_med[i];
       }
-    std::vector<double> sw_ms(sw_cands.size(), 1e300);
-    sw_ms[0] = best_ms;
-    for (int round = 0; round < 2 && st == QMIV_OK; round++)
-      for (size_t k = 1; k < sw_cands.size() && st == QMIV_OK; k++) st = time_one(best_br, sw_cands[k], sw_ms[k]);
-    for (size_t k = 0; k < sw_cands.size(); k++)
-      if (sw_ms[k] < best_ms) {
-        best_ms = sw_ms[k];
-        best_sw = sw_cands[k];
+    if (sw_cands.size() > 1 && st == QMIV_OK) {
+      std::vector<std::vector<double>> sw_t(sw_cands.size());
+      for (int round = 0; round < kRounds && st == QMIV_OK; round++)
+        for (size_t k = 0; k < sw_cands.size() && st == QMIV_OK; k++) st = time_one(best_br, sw_cands[k], sw_t[k]);
+      if (st == QMIV_OK) {
+        const double rule = median(sw_t[0]);
+        best_ms = rule;
+        best_sw = sw_cands[0];
+        for (size_t k = 1; k < sw_cands.size(); k++) {
+          const double m = median(sw_t[k]);
+          if (m < best_ms && m < rule / 1.02) {
+            best_ms = m;
+            best_sw = sw_cands[k];
+          }
+        }
       }
+    }
+    if (st == QMIV_OK) ctx->batch_tuned.emplace_back(sig, std::make_pair(best_br, best_sw));
   }
   if (st == QMIV_OK) st = batch_build(b, best_br, best_sw);
   if (st != QMIV_OK) {

@@ -60,8 +60,8 @@ def test_batch_equals_eager_over_replays(engine, branches, sweep):
     assert info["views"] == len(VIEWS)
     if branches:
         assert info["branches"] == min(branches, len(VIEWS)) and info["candidates_tried"] == 0
-    else:
-        assert info["candidates_tried"] >= 4 and info["tuned_ms"] > 0
+    elif not info["from_cache"]:
+        assert info["tune_rounds"] == 5 and info["candidates_tried"] >= 5 * 4 and info["tuned_ms"] > 0
     want = eager(engine, pics)
     for rep in range(3):
         got = bt.run()
@@ -101,3 +101,16 @@ def test_batch_rejects_bad_arguments(engine):
         engine.batch([(a, b)], [IVPSNR])
     with pytest.raises(Exception):
         engine.batch([(a, a)], [IVPSNR], branches=17)
+
+
+def test_tuned_choice_is_remembered(engine):
+    """A second autotuned batch of the same signature takes the first one's choice without timing."""
+    pics, _ = make_views(engine, 300)
+    prms = [default_params(erp=v[4]) for v in VIEWS]
+    a = engine.batch(pics, [v[5] for v in VIEWS], prms).info()
+    b = engine.batch(pics, [v[5] for v in VIEWS], prms).info()
+    assert b["from_cache"] == 1 and b["candidates_tried"] == 0
+    assert (a["branches"], a["cps_sweep"]) == (b["branches"], b["cps_sweep"])
+    other = [default_params(erp=not v[4]) for v in VIEWS]  # another signature is tuned anew
+    c = engine.batch(pics, [v[5] for v in VIEWS], other).info()
+    assert c["from_cache"] == 0 and c["candidates_tried"] > 0
