@@ -7,21 +7,16 @@
 namespace qmiv {
 namespace {
 
-// Tile shape: each lane owns kSsimC adjacent output columns x kSsimK output rows; 4 waves stack
-// their rows. A loaded window sample feeds up to kSsimC x kSsimK pels.
-#ifndef QMIV_SSIM_COLS
-#define QMIV_SSIM_COLS 1
-#endif
+// Tile shape: each lane owns one output column x kSsimK output rows; 4 waves stack their rows. A
+// loaded window sample feeds up to kSsimK pels.
 #ifndef QMIV_SSIM_ROWS
 #define QMIV_SSIM_ROWS 4
 #endif
-constexpr int kSsimC = QMIV_SSIM_COLS;
 constexpr int kSsimK = QMIV_SSIM_ROWS;
-constexpr int kSsimTW = 64 * kSsimC;                 // output columns per workgroup
+constexpr int kSsimTW = 64;                          // output columns per workgroup
 constexpr int kSsimTH = 4 * kSsimK;                  // output rows per workgroup
 constexpr int kSsimLW = (kSsimTW + 9 + 1) & ~1;      // window columns dx in [-5, 4], even row stride
 constexpr int kSsimLH = kSsimTH + 9;
-static_assert(kSsimC == 1 || kSsimC == 2, "one or two columns per lane");
 
 constexpr int32_t kBufDword3 = 0x00020000;
 
@@ -45,16 +40,9 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
   return v;
 }
 
-// The window is walked as steps (R, c): loaded window row R = 0 .. K + 8 of the lane's K x C output
-// pels, sample column c = 0 .. 8 + C relative to the lane's first window column. Sample column c
-// feeds output column j with window column dxi = c - j and output row k with window row dy = R - k,
-// weight F[dy][dxi]; k runs over max(0, R - 9) .. min(K - 1, R).
-constexpr int kSsimSW = 9 + kSsimC;               // sample columns per loaded row
-constexpr int kSsimSteps = (kSsimK + 9) * kSsimSW;
-
 // The window F is symmetric: F[a][b] = F[b][a], and F[5 + d][b] = F[5 - d][b] (a Gaussian of the
 // offsets -5 .. 4). Its 100 taps hold 21 distinct weights, kept in scalar registers for the whole
-// kernel: tap (dy, dx) uses weight sym_id(dy, dx). launch_ssim checks the symmetry of the table.
+// kernel: tap (dy, dx) uses weight sym_id(dy, dx). The launchers check the symmetry of the table.
 constexpr int kSsimSym = 21;
 __host__ __device__ constexpr int sym_off(int i) { return i < 5 ? 5 - i : i - 5; }  // |offset| of tap index i
 __host__ __device__ constexpr int sym_id(int dy, int dx) {
@@ -69,52 +57,65 @@ __host__ __device__ constexpr int sym_rep(int u) {  // a tap index (dy * 10 + dx
   return (5 - a) * 10 + (5 - b);
 }
 
-// One step's operands: the two samples.
+// The window is walked as steps (R, u): loaded window row R = 0 .. K + 8 of the lane's K output
+// rows and column group u of the window: u = 0 the column of offset 0 (window column 5), u = 1 .. 4
+// the two columns of offsets -u and +u (window columns 5 - u and 5 + u, one weight for both), u = 5
+// the column of offset -5 (window column 0). Window row R feeds output row k with window row
+// dy = R - k, k over max(0, R - 9) .. min(K - 1, R). A pair's two samples are added first, their
+// squares and cross products with one multiply and one fused multiply-add: samples and products are
+// integers below 2^53, so these sums are exact and each moment is the reference's sum of weighted
+// terms (xStructSim.cpp:15-60) regrouped by equal weight, a rounding-level change.
+constexpr int kSsimGroups = 6;
+__host__ __device__ constexpr bool grp_pair(int u) { return u >= 1 && u <= 4; }
+constexpr int kSsimSteps = (kSsimK + 9) * kSsimGroups;
+
+// One step's operands: the samples of the group's one or two columns.
 struct SsimStep {
-  float t, r;
+  float t[2], r[2];
 };
 
 template <int S>
 __device__ __forceinline__ void ssim_load(const float* s_t, const float* s_r, int base, SsimStep& d) {
-  constexpr int R = S / kSsimSW, c = S % kSsimSW;
-  d.t = s_t[base + R * kSsimLW + c];
-  d.r = s_r[base + R * kSsimLW + c];
+  constexpr int R = S / kSsimGroups, u = S % kSsimGroups;
+  const int o = base + R * kSsimLW;
+  d.t[0] = s_t[o + 5 - u];
+  d.r[0] = s_r[o + 5 - u];
+  if constexpr (grp_pair(u)) {
+    d.t[1] = s_t[o + 5 + u];
+    d.r[1] = s_r[o + 5 + u];
+  }
 }
 
 template <int S>
-__device__ __forceinline__ void ssim_fma(const SsimStep& d, const double (&wv)[kSsimSym],
-                                         double (&acc)[kSsimC][kSsimK][5]) {
-  constexpr int R = S / kSsimSW, c = S % kSsimSW;
+__device__ __forceinline__ void ssim_fma(const SsimStep& d, const double (&wv)[kSsimSym], double (&acc)[kSsimK][5]) {
+  constexpr int R = S / kSsimGroups, u = S % kSsimGroups;
   constexpr int k_lo = R > 9 ? R - 9 : 0;
   constexpr int k_hi = R < kSsimK - 1 ? R : kSsimK - 1;
-  const double tv = (double)d.t, rv = (double)d.r;
-  const double rr = rv * rv, tt = tv * tv, rt = rv * tv;
+  double tv = (double)d.t[0], rv = (double)d.r[0];
+  double rr = rv * rv, tt = tv * tv, rt = rv * tv;
+  if constexpr (grp_pair(u)) {
+    const double tb = (double)d.t[1], rb = (double)d.r[1];
+    rr = __builtin_fma(rb, rb, rr);
+    tt = __builtin_fma(tb, tb, tt);
+    rt = __builtin_fma(rb, tb, rt);
+    rv += rb;
+    tv += tb;
+  }
 #pragma unroll
-  for (int j = 0; j < kSsimC; j++) {
-    const int dxi = c - j;
-    if (dxi < 0 || dxi > 9) continue;
-#pragma unroll
-    for (int k = k_lo; k <= k_hi; k++) {
-      const double w = wv[sym_id(R - k, dxi)];  // F[R - k][dxi]
-      acc[j][k][0] = __builtin_fma(rv, w, acc[j][k][0]);
-      acc[j][k][1] = __builtin_fma(tv, w, acc[j][k][1]);
-      acc[j][k][2] = __builtin_fma(rr, w, acc[j][k][2]);
-      acc[j][k][3] = __builtin_fma(tt, w, acc[j][k][3]);
-      acc[j][k][4] = __builtin_fma(rt, w, acc[j][k][4]);
-    }
+  for (int k = k_lo; k <= k_hi; k++) {
+    const double w = wv[sym_id(R - k, 5 - u)];  // F[R - k][5 - u] = F[R - k][5 + u]
+    acc[k][0] = __builtin_fma(rv, w, acc[k][0]);
+    acc[k][1] = __builtin_fma(tv, w, acc[k][1]);
+    acc[k][2] = __builtin_fma(rr, w, acc[k][2]);
+    acc[k][3] = __builtin_fma(tt, w, acc[k][3]);
+    acc[k][4] = __builtin_fma(rt, w, acc[k][4]);
   }
   // Passing the step's accumulators through an ordered no-op pins its FMAs in place: the body is
   // one basic block, and without these the instruction DAG would hoist every load and product above
   // every FMA (the memory clobber keeps the next loads below).
 #pragma unroll
-  for (int j = 0; j < kSsimC; j++) {
-    const int dxi = c - j;
-    if (dxi < 0 || dxi > 9) continue;
-#pragma unroll
-    for (int k = k_lo; k <= k_hi; k++)
-      asm volatile("" : "+v"(acc[j][k][0]), "+v"(acc[j][k][1]), "+v"(acc[j][k][2]), "+v"(acc[j][k][3]),
-                   "+v"(acc[j][k][4])::"memory");
-  }
+  for (int k = k_lo; k <= k_hi; k++)
+    asm volatile("" : "+v"(acc[k][0]), "+v"(acc[k][1]), "+v"(acc[k][2]), "+v"(acc[k][3]), "+v"(acc[k][4])::"memory");
   __builtin_amdgcn_sched_barrier(0);
 }
 
@@ -122,7 +123,7 @@ __device__ __forceinline__ void ssim_fma(const SsimStep& d, const double (&wv)[k
 // of step S, so the LDS latency of every step overlaps the previous step's arithmetic.
 template <int S>
 __device__ __forceinline__ void ssim_steps(const float* s_t, const float* s_r, int base, const double (&wv)[kSsimSym],
-                                           double (&acc)[kSsimC][kSsimK][5], const SsimStep& cur) {
+                                           double (&acc)[kSsimK][5], const SsimStep& cur) {
   if constexpr (S + 1 < kSsimSteps) {
     SsimStep nxt;
     ssim_load<S + 1>(s_t, s_r, base, nxt);
@@ -171,18 +172,16 @@ __device__ __forceinline__ void ssim_tile(const PlaneView& tst, const PlaneView&
   }
   __syncthreads();
 
-  double acc[kSsimC][kSsimK][5];
+  double acc[kSsimK][5];
 #pragma unroll
-  for (int j = 0; j < kSsimC; j++)
+  for (int k = 0; k < kSsimK; k++)
 #pragma unroll
-    for (int k = 0; k < kSsimK; k++)
-#pragma unroll
-      for (int m = 0; m < 5; m++) acc[j][k][m] = 0.0;
+    for (int m = 0; m < 5; m++) acc[k][m] = 0.0;
 
   // Every step as its own compile-time instance (the loop unroller gives up on this many
   // instructions): straight-line code, constant LDS offsets, no branches.
   {
-    const int base = (kSsimK * wave) * kSsimLW + kSsimC * lane;
+    const int base = (kSsimK * wave) * kSsimLW + lane;
     SsimStep first;
     ssim_load<0>(s_t, s_r, base, first);
     ssim_steps<0>(s_t, s_r, base, wv, acc, first);
@@ -192,21 +191,18 @@ __device__ __forceinline__ void ssim_tile(const PlaneView& tst, const PlaneView&
   for (int k = 0; k < kSsimK; k++) {
     const int y = y0 + kSsimK * wave + k;
     double v = 0.0;
-#pragma unroll
-    for (int j = 0; j < kSsimC; j++) {
-      const int x = x0 + kSsimC * lane + j;
-      if (x < w - 5 && y < h - 5) {
-        const double aR = acc[j][k][0], aT = acc[j][k][1];
-        const double vR = acc[j][k][2] - aR * aR;
-        const double vT = acc[j][k][3] - aT * aT;
-        const double cov = acc[j][k][4] - aR * aT;
-        const double cs = (2 * cov + c2) / (vR + vT + c2);
-        if (calc_l) {
-          const double l = (2 * aR * aT + c1) / (aR * aR + aT * aT + c1);
-          v += l * cs;
-        } else {
-          v += cs;
-        }
+    const int x = x0 + lane;
+    if (x < w - 5 && y < h - 5) {
+      const double aR = acc[k][0], aT = acc[k][1];
+      const double vR = acc[k][2] - aR * aR;
+      const double vT = acc[k][3] - aT * aT;
+      const double cov = acc[k][4] - aR * aT;
+      const double cs = (2 * cov + c2) / (vR + vT + c2);
+      if (calc_l) {
+        const double l = (2 * aR * aT + c1) / (aR * aR + aT * aT + c1);
+        v = l * cs;
+      } else {
+        v = cs;
       }
     }
     v = wave_sum_f64(v);
@@ -249,23 +245,30 @@ template <int KM>
 struct MultiTile {
   static constexpr int TH = 4 * KM;
   static constexpr int LH = TH + 9;
-  static constexpr int Steps = (KM + 9) * 10;
+  static constexpr int Steps = (KM + 9) * kSsimGroups;
 };
 
+// The samples of one window column of each staged picture: a step's first column is loaded a step
+// ahead, a pair's second column at the start of its own step (behind the first column's products).
 template <bool HAS_C, bool HAS_D>
 struct MultiStep {
   float p0, p1, sc, sd;
 };
 
-template <int KM, bool HAS_C, bool HAS_D, int S>
-__device__ __forceinline__ void multi_load(const float* s0, const float* s1, const float* sc, const float* sd,
-                                           int base, MultiStep<HAS_C, HAS_D>& d) {
-  constexpr int R = S / 10, c = S % 10;
-  const int o = base + R * kSsimLW + c;
+template <bool HAS_C, bool HAS_D>
+__device__ __forceinline__ void multi_col(const float* s0, const float* s1, const float* sc, const float* sd, int o,
+                                          MultiStep<HAS_C, HAS_D>& d) {
   d.p0 = s0[o];
   d.p1 = s1[o];
   if (HAS_C) d.sc = sc[o];
   if (HAS_D) d.sd = sd[o];
+}
+
+template <int KM, bool HAS_C, bool HAS_D, int S>
+__device__ __forceinline__ void multi_load(const float* s0, const float* s1, const float* sc, const float* sd,
+                                           int base, MultiStep<HAS_C, HAS_D>& d) {
+  constexpr int R = S / kSsimGroups, u = S % kSsimGroups;
+  multi_col<HAS_C, HAS_D>(s0, s1, sc, sd, base + R * kSsimLW + 5 - u, d);
 }
 
 // Moment slots per output row k: 0 Σw·P0, 1 Σw·P1, 2 Σw·P0², 3 Σw·P1², 4 Σw·P0·P1, then for SCP1
@@ -274,43 +277,49 @@ template <bool HAS_C, bool HAS_D>
 constexpr int multi_moments() { return 5 + (HAS_C ? 3 : 0) + (HAS_D ? 3 : 0); }
 
 template <int KM, bool HAS_C, bool HAS_D, int S>
-__device__ __forceinline__ void multi_fma(const MultiStep<HAS_C, HAS_D>& d, const double (&wv)[kSsimSym],
+__device__ __forceinline__ void multi_fma(const float* s0, const float* s1, const float* sc, const float* sd, int base,
+                                          const MultiStep<HAS_C, HAS_D>& d, const double (&wv)[kSsimSym],
                                           double (&acc)[KM][multi_moments<HAS_C, HAS_D>()]) {
-  constexpr int R = S / 10, c = S % 10;
+  constexpr int R = S / kSsimGroups, u = S % kSsimGroups;
   constexpr int k_lo = R > 9 ? R - 9 : 0;
   constexpr int k_hi = R < KM - 1 ? R : KM - 1;
   constexpr int oD = HAS_C ? 8 : 5;
-  const double v0 = (double)d.p0, v1 = (double)d.p1;
-  const double q00 = v0 * v0, q11 = v1 * v1, q10 = v1 * v0;  // k_ssim's rr, tt, rt as (ref, tst) = (P1, P0)
-  double vc = 0, qcc = 0, qc0 = 0, vd = 0, qdd = 0, qd1 = 0;
+  MultiStep<HAS_C, HAS_D> e;
+  if constexpr (grp_pair(u)) multi_col<HAS_C, HAS_D>(s0, s1, sc, sd, base + R * kSsimLW + 5 + u, e);
+  // The column values (and, for a pair, the second column's), then each moment's value formed just
+  // before its fused multiply-adds, so that only the column values stay live across the step.
+  const double a0 = (double)d.p0, a1 = (double)d.p1;
+  const double ac = HAS_C ? (double)d.sc : 0.0, ad = HAS_D ? (double)d.sd : 0.0;
+  double b0 = 0, b1 = 0, bc = 0, bd = 0;
+  if constexpr (grp_pair(u)) {
+    b0 = (double)e.p0;
+    b1 = (double)e.p1;
+    if (HAS_C) bc = (double)e.sc;
+    if (HAS_D) bd = (double)e.sd;
+  }
+  constexpr bool P = grp_pair(u);
+  // Sample sum and product sum of the group: x + x' and x y + x' y' (exact).
+  auto sum1 = [&](double x, double xp) { return P ? x + xp : x; };
+  auto sum2 = [&](double x, double y, double xp, double yp) { return P ? __builtin_fma(xp, yp, x * y) : x * y; };
+  auto put = [&](int m, double v) {
+#pragma unroll
+    for (int k = k_lo; k <= k_hi; k++) acc[k][m] = __builtin_fma(v, wv[sym_id(R - k, 5 - u)], acc[k][m]);
+  };
+  // k_ssim's rr, tt, rt as (ref, tst) = (P1, P0); (SCP1, P0); (SCP0, P1)
+  put(0, sum1(a0, b0));
+  put(1, sum1(a1, b1));
+  put(2, sum2(a0, a0, b0, b0));
+  put(3, sum2(a1, a1, b1, b1));
+  put(4, sum2(a1, a0, b1, b0));
   if (HAS_C) {
-    vc = (double)d.sc;
-    qcc = vc * vc;
-    qc0 = vc * v0;  // (ref, tst) = (SCP1, P0)
+    put(5, sum1(ac, bc));
+    put(6, sum2(ac, ac, bc, bc));
+    put(7, sum2(ac, a0, bc, b0));
   }
   if (HAS_D) {
-    vd = (double)d.sd;
-    qdd = vd * vd;
-    qd1 = vd * v1;  // (ref, tst) = (SCP0, P1)
-  }
-#pragma unroll
-  for (int k = k_lo; k <= k_hi; k++) {
-    const double w = wv[sym_id(R - k, c)];
-    acc[k][0] = __builtin_fma(v0, w, acc[k][0]);
-    acc[k][1] = __builtin_fma(v1, w, acc[k][1]);
-    acc[k][2] = __builtin_fma(q00, w, acc[k][2]);
-    acc[k][3] = __builtin_fma(q11, w, acc[k][3]);
-    acc[k][4] = __builtin_fma(q10, w, acc[k][4]);
-    if (HAS_C) {
-      acc[k][5] = __builtin_fma(vc, w, acc[k][5]);
-      acc[k][6] = __builtin_fma(qcc, w, acc[k][6]);
-      acc[k][7] = __builtin_fma(qc0, w, acc[k][7]);
-    }
-    if (HAS_D) {
-      acc[k][oD] = __builtin_fma(vd, w, acc[k][oD]);
-      acc[k][oD + 1] = __builtin_fma(qdd, w, acc[k][oD + 1]);
-      acc[k][oD + 2] = __builtin_fma(qd1, w, acc[k][oD + 2]);
-    }
+    put(oD, sum1(ad, bd));
+    put(oD + 1, sum2(ad, ad, bd, bd));
+    put(oD + 2, sum2(ad, a1, bd, b1));
   }
 #pragma unroll
   for (int k = k_lo; k <= k_hi; k++)
@@ -327,10 +336,10 @@ __device__ __forceinline__ void multi_steps(const float* s0, const float* s1, co
   if constexpr (S + 1 < MultiTile<KM>::Steps) {
     MultiStep<HAS_C, HAS_D> nxt;
     multi_load<KM, HAS_C, HAS_D, S + 1>(s0, s1, sc, sd, base, nxt);
-    multi_fma<KM, HAS_C, HAS_D, S>(cur, wv, acc);
+    multi_fma<KM, HAS_C, HAS_D, S>(s0, s1, sc, sd, base, cur, wv, acc);
     multi_steps<KM, HAS_C, HAS_D, S + 1>(s0, s1, sc, sd, base, wv, acc, nxt);
   } else {
-    multi_fma<KM, HAS_C, HAS_D, S>(cur, wv, acc);
+    multi_fma<KM, HAS_C, HAS_D, S>(s0, s1, sc, sd, base, cur, wv, acc);
   }
 }
 
