@@ -34,6 +34,14 @@ struct ViewBuf {
   }
 };
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b to XCD b mod 8). Renumbered by this
+// bijection of [0, n), XCD x takes the contiguous run [x q + min(x, r), ...) of q + (x < r) tiles
+// (q = n / 8, r = n mod 8), so neighbouring tiles, whose window halos overlap, share one L2.
+__device__ __forceinline__ int xcd_order(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7, j = b >> 3;
+  return x * q + min(x, r) + j;
+}
+
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -214,7 +222,9 @@ template <int CALC_L>
 __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
   __shared__ float s_t[kSsimLH * kSsimLW];
   __shared__ float s_r[kSsimLH * kSsimLW];
-  ssim_tile(a.tst, a.ref, a.w, a.h, CALC_L != 0, a.c1, a.c2, a.f, a.partials, blockIdx.x, blockIdx.y, s_t, s_r);
+  const int t = xcd_order(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const int ty = t / gridDim.x, tx = t - ty * gridDim.x;
+  ssim_tile(a.tst, a.ref, a.w, a.h, CALC_L != 0, a.c1, a.c2, a.f, a.partials, tx, ty, s_t, s_r);
 }
 
 // Many small evaluations (the MS-SSIM pyramid levels of all components) in one launch: the
@@ -222,7 +232,7 @@ __global__ __launch_bounds__(256) void k_ssim(SsimArgs a) {
 __global__ __launch_bounds__(256) void k_ssim_batch(SsimBatchArgs a) {
   __shared__ float s_t[kSsimLH * kSsimLW];
   __shared__ float s_r[kSsimLH * kSsimLW];
-  const int t = blockIdx.x;
+  const int t = xcd_order(blockIdx.x, gridDim.x);
   int j = 0;
   while (j + 1 < a.n && t >= a.job[j + 1].tile0) j++;
   const SsimBatchJob& J = a.job[j];
@@ -366,8 +376,9 @@ __global__ __launch_bounds__(256) void k_ssim_multi(SsimMultiArgs a) {
   __shared__ float sd[HAS_D ? Tile::LH * kSsimLW : 1];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int tx = blockIdx.x;
-  const int ty = blockIdx.y;
+  const int t = xcd_order(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const int ty = t / gridDim.x;
+  const int tx = t - ty * gridDim.x;
   const int x0 = 5 + tx * kSsimTW;
   const int y0 = 5 + ty * Tile::TH;
   const ViewBuf P0(a.p0), P1(a.p1), SC(a.scp1), SD(a.scp0);
