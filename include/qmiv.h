@@ -464,9 +464,9 @@ typedef struct qmiv_batch_options {
   int32_t cps_sweep;  /* cap on CPS tiles per workgroup; 0: occupancy rule; -1: autotuned over {rule, 1, 2, 4} */
   int32_t tune_reps;  /* timed replays per autotuner candidate and round (default 3); the fastest counts */
 } qmiv_batch_options;
-/* The autotuner times every candidate in 5 interleaved rounds and ranks candidates by the median
-   of their rounds; a candidate with more branches (or the occupancy rule for the sweep) is kept
-   unless another one is more than 2% faster. Its choice is remembered per context and batch
+/* The autotuner times every candidate in 5 interleaved rounds and ranks candidates by the mean of
+   their 3 fastest rounds; a candidate with more branches (or the occupancy rule for the sweep) is
+   kept unless another one is more than 3% faster. Its choice is remembered per context and batch
    signature (the views' sizes, formats, metrics and parameters): a later qmiv_batch_create with the
    same signature and autotuned options takes it without timing again (from_cache = 1). */
 
@@ -475,7 +475,7 @@ typedef struct qmiv_batch_info {
   int32_t branches;          /* chosen branch count */
   int32_t cps_sweep;         /* chosen sweep cap (0 = occupancy rule) */
   int32_t candidates_tried;  /* autotuner candidate timings (rounds included) */
-  double tuned_ms;           /* median replay time of the chosen graph while tuning (0 when not tuned) */
+  double tuned_ms;           /* replay time of the chosen graph while tuning, mean of its 3 fastest rounds (0: not tuned) */
   int32_t tune_rounds;       /* rounds per candidate (0 when not tuned) */
   int32_t from_cache;        /* 1: the choice came from an earlier batch of the same signature */
 } qmiv_batch_info;

@@ -2149,12 +2149,16 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
       }
   if (tune) {
     // Every candidate is timed in kRounds interleaved rounds (so that clock ramps and other work
-    // spread over all of them) and ranked by the median of its rounds.
+    // spread over all of them) and ranked by the mean of its three fastest rounds: a round that
+    // shares the device with something else only runs slower, so the slowest rounds are dropped.
     constexpr int kRounds = 5;
     b->info.tune_rounds = kRounds;
-    auto median = [](std::vector<double> v) {
+    auto fastest3 = [](std::vector<double> v) {
       std::sort(v.begin(), v.end());
-      return v[v.size() / 2];
+      const size_t k = std::min<size_t>(3, v.size());
+      double t = 0.0;
+      for (size_t i = 0; i < k; i++) t += v[i];
+      return t / (double)k;
     };
     auto time_one = [&](int32_t br, int32_t sw, std::vector<double>& samples) -> qmiv_status {
       QMIV_TRY(batch_build(b, br, sw));
@@ -2169,10 +2173,10 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
       for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) st = time_one(br_cands[i], sw_cands[0], br_t[i]);
     std::vector<double> br_med(br_cands.size(), 1e300);
     double fastest = 1e300;
-    for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) fastest = std::min(fastest, br_med[i] = median(br_t[i]));
-    // More branches unless fewer are more than 2% faster: the choice does not flip on noise.
+    for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) fastest = std::min(fastest, br_med[i] = fastest3(br_t[i]));
+    // More branches unless fewer are more than 3% faster: the choice does not flip on noise.
     for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++)
-      if (br_med[i] <= 1.02 * fastest) {
+      if (br_med[i] <= 1.03 * fastest) {
         best_br = br_cands[i];
         best_ms = br_med[i];
       }
@@ -2181,12 +2185,12 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
       for (int round = 0; round < kRounds && st == QMIV_OK; round++)
         for (size_t k = 0; k < sw_cands.size() && st == QMIV_OK; k++) st = time_one(best_br, sw_cands[k], sw_t[k]);
       if (st == QMIV_OK) {
-        const double rule = median(sw_t[0]);
+        const double rule = fastest3(sw_t[0]);
         best_ms = rule;
         best_sw = sw_cands[0];
         for (size_t k = 1; k < sw_cands.size(); k++) {
-          const double m = median(sw_t[k]);
-          if (m < best_ms && m < rule / 1.02) {
+          const double m = fastest3(sw_t[k]);
+          if (m < best_ms && m < rule / 1.03) {
             best_ms = m;
             best_sw = sw_cands[k];
           }
