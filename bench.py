@@ -43,7 +43,7 @@ WORKLOADS = {
         w=4096, h=2048, bd=10, frames=8, erp=True, ml="PSNR,WSPSNR,IVPSNR", kernel="cps",
         desc="IV-PSNR+WS-PSNR+PSNR, 8 frame pairs 4096x2048 yuv420p10le ERP, B=2, GCD on",
         cpu_stages=("validate", "preproc", "gcd", "psnr", "wspsnr", "ivpsnr"), dtype="int32",
-        traffic_file="k_cps_fast420_4096x2048_10b_erp.txt"),
+        cps_kernel="k_cps420s", traffic_file="k_cps420s_4096x2048_10b_erp.txt"),
     "ssim-1080p10": dict(
         metric="SSIM+MS-SSIM+IV-SSIM Mpix/s @1920x1080 10b 420",
         w=1920, h=1080, bd=10, frames=16, erp=False, ml="SSIM,MSSSIM,IVSSIM", kernel="ssim",
@@ -500,7 +500,7 @@ def main():
         achieved = per_launch / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic(wl["traffic_file"]),
-                "kernel": "k_cps_fast420", "kernel_avg_us": round(k_avg_ms * 1e3, 2),
+                "kernel": wl["cps_kernel"], "kernel_avg_us": round(k_avg_ms * 1e3, 2),
                 # the kernel's launches of one timed step, summed (at most ms_per_step: they run in order)
                 "kernel_us_per_step": round(k_ms / args.steps * 1e3, 1), "bytes_per_launch": per_launch,
                 # The search is bound by int32 VALU issue, not HBM: 25 candidates x 2 directions per

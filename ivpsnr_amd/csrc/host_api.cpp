@@ -247,42 +247,71 @@ bool fast_cps_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) 
   return true;
 }
 
-// Tile of the fast CPS kernel that serves this geometry: the 4:2:0 kernel (two columns per lane)
-// or the general one (one column per lane).
-void cps_tile(bool fast420, int32_t* tw, int32_t* th) {
-  if (fast420) cps_fast420_tile(tw, th);
+// 4:2:0 search with 16-bit key operands (kernels_cps420s.hip): samples below 2^10, so 32 Y fits a
+// signed 16-bit half, and -2 w_c X_c fits one for every |X_c| up to maxv + |gcd| (w_c U fits an
+// unsigned half for the cell constant); widths that are a multiple of its tile width. Holds for the
+// reference's default weights at 8 to 10 bits and the common picture widths (1280, 1920, 2048, 3840,
+// 4096); other widths use the 4:2:0 kernel with 32-bit operands.
+bool cps420s_ok(const Geom& g, const qmiv_params& p, const int32_t* gcd_bound) {
+  int32_t tw, th;
+  cps420s_tile(&tw, &th);
+  if (g.cs != 1 || g.bd > 10 || g.bytes_per_sample > 2 || g.w % tw != 0 || (g.h & 1)) return false;
+  const int64_t maxv = (1 << g.bd) - 1;
+  if (32 * maxv > 32767) return false;
+  for (int c = 0; c < 3; c++) {
+    const int64_t m = maxv + std::abs((int64_t)gcd_bound[c]);
+    if (2 * (int64_t)p.cmp_weights_search[c] * m > 32767) return false;
+    if ((int64_t)p.cmp_weights_search[c] * maxv > 65535) return false;
+  }
+  return true;
+}
+
+// Tile of the fast CPS kernel that serves this geometry: the 4:4:4 kernel (one column per lane),
+// the 4:2:0 kernel (two columns per lane) or the 4:2:0 kernel with 16-bit operands (2 x 4 pels per lane).
+void cps_tile(int kind, int32_t* tw, int32_t* th) {
+  if (kind == kCps420s) cps420s_tile(tw, th);
+  else if (kind == kCps420) cps_fast420_tile(tw, th);
   else cps_fast_tile(tw, th);
 }
 
 // Tiles per workgroup along x: just enough that every workgroup is resident at once
 // (CUs x resident workgroups per CU), which gives one round with no tail, and within the bound
-// that keeps the per-lane row sums (one pel per tile and lane column) below 2^32.
-int32_t pick_sweep(qmiv_ctx* ctx, const Geom& g, bool fast420, const int32_t* gcd_bound) {
+// that keeps the per-lane row sums below 2^32: one pel per tile and lane column for the 4:4:4
+// kernel, two for the 4:2:0 kernel, and for the 16-bit 4:2:0 kernel the wave totals too
+// (128 pels per tile and row).
+int32_t pick_sweep(qmiv_ctx* ctx, const Geom& g, int kind, const int32_t* gcd_bound) {
   int32_t tw, th;
-  cps_tile(fast420, &tw, &th);
+  cps_tile(kind, &tw, &th);
   const int32_t bands = (g.h + th - 1) / th;
   const int32_t tiles_x = (g.w + tw - 1) / tw;
   const int64_t tiles = (int64_t)bands * tiles_x;
-  int32_t& bpc = ctx->cps_blocks_per_cu[fast420 ? 1 : 0][g.bytes_per_sample == 1 ? 0 : 1];
-  if (bpc == 0) bpc = fast420 ? cps_fast420_blocks_per_cu(g.bytes_per_sample) : cps_fast_blocks_per_cu(g.bytes_per_sample);
-  const int64_t resident = std::max<int64_t>(1, (int64_t)ctx->num_cus * bpc);
+  int32_t& bpc = ctx->cps_blocks_per_cu[kind][g.bytes_per_sample == 1 ? 0 : 1];
+  if (bpc == 0) {
+    bpc = kind == kCps420s ? cps420s_blocks_per_cu(g.bytes_per_sample)
+          : kind == kCps420 ? cps_fast420_blocks_per_cu(g.bytes_per_sample)
+                            : cps_fast_blocks_per_cu(g.bytes_per_sample);
+  }
+  // The 16-bit kernel runs each (band, segment) as two workgroups, one per direction.
+  const int64_t resident = std::max<int64_t>(1, (int64_t)ctx->num_cus * bpc / (kind == kCps420s ? 2 : 1));
   int64_t sweep = (tiles + resident - 1) / resident;
   const int64_t maxv = (1 << g.bd) - 1;
   int64_t m = 0;
   for (int c = 0; c < 3; c++) m = std::max<int64_t>(m, maxv + std::abs((int64_t)gcd_bound[c]));
-  const int64_t sq = (m + maxv) * (m + maxv) * (fast420 ? 2 : 1);
+  // Largest squared error of one pel: |X - Y| <= maxv + |gcd| for the 16-bit kernel's in-range
+  // samples; the other kernels keep their wider bound.
+  const int64_t sq = kind == kCps420s ? m * m * 128 : (m + maxv) * (m + maxv) * (kind == kCps420 ? 2 : 1);
   sweep = std::min<int64_t>(sweep, std::max<int64_t>(1, (int64_t)0xffffffffLL / std::max<int64_t>(sq, 1)));
   return (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles_x, sweep));
 }
 
 // conceal: a second summary (concealment); fallback: partials of the general search beside the
 // fast one's (a frame flow that must search frames with out-of-range pels).
-Layout make_layout(const Geom& g, bool fast, bool fast420, int32_t sweep, uint32_t ssim_metrics, bool conceal = false,
+Layout make_layout(const Geom& g, bool fast, int kind, int32_t sweep, uint32_t ssim_metrics, bool conceal = false,
                    bool fallback = false) {
   Layout L{};
   const int32_t nrows = g.h + 2 * g.ch;
   int32_t tw, th;
-  cps_tile(fast420, &tw, &th);
+  cps_tile(kind, &tw, &th);
   const int32_t tiles_x = (g.w + tw - 1) / tw;
   L.nseg = fast ? (tiles_x + sweep - 1) / sweep : (g.w + kGenericCols - 1) / kGenericCols;
   size_t o = 0;
@@ -409,6 +438,7 @@ struct FramePlan {
   Layout L;
   bool fast = false;     // fast CPS kernel (else the generic one)
   bool fast420 = false;  // ... its 4:2:0 form
+  int kind = kCps444;     // which fast kernel (cps_tile)
   int32_t sweep = 1;
   int32_t md[4];
   bool conceal = false;   // CNCL: clip out-of-range pictures, then the statistics again
@@ -427,12 +457,13 @@ FramePlan plan_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm, const
   const int32_t* gbound = fw.gcd ? fw.gcd : P.md;
   P.fast = fast_cps_ok(g, prm, gbound);
   P.fast420 = P.fast && g.cs == 1;
-  P.sweep = pick_sweep(ctx, g, P.fast420, gbound);
+  P.kind = !P.fast420 ? kCps444 : (cps420s_ok(g, prm, gbound) ? kCps420s : kCps420);
+  P.sweep = pick_sweep(ctx, g, P.kind, gbound);
   if (sweep_override > 0) P.sweep = std::min(P.sweep, sweep_override);
   P.conceal = fw.ipa == QMIV_PEL_CNCL && may_exceed(g);
   P.fallback = (fw.ipa == QMIV_PEL_WARN || fw.ipa == QMIV_PEL_SKIP) && may_exceed(g) && P.fast &&
                (fw.need_cps || fw.need_scp);
-  P.L = make_layout(g, P.fast, P.fast420, P.sweep, fw.ssim_metrics, P.conceal, P.fallback);
+  P.L = make_layout(g, P.fast, P.kind, P.sweep, fw.ssim_metrics, P.conceal, P.fallback);
   return P;
 }
 
@@ -669,7 +700,7 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     for (int c = 0; c < 4; c++) a.ws[c] = prm.cmp_weights_search[c];
     a.B = prm.search_range;
     int32_t tw, th;
-    cps_tile(P.fast420, &tw, &th);
+    cps_tile(P.kind, &tw, &th);
     a.tiles_x = (g.w + tw - 1) / tw;
     a.sweep = P.sweep;
     a.nseg = L.nseg;
@@ -687,7 +718,8 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
     if (!P.fast) a.gate = any_range ? kGateAlways : kGateInRange;
     {
       KernelTimer t(ctx, P.fast ? QMIV_K_CPS : QMIV_K_CPS_GENERAL, st);
-      if (P.fast420) QMIV_HIP(launch_cps_fast420(a, st));
+      if (P.kind == kCps420s) QMIV_HIP(launch_cps420s(a, st));
+      else if (P.fast420) QMIV_HIP(launch_cps_fast420(a, st));
       else if (P.fast) QMIV_HIP(launch_cps_fast(a, st));
       else QMIV_HIP(launch_cps_generic(a, st));
     }
@@ -2593,9 +2625,9 @@ qmiv_status plan_masked(qmiv_ctx* ctx, const Geom& g, const Geom& gm, const qmiv
   if (P.fast) {
     const bool f420 = g.cs == 1;
     int32_t tw, th;
-    cps_tile(f420, &tw, &th);
+    cps_tile(f420 ? kCps420 : kCps444, &tw, &th);
     P.fa.tiles_x = (g.w + tw - 1) / tw;
-    P.fa.sweep = pick_sweep(ctx, g, f420, gbound);
+    P.fa.sweep = pick_sweep(ctx, g, f420 ? kCps420 : kCps444, gbound);
     P.fa.nseg = (P.fa.tiles_x + P.fa.sweep - 1) / P.fa.sweep;
   }
   P.ipa = ipa;

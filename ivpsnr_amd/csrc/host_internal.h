@@ -80,7 +80,8 @@ struct qmiv_ctx {
   hipError_t fixed_err = hipSuccess;
   void* h_fixed = nullptr;
   int32_t num_cus = 256;
-  int32_t cps_blocks_per_cu[2][2] = {{0, 0}, {0, 0}};  // [4:4:4 kernel, 4:2:0 kernel][u8, u16], queried on first use
+  // [4:4:4 kernel, 4:2:0 kernel, 4:2:0 16-bit kernel][u8, u16], queried on first use
+  int32_t cps_blocks_per_cu[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   // The batch autotuner's choice (branches, sweep cap) per batch signature: the views' geometry,
   // metrics and parameters. A later batch of the same signature takes it without tuning again.
   std::vector<std::pair<std::string, std::pair<int32_t, int32_t>>> batch_tuned;

@@ -201,6 +201,13 @@ int cps_fast_blocks_per_cu(int bytes_per_sample);
 hipError_t launch_cps_fast420(const CpsArgs& a, hipStream_t stream);
 void cps_fast420_tile(int32_t* tile_w, int32_t* tile_h);
 int cps_fast420_blocks_per_cu(int bytes_per_sample);
+// The 4:2:0 CPS kernel with 16-bit key operands (kernels_cps420s.hip): samples of at most 10 bits,
+// row sums added into CpsArgs::rows, no mask.
+hipError_t launch_cps420s(const CpsArgs& a, hipStream_t stream);
+void cps420s_tile(int32_t* tile_w, int32_t* tile_h);
+int cps420s_blocks_per_cu(int bytes_per_sample);
+// Fast CPS kernels, as the host plans them.
+constexpr int kCps444 = 0, kCps420 = 1, kCps420s = 2;
 
 // Nearest-neighbour 4:2:0 -> 4:4:4 upsampling (a copy for 4:4:4 sources) of a packed frame into a
 // planar 4:4:4 frame of the same sample type.
