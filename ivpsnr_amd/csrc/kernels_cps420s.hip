@@ -430,6 +430,18 @@ __device__ __forceinline__ void cell_pass(const uint32_t (&A)[4], uint32_t B, co
   }
 }
 
+// Sum over the wave of a 32-bit value, complete in lane 63: an inclusive scan within each row of 16
+// lanes (DPP row shifts, zero shifted in), then the row totals carried by row broadcasts.
+__device__ __forceinline__ uint32_t wave_total_u32(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
 // One candidate row pair (cell row CR of the lane's four) as read from LDS: the L records and packed
 // 32 Y of record rows 4 wave + 2 CR + rr, columns 2 lane + 2 .. 2 lane + 7, and the cells of cell
 // row 2 wave + CR, columns lane + 1 .. lane + 3.
@@ -667,16 +679,20 @@ __global__ __launch_bounds__(kThreads, 4) void k_cps420s(CpsArgs a) {
   // One sum per (component, row) of the workgroup's direction: reduce over the wave's 64 lanes in
   // 32 bits, then add it into the row (integer sums: any order gives the same).
   uint64_t* rows = a.rows + (t2r ? 3 * (int64_t)g.h : 0);
+  uint32_t tot[3][4];
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int y = y0 + 4 * wave + r;
+  for (int r = 0; r < 4; r++)
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      uint32_t v = acc[c][r];
+    for (int c = 0; c < 3; c++) tot[c][r] = wave_total_u32(acc[c][r]);
+  if (lane == 63) {
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
-      if (lane == 0 && y < g.h)
-        atomicAdd(reinterpret_cast<unsigned long long*>(&rows[(int64_t)c * g.h + y]), (unsigned long long)v);
+    for (int r = 0; r < 4; r++) {
+      const int y = y0 + 4 * wave + r;
+      if (y < g.h) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+          atomicAdd(reinterpret_cast<unsigned long long*>(&rows[(int64_t)c * g.h + y]), (unsigned long long)tot[c][r]);
+      }
     }
   }
 }
