@@ -462,22 +462,30 @@ typedef struct qmiv_batch qmiv_batch;
 typedef struct qmiv_batch_options {
   int32_t branches;   /* concurrent graph branches, 1..16; 0: autotuned over {1, 2, 4, 8, 16} */
   int32_t cps_sweep;  /* cap on CPS tiles per workgroup; 0: occupancy rule; -1: autotuned over {rule, 1, 2, 4} */
-  int32_t tune_reps;  /* timed replays per autotuner candidate and round (default 3); the fastest counts */
+  int32_t tune_reps;  /* replays per autotuner candidate and round (default 12), each launched and waited for */
 } qmiv_batch_options;
-/* The autotuner times every candidate in 5 interleaved rounds and ranks candidates by the mean of
-   their 3 fastest rounds; a candidate with more branches (or the occupancy rule for the sweep) is
-   kept unless another one is more than 3% faster. Its choice is remembered per context and batch
-   signature (the views' sizes, formats, metrics and parameters): a later qmiv_batch_create with the
-   same signature and autotuned options takes it without timing again (from_cache = 1). */
+/* The autotuner first replays the first candidate for about 30 ms (device clocks out of idle), then
+   builds and times every candidate once per round in 5 interleaved rounds (tune_reps replays per
+   round, each launched and waited for on the host clock as qmiv_batch_run does; their mean) and
+   ranks candidates by their fastest round. A captured graph's replay time depends on its instance
+   as well as its shape (the runtime lays each instance's branches over the device's hardware
+   queues; instances of one shape replay up to 25% apart), hence the fastest round. A candidate with
+   more branches (or the occupancy rule for the sweep) is kept unless another one is more than 3%
+   faster. The choice is remembered per context and batch signature (the views' sizes, formats,
+   metrics and parameters): a later qmiv_batch_create with the same signature and autotuned options
+   takes it without timing again (from_cache = 1). Tuned or cached, the instance kept is timed and
+   built again while it replays more than 8% slower than the shape's fastest, at most 6 builds. */
 
 typedef struct qmiv_batch_info {
   int32_t views;
   int32_t branches;          /* chosen branch count */
   int32_t cps_sweep;         /* chosen sweep cap (0 = occupancy rule) */
   int32_t candidates_tried;  /* autotuner candidate timings (rounds included) */
-  double tuned_ms;           /* replay time of the chosen graph while tuning, mean of its 3 fastest rounds (0: not tuned) */
+  double tuned_ms;           /* replay time of the chosen shape's fastest instance while tuning (0: shape given) */
   int32_t tune_rounds;       /* rounds per candidate (0 when not tuned) */
   int32_t from_cache;        /* 1: the choice came from an earlier batch of the same signature */
+  int32_t instances;         /* graph instances built by the instance check (0: no check, shape given) */
+  double instance_ms;        /* replay time of the instance kept, as the check measured it */
 } qmiv_batch_info;
 
 void qmiv_batch_options_default(qmiv_batch_options* opt);

@@ -85,7 +85,7 @@ class BatchOptions(C.Structure):
 class BatchInfo(C.Structure):
     _fields_ = [("views", C.c_int32), ("branches", C.c_int32), ("cps_sweep", C.c_int32),
                 ("candidates_tried", C.c_int32), ("tuned_ms", C.c_double), ("tune_rounds", C.c_int32),
-                ("from_cache", C.c_int32)]
+                ("from_cache", C.c_int32), ("instances", C.c_int32), ("instance_ms", C.c_double)]
 
 
 _ctx_p = C.c_void_p

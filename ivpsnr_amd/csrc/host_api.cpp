@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <functional>
@@ -1117,35 +1118,23 @@ qmiv_status batch_build(qmiv_batch* b, int32_t branches, int32_t sweep) {
   return QMIV_OK;
 }
 
-// Average replay time of the current graph over `reps` launches (after one untimed launch).
-// The fastest of `reps` replays (after one untimed), each between its own event pair: the minimum
-// is robust against clock ramps and other work on the device, which a mean is not.
+// Mean time of `reps` replays of the current graph (after one untimed), each launched and waited
+// for on the host clock as qmiv_batch_run does: a replay's cost to a frame-set loop is its launch
+// (enqueueing the graph's nodes on the host) and its run up to the join. Back-to-back replays
+// between device events hide the launch behind the previous replay and rank candidates by a
+// figure that such a loop never sees.
 qmiv_status batch_time(qmiv_batch* b, int32_t reps, double& ms) {
   qmiv_ctx* ctx = b->ctx;
   reps = std::max(reps, 1);
-  std::vector<hipEvent_t> ev((size_t)reps + 1, nullptr);
-  qmiv_status st = QMIV_OK;
-  for (auto& e : ev)
-    if (st == QMIV_OK && hipEventCreate(&e) != hipSuccess) st = fail(QMIV_ERR_HIP, "timing events while tuning");
-  if (st == QMIV_OK && (hipGraphLaunch(b->exec, ctx->stream) != hipSuccess ||
-                        hipEventRecord(ev[0], ctx->stream) != hipSuccess))
-    st = fail(QMIV_ERR_HIP, "graph launch while tuning");
-  for (int32_t r = 0; r < reps && st == QMIV_OK; r++)
-    if (hipGraphLaunch(b->exec, ctx->stream) != hipSuccess || hipEventRecord(ev[(size_t)r + 1], ctx->stream) != hipSuccess)
-      st = fail(QMIV_ERR_HIP, "graph launch while tuning");
-  if (st == QMIV_OK && hipEventSynchronize(ev[(size_t)reps]) != hipSuccess)
-    st = fail(QMIV_ERR_HIP, "timing events while tuning");
-  ms = 1e300;
-  for (int32_t r = 0; r < reps && st == QMIV_OK; r++) {
-    float el = 0.f;
-    if (hipEventElapsedTime(&el, ev[(size_t)r], ev[(size_t)r + 1]) != hipSuccess)
-      st = fail(QMIV_ERR_HIP, "timing events while tuning");
-    ms = std::min(ms, (double)el);
+  QMIV_HIP(hipGraphLaunch(b->exec, ctx->stream));
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int32_t r = 0; r < reps; r++) {
+    QMIV_HIP(hipGraphLaunch(b->exec, ctx->stream));
+    QMIV_HIP(hipStreamSynchronize(ctx->stream));
   }
-  (void)hipStreamSynchronize(ctx->stream);
-  for (auto& e : ev)
-    if (e) (void)hipEventDestroy(e);
-  return st;
+  ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+  return QMIV_OK;
 }
 
 }  // namespace
@@ -2109,7 +2098,7 @@ void qmiv_batch_options_default(qmiv_batch_options* o) {
   if (!o) return;
   o->branches = 0;
   o->cps_sweep = -1;
-  o->tune_reps = 3;
+  o->tune_reps = 12;
 }
 
 qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
@@ -2122,7 +2111,7 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
   if (opt_in) opt = *opt_in;
   if (opt.branches < 0 || opt.branches > kMaxBranches) return fail(QMIV_ERR_INVALID_ARG, "branches must be 0..16");
   if (opt.cps_sweep < -1) return fail(QMIV_ERR_INVALID_ARG, "cps_sweep must be -1, 0 or positive");
-  if (opt.tune_reps <= 0) opt.tune_reps = 3;
+  if (opt.tune_reps <= 0) opt.tune_reps = 12;
   for (int32_t i = 0; i < n; i++) {
     QMIV_TRY(check_pair(p0[i], p1[i]));
     if (p0[i]->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
@@ -2175,71 +2164,97 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
   }
   if (tune)
     for (const auto& e : ctx->batch_tuned)
-      if (e.first == sig) {
-        best_br = e.second.first;
-        best_sw = e.second.second;
+      if (e.sig == sig) {
+        best_br = e.branches;
+        best_sw = e.sweep;
+        best_ms = e.ms;
         b->info.from_cache = 1;
         tune = false;
         break;
       }
+  // A captured graph's replay time depends on its instance as well as its shape: instances of one
+  // shape replay in one of a few distinct times (up to 25% apart for 24 views in 16 branches),
+  // each instance always in the same one, as the runtime lays the instance's branches over the
+  // device's hardware queues. Candidates are therefore ranked by their fastest instance, and the
+  // instance kept at the end is checked against that figure.
   if (tune) {
-    // Every candidate is timed in kRounds interleaved rounds (so that clock ramps and other work
-    // spread over all of them) and ranked by the mean of its three fastest rounds: a round that
-    // shares the device with something else only runs slower, so the slowest rounds are dropped.
+    // Every candidate is built and timed once per round, in kRounds interleaved rounds, and ranked
+    // by its fastest round. The candidates are built in a batch of their own (own streams and
+    // scratch) that ends with the tuning: the batch kept starts from fresh streams.
     constexpr int kRounds = 5;
     b->info.tune_rounds = kRounds;
-    auto fastest3 = [](std::vector<double> v) {
-      std::sort(v.begin(), v.end());
-      const size_t k = std::min<size_t>(3, v.size());
+    std::unique_ptr<qmiv_batch, void (*)(qmiv_batch*)> tb(new qmiv_batch, qmiv_batch_destroy);
+    tb->ctx = ctx;
+    tb->n = n;
+    tb->p0 = b->p0;
+    tb->p1 = b->p1;
+    tb->prm = b->prm;
+    tb->metrics = b->metrics;
+    tb->fw = b->fw;
+    auto time_one = [&](int32_t br, int32_t sw, double& fastest_t) -> qmiv_status {
+      QMIV_TRY(batch_build(tb.get(), br, sw));
       double t = 0.0;
-      for (size_t i = 0; i < k; i++) t += v[i];
-      return t / (double)k;
-    };
-    auto time_one = [&](int32_t br, int32_t sw, std::vector<double>& samples) -> qmiv_status {
-      QMIV_TRY(batch_build(b, br, sw));
-      double t = 0.0;
-      QMIV_TRY(batch_time(b, opt.tune_reps, t));
+      QMIV_TRY(batch_time(tb.get(), opt.tune_reps, t));
       b->info.candidates_tried++;
-      samples.push_back(t);
+      fastest_t = std::min(fastest_t, t);
       return QMIV_OK;
     };
-    std::vector<std::vector<double>> br_t(br_cands.size());
+    // Warm-up: replays of the first candidate until about 30 ms of work have run, so that the
+    // device clocks have left their idle state before the first timed round.
+    st = batch_build(tb.get(), br_cands[0], sw_cands[0]);
+    double warm = 0.0;
+    for (int k = 0; k < 64 && st == QMIV_OK && warm < 30.0; k++) {
+      double t = 0.0;
+      st = batch_time(tb.get(), opt.tune_reps, t);
+      warm += t * opt.tune_reps;
+    }
+    std::vector<double> br_t(br_cands.size(), 1e300);
     for (int round = 0; round < kRounds && st == QMIV_OK; round++)
       for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) st = time_one(br_cands[i], sw_cands[0], br_t[i]);
-    std::vector<double> br_med(br_cands.size(), 1e300);
-    double fastest = 1e300;
-    for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) fastest = std::min(fastest, br_med[i] = fastest3(br_t[i]));
+    const double fastest = *std::min_element(br_t.begin(), br_t.end());
     // More branches unless fewer are more than 3% faster: the choice does not flip on noise.
     for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++)
-      if (br_med[i] <= 1.03 * fastest) {
+      if (br_t[i] <= 1.03 * fastest) {
         best_br = br_cands[i];
-        best_ms = br_med[i];
+        best_ms = br_t[i];
       }
     if (sw_cands.size() > 1 && st == QMIV_OK) {
-      std::vector<std::vector<double>> sw_t(sw_cands.size());
+      std::vector<double> sw_t(sw_cands.size(), 1e300);
       for (int round = 0; round < kRounds && st == QMIV_OK; round++)
         for (size_t k = 0; k < sw_cands.size() && st == QMIV_OK; k++) st = time_one(best_br, sw_cands[k], sw_t[k]);
       if (st == QMIV_OK) {
-        const double rule = fastest3(sw_t[0]);
+        // The occupancy rule unless another cap is more than 3% faster.
+        const double rule = std::min(sw_t[0], best_ms);
         best_ms = rule;
         best_sw = sw_cands[0];
-        for (size_t k = 1; k < sw_cands.size(); k++) {
-          const double m = fastest3(sw_t[k]);
-          if (m < best_ms && m < rule / 1.03) {
-            best_ms = m;
+        for (size_t k = 1; k < sw_cands.size(); k++)
+          if (sw_t[k] < best_ms && sw_t[k] < rule / 1.03) {
+            best_ms = sw_t[k];
             best_sw = sw_cands[k];
           }
-        }
       }
     }
-    if (st == QMIV_OK) ctx->batch_tuned.emplace_back(sig, std::make_pair(best_br, best_sw));
+    if (st == QMIV_OK) ctx->batch_tuned.push_back({sig, best_br, best_sw, best_ms});
   }
   if (st == QMIV_OK) st = batch_build(b, best_br, best_sw);
+  // Instance check (tuned or cached shapes): a new instance while this one replays more than 8%
+  // slower than the shape's fastest, at most kInstances builds in all.
+  if (st == QMIV_OK && best_ms > 0.0) {
+    constexpr int kInstances = 6;
+    for (int k = 1; st == QMIV_OK; k++) {
+      double t = 0.0;
+      st = batch_time(b, opt.tune_reps, t);
+      b->info.instances = k;
+      b->info.instance_ms = t;
+      if (st != QMIV_OK || t <= 1.08 * best_ms || k == kInstances) break;
+      st = batch_build(b, best_br, best_sw);
+    }
+  }
   if (st != QMIV_OK) {
     qmiv_batch_destroy(b);
     return st;
   }
-  b->info.tuned_ms = tune ? best_ms : 0.0;
+  b->info.tuned_ms = best_ms;
   *out = b;
   return QMIV_OK;
 }

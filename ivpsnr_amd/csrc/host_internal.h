@@ -82,9 +82,15 @@ struct qmiv_ctx {
   int32_t num_cus = 256;
   // [4:4:4 kernel, 4:2:0 kernel, 4:2:0 16-bit kernel][u8, u16], queried on first use
   int32_t cps_blocks_per_cu[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-  // The batch autotuner's choice (branches, sweep cap) per batch signature: the views' geometry,
-  // metrics and parameters. A later batch of the same signature takes it without tuning again.
-  std::vector<std::pair<std::string, std::pair<int32_t, int32_t>>> batch_tuned;
+  // The batch autotuner's choice per batch signature (the views' geometry, metrics and parameters):
+  // branches, sweep cap and the replay time of the fastest graph instance of that shape. A later
+  // batch of the same signature takes it without tuning again and checks its instance against it.
+  struct BatchChoice {
+    std::string sig;
+    int32_t branches, sweep;
+    double ms;
+  };
+  std::vector<BatchChoice> batch_tuned;
   // Kernel timing: event pairs recorded around launches since the last harvest.
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;

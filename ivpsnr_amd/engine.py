@@ -276,7 +276,7 @@ class Engine:
         return arr
 
     def batch(self, pairs, metrics, params=None, branches: int = 0, cps_sweep: int = -1,
-              tune_reps: int = 3) -> "Batch":
+              tune_reps: int = 12) -> "Batch":
         """A captured frame set over the (p0, p1) pairs (qmiv_batch_create). `metrics` and
         `params` are one value for every view or one per view."""
         bt = Batch(self, pairs, metrics, params, branches, cps_sweep, tune_reps)

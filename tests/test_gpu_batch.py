@@ -60,8 +60,12 @@ def test_batch_equals_eager_over_replays(engine, branches, sweep):
     assert info["views"] == len(VIEWS)
     if branches:
         assert info["branches"] == min(branches, len(VIEWS)) and info["candidates_tried"] == 0
-    elif not info["from_cache"]:
-        assert info["tune_rounds"] == 5 and info["candidates_tried"] >= 5 * 4 and info["tuned_ms"] > 0
+        assert info["instances"] == 0  # a given shape is not timed
+    else:
+        if not info["from_cache"]:
+            assert info["tune_rounds"] == 5 and info["candidates_tried"] >= 5 * 4
+        # the kept instance was checked against the shape's fastest
+        assert info["tuned_ms"] > 0 and 1 <= info["instances"] <= 6 and info["instance_ms"] > 0
     want = eager(engine, pics)
     for rep in range(3):
         got = bt.run()
@@ -110,7 +114,8 @@ def test_tuned_choice_is_remembered(engine):
     a = engine.batch(pics, [v[5] for v in VIEWS], prms).info()
     b = engine.batch(pics, [v[5] for v in VIEWS], prms).info()
     assert b["from_cache"] == 1 and b["candidates_tried"] == 0
-    assert (a["branches"], a["cps_sweep"]) == (b["branches"], b["cps_sweep"])
+    assert (a["branches"], a["cps_sweep"], a["tuned_ms"]) == (b["branches"], b["cps_sweep"], b["tuned_ms"])
+    assert 1 <= b["instances"] <= 6
     other = [default_params(erp=not v[4]) for v in VIEWS]  # another signature is tuned anew
     c = engine.batch(pics, [v[5] for v in VIEWS], other).info()
     assert c["from_cache"] == 0 and c["candidates_tried"] > 0
