@@ -398,7 +398,7 @@ qmiv_status qmiv_frame_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic*
                               const qmiv_params* prm, uint32_t metrics, qmiv_masked_result* out);
 
 /* The masked flow for n frames (sizes may differ between frames, each with its own mask): every
-   frame's kernels are enqueued back to back (alternating over two streams), then the host
+   frame's kernels are enqueued back to back (dealt round-robin over three streams), then the host
    finalisation of each as its results arrive. Replaces the reference application's frame loop in
    mask mode (apps/QMIV/src/xAppQMIV.cpp:666-748 with the masked calls of :825-889). out[i]
    receives frame i; the first failing frame's status is returned. */

@@ -131,7 +131,7 @@ qmiv_status ensure_scratch(qmiv_ctx* ctx, size_t bytes) {
   if (ctx->scratch_bytes >= bytes) return QMIV_OK;
   if (ctx->d_scratch) {
     QMIV_HIP(hipStreamSynchronize(ctx->stream));
-    QMIV_HIP(hipStreamSynchronize(ctx->aux_stream));
+    for (hipStream_t a : ctx->aux_stream) QMIV_HIP(hipStreamSynchronize(a));
     QMIV_HIP(hipStreamSynchronize(ctx->copy_stream));
     QMIV_HIP(hipFree(ctx->d_scratch));
     ctx->d_scratch = nullptr;
@@ -147,7 +147,7 @@ qmiv_status ensure_results(qmiv_ctx* ctx, size_t bytes) {
   if (ctx->results_bytes >= bytes) return QMIV_OK;
   if (ctx->h_results) {
     QMIV_HIP(hipStreamSynchronize(ctx->stream));
-    QMIV_HIP(hipStreamSynchronize(ctx->aux_stream));
+    for (hipStream_t a : ctx->aux_stream) QMIV_HIP(hipStreamSynchronize(a));
     QMIV_HIP(hipStreamSynchronize(ctx->copy_stream));
     QMIV_HIP(hipHostFree(ctx->h_results));
     ctx->h_results = nullptr;
@@ -178,7 +178,7 @@ qmiv_status ensure_scp(qmiv_ctx* ctx, size_t bytes) {
   if (ctx->scp_bytes >= bytes) return QMIV_OK;
   if (ctx->d_scp) {
     QMIV_HIP(hipStreamSynchronize(ctx->stream));
-    QMIV_HIP(hipStreamSynchronize(ctx->aux_stream));
+    for (hipStream_t a : ctx->aux_stream) QMIV_HIP(hipStreamSynchronize(a));
     QMIV_HIP(hipStreamSynchronize(ctx->copy_stream));
     QMIV_HIP(hipFree(ctx->d_scp));
     ctx->d_scp = nullptr;
@@ -760,7 +760,7 @@ qmiv_status enqueue_frame(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1,
 
 qmiv_status wait_stream(qmiv_ctx* ctx) {
   QMIV_HIP(hipStreamSynchronize(ctx->stream));
-  QMIV_HIP(hipStreamSynchronize(ctx->aux_stream));
+  for (hipStream_t a : ctx->aux_stream) QMIV_HIP(hipStreamSynchronize(a));
   QMIV_HIP(hipStreamSynchronize(ctx->copy_stream));
   harvest_events(ctx);
   return QMIV_OK;
@@ -1293,7 +1293,8 @@ qmiv_status qmiv_create(int32_t device, qmiv_ctx** out) {
     ctx->num_cus = cus;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);
+  for (hipStream_t& a : ctx->aux_stream)
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
   // The allocation-free entry point's buffers: the device's fixed scratch (its address resolved with
@@ -1309,7 +1310,8 @@ qmiv_status qmiv_create(int32_t device, qmiv_ctx** out) {
     if (ctx->h_fixed) (void)hipHostFree(ctx->h_fixed);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
-    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+    for (hipStream_t a : ctx->aux_stream)
+      if (a) (void)hipStreamDestroy(a);
     if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1324,7 +1326,8 @@ void qmiv_destroy(qmiv_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
-  if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
+  for (hipStream_t a : ctx->aux_stream)
+    if (a) (void)hipStreamSynchronize(a);
   for (hipEvent_t ev : ctx->copy_ev) (void)hipEventDestroy(ev);
   for (hipEvent_t ev : ctx->done_ev) (void)hipEventDestroy(ev);
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
@@ -1344,7 +1347,8 @@ void qmiv_destroy(qmiv_ctx* ctx) {
     (void)hipEventDestroy(pr.second);
   }
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
-  if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
+  for (hipStream_t a : ctx->aux_stream)
+    if (a) (void)hipStreamDestroy(a);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1701,9 +1705,9 @@ qmiv_status frames_impl(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
   }
   QMIV_TRY(ensure_scratch(ctx, dtot));
   QMIV_TRY(ensure_results(ctx, htot));
-  // Frames alternate over two streams; each stream's frames run in order, so one set of
-  // shift-compensated pictures per stream serves them.
-  const int32_t nstreams = n > 1 ? 2 : 1;
+  // Frames are dealt round-robin over the context stream and the aux streams; each stream's frames
+  // run in order, so one set of shift-compensated pictures per stream serves them.
+  const int32_t nstreams = std::min<int32_t>(n, 1 + qmiv_ctx::kAuxStreams);
   uint16_t* scp = nullptr;
   size_t scp_stride = 0;
   if (fw.need_scp) {
@@ -1723,10 +1727,10 @@ qmiv_status frames_impl(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
     QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ctx->copy_ev.push_back(e);
   }
-  const hipStream_t streams[2] = {ctx->stream, ctx->aux_stream};
+  const hipStream_t streams[1 + qmiv_ctx::kAuxStreams] = {ctx->stream, ctx->aux_stream[0], ctx->aux_stream[1]};
   if (nstreams > 1 || h0) {  // the aux and copy streams start after everything on the context stream
     QMIV_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
-    if (nstreams > 1) QMIV_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_ev, 0));
+    for (int32_t k = 1; k < nstreams; k++) QMIV_HIP(hipStreamWaitEvent(streams[k], ctx->fork_ev, 0));
     if (h0) QMIV_HIP(hipStreamWaitEvent(ctx->copy_stream, ctx->fork_ev, 0));
   }
   // Each frame is finalised on the host as soon as its results have arrived, while the device
@@ -1757,14 +1761,14 @@ qmiv_status frames_impl(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
     if (st != QMIV_OK) {
       // Drain what was enqueued: scratch and result buffers must be idle before any later reuse.
       (void)hipStreamSynchronize(ctx->stream);
-      (void)hipStreamSynchronize(ctx->aux_stream);
+      for (hipStream_t a : ctx->aux_stream) (void)hipStreamSynchronize(a);
       (void)hipStreamSynchronize(ctx->copy_stream);
       return st;
     }
     while (done < i && hipEventQuery(ctx->done_ev[(size_t)done]) == hipSuccess) finalize_next();
   }
-  if (nstreams > 1) {  // later work on the context stream follows the whole set
-    QMIV_HIP(hipEventRecord(ctx->join_ev, ctx->aux_stream));
+  for (int32_t k = 1; k < nstreams; k++) {  // later work on the context stream follows the whole set
+    QMIV_HIP(hipEventRecord(ctx->join_ev, streams[k]));
     QMIV_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
   }
   while (done < n) {
@@ -2877,12 +2881,13 @@ qmiv_status qmiv_frames_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* 
     QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ctx->done_ev.push_back(e);
   }
-  // Frames alternate over two streams, so one frame's statistics overlap the other's search.
-  const int32_t nstreams = n > 1 ? 2 : 1;
-  const hipStream_t streams[2] = {ctx->stream, ctx->aux_stream};
-  if (nstreams > 1) {  // the aux stream starts after everything already on the context stream
+  // Frames are dealt round-robin over the context stream and the aux streams, so one frame's
+  // statistics overlap the others' searches.
+  const int32_t nstreams = std::min<int32_t>(n, 1 + qmiv_ctx::kAuxStreams);
+  const hipStream_t streams[1 + qmiv_ctx::kAuxStreams] = {ctx->stream, ctx->aux_stream[0], ctx->aux_stream[1]};
+  if (nstreams > 1) {  // the aux streams start after everything already on the context stream
     QMIV_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
-    QMIV_HIP(hipStreamWaitEvent(ctx->aux_stream, ctx->fork_ev, 0));
+    for (int32_t k = 1; k < nstreams; k++) QMIV_HIP(hipStreamWaitEvent(streams[k], ctx->fork_ev, 0));
   }
   // Each frame is finalised on the host as soon as its results have arrived, between enqueues too.
   qmiv_status first_err = QMIV_OK;
@@ -2902,20 +2907,20 @@ qmiv_status qmiv_frames_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* 
     if (e != QMIV_OK) {
       // Drain what was enqueued: scratch and result buffers must be idle before any later reuse.
       (void)hipStreamSynchronize(ctx->stream);
-      (void)hipStreamSynchronize(ctx->aux_stream);
+      for (hipStream_t a : ctx->aux_stream) (void)hipStreamSynchronize(a);
       return e;
     }
     while (done < i && hipEventQuery(ctx->done_ev[(size_t)done]) == hipSuccess) finalize_next();
   }
-  if (nstreams > 1) {  // later work on the context stream follows the whole set
-    QMIV_HIP(hipEventRecord(ctx->join_ev, ctx->aux_stream));
+  for (int32_t k = 1; k < nstreams; k++) {  // later work on the context stream follows the whole set
+    QMIV_HIP(hipEventRecord(ctx->join_ev, streams[k]));
     QMIV_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
   }
   while (done < n) {
     QMIV_HIP(hipEventSynchronize(ctx->done_ev[(size_t)done]));
     finalize_next();
   }
-  QMIV_TRY(wait_stream(ctx));  // both streams idle; the set's kernel timings are harvested
+  QMIV_TRY(wait_stream(ctx));  // every stream idle; the set's kernel timings are harvested
   return first_err;
 }
 

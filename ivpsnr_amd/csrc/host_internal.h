@@ -51,9 +51,13 @@ struct qmiv_ctx {
     double corr;
   };
   std::vector<WsTable> ws_cache;
-  // A frame set alternates its frames over `stream` and `aux_stream` (independent scratch), so one
-  // frame's short kernels overlap the other's search; fork/join events order them with the rest.
-  hipStream_t aux_stream = nullptr;
+  // A frame set deals its frames round-robin over `stream` and the aux streams (independent scratch),
+  // so up to three frames are in flight: one frame's short kernels and the start and end of its
+  // search overlap the others' searches. Three compute streams and the copy stream fit the device's
+  // four hardware queues per process (a fourth compute stream shares a queue and measured slower).
+  // Fork/join events order them with the rest.
+  static constexpr int kAuxStreams = 2;
+  hipStream_t aux_stream[kAuxStreams] = {nullptr, nullptr};
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   // Differentiable SSIM family: float workspace and the quantised pictures of the shift search.
   void* d_fwork = nullptr;
