@@ -312,9 +312,22 @@ def run_masked(args):
         eng.profile(False)
         return el, pr, r
 
+    mpix = nframes * W * H / 1e6
+    # Kernel durations for the roofline come from frames run one at a time (events around launches
+    # inside the frame set also hold the waits for CUs that the other frames occupy).
+    prof_steps = max(1, min(args.steps, 5))
+    if args.legs == "profile":
+        print(f"[bench] masked per-frame profile leg: warmup 1, {prof_steps} steps", file=sys.stderr, flush=True)
+        el_p, prof_alone, _ = timed(prof_steps, 1, True, step_loop)
+        print(json.dumps({"legs": "profile", "steps": prof_steps, "mpix_s_profiled_plain_loop":
+                          round(mpix / (el_p / prof_steps), 2),
+                          "kernels_us_per_frame_alone": {k: round(v[0] / nframes / prof_steps * 1e3, 2)
+                                                         for k, v in prof_alone.items()}}))
+        eng.close()
+        return
     print(f"[bench] masked: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
     elapsed, prof, res = timed(args.steps, args.warmup, True)
-    mpix = nframes * W * H / 1e6
+    prof_in_set = prof
     plain_mpix_s = None
     if args.legs != "stream":
         el2, _, _ = timed(args.steps, 0, False)
@@ -322,9 +335,14 @@ def run_masked(args):
         el_loop, _, res_loop = timed(args.steps, 1, False, step_loop)
         plain_mpix_s = mpix / (el_loop / args.steps)
         assert [r.as_dict() for r in res] == [r.as_dict() for r in res_loop], "frame set differs from the per-frame calls"
+        print(f"[bench] masked per-frame profile leg: warmup 1, {prof_steps} steps", file=sys.stderr, flush=True)
+        _, prof, _ = timed(prof_steps, 1, True, step_loop)
+    else:
+        prof_steps = args.steps
     ms_per_step = elapsed / args.steps * 1e3
     k_ms, k_n = prof.get("cps", (0.0, 0))
     k_avg_ms = k_ms / max(k_n, 1)
+    k_set_ms, k_set_n = prof_in_set.get("cps", (0.0, 0))
     per_launch = frame_pair_bytes(W, H, BD) + W * H  # two pictures and the mask's luma plane
     achieved = per_launch / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
     cpu = None
@@ -363,15 +381,19 @@ def run_masked(args):
         "dtype": "int32", "data": "synthetic",
         "config": {"workload": "masked PSNR+WS-PSNR+IV-PSNR, 8 frame pairs 1920x1080 yuv420p, 8-bit object "
                                "masks, B=2, GCD on", "frames_per_step": nframes,
-                   "mode": "stream (qmiv_frames_masked: frame set enqueued over two streams, one wait)",
+                   "mode": "stream (qmiv_frames_masked: frame set enqueued over three streams, one wait)",
                    "plain_loop_mpix_s": round(plain_mpix_s, 2) if plain_mpix_s else None,
                    "stream_vs_plain": round(mpix / (ms_per_step / 1e3) / plain_mpix_s, 3) if plain_mpix_s else None,
                    "ivpsnr_last_frame": round(res[-1].ivpsnr, 6), "nnm_last_frame": res[-1].num_non_masked},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic("k_cps_fast420_masked_1920x1080_8b.txt"),
                      "kernel": "k_cps_fast420 (masked)", "kernel_avg_us": round(k_avg_ms * 1e3, 2),
+                     "kernel_avg_us_in_set": round(k_set_ms / max(k_set_n, 1) * 1e3, 2),
+                     "timing": "kernel_avg_us: HIP events around each launch with the frame's kernels alone on the "
+                               "device (per-frame leg); kernel_avg_us_in_set: inside the timed frame-set steps",
                      "bytes_per_launch": per_launch},
-        "kernels_us_per_frame": {k: round(v[0] / nframes / args.steps * 1e3, 2) for k, v in prof.items()},
+        "kernels_us_per_frame": {k: round(v[0] / nframes / prof_steps * 1e3, 2) for k, v in prof.items()},
+        "kernels_us_per_frame_in_set": {k: round(v[0] / nframes / args.steps * 1e3, 2) for k, v in prof_in_set.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -390,9 +412,10 @@ def main():
     ap.add_argument("--frames", type=int, default=0, help="frame pairs per step (default: the workload's)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 512 MiB (calibrates rocprof byte counters)")
-    ap.add_argument("--legs", choices=["all", "stream"], default="all",
-                    help="stream: run only the timed frame-set steps (no plain loop, no upload leg), so that "
-                         "a kernel trace of the run holds exactly the launches the roofline figure is taken from")
+    ap.add_argument("--legs", choices=["all", "stream", "profile"], default="all",
+                    help="stream: run only the timed frame-set steps (no plain loop, no upload leg); profile: run "
+                         "only the per-frame leg whose kernel durations the roofline figure is taken from, so that "
+                         "a kernel trace of the run holds exactly those launches")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -454,8 +477,22 @@ def main():
         eng.profile(False)
         return el, pr, r
 
-    only_stream = args.legs == "stream"
     key = "ivpsnr" if "IVPSNR" in wl["ml"] else "ivssim"
+    # Kernel durations for the roofline come from frames run one at a time (the plain loop with
+    # kernel events): in the frame-set steps up to three frames share the device, and an event pair
+    # around a launch there also holds the time the launch waits for CUs the other frames occupy.
+    prof_steps = max(1, min(args.steps, 5))
+    if args.legs == "profile":
+        print(f"[bench] per-frame profile leg: warmup 1, {prof_steps} steps", file=sys.stderr, flush=True)
+        el_p, prof_alone, _ = timed(step_plain, prof_steps, 1, True)
+        mpix = nframes * W * H / 1e6
+        print(json.dumps({"legs": "profile", "steps": prof_steps, "mpix_s_profiled_plain_loop":
+                          round(mpix / (el_p / prof_steps), 2),
+                          "kernels_us_per_frame_alone": {k: round(v[0] / nframes / prof_steps * 1e3, 2)
+                                                         for k, v in prof_alone.items()}}))
+        eng.close()
+        return
+    only_stream = args.legs == "stream"
     el_plain = None
     if not only_stream:
         print(f"[bench] plain loop: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
@@ -490,33 +527,47 @@ def main():
         assert [getattr(r, key) for r in got] == [getattr(r, key) for r in res]
         el_up = timed(lambda: eng.frames_upload(pics, src_pinned, metrics, prm), up_steps, 1, False)[0]
         el_up_pageable = timed(lambda: eng.frames_upload(pics, src_pageable, metrics, prm), up_steps, 1, False)[0]
+    prof_in_set = prof
+    if not only_stream:
+        print(f"[bench] per-frame profile leg: warmup 1, {prof_steps} steps", file=sys.stderr, flush=True)
+        _, prof, _ = timed(step_plain, prof_steps, 1, True)
+    else:
+        prof_steps = args.steps
 
     ms_per_step = elapsed / args.steps * 1e3
     mpix = nframes * W * H / 1e6
     value = mpix / (ms_per_step / 1e3)
     k_ms, k_n = prof.get(wl["kernel"], (0.0, 0))
     k_avg_ms = k_ms / max(k_n, 1)
+    k_set_ms, k_set_n = prof_in_set.get(wl["kernel"], (0.0, 0))
+    timing_note = ("kernel_avg_us: HIP events around each launch with the frame's kernels alone on the device "
+                   "(per-frame leg); kernel_avg_us_in_set: the same launches inside the timed frame-set steps, "
+                   "waits for CUs held by the other frames included") if not only_stream else \
+        "kernel_avg_us: HIP events around each launch inside the timed frame-set steps"
     if wl["kernel"] == "cps":
         per_launch = frame_pair_bytes(W, H, BD)
         achieved = per_launch / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic(wl["traffic_file"]),
                 "kernel": wl["cps_kernel"], "kernel_avg_us": round(k_avg_ms * 1e3, 2),
-                # the kernel's launches of one timed step, summed (at most ms_per_step: they run in order)
-                "kernel_us_per_step": round(k_ms / args.steps * 1e3, 1), "bytes_per_launch": per_launch,
+                "kernel_avg_us_in_set": round(k_set_ms / max(k_set_n, 1) * 1e3, 2), "timing": timing_note,
+                # the kernel's launches of one step, summed (below ms_per_step while the launches of a set
+                # do not overlap each other)
+                "kernel_us_per_step": round(k_ms / prof_steps * 1e3, 1), "bytes_per_launch": per_launch,
                 # The search is bound by int32 VALU issue, not HBM: 25 candidates x 2 directions per
                 # pel, and the VALU busy fraction of the same kernel from its PMC summary.
                 "candidate_evals_per_s": round(50 * W * H / (k_avg_ms / 1e3), -6) if k_avg_ms > 0 else 0.0,
                 "valu_busy_frac": read_profile_value(wl["traffic_file"], "valu_busy_frac")}
     else:
         flops = ssim_flops_per_frame(W, H)
-        frames_prof = args.steps * nframes
+        frames_prof = prof_steps * nframes
         launches = k_n / max(frames_prof, 1)  # SSIM-family launches per frame (fused, batched, single)
         per_launch = flops / max(launches, 1)
         achieved = flops / (k_ms / frames_prof / 1e3) / 1e12 if k_ms > 0 else 0.0
         roof = {"bound": "fp64-valu", "achieved": round(achieved, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": read_traffic(wl["traffic_file"]),
                 "kernel": "k_ssim_multi + k_ssim_batch", "kernel_avg_us": round(k_avg_ms * 1e3, 2),
+                "kernel_avg_us_in_set": round(k_set_ms / max(k_set_n, 1) * 1e3, 2), "timing": timing_note,
                 "launches_per_frame": round(launches, 2), "flops_per_launch": round(per_launch)}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(wl, pairs, 4 if wl["kernel"] == "cps" else 2)
@@ -542,7 +593,8 @@ def main():
                        round(mpix / (el_up_pageable / up_steps), 2) if el_up_pageable else None,
                    key + "_last_frame": round(getattr(res[-1], key), 6)},
         "roofline": roof,
-        "kernels_us_per_frame": {k: round(v[0] / nframes / args.steps * 1e3, 2) for k, v in prof.items()},
+        "kernels_us_per_frame": {k: round(v[0] / nframes / prof_steps * 1e3, 2) for k, v in prof.items()},
+        "kernels_us_per_frame_in_set": {k: round(v[0] / nframes / args.steps * 1e3, 2) for k, v in prof_in_set.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
