@@ -465,7 +465,7 @@ typedef struct qmiv_batch_options {
   int32_t tune_reps;  /* replays per autotuner candidate and round (default 12), each launched and waited for */
 } qmiv_batch_options;
 /* The autotuner first replays the first candidate for about 30 ms (device clocks out of idle), then
-   builds and times every candidate once per round in 5 interleaved rounds (tune_reps replays per
+   builds and times every candidate once per round in 8 interleaved rounds (tune_reps replays per
    round, each launched and waited for on the host clock as qmiv_batch_run does; their mean) and
    ranks candidates by their fastest round. A captured graph's replay time depends on its instance
    as well as its shape (the runtime lays each instance's branches over the device's hardware
@@ -493,8 +493,19 @@ void qmiv_batch_options_default(qmiv_batch_options* opt);
 qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
                               const qmiv_params* prm, const uint32_t* metrics, const qmiv_batch_options* opt,
                               qmiv_batch** out);
-/* One replay of the captured frame set; out receives n results (as qmiv_frame's). */
+/* One replay of the captured frame set; out receives n results (as qmiv_frame's). Equals
+   qmiv_batch_launch followed by qmiv_batch_collect. */
 qmiv_status qmiv_batch_run(qmiv_batch* batch, qmiv_frame_result* out);
+/* The two halves of qmiv_batch_run (no counterpart in the reference; for callers that keep the
+   device busy across frame sets). qmiv_batch_launch enqueues one replay on the context stream and
+   returns; qmiv_batch_collect waits for that replay and finalises its views on the host. Replays of
+   different batches run one after the other in launch order, so a caller that launches frame set
+   k + 1 before collecting set k finalises set k while the device computes set k + 1. A batch holds
+   at most one replay in flight (its scratch and results are reused): launching it again before the
+   collect, or collecting with nothing launched, fails with QMIV_ERR_INVALID_ARG. Uploads into a
+   batch's pictures go on the context stream, after the replays launched before them. */
+qmiv_status qmiv_batch_launch(qmiv_batch* batch);
+qmiv_status qmiv_batch_collect(qmiv_batch* batch, qmiv_frame_result* out);
 qmiv_status qmiv_batch_get_info(const qmiv_batch* batch, qmiv_batch_info* info);
 void qmiv_batch_destroy(qmiv_batch* batch);
 

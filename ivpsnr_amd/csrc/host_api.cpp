@@ -998,6 +998,8 @@ struct qmiv_batch {
   hipEvent_t fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;  // recorded after a launched replay
+  bool pending = false;       // a replay launched and not yet collected
   qmiv_batch_info info{};
 };
 
@@ -2185,7 +2187,7 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
     // Every candidate is built and timed once per round, in kRounds interleaved rounds, and ranked
     // by its fastest round. The candidates are built in a batch of their own (own streams and
     // scratch) that ends with the tuning: the batch kept starts from fresh streams.
-    constexpr int kRounds = 5;
+    constexpr int kRounds = 8;
     b->info.tune_rounds = kRounds;
     std::unique_ptr<qmiv_batch, void (*)(qmiv_batch*)> tb(new qmiv_batch, qmiv_batch_destroy);
     tb->ctx = ctx;
@@ -2263,11 +2265,15 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
   return QMIV_OK;
 }
 
-qmiv_status qmiv_batch_run(qmiv_batch* b, qmiv_frame_result* out) {
-  if (!b || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+qmiv_status qmiv_batch_launch(qmiv_batch* b) {
+  if (!b) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (b->pending) return fail(QMIV_ERR_INVALID_ARG, "the batch has a replay in flight: collect it first");
   qmiv_ctx* ctx = b->ctx;
   QMIV_HIP(hipSetDevice(ctx->device));
+  if (!b->done) QMIV_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
   QMIV_HIP(hipGraphLaunch(b->exec, ctx->stream));
+  QMIV_HIP(hipEventRecord(b->done, ctx->stream));
+  b->pending = true;
   // The spherical weights every view needs exist before the views are finalised in parallel (the
   // table is only read there).
   for (int32_t i = 0; i < b->n; i++)
@@ -2276,7 +2282,15 @@ qmiv_status qmiv_batch_run(qmiv_batch* b, qmiv_frame_result* out) {
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     b->pool.reset(new HostPool((int)std::min<unsigned>(7u, std::min<unsigned>(hw, (unsigned)b->n) - 1u)));
   }
-  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_batch_collect(qmiv_batch* b, qmiv_frame_result* out) {
+  if (!b || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (!b->pending) return fail(QMIV_ERR_INVALID_ARG, "the batch has no replay in flight");
+  QMIV_HIP(hipSetDevice(b->ctx->device));
+  b->pending = false;
+  QMIV_HIP(hipEventSynchronize(b->done));
   std::vector<qmiv_status> st((size_t)b->n, QMIV_OK);
   auto fin_view = [&](int i) {
     const size_t k = (size_t)i;
@@ -2289,6 +2303,12 @@ qmiv_status qmiv_batch_run(qmiv_batch* b, qmiv_frame_result* out) {
     if (st[(size_t)i] != QMIV_OK)
       return fail(st[(size_t)i], "view " + std::to_string(i) + ": input contains pels above the bit-depth maximum");
   return QMIV_OK;
+}
+
+qmiv_status qmiv_batch_run(qmiv_batch* b, qmiv_frame_result* out) {
+  if (!b || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  QMIV_TRY(qmiv_batch_launch(b));
+  return qmiv_batch_collect(b, out);
 }
 
 qmiv_status qmiv_batch_get_info(const qmiv_batch* b, qmiv_batch_info* info) {
@@ -2305,6 +2325,7 @@ void qmiv_batch_destroy(qmiv_batch* b) {
   for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
   for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->done) (void)hipEventDestroy(b->done);
   if (b->d_scratch) (void)hipFree(b->d_scratch);
   if (b->d_scp) (void)hipFree(b->d_scp);
   if (b->h_results) (void)hipHostFree(b->h_results);

@@ -515,6 +515,18 @@ class Batch:
             check(st, "qmiv_batch_run")
         return list(out)
 
+    def launch(self):
+        """Enqueues one replay and returns (qmiv_batch_launch); collect() waits for it."""
+        check(lib().qmiv_batch_launch(self.handle), "qmiv_batch_launch")
+
+    def collect(self, raise_on_error: bool = True):
+        """Waits for the launched replay and finalises its views (qmiv_batch_collect)."""
+        out = (FrameResult * self.n)()
+        st = lib().qmiv_batch_collect(self.handle, out)
+        if raise_on_error:
+            check(st, "qmiv_batch_collect")
+        return list(out)
+
     def info(self) -> dict:
         i = capi.BatchInfo()
         check(lib().qmiv_batch_get_info(self.handle, C.byref(i)), "qmiv_batch_get_info")

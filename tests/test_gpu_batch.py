@@ -63,7 +63,7 @@ def test_batch_equals_eager_over_replays(engine, branches, sweep):
         assert info["instances"] == 0  # a given shape is not timed
     else:
         if not info["from_cache"]:
-            assert info["tune_rounds"] == 5 and info["candidates_tried"] >= 5 * 4
+            assert info["tune_rounds"] == 8 and info["candidates_tried"] >= 8 * 4
         # the kept instance was checked against the shape's fastest
         assert info["tuned_ms"] > 0 and 1 <= info["instances"] <= 6 and info["instance_ms"] > 0
     want = eager(engine, pics)
@@ -119,3 +119,28 @@ def test_tuned_choice_is_remembered(engine):
     other = [default_params(erp=not v[4]) for v in VIEWS]  # another signature is tuned anew
     c = engine.batch(pics, [v[5] for v in VIEWS], other).info()
     assert c["from_cache"] == 0 and c["candidates_tried"] > 0
+
+
+def test_launch_collect_pipelines_frame_sets(engine):
+    """qmiv_batch_launch / qmiv_batch_collect: several batches launched back to back, then
+    collected, give the eager results; a second launch before the collect and a collect with
+    nothing launched are refused."""
+    sets = [make_views(engine, 700 + 10 * k)[0] for k in range(3)]
+    prms = [default_params(erp=v[4]) for v in VIEWS]
+    bts = [engine.batch(pics, [v[5] for v in VIEWS], prms, branches=4, cps_sweep=0) for pics in sets]
+    want = [eager(engine, pics) for pics in sets]
+    for rep in range(2):
+        for bt in bts:
+            bt.launch()
+        with pytest.raises(Exception):
+            bts[0].launch()  # one replay in flight per batch
+        for k in (1, 0, 2):  # any collect order
+            got = bts[k].collect()
+            for i, (g, w) in enumerate(zip(got, want[k])):
+                assert same(g, w), (rep, k, i)
+    with pytest.raises(Exception):
+        bts[0].collect()  # nothing in flight
+    got = bts[2].run()  # run still works after launch / collect
+    assert all(same(g, w) for g, w in zip(got, want[2]))
+    for bt in bts:
+        bt.close()
