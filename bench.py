@@ -598,9 +598,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic(wl["traffic_file"]),
                 "kernel": wl["cps_kernel"], "kernel_avg_us": round(k_avg_ms * 1e3, 2),
                 "kernel_avg_us_in_set": round(k_set_ms / max(k_set_n, 1) * 1e3, 2), "timing": timing_note,
-                # the kernel's launches of one step, summed (below ms_per_step while the launches of a set
-                # do not overlap each other)
+                # the kernel's launches of one step, each alone on the device, summed: in pipelined frame
+                # sets the launches of different frames overlap, so a step can take less than this sum
                 "kernel_us_per_step": round(k_ms / prof_steps * 1e3, 1), "bytes_per_launch": per_launch,
+                # the whole job's compulsory bytes over its time per step (every kernel and overlap included)
+                "achieved_whole_job": round(nframes * per_launch / (ms_per_step / 1e3) / 1e9, 1),
                 # The search is bound by int32 VALU issue, not HBM: 25 candidates x 2 directions per
                 # pel, and the VALU busy fraction of the same kernel from its PMC summary.
                 "candidate_evals_per_s": round(50 * W * H / (k_avg_ms / 1e3), -6) if k_avg_ms > 0 else 0.0,
