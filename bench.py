@@ -246,8 +246,11 @@ def run_grad(args):
     eng = Engine(0)
 
     def step():
-        return [eng.ivssim_backward(w, h, tst[i].data_ptr(), ref[i].data_ptr(), maxv, None, gt[i].data_ptr(),
-                                    gr[i].data_ptr())["ivssim"] for i in range(n)]
+        # the batch's pairs together: one wait for all values, one for all gradients
+        res = eng.ivssim_backward_batch(w, h, [tst[i].data_ptr() for i in range(n)],
+                                        [ref[i].data_ptr() for i in range(n)], maxv, None,
+                                        [gt[i].data_ptr() for i in range(n)], [gr[i].data_ptr() for i in range(n)])
+        return [r["ivssim"] for r in res]
 
     for _ in range(args.warmup):
         step()

@@ -533,6 +533,16 @@ qmiv_status qmiv_msssim_backward(qmiv_ctx* ctx, int32_t width, int32_t height, i
 qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t width, int32_t height, const float* tst, const float* ref,
                                  double max_value, const qmiv_params* prm, double* ivssim, double* r2t, double* t2r,
                                  float* grad_tst, float* grad_ref);
+/* qmiv_ivssim_backward over n picture pairs of one size (no counterpart in the reference; for
+   training batches): pair i is (tst[i], ref[i]), its results go to ivssim[i], r2t[i], t2r[i] and the
+   gradients to grad_tst[i] / grad_ref[i] (each array nullable, and so is each gradient pointer).
+   The pairs' searches and SSIM evaluations run on the context's streams together, with one wait
+   for all values and one for all gradients; each pair's results equal those of
+   qmiv_ivssim_backward on it alone. */
+qmiv_status qmiv_ivssim_backward_batch(qmiv_ctx* ctx, int32_t n, int32_t width, int32_t height,
+                                       const float* const* tst, const float* const* ref, double max_value,
+                                       const qmiv_params* prm, double* ivssim, double* r2t, double* t2r,
+                                       float* const* grad_tst, float* const* grad_ref);
 
 #ifdef __cplusplus
 }

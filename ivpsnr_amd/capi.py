@@ -177,6 +177,9 @@ _SIGNATURES = {
                                        _f64p, _f64p, C.c_void_p, C.c_void_p]),
     "qmiv_ivssim_backward": (C.c_int, [_ctx_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_double,
                                        C.POINTER(Params), _f64p, _f64p, _f64p, C.c_void_p, C.c_void_p]),
+    "qmiv_ivssim_backward_batch": (C.c_int, [_ctx_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
+                                             C.POINTER(C.c_void_p), C.c_double, C.POINTER(Params), _f64p, _f64p,
+                                             _f64p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
 }
 EXPORTS.extend(_SIGNATURES)
 

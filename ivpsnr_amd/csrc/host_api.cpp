@@ -1183,8 +1183,8 @@ void ssimf_consts(double max_value, double& c1, double& c2, double f[100]) {
 // maps is not null the gradient maps; then, when g.grad_t or g.grad_r is set, the gradient.
 qmiv_status enqueue_ssimf(qmiv_ctx* ctx, const void* tst, int32_t tst_f64, double scale_t, const void* ref,
                           int32_t ref_f64, double scale_r, int32_t w, int32_t h, int32_t calc_l, double max_value,
-                          double* partials, double* rows_dev, double* maps, SsimGArgs* g) {
-  hipStream_t st = ctx->stream;
+                          double* partials, double* rows_dev, double* maps, SsimGArgs* g, hipStream_t st = nullptr) {
+  if (!st) st = ctx->stream;
   SsimFArgs a{};
   a.tst = tst;
   a.ref = ref;
@@ -1234,11 +1234,12 @@ qmiv_status check_fargs(qmiv_ctx* ctx, int32_t w, int32_t h, int32_t planes, con
 
 // Quantised 4:4:4 picture of the context for the shift search (recreated on a geometry change).
 qmiv_status fq_picture(qmiv_ctx* ctx, int k, int32_t w, int32_t h, int32_t bd, qmiv_pic** out) {
-  qmiv_pic*& q = ctx->fq[k];
+  if ((int)ctx->fq.size() <= k) ctx->fq.resize((size_t)k + 1, nullptr);
+  qmiv_pic*& q = ctx->fq[(size_t)k];
   Geom g;
   QMIV_TRY(make_geom(w, h, bd, 444, g));
   if (q && !same_geom(q->g, g)) {
-    QMIV_HIP(hipStreamSynchronize(ctx->stream));
+    QMIV_TRY(wait_stream(ctx));  // every stream that may read it
     (void)hipFree(q->dev);
     delete q;
     q = nullptr;
@@ -2507,8 +2508,8 @@ qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, const floa
   int32_t gcd[3] = {sum->gcd[0], sum->gcd[1], sum->gcd[2]};
   const int32_t maxv = (1 << bd) - 1;
   // SCP1 = clip(P1(best0) - gcd) (R2T winners), SCP0 = clip(P0(best1) + gcd) (T2R winners), in samples.
-  ScpFArgs s1{ref, best, w, h, prm.search_range, max_value, 1, {gcd[0], gcd[1], gcd[2]}, maxv, scp + 3 * plane};
-  ScpFArgs s0{tst, best + plane, w, h, prm.search_range, max_value, -1, {gcd[0], gcd[1], gcd[2]}, maxv, scp};
+  ScpFArgs s1{ref, best, w, h, prm.search_range, max_value, 1, {gcd[0], gcd[1], gcd[2]}, nullptr, maxv, scp + 3 * plane};
+  ScpFArgs s0{tst, best + plane, w, h, prm.search_range, max_value, -1, {gcd[0], gcd[1], gcd[2]}, nullptr, maxv, scp};
   QMIV_HIP(launch_scp_gatherf(s1, st));
   QMIV_HIP(launch_scp_gatherf(s0, st));
   // T2R_c = SSIM(P0_c, SCP1_c), R2T_c = SSIM(P1_c, SCP0_c) (libs/pmbb-ivqm/src/xSSIM.cpp:197-207).
@@ -2557,6 +2558,151 @@ qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, const floa
   }
   if (g_other) QMIV_HIP(launch_scp_scatterf(use_r2t ? s0 : s1, gscp, 1.0, g_other, st));
   QMIV_HIP(hipStreamSynchronize(st));
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_ivssim_backward_batch(qmiv_ctx* ctx, int32_t n, int32_t w, int32_t h, const float* const* tst,
+                                       const float* const* ref, double max_value, const qmiv_params* prm_in,
+                                       double* ivssim, double* r2t, double* t2r, float* const* grad_tst,
+                                       float* const* grad_ref) {
+  if (!ctx || n < 0 || (n > 0 && (!tst || !ref))) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return QMIV_OK;
+  for (int32_t i = 0; i < n; i++) QMIV_TRY(check_fargs(ctx, w, h, 3, tst[i], ref[i], max_value));
+  int32_t bd = 0;
+  for (int b = 8; b <= 16; b++)
+    if (max_value == (double)((1 << b) - 1)) bd = b;
+  if (!bd) return fail(QMIV_ERR_UNSUPPORTED, "IV-SSIM needs max_value = 2^bd - 1 with bd in 8..16");
+  qmiv_params prm;
+  if (prm_in) prm = *prm_in;
+  else qmiv_params_default(&prm);
+  QMIV_HIP(hipSetDevice(ctx->device));
+  const size_t plane = (size_t)w * h;
+  auto want_grad = [&](int32_t i) { return (grad_tst && grad_tst[i]) || (grad_ref && grad_ref[i]); };
+  bool any_grad = false;
+  for (int32_t i = 0; i < n; i++) any_grad = any_grad || want_grad(i);
+  // The search of every pair on its quantised pictures, as qmiv_ivssim_backward's; one plan serves all.
+  Geom g;
+  QMIV_TRY(make_geom(w, h, bd, 444, g));
+  FrameWork fw0;
+  fw0.ipa = QMIV_PEL_SKIP;
+  fw0.need_gcd = true;
+  fw0.need_cps = true;
+  const FramePlan P = plan_frame(ctx, g, prm, fw0);
+  const Layout& L = P.L;
+  const size_t dstep = align_up(L.total, 256);
+  const size_t hstep = align_up(L.results_bytes + sizeof(double) * 6 * (size_t)h, 256);
+  // Each pair's float workspace: winners, both SCP pictures, the SCP gradient, partials, rows, maps.
+  struct Work {
+    int16_t* best;
+    float *scp, *gscp;
+    double *partials, *rows, *maps;
+  };
+  auto layout = [&](Bump& bb, Work& wk) {
+    wk.best = bb.take<int16_t>(2 * plane);
+    wk.scp = bb.take<float>(6 * plane);
+    wk.gscp = any_grad ? bb.take<float>(3 * plane) : nullptr;
+    wk.partials = bb.take<double>(std::max<size_t>((size_t)ssimf_tiles_x(w) * h, 1));
+    wk.rows = bb.take<double>(6 * (size_t)h);
+    wk.maps = any_grad ? bb.take<double>(4 * plane) : nullptr;
+  };
+  Work probe;
+  Bump b0{nullptr};
+  layout(b0, probe);
+  const size_t fstep = align_up(b0.off, 256);
+  std::vector<qmiv_pic*> q((size_t)2 * n);
+  for (int32_t i = 0; i < 2 * n; i++) QMIV_TRY(fq_picture(ctx, i, w, h, bd, &q[(size_t)i]));
+  QMIV_TRY(wait_stream(ctx));  // the shared blocks below are reused: nothing may still read them
+  QMIV_TRY(ensure_scratch(ctx, dstep * n));
+  QMIV_TRY(ensure_results(ctx, hstep * n));
+  QMIV_TRY(ensure_fwork(ctx, fstep * n));
+  std::vector<Work> wk((size_t)n);
+  std::vector<ScpFArgs> s0((size_t)n), s1((size_t)n);
+  const hipStream_t streams[1 + qmiv_ctx::kAuxStreams] = {ctx->stream, ctx->aux_stream[0], ctx->aux_stream[1]};
+  const int32_t nstreams = std::min<int32_t>(n, 1 + qmiv_ctx::kAuxStreams);
+  if (nstreams > 1) {  // the aux streams start after everything already on the context stream
+    QMIV_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+    for (int32_t k = 1; k < nstreams; k++) QMIV_HIP(hipStreamWaitEvent(streams[k], ctx->fork_ev, 0));
+  }
+  const int32_t maxv = (1 << bd) - 1;
+  const int32_t bytes = bd <= 8 ? 1 : 2;
+  // Values: quantisation, search, shift-compensated pictures and the six SSIM evaluations of every
+  // pair, its pairs dealt round-robin over the streams; the GCD goes from the search's summary to the
+  // gathers on the device.
+  for (int32_t i = 0; i < n; i++) {
+    const hipStream_t st = streams[i % nstreams];
+    Bump bb{(char*)ctx->d_fwork + fstep * i};
+    Work& k = wk[(size_t)i];
+    layout(bb, k);
+    char* dbase = (char*)ctx->d_scratch + dstep * i;
+    char* hbase = (char*)ctx->h_results + hstep * i;
+    QMIV_HIP(launch_quantf(tst[i], 3 * (int64_t)plane, max_value, maxv, q[2 * (size_t)i]->dev, bytes, st));
+    QMIV_HIP(launch_quantf(ref[i], 3 * (int64_t)plane, max_value, maxv, q[2 * (size_t)i + 1]->dev, bytes, st));
+    FrameWork fw = fw0;
+    fw.best0 = k.best;
+    fw.best1 = k.best + plane;
+    QMIV_TRY(enqueue_frame(ctx, q[2 * (size_t)i], q[2 * (size_t)i + 1], prm, fw, P, dbase, hbase, nullptr, st));
+    const int32_t* gcd_dev = ((const FrameSummary*)(dbase + L.off_summary2))->gcd;
+    s1[(size_t)i] = ScpFArgs{ref[i], k.best, w, h, prm.search_range, max_value, 1, {0, 0, 0}, gcd_dev, maxv,
+                             k.scp + 3 * plane};
+    s0[(size_t)i] = ScpFArgs{tst[i], k.best + plane, w, h, prm.search_range, max_value, -1, {0, 0, 0}, gcd_dev, maxv,
+                             k.scp};
+    QMIV_HIP(launch_scp_gatherf(s1[(size_t)i], st));
+    QMIV_HIP(launch_scp_gatherf(s0[(size_t)i], st));
+    for (int c = 0; c < 3; c++) {
+      QMIV_TRY(enqueue_ssimf(ctx, tst[i] + c * plane, 0, max_value, k.scp + 3 * plane + c * plane, 0, 1.0, w, h, 1,
+                             max_value, k.partials, k.rows + (size_t)c * h, nullptr, nullptr, st));
+      QMIV_TRY(enqueue_ssimf(ctx, ref[i] + c * plane, 0, max_value, k.scp + c * plane, 0, 1.0, w, h, 1, max_value,
+                             k.partials, k.rows + (size_t)(3 + c) * h, nullptr, nullptr, st));
+    }
+    QMIV_HIP(hipMemcpyAsync(hbase + L.results_bytes, k.rows, sizeof(double) * 6 * (size_t)h, hipMemcpyDeviceToHost,
+                            st));
+  }
+  for (int32_t k = 0; k < nstreams; k++) QMIV_HIP(hipStreamSynchronize(streams[k]));
+  const double wsum = (double)(prm.cmp_weights_average[0] + prm.cmp_weights_average[1] +
+                               prm.cmp_weights_average[2] + prm.cmp_weights_average[3]);
+  std::vector<char> use_r2t((size_t)n, 0);
+  for (int32_t i = 0; i < n; i++) {
+    const double* hr = (const double*)((const char*)ctx->h_results + hstep * i + L.results_bytes);
+    double vt2r[3], vr2t[3];
+    for (int c = 0; c < 3; c++) {
+      vt2r[c] = fin::ssim_from_rows(hr + (size_t)c * h, h, w);
+      vr2t[c] = fin::ssim_from_rows(hr + (size_t)(3 + c) * h, h, w);
+    }
+    const double T2R = fin::weighted_average4(vt2r, prm.cmp_weights_average);
+    const double R2T = fin::weighted_average4(vr2t, prm.cmp_weights_average);
+    if (ivssim) ivssim[i] = fin::min2(T2R, R2T);
+    if (t2r) t2r[i] = T2R;
+    if (r2t) r2t[i] = R2T;
+    use_r2t[(size_t)i] = R2T < T2R;  // min2 keeps T2R on ties
+  }
+  if (!any_grad) return QMIV_OK;
+  // Gradients (the subgradient of qmiv_ivssim_backward: shifts, GCD and the minimum's direction fixed).
+  for (int32_t i = 0; i < n; i++) {
+    if (!want_grad(i)) continue;
+    const hipStream_t st = streams[i % nstreams];
+    const Work& k = wk[(size_t)i];
+    float* gt = grad_tst ? grad_tst[i] : nullptr;
+    float* gr = grad_ref ? grad_ref[i] : nullptr;
+    if (gt) QMIV_HIP(hipMemsetAsync(gt, 0, 3 * plane * sizeof(float), st));
+    if (gr) QMIV_HIP(hipMemsetAsync(gr, 0, 3 * plane * sizeof(float), st));
+    const bool r2 = use_r2t[(size_t)i] != 0;
+    const float* direct = r2 ? ref[i] : tst[i];
+    float* g_direct = r2 ? gr : gt;
+    float* g_other = r2 ? gt : gr;
+    const float* scp_sel = r2 ? k.scp : k.scp + 3 * plane;
+    for (int c = 0; c < 3; c++) {
+      SsimGArgs ga{};
+      ga.grad_t = g_direct ? g_direct + c * plane : nullptr;
+      ga.grad_r = g_other ? k.gscp + c * plane : nullptr;
+      ga.coef_t = ga.coef_r = (double)prm.cmp_weights_average[c] / wsum;
+      ga.accumulate = 1;
+      if (ga.grad_r) QMIV_HIP(hipMemsetAsync(ga.grad_r, 0, plane * sizeof(float), st));
+      QMIV_TRY(enqueue_ssimf(ctx, direct + c * plane, 0, max_value, scp_sel + c * plane, 0, 1.0, w, h, 1, max_value,
+                             k.partials, k.rows + (size_t)c * h, k.maps, &ga, st));
+    }
+    if (g_other) QMIV_HIP(launch_scp_scatterf(r2 ? s0[(size_t)i] : s1[(size_t)i], k.gscp, 1.0, g_other, st));
+  }
+  for (int32_t k = 0; k < nstreams; k++) QMIV_HIP(hipStreamSynchronize(streams[k]));
   return QMIV_OK;
 }
 

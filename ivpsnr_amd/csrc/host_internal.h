@@ -62,7 +62,7 @@ struct qmiv_ctx {
   // Differentiable SSIM family: float workspace and the quantised pictures of the shift search.
   void* d_fwork = nullptr;
   size_t fwork_bytes = 0;
-  qmiv_pic* fq[2] = {nullptr, nullptr};
+  std::vector<qmiv_pic*> fq;  // quantised pictures of the float SSIM family's shift searches, by slot
   // Device scratch (grown on demand; one frame evaluation at a time per context).
   void* d_scratch = nullptr;
   size_t scratch_bytes = 0;

@@ -262,6 +262,10 @@ __global__ __launch_bounds__(256) void k_quantf(const float* __restrict__ src, i
   else ((uint16_t*)dst)[i] = (uint16_t)q;
 }
 
+// The GCD component c of a shift-compensated picture: from device memory when the search's summary
+// holds it, else the value given.
+__device__ __forceinline__ int32_t scp_gcd(const ScpFArgs& a, int c) { return a.gcd_dev ? a.gcd_dev[c] : a.gcd[c]; }
+
 // Shift-compensated picture in sample units (xShftCompPic::xGenShftCompRow on real samples):
 // out_c(p) = clamp(src_c(clamp(p + o(best(p)))) * scale - sgn gcd_c, 0, maxv), 3 planes.
 __global__ __launch_bounds__(256) void k_scp_gatherf(ScpFArgs a) {
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(256) void k_scp_gatherf(ScpFArgs a) {
   const int64_t s = (int64_t)sy * a.w + sx;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const double v = (double)a.src[c * plane + s] * a.scale - a.sgn * a.gcd[c];
+    const double v = (double)a.src[c * plane + s] * a.scale - a.sgn * scp_gcd(a, c);
     a.out[c * plane + i] = (float)fmin(fmax(v, 0.0), (double)a.maxv);
   }
 }
@@ -306,7 +310,7 @@ struct ScpAdj {
 __device__ __forceinline__ bool scp_open(const ScpFArgs& a, int64_t q, int64_t plane, bool (&open)[3]) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const double v = (double)a.src[c * plane + q] * a.scale - a.sgn * a.gcd[c];
+    const double v = (double)a.src[c * plane + q] * a.scale - a.sgn * scp_gcd(a, c);
     open[c] = v >= 0.0 && v <= (double)a.maxv;
   }
   return open[0] || open[1] || open[2];

@@ -178,6 +178,7 @@ struct ScpFArgs {
   double scale;
   int32_t sgn;           // +1: Y - gcd (SCP of InputFile1); -1: Y + gcd (SCP of InputFile0)
   int32_t gcd[3];
+  const int32_t* gcd_dev = nullptr;  // when set, the GCD is read from device memory instead (3 values)
   int32_t maxv;
   float* out;            // [3][h][w] in sample units
 };

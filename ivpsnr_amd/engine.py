@@ -308,6 +308,18 @@ class Engine:
                                          C.c_void_p(grad_ref_ptr or None)), "qmiv_ivssim_backward")
         return {"ivssim": iv.value, "r2t": r2t.value, "t2r": t2r.value}
 
+    def ivssim_backward_batch(self, w, h, tst_ptrs, ref_ptrs, max_value, params: Params | None = None,
+                              grad_tst_ptrs=None, grad_ref_ptrs=None):
+        """qmiv_ivssim_backward_batch over the pairs (tst_ptrs[i], ref_ptrs[i]); one dict per pair."""
+        n = len(tst_ptrs)
+        ptrs = lambda v: (C.c_void_p * n)(*[C.c_void_p(p or None) for p in v]) if v is not None else None
+        iv, r2t, t2r = (C.c_double * n)(), (C.c_double * n)(), (C.c_double * n)()
+        check(lib().qmiv_ivssim_backward_batch(self.ctx, n, w, h, ptrs(tst_ptrs), ptrs(ref_ptrs), max_value,
+                                               C.byref(params) if params else None, iv, r2t, t2r,
+                                               ptrs(grad_tst_ptrs), ptrs(grad_ref_ptrs)),
+              "qmiv_ivssim_backward_batch")
+        return [{"ivssim": iv[i], "r2t": r2t[i], "t2r": t2r[i]} for i in range(n)]
+
     def calc_gcd(self, tst: Picture, ref: Picture, params: Params | None = None) -> np.ndarray:
         g = (C.c_int32 * 4)()
         check(lib().qmiv_calc_gcd(self.ctx, tst.handle, ref.handle, C.byref(params) if params else None, g),

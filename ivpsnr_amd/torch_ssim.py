@@ -51,7 +51,16 @@ class _SSIMFamily(torch.autograd.Function):
         eng = _engine(tb.device)
         torch.cuda.current_stream(tb.device).synchronize()  # the engine runs on its own stream
         vals = []
-        for i in range(n):
+        if kind == "ivssim" and n > 1:
+            # the pictures of the batch together (qmiv_ivssim_backward_batch)
+            if planes != 3:
+                raise ValueError("IV-SSIM needs 3 planes")
+            res = eng.ivssim_backward_batch(w, h, [tb[i].data_ptr() for i in range(n)],
+                                            [rb[i].data_ptr() for i in range(n)], max_value, params,
+                                            [gt[i].data_ptr() for i in range(n)] if need else None,
+                                            [gr[i].data_ptr() for i in range(n)] if need else None)
+            vals = [[r["ivssim"]] for r in res]
+        for i in range(n if not vals else 0):
             ptr_gt = gt[i].data_ptr() if need else 0
             ptr_gr = gr[i].data_ptr() if need else 0
             if kind == "ssim":

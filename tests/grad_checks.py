@@ -421,6 +421,36 @@ def test_workload_batch_1024(engine):
                 assert abs(fd - gi[p, y, x]) <= 1e-4 * abs(gi[p, y, x]), (kind, i, y, x, fd, gi[p, y, x])
 
 
+def test_ivssim_batch_equals_single(engine):
+    """qmiv_ivssim_backward_batch: every pair's values and gradients equal qmiv_ivssim_backward on it
+    alone, bit for bit, with more pairs than streams, one pair of identical pictures (tied
+    directions), and pairs asking for one gradient or none."""
+    h, w = 72, 200
+    pairs = [field((3, h, w), 300 + i) for i in range(5)]
+    pairs[2] = (pairs[2][1].copy(), pairs[2][1].copy())
+    tst = [cuda(t) for t, _ in pairs]
+    ref = [cuda(r) for _, r in pairs]
+    want_t = [True, True, False, True, False]
+    want_r = [True, False, True, True, False]
+    gt = [torch.zeros_like(t) for t in tst]
+    gr = [torch.zeros_like(r) for r in ref]
+    torch.cuda.synchronize()
+    got = engine.ivssim_backward_batch(w, h, [t.data_ptr() for t in tst], [r.data_ptr() for r in ref], MAXV, None,
+                                       [g.data_ptr() if k else 0 for g, k in zip(gt, want_t)],
+                                       [g.data_ptr() if k else 0 for g, k in zip(gr, want_r)])
+    for i in range(len(pairs)):
+        st, sr = torch.zeros_like(tst[i]), torch.zeros_like(ref[i])
+        one = engine.ivssim_backward(w, h, tst[i].data_ptr(), ref[i].data_ptr(), MAXV, None,
+                                     st.data_ptr() if want_t[i] else 0, sr.data_ptr() if want_r[i] else 0)
+        torch.cuda.synchronize()
+        assert got[i] == one, (i, got[i], one)
+        if want_t[i]:
+            assert torch.equal(gt[i], st), ("grad_tst", i)
+        if want_r[i]:
+            assert torch.equal(gr[i], sr), ("grad_ref", i)
+        print(f"  pair {i}: IV-SSIM {got[i]['ivssim']:.9f}", flush=True)
+
+
 CHECKS = [
     ("forward_equals_integer_engine", test_forward_equals_integer_engine, {}),
     ("ssim_gradient_fp64_autograd[L]", test_ssim_gradient_against_fp64_autograd, {"calc_l": True}),
@@ -434,6 +464,7 @@ CHECKS = [
     ("finite_differences[msssim]", test_gradient_against_finite_differences,
      {"kind": "msssim", "shape": (1, 180, 184)}),
     ("finite_differences[ivssim]", test_gradient_against_finite_differences, {"kind": "ivssim", "shape": (3, 40, 48)}),
+    ("ivssim_batch_equals_single", test_ivssim_batch_equals_single, {}),
     ("autograd_wrappers", test_autograd_wrappers, {}),
     ("workload_batch_8x3x1024x1024", test_workload_batch_1024, {}),
 ]
