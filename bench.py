@@ -394,8 +394,8 @@ def run_masked(args):
                    "stream_vs_plain": round(mpix / (ms_per_step / 1e3) / plain_mpix_s, 3) if plain_mpix_s else None,
                    "ivpsnr_last_frame": round(res[-1].ivpsnr, 6), "nnm_last_frame": res[-1].num_non_masked},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic("k_cps_fast420_masked_1920x1080_8b.txt"),
-                     "kernel": "k_cps_fast420 (masked)", "kernel_avg_us": round(k_avg_ms * 1e3, 2),
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic("k_cps420s_masked_1920x1080_8b.txt"),
+                     "kernel": "k_cps420s (masked)", "kernel_avg_us": round(k_avg_ms * 1e3, 2),
                      "kernel_avg_us_in_set": round(k_set_ms / max(k_set_n, 1) * 1e3, 2),
                      "timing": "kernel_avg_us: HIP events around each launch with the frame's kernels alone on the "
                                "device (per-frame leg); kernel_avg_us_in_set: inside the timed frame-set steps",

@@ -2791,6 +2791,7 @@ struct MaskedPlan {
   MaskedLayout L;
   bool cps = false;
   bool fast = false;  // the search runs on a fast kernel (the 4:2:0 one when g.cs == 1)
+  bool s16 = false;   // ... the 16-bit 4:2:0 kernel
   CpsArgs fa{};       // its tiling (tiles_x, sweep, nseg) when fast
   int32_t md[4];
   int32_t ipa = QMIV_PEL_STOP;
@@ -2813,10 +2814,22 @@ qmiv_status plan_masked(qmiv_ctx* ctx, const Geom& g, const Geom& gm, const qmiv
   P.fast = cps && prm.search_range == 2 && fast_masked_cps_ok(g, prm, gbound);
   if (P.fast) {
     const bool f420 = g.cs == 1;
+    // The 16-bit kernel keeps each lane's masked sums of a sweep in 32 bits: 2 pels per row and tile,
+    // each square (|X - Y| <= maxv + |gcd|) times a mask value of at most maxm.
+    int64_t s16_sweep = 0;
+    if (f420 && cps420s_ok(g, prm, gbound)) {
+      const int64_t maxv = (1 << g.bd) - 1, maxm = (1 << gm.bd) - 1;
+      int64_t m = 0;
+      for (int c = 0; c < 3; c++) m = std::max<int64_t>(m, maxv + std::abs((int64_t)gbound[c]));
+      s16_sweep = (int64_t)0xffffffffLL / (2 * m * m * maxm);
+    }
+    P.s16 = s16_sweep >= 1;
+    const int kind = P.s16 ? kCps420s : f420 ? kCps420 : kCps444;
     int32_t tw, th;
-    cps_tile(f420 ? kCps420 : kCps444, &tw, &th);
+    cps_tile(kind, &tw, &th);
     P.fa.tiles_x = (g.w + tw - 1) / tw;
-    P.fa.sweep = pick_sweep(ctx, g, f420 ? kCps420 : kCps444, gbound);
+    P.fa.sweep = pick_sweep(ctx, g, kind, gbound);
+    if (P.s16) P.fa.sweep = (int32_t)std::min<int64_t>(P.fa.sweep, s16_sweep);
     P.fa.nseg = (P.fa.tiles_x + P.fa.sweep - 1) / P.fa.sweep;
   }
   P.ipa = ipa;
@@ -2885,7 +2898,8 @@ qmiv_status enqueue_masked(qmiv_ctx* ctx, const qmiv_pic* p0, const qmiv_pic* p1
         fa.mask_bytes = (uint32_t)((size_t)g.w * g.h * mask->g.bytes_per_sample);  // its luma plane
         fa.mask_bytes_per_sample = mask->g.bytes_per_sample;
         fa.msum = sum2;
-        if (g.cs == 1) QMIV_HIP(launch_cps_fast420(fa, st));
+        if (P.s16) QMIV_HIP(launch_cps420s(fa, st));
+        else if (g.cs == 1) QMIV_HIP(launch_cps_fast420(fa, st));
         else QMIV_HIP(launch_cps_fast(fa, st));
       } else {
         QMIV_HIP(launch_cps_masked(p0->dev, p1->dev, g, mask->dev, mask->g, sum2, prm.search_range,

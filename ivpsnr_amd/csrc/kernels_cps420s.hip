@@ -147,6 +147,24 @@ struct PicRegs {
   uint32_t l0[kLumaSteps], l1[kLumaSteps], u[kCellSteps], v[kCellSteps];
 };
 
+// Masked search: a searched pel whose mask is 0 gets the records {32 Y = 0, L = kMaskedC}, so its
+// keys are kMaskedC + T(P, Q) + h > 2^29 while every real key lies in (-2^29, 2^29) and |T| < 2^29
+// (host check), and it never beats an unmasked candidate; kMaskedC = 0 (mod 32) keeps the raster
+// term, so a winner still decodes to a window position. A tested pel whose mask is 0 contributes
+// nothing; any other has itself as an unmasked candidate, so its winner is the first strict minimum
+// over the unmasked ones, as in xCorrespPixelShiftSTD::FindBestPixelWithinBlockM
+// (libs/pmbb-ivqm/src/xCorrespPixelShiftSTD.cpp).
+constexpr int32_t kMaskedC = 1 << 30;
+// The mask samples of a luma item (searched tile) as loaded: 16-bit masks m0 = M0 | M1 << 16,
+// m1 = M2 | M3 << 16; 8-bit masks m0 = the four bytes.
+struct MaskRegs {
+  uint32_t m0[kLumaSteps], m1[kLumaSteps];
+};
+// The tested tile's mask samples as loaded (the items of load_test).
+struct TestMaskRegs {
+  uint32_t m0[kTestLumaSteps], m1[kTestLumaSteps];
+};
+
 // The thread's fill position: fill row r and quad column q (r >= kFillRows: no fill items).
 struct Items {
   int32_t r, q;
@@ -240,14 +258,60 @@ __device__ __forceinline__ void load_pic(const void* base, const Geom& g, const 
   }
 }
 
+// Loads the mask samples of the searched tile's luma items for the tile at (x0, y0), as load_pic
+// loads the luma: vector loads on interior tiles (an item past the tile reads 0 through the range
+// check), clamped single-sample loads at the edges. The mask's luma plane starts at offset 0.
+template <typename TM>
+__device__ __forceinline__ void load_mask(const void* base, uint32_t bytes, const Geom& g, const Items& it, int x0,
+                                          int y0, bool interior, MaskRegs& f) {
+  const Buf<TM> M(base, bytes);
+  constexpr uint32_t S = sizeof(TM);
+  if (interior) {
+    const uint32_t lofs = (uint32_t)(it.r * g.w + 4 * it.q) * S;
+    const uint32_t tl = (uint32_t)((y0 - 2) * g.w + x0 - 4) * S;
+    constexpr uint32_t kNone = 0x80000000u;
+#pragma unroll
+    for (int k = 0; k < kLumaSteps; k++) {
+      const uint32_t vo = luma_item(it, k) ? lofs : kNone;
+      const uint32_t so = tl + (uint32_t)(kFillRows * k * g.w) * S;
+      if constexpr (S == 2) {
+        const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(M.r, vo, so, 0));
+        f.m0[k] = w.x;
+        f.m1[k] = w.y;
+      } else {
+        f.m0[k] = __builtin_amdgcn_raw_buffer_load_b32(M.r, vo, so, 0);
+        f.m1[k] = 0u;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < kLumaSteps; k++) {
+    if (!luma_item(it, k)) break;
+    const int ly = it.r + kFillRows * k;
+    const int lx = 4 * it.q;
+    const uint32_t row = __umul24((uint32_t)clampi(y0 - 2 + ly, 0, g.h - 1), (uint32_t)g.w);
+    uint32_t q[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) q[m] = M.ld(row + (uint32_t)clampi(x0 - 4 + lx + m, 0, g.w - 1));
+    if constexpr (S == 2) {
+      f.m0[k] = q[0] | (q[1] << 16);
+      f.m1[k] = q[2] | (q[3] << 16);
+    } else {
+      f.m0[k] = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
+      f.m1[k] = 0u;
+    }
+  }
+}
+
 // Builds one picture's tile from its loaded words. lb: the thread's LDS byte offset of its first L
 // record (r, 4 q), which is also that of its first cell (r, 2 q) and twice that of its first Y record;
 // hc = 10 r + 4 q, the raster term 5 ly + lx of the first cell's top-left record. w0s = 32 w0;
 // w12 = w1 | w2 << 16. Samples are below 2^10 (host check), so every shift by 5 stays inside its
 // 16-bit half.
-template <typename T>
-__device__ __forceinline__ void store_pic(const PicRegs& f, const Items& it, uint32_t lb, uint32_t hc, uint32_t w0s,
-                                          uint32_t w12, char* lds) {
+template <typename T, typename TM, bool MASKED>
+__device__ __forceinline__ void store_pic(const PicRegs& f, const MaskRegs& fm, const Items& it, uint32_t lb,
+                                          uint32_t hc, uint32_t w0s, uint32_t w12, char* lds) {
   constexpr uint32_t tile = kOffPic;
 #pragma unroll
   for (int k = 0; k < kLumaSteps; k++) {
@@ -266,9 +330,31 @@ __device__ __forceinline__ void store_pic(const PicRegs& f, const Items& it, uin
     L.y = (int32_t)__umul24(__umul24(y1, y1), w0s);
     L.z = (int32_t)__umul24(__umul24(y2, y2), w0s);
     L.w = (int32_t)__umul24(__umul24(y3, y3), w0s);
+    uint32_t s0 = p0 << 5, s1 = p1 << 5;
+    if constexpr (MASKED) {
+      // masked records: 32 Y = 0 and L = kMaskedC
+      uint32_t k0, k1, k2, k3;  // mask samples of the four records
+      if constexpr (sizeof(TM) == 2) {
+        k0 = fm.m0[k] & 0xffffu;
+        k1 = fm.m0[k] >> 16;
+        k2 = fm.m1[k] & 0xffffu;
+        k3 = fm.m1[k] >> 16;
+      } else {
+        k0 = fm.m0[k] & 0xffu;
+        k1 = (fm.m0[k] >> 8) & 0xffu;
+        k2 = (fm.m0[k] >> 16) & 0xffu;
+        k3 = fm.m0[k] >> 24;
+      }
+      L.x = k0 ? L.x : kMaskedC;
+      L.y = k1 ? L.y : kMaskedC;
+      L.z = k2 ? L.z : kMaskedC;
+      L.w = k3 ? L.w : kMaskedC;
+      s0 &= (k0 ? 0xffffu : 0u) | (k1 ? 0xffff0000u : 0u);
+      s1 &= (k2 ? 0xffffu : 0u) | (k3 ? 0xffff0000u : 0u);
+    }
     const uint32_t ko = (uint32_t)(kFillRows * k * kLW);
     *reinterpret_cast<int4*>(lds + lb + tile + kOffL + ko * 4) = L;
-    *reinterpret_cast<uint2*>(lds + (lb >> 1) + tile + kOffY + ko * 2) = make_uint2(p0 << 5, p1 << 5);
+    *reinterpret_cast<uint2*>(lds + (lb >> 1) + tile + kOffY + ko * 2) = make_uint2(s0, s1);
   }
 #pragma unroll
   for (int k = 0; k < kCellSteps; k++) {
@@ -347,6 +433,46 @@ __device__ __forceinline__ void store_test(const TestRegs& f, uint32_t tb, char*
   }
   // cell pair (t / 32, 2 (t mod 32)): byte offset (64 (t / 32) + 2 (t mod 32)) * 4 = 2 tb
   *reinterpret_cast<uint2*>(lds + tb * 2 + kOffTUV) = make_uint2(uva, uvb);
+}
+
+// Loads the mask samples of the tested tile (the items of load_test): rows [y0, y0 + 16), columns
+// [x0, x0 + 128), rows past the picture clamped (their sums are dropped).
+template <typename TM>
+__device__ __forceinline__ void load_test_mask(const void* base, uint32_t bytes, const Geom& g, int x0, int y0,
+                                               TestMaskRegs& f) {
+  const Buf<TM> M(base, bytes);
+  constexpr uint32_t S = sizeof(TM);
+  const int t = threadIdx.x;
+  const int r = t >> 5, q = t & 31;
+#pragma unroll
+  for (int k = 0; k < kTestLumaSteps; k++) {
+    const uint32_t o = ((uint32_t)min(y0 + r + 8 * k, g.h - 1) * (uint32_t)g.w + (uint32_t)(x0 + 4 * q)) * S;
+    if constexpr (S == 2) {
+      const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(M.r, o, 0, 0));
+      f.m0[k] = w.x;
+      f.m1[k] = w.y;
+    } else {
+      f.m0[k] = __builtin_amdgcn_raw_buffer_load_b32(M.r, o, 0, 0);
+      f.m1[k] = 0u;
+    }
+  }
+}
+
+// Stores the tested tile's mask samples as 16-bit values: s_tm[row][col].
+template <typename TM>
+__device__ __forceinline__ void store_test_mask(const TestMaskRegs& f, uint32_t tb, uint16_t* s_tm) {
+#pragma unroll
+  for (int k = 0; k < kTestLumaSteps; k++) {
+    uint32_t p0, p1;
+    if constexpr (sizeof(TM) == 2) {
+      p0 = f.m0[k];
+      p1 = f.m1[k];
+    } else {
+      p0 = __builtin_amdgcn_perm(0u, f.m0[k], 0x0c010c00u);
+      p1 = __builtin_amdgcn_perm(0u, f.m0[k], 0x0c030c02u);
+    }
+    *reinterpret_cast<uint2*>(&s_tm[tb + (uint32_t)(8 * k * kTW)]) = make_uint2(p0, p1);
+  }
 }
 
 // Winner decode table: entry [2 (py & 1) + px][idx], idx = 5 dy + dx the window raster index of
@@ -484,10 +610,10 @@ __device__ __forceinline__ void row_pair(const RowPair& d, const uint32_t (&A)[4
 // against P0. acc[c][py]: the lane's row sums (both of its columns) kept across the tile sweep. Every
 // column of the tile lies inside the picture (the width is a multiple of the tile width); rows past it
 // are dropped with their sums. OUT: bit 0 writes the shift-compensated pels, bit 1 the winning offsets.
-template <int DIR, int OUT>
+template <int DIR, int OUT, bool MASKED>
 __device__ __forceinline__ void search_dir(const CpsArgs& a, const Geom& g, int x0, int y0, int lane, int wave,
                                            const int32_t (&gcd)[3], const char* lds, const Bases& b,
-                                           uint32_t (&acc)[3][4]) {
+                                           const uint16_t* s_tm, uint32_t (&acc)[3][4]) {
   constexpr uint32_t st = kOffPic;  // searched picture's tile
   constexpr int32_t sgn = DIR == 0 ? 1 : -1;
   const int32_t w0 = a.ws[0], w1 = a.ws[1], w2 = a.ws[2];
@@ -547,6 +673,9 @@ __device__ __forceinline__ void search_dir(const CpsArgs& a, const Geom& g, int 
 #pragma unroll
   for (int py = 0; py < 4; py++) {
     const int P = py >> 1;
+    // masked search: the mask values of the pel pair (4 wave + py, 2 lane .. 2 lane + 1)
+    uint32_t mp = 0u;
+    if constexpr (MASKED) mp = *reinterpret_cast<const uint32_t*>(&s_tm[(4 * wave + py) * kTW + 2 * lane]);
 #pragma unroll
     for (int px = 0; px < 2; px++) {
       const uint32_t ti = ((uint32_t)(best[py][px] + nh - 5 * py - px) << 2) & 124u;
@@ -560,9 +689,18 @@ __device__ __forceinline__ void search_dir(const CpsArgs& a, const Geom& g, int 
       const uint32_t d12 = pk_sub(x12[P], uvw);  // X1 - U | X2 - V << 16, |d| < 2^15
       const int y = y0 + 4 * wave + py;
       const int x = x0 + 2 * lane + px;
-      acc[0][py] = (uint32_t)mad24(d0, d0, (int32_t)acc[0][py]);
-      acc[1][py] = (uint32_t)mad16<0, 0>(d12, d12, (int32_t)acc[1][py]);
-      acc[2][py] = (uint32_t)mad16<1, 1>(d12, d12, (int32_t)acc[2][py]);
+      if constexpr (MASKED) {
+        // xCorrespPixelShiftSTD::CalcDistAsymmetricRowM: the winner's squared errors times the tested
+        // pel's mask value (squares below 2^24; the host bounds the lane sums below 2^32)
+        const uint32_t mv = px == 0 ? (mp & 0xffffu) : (mp >> 16);
+        acc[0][py] += __umul24((uint32_t)__mul24(d0, d0), mv);
+        acc[1][py] += __umul24((uint32_t)mad16<0, 0>(d12, d12, 0), mv);
+        acc[2][py] += __umul24((uint32_t)mad16<1, 1>(d12, d12, 0), mv);
+      } else {
+        acc[0][py] = (uint32_t)mad24(d0, d0, (int32_t)acc[0][py]);
+        acc[1][py] = (uint32_t)mad16<0, 0>(d12, d12, (int32_t)acc[1][py]);
+        acc[2][py] = (uint32_t)mad16<1, 1>(d12, d12, (int32_t)acc[2][py]);
+      }
       if constexpr ((OUT & 3) != 0) {
         if (x < g.w && y < g.h) {
           const uint32_t o = __umul24((uint32_t)y, (uint32_t)g.w) + (uint32_t)x;
@@ -586,10 +724,10 @@ __device__ __forceinline__ void search_dir(const CpsArgs& a, const Geom& g, int 
 }
 
 // The tile sweep of one direction: DIR 0 searches P1 for P0 + gcd (R2T), DIR 1 P0 for P1 - gcd (T2R).
-template <typename T, int OUT, int DIR>
-__device__ __forceinline__ void sweep_dir(const CpsArgs& a, const Geom& g, char* lds, int y0, int tx_begin, int tx_end,
-                                          int lane, int wave, const int32_t (&gcd)[3], uint32_t w0s, uint32_t w12,
-                                          uint32_t (&acc)[3][4]) {
+template <typename T, typename TM, bool MASKED, int OUT, int DIR>
+__device__ __forceinline__ void sweep_dir(const CpsArgs& a, const Geom& g, char* lds, uint16_t* s_tm, int y0,
+                                          int tx_begin, int tx_end, int lane, int wave, const int32_t (&gcd)[3],
+                                          uint32_t w0s, uint32_t w12, uint32_t (&acc)[3][4]) {
   const void* sp = DIR == 0 ? a.p1 : a.p0;  // searched picture
   const void* tp = DIR == 0 ? a.p0 : a.p1;  // tested picture
   Items it0;
@@ -597,10 +735,16 @@ __device__ __forceinline__ void sweep_dir(const CpsArgs& a, const Geom& g, char*
   it0.q = (int)threadIdx.x - ((int)threadIdx.x / kQW) * kQW;
   PicRegs f;
   TestRegs ft;
+  MaskRegs fm;
+  TestMaskRegs ftm;
   if (tx_begin < tx_end) {
     const int x0 = tx_begin * kTW;
     load_pic<T>(sp, g, it0, x0, y0, tile_interior(g, x0, y0), f);
     load_test<T>(tp, g, x0, y0, ft);
+    if constexpr (MASKED) {
+      load_mask<TM>(a.mask, a.mask_bytes, g, it0, x0, y0, tile_interior(g, x0, y0), fm);
+      load_test_mask<TM>(a.mask, a.mask_bytes, g, x0, y0, ftm);
+    }
   }
   for (int tx = tx_begin; tx < tx_end; tx++) {
     const int x0 = tx * kTW;
@@ -617,12 +761,17 @@ __device__ __forceinline__ void sweep_dir(const CpsArgs& a, const Geom& g, char*
     tt &= 255u;
     const uint32_t tb = (tt >> 5) * (uint32_t)kTW + 4u * (tt & 31u);  // tested item: sample (t / 32, 4 (t mod 32))
     __syncthreads();  // the previous tile's searches are done with LDS
-    store_pic<T>(f, it, lb, hc, w0s, w12, lds);
+    store_pic<T, TM, MASKED>(f, fm, it, lb, hc, w0s, w12, lds);
     store_test<T>(ft, tb, lds);
+    if constexpr (MASKED) store_test_mask<TM>(ftm, tb, s_tm);
     __syncthreads();
     if (tx + 1 < tx_end) {
       load_pic<T>(sp, g, it, x0 + kTW, y0, tile_interior(g, x0 + kTW, y0), f);
       load_test<T>(tp, g, x0 + kTW, y0, ft);
+      if constexpr (MASKED) {
+        load_mask<TM>(a.mask, a.mask_bytes, g, it, x0 + kTW, y0, tile_interior(g, x0 + kTW, y0), fm);
+        load_test_mask<TM>(a.mask, a.mask_bytes, g, x0 + kTW, y0, ftm);
+      }
     }
     // The lane's bases, formed in the loop so that the compiler keeps a few registers rather than
     // one per access.
@@ -635,17 +784,22 @@ __device__ __forceinline__ void sweep_dir(const CpsArgs& a, const Geom& g, char*
     b.c = ln * 8u + (uint32_t)wave * (2 * kCW * 8);
     b.t = ln * 4u + (uint32_t)wave * (4 * kTW * 2);
     b.tc = ln * 4u + (uint32_t)wave * (2 * (kTW / 2) * 4);
-    search_dir<DIR, OUT>(a, g, x0, y0, (int)ln, wave, gcd, lds, b, acc);
+    search_dir<DIR, OUT, MASKED>(a, g, x0, y0, (int)ln, wave, gcd, lds, b, s_tm, acc);
   }
 }
 
-template <typename T, int OUT>
+template <typename T, typename TM, bool MASKED, int OUT>
 __global__ __launch_bounds__(kThreads, 4) void k_cps420s(CpsArgs a) {
   __shared__ Lds smem;
+  __shared__ __attribute__((aligned(16))) uint16_t s_tm[MASKED ? kTH * kTW : 8];  // tested tile's mask
   char* lds = reinterpret_cast<char*>(&smem);
 
   // Nothing to do when a pel exceeds the bit depth: the host reports that error.
-  if (!a.sum->in_range) return;
+  if constexpr (MASKED) {
+    if (!a.msum->all_in_range) return;
+  } else {
+    if (!a.sum->in_range) return;
+  }
   const Geom g = a.g;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) & 3;  // rows 4 wave .. 4 wave + 3
@@ -661,7 +815,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_cps420s(CpsArgs a) {
   const int y0 = band * kTH;
   const int tx_begin = seg * a.sweep;
   const int tx_end = min(a.tiles_x, tx_begin + a.sweep);
-  const int32_t gcd[3] = {a.sum->gcd[0], a.sum->gcd[1], a.sum->gcd[2]};
+  const int32_t* gsrc = MASKED ? a.msum->gcd : a.sum->gcd;
+  const int32_t gcd[3] = {gsrc[0], gsrc[1], gsrc[2]};
   const uint32_t w0s = 32u * (uint32_t)a.ws[0];
   const uint32_t w12 = (uint32_t)a.ws[1] | ((uint32_t)a.ws[2] << 16);
   init_decode_table(smem.tbl);
@@ -673,8 +828,27 @@ __global__ __launch_bounds__(kThreads, 4) void k_cps420s(CpsArgs a) {
   for (int c = 0; c < 3; c++)
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[c][r] = 0u;
-  if (t2r == 0) sweep_dir<T, OUT, 0>(a, g, lds, y0, tx_begin, tx_end, lane, wave, gcd, w0s, w12, acc);
-  else sweep_dir<T, OUT, 1>(a, g, lds, y0, tx_begin, tx_end, lane, wave, gcd, w0s, w12, acc);
+  if (t2r == 0) sweep_dir<T, TM, MASKED, OUT, 0>(a, g, lds, s_tm, y0, tx_begin, tx_end, lane, wave, gcd, w0s, w12, acc);
+  else sweep_dir<T, TM, MASKED, OUT, 1>(a, g, lds, s_tm, y0, tx_begin, tx_end, lane, wave, gcd, w0s, w12, acc);
+
+  if constexpr (MASKED) {
+    // The segment's masked sums per (component, row): 64-bit wave totals from the lane sums' 16-bit
+    // halves, written to the segment's partials (one workgroup per segment, direction and band). The
+    // masked processor's R2T is the search of P1 - g in P0 (xIVPSNR.cpp:152-165): the second
+    // direction here.
+    const int d = t2r ? 0 : 1;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int y = y0 + 4 * wave + r;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const uint64_t lo = wave_total_u32(acc[c][r] & 0xffffu);
+        const uint64_t hi = wave_total_u32(acc[c][r] >> 16);
+        if (lane == 63 && y < g.h) a.partials[(((int64_t)seg * 2 + d) * 3 + c) * g.h + y] = lo + (hi << 16);
+      }
+    }
+    return;
+  }
 
   // One sum per (component, row) of the workgroup's direction: reduce over the wave's 64 lanes in
   // 32 bits, then add it into the row (integer sums: any order gives the same).
@@ -702,7 +876,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_cps420s(CpsArgs a) {
 hipError_t launch_cps420s(const CpsArgs& a, hipStream_t stream) {
   if (a.g.cs != 1 || a.g.w % kTW != 0 || (a.g.h & 1) || a.g.w < kTW || a.g.h < 2 || a.g.bd > 10)
     return hipErrorInvalidValue;
-  if (a.rows == nullptr || a.mask != nullptr) return hipErrorInvalidValue;
+  const bool masked = a.mask != nullptr;
+  // The unmasked search adds into rows; the masked one writes per-segment partials.
+  if (masked ? (a.partials == nullptr || a.msum == nullptr) : a.rows == nullptr) return hipErrorInvalidValue;
   const int bands = (a.g.h + kTH - 1) / kTH;
   if (a.tiles_x != (a.g.w + kTW - 1) / kTW || a.sweep < 1 || a.nseg != (a.tiles_x + a.sweep - 1) / a.sweep)
     return hipErrorInvalidValue;
@@ -710,19 +886,30 @@ hipError_t launch_cps420s(const CpsArgs& a, hipStream_t stream) {
   if ((a.scp0 == nullptr) != (a.scp1 == nullptr) || (a.best0 == nullptr) != (a.best1 == nullptr))
     return hipErrorInvalidValue;
   const int out = (a.scp0 ? 1 : 0) | (a.best0 ? 2 : 0);
+  if (masked) {  // no optional outputs
+    if (out != 0 || (a.mask_bytes_per_sample != 1 && a.mask_bytes_per_sample != 2)) return hipErrorInvalidValue;
+    if (a.g.bytes_per_sample == 1) {
+      if (a.mask_bytes_per_sample == 1) k_cps420s<uint8_t, uint8_t, true, 0><<<grid, block, 0, stream>>>(a);
+      else k_cps420s<uint8_t, uint16_t, true, 0><<<grid, block, 0, stream>>>(a);
+    } else {
+      if (a.mask_bytes_per_sample == 1) k_cps420s<uint16_t, uint8_t, true, 0><<<grid, block, 0, stream>>>(a);
+      else k_cps420s<uint16_t, uint16_t, true, 0><<<grid, block, 0, stream>>>(a);
+    }
+    return hipGetLastError();
+  }
   if (a.g.bytes_per_sample == 1) {
     switch (out) {
-      case 0: k_cps420s<uint8_t, 0><<<grid, block, 0, stream>>>(a); break;
-      case 1: k_cps420s<uint8_t, 1><<<grid, block, 0, stream>>>(a); break;
-      case 2: k_cps420s<uint8_t, 2><<<grid, block, 0, stream>>>(a); break;
-      default: k_cps420s<uint8_t, 3><<<grid, block, 0, stream>>>(a); break;
+      case 0: k_cps420s<uint8_t, uint8_t, false, 0><<<grid, block, 0, stream>>>(a); break;
+      case 1: k_cps420s<uint8_t, uint8_t, false, 1><<<grid, block, 0, stream>>>(a); break;
+      case 2: k_cps420s<uint8_t, uint8_t, false, 2><<<grid, block, 0, stream>>>(a); break;
+      default: k_cps420s<uint8_t, uint8_t, false, 3><<<grid, block, 0, stream>>>(a); break;
     }
   } else {
     switch (out) {
-      case 0: k_cps420s<uint16_t, 0><<<grid, block, 0, stream>>>(a); break;
-      case 1: k_cps420s<uint16_t, 1><<<grid, block, 0, stream>>>(a); break;
-      case 2: k_cps420s<uint16_t, 2><<<grid, block, 0, stream>>>(a); break;
-      default: k_cps420s<uint16_t, 3><<<grid, block, 0, stream>>>(a); break;
+      case 0: k_cps420s<uint16_t, uint8_t, false, 0><<<grid, block, 0, stream>>>(a); break;
+      case 1: k_cps420s<uint16_t, uint8_t, false, 1><<<grid, block, 0, stream>>>(a); break;
+      case 2: k_cps420s<uint16_t, uint8_t, false, 2><<<grid, block, 0, stream>>>(a); break;
+      default: k_cps420s<uint16_t, uint8_t, false, 3><<<grid, block, 0, stream>>>(a); break;
     }
   }
   return hipGetLastError();
@@ -736,8 +923,8 @@ void cps420s_tile(int32_t* tile_w, int32_t* tile_h) {
 int cps420s_blocks_per_cu(int bytes_per_sample) {
   int n = 0;
   hipError_t e = bytes_per_sample == 1
-                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps420s<uint8_t, 0>, kThreads, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps420s<uint16_t, 0>, kThreads, 0);
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps420s<uint8_t, uint8_t, false, 0>, kThreads, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_cps420s<uint16_t, uint8_t, false, 0>, kThreads, 0);
   return e == hipSuccess && n > 0 ? n : 1;
 }
 

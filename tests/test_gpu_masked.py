@@ -93,6 +93,41 @@ def test_frame_masked_matches_oracle(engine, oracle, w, h, bd, cf, bdm, kind, er
     assert r["ivpsnr_r2t"] == o["r2t"] and r["ivpsnr_t2r"] == o["t2r"]
 
 
+@pytest.mark.parametrize("w,h,bd,bdm,kind,erp", [
+    (128, 16, 8, 8, "objects", False), (256, 64, 8, 8, "speckle", True), (384, 40, 10, 8, "soft", False),
+    (1024, 34, 10, 10, "speckle", False), (512, 66, 10, 10, "soft", True), (256, 64, 10, 16, "speckle", False),
+    (256, 62, 8, 16, "soft", False), (1152, 18, 8, 8, "full", False),
+])
+def test_sixteen_bit_masked_search_matches_oracle(engine, oracle, w, h, bd, bdm, kind, erp):
+    """4:2:0 pictures of 8 and 10 bits whose width is a multiple of 128 run the masked search on
+    the 16-bit kernel while every lane's masked sum of a sweep stays below 2^32 (2 pels per row
+    and tile, each square times a mask value): 8- and 10-bit masks, heights that leave a partial
+    band (40, 34, 66, 18), several segments per row (a 10-bit mask over 10-bit pictures allows one tile per sweep). A
+    16-bit mask over these bit depths exceeds the bound and takes the 32-bit kernel: the same
+    values."""
+    print(f"masked s16 {w}x{h} bd{bd} bdm{bdm} {kind} erp{int(erp)}", flush=True)
+    pair = make_pair(w, h, bd, 91, luma_offset=3.0)
+    mask = make_mask(w, h, bdm, 91, kind=kind)
+    p0 = engine.picture(w, h, bd, 420, data=pair.ref)
+    p1 = engine.picture(w, h, bd, 420, data=pair.tst)
+    pm = engine.picture(w, h, bdm, 420, data=mask)
+    prm = default_params(erp=erp)
+    engine.profile(True)
+    engine.profile_read(reset=True)
+    r = engine.frame_masked(p0, p1, pm, PSNRS, prm).as_dict()
+    prof = engine.profile_read(reset=True)
+    engine.profile(False)
+    # the fast search was planned (a 10-bit frame also enqueues the general search, gated to run
+    # only if a pel is out of range, which returns at once here)
+    assert prof.get("cps", (0, 0))[1] == 1, prof
+    o = oracle.frame_masked(pair.ref, pair.tst, mask, bdm, 420,
+                            oracle.params(w, h, bd, 420, metrics=PSNRS, erp=erp)).as_dict()
+    for k in ("num_non_masked", "gcd", "in_range", "exact", "psnr", "wspsnr", "psnr_pic", "wspsnr_pic", "ivpsnr",
+              "cps_sum"):
+        assert r[k] == o[k], k
+    assert r["ivpsnr_r2t"] == o["r2t"] and r["ivpsnr_t2r"] == o["t2r"]
+
+
 @pytest.mark.parametrize("w,h,cf,cfm,bd,bdm", [
     (256, 96, 420, 444, 10, 8), (256, 96, 444, 420, 10, 8), (256, 96, 444, 444, 8, 16), (252, 98, 444, 444, 12, 10),
     (254, 98, 420, 420, 10, 8), (130, 40, 444, 420, 8, 8),
@@ -115,13 +150,16 @@ def test_chroma_formats_match_oracle(engine, oracle, w, h, cf, cfm, bd, bdm):
         assert r[k] == o[k], k
 
 
-@pytest.mark.parametrize("bd,cf,g", [(8, 420, [200, -150, 90, 0]), (10, 420, [-300, 250, -280, 0]),
-                                     (10, 444, [300, -260, 120, 0])])
-def test_large_given_gcd_matches_oracle(engine, oracle, bd, cf, g):
+@pytest.mark.parametrize("w,bd,cf,g", [(200, 8, 420, [200, -150, 90, 0]), (200, 10, 420, [-300, 250, -280, 0]),
+                                       (200, 10, 444, [300, -260, 120, 0]), (256, 8, 420, [200, -150, 90, 0]),
+                                       (256, 10, 420, [-300, 250, -280, 0])])
+def test_large_given_gcd_matches_oracle(engine, oracle, w, bd, cf, g):
     """A caller-given GCD far beyond the unnoticeable bound: tested values well outside [0, max]
     (no clipping, xc_CLIP_CURR_TST_RANGE = false) still fit the fast kernels' key bounds, so the
-    fast masked search runs and must equal the oracle's search."""
-    w, h = 200, 72
+    fast masked search runs and must equal the oracle's search. Width 200 takes the 32-bit 4:2:0
+    kernel, width 256 (a multiple of its 128-pel tile) the 16-bit one, whose sweep the larger
+    differences shorten (10 bits: 4 tiles per lane sum)."""
+    h = 72
     pair = make_pair(w, h, bd, 81, chroma_format=cf, luma_offset=1.0)
     mask = make_mask(w, h, 8, 81, chroma_format=cf, kind="speckle")
     P0, P1 = oracle.unpack(pair.ref, w, h, bd, cf), oracle.unpack(pair.tst, w, h, bd, cf)
