@@ -573,8 +573,9 @@ def main():
             ha[:] = a
             hb[:] = b
             src_pinned.append((ha, hb))
-        got = eng.frames_upload(pics, src_pinned, metrics, prm)
-        assert [getattr(r, key) for r in got] == [getattr(r, key) for r in res]
+        for src in (src_pinned, src_pageable):
+            got = eng.frames_upload(pics, src, metrics, prm)
+            assert [getattr(r, key) for r in got] == [getattr(r, key) for r in res]
         el_up = timed(lambda: eng.frames_upload(pics, src_pinned, metrics, prm), up_steps, 1, False)[0]
         el_up_pageable = timed(lambda: eng.frames_upload(pics, src_pageable, metrics, prm), up_steps, 1, False)[0]
     prof_in_set = prof

@@ -145,8 +145,9 @@ void* qmiv_pic_device_ptr(qmiv_pic* pic);
 /* Copies one packed frame from host memory (qmiv_frame_bytes bytes) onto the device with one
    hipMemcpyAsync ordered on the context's stream (work enqueued afterwards sees the frame). From
    pinned host memory the copy is asynchronous: keep the memory unchanged until the stream has
-   passed it (qmiv_sync). From pageable memory the HIP runtime stages the data through its own
-   pinned buffers and the call returns once the source may be reused. Replaces
+   passed it (qmiv_sync). From pageable memory the data goes through the context's pinned staging
+   slots (four of 8 MiB, created on first use; host threads fill a slot while the previous one is
+   copied) and the call returns once the source has been read in full. Replaces
    xSeqBase::readFrame's unpack step (libs/pmbb-core/src/xSeq.cpp:179-249): the engine keeps the
    packed layout. */
 qmiv_status qmiv_pic_upload(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes);
@@ -288,8 +289,8 @@ qmiv_status qmiv_frames(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
    and frame i's kernels wait for its own two copies only, so the copies of frame i + 1 run while
    frame i is computed. Replaces the reference application's read-then-process frame loop
    (libs/pmbb-core/src/xSeq.cpp:155-174 per frame, apps/QMIV/src/xAppQMIV.cpp:666-748). The copies
-   run at the link's rate from pinned memory (qmiv_host_alloc); from ordinary memory the runtime
-   stages them and the call returns later. No picture may appear twice among the 2n pictures
+   run at the link's rate from pinned memory (qmiv_host_alloc); from ordinary memory through the
+   context's pinned staging slots, as qmiv_pic_upload. No picture may appear twice among the 2n pictures
    (QMIV_ERR_INVALID_ARG). The sources are no longer read once the call has returned. */
 qmiv_status qmiv_frames_upload(qmiv_ctx* ctx, int32_t n, qmiv_pic* const* p0, qmiv_pic* const* p1,
                                const void* const* h0, const void* const* h1, const qmiv_params* prm,

@@ -911,7 +911,7 @@ qmiv_status finalize_frame(qmiv_ctx* ctx, const Geom& g, const qmiv_params& prm,
 // Batched frame sets: every view's work captured once into one HIP graph with concurrent
 // branches, replayed per frame set.
 // =============================================================================================
-namespace {
+namespace qmiv {
 
 // A few persistent host threads for a parallel loop over independent items (the host finalisation
 // of a frame set's views); the calling thread takes items too.
@@ -972,6 +972,83 @@ class HostPool {
   uint64_t gen_ = 0;
   bool stop_ = false;
 };
+
+// Pinned staging of host-to-device copies from pageable memory. A copy from pageable memory is
+// split into slot-sized chunks; the pool's threads copy a chunk into the next pinned slot together
+// and the slot goes to the device on the copy's stream while the host fills the following slot. A
+// slot is refilled once its previous transfer has completed (its event).
+struct Stager {
+  static constexpr int kSlots = 4;
+  static constexpr size_t kSlotBytes = (size_t)8 << 20;
+  static constexpr size_t kPiece = (size_t)1 << 20;  // one thread's share of a chunk
+  explicit Stager(int workers) : pool(workers) {}
+  HostPool pool;
+  void* slot[kSlots] = {};
+  hipEvent_t ev[kSlots] = {};
+  bool used[kSlots] = {};
+  int next = 0;
+};
+
+void destroy_stager(Stager* s) {
+  if (!s) return;
+  for (int k = 0; k < Stager::kSlots; k++) {
+    if (s->used[k]) (void)hipEventSynchronize(s->ev[k]);
+    if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    if (s->slot[k]) (void)hipHostFree(s->slot[k]);
+  }
+  delete s;
+}
+
+}  // namespace qmiv
+
+namespace {
+
+// Host-to-device copy of `bytes` from caller memory, ordered on stream st. Pinned memory goes to
+// the copy engine directly (asynchronous: the caller keeps it unchanged until st has passed the
+// copy); pageable memory through the context's pinned staging slots, and the source has been read
+// in full when the call returns.
+qmiv_status host_to_device(qmiv_ctx* ctx, void* dev, const void* src, size_t bytes, hipStream_t st) {
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, src) == hipSuccess && at.type == hipMemoryTypeHost) {
+    QMIV_HIP(hipMemcpyAsync(dev, src, bytes, hipMemcpyHostToDevice, st));
+    return QMIV_OK;
+  }
+  (void)hipGetLastError();  // an unregistered pointer leaves an error behind
+  if (!ctx->stager) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    std::unique_ptr<Stager> s(new Stager((int)std::min(7u, hw - 1)));
+    for (int k = 0; k < Stager::kSlots; k++) {
+      if (hipHostMalloc(&s->slot[k], Stager::kSlotBytes, hipHostMallocDefault) != hipSuccess) {
+        s->slot[k] = nullptr;
+        destroy_stager(s.release());
+        return fail(QMIV_ERR_OUT_OF_MEMORY, "pinned staging allocation failed");
+      }
+      if (hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming) != hipSuccess) {
+        destroy_stager(s.release());
+        return fail(QMIV_ERR_HIP, "staging event creation failed");
+      }
+    }
+    ctx->stager = s.release();
+  }
+  Stager& S = *ctx->stager;
+  for (size_t off = 0; off < bytes; off += Stager::kSlotBytes) {
+    const size_t len = std::min(Stager::kSlotBytes, bytes - off);
+    const int k = S.next;
+    S.next = (k + 1) % Stager::kSlots;
+    if (S.used[k]) QMIV_HIP(hipEventSynchronize(S.ev[k]));
+    char* dst = (char*)S.slot[k];
+    const char* from = (const char*)src + off;
+    const int pieces = (int)((len + Stager::kPiece - 1) / Stager::kPiece);
+    S.pool.run(pieces, [&](int i) {
+      const size_t o = (size_t)i * Stager::kPiece;
+      std::memcpy(dst + o, from + o, std::min(Stager::kPiece, len - o));
+    });
+    QMIV_HIP(hipMemcpyAsync((char*)dev + off, dst, len, hipMemcpyHostToDevice, st));
+    QMIV_HIP(hipEventRecord(S.ev[k], st));
+    S.used[k] = true;
+  }
+  return QMIV_OK;
+}
 
 }  // namespace
 
@@ -1345,6 +1422,7 @@ void qmiv_destroy(qmiv_ctx* ctx) {
     }
   if (ctx->h_results) (void)hipHostFree(ctx->h_results);
   if (ctx->h_fixed) (void)hipHostFree(ctx->h_fixed);
+  destroy_stager(ctx->stager);
   for (auto& pr : ctx->ev_pool) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
@@ -1418,8 +1496,8 @@ void* qmiv_pic_device_ptr(qmiv_pic* pic) { return pic ? pic->dev : nullptr; }
 qmiv_status qmiv_pic_upload(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes) {
   if (!ctx || !pic || !host) return fail(QMIV_ERR_INVALID_ARG, "null argument");
   if (bytes != pic->g.frame_bytes) return fail(QMIV_ERR_INVALID_ARG, "byte count differs from the frame size");
-  QMIV_HIP(hipMemcpyAsync(pic->dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
-  return QMIV_OK;
+  QMIV_HIP(hipSetDevice(ctx->device));
+  return host_to_device(ctx, pic->dev, host, bytes, ctx->stream);
 }
 
 qmiv_status qmiv_pic_upload_unpack(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes,
@@ -1626,12 +1704,9 @@ qmiv_status qmiv::upload_file_frame(qmiv_ctx* ctx, qmiv_pic* pic, const void* ho
   if (bytes != file_frame_bytes(g.w, g.h, g.bd, fcf))
     return fail(QMIV_ERR_INVALID_ARG, "byte count differs from the file frame size");
   QMIV_HIP(hipSetDevice(ctx->device));
-  if (!unpack) {
-    QMIV_HIP(hipMemcpyAsync(pic->dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
-    return QMIV_OK;
-  }
+  if (!unpack) return host_to_device(ctx, pic->dev, host, bytes, ctx->stream);
   QMIV_TRY(ensure_stage(ctx, bytes));
-  QMIV_HIP(hipMemcpyAsync(ctx->d_stage, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  QMIV_TRY(host_to_device(ctx, ctx->d_stage, host, bytes, ctx->stream));
   QMIV_HIP(launch_unpack(ctx->d_stage, bytes, g.w, g.h, g.bd, fcf, pic->dev, ctx->stream));
   return QMIV_OK;
 }
@@ -1750,12 +1825,13 @@ qmiv_status frames_impl(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
     const int32_t k = i % nstreams;
     qmiv_status st = QMIV_OK;
     if (h0) {  // xSeq::readFrame per frame (libs/pmbb-core/src/xSeq.cpp:155-174), here an async copy
-      hipError_t e = hipMemcpyAsync(p0[i]->dev, h0[i], p0[i]->g.frame_bytes, hipMemcpyHostToDevice, ctx->copy_stream);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(p1[i]->dev, h1[i], p1[i]->g.frame_bytes, hipMemcpyHostToDevice, ctx->copy_stream);
-      if (e == hipSuccess) e = hipEventRecord(ctx->copy_ev[(size_t)i], ctx->copy_stream);
-      if (e == hipSuccess) e = hipStreamWaitEvent(streams[k], ctx->copy_ev[(size_t)i], 0);
-      if (e != hipSuccess) st = fail(QMIV_ERR_HIP, std::string("frame upload: ") + hipGetErrorString(e));
+      st = host_to_device(ctx, p0[i]->dev, h0[i], p0[i]->g.frame_bytes, ctx->copy_stream);
+      if (st == QMIV_OK) st = host_to_device(ctx, p1[i]->dev, h1[i], p1[i]->g.frame_bytes, ctx->copy_stream);
+      if (st == QMIV_OK) {
+        hipError_t e = hipEventRecord(ctx->copy_ev[(size_t)i], ctx->copy_stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(streams[k], ctx->copy_ev[(size_t)i], 0);
+        if (e != hipSuccess) st = fail(QMIV_ERR_HIP, std::string("frame upload: ") + hipGetErrorString(e));
+      }
     }
     if (st == QMIV_OK)
       st = enqueue_frame(ctx, p0[i], p1[i], prm, fw, plans[(size_t)i], (char*)ctx->d_scratch + doff[(size_t)i],

@@ -25,6 +25,9 @@ qmiv_status file_route(const qmiv_pic* pic, int32_t w, int32_t h, int32_t bd, in
 // Enqueues on the context's stream the upload of one packed raw-file frame (host memory) into pic:
 // one copy, then the unpack when the formats differ (through the context's device staging buffer).
 qmiv_status upload_file_frame(qmiv_ctx* ctx, qmiv_pic* pic, const void* host, size_t bytes, int32_t fcf);
+// Pinned staging slots and host threads of the context's uploads from pageable memory.
+struct Stager;
+void destroy_stager(Stager* s);
 }  // namespace qmiv
 
 struct qmiv_pic {
@@ -73,6 +76,9 @@ struct qmiv_ctx {
   // (planar or interleaved uint16) uploads and downloads.
   void* d_stage = nullptr;
   size_t stage_bytes = 0;
+  // Host-to-device copies from pageable memory go through pinned slots filled by host threads
+  // (created on the first such copy).
+  qmiv::Stager* stager = nullptr;
   // Shift-compensated pictures (IV-SSIM), 2 x [3][h][w] uint16.
   void* d_scp = nullptr;
   size_t scp_bytes = 0;

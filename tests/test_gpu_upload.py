@@ -1,5 +1,7 @@
 """Frame sets with their uploads (qmiv_frames_upload): every frame's copies on the copy stream,
-overlapping the frames before it, from pinned (qmiv_host_alloc) and from ordinary host memory.
+overlapping the frames before it, from pinned (qmiv_host_alloc) and from ordinary host memory
+(staged through the context's four pinned slots of 8 MiB: a 4096x2048 10-bit frame takes three
+chunks per picture, so the slots are reused while earlier transfers may still run).
 
 The results must equal the resident path's (qmiv_frames over pictures uploaded beforehand) field
 by field, every double included; one frame is checked against the oracle as well.
@@ -16,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 ALL = PSNR | WSPSNR | IVPSNR | SSIM | MSSSIM | IVSSIM
 FRAMES = [(256, 128, 10, 420, 1), (130, 66, 10, 420, 2), (67, 45, 8, 444, 3), (256, 128, 10, 420, 4),
-          (1920, 1080, 10, 420, 5)]
+          (1920, 1080, 10, 420, 5), (4096, 2048, 10, 420, 6)]
 
 
 def _same(a, b):
@@ -74,3 +76,26 @@ def test_picture_twice_is_rejected(engine):
     p0, p1 = _pics(engine, [pair])[0]
     with pytest.raises(QmivError, match="twice"):
         engine.frames_upload([(p0, p1), (p0, p1)], [(pair.ref, pair.tst)] * 2, PSNR)
+
+
+def test_pageable_uploads_through_staging(engine):
+    """qmiv_pic_upload from ordinary host memory goes through the pinned staging slots of 8 MiB:
+    4000x2000 10-bit pictures (24.0 MB: two full chunks and a partial one, whose last 1 MiB piece
+    is partial too) come back unchanged several times in a row, and so does a source starting at an
+    odd address."""
+    rng = np.random.default_rng(5)
+    w, h = 4000, 2000
+    pics = [engine.picture(w, h, 10, 420) for _ in range(2)]
+    for rep in range(3):
+        datas = [rng.integers(0, 1024, w * h * 3 // 2, dtype=np.uint16) for _ in pics]
+        for p, d in zip(pics, datas):
+            p.upload(d)
+        for p, d in zip(pics, datas):
+            assert np.array_equal(p.download(), d), rep
+    w8, h8 = 1920, 1080
+    raw = rng.integers(0, 256, w8 * h8 * 3 // 2 + 1, dtype=np.uint8)
+    src = raw[1:]
+    assert src.ctypes.data % 2 == 1
+    p = engine.picture(w8, h8, 8, 420)
+    p.upload(src)
+    assert np.array_equal(p.download(), src)
