@@ -262,9 +262,10 @@ def run_grad(args):
     el = time.perf_counter() - t0
     ms_per_step = el / args.steps * 1e3
     mpix = n * h * w / 1e6
-    # fp64 flops per picture pair: 6 forward evaluations (value) + 3 forward with maps + 3 gradient
-    # correlations (4 maps x 100 taps x 2), per interior / whole pel
-    flops = n * c * ((6 + 3) * 1000 * (h - 10) * (w - 10) + 3 * 800 * h * w) / 3
+    # fp64 flops per picture pair: 6 forward evaluations (the values of both directions; the maps of
+    # the chosen direction come with them) + 3 gradient correlations (4 maps x 100 taps x 2), per
+    # interior / whole pel
+    flops = n * c * (6 * 1000 * (h - 10) * (w - 10) + 3 * 800 * h * w) / 3
     achieved = flops / (ms_per_step / 1e3) / 1e12
     out = {
         "metric": "Differentiable IV-SSIM forward+backward Mpix/s @3x1024x1024 fp32, batch 8",
@@ -293,6 +294,7 @@ def run_masked(args):
     pairs = [make_pair(W, H, BD, seed=1 + f) for f in range(nframes)]
     masks = [make_mask(W, H, 8, seed=1 + f, kind="objects") for f in range(nframes)]
     prm = default_params()
+    prm.search_range = args.search_range
     eng = Engine(0)
     pics = [(eng.picture(W, H, BD, 420, data=p.ref), eng.picture(W, H, BD, 420, data=p.tst),
              eng.picture(W, H, 8, 420, data=m)) for p, m in zip(pairs, masks)]
@@ -327,7 +329,8 @@ def run_masked(args):
     if args.legs == "profile":
         print(f"[bench] masked per-frame profile leg: warmup 1, {prof_steps} steps", file=sys.stderr, flush=True)
         el_p, prof_alone, _ = timed(prof_steps, 1, True, step_loop)
-        print(json.dumps({"legs": "profile", "steps": prof_steps, "mpix_s_profiled_plain_loop":
+        print(json.dumps({"legs": "profile", "steps": prof_steps, "search_range": args.search_range,
+                          "mpix_s_profiled_plain_loop":
                           round(mpix / (el_p / prof_steps), 2),
                           "kernels_us_per_frame_alone": {k: round(v[0] / nframes / prof_steps * 1e3, 2)
                                                          for k, v in prof_alone.items()}}))
@@ -348,9 +351,10 @@ def run_masked(args):
     else:
         prof_steps = args.steps
     ms_per_step = elapsed / args.steps * 1e3
-    k_ms, k_n = prof.get("cps", (0.0, 0))
+    kkey = "cps" if args.search_range == 2 else "cps_general"
+    k_ms, k_n = prof.get(kkey, (0.0, 0))
     k_avg_ms = k_ms / max(k_n, 1)
-    k_set_ms, k_set_n = prof_in_set.get("cps", (0.0, 0))
+    k_set_ms, k_set_n = prof_in_set.get(kkey, (0.0, 0))
     per_launch = frame_pair_bytes(W, H, BD) + W * H  # two pictures and the mask's luma plane
     achieved = per_launch / (k_avg_ms / 1e3) / 1e9 if k_avg_ms > 0 else 0.0
     cpu = None
@@ -388,14 +392,16 @@ def run_masked(args):
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int32", "data": "synthetic",
         "config": {"workload": "masked PSNR+WS-PSNR+IV-PSNR, 8 frame pairs 1920x1080 yuv420p, 8-bit object "
-                               "masks, B=2, GCD on", "frames_per_step": nframes,
+                               f"masks, B={args.search_range}, GCD on", "frames_per_step": nframes,
                    "mode": "stream (qmiv_frames_masked: frame set enqueued over three streams, one wait)",
                    "plain_loop_mpix_s": round(plain_mpix_s, 2) if plain_mpix_s else None,
                    "stream_vs_plain": round(mpix / (ms_per_step / 1e3) / plain_mpix_s, 3) if plain_mpix_s else None,
                    "ivpsnr_last_frame": round(res[-1].ivpsnr, 6), "nnm_last_frame": res[-1].num_non_masked},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": read_traffic("k_cps420s_masked_1920x1080_8b.txt"),
-                     "kernel": "k_cps420s (masked)", "kernel_avg_us": round(k_avg_ms * 1e3, 2),
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": read_traffic("k_cps420s_masked_1920x1080_8b.txt") if kkey == "cps" else None,
+                     "kernel": "k_cps420s (masked)" if kkey == "cps" else "k_cps_masked",
+                     "kernel_avg_us": round(k_avg_ms * 1e3, 2),
                      "kernel_avg_us_in_set": round(k_set_ms / max(k_set_n, 1) * 1e3, 2),
                      "timing": "kernel_avg_us: HIP events around each launch with the frame's kernels alone on the "
                                "device (per-frame leg); kernel_avg_us_in_set: inside the timed frame-set steps",
@@ -420,6 +426,9 @@ def main():
     ap.add_argument("--frames", type=int, default=0, help="frame pairs per step (default: the workload's)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 512 MiB (calibrates rocprof byte counters)")
+    ap.add_argument("--search-range", type=int, default=2,
+                    help="masked-1080p8 only: the CPS search range B (any B other than 2 runs the general masked "
+                         "search kernel k_cps_masked; the line is then not the workload's figure)")
     ap.add_argument("--legs", choices=["all", "stream", "profile"], default="all",
                     help="stream: run only the timed frame-set steps (no plain loop, no upload leg); profile: run "
                          "only the per-frame leg whose kernel durations the roofline figure is taken from, so that "
@@ -494,7 +503,8 @@ def main():
         print(f"[bench] per-frame profile leg: warmup 1, {prof_steps} steps", file=sys.stderr, flush=True)
         el_p, prof_alone, _ = timed(step_plain, prof_steps, 1, True)
         mpix = nframes * W * H / 1e6
-        print(json.dumps({"legs": "profile", "steps": prof_steps, "mpix_s_profiled_plain_loop":
+        print(json.dumps({"legs": "profile", "steps": prof_steps, "search_range": args.search_range,
+                          "mpix_s_profiled_plain_loop":
                           round(mpix / (el_p / prof_steps), 2),
                           "kernels_us_per_frame_alone": {k: round(v[0] / nframes / prof_steps * 1e3, 2)
                                                          for k, v in prof_alone.items()}}))

@@ -53,5 +53,36 @@ __device__ __forceinline__ int32_t mad24(int32_t a, int32_t b, int32_t c) {
   return d;
 }
 
+// The SSIM window F is symmetric: F[a][b] = F[b][a], and F[5 + d][b] = F[5 - d][b] (a Gaussian of
+// the offsets -5 .. 4). Its 100 taps hold 21 distinct weights: tap (dy, dx) has weight sym_id(dy, dx),
+// and sym_rep(u) is a tap index (dy * 10 + dx) of weight u. The launchers check the table's symmetry.
+constexpr int kSsimSym = 21;
+__host__ __device__ constexpr int sym_off(int i) { return i < 5 ? 5 - i : i - 5; }  // |offset| of tap index i
+__host__ __device__ constexpr int sym_id(int dy, int dx) {
+  const int a = sym_off(dy) < sym_off(dx) ? sym_off(dy) : sym_off(dx);
+  const int b = sym_off(dy) < sym_off(dx) ? sym_off(dx) : sym_off(dy);
+  return b * (b + 1) / 2 + a;  // pairs a <= b of 0 .. 5
+}
+__host__ __device__ constexpr int sym_rep(int u) {
+  int b = 0;
+  while ((b + 1) * (b + 2) / 2 <= u) b++;
+  const int a = u - b * (b + 1) / 2;
+  return (5 - a) * 10 + (5 - b);
+}
+// Whether a 10 x 10 table has that symmetry (every tap equals its weight's representative).
+inline bool ssim_window_symmetric(const double* f) {
+  for (int i = 0; i < 100; i++)
+    if (f[i] != f[sym_rep(sym_id(i / 10, i % 10))]) return false;
+  return true;
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b to XCD b mod 8). Renumbered by this
+// bijection of [0, n), XCD x takes the contiguous run [x q + min(x, r), ...) of q + (x < r) tiles
+// (q = n / 8, r = n mod 8), so neighbouring tiles, whose halos overlap, share one L2.
+__device__ __forceinline__ int xcd_order(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7, j = b >> 3;
+  return x * q + min(x, r) + j;
+}
+
 }  // namespace
 }  // namespace qmiv

@@ -1256,8 +1256,45 @@ void ssimf_consts(double max_value, double& c1, double& c2, double f[100]) {
   fin::ssim_window(f);
 }
 
-// One float SSIM evaluation on the context stream: row sums into rows_dev (h doubles), and when
-// maps is not null the gradient maps; then, when g.grad_t or g.grad_r is set, the gradient.
+// The gradient correlation (k_ssimf_grad) of the maps an enqueue_ssimf call computed for the same
+// pictures, scales and max_value; g holds the outputs, their coefficients and accumulate.
+qmiv_status enqueue_ssimf_grad(const void* tst, int32_t tst_f64, double scale_t, const void* ref, int32_t ref_f64,
+                               double scale_r, int32_t w, int32_t h, double max_value, const double* maps,
+                               SsimGArgs& g, hipStream_t st) {
+  double c1, c2;
+  ssimf_consts(max_value, c1, c2, g.f);
+  g.tst = tst;
+  g.ref = ref;
+  g.tst_f64 = tst_f64;
+  g.ref_f64 = ref_f64;
+  g.scale_t = scale_t;
+  g.scale_r = scale_r;
+  g.w = w;
+  g.h = h;
+  g.maps = maps;
+  QMIV_HIP(launch_ssimf_grad(g, st));
+  return QMIV_OK;
+}
+
+// The row reduction of IV-SSIM's six evaluations in one launch: evaluation e (T2R components 0-2,
+// then R2T components 0-2) has its tile partials at partials + e ssimf_tiles_x(w) h and its rows at
+// rows + e h.
+SsimRowsJobs ivssim_rows_jobs(int32_t w, int32_t h, const double* partials, double* rows) {
+  SsimRowsJobs j{};
+  j.n = 6;
+  j.max_h = h;
+  for (int e = 0; e < 6; e++) {
+    j.tiles_x[e] = ssimf_tiles_x(w);
+    j.h[e] = h;
+    j.part[e] = partials + (size_t)e * ssimf_tiles_x(w) * h;
+    j.rows[e] = rows + (size_t)e * h;
+  }
+  return j;
+}
+
+// One float SSIM evaluation on stream st (null: the context stream): row sums into rows_dev (h
+// doubles; null: the tile partials are left for the caller to reduce), and when maps is not null the
+// gradient maps; then, when g.grad_t or g.grad_r is set, the gradient.
 qmiv_status enqueue_ssimf(qmiv_ctx* ctx, const void* tst, int32_t tst_f64, double scale_t, const void* ref,
                           int32_t ref_f64, double scale_r, int32_t w, int32_t h, int32_t calc_l, double max_value,
                           double* partials, double* rows_dev, double* maps, SsimGArgs* g, hipStream_t st = nullptr) {
@@ -1276,27 +1313,18 @@ qmiv_status enqueue_ssimf(qmiv_ctx* ctx, const void* tst, int32_t tst_f64, doubl
   a.partials = partials;
   a.maps = maps;
   QMIV_HIP(launch_ssimf(a, calc_l, maps ? 1 : 0, st));
-  SsimRowsJobs j{};
-  j.n = 1;
-  j.max_h = h;
-  j.tiles_x[0] = a.tiles_x;
-  j.h[0] = h;
-  j.part[0] = partials;
-  j.rows[0] = rows_dev;
-  QMIV_HIP(launch_ssim_rows(j, st));
-  if (g && maps && (g->grad_t || g->grad_r)) {
-    g->tst = tst;
-    g->ref = ref;
-    g->tst_f64 = tst_f64;
-    g->ref_f64 = ref_f64;
-    g->scale_t = scale_t;
-    g->scale_r = scale_r;
-    g->w = w;
-    g->h = h;
-    g->maps = maps;
-    std::memcpy(g->f, a.f, sizeof(a.f));
-    QMIV_HIP(launch_ssimf_grad(*g, st));
+  if (rows_dev) {
+    SsimRowsJobs j{};
+    j.n = 1;
+    j.max_h = h;
+    j.tiles_x[0] = a.tiles_x;
+    j.h[0] = h;
+    j.part[0] = partials;
+    j.rows[0] = rows_dev;
+    QMIV_HIP(launch_ssim_rows(j, st));
   }
+  if (g && maps && (g->grad_t || g->grad_r))
+    QMIV_TRY(enqueue_ssimf_grad(tst, tst_f64, scale_t, ref, ref_f64, scale_r, w, h, max_value, maps, *g, st));
   return QMIV_OK;
 }
 
@@ -2557,9 +2585,9 @@ qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, const floa
     best = bb.take<int16_t>(2 * plane);
     scp = bb.take<float>(6 * plane);
     gscp = grad ? bb.take<float>(3 * plane) : nullptr;
-    partials = bb.take<double>(std::max<size_t>((size_t)ssimf_tiles_x(w) * h, 1));
+    partials = bb.take<double>(std::max<size_t>(6 * (size_t)ssimf_tiles_x(w) * h, 1));  // one per evaluation
     rows = bb.take<double>(6 * (size_t)h);
-    maps = grad ? bb.take<double>(4 * plane) : nullptr;
+    maps = grad ? bb.take<double>(24 * plane) : nullptr;  // 4 maps of each of the six evaluations
   };
   int16_t* best;
   float *scp, *gscp;
@@ -2588,13 +2616,18 @@ qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, const floa
   ScpFArgs s0{tst, best + plane, w, h, prm.search_range, max_value, -1, {gcd[0], gcd[1], gcd[2]}, nullptr, maxv, scp};
   QMIV_HIP(launch_scp_gatherf(s1, st));
   QMIV_HIP(launch_scp_gatherf(s0, st));
-  // T2R_c = SSIM(P0_c, SCP1_c), R2T_c = SSIM(P1_c, SCP0_c) (libs/pmbb-ivqm/src/xSSIM.cpp:197-207).
+  // T2R_c = SSIM(P0_c, SCP1_c), R2T_c = SSIM(P1_c, SCP0_c) (libs/pmbb-ivqm/src/xSSIM.cpp:197-207). With
+  // gradients, every evaluation also keeps its maps (evaluation e = 3 dir + c, dir 0 = T2R), so the
+  // gradient pass after the minimum's direction is known only correlates the chosen three.
+  const size_t pstep = (size_t)ssimf_tiles_x(w) * h;
   for (int c = 0; c < 3; c++) {
     QMIV_TRY(enqueue_ssimf(ctx, tst + c * plane, 0, max_value, scp + 3 * plane + c * plane, 0, 1.0, w, h, 1, max_value,
-                           partials, rows + (size_t)c * h, nullptr, nullptr));
-    QMIV_TRY(enqueue_ssimf(ctx, ref + c * plane, 0, max_value, scp + c * plane, 0, 1.0, w, h, 1, max_value, partials,
-                           rows + (size_t)(3 + c) * h, nullptr, nullptr));
+                           partials + c * pstep, nullptr, maps ? maps + (size_t)c * 4 * plane : nullptr, nullptr));
+    QMIV_TRY(enqueue_ssimf(ctx, ref + c * plane, 0, max_value, scp + c * plane, 0, 1.0, w, h, 1, max_value,
+                           partials + (3 + c) * pstep, nullptr, maps ? maps + (size_t)(3 + c) * 4 * plane : nullptr,
+                           nullptr));
   }
+  QMIV_HIP(launch_ssim_rows(ivssim_rows_jobs(w, h, partials, rows), st));
   QMIV_TRY(ensure_results(ctx, sizeof(double) * 6 * (size_t)h));
   QMIV_HIP(hipMemcpyAsync(ctx->h_results, rows, sizeof(double) * 6 * (size_t)h, hipMemcpyDeviceToHost, st));
   QMIV_HIP(hipStreamSynchronize(st));
@@ -2610,9 +2643,8 @@ qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, const floa
   if (t2r) *t2r = T2R;
   if (r2t) *r2t = R2T;
   if (!grad) return QMIV_OK;
-  // Subgradient: the shifts, the GCD and the direction of the minimum stay fixed.
-  if (grad_tst) QMIV_HIP(hipMemsetAsync(grad_tst, 0, 3 * plane * sizeof(float), st));
-  if (grad_ref) QMIV_HIP(hipMemsetAsync(grad_ref, 0, 3 * plane * sizeof(float), st));
+  // Subgradient: the shifts, the GCD and the direction of the minimum stay fixed. Every output pel
+  // is written: the direct argument's by the correlation, the other's by the SCP adjoint.
   const bool use_r2t = R2T < T2R;  // min2 keeps T2R on ties
   const double wsum = (double)(prm.cmp_weights_average[0] + prm.cmp_weights_average[1] + prm.cmp_weights_average[2] +
                                prm.cmp_weights_average[3]);
@@ -2622,15 +2654,15 @@ qmiv_status qmiv_ivssim_backward(qmiv_ctx* ctx, int32_t w, int32_t h, const floa
   float* g_direct = use_r2t ? grad_ref : grad_tst;
   float* g_other = use_r2t ? grad_tst : grad_ref;
   const float* scp_sel = use_r2t ? scp : scp + 3 * plane;
+  const double* maps_sel = maps + (use_r2t ? (size_t)3 * 4 * plane : 0);
   for (int c = 0; c < 3; c++) {
     SsimGArgs g{};
     g.grad_t = g_direct ? g_direct + c * plane : nullptr;
     g.grad_r = g_other ? gscp + c * plane : nullptr;
     g.coef_t = g.coef_r = (double)prm.cmp_weights_average[c] / wsum;
-    g.accumulate = 1;
-    if (g.grad_r) QMIV_HIP(hipMemsetAsync(g.grad_r, 0, plane * sizeof(float), st));
-    QMIV_TRY(enqueue_ssimf(ctx, direct + c * plane, 0, max_value, scp_sel + c * plane, 0, 1.0, w, h, 1, max_value,
-                           partials, rows + (size_t)c * h, maps, &g));
+    g.accumulate = 0;
+    QMIV_TRY(enqueue_ssimf_grad(direct + c * plane, 0, max_value, scp_sel + c * plane, 0, 1.0, w, h, max_value,
+                                maps_sel + (size_t)c * 4 * plane, g, st));
   }
   if (g_other) QMIV_HIP(launch_scp_scatterf(use_r2t ? s0 : s1, gscp, 1.0, g_other, st));
   QMIV_HIP(hipStreamSynchronize(st));
@@ -2677,9 +2709,9 @@ qmiv_status qmiv_ivssim_backward_batch(qmiv_ctx* ctx, int32_t n, int32_t w, int3
     wk.best = bb.take<int16_t>(2 * plane);
     wk.scp = bb.take<float>(6 * plane);
     wk.gscp = any_grad ? bb.take<float>(3 * plane) : nullptr;
-    wk.partials = bb.take<double>(std::max<size_t>((size_t)ssimf_tiles_x(w) * h, 1));
+    wk.partials = bb.take<double>(std::max<size_t>(6 * (size_t)ssimf_tiles_x(w) * h, 1));  // one per evaluation
     wk.rows = bb.take<double>(6 * (size_t)h);
-    wk.maps = any_grad ? bb.take<double>(4 * plane) : nullptr;
+    wk.maps = any_grad ? bb.take<double>(24 * plane) : nullptr;  // 4 maps of each of the six evaluations
   };
   Work probe;
   Bump b0{nullptr};
@@ -2724,12 +2756,18 @@ qmiv_status qmiv_ivssim_backward_batch(qmiv_ctx* ctx, int32_t n, int32_t w, int3
                              k.scp};
     QMIV_HIP(launch_scp_gatherf(s1[(size_t)i], st));
     QMIV_HIP(launch_scp_gatherf(s0[(size_t)i], st));
+    // With gradients the evaluations keep their maps, as in qmiv_ivssim_backward.
+    double* maps = want_grad(i) ? k.maps : nullptr;
+    const size_t pstep = (size_t)ssimf_tiles_x(w) * h;
     for (int c = 0; c < 3; c++) {
       QMIV_TRY(enqueue_ssimf(ctx, tst[i] + c * plane, 0, max_value, k.scp + 3 * plane + c * plane, 0, 1.0, w, h, 1,
-                             max_value, k.partials, k.rows + (size_t)c * h, nullptr, nullptr, st));
+                             max_value, k.partials + c * pstep, nullptr,
+                             maps ? maps + (size_t)c * 4 * plane : nullptr, nullptr, st));
       QMIV_TRY(enqueue_ssimf(ctx, ref[i] + c * plane, 0, max_value, k.scp + c * plane, 0, 1.0, w, h, 1, max_value,
-                             k.partials, k.rows + (size_t)(3 + c) * h, nullptr, nullptr, st));
+                             k.partials + (3 + c) * pstep, nullptr,
+                             maps ? maps + (size_t)(3 + c) * 4 * plane : nullptr, nullptr, st));
     }
+    QMIV_HIP(launch_ssim_rows(ivssim_rows_jobs(w, h, k.partials, k.rows), st));
     QMIV_HIP(hipMemcpyAsync(hbase + L.results_bytes, k.rows, sizeof(double) * 6 * (size_t)h, hipMemcpyDeviceToHost,
                             st));
   }
@@ -2759,22 +2797,20 @@ qmiv_status qmiv_ivssim_backward_batch(qmiv_ctx* ctx, int32_t n, int32_t w, int3
     const Work& k = wk[(size_t)i];
     float* gt = grad_tst ? grad_tst[i] : nullptr;
     float* gr = grad_ref ? grad_ref[i] : nullptr;
-    if (gt) QMIV_HIP(hipMemsetAsync(gt, 0, 3 * plane * sizeof(float), st));
-    if (gr) QMIV_HIP(hipMemsetAsync(gr, 0, 3 * plane * sizeof(float), st));
     const bool r2 = use_r2t[(size_t)i] != 0;
     const float* direct = r2 ? ref[i] : tst[i];
     float* g_direct = r2 ? gr : gt;
     float* g_other = r2 ? gt : gr;
     const float* scp_sel = r2 ? k.scp : k.scp + 3 * plane;
+    const double* maps_sel = k.maps + (r2 ? (size_t)3 * 4 * plane : 0);
     for (int c = 0; c < 3; c++) {
       SsimGArgs ga{};
       ga.grad_t = g_direct ? g_direct + c * plane : nullptr;
       ga.grad_r = g_other ? k.gscp + c * plane : nullptr;
       ga.coef_t = ga.coef_r = (double)prm.cmp_weights_average[c] / wsum;
-      ga.accumulate = 1;
-      if (ga.grad_r) QMIV_HIP(hipMemsetAsync(ga.grad_r, 0, plane * sizeof(float), st));
-      QMIV_TRY(enqueue_ssimf(ctx, direct + c * plane, 0, max_value, scp_sel + c * plane, 0, 1.0, w, h, 1, max_value,
-                             k.partials, k.rows + (size_t)c * h, k.maps, &ga, st));
+      ga.accumulate = 0;
+      QMIV_TRY(enqueue_ssimf_grad(direct + c * plane, 0, max_value, scp_sel + c * plane, 0, 1.0, w, h, max_value,
+                                  maps_sel + (size_t)c * 4 * plane, ga, st));
     }
     if (g_other) QMIV_HIP(launch_scp_scatterf(r2 ? s0[(size_t)i] : s1[(size_t)i], k.gscp, 1.0, g_other, st));
   }

@@ -321,7 +321,25 @@ __global__ __launch_bounds__(256) void k_msummary(Geom g, int32_t maxv, int32_t 
 // searched in P1 (its T2R).
 // Tile: 64 columns x kMTileH rows; lane = column, wave w takes rows [w * kMRows, (w + 1) * kMRows).
 // LDS holds both pictures' three planes and the mask over the tile plus a B-pel halo, read at
-// clamped coordinates. grid = (ceil(w / 64), ceil(h / kMTileH)).
+// clamped coordinates. grid = (ceil(w / 64), ceil(h / kMTileH)). Each XCD takes whole tile rows
+// (the halos of horizontally neighbouring tiles are read through one L2), the rows 8 apart, and
+// the workgroups it receives one after the other go down a column of its rows: the work follows
+// the mask, and consecutive workgroups of an XCD tend to share a CU, so a CU's tiles are spread
+// over the picture's height rather than bunched along a row.
+//
+// Position t (contiguous per XCD, xcd_order) lists the tile rows by their index mod 8: class m holds
+// the n_m rows m, m + 8, ..., its positions column by column, each column top to bottom.
+__device__ __forceinline__ void masked_tile(int t, int gx, int gy, int& bx, int& by) {
+  int m = 0, start = 0, n = 0;
+  for (; m < 8; m++) {
+    n = m < gy ? (gy - m + 7) >> 3 : 0;  // tile rows of class m
+    if (t < start + n * gx) break;
+    start += n * gx;
+  }
+  const int l = t - start;
+  bx = l / n;
+  by = m + 8 * (l - bx * n);
+}
 constexpr int kMTileW = 64;
 constexpr int kMRows = 4;
 constexpr int kMTileH = 4 * kMRows;
@@ -337,7 +355,9 @@ __global__ __launch_bounds__(256) void k_cps_masked(const T* __restrict__ p0, co
   const int lsz = lw * lh;
   uint16_t* s_pic[2][3] = {{s_lds, s_lds + lsz, s_lds + 2 * lsz}, {s_lds + 3 * lsz, s_lds + 4 * lsz, s_lds + 5 * lsz}};
   uint16_t* s_msk = s_lds + 6 * lsz;
-  const int x0 = blockIdx.x * kMTileW, y0 = blockIdx.y * kMTileH;
+  int bx, by;
+  masked_tile(xcd_order(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y), gridDim.x, gridDim.y, bx, by);
+  const int x0 = bx * kMTileW, y0 = by * kMTileH;
   for (int i = threadIdx.x; i < lsz; i += blockDim.x) {
     const int ly = i / lw, lx = i - ly * lw;
     const int gy = clampi(y0 - B + ly, 0, g.h - 1), gx = clampi(x0 - B + lx, 0, g.w - 1);
@@ -405,7 +425,7 @@ __global__ __launch_bounds__(256) void k_cps_masked(const T* __restrict__ p0, co
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const uint64_t v = wave_sum(acc[dir][c]);
-        if (lane == 0 && y < g.h) partials[(((int64_t)blockIdx.x * 2 + dir) * 3 + c) * g.h + y] = v;
+        if (lane == 0 && y < g.h) partials[(((int64_t)bx * 2 + dir) * 3 + c) * g.h + y] = v;
       }
   }
 }

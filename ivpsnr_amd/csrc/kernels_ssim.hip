@@ -2,6 +2,7 @@
 // integer 2x2 pyramid of MS-SSIM and the per-row reduction of tile partial sums.
 #include <utility>
 
+#include "device_common.h"
 #include "qmiv_internal.h"
 
 namespace qmiv {
@@ -18,8 +19,6 @@ constexpr int kSsimTH = 4 * kSsimK;                  // output rows per workgrou
 constexpr int kSsimLW = (kSsimTW + 9 + 1) & ~1;      // window columns dx in [-5, 4], even row stride
 constexpr int kSsimLH = kSsimTH + 9;
 
-constexpr int32_t kBufDword3 = 0x00020000;
-
 // One plane read through its 4:4:4 view: value(x, y) = plane[(y >> shift) * stride + (x >> shift)].
 struct ViewBuf {
   __amdgpu_buffer_rsrc_t r;
@@ -34,36 +33,14 @@ struct ViewBuf {
   }
 };
 
-// Workgroups are dealt to the 8 XCDs round-robin (workgroup b to XCD b mod 8). Renumbered by this
-// bijection of [0, n), XCD x takes the contiguous run [x q + min(x, r), ...) of q + (x < r) tiles
-// (q = n / 8, r = n mod 8), so neighbouring tiles, whose window halos overlap, share one L2.
-__device__ __forceinline__ int xcd_order(int b, int n) {
-  const int q = n >> 3, r = n & 7, x = b & 7, j = b >> 3;
-  return x * q + min(x, r) + j;
-}
-
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
 
-// The window F is symmetric: F[a][b] = F[b][a], and F[5 + d][b] = F[5 - d][b] (a Gaussian of the
-// offsets -5 .. 4). Its 100 taps hold 21 distinct weights, kept in scalar registers for the whole
-// kernel: tap (dy, dx) uses weight sym_id(dy, dx). The launchers check the symmetry of the table.
-constexpr int kSsimSym = 21;
-__host__ __device__ constexpr int sym_off(int i) { return i < 5 ? 5 - i : i - 5; }  // |offset| of tap index i
-__host__ __device__ constexpr int sym_id(int dy, int dx) {
-  const int a = sym_off(dy) < sym_off(dx) ? sym_off(dy) : sym_off(dx);
-  const int b = sym_off(dy) < sym_off(dx) ? sym_off(dx) : sym_off(dy);
-  return b * (b + 1) / 2 + a;  // pairs a <= b of 0 .. 5
-}
-__host__ __device__ constexpr int sym_rep(int u) {  // a tap index (dy * 10 + dx) of weight u
-  int b = 0;
-  while ((b + 1) * (b + 2) / 2 <= u) b++;
-  const int a = u - b * (b + 1) / 2;
-  return (5 - a) * 10 + (5 - b);
-}
+// The window's 21 distinct weights are kept in scalar registers for the whole kernel: tap (dy, dx)
+// uses weight sym_id(dy, dx) (the symmetric-window helpers of device_common.h).
 
 // The window is walked as steps (R, u): loaded window row R = 0 .. K + 8 of the lane's K output
 // rows and column group u of the window: u = 0 the column of offset 0 (window column 5), u = 1 .. 4

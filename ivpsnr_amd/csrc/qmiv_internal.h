@@ -192,6 +192,9 @@ hipError_t launch_up2_addf(const float* coarse, double coef, int32_t w, int32_t 
 hipError_t launch_quantf(const float* src, int64_t n, double scale, int32_t maxv, void* dst, int32_t bytes,
                          hipStream_t stream);
 hipError_t launch_scp_gatherf(const ScpFArgs& a, hipStream_t stream);
+// The adjoint of launch_scp_gatherf: grad_src[q] = coef * scale * (sum of g over the pels whose winner
+// is q, for the components whose shifted value at q is not clamped), written for every pel q (0 where
+// nothing applies).
 hipError_t launch_scp_scatterf(const ScpFArgs& a, const float* g, double coef, float* grad_src, hipStream_t stream);
 
 // Tile shape of the fast CPS kernel (pels per workgroup tile).
