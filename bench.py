@@ -503,7 +503,7 @@ def main():
         print(f"[bench] per-frame profile leg: warmup 1, {prof_steps} steps", file=sys.stderr, flush=True)
         el_p, prof_alone, _ = timed(step_plain, prof_steps, 1, True)
         mpix = nframes * W * H / 1e6
-        print(json.dumps({"legs": "profile", "steps": prof_steps, "search_range": args.search_range,
+        print(json.dumps({"legs": "profile", "steps": prof_steps,
                           "mpix_s_profiled_plain_loop":
                           round(mpix / (el_p / prof_steps), 2),
                           "kernels_us_per_frame_alone": {k: round(v[0] / nframes / prof_steps * 1e3, 2)

@@ -31,6 +31,13 @@ struct ViewBuf {
     return bytes == 2 ? (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, i * 2u, 0, 0)
                       : (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, i, 0, 0);
   }
+  // The same with the sample size known at compile time (no branch around the load).
+  template <int BYTES>
+  __device__ __forceinline__ uint32_t at_t(int x, int y) const {
+    const uint32_t i = (uint32_t)((y >> shift) * stride + (x >> shift));
+    if constexpr (BYTES == 2) return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, i * 2u, 0, 0);
+    else return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, i, 0, 0);
+  }
 };
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
@@ -119,6 +126,35 @@ __device__ __forceinline__ void ssim_steps(const float* s_t, const float* s_r, i
   }
 }
 
+// The tile fill of one plane pair with their sample size BYTES known: every load of a thread issued
+// before its first LDS store (one load latency per tile instead of one per element; a load under a
+// branch on the sample size is waited for at the branch's end). Loads past the tile's last element
+// read that element again and are not stored.
+template <int BYTES>
+__device__ __forceinline__ void pair_fill(const ViewBuf& T, const ViewBuf& R, int x0, int y0, int w, int h,
+                                          float* s_t, float* s_r) {
+  constexpr int N = kSsimLH * kSsimLW, IT = (N + 255) / 256;
+  uint32_t v[IT][2];
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const int i = min((int)threadIdx.x + 256 * j, N - 1);
+    const int ly = i / kSsimLW;
+    const int lx = i - ly * kSsimLW;
+    const int gy = min(y0 - 5 + ly, h - 1);
+    const int gx = min(x0 - 5 + lx, w - 1);
+    v[j][0] = T.at_t<BYTES>(gx, gy);
+    v[j][1] = R.at_t<BYTES>(gx, gy);
+  }
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const int i = (int)threadIdx.x + 256 * j;
+    if (i < N) {
+      s_t[i] = (float)v[j][0];
+      s_r[i] = (float)v[j][1];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // SSIM of one component: xSSIM::xCalcRowSSIM / xStructSim<flt64, CalcL>::CalcPel
 // (libs/pmbb-ivqm/src/xSSIM.cpp:114-141, libs/pmbb-ivqm/src/xStructSim.cpp:15-60).
@@ -147,14 +183,17 @@ __device__ __forceinline__ void ssim_tile(const PlaneView& tst, const PlaneView&
   }
   // Stage the window rows [y0 - 5, y0 + TH + 4) x [x0 - 5, x0 + TW + 4), clamped inside the
   // picture (clamped values only feed pels outside the interior, which are discarded).
-  for (int i = threadIdx.x; i < kSsimLH * kSsimLW; i += 256) {
-    const int ly = i / kSsimLW;
-    const int lx = i - ly * kSsimLW;
-    const int gy = min(y0 - 5 + ly, h - 1);
-    const int gx = min(x0 - 5 + lx, w - 1);
-    s_t[i] = (float)T.at(gx, gy);
-    s_r[i] = (float)R.at(gx, gy);
-  }
+  if (T.bytes == 2 && R.bytes == 2) pair_fill<2>(T, R, x0, y0, w, h, s_t, s_r);
+  else if (T.bytes == 1 && R.bytes == 1) pair_fill<1>(T, R, x0, y0, w, h, s_t, s_r);
+  else
+    for (int i = threadIdx.x; i < kSsimLH * kSsimLW; i += 256) {
+      const int ly = i / kSsimLW;
+      const int lx = i - ly * kSsimLW;
+      const int gy = min(y0 - 5 + ly, h - 1);
+      const int gx = min(x0 - 5 + lx, w - 1);
+      s_t[i] = (float)T.at(gx, gy);
+      s_r[i] = (float)R.at(gx, gy);
+    }
   __syncthreads();
 
   double acc[kSsimK][5];
@@ -342,6 +381,40 @@ __device__ __forceinline__ double ssim_value(double aR, double aT, double eRR, d
   return l * cs;
 }
 
+// The tile fill of k_ssim_multi with the pictures' sample size BYTES known (the shift-compensated
+// planes hold 16-bit samples): every load of a thread is issued before its first LDS store, so a
+// workgroup waits for one load latency per tile instead of one per element (a load under a branch
+// on the sample size is waited for at the branch's end). Loads past the tile's last element read
+// that element again and are not stored.
+template <int LH, bool HAS_C, bool HAS_D, int BYTES>
+__device__ __forceinline__ void multi_fill(const ViewBuf& P0, const ViewBuf& P1, const ViewBuf& SC, const ViewBuf& SD,
+                                           int x0, int y0, int w, int h, float* s0, float* s1, float* sc, float* sd) {
+  constexpr int N = LH * kSsimLW, IT = (N + 255) / 256;
+  uint32_t v[IT][4];
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const int i = min((int)threadIdx.x + 256 * j, N - 1);
+    const int ly = i / kSsimLW;
+    const int lx = i - ly * kSsimLW;
+    const int gy = min(y0 - 5 + ly, h - 1);
+    const int gx = min(x0 - 5 + lx, w - 1);
+    v[j][0] = P0.at_t<BYTES>(gx, gy);
+    v[j][1] = P1.at_t<BYTES>(gx, gy);
+    if constexpr (HAS_C) v[j][2] = SC.at_t<2>(gx, gy);
+    if constexpr (HAS_D) v[j][3] = SD.at_t<2>(gx, gy);
+  }
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const int i = (int)threadIdx.x + 256 * j;
+    if (i < N) {
+      s0[i] = (float)v[j][0];
+      s1[i] = (float)v[j][1];
+      if constexpr (HAS_C) sc[i] = (float)v[j][2];
+      if constexpr (HAS_D) sd[i] = (float)v[j][3];
+    }
+  }
+}
+
 template <int KM, bool HAS_C, bool HAS_D>
 __global__ __launch_bounds__(256) void k_ssim_multi(SsimMultiArgs a) {
   using Tile = MultiTile<KM>;
@@ -366,16 +439,21 @@ __global__ __launch_bounds__(256) void k_ssim_multi(SsimMultiArgs a) {
     wv[u] = a.f[sym_rep(u)];
     asm volatile("" : "+s"(wv[u]));
   }
-  for (int i = threadIdx.x; i < Tile::LH * kSsimLW; i += 256) {
-    const int ly = i / kSsimLW;
-    const int lx = i - ly * kSsimLW;
-    const int gy = min(y0 - 5 + ly, a.h - 1);
-    const int gx = min(x0 - 5 + lx, a.w - 1);
-    s0[i] = (float)P0.at(gx, gy);
-    s1[i] = (float)P1.at(gx, gy);
-    if (HAS_C) sc[i] = (float)SC.at(gx, gy);
-    if (HAS_D) sd[i] = (float)SD.at(gx, gy);
-  }
+  if (P0.bytes == 2 && P1.bytes == 2 && (!HAS_C || SC.bytes == 2) && (!HAS_D || SD.bytes == 2))
+    multi_fill<Tile::LH, HAS_C, HAS_D, 2>(P0, P1, SC, SD, x0, y0, a.w, a.h, s0, s1, sc, sd);
+  else if (P0.bytes == 1 && P1.bytes == 1 && (!HAS_C || SC.bytes == 2) && (!HAS_D || SD.bytes == 2))
+    multi_fill<Tile::LH, HAS_C, HAS_D, 1>(P0, P1, SC, SD, x0, y0, a.w, a.h, s0, s1, sc, sd);
+  else
+    for (int i = threadIdx.x; i < Tile::LH * kSsimLW; i += 256) {
+      const int ly = i / kSsimLW;
+      const int lx = i - ly * kSsimLW;
+      const int gy = min(y0 - 5 + ly, a.h - 1);
+      const int gx = min(x0 - 5 + lx, a.w - 1);
+      s0[i] = (float)P0.at(gx, gy);
+      s1[i] = (float)P1.at(gx, gy);
+      if (HAS_C) sc[i] = (float)SC.at(gx, gy);
+      if (HAS_D) sd[i] = (float)SD.at(gx, gy);
+    }
   __syncthreads();
 
   double acc[KM][NM];
