@@ -339,13 +339,53 @@ def run_masked(args):
     print(f"[bench] masked: warmup {args.warmup}, {args.steps} steps", file=sys.stderr, flush=True)
     elapsed, prof, res = timed(args.steps, args.warmup, True)
     prof_in_set = prof
-    plain_mpix_s = None
+    plain_mpix_s = stream_mpix_s = batched_mpix_s = pipelined_mpix_s = None
+    tuned = None
+    mode = "stream (qmiv_frames_masked: frame set enqueued over three streams, one wait)"
     if args.legs != "stream":
         el2, _, _ = timed(args.steps, 0, False)
         elapsed = min(elapsed, el2)
+        stream_mpix_s = mpix / (elapsed / args.steps)
         el_loop, _, res_loop = timed(args.steps, 1, False, step_loop)
         plain_mpix_s = mpix / (el_loop / args.steps)
         assert [r.as_dict() for r in res] == [r.as_dict() for r in res_loop], "frame set differs from the per-frame calls"
+        # The same frame set captured once (qmiv_batch_create_masked, branch count tuned) and
+        # replayed: one replay per step (run), and pipelined with a second batch over the same
+        # pictures (launch the next replay, then collect the previous one, so the host finalises
+        # one set while the device computes the next).
+        print("[bench] masked: capturing and tuning the masked batches", file=sys.stderr, flush=True)
+        bt_a = eng.batch_masked(pics, metrics, prm)
+        bt_b = eng.batch_masked(pics, metrics, prm)
+        tuned = bt_a.info()
+        want = [r.as_dict() for r in res]
+        assert [r.as_dict() for r in bt_a.run()] == want, "masked batch differs from the frame set"
+        assert [r.as_dict() for r in bt_b.run()] == want, "masked batch differs from the frame set"
+        el_b, _, _ = timed(args.steps, args.warmup, False, bt_a.run)
+        batched_mpix_s = mpix / (el_b / args.steps)
+
+        def pipelined(steps):
+            bt_a.launch()
+            cur, nxt = bt_a, bt_b
+            eng.sync()
+            out = None
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                nxt.launch()
+                out = cur.collect()
+                cur, nxt = nxt, cur
+            last = cur.collect()
+            el = time.perf_counter() - t0
+            return el, out, last
+
+        pipelined(args.warmup)
+        el_p, out_p, last_p = pipelined(args.steps)
+        assert [r.as_dict() for r in out_p] == want and [r.as_dict() for r in last_p] == want
+        pipelined_mpix_s = mpix / (el_p / args.steps)
+        elapsed = el_p
+        mode = ("pipelined batches (qmiv_batch_create_masked: the frame set captured into a graph, tuned "
+                "branches; two batches, the next replay launched before the previous one is collected)")
+        bt_a.close()
+        bt_b.close()
         print(f"[bench] masked per-frame profile leg: warmup 1, {prof_steps} steps", file=sys.stderr, flush=True)
         _, prof, _ = timed(prof_steps, 1, True, step_loop)
     else:
@@ -393,9 +433,13 @@ def run_masked(args):
         "dtype": "int32", "data": "synthetic",
         "config": {"workload": "masked PSNR+WS-PSNR+IV-PSNR, 8 frame pairs 1920x1080 yuv420p, 8-bit object "
                                f"masks, B={args.search_range}, GCD on", "frames_per_step": nframes,
-                   "mode": "stream (qmiv_frames_masked: frame set enqueued over three streams, one wait)",
+                   "mode": mode,
                    "plain_loop_mpix_s": round(plain_mpix_s, 2) if plain_mpix_s else None,
-                   "stream_vs_plain": round(mpix / (ms_per_step / 1e3) / plain_mpix_s, 3) if plain_mpix_s else None,
+                   "stream_mpix_s": round(stream_mpix_s, 2) if stream_mpix_s else None,
+                   "batched_mpix_s": round(batched_mpix_s, 2) if batched_mpix_s else None,
+                   "pipelined_mpix_s": round(pipelined_mpix_s, 2) if pipelined_mpix_s else None,
+                   "value_vs_plain": round(mpix / (ms_per_step / 1e3) / plain_mpix_s, 3) if plain_mpix_s else None,
+                   "autotuned": tuned,
                    "ivpsnr_last_frame": round(res[-1].ivpsnr, 6), "nnm_last_frame": res[-1].num_non_masked},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4),

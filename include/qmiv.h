@@ -507,6 +507,19 @@ qmiv_status qmiv_batch_run(qmiv_batch* batch, qmiv_frame_result* out);
    batch's pictures go on the context stream, after the replays launched before them. */
 qmiv_status qmiv_batch_launch(qmiv_batch* batch);
 qmiv_status qmiv_batch_collect(qmiv_batch* batch, qmiv_frame_result* out);
+/* A masked frame set captured once into a HIP graph (no counterpart in the reference, whose
+   application runs masked frames one by one, apps/QMIV/src/xAppQMIV.cpp:666-748 with :825-889): the
+   n frames (p0[i], p1[i], mask[i]) with one parameter set and metric mask, each view's flow as
+   qmiv_frames_masked enqueues it, its branch count tuned as qmiv_batch_create tunes it (the sweep
+   option does not apply: the masked search takes its own). Replayed by qmiv_batch_run_masked, or
+   qmiv_batch_launch followed by qmiv_batch_collect_masked; out receives n results, as
+   qmiv_frame_masked's. The unmasked run and collect refuse a masked batch and the masked ones an
+   unmasked batch (QMIV_ERR_INVALID_ARG); qmiv_batch_get_info and qmiv_batch_destroy serve both. */
+qmiv_status qmiv_batch_create_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                                     const qmiv_pic* const* mask, const qmiv_params* prm, uint32_t metrics,
+                                     const qmiv_batch_options* opt, qmiv_batch** out);
+qmiv_status qmiv_batch_run_masked(qmiv_batch* batch, qmiv_masked_result* out);
+qmiv_status qmiv_batch_collect_masked(qmiv_batch* batch, qmiv_masked_result* out);
 qmiv_status qmiv_batch_get_info(const qmiv_batch* batch, qmiv_batch_info* info);
 void qmiv_batch_destroy(qmiv_batch* batch);
 

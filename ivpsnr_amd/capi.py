@@ -169,6 +169,11 @@ _SIGNATURES = {
     "qmiv_batch_run": (C.c_int, [C.c_void_p, C.POINTER(FrameResult)]),
     "qmiv_batch_launch": (C.c_int, [C.c_void_p]),
     "qmiv_batch_collect": (C.c_int, [C.c_void_p, C.POINTER(FrameResult)]),
+    "qmiv_batch_create_masked": (C.c_int, [_ctx_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_void_p), C.POINTER(Params), C.c_uint32,
+                                           C.POINTER(BatchOptions), C.POINTER(C.c_void_p)]),
+    "qmiv_batch_run_masked": (C.c_int, [C.c_void_p, C.POINTER(MaskedResult)]),
+    "qmiv_batch_collect_masked": (C.c_int, [C.c_void_p, C.POINTER(MaskedResult)]),
     "qmiv_batch_get_info": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
     "qmiv_batch_destroy": (None, [C.c_void_p]),
     "qmiv_ssim_backward": (C.c_int, [_ctx_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_double,

@@ -1050,6 +1050,14 @@ qmiv_status host_to_device(qmiv_ctx* ctx, void* dev, const void* src, size_t byt
   return QMIV_OK;
 }
 
+// The masked views of a masked batch (mask pictures, parameters, plans), defined with the masked
+// mode below.
+struct MaskedBatchData;
+qmiv_status batch_build_masked(qmiv_batch* b, int32_t branches);
+void destroy_masked_batch(MaskedBatchData* m);
+MaskedBatchData* copy_masked_batch(const MaskedBatchData* m);
+const qmiv_params& masked_batch_params(const MaskedBatchData* m);
+
 }  // namespace
 
 struct qmiv_batch {
@@ -1078,6 +1086,7 @@ struct qmiv_batch {
   hipEvent_t done = nullptr;  // recorded after a launched replay
   bool pending = false;       // a replay launched and not yet collected
   qmiv_batch_info info{};
+  MaskedBatchData* mb = nullptr;  // a masked batch's views (null: an unmasked batch)
 };
 
 namespace {
@@ -1107,9 +1116,55 @@ qmiv_status batch_grow(void*& ptr, size_t& have, size_t need, bool host) {
   return QMIV_OK;
 }
 
+// Captures the graph of a batch whose views are planned, laid out and assigned to branches
+// (b->branch_of): enqueue(i, st) enqueues view i on stream st, its branch's. The branches fork from
+// and join into the context stream; the graph is instantiated.
+qmiv_status batch_capture(qmiv_batch* b, int32_t branches,
+                          const std::function<qmiv_status(size_t, hipStream_t)>& enqueue) {
+  qmiv_ctx* ctx = b->ctx;
+  while ((int32_t)b->streams.size() < branches) {
+    hipStream_t s;
+    QMIV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    b->streams.push_back(s);
+    hipEvent_t e;
+    QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->join.push_back(e);
+  }
+  if (!b->fork) QMIV_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+
+  // Capture. Kernel timing is off inside the graph; enqueue errors end the capture first.
+  const bool prof = ctx->profiling;
+  ctx->profiling = false;
+  QMIV_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  qmiv_status st = QMIV_OK;
+  if (hipEventRecord(b->fork, ctx->stream) != hipSuccess) st = fail(QMIV_ERR_HIP, "hipEventRecord (fork)");
+  for (int32_t br = 0; br < branches && st == QMIV_OK; br++)
+    if (hipStreamWaitEvent(b->streams[(size_t)br], b->fork, 0) != hipSuccess)
+      st = fail(QMIV_ERR_HIP, "hipStreamWaitEvent (fork)");
+  for (size_t i = 0; i < (size_t)b->n && st == QMIV_OK; i++) st = enqueue(i, b->streams[(size_t)b->branch_of[i]]);
+  for (int32_t br = 0; br < branches && st == QMIV_OK; br++) {
+    if (hipEventRecord(b->join[(size_t)br], b->streams[(size_t)br]) != hipSuccess ||
+        hipStreamWaitEvent(ctx->stream, b->join[(size_t)br], 0) != hipSuccess)
+      st = fail(QMIV_ERR_HIP, "graph join");
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
+  ctx->profiling = prof;
+  if (st != QMIV_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (ee != hipSuccess) return fail(QMIV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+  b->graph = graph;
+  QMIV_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->info.branches = branches;
+  return QMIV_OK;
+}
+
 // Plans every view for (branches, sweep cap), assigns views to branches (longest work first onto
 // the least loaded branch), lays out the buffers and captures the graph.
 qmiv_status batch_build(qmiv_batch* b, int32_t branches, int32_t sweep) {
+  if (b->mb) return batch_build_masked(b, branches);
   qmiv_ctx* ctx = b->ctx;
   batch_drop_graph(b);
   const size_t n = (size_t)b->n;
@@ -1152,46 +1207,12 @@ qmiv_status batch_build(qmiv_batch* b, int32_t branches, int32_t sweep) {
   QMIV_TRY(batch_grow(b->h_results, b->results_bytes, htot, true));
   if (stot) QMIV_TRY(batch_grow(b->d_scp, b->scp_bytes, stot * sizeof(uint16_t), false));
 
-  while ((int32_t)b->streams.size() < branches) {
-    hipStream_t s;
-    QMIV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    b->streams.push_back(s);
-    hipEvent_t e;
-    QMIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    b->join.push_back(e);
-  }
-  if (!b->fork) QMIV_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-
-  // Capture. Kernel timing is off inside the graph; enqueue errors end the capture first.
-  const bool prof = ctx->profiling;
-  ctx->profiling = false;
-  QMIV_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  qmiv_status st = QMIV_OK;
-  if (hipEventRecord(b->fork, ctx->stream) != hipSuccess) st = fail(QMIV_ERR_HIP, "hipEventRecord (fork)");
-  for (int32_t br = 0; br < branches && st == QMIV_OK; br++)
-    if (hipStreamWaitEvent(b->streams[(size_t)br], b->fork, 0) != hipSuccess)
-      st = fail(QMIV_ERR_HIP, "hipStreamWaitEvent (fork)");
-  for (size_t i = 0; i < n && st == QMIV_OK; i++) {
+  QMIV_TRY(batch_capture(b, branches, [&](size_t i, hipStream_t st) {
     const int32_t br = b->branch_of[i];
     uint16_t* scp = b->fw[i].need_scp ? (uint16_t*)b->d_scp + b->scp_off[(size_t)br] : nullptr;
-    st = enqueue_frame(ctx, b->p0[i], b->p1[i], b->prm[i], b->fw[i], b->plans[i], (char*)b->d_scratch + b->doff[i],
-                       (char*)b->h_results + b->hoff[i], scp, b->streams[(size_t)br]);
-  }
-  for (int32_t br = 0; br < branches && st == QMIV_OK; br++) {
-    if (hipEventRecord(b->join[(size_t)br], b->streams[(size_t)br]) != hipSuccess ||
-        hipStreamWaitEvent(ctx->stream, b->join[(size_t)br], 0) != hipSuccess)
-      st = fail(QMIV_ERR_HIP, "graph join");
-  }
-  hipGraph_t graph = nullptr;
-  const hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
-  ctx->profiling = prof;
-  if (st != QMIV_OK) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return st;
-  }
-  if (ee != hipSuccess) return fail(QMIV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
-  b->graph = graph;
-  QMIV_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return enqueue_frame(ctx, b->p0[i], b->p1[i], b->prm[i], b->fw[i], b->plans[i], (char*)b->d_scratch + b->doff[i],
+                         (char*)b->h_results + b->hoff[i], scp, st);
+  }));
   b->info.branches = branches;
   b->info.cps_sweep = sweep;
   return QMIV_OK;
@@ -1888,6 +1909,136 @@ qmiv_status frames_impl(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, con
 
 }  // namespace
 
+namespace {
+
+// Tunes (unless the options fix the shape), builds and checks the batch's graph; *out receives b, or
+// b is destroyed on failure. views_sig: the views' geometry, metrics and parameters.
+qmiv_status batch_tune_finish(qmiv_batch* b, const qmiv_batch_options& opt, const std::string& views_sig,
+                              qmiv_batch** out) {
+  qmiv_ctx* ctx = b->ctx;
+  const int32_t n = b->n;
+  b->info.views = n;
+  // Candidates: branch counts first (with the occupancy sweep rule), then sweep caps (unmasked
+  // batches only: the masked search takes its own).
+  std::vector<int32_t> br_cands, sw_cands;
+  if (opt.branches > 0) br_cands = {std::min(opt.branches, n)};
+  else
+    for (int32_t c : {1, 2, 4, 8, 16})
+      if (c <= n) br_cands.push_back(c);
+  if (b->mb) sw_cands = {0};
+  else if (opt.cps_sweep >= 0) sw_cands = {opt.cps_sweep};
+  else sw_cands = {0, 1, 2, 4};
+  int32_t best_br = br_cands[0], best_sw = sw_cands[0];
+  bool tune = br_cands.size() > 1 || sw_cands.size() > 1;
+  double best_ms = 0.0;
+  qmiv_status st = QMIV_OK;
+  // The signature: which knobs are tuned, the batch's kind and the views' own signature.
+  std::string sig;
+  {
+    const int32_t knobs[3] = {opt.branches, opt.cps_sweep, b->mb ? 1 : 0};
+    sig.append((const char*)knobs, sizeof(knobs));
+    sig += views_sig;
+  }
+  if (tune)
+    for (const auto& e : ctx->batch_tuned)
+      if (e.sig == sig) {
+        best_br = e.branches;
+        best_sw = e.sweep;
+        best_ms = e.ms;
+        b->info.from_cache = 1;
+        tune = false;
+        break;
+      }
+  // A captured graph's replay time depends on its instance as well as its shape: instances of one
+  // shape replay in one of a few distinct times (up to 25% apart for 24 views in 16 branches),
+  // each instance always in the same one, as the runtime lays the instance's branches over the
+  // device's hardware queues. Candidates are therefore ranked by their fastest instance, and the
+  // instance kept at the end is checked against that figure.
+  if (tune) {
+    // Every candidate is built and timed once per round, in kRounds interleaved rounds, and ranked
+    // by its fastest round. The candidates are built in a batch of their own (own streams and
+    // scratch) that ends with the tuning: the batch kept starts from fresh streams.
+    constexpr int kRounds = 8;
+    b->info.tune_rounds = kRounds;
+    std::unique_ptr<qmiv_batch, void (*)(qmiv_batch*)> tb(new qmiv_batch, qmiv_batch_destroy);
+    tb->ctx = ctx;
+    tb->n = n;
+    tb->p0 = b->p0;
+    tb->p1 = b->p1;
+    tb->prm = b->prm;
+    tb->metrics = b->metrics;
+    tb->fw = b->fw;
+    if (b->mb) tb->mb = copy_masked_batch(b->mb);
+    auto time_one = [&](int32_t br, int32_t sw, double& fastest_t) -> qmiv_status {
+      QMIV_TRY(batch_build(tb.get(), br, sw));
+      double t = 0.0;
+      QMIV_TRY(batch_time(tb.get(), opt.tune_reps, t));
+      b->info.candidates_tried++;
+      fastest_t = std::min(fastest_t, t);
+      return QMIV_OK;
+    };
+    // Warm-up: replays of the first candidate until about 30 ms of work have run, so that the
+    // device clocks have left their idle state before the first timed round.
+    st = batch_build(tb.get(), br_cands[0], sw_cands[0]);
+    double warm = 0.0;
+    for (int k = 0; k < 64 && st == QMIV_OK && warm < 30.0; k++) {
+      double t = 0.0;
+      st = batch_time(tb.get(), opt.tune_reps, t);
+      warm += t * opt.tune_reps;
+    }
+    std::vector<double> br_t(br_cands.size(), 1e300);
+    for (int round = 0; round < kRounds && st == QMIV_OK; round++)
+      for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) st = time_one(br_cands[i], sw_cands[0], br_t[i]);
+    const double fastest = *std::min_element(br_t.begin(), br_t.end());
+    // More branches unless fewer are more than 3% faster: the choice does not flip on noise.
+    for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++)
+      if (br_t[i] <= 1.03 * fastest) {
+        best_br = br_cands[i];
+        best_ms = br_t[i];
+      }
+    if (sw_cands.size() > 1 && st == QMIV_OK) {
+      std::vector<double> sw_t(sw_cands.size(), 1e300);
+      for (int round = 0; round < kRounds && st == QMIV_OK; round++)
+        for (size_t k = 0; k < sw_cands.size() && st == QMIV_OK; k++) st = time_one(best_br, sw_cands[k], sw_t[k]);
+      if (st == QMIV_OK) {
+        // The occupancy rule unless another cap is more than 3% faster.
+        const double rule = std::min(sw_t[0], best_ms);
+        best_ms = rule;
+        best_sw = sw_cands[0];
+        for (size_t k = 1; k < sw_cands.size(); k++)
+          if (sw_t[k] < best_ms && sw_t[k] < rule / 1.03) {
+            best_ms = sw_t[k];
+            best_sw = sw_cands[k];
+          }
+      }
+    }
+    if (st == QMIV_OK) ctx->batch_tuned.push_back({sig, best_br, best_sw, best_ms});
+  }
+  if (st == QMIV_OK) st = batch_build(b, best_br, best_sw);
+  // Instance check (tuned or cached shapes): a new instance while this one replays more than 8%
+  // slower than the shape's fastest, at most kInstances builds in all.
+  if (st == QMIV_OK && best_ms > 0.0) {
+    constexpr int kInstances = 6;
+    for (int k = 1; st == QMIV_OK; k++) {
+      double t = 0.0;
+      st = batch_time(b, opt.tune_reps, t);
+      b->info.instances = k;
+      b->info.instance_ms = t;
+      if (st != QMIV_OK || t <= 1.08 * best_ms || k == kInstances) break;
+      st = batch_build(b, best_br, best_sw);
+    }
+  }
+  if (st != QMIV_OK) {
+    qmiv_batch_destroy(b);
+    return st;
+  }
+  b->info.tuned_ms = best_ms;
+  *out = b;
+  return QMIV_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 qmiv_status qmiv_host_alloc(qmiv_ctx* ctx, size_t bytes, void** ptr) {
@@ -2248,126 +2399,14 @@ qmiv_status qmiv_batch_create(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p
     }
     b->fw[(size_t)i] = frame_work(metrics[i] & (QMIV_PSNR | QMIV_WSPSNR | QMIV_IVPSNR | QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM), pi);
   }
-  b->info.views = n;
-  // Candidates: branch counts first (with the occupancy sweep rule), then sweep caps.
-  std::vector<int32_t> br_cands, sw_cands;
-  if (opt.branches > 0) br_cands = {std::min(opt.branches, n)};
-  else
-    for (int32_t c : {1, 2, 4, 8, 16})
-      if (c <= n) br_cands.push_back(c);
-  if (opt.cps_sweep >= 0) sw_cands = {opt.cps_sweep};
-  else sw_cands = {0, 1, 2, 4};
-  int32_t best_br = br_cands[0], best_sw = sw_cands[0];
-  bool tune = br_cands.size() > 1 || sw_cands.size() > 1;
-  double best_ms = 0.0;
-  qmiv_status st = QMIV_OK;
-  // The batch signature: every view's geometry, metrics and parameters, and which knobs are tuned.
+  // The batch signature: every view's geometry, metrics and parameters.
   std::string sig;
-  {
-    auto put = [&](const void* p, size_t k) { sig.append((const char*)p, k); };
-    const int32_t knobs[2] = {opt.branches, opt.cps_sweep};
-    put(knobs, sizeof(knobs));
-    for (int32_t i = 0; i < n; i++) {
-      put(&p0[i]->g, sizeof(Geom));
-      put(&metrics[i], sizeof(uint32_t));
-      put(&b->prm[(size_t)i], sizeof(qmiv_params));
-    }
+  for (int32_t i = 0; i < n; i++) {
+    sig.append((const char*)&p0[i]->g, sizeof(Geom));
+    sig.append((const char*)&metrics[i], sizeof(uint32_t));
+    sig.append((const char*)&b->prm[(size_t)i], sizeof(qmiv_params));
   }
-  if (tune)
-    for (const auto& e : ctx->batch_tuned)
-      if (e.sig == sig) {
-        best_br = e.branches;
-        best_sw = e.sweep;
-        best_ms = e.ms;
-        b->info.from_cache = 1;
-        tune = false;
-        break;
-      }
-  // A captured graph's replay time depends on its instance as well as its shape: instances of one
-  // shape replay in one of a few distinct times (up to 25% apart for 24 views in 16 branches),
-  // each instance always in the same one, as the runtime lays the instance's branches over the
-  // device's hardware queues. Candidates are therefore ranked by their fastest instance, and the
-  // instance kept at the end is checked against that figure.
-  if (tune) {
-    // Every candidate is built and timed once per round, in kRounds interleaved rounds, and ranked
-    // by its fastest round. The candidates are built in a batch of their own (own streams and
-    // scratch) that ends with the tuning: the batch kept starts from fresh streams.
-    constexpr int kRounds = 8;
-    b->info.tune_rounds = kRounds;
-    std::unique_ptr<qmiv_batch, void (*)(qmiv_batch*)> tb(new qmiv_batch, qmiv_batch_destroy);
-    tb->ctx = ctx;
-    tb->n = n;
-    tb->p0 = b->p0;
-    tb->p1 = b->p1;
-    tb->prm = b->prm;
-    tb->metrics = b->metrics;
-    tb->fw = b->fw;
-    auto time_one = [&](int32_t br, int32_t sw, double& fastest_t) -> qmiv_status {
-      QMIV_TRY(batch_build(tb.get(), br, sw));
-      double t = 0.0;
-      QMIV_TRY(batch_time(tb.get(), opt.tune_reps, t));
-      b->info.candidates_tried++;
-      fastest_t = std::min(fastest_t, t);
-      return QMIV_OK;
-    };
-    // Warm-up: replays of the first candidate until about 30 ms of work have run, so that the
-    // device clocks have left their idle state before the first timed round.
-    st = batch_build(tb.get(), br_cands[0], sw_cands[0]);
-    double warm = 0.0;
-    for (int k = 0; k < 64 && st == QMIV_OK && warm < 30.0; k++) {
-      double t = 0.0;
-      st = batch_time(tb.get(), opt.tune_reps, t);
-      warm += t * opt.tune_reps;
-    }
-    std::vector<double> br_t(br_cands.size(), 1e300);
-    for (int round = 0; round < kRounds && st == QMIV_OK; round++)
-      for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++) st = time_one(br_cands[i], sw_cands[0], br_t[i]);
-    const double fastest = *std::min_element(br_t.begin(), br_t.end());
-    // More branches unless fewer are more than 3% faster: the choice does not flip on noise.
-    for (size_t i = 0; i < br_cands.size() && st == QMIV_OK; i++)
-      if (br_t[i] <= 1.03 * fastest) {
-        best_br = br_cands[i];
-        best_ms = br_t[i];
-      }
-    if (sw_cands.size() > 1 && st == QMIV_OK) {
-      std::vector<double> sw_t(sw_cands.size(), 1e300);
-      for (int round = 0; round < kRounds && st == QMIV_OK; round++)
-        for (size_t k = 0; k < sw_cands.size() && st == QMIV_OK; k++) st = time_one(best_br, sw_cands[k], sw_t[k]);
-      if (st == QMIV_OK) {
-        // The occupancy rule unless another cap is more than 3% faster.
-        const double rule = std::min(sw_t[0], best_ms);
-        best_ms = rule;
-        best_sw = sw_cands[0];
-        for (size_t k = 1; k < sw_cands.size(); k++)
-          if (sw_t[k] < best_ms && sw_t[k] < rule / 1.03) {
-            best_ms = sw_t[k];
-            best_sw = sw_cands[k];
-          }
-      }
-    }
-    if (st == QMIV_OK) ctx->batch_tuned.push_back({sig, best_br, best_sw, best_ms});
-  }
-  if (st == QMIV_OK) st = batch_build(b, best_br, best_sw);
-  // Instance check (tuned or cached shapes): a new instance while this one replays more than 8%
-  // slower than the shape's fastest, at most kInstances builds in all.
-  if (st == QMIV_OK && best_ms > 0.0) {
-    constexpr int kInstances = 6;
-    for (int k = 1; st == QMIV_OK; k++) {
-      double t = 0.0;
-      st = batch_time(b, opt.tune_reps, t);
-      b->info.instances = k;
-      b->info.instance_ms = t;
-      if (st != QMIV_OK || t <= 1.08 * best_ms || k == kInstances) break;
-      st = batch_build(b, best_br, best_sw);
-    }
-  }
-  if (st != QMIV_OK) {
-    qmiv_batch_destroy(b);
-    return st;
-  }
-  b->info.tuned_ms = best_ms;
-  *out = b;
-  return QMIV_OK;
+  return batch_tune_finish(b, opt, sig, out);
 }
 
 qmiv_status qmiv_batch_launch(qmiv_batch* b) {
@@ -2381,8 +2420,10 @@ qmiv_status qmiv_batch_launch(qmiv_batch* b) {
   b->pending = true;
   // The spherical weights every view needs exist before the views are finalised in parallel (the
   // table is only read there).
-  for (int32_t i = 0; i < b->n; i++)
-    if (b->prm[(size_t)i].erp) (void)ws_table(ctx, b->p0[(size_t)i]->g.h, b->prm[(size_t)i].lat_range_deg);
+  for (int32_t i = 0; i < b->n; i++) {
+    const qmiv_params& pi = b->mb ? masked_batch_params(b->mb) : b->prm[(size_t)i];
+    if (pi.erp) (void)ws_table(ctx, b->p0[(size_t)i]->g.h, pi.lat_range_deg);
+  }
   if (!b->pool && b->n > 1) {
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     b->pool.reset(new HostPool((int)std::min<unsigned>(7u, std::min<unsigned>(hw, (unsigned)b->n) - 1u)));
@@ -2392,6 +2433,7 @@ qmiv_status qmiv_batch_launch(qmiv_batch* b) {
 
 qmiv_status qmiv_batch_collect(qmiv_batch* b, qmiv_frame_result* out) {
   if (!b || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (b->mb) return fail(QMIV_ERR_INVALID_ARG, "a masked batch: use qmiv_batch_collect_masked");
   if (!b->pending) return fail(QMIV_ERR_INVALID_ARG, "the batch has no replay in flight");
   QMIV_HIP(hipSetDevice(b->ctx->device));
   b->pending = false;
@@ -2412,6 +2454,7 @@ qmiv_status qmiv_batch_collect(qmiv_batch* b, qmiv_frame_result* out) {
 
 qmiv_status qmiv_batch_run(qmiv_batch* b, qmiv_frame_result* out) {
   if (!b || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (b->mb) return fail(QMIV_ERR_INVALID_ARG, "a masked batch: use qmiv_batch_run_masked");
   QMIV_TRY(qmiv_batch_launch(b));
   return qmiv_batch_collect(b, out);
 }
@@ -2434,6 +2477,7 @@ void qmiv_batch_destroy(qmiv_batch* b) {
   if (b->d_scratch) (void)hipFree(b->d_scratch);
   if (b->d_scp) (void)hipFree(b->d_scp);
   if (b->h_results) (void)hipHostFree(b->h_results);
+  destroy_masked_batch(b->mb);
   delete b;
 }
 
@@ -3317,6 +3361,148 @@ qmiv_status qmiv_calc_ivpsnr_masked(qmiv_ctx* ctx, const qmiv_pic* tst, const qm
   if (t2r) *t2r = b;
   *ivpsnr = fin::min2(a, b);
   return QMIV_OK;
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// Masked batches: a masked frame set (as qmiv_frames_masked) captured once into a graph.
+// =============================================================================================
+namespace {
+
+struct MaskedBatchData {
+  std::vector<const qmiv_pic*> mask;
+  qmiv_params prm{};
+  uint32_t metrics = 0;
+  std::vector<MaskedPlan> plans;
+};
+
+void destroy_masked_batch(MaskedBatchData* m) { delete m; }
+
+MaskedBatchData* copy_masked_batch(const MaskedBatchData* m) { return m ? new MaskedBatchData(*m) : nullptr; }
+
+const qmiv_params& masked_batch_params(const MaskedBatchData* m) { return m->prm; }
+
+// The masked counterpart of batch_build: plans every view as qmiv_frames_masked does, assigns views
+// to branches (longest work first onto the least loaded branch), lays out the buffers and captures
+// each view's masked flow (statistics, masked GCD, search, results copy) on its branch.
+qmiv_status batch_build_masked(qmiv_batch* b, int32_t branches) {
+  qmiv_ctx* ctx = b->ctx;
+  MaskedBatchData& m = *b->mb;
+  batch_drop_graph(b);
+  const size_t n = (size_t)b->n;
+  const bool iv = (m.metrics & QMIV_IVPSNR) != 0;
+  // Without IV-PSNR the application computes no colour difference and reports zeros.
+  static const int32_t kZero[4] = {0, 0, 0, 0};
+  const int32_t* gcd_given = iv ? nullptr : kZero;
+  m.plans.assign(n, MaskedPlan{});
+  b->doff.assign(n, 0);
+  b->hoff.assign(n, 0);
+  size_t dtot = 0, htot = 0;
+  std::vector<double> cost(n);
+  for (size_t i = 0; i < n; i++) {
+    QMIV_TRY(plan_masked(ctx, b->p0[i]->g, m.mask[i]->g, m.prm, m.prm.invalid_pel_action, iv, gcd_given, m.plans[i]));
+    b->doff[i] = dtot;
+    b->hoff[i] = htot;
+    dtot += align_up(m.plans[i].L.total, 256);
+    htot += align_up(m.plans[i].L.results_bytes, 256);
+    const Geom& g = b->p0[i]->g;
+    cost[i] = (double)g.w * g.h * (1.0 + (iv ? 4.0 : 0.0));
+  }
+  std::vector<size_t> order(n);
+  for (size_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return cost[a] > cost[c]; });
+  std::vector<double> load((size_t)branches, 0.0);
+  b->branch_of.assign(n, 0);
+  for (size_t i : order) {
+    const size_t br = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    b->branch_of[i] = (int32_t)br;
+    load[br] += cost[i];
+  }
+  QMIV_HIP(hipStreamSynchronize(ctx->stream));
+  QMIV_TRY(batch_grow(b->d_scratch, b->scratch_bytes, dtot, false));
+  QMIV_TRY(batch_grow(b->h_results, b->results_bytes, htot, true));
+  QMIV_TRY(batch_capture(b, branches, [&](size_t i, hipStream_t st) {
+    return enqueue_masked(ctx, b->p0[i], b->p1[i], m.mask[i], m.prm, m.plans[i], -1, gcd_given,
+                          (char*)b->d_scratch + b->doff[i], (char*)b->h_results + b->hoff[i], st);
+  }));
+  b->info.cps_sweep = 0;
+  return QMIV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+qmiv_status qmiv_batch_create_masked(qmiv_ctx* ctx, int32_t n, const qmiv_pic* const* p0, const qmiv_pic* const* p1,
+                                     const qmiv_pic* const* mask, const qmiv_params* prm_in, uint32_t metrics,
+                                     const qmiv_batch_options* opt_in, qmiv_batch** out) {
+  if (!ctx || !p0 || !p1 || !mask || !out || n <= 0) return fail(QMIV_ERR_INVALID_ARG, "null argument or empty batch");
+  *out = nullptr;
+  qmiv_batch_options opt;
+  qmiv_batch_options_default(&opt);
+  if (opt_in) opt = *opt_in;
+  if (opt.branches < 0 || opt.branches > kMaxBranches) return fail(QMIV_ERR_INVALID_ARG, "branches must be 0..16");
+  if (opt.tune_reps <= 0) opt.tune_reps = 12;
+  for (int32_t i = 0; i < n; i++) {
+    QMIV_TRY(check_pair(p0[i], p1[i]));
+    QMIV_TRY(check_mask(p0[i], mask[i]));
+    if (p0[i]->ctx != ctx || mask[i]->ctx != ctx) return fail(QMIV_ERR_INVALID_ARG, "pictures belong to another context");
+  }
+  if (metrics & (QMIV_SSIM | QMIV_MSSSIM | QMIV_IVSSIM))
+    return fail(QMIV_ERR_INVALID_ARG, "structural similarity metrics cannot be combined with mask mode");
+  if (metrics & ~(uint32_t)(QMIV_PSNR | QMIV_WSPSNR | QMIV_IVPSNR)) return fail(QMIV_ERR_INVALID_ARG, "unknown metric bits");
+  const qmiv_params prm = masked_params(prm_in, false);
+  if (prm.invalid_pel_action < QMIV_PEL_STOP || prm.invalid_pel_action > QMIV_PEL_CNCL)
+    return fail(QMIV_ERR_INVALID_ARG, "invalid_pel_action must be 0..3");
+  QMIV_HIP(hipSetDevice(ctx->device));
+  qmiv_batch* b = new qmiv_batch;
+  b->ctx = ctx;
+  b->n = n;
+  b->p0.assign(p0, p0 + n);
+  b->p1.assign(p1, p1 + n);
+  b->mb = new MaskedBatchData;
+  b->mb->mask.assign(mask, mask + n);
+  b->mb->prm = prm;
+  b->mb->metrics = metrics;
+  std::string sig;
+  sig.append((const char*)&metrics, sizeof(metrics));
+  sig.append((const char*)&prm, sizeof(prm));
+  for (int32_t i = 0; i < n; i++) {
+    sig.append((const char*)&p0[i]->g, sizeof(Geom));
+    sig.append((const char*)&mask[i]->g, sizeof(Geom));
+  }
+  return batch_tune_finish(b, opt, sig, out);
+}
+
+qmiv_status qmiv_batch_collect_masked(qmiv_batch* b, qmiv_masked_result* out) {
+  if (!b || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (!b->mb) return fail(QMIV_ERR_INVALID_ARG, "not a masked batch: use qmiv_batch_collect");
+  if (!b->pending) return fail(QMIV_ERR_INVALID_ARG, "the batch has no replay in flight");
+  QMIV_HIP(hipSetDevice(b->ctx->device));
+  b->pending = false;
+  QMIV_HIP(hipEventSynchronize(b->done));
+  const MaskedBatchData& m = *b->mb;
+  std::vector<qmiv_status> st((size_t)b->n, QMIV_OK);
+  auto fin_view = [&](int i) {
+    const size_t k = (size_t)i;
+    std::memset(&out[i], 0, sizeof(out[i]));
+    st[k] = finalize_masked(b->ctx, b->p0[k]->g, m.mask[k]->g.bd, m.prm, m.metrics, m.plans[k].L,
+                            (const char*)b->h_results + b->hoff[k], &out[i]);
+  };
+  if (b->pool) b->pool->run(b->n, fin_view);
+  else for (int32_t i = 0; i < b->n; i++) fin_view(i);
+  for (int32_t i = 0; i < b->n; i++)
+    if (st[(size_t)i] != QMIV_OK)
+      return fail(st[(size_t)i], "view " + std::to_string(i) + ": input or mask contains pels above the bit-depth maximum");
+  return QMIV_OK;
+}
+
+qmiv_status qmiv_batch_run_masked(qmiv_batch* b, qmiv_masked_result* out) {
+  if (!b || !out) return fail(QMIV_ERR_INVALID_ARG, "null argument");
+  if (!b->mb) return fail(QMIV_ERR_INVALID_ARG, "not a masked batch: use qmiv_batch_run");
+  QMIV_TRY(qmiv_batch_launch(b));
+  return qmiv_batch_collect_masked(b, out);
 }
 
 }  // extern "C"

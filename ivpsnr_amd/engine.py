@@ -283,6 +283,14 @@ class Engine:
         self._batches.append(weakref.ref(bt))
         return bt
 
+    def batch_masked(self, triples, metrics: int, params: Params | None = None, branches: int = 0,
+                     tune_reps: int = 12) -> "MaskedBatch":
+        """A captured masked frame set over the (p0, p1, mask) triples (qmiv_batch_create_masked):
+        one metric mask and parameter set for every view, as qmiv_frames_masked."""
+        bt = MaskedBatch(self, triples, metrics, params, branches, tune_reps)
+        self._batches.append(weakref.ref(bt))
+        return bt
+
     # ---- differentiable SSIM family on float planes in device memory (pointers) ----
     def ssim_backward(self, w, h, planes, tst_ptr, ref_ptr, max_value, calc_l=True, grad_tst_ptr=0, grad_ref_ptr=0):
         v = (C.c_double * planes)()
@@ -554,3 +562,36 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class MaskedBatch(Batch):
+    """Masked views bound to one captured HIP graph; run() / collect() return masked results."""
+
+    def __init__(self, engine: Engine, triples, metrics: int, params, branches, tune_reps):
+        n = len(triples)
+        self.engine = engine
+        self.n = n
+        self._pairs = list(triples)  # keep the pictures alive as long as the batch
+        a = (C.c_void_p * n)(*[t[0].handle.value for t in triples])
+        b = (C.c_void_p * n)(*[t[1].handle.value for t in triples])
+        m = (C.c_void_p * n)(*[t[2].handle.value for t in triples])
+        opt = capi.BatchOptions(branches, -1, tune_reps)
+        h = C.c_void_p()
+        check(lib().qmiv_batch_create_masked(engine.ctx, n, a, b, m, C.byref(params) if params else None, metrics,
+                                             C.byref(opt), C.byref(h)), "qmiv_batch_create_masked")
+        self.handle = h
+
+    def run(self, raise_on_error: bool = True):
+        out = (capi.MaskedResult * self.n)()
+        st = lib().qmiv_batch_run_masked(self.handle, out)
+        if raise_on_error:
+            check(st, "qmiv_batch_run_masked")
+        return list(out)
+
+    def collect(self, raise_on_error: bool = True):
+        """Waits for the launched replay and finalises its views (qmiv_batch_collect_masked)."""
+        out = (capi.MaskedResult * self.n)()
+        st = lib().qmiv_batch_collect_masked(self.handle, out)
+        if raise_on_error:
+            check(st, "qmiv_batch_collect_masked")
+        return list(out)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from ivpsnr_amd import default_params
-from ivpsnr_amd.capi import QmivError
+from ivpsnr_amd.capi import FrameResult, QmivError, lib
 from ivpsnr_amd.synth import make_mask, make_pair, sha256
 
 pytestmark = pytest.mark.gpu
@@ -288,3 +288,53 @@ def test_errors_and_range_flags(engine):
     prm.search_range = 13
     with pytest.raises(QmivError):
         engine.frame_masked(p0, p1, pm, PSNRS, prm)
+
+
+def test_masked_batch_equals_frame_set(engine):
+    """qmiv_batch_create_masked: the captured masked frame set gives each frame exactly what
+    qmiv_frames_masked gives, over frames of different sizes, formats, masks and bit depths (the
+    16-bit search, the 24-bit one, 4:4:4, the general search of 16-bit pictures), replayed several
+    times, tuned and with a fixed branch count, run and launch / collect; the unmasked calls refuse
+    it; a frame with a pel above its bit depth fails the replay with its flags reported."""
+    specs = [(1920, 1080, 8, 420, 8, "objects"), (256, 96, 8, 444, 16, "speckle"), (130, 66, 10, 420, 10, "soft"),
+             (640, 360, 10, 420, 8, "speckle"), (96, 70, 16, 420, 16, "soft"), (384, 40, 10, 420, 8, "soft")]
+    triples = []
+    for k, (w, h, bd, cf, bdm, kind) in enumerate(specs):
+        pair = make_pair(w, h, bd, 90 + k, chroma_format=cf, luma_offset=2.0)
+        mask = make_mask(w, h, bdm, 90 + k, chroma_format=cf, kind=kind)
+        triples.append((engine.picture(w, h, bd, cf, data=pair.ref), engine.picture(w, h, bd, cf, data=pair.tst),
+                        engine.picture(w, h, bdm, cf, data=mask)))
+    for erp, branches in ((False, 0), (True, 2)):
+        print(f"masked batch erp={erp} branches={branches}", flush=True)
+        prm = default_params(erp=erp)
+        want, st = engine.frames_masked(triples, PSNRS, prm)
+        assert st == 0
+        bt = engine.batch_masked(triples, PSNRS, prm, branches=branches)
+        info = bt.info()
+        assert info["views"] == len(triples) and (branches == 0 or info["branches"] == branches)
+        for rep in range(3):
+            if rep == 2:
+                bt.launch()
+                got = bt.collect()
+            else:
+                got = bt.run()
+            for i, (g, w_) in enumerate(zip(got, want)):
+                assert g.as_dict() == w_.as_dict(), (erp, rep, i)
+        out = (FrameResult * len(triples))()
+        assert lib().qmiv_batch_run(bt.handle, out) != 0  # the unmasked run refuses a masked batch
+        bt.launch()
+        assert lib().qmiv_batch_collect(bt.handle, out) != 0  # ... and so does the unmasked collect
+        assert [g.as_dict() for g in bt.collect()] == [w_.as_dict() for w_ in want]
+        bt.close()
+    # frame 2 out of range: pel 1023 + 1 in a 10-bit picture, reported by the replay
+    w, h = 130, 66
+    bad = make_pair(w, h, 10, 99, luma_offset=0.0).tst.copy()
+    bad[5] = 1024
+    triples[2] = (triples[2][0], engine.picture(w, h, 10, 420, data=bad), triples[2][2])
+    bt = engine.batch_masked(triples, PSNRS, default_params(), branches=1)
+    got = bt.run(raise_on_error=False)
+    assert got[2].as_dict()["in_range"][1] == 0
+    assert got[0].as_dict()["ivpsnr"] == engine.frame_masked(*triples[0], PSNRS, default_params()).ivpsnr
+    with pytest.raises(QmivError):
+        bt.run()
+    bt.close()
