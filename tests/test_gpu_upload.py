@@ -99,3 +99,21 @@ def test_pageable_uploads_through_staging(engine):
     p = engine.picture(w8, h8, 8, 420)
     p.upload(src)
     assert np.array_equal(p.download(), src)
+
+
+def test_large_unpack_through_staging(engine):
+    """qmiv_pic_upload_unpack of a raw 4:2:2 10-bit frame of 32 MB from ordinary memory: the
+    copy goes through the pinned staging slots in four chunks (the last partial) into the device
+    staging buffer, then the unpack; the 4:4:4 picture holds the luma and the chroma repeated
+    horizontally (libs/pmbb-core/src/xSeq.cpp:197-229)."""
+    w, h, bd = 4000, 2000, 10
+    rng = np.random.default_rng(3)
+    y = rng.integers(0, 1 << bd, (h, w), dtype=np.uint16)
+    cb = rng.integers(0, 1 << bd, (h, w // 2), dtype=np.uint16)
+    cr = rng.integers(0, 1 << bd, (h, w // 2), dtype=np.uint16)
+    frame = np.concatenate([y.ravel(), cb.ravel(), cr.ravel()])
+    assert frame.nbytes > 3 * (8 << 20)
+    p = engine.picture(w, h, bd, 444).upload_unpack(frame, 422)
+    got = p.download().reshape(3, h, w)
+    assert np.array_equal(got[0], y)
+    assert np.array_equal(got[1], np.repeat(cb, 2, axis=1)) and np.array_equal(got[2], np.repeat(cr, 2, axis=1))
