@@ -338,3 +338,25 @@ def test_masked_batch_equals_frame_set(engine):
     with pytest.raises(QmivError):
         bt.run()
     bt.close()
+
+
+def test_masked_batch_argument_errors(engine):
+    """qmiv_batch_create_masked validates as qmiv_frames_masked does: structural-similarity metrics
+    and unknown metric bits, mismatched pictures, a mask of another size and an empty set fail with
+    QMIV_ERR_INVALID_ARG before anything is captured."""
+    w, h = 128, 64
+    pair = make_pair(w, h, 8, 5)
+    mask = make_mask(w, h, 8, 5, kind="soft")
+    p0 = engine.picture(w, h, 8, 420, data=pair.ref)
+    p1 = engine.picture(w, h, 8, 420, data=pair.tst)
+    pm = engine.picture(w, h, 8, 420, data=mask)
+    other = engine.picture(w, h + 2, 8, 420)
+    for metrics, triple in ((PSNRS | 8, (p0, p1, pm)), (PSNRS | (1 << 20), (p0, p1, pm)), (PSNRS, (p0, other, pm)),
+                            (PSNRS, (p0, p1, other))):
+        with pytest.raises(QmivError):
+            engine.batch_masked([triple], metrics, default_params())
+    with pytest.raises(QmivError):
+        engine.batch_masked([], PSNRS, default_params())
+    bt = engine.batch_masked([(p0, p1, pm)], PSNRS, default_params(), branches=1)
+    assert bt.run()[0].as_dict() == engine.frame_masked(p0, p1, pm, PSNRS, default_params()).as_dict()
+    bt.close()
